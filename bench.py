@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark: join -> grouped aggregate -> top-10 on one MI355X.
+
+Workload: customers 1.5M, orders 15M, lines ~60M (seeded synthetic tables,
+see mettle_amd/datagen.py); query = filter segment, o_date < cutoff <
+l_ship, join lines -> orders -> customers, SUM(price * (100 - discount))
+per order (exact int64), top-10 by (revenue desc, o_date asc, o_key asc).
+
+One step = one full query over the resident columns (the star pipeline of
+libmettle_hip.so).  Steps are enqueued back to back on the library stream
+and bracketed by device synchronisation; value = input rows / second.
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("rows/s & % HBM roofline (join+agg+top-K); queries/s + p99, batched vs serial")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_profile(L):
+    out = {}
+    name = C.create_string_buffer(128)
+    ms = C.c_double()
+    cnt = C.c_uint64()
+    i = 0
+    while L.mtl_profile_get(i, name, 128, C.byref(ms), C.byref(cnt)) == 0:
+        out[name.value.decode()] = (ms.value, cnt.value)
+        i += 1
+    return out
+
+
+def cpu_baseline(scale: float, seconds: float):
+    """Oracle (C restatement, 1 thread) on a bounded sample of the workload;
+    also checks the GPU result on the same sample."""
+    from mettle_amd import abi, datagen, ops
+    from oracle import oracle as orc
+    tables = datagen.star_tables(scale=scale)
+    n_c, n_o, n_l = (len(tables[0]["c_key"]), len(tables[1]["o_key"]),
+                     len(tables[2]["l_order"]))
+    by = dict(zip(["customers", "orders", "lines"], tables))
+    hd = datagen.star_desc(lambda t, c: by[t][c].ctypes.data, n_c, n_o, n_l)
+    runs, t0 = 0, time.perf_counter()
+    while True:
+        st, ref = orc.star_topk(hd)
+        runs += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    rows = n_c + n_o + n_l
+    dev = {t: {c: ops.to_device(v) for c, v in cols.items()} for t, cols in by.items()}
+    dd = datagen.star_desc(lambda t, c: dev[t][c].data_ptr(), n_c, n_o, n_l)
+    got = ops.star_topk(dd)
+    same = (st == abi.MTL_OK and [(r.p_row, r.count, r.sum_i64) for r in got] ==
+            [(r.p_row, r.count, r.sum_i64) for r in ref])
+    return {"value": rows * runs / el, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle/mettle_oracle.c star pipeline, scale {scale} "
+                       f"({n_c}+{n_o}+{n_l} rows), {runs} runs in {el:.1f}s, 1 thread"),
+            "gpu_matches_oracle_on_sample": bool(same)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-scale", type=float, default=0.1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(sharding is described in DESIGN.md, not run)"}))
+        return 2
+
+    import torch
+    from mettle_amd import abi, datagen, ops
+
+    L = abi.lib()
+    log("generating tables")
+    t0 = time.time()
+    cust, orders, lines = datagen.star_tables(scale=args.scale)
+    n_c, n_o, n_l = len(cust["c_key"]), len(orders["o_key"]), len(lines["l_order"])
+    log(f"tables: customers {n_c}, orders {n_o}, lines {n_l} ({time.time() - t0:.1f}s)")
+    dev = {}
+    for tname, cols in (("customers", cust), ("orders", orders), ("lines", lines)):
+        dev[tname] = {c: ops.to_device(v) for c, v in cols.items()}
+    torch.cuda.synchronize()
+    log("columns resident in HBM")
+    desc = datagen.star_desc(lambda t, c: dev[t][c].data_ptr(), n_c, n_o, n_l)
+
+    s = C.c_void_p()
+    abi.check(L.mtl_stream_create(C.byref(s)))
+    out_dev = torch.empty(desc.k * C.sizeof(abi.StarRow) * (args.steps + args.warmup + 1),
+                          dtype=torch.uint8, device="cuda")
+    out_n = torch.zeros(2 * (args.steps + args.warmup + 1), dtype=torch.int32, device="cuda")
+    row_bytes = desc.k * C.sizeof(abi.StarRow)
+
+    def step(i):
+        abi.check(L.mtl_star_topk_async(C.byref(desc), out_dev.data_ptr() + i * row_bytes,
+                                        out_n.data_ptr() + 8 * i, s))
+
+    # correctness of one synchronous run (result also checked vs the oracle
+    # in the cpu_baseline leg on a sample)
+    rows = (abi.StarRow * desc.k)()
+    nrows = C.c_uint32()
+    abi.check(L.mtl_star_topk(C.byref(desc), rows, C.byref(nrows), s))
+    log(f"top row: order {int(orders['o_key'][rows[0].p_row])} revenue {rows[0].sum_i64}")
+
+    for i in range(args.warmup):
+        step(i)
+    abi.check(L.mtl_stream_sync(s))
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} done; timing {args.steps} steps")
+
+    ext = torch.cuda.ExternalStream(L.mtl_stream_native(s))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    e0.record(ext)
+    for i in range(args.steps):
+        step(args.warmup + i)
+    e1.record(ext)
+    abi.check(L.mtl_stream_sync(s))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    dev_ms = e0.elapsed_time(e1)
+    err_bits = out_n.cpu().numpy()[1::2]
+    if (err_bits != 0).any():
+        abi.check(L.mtl_star_status(int(err_bits[err_bits != 0][0])))
+    ms_per_step = dev_ms / args.steps
+    in_rows = n_c + n_o + n_l
+    value = in_rows * args.steps / (dev_ms / 1e3)
+    alg_bytes = datagen.star_algorithmic_bytes(n_c, n_o, n_l)
+    log(f"{ms_per_step:.4f} ms/step device, wall {wall / args.steps * 1e3:.4f} ms/step")
+
+    # serial latency: synchronous single queries (host sync + 10-row download)
+    lat = []
+    for _ in range(min(args.steps, 30)):
+        t = time.perf_counter()
+        abi.check(L.mtl_star_topk(C.byref(desc), rows, C.byref(nrows), s))
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat.sort()
+
+    # per-kernel durations (hipEvents on the library stream)
+    L.mtl_profile_reset()
+    L.mtl_profile_enable(1)
+    prof_steps = min(args.steps, 20)
+    for i in range(prof_steps):
+        step(i)
+    abi.check(L.mtl_stream_sync(s))
+    L.mtl_profile_enable(0)
+    prof = kernel_profile(L)
+    per_kernel = {k: round(ms / max(c, 1) * 1e3, 2) for k, (ms, c) in prof.items()}
+    dom = max(prof, key=lambda k: prof[k][0])
+    dom_us = prof[dom][0] / prof[dom][1] * 1e3
+    kernel_bytes = {"k_fact": n_l * datagen.STAR_BYTES_PER_ROW["lines"],
+                    "k_filter_parent": n_o * datagen.STAR_BYTES_PER_ROW["orders"],
+                    "k_mark_dim": n_c * datagen.STAR_BYTES_PER_ROW["customers"]}
+    dom_bytes = kernel_bytes.get(dom, 0)
+    achieved = dom_bytes / (dom_us * 1e-6) / 1e9
+    log(f"kernels (us/launch): {per_kernel}")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (numpy PCG64 seed 20261203+3), columns resident in HBM",
+        "config": {"workload": "join->group-agg->top-10: customers 1.5M, orders 15M, "
+                               f"lines {n_l} (config 3)",
+                   "rows": in_rows, "scale": args.scale, "k": 10},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "avg_launch_us": dom_us},
+        "pipeline": {"algorithmic_bytes": alg_bytes, "device_ms": ms_per_step,
+                     "achieved_GBs": alg_bytes / (ms_per_step * 1e-3) / 1e9,
+                     "frac": alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "kernel_us": per_kernel},
+        "latency_ms": {"serial_p50": lat[len(lat) // 2], "serial_p99": lat[-1],
+                       "queries_per_s_serial": 1e3 / (sum(lat) / len(lat)),
+                       "queries_per_s_batched": args.steps / (dev_ms / 1e3)},
+    }
+    L.mtl_stream_destroy(s)
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread)")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

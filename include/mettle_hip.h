@@ -1,0 +1,337 @@
+/*
+ * mettle_hip.h — C ABI of the MI355X (gfx950) execution backend for mettle.
+ *
+ * This header is the drop-in boundary.  In the reference, the planner
+ * (src/plan/...) reaches the GPU only through the free functions of
+ * src/metal/mod.rs:96-609; every declaration below names the reference
+ * function(s) it replaces.  All signatures use plain C types: device
+ * pointers are `const void*` / `void*`, sizes are uint64_t, and streams are
+ * opaque `mtl_stream*` handles (NULL = the library's default stream).
+ *
+ * Conventions
+ *   - Every entry point returns an int32_t status (MTL_OK or an error code);
+ *     the message of the last error on the calling thread is read with
+ *     mtl_last_error().  Status codes mirror MettleError (src/error.rs:5-11)
+ *     plus MTL_UNSUPPORTED, which tells the host to use DuckDB's CPU path.
+ *   - Row ids and selection vectors are uint32_t (limit 2^32-1 rows per
+ *     table, the same ceiling the reference has: helpers.rs:215-218).
+ *   - A selection vector lives on the device together with a device-resident
+ *     uint32_t count (the reference's GpuVector, gpu_buffer.rs:43-52).  A NULL
+ *     selection means "all rows 0..n-1".  Kernels read the count on the
+ *     device, so no host synchronisation is needed between operators.
+ *   - Selection vectors produced here are stable: ascending row ids.  (The
+ *     reference's atomic-counter compaction is unordered:
+ *     kernels.metal:684-696.)
+ *   - Integer arithmetic is exact two's complement with overflow detection
+ *     (DuckDB raises on overflow); FLOAT/DOUBLE follow IEEE round-to-nearest.
+ */
+#ifndef METTLE_HIP_H
+#define METTLE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status codes — MettleError (src/error.rs:5-11) + fallback/overflow.    */
+/* ---------------------------------------------------------------------- */
+#define MTL_OK 0
+#define MTL_ERR_INVALID_BUFFER_CONFIG 1 /* MettleError::InvalidBufferConfig */
+#define MTL_ERR_DEVICE_UNAVAILABLE 2    /* MettleError::MetalUnavailable    */
+#define MTL_ERR_NOT_INITIALIZED 3       /* MettleError::NotInitialized      */
+#define MTL_ERR_PROCESSING 4            /* MettleError::Processing          */
+#define MTL_UNSUPPORTED 5     /* shape not handled: run DuckDB CPU fallback  */
+#define MTL_ERR_INVALID_ARGUMENT 6
+#define MTL_ERR_OVERFLOW 7    /* integer overflow (DuckDB "Overflow in ...") */
+#define MTL_ERR_BUDGET 8      /* request exceeds gpu_buffer_init budgets    */
+
+/* ---------------------------------------------------------------------- */
+/* Logical column types (DuckDB names).  Physical widths in brackets.     */
+/* ---------------------------------------------------------------------- */
+#define MTL_INTEGER 1   /* int32   [4]                                     */
+#define MTL_BIGINT 2    /* int64   [8]                                     */
+#define MTL_FLOAT 3     /* float   [4]                                     */
+#define MTL_DOUBLE 4    /* double  [8]                                     */
+#define MTL_DATE 5      /* int32 days since 1970-01-01 [4]                 */
+#define MTL_TIMESTAMP 6 /* int64 microseconds since epoch [8]              */
+#define MTL_DECIMAL 7   /* int64 scaled integer (scale kept by caller) [8] */
+#define MTL_DICT8 8     /* uint8 dictionary code [1]                       */
+#define MTL_DICT32 9    /* int32 dictionary code [4]                       */
+#define MTL_HUGEINT 10  /* int128 {uint64 lower; int64 upper} [16]: SUM()  */
+#define MTL_VARCHAR 11  /* result-only: dictionary-decoded strings         */
+
+/* Compare ops — ScalarCompareOp codes (src/metal/shaders/mod.rs:27-36). */
+#define MTL_CMP_EQ 0
+#define MTL_CMP_LT 1
+#define MTL_CMP_LE 2
+#define MTL_CMP_GT 3
+#define MTL_CMP_GE 4
+
+/* Projection ops — ScalarProjectOp codes (src/metal/shaders/mod.rs:16-25). */
+#define MTL_PROJ_ADD 0
+#define MTL_PROJ_MUL 1
+#define MTL_PROJ_SUB 2     /* col - s */
+#define MTL_PROJ_SUBFROM 3 /* s - col */
+
+/* Aggregate functions — AggFunc (src/plan/ir.rs:100-107). */
+#define MTL_AGG_SUM 0
+#define MTL_AGG_COUNT 1 /* COUNT(col); COUNT(*) passes values == NULL */
+#define MTL_AGG_MIN 2
+#define MTL_AGG_MAX 3
+#define MTL_AGG_AVG 4
+
+/* Join strategies — GpuJoinStrategy (src/plan/ir.rs:176-180). */
+#define MTL_JOIN_DENSE 0
+#define MTL_JOIN_HASH 1
+
+/* A scalar literal as raw bits: int64 for integer-like types, the IEEE bits
+ * of a double for FLOAT/DOUBLE columns (rounded to float for FLOAT). */
+typedef struct mtl_pred {
+  const void *column; /* device pointer to the column (row-indexed)      */
+  int32_t type;       /* MTL_INTEGER ... MTL_DICT32                       */
+  int32_t op;         /* MTL_CMP_*                                        */
+  int64_t scalar;     /* literal bits                                     */
+} mtl_pred;
+
+#define MTL_MAX_PREDS 8
+
+typedef struct mtl_stream mtl_stream; /* HIP stream + scratch arena       */
+typedef struct mtl_result mtl_result; /* host-side materialised result    */
+typedef struct mtl_request mtl_request; /* in-flight serving request      */
+
+/* ====================================================================== */
+/* Lifecycle                                                              */
+/* ====================================================================== */
+
+/* Parse a human-readable size ("1.5 MB", "2gb", "64") with 1024-based units.
+ * Replaces parse_human_bytes (src/config.rs:61-123). */
+int32_t mtl_parse_bytes(const char *text, uint64_t *out_bytes);
+
+/* Initialise the backend with budgets; pinned_bytes == 0 means cache +
+ * processing.  Replaces BufferConfig::parse + state::initialize +
+ * metal::validate_runtime (config.rs:11-52, state.rs:23-40,
+ * metal/mod.rs:96-98).  device_name receives the HIP device name. */
+int32_t mtl_init(uint64_t cache_bytes, uint64_t processing_bytes,
+                 uint64_t pinned_bytes, char *device_name,
+                 size_t device_name_len);
+
+/* Same, parsing the three SQL string arguments of
+ * CALL gpu_buffer_init(cache, processing, pinned_memory_size := ...)
+ * (table_functions.rs:30-52).  pinned may be NULL. */
+int32_t mtl_buffer_init(const char *cache, const char *processing,
+                        const char *pinned, char *device_name,
+                        size_t device_name_len);
+
+/* Budgets after init; returns MTL_ERR_NOT_INITIALIZED before. */
+int32_t mtl_budgets(uint64_t *cache_bytes, uint64_t *processing_bytes,
+                    uint64_t *pinned_bytes);
+
+/* Drop every cached column, result and scratch arena (state::reset_for_tests,
+ * metal::reset_runtime_cache: state.rs:113-129, metal/mod.rs:106-117). */
+int32_t mtl_shutdown(void);
+
+/* Copy the calling thread's last error message (NUL-terminated). */
+int32_t mtl_last_error(char *buf, size_t len);
+
+/* Device count visible to HIP without initialising the device context. */
+int32_t mtl_device_count(int32_t *out);
+
+/* ====================================================================== */
+/* Streams (replace begin_batch/end_batch command-buffer batching,        */
+/* runtime/mod.rs:182-236).                                               */
+/* ====================================================================== */
+int32_t mtl_stream_create(mtl_stream **out);
+int32_t mtl_stream_destroy(mtl_stream *s);
+int32_t mtl_stream_sync(mtl_stream *s); /* = end_batch's wait              */
+/* The hipStream_t behind a handle (for callers that share streams). */
+void *mtl_stream_native(mtl_stream *s);
+
+/* Device memory from the processing budget (GpuBuffer::new / upload).    */
+int32_t mtl_alloc(uint64_t bytes, void **out_dev);
+int32_t mtl_free(void *dev);
+int32_t mtl_memcpy_h2d(void *dst_dev, const void *src_host, uint64_t bytes,
+                       mtl_stream *s);
+int32_t mtl_memcpy_d2h(void *dst_host, const void *src_dev, uint64_t bytes,
+                       mtl_stream *s);
+
+/* ====================================================================== */
+/* Operators (device pointers in, device pointers out)                    */
+/* ====================================================================== */
+
+/* Fused predicate conjunction + stable selection-vector compaction.
+ * out_sel[0..*out_count) = ascending row ids r (from in_sel when given,
+ * else 0..n_rows-1) for which every preds[i] holds.
+ * Replaces compare_{i32,i64,f32,date32}_to_gpu + and_masks_gpu +
+ * filter_to_indices_{gpu,gv} (metal/mod.rs:200-265, scalar_ops.rs:282-439,
+ * gather_ops.rs:23-159; kernels K4/K5/K6/K23/K24/K38).
+ * in_count: device count of in_sel (NULL -> in_capacity rows).  out_sel must
+ * hold in_capacity (or n_rows) entries; out_count is a device uint32. */
+int32_t mtl_filter(const mtl_pred *preds, uint32_t n_preds, uint64_t n_rows,
+                   const uint32_t *in_sel, const uint32_t *in_count,
+                   uint64_t in_capacity, uint32_t *out_sel,
+                   uint32_t *out_count, mtl_stream *s);
+
+/* Per-row mask (uint8 0/1) of a predicate conjunction over rows 0..n-1.
+ * Replaces compare_*_values (metal/mod.rs:146-198; scalar_ops.rs:85-279). */
+int32_t mtl_compare(const mtl_pred *preds, uint32_t n_preds, uint64_t n_rows,
+                    uint8_t *out_mask, mtl_stream *s);
+
+/* out[i] = column[sel[i]] for i < count; width 1, 4 or 8 bytes.
+ * Replaces gather_{i32,f32,i32_as_f32}_gv (gather_ops.rs:162-236, K26-K28)
+ * and reindex_gv (gather_ops.rs:250-295, K29) when column is a selection. */
+int32_t mtl_gather(const void *column, uint32_t width, const uint32_t *sel,
+                   const uint32_t *count, uint64_t capacity, void *out,
+                   mtl_stream *s);
+
+/* Scalar projection out[i] = column[sel[i]] op scalar (sel NULL -> i).
+ * INTEGER/BIGINT/DATE(+/-)/TIMESTAMP(+/-)/DECIMAL: exact, sets
+ * *overflow_flag (device uint32) to 1 on overflow.  FLOAT: IEEE f32.
+ * DOUBLE: IEEE f64.  Replaces project_f32_values / project_f32_scalar_gv
+ * (metal/mod.rs:146-156; scalar_ops.rs:32-82, 512-561; K3/K35), which
+ * computed everything in f32. */
+int32_t mtl_project(const void *column, int32_t type, const uint32_t *sel,
+                    const uint32_t *count, uint64_t capacity, int32_t op,
+                    int64_t scalar, void *out, uint32_t *overflow_flag,
+                    mtl_stream *s);
+
+/* Bytes of the device table a join build needs (dense: slots x 4 B;
+ * hash: next_pow2(max(2 x rows, 16)) x 16 B).  Mirrors the capacity rules of
+ * choose_gpu_join_strategy / hash_join_capacity (join_plan.rs:29-56,
+ * 300-303). */
+uint64_t mtl_join_table_bytes(int32_t strategy, uint64_t build_rows,
+                              uint64_t dense_slots);
+
+/* Build side of an inner PK/FK equi-join on an INTEGER or BIGINT key.
+ * Dense: table[key - min_key] = build row (slots = span).  Hash: open
+ * addressing on 64-bit keys with a mixing hash.  *dup_flag (device uint32)
+ * becomes 1 if a key repeats (the reference overwrote duplicates:
+ * kernels.metal:202-206); the host then falls back.
+ * Replaces hash_join_build_gv / dense_lookup_build_i32_gv /
+ * dense_lookup_build_from_indices_i32_gv (join_ops.rs:22-70, 128-180,
+ * 233-290; K31, K32, K33). */
+int32_t mtl_join_build(int32_t strategy, const void *keys, int32_t key_type,
+                       const uint32_t *sel, const uint32_t *count,
+                       uint64_t capacity, int64_t min_key, uint64_t slots,
+                       void *table, uint64_t table_bytes, uint32_t *dup_flag,
+                       mtl_stream *s);
+
+/* Probe side: for probe rows (sel or 0..n-1) whose key matches, emit the
+ * pair (probe row, build row) in ascending probe order.  `slots` is the
+ * dense key span, or for MTL_JOIN_HASH the build capacity (rows) that was
+ * passed to mtl_join_build.
+ * Replaces hash_join_probe_gv / dense_lookup_probe_i32_gv /
+ * compact_matches_gv (join_ops.rs:72-125, 182-231; gather_ops.rs:298-393;
+ * K30, K31, K32). */
+int32_t mtl_join_probe(int32_t strategy, const void *table,
+                       uint64_t table_bytes, int64_t min_key, uint64_t slots,
+                       const void *probe_keys, int32_t key_type,
+                       const uint32_t *sel, const uint32_t *count,
+                       uint64_t capacity, uint32_t *out_probe,
+                       uint32_t *out_build, uint32_t *out_count,
+                       mtl_stream *s);
+
+/* ---------------------------------------------------------------------- */
+/* Star join -> grouped SUM -> top-K (the TPC-H Q3 shape).                */
+/* ---------------------------------------------------------------------- */
+
+/* Expression program over fact-table columns, evaluated per qualifying
+ * fact row in int64 (exact, overflow-checked) or double.  Postfix ops: */
+#define MTL_EXPR_COL 0   /* push column arg (index into cols)            */
+#define MTL_EXPR_CONST 1 /* push constant arg (int64 or double bits)     */
+#define MTL_EXPR_ADD 2
+#define MTL_EXPR_SUB 3
+#define MTL_EXPR_MUL 4
+#define MTL_EXPR_MAX_OPS 16
+#define MTL_EXPR_MAX_COLS 4
+
+typedef struct mtl_expr {
+  uint32_t n_ops;
+  int32_t is_double; /* 0: int64 arithmetic, 1: double arithmetic       */
+  int32_t op[MTL_EXPR_MAX_OPS];
+  int64_t arg[MTL_EXPR_MAX_OPS];
+  uint32_t n_cols;
+  const void *col[MTL_EXPR_MAX_COLS];
+  int32_t col_type[MTL_EXPR_MAX_COLS];
+} mtl_expr;
+
+#define MTL_STAR_MAX_TIES 3
+
+typedef struct mtl_star_desc {
+  /* dimension table D (optional: d_rows == 0 skips it) */
+  uint64_t d_rows;
+  const void *d_key; /* unique integer key of D            */
+  int32_t d_key_type;
+  int64_t d_key_min, d_key_max; /* column stats            */
+  uint32_t d_n_preds;
+  mtl_pred d_preds[MTL_MAX_PREDS];
+  /* parent table P: grouped by its unique key (the group representative) */
+  uint64_t p_rows;
+  const void *p_key; /* unique integer key of P            */
+  int32_t p_key_type;
+  int64_t p_key_min, p_key_max;
+  const void *p_fk; /* P -> D foreign key (ignored when d_rows == 0) */
+  int32_t p_fk_type;
+  uint32_t p_n_preds;
+  mtl_pred p_preds[MTL_MAX_PREDS];
+  /* fact table F */
+  uint64_t f_rows;
+  const void *f_fk; /* F -> P foreign key                  */
+  int32_t f_fk_type;
+  uint32_t f_n_preds;
+  mtl_pred f_preds[MTL_MAX_PREDS];
+  mtl_expr agg; /* SUM(agg) per P row                     */
+  /* ORDER BY SUM(agg) DESC, then ties on P columns, then P key ASC */
+  uint32_t k;
+  uint32_t n_ties;
+  const void *tie_col[MTL_STAR_MAX_TIES];
+  int32_t tie_type[MTL_STAR_MAX_TIES];
+  int32_t tie_desc[MTL_STAR_MAX_TIES];
+} mtl_star_desc;
+
+/* One result row of the star operator (host side). */
+typedef struct mtl_star_row {
+  uint32_t p_row;  /* row id in P (gather the group keys from it)   */
+  uint32_t count;  /* number of fact rows summed                    */
+  int64_t sum_i64; /* SUM when agg.is_double == 0                   */
+  double sum_f64;  /* SUM when agg.is_double == 1                   */
+} mtl_star_row;
+
+/* Runs filter(D) -> semi-join bitmap -> filter(P) + probe -> filter(F) +
+ * probe -> SUM(expr) per P row -> top-K, all device-resident with one
+ * host synchronisation at the end (TARGET_ARCHITECTURE.md:11-17).
+ * out_rows receives min(k, groups) rows in order; *out_n their number.
+ * Replaces the GPU-native Q3 pipeline execute_gpu_native
+ * (executor/fast_path/mod.rs:58-141) and its kernels K23/K24/K26/K29-K33/
+ * K36/K37/K39-K44 (aggregate_ops.rs:30-98, 572-932). */
+int32_t mtl_star_topk(const mtl_star_desc *desc, mtl_star_row *out_rows,
+                      uint32_t *out_n, mtl_stream *s);
+
+/* Same, but only enqueues the work; results land in the device buffer
+ * out_dev (k x sizeof(mtl_star_row)) and out_n_dev (device, 2 x uint32:
+ * [0] = rows, [1] = error bits, 0 when the run is valid; decode with
+ * mtl_star_status).  Used by the serving loop and by graph capture. */
+int32_t mtl_star_topk_async(const mtl_star_desc *desc, void *out_dev,
+                            uint32_t *out_n_dev, mtl_stream *s);
+
+/* Map the error bits of an asynchronous star run to a status (+ message). */
+int32_t mtl_star_status(uint32_t error_bits);
+
+/* ====================================================================== */
+/* Kernel timing: hipEvent pairs around every kernel launch, resolved at  */
+/* mtl_stream_sync.  Counterpart of METTLE_PROFILE                        */
+/* (src/plan/executor/cache.rs:10-21), which timed phases on the host.    */
+/* ====================================================================== */
+int32_t mtl_profile_enable(int32_t on);
+int32_t mtl_profile_reset(void);
+/* Entry `index` (0-based, sorted by name): kernel name, summed ms, count.
+ * Returns MTL_ERR_INVALID_ARGUMENT past the last entry. */
+int32_t mtl_profile_get(uint32_t index, char *name, size_t name_len,
+                        double *total_ms, uint64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* METTLE_HIP_H */

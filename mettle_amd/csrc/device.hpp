@@ -1,0 +1,196 @@
+// Device-side helpers for gfx950 (CDNA4, wave64): atomics with explicit
+// agent scope, wave reductions, the decoupled look-back tile scan used by
+// every stable compaction, and typed predicate evaluation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mettle_hip.h"
+
+namespace mtl {
+namespace dev {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_inclusive_u32(uint32_t v) {
+  int l = lane_id();
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    uint32_t o = __shfl_up(v, off, WAVE);
+    if (l >= off) v += o;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Decoupled look-back (single-pass prefix over tiles).  Status word per tile:
+// low 32 bits = value, bits 32..33 = flag (0 not ready, 1 aggregate,
+// 2 inclusive prefix).  Tiles are numbered in launch order through an atomic
+// ticket, so a tile only ever waits on tiles that are already running, and
+// every wait ends.  A spin cap sets *err and stops waiting (defensive; it is
+// never reached when the status array was cleared before the launch).
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFlagAgg = 1ull << 32;
+constexpr uint64_t kFlagInc = 2ull << 32;
+constexpr uint32_t kSpinCap = 1u << 24;
+
+// Executed by one full wave (all 64 lanes active).  Returns the exclusive
+// prefix of `tile`, whose own total is `total`.
+__device__ __forceinline__ uint32_t lookback_wave(uint64_t* status,
+                                                  uint32_t tile,
+                                                  uint32_t total,
+                                                  uint32_t* err) {
+  const int lane = lane_id();
+  if (tile == 0) {
+    if (lane == 0) st_agent(&status[0], kFlagInc | total);
+    return 0;
+  }
+  if (lane == 0) st_agent(&status[tile], kFlagAgg | total);
+  uint32_t excl = 0;
+  int64_t top = (int64_t)tile - 1;
+  uint32_t spins = 0;
+  while (true) {
+    int64_t idx = top - lane;
+    uint64_t st = idx >= 0 ? ld_agent(&status[idx]) : kFlagInc;
+    uint32_t flag = (uint32_t)(st >> 32);
+    uint64_t inc = __ballot(flag == 2);
+    uint64_t nr = __ballot(flag == 0);
+    int first_inc = inc ? __ffsll((unsigned long long)inc) - 1 : 64;
+    int first_nr = nr ? __ffsll((unsigned long long)nr) - 1 : 64;
+    if (first_nr < first_inc) {
+      // a closer predecessor has not published yet: wait for it
+      if (++spins > kSpinCap) {
+        if (lane == 0) atomicOr(err, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint32_t v = (uint32_t)st;
+    uint32_t contrib = (lane <= first_inc) ? v : 0u;
+    excl += wave_sum_u32(contrib);
+    if (first_inc < 64) break;
+    top -= 64;
+  }
+  if (lane == 0) st_agent(&status[tile], kFlagInc | (excl + total));
+  return excl;
+}
+
+// ---------------------------------------------------------------------------
+// Predicates.  DuckDB orders floats totally: NaN equals NaN and is greater
+// than every other value (DuckDB's Equals/GreaterThan on float/double).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool cmp_op(T a, T b, int op) {
+  switch (op) {
+    case MTL_CMP_EQ: return a == b;
+    case MTL_CMP_LT: return a < b;
+    case MTL_CMP_LE: return a <= b;
+    case MTL_CMP_GT: return a > b;
+    default: return a >= b;
+  }
+}
+
+template <typename F>
+__device__ __forceinline__ bool cmp_float(F a, F b, int op) {
+  bool an = a != a, bn = b != b;
+  if (!an && !bn) return cmp_op<F>(a, b, op);
+  // total order with NaN as the largest value
+  int c = (an && bn) ? 0 : (an ? 1 : -1);
+  switch (op) {
+    case MTL_CMP_EQ: return c == 0;
+    case MTL_CMP_LT: return c < 0;
+    case MTL_CMP_LE: return c <= 0;
+    case MTL_CMP_GT: return c > 0;
+    default: return c >= 0;
+  }
+}
+
+// Evaluate one predicate for a row.
+__device__ __forceinline__ bool eval_pred_row(const mtl_pred& p, uint64_t row) {
+  switch (p.type) {
+    case MTL_INTEGER:
+    case MTL_DATE:
+    case MTL_DICT32:
+      return cmp_op<int32_t>(((const int32_t*)p.column)[row], (int32_t)p.scalar,
+                             p.op);
+    case MTL_BIGINT:
+    case MTL_TIMESTAMP:
+    case MTL_DECIMAL:
+      return cmp_op<int64_t>(((const int64_t*)p.column)[row], p.scalar, p.op);
+    case MTL_DICT8:
+      return cmp_op<int32_t>((int32_t)((const uint8_t*)p.column)[row],
+                             (int32_t)p.scalar, p.op);
+    case MTL_FLOAT:
+      return cmp_float<float>(((const float*)p.column)[row],
+                              (float)__longlong_as_double(p.scalar), p.op);
+    case MTL_DOUBLE:
+      return cmp_float<double>(((const double*)p.column)[row],
+                               __longlong_as_double(p.scalar), p.op);
+    default:
+      return false;
+  }
+}
+
+// Read an integer-like key column as int64.
+__device__ __forceinline__ int64_t load_key(const void* col, int32_t type,
+                                            uint64_t row) {
+  switch (type) {
+    case MTL_BIGINT:
+    case MTL_TIMESTAMP:
+    case MTL_DECIMAL:
+      return ((const int64_t*)col)[row];
+    case MTL_DICT8:
+      return (int64_t)((const uint8_t*)col)[row];
+    default:
+      return (int64_t)((const int32_t*)col)[row];
+  }
+}
+
+// Order-preserving unsigned encodings (smaller encoded = smaller value).
+__device__ __forceinline__ uint64_t enc_i64(int64_t v) {
+  return (uint64_t)v ^ 0x8000000000000000ull;
+}
+__device__ __forceinline__ uint64_t enc_f64(double d) {
+  // DuckDB total order: NaN largest; -0.0 == 0.0
+  if (d != d) return ~0ull;
+  if (d == 0.0) d = 0.0;
+  uint64_t b = (uint64_t)__double_as_longlong(d);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ uint64_t enc_value(const void* col, int32_t type,
+                                              uint64_t row) {
+  switch (type) {
+    case MTL_FLOAT:
+      return enc_f64((double)((const float*)col)[row]);
+    case MTL_DOUBLE:
+      return enc_f64(((const double*)col)[row]);
+    default:
+      return enc_i64(load_key(col, type, row));
+  }
+}
+
+}  // namespace dev
+}  // namespace mtl

@@ -1,0 +1,356 @@
+// Runtime: errors, budgets (gpu_buffer_init), device bring-up, stream
+// handles and their scratch arenas, device memory helpers.
+//
+// Reference counterparts: src/config.rs (budget parsing), src/state.rs
+// (initialise / not-initialised error), src/metal/runtime/mod.rs:72-151
+// (device + queue creation), 182-236 (batching -> streams here).
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "mtl_internal.hpp"
+#include "prof.hpp"
+
+namespace mtl {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int32_t set_error(int32_t code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+Runtime& runtime() {
+  static Runtime* rt = new Runtime();  // never destroyed: outlives atexit
+  return *rt;
+}
+
+int32_t Arena::alloc(size_t bytes, void** out) {
+  size_t need = (bytes + 255) & ~size_t(255);
+  if (need == 0) need = 256;
+  if (chunks.empty() || off + need > chunks.back().cap) {
+    size_t cap = chunks.empty() ? (size_t(64) << 20) : chunks.back().cap * 2;
+    while (cap < need) cap *= 2;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cap);
+    if (e != hipSuccess)
+      return set_error(MTL_ERR_PROCESSING,
+                       "processing error: scratch arena hipMalloc(%zu): %s",
+                       cap, hipGetErrorString(e));
+    runtime().device_bytes_in_use += cap;
+    chunks.push_back({(char*)p, cap});
+    off = 0;
+  }
+  *out = chunks.back().base + off;
+  off += need;
+  if (off > high_water) high_water = off;
+  return MTL_OK;
+}
+
+void Arena::release_retired() {
+  while (chunks.size() > 1) {
+    hipFree(chunks.front().base);
+    runtime().device_bytes_in_use -= chunks.front().cap;
+    chunks.erase(chunks.begin());
+  }
+}
+
+void Arena::release_all() {
+  for (auto& c : chunks) {
+    hipFree(c.base);
+    runtime().device_bytes_in_use -= c.cap;
+  }
+  chunks.clear();
+  off = 0;
+}
+
+int32_t ensure_device() {
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> g(rt.mu);
+  if (rt.default_stream) return MTL_OK;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0)
+    return set_error(MTL_ERR_DEVICE_UNAVAILABLE,
+                     "metal unavailable: no HIP device is visible (%s)",
+                     e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+  MTL_HIP(hipSetDevice(rt.device));
+  hipDeviceProp_t prop;
+  MTL_HIP(hipGetDeviceProperties(&prop, rt.device));
+  rt.device_name = prop.name;
+  rt.arch_name = prop.gcnArchName;
+  rt.cu_count = prop.multiProcessorCount;
+  mtl_stream* s = new mtl_stream();
+  e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return set_error(MTL_ERR_DEVICE_UNAVAILABLE,
+                     "metal unavailable: hipStreamCreate: %s",
+                     hipGetErrorString(e));
+  }
+  s->owns_stream = true;
+  rt.default_stream = s;
+  return MTL_OK;
+}
+
+int32_t resolve_stream(mtl_stream* s, mtl_stream** out) {
+  if (s) {
+    *out = s;
+    return MTL_OK;
+  }
+  MTL_TRY(ensure_device());
+  *out = runtime().default_stream;
+  return MTL_OK;
+}
+
+int32_t stream_pinned(mtl_stream* s, size_t bytes, void** out) {
+  if (s->pinned_small_bytes < bytes) {
+    if (s->pinned_small) hipHostFree(s->pinned_small);
+    s->pinned_small = nullptr;
+    size_t cap = 4096;
+    while (cap < bytes) cap *= 2;
+    MTL_HIP(hipHostMalloc(&s->pinned_small, cap, hipHostMallocDefault));
+    s->pinned_small_bytes = cap;
+  }
+  *out = s->pinned_small;
+  return MTL_OK;
+}
+
+// parse_human_bytes semantics (src/config.rs:61-123): digits and '.' form
+// the number, whitespace is ignored, the rest is a 1024-based unit.
+static int32_t parse_bytes(const char* raw, uint64_t* out) {
+  if (!raw) return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                             "invalid buffer config: buffer sizes cannot be empty");
+  std::string s(raw);
+  size_t a = s.find_first_not_of(" \t\r\n");
+  size_t b = s.find_last_not_of(" \t\r\n");
+  if (a == std::string::npos)
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: buffer sizes cannot be empty");
+  s = s.substr(a, b - a + 1);
+  std::string num, unit;
+  for (char ch : s) {
+    if ((ch >= '0' && ch <= '9') || ch == '.') {
+      if (!unit.empty())
+        return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                         "invalid buffer config: invalid size literal '%s'", raw);
+      num.push_back(ch);
+    } else if (ch == ' ' || ch == '\t' || ch == '\r' || ch == '\n') {
+      continue;
+    } else {
+      unit.push_back(ch);
+    }
+  }
+  if (num.empty())
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: missing numeric value in '%s'", raw);
+  // exactly one optional '.', digits elsewhere (Rust f64::from_str accepts
+  // "1.", ".5", rejects "1.2.3")
+  size_t dots = 0;
+  for (char ch : num) dots += ch == '.';
+  if (dots > 1 || num == ".")
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: invalid numeric value in '%s'", raw);
+  double q = strtod(num.c_str(), nullptr);
+  if (!std::isfinite(q) || q < 0)
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: invalid size literal '%s'", raw);
+  std::string u;
+  for (char ch : unit) u.push_back((char)tolower((unsigned char)ch));
+  double mult;
+  if (u.empty() || u == "b") mult = 1;
+  else if (u == "k" || u == "kb" || u == "kib") mult = 1024.0;
+  else if (u == "m" || u == "mb" || u == "mib") mult = 1024.0 * 1024.0;
+  else if (u == "g" || u == "gb" || u == "gib") mult = 1024.0 * 1024.0 * 1024.0;
+  else if (u == "t" || u == "tb" || u == "tib")
+    mult = 1024.0 * 1024.0 * 1024.0 * 1024.0;
+  else
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: unsupported size unit '%s' in '%s'",
+                     u.c_str(), raw);
+  double bytes = q * mult;
+  if (bytes > 18446744073709551615.0)
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: size literal '%s' overflowed u64", raw);
+  *out = (uint64_t)llround(bytes);
+  return MTL_OK;
+}
+
+}  // namespace mtl
+
+using namespace mtl;
+
+extern "C" {
+
+int32_t mtl_parse_bytes(const char* text, uint64_t* out_bytes) {
+  if (!out_bytes) return set_error(MTL_ERR_INVALID_ARGUMENT, "out_bytes is NULL");
+  return parse_bytes(text, out_bytes);
+}
+
+int32_t mtl_init(uint64_t cache_bytes, uint64_t processing_bytes,
+                 uint64_t pinned_bytes, char* device_name,
+                 size_t device_name_len) {
+  if (pinned_bytes == 0) {
+    if (cache_bytes + processing_bytes < cache_bytes)
+      return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                       "invalid buffer config: pinned memory default overflowed "
+                       "the configured budgets");
+    pinned_bytes = cache_bytes + processing_bytes;
+  }
+  if (cache_bytes == 0)
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: gpu_buffer_init requires a non-zero "
+                     "cache region");
+  if (processing_bytes == 0)
+    return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                     "invalid buffer config: gpu_buffer_init requires a non-zero "
+                     "processing region");
+  MTL_TRY(ensure_device());
+  Runtime& rt = runtime();
+  {
+    std::lock_guard<std::mutex> g(rt.mu);
+    rt.cache_bytes = cache_bytes;
+    rt.processing_bytes = processing_bytes;
+    rt.pinned_bytes = pinned_bytes;
+    rt.initialized = true;
+  }
+  if (device_name && device_name_len) {
+    snprintf(device_name, device_name_len, "%s", rt.device_name.c_str());
+  }
+  return MTL_OK;
+}
+
+int32_t mtl_buffer_init(const char* cache, const char* processing,
+                        const char* pinned, char* device_name,
+                        size_t device_name_len) {
+  uint64_t c = 0, p = 0, pin = 0;
+  MTL_TRY(parse_bytes(cache, &c));
+  MTL_TRY(parse_bytes(processing, &p));
+  if (pinned) {
+    MTL_TRY(parse_bytes(pinned, &pin));
+    if (pin == 0)
+      return set_error(MTL_ERR_INVALID_BUFFER_CONFIG,
+                       "invalid buffer config: gpu_buffer_init requires a non-zero "
+                       "pinned region");
+  }
+  return mtl_init(c, p, pin, device_name, device_name_len);
+}
+
+int32_t mtl_budgets(uint64_t* cache_bytes, uint64_t* processing_bytes,
+                    uint64_t* pinned_bytes) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> g(rt.mu);
+  if (!rt.initialized)
+    return set_error(MTL_ERR_NOT_INITIALIZED,
+                     "not initialized: call gpu_buffer_init before gpu_processing");
+  if (cache_bytes) *cache_bytes = rt.cache_bytes;
+  if (processing_bytes) *processing_bytes = rt.processing_bytes;
+  if (pinned_bytes) *pinned_bytes = rt.pinned_bytes;
+  return MTL_OK;
+}
+
+int32_t mtl_shutdown(void) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> g(rt.mu);
+  if (rt.default_stream) {
+    hipStreamSynchronize(rt.default_stream->stream);
+    rt.default_stream->arena.release_all();
+  }
+  rt.initialized = false;
+  rt.cache_bytes = rt.processing_bytes = rt.pinned_bytes = 0;
+  return MTL_OK;
+}
+
+int32_t mtl_last_error(char* buf, size_t len) {
+  if (buf && len) snprintf(buf, len, "%s", last_error());
+  return MTL_OK;
+}
+
+int32_t mtl_device_count(int32_t* out) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  if (out) *out = n;
+  return MTL_OK;
+}
+
+int32_t mtl_stream_create(mtl_stream** out) {
+  if (!out) return set_error(MTL_ERR_INVALID_ARGUMENT, "out is NULL");
+  MTL_TRY(ensure_device());
+  mtl_stream* s = new mtl_stream();
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return set_error(MTL_ERR_PROCESSING, "processing error: hipStreamCreate: %s",
+                     hipGetErrorString(e));
+  }
+  s->owns_stream = true;
+  *out = s;
+  return MTL_OK;
+}
+
+int32_t mtl_stream_destroy(mtl_stream* s) {
+  if (!s) return MTL_OK;
+  if (s == runtime().default_stream)
+    return set_error(MTL_ERR_INVALID_ARGUMENT, "cannot destroy the default stream");
+  hipStreamSynchronize(s->stream);
+  s->arena.release_all();
+  if (s->pinned_small) hipHostFree(s->pinned_small);
+  if (s->owns_stream) hipStreamDestroy(s->stream);
+  delete s;
+  return MTL_OK;
+}
+
+int32_t mtl_stream_sync(mtl_stream* s) {
+  MTL_TRY(resolve_stream(s, &s));
+  MTL_HIP(hipStreamSynchronize(s->stream));
+  prof_collect(s);
+  s->arena.release_retired();
+  return MTL_OK;
+}
+
+void* mtl_stream_native(mtl_stream* s) {
+  if (resolve_stream(s, &s) != MTL_OK) return nullptr;
+  return (void*)s->stream;
+}
+
+int32_t mtl_alloc(uint64_t bytes, void** out_dev) {
+  if (!out_dev) return set_error(MTL_ERR_INVALID_ARGUMENT, "out_dev is NULL");
+  MTL_TRY(ensure_device());
+  MTL_HIP(hipMalloc(out_dev, bytes ? bytes : 1));
+  runtime().device_bytes_in_use += bytes;
+  return MTL_OK;
+}
+
+int32_t mtl_free(void* dev) {
+  if (dev) MTL_HIP(hipFree(dev));
+  return MTL_OK;
+}
+
+int32_t mtl_memcpy_h2d(void* dst_dev, const void* src_host, uint64_t bytes,
+                       mtl_stream* s) {
+  MTL_TRY(resolve_stream(s, &s));
+  MTL_HIP(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice,
+                         s->stream));
+  MTL_HIP(hipStreamSynchronize(s->stream));
+  return MTL_OK;
+}
+
+int32_t mtl_memcpy_d2h(void* dst_host, const void* src_dev, uint64_t bytes,
+                       mtl_stream* s) {
+  MTL_TRY(resolve_stream(s, &s));
+  MTL_HIP(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost,
+                         s->stream));
+  MTL_HIP(hipStreamSynchronize(s->stream));
+  return MTL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// Block-level top-K over candidates keyed by up to four 64-bit words plus a
+// 32-bit id, lexicographically smaller = better.  Used by the star pipeline
+// and by the grouped-aggregate top-K.
+//
+// Reference: grouped_candidate_better (kernels.metal:1102-1118) compared
+// (sum desc, secondary asc, row id asc) and topk_dense_grouped_rows_parallel_v
+// (:1249-1390) ran one 128-thread group with K <= 20.  Here every block
+// keeps a candidate buffer in LDS, admits only entries whose primary key
+// does not lose to the current K-th best, and the last block to finish
+// merges all block results; K <= TOPK_MAX_K.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mtl {
+namespace dev {
+
+constexpr int TOPK_BLOCK = 256;
+constexpr int TOPK_CAP = 1024;          // LDS candidate buffer entries
+constexpr int TOPK_ROUND = 2 * TOPK_BLOCK;  // entries admitted per round
+constexpr int TOPK_MAX_K = TOPK_CAP - TOPK_ROUND;  // 512
+
+struct TopkLds {
+  uint64_t k[4][TOPK_CAP];
+  uint32_t id[TOPK_CAP];
+  uint32_t n;
+  uint64_t thr;  // admit entries with k[0] <= thr
+};
+
+__device__ __forceinline__ bool cand_less(const TopkLds& s, int a, int b) {
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (s.k[f][a] != s.k[f][b]) return s.k[f][a] < s.k[f][b];
+  }
+  return s.id[a] < s.id[b];
+}
+
+__device__ __forceinline__ void cand_swap(TopkLds& s, int a, int b) {
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    uint64_t t = s.k[f][a];
+    s.k[f][a] = s.k[f][b];
+    s.k[f][b] = t;
+  }
+  uint32_t t = s.id[a];
+  s.id[a] = s.id[b];
+  s.id[b] = t;
+}
+
+// Sort the first n entries ascending (best first); pads to a power of two.
+// All threads of the block must call it.
+__device__ void topk_sort(TopkLds& s, uint32_t n) {
+  uint32_t p = 2;
+  while (p < n) p <<= 1;
+  for (uint32_t i = n + threadIdx.x; i < p; i += blockDim.x) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) s.k[f][i] = ~0ull;
+    s.id[i] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= p; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = threadIdx.x; t < p / 2; t += blockDim.x) {
+        uint32_t i = 2 * t - (t & (stride - 1));
+        uint32_t j = i + stride;
+        bool up = (i & size) == 0;
+        bool worse = cand_less(s, j, i);  // a[i] > a[j]
+        if (worse == up) cand_swap(s, i, j);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// After a sort: keep the best k and tighten the admission threshold.
+__device__ __forceinline__ void topk_truncate(TopkLds& s, uint32_t k) {
+  if (threadIdx.x == 0) {
+    if (s.n > k) s.n = k;
+    if (s.n == k && k > 0) s.thr = s.k[0][k - 1];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void topk_init(TopkLds& s) {
+  if (threadIdx.x == 0) {
+    s.n = 0;
+    s.thr = ~0ull;
+  }
+  __syncthreads();
+}
+
+// Append a candidate (caller checked k0 <= thr).
+__device__ __forceinline__ void topk_push(TopkLds& s, uint64_t k0, uint64_t k1,
+                                          uint64_t k2, uint64_t k3, uint32_t id) {
+  uint32_t pos = atomicAdd(&s.n, 1u);
+  s.k[0][pos] = k0;
+  s.k[1][pos] = k1;
+  s.k[2][pos] = k2;
+  s.k[3][pos] = k3;
+  s.id[pos] = id;
+}
+
+// Called by all threads after each admission round.
+__device__ __forceinline__ void topk_maybe_compact(TopkLds& s, uint32_t k) {
+  __syncthreads();
+  if (s.n > (uint32_t)(TOPK_CAP - TOPK_ROUND)) {
+    uint32_t n = s.n;
+    __syncthreads();
+    topk_sort(s, n);
+    topk_truncate(s, k);
+  }
+}
+
+}  // namespace dev
+}  // namespace mtl

@@ -1,0 +1,133 @@
+"""Operator calls on device tensors.
+
+Thin wrappers that pass torch CUDA tensor addresses to libmettle_hip.so.
+torch only provides device memory here; every computation runs in the HIP
+library.  Each call synchronises torch's stream first (inputs made by torch
+are then complete) and, for synchronous helpers, the library stream after.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import abi
+
+_NP_TO_TORCH = {np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64,
+                np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64,
+                np.dtype(np.uint8): torch.uint8, np.dtype(np.uint32): torch.int32}
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the mettle HIP backend needs an MI355X")
+    return torch.device("cuda:0")
+
+
+def to_device(a: np.ndarray) -> torch.Tensor:
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device())
+
+
+def to_host(t: torch.Tensor, dtype=None) -> np.ndarray:
+    a = t.detach().cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def sync() -> None:
+    torch.cuda.synchronize()
+    abi.check(abi.lib().mtl_stream_sync(None))
+
+
+def _preds(preds):
+    arr = (abi.Pred * max(1, len(preds)))()
+    for i, (col, t, op, val) in enumerate(preds):
+        arr[i] = abi.make_pred(ptr(col), t, op, val)
+    return arr
+
+
+def filter(preds, n_rows: int, in_sel: torch.Tensor | None = None,
+           in_count: torch.Tensor | None = None):
+    """preds: list of (device column, type, op, literal).  Returns (sel, count)."""
+    torch.cuda.synchronize()
+    cap = in_sel.numel() if in_sel is not None else n_rows
+    out = torch.empty(max(cap, 4), dtype=torch.int32, device=device())
+    cnt = torch.zeros(1, dtype=torch.int32, device=device())
+    abi.check(abi.lib().mtl_filter(_preds(preds), len(preds), n_rows, ptr(in_sel),
+                                   ptr(in_count), cap, ptr(out), ptr(cnt), None))
+    sync()
+    n = int(cnt.item())
+    return out[:n], n
+
+
+def compare(preds, n_rows: int) -> torch.Tensor:
+    torch.cuda.synchronize()
+    out = torch.empty(max(n_rows, 1), dtype=torch.uint8, device=device())
+    abi.check(abi.lib().mtl_compare(_preds(preds), len(preds), n_rows, ptr(out), None))
+    sync()
+    return out[:n_rows]
+
+
+def gather(col: torch.Tensor, sel: torch.Tensor) -> torch.Tensor:
+    torch.cuda.synchronize()
+    out = torch.empty(max(sel.numel(), 1), dtype=col.dtype, device=device())
+    abi.check(abi.lib().mtl_gather(ptr(col), col.element_size(), ptr(sel), None, sel.numel(),
+                                   ptr(out), None))
+    sync()
+    return out[:sel.numel()]
+
+
+def project(col: torch.Tensor, t: int, sel: torch.Tensor | None, op: int, value):
+    torch.cuda.synchronize()
+    n = sel.numel() if sel is not None else col.numel()
+    out = torch.empty(max(n, 1), dtype=col.dtype, device=device())
+    flag = torch.zeros(1, dtype=torch.int32, device=device())
+    abi.check(abi.lib().mtl_project(ptr(col), t, ptr(sel), None, n, op,
+                                    abi.scalar_bits(value, t), ptr(out), ptr(flag), None))
+    sync()
+    return out[:n], bool(flag.item())
+
+
+def join(strategy: int, bkeys: torch.Tensor, btype: int, pkeys: torch.Tensor, ptype: int,
+         min_key: int = 0, slots: int = 0, bsel: torch.Tensor | None = None,
+         psel: torch.Tensor | None = None):
+    """Build on bkeys, probe with pkeys; returns (probe rows, build rows)."""
+    torch.cuda.synchronize()
+    L = abi.lib()
+    bn = bsel.numel() if bsel is not None else bkeys.numel()
+    pn = psel.numel() if psel is not None else pkeys.numel()
+    nbytes = L.mtl_join_table_bytes(strategy, bn, slots)
+    table = torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device())
+    dup = torch.zeros(1, dtype=torch.int32, device=device())
+    abi.check(L.mtl_join_build(strategy, ptr(bkeys), btype, ptr(bsel), None, bn, min_key,
+                               slots, ptr(table), table.numel(), ptr(dup), None))
+    sync()
+    if int(dup.item()) & 2:
+        raise abi.Unsupported(abi.MTL_UNSUPPORTED, "duplicate build key")
+    if int(dup.item()) & 8:
+        raise abi.MettleError(abi.MTL_ERR_INVALID_ARGUMENT, "build key outside the dense span")
+    out_p = torch.empty(max(pn, 4), dtype=torch.int32, device=device())
+    out_b = torch.empty(max(pn, 4), dtype=torch.int32, device=device())
+    cnt = torch.zeros(1, dtype=torch.int32, device=device())
+    probe_slots = slots if strategy == abi.JOIN_DENSE else bn
+    abi.check(L.mtl_join_probe(strategy, ptr(table), table.numel(), min_key, probe_slots,
+                               ptr(pkeys), ptype, ptr(psel), None, pn, ptr(out_p), ptr(out_b),
+                               ptr(cnt), None))
+    sync()
+    n = int(cnt.item())
+    return out_p[:n], out_b[:n]
+
+
+def star_topk(desc: abi.StarDesc, stream=None):
+    """Run the star pipeline synchronously; returns a list of StarRow."""
+    torch.cuda.synchronize()
+    out = (abi.StarRow * max(1, desc.k))()
+    n = C.c_uint32(0)
+    abi.check(abi.lib().mtl_star_topk(C.byref(desc), out, C.byref(n), stream))
+    return [out[i] for i in range(n.value)]
