@@ -234,6 +234,64 @@ int32_t mtl_join_probe(int32_t strategy, const void *table,
                        mtl_stream *s);
 
 /* ---------------------------------------------------------------------- */
+/* Grouped aggregate engine.                                              */
+/* ---------------------------------------------------------------------- */
+#define MTL_GROUP_MAX_KEYS 4
+#define MTL_GROUP_MAX_AGGS 8
+
+typedef struct mtl_agg_spec {
+  int32_t func;       /* MTL_AGG_*                                        */
+  int32_t type;       /* value column type (ignored when values == NULL)  */
+  const void *values; /* device column; NULL = COUNT(*)                   */
+} mtl_agg_spec;
+
+typedef struct mtl_group_desc {
+  uint64_t n_rows;            /* rows of the key / value columns          */
+  const uint32_t *sel;        /* optional selection (device)              */
+  const uint32_t *sel_count;  /* its device count (NULL = sel_capacity)   */
+  uint64_t sel_capacity;
+  uint32_t n_keys;            /* 1..MTL_GROUP_MAX_KEYS integer columns    */
+  const void *key_col[MTL_GROUP_MAX_KEYS];
+  int32_t key_type[MTL_GROUP_MAX_KEYS];
+  int64_t key_min[MTL_GROUP_MAX_KEYS]; /* column stats: every key value   */
+  int64_t key_max[MTL_GROUP_MAX_KEYS]; /* must lie in [min, max]          */
+  uint32_t n_aggs;
+  mtl_agg_spec aggs[MTL_GROUP_MAX_AGGS];
+} mtl_group_desc;
+
+typedef struct mtl_groups mtl_groups; /* device-resident group table     */
+
+/* GROUP BY keys -> COUNT/SUM/MIN/MAX/AVG per group.  Keys are packed by
+ * their [min, max] ranges (mixed radix, first key most significant); when
+ * the packed domain is small the table is indexed directly, otherwise it
+ * is an open-addressing hash table.  Hot keys are pre-aggregated in an
+ * LDS cache per block.  SUM over integer types is exact (128-bit, DuckDB
+ * HUGEINT); SUM/AVG over FLOAT/DOUBLE accumulate in double.
+ * Replaces grouped_aggregate_exact_gv (metal/mod.rs:301-309,
+ * aggregate_ops.rs:101-457; K13-K18), which used <= 4 i32 key lanes and
+ * float32 atomics, and the CPU HashMap path of execute_aggregate_impl
+ * (aggregate.rs:88-236) for BIGINT keys. */
+int32_t mtl_group_agg(const mtl_group_desc *desc, mtl_groups **out,
+                      mtl_stream *s);
+/* Number of groups (synchronises the stream). */
+int32_t mtl_groups_count(mtl_groups *g, uint64_t *n);
+/* Copy every group to host buffers, ascending by packed key for a dense
+ * table (arbitrary but complete for a hash table).  keys: n x n_keys int64
+ * (row-major).  agg_out[a] receives per group: COUNT -> int64; SUM of an
+ * integer type -> int128 {uint64 lower; int64 upper}; SUM/AVG of a float
+ * type and AVG of an integer type -> double; MIN/MAX -> int64 (integer
+ * types) or double (float types). */
+int32_t mtl_groups_fetch(mtl_groups *g, int64_t *keys, void *const *agg_out,
+                         uint64_t capacity, uint64_t *n_out);
+/* Top-K groups by aggregate `agg` (desc != 0: largest first), ties broken
+ * by the group keys ascending; same output layout as mtl_groups_fetch.
+ * Replaces topk_dense_grouped_rows{,_parallel_v} (kernels.metal:1162-1390,
+ * aggregate_ops.rs:572-686; K <= 64 there, K <= 512 here). */
+int32_t mtl_groups_topk(mtl_groups *g, uint32_t agg, int32_t desc, uint32_t k,
+                        int64_t *keys, void *const *agg_out, uint32_t *n_out);
+int32_t mtl_groups_release(mtl_groups *g);
+
+/* ---------------------------------------------------------------------- */
 /* Star join -> grouped SUM -> top-K (the TPC-H Q3 shape).                */
 /* ---------------------------------------------------------------------- */
 

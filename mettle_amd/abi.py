@@ -76,6 +76,61 @@ class StarDesc(C.Structure):
     ]
 
 
+GROUP_MAX_KEYS = 4
+GROUP_MAX_AGGS = 8
+AGG_SUM, AGG_COUNT, AGG_MIN, AGG_MAX, AGG_AVG = range(5)
+
+
+class AggSpec(C.Structure):
+    _fields_ = [("func", C.c_int32), ("type", C.c_int32), ("values", C.c_void_p)]
+
+
+class GroupDesc(C.Structure):
+    _fields_ = [
+        ("n_rows", C.c_uint64), ("sel", C.c_void_p), ("sel_count", C.c_void_p),
+        ("sel_capacity", C.c_uint64), ("n_keys", C.c_uint32),
+        ("key_col", C.c_void_p * GROUP_MAX_KEYS), ("key_type", C.c_int32 * GROUP_MAX_KEYS),
+        ("key_min", C.c_int64 * GROUP_MAX_KEYS), ("key_max", C.c_int64 * GROUP_MAX_KEYS),
+        ("n_aggs", C.c_uint32), ("aggs", AggSpec * GROUP_MAX_AGGS),
+    ]
+
+
+def agg_result_dtype(func: int, type_code: int):
+    """numpy dtype of one aggregate's per-group output (see mtl_groups_fetch)."""
+    import numpy as np
+    is_f = type_code in (FLOAT, DOUBLE)
+    if func == AGG_COUNT:
+        return np.dtype(np.int64)
+    if func == AGG_SUM:
+        return np.dtype(np.float64) if is_f else np.dtype([("lower", "<u8"), ("upper", "<i8")])
+    if func == AGG_AVG:
+        return np.dtype(np.float64)
+    return np.dtype(np.float64) if is_f else np.dtype(np.int64)
+
+
+def make_group_desc(n_rows, keys, aggs, sel=None, sel_count=None, sel_capacity=0):
+    """keys: list of (ptr, type, min, max); aggs: list of (func, ptr, type)."""
+    d = GroupDesc()
+    d.n_rows = n_rows
+    d.sel = sel
+    d.sel_count = sel_count
+    d.sel_capacity = sel_capacity
+    d.n_keys = len(keys)
+    for i, (p, t, mn, mx) in enumerate(keys):
+        d.key_col[i] = p
+        d.key_type[i] = t
+        d.key_min[i] = int(mn)
+        d.key_max[i] = int(mx)
+    d.n_aggs = len(aggs)
+    for i, (f, p, t) in enumerate(aggs):
+        d.aggs[i] = AggSpec(f, t, p)
+    return d
+
+
+def hugeint_to_int(rec) -> int:
+    return (int(rec["upper"]) << 64) | int(rec["lower"])
+
+
 class StarRow(C.Structure):
     _fields_ = [("p_row", C.c_uint32), ("count", C.c_uint32), ("sum_i64", C.c_int64),
                 ("sum_f64", C.c_double)]
@@ -149,6 +204,16 @@ def lib() -> C.CDLL:
         "mtl_join_probe": (i32, [i32, vp, u64, i64, u64, vp, i32, vp, vp, u64, vp, vp, vp, vp]),
         "mtl_star_topk": (i32, [C.POINTER(StarDesc), C.POINTER(StarRow), C.POINTER(u32), vp]),
         "mtl_star_topk_async": (i32, [C.POINTER(StarDesc), vp, vp, vp]),
+        "mtl_star_status": (i32, [u32]),
+        "mtl_profile_enable": (i32, [i32]),
+        "mtl_profile_reset": (i32, []),
+        "mtl_profile_get": (i32, [u32, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
+                                  C.POINTER(u64)]),
+        "mtl_group_agg": (i32, [C.POINTER(GroupDesc), C.POINTER(vp), vp]),
+        "mtl_groups_count": (i32, [vp, C.POINTER(u64)]),
+        "mtl_groups_fetch": (i32, [vp, vp, C.POINTER(vp), u64, C.POINTER(u64)]),
+        "mtl_groups_topk": (i32, [vp, u32, i32, u32, vp, C.POINTER(vp), C.POINTER(u32)]),
+        "mtl_groups_release": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)

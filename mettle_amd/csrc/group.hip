@@ -1,0 +1,858 @@
+// Grouped aggregate engine: GROUP BY integer keys -> COUNT / SUM / MIN /
+// MAX / AVG, plus top-K over the groups.
+//
+// Reference: grouped_aggregate_build_keys_v and grouped_aggregate_update_*_v
+// (kernels.metal:417-621) with helpers grouped_key_hash / probe_grouped_slot
+// (:291-350); host grouped_aggregate_exact_gv (aggregate_ops.rs:101-457);
+// top-K topk_dense_grouped_rows{,_parallel_v} (:1162-1390).  The reference
+// did one pass per aggregate, float32 CAS-loop atomics, a slot state 1
+// (busy) that other threads spin on, and a host scan of the whole table.
+// Here:
+//   - keys are packed by their [min, max] ranges into one 64-bit value
+//     (first key most significant, so packed order = key order); a packed
+//     domain up to kDenseMax is indexed directly, larger domains use an
+//     open-addressing table whose key is published by one 64-bit CAS (no
+//     busy state, so no lane ever waits on another lane of its wave);
+//   - every block keeps an LDS cache of hot keys with their partial states
+//     (Zipf heads collapse there) and flushes it to the global table once;
+//   - one pass updates all aggregates of a row (state words of a group are
+//     contiguous, one cache line for typical aggregate lists);
+//   - integer SUM is exact 128-bit: the low word is added with a returning
+//     64-bit atomic and its carry goes into the high word, which gives the
+//     exact total in any order; FLOAT/DOUBLE sums accumulate in double.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "filter.hpp"
+#include "mtl_internal.hpp"
+#include "prof.hpp"
+#include "topk.hpp"
+
+namespace mtl {
+using namespace dev;
+
+constexpr uint64_t GEMPTY = ~0ull;
+constexpr uint64_t kDenseMax = 1ull << 26;  // directly indexed packed domain
+constexpr int kMaxW = 1 + 2 * MTL_GROUP_MAX_AGGS;
+
+enum Kind : uint32_t { K_COUNT = 0, K_SUMI, K_SUMF, K_MINI, K_MAXI, K_MINF, K_MAXF };
+
+struct AggSlot {
+  uint32_t kind;
+  uint32_t word;
+  int32_t type;
+  const void* col;
+};
+
+struct GroupArgs {
+  uint64_t n_rows;
+  const uint32_t* sel;
+  const uint32_t* sel_count;
+  uint64_t sel_cap;
+  uint32_t n_keys;
+  uint32_t raw;  // single key used unpacked (hash table only)
+  const void* key_col[MTL_GROUP_MAX_KEYS];
+  int32_t key_type[MTL_GROUP_MAX_KEYS];
+  int64_t key_min[MTL_GROUP_MAX_KEYS];
+  uint64_t key_span[MTL_GROUP_MAX_KEYS];
+  uint64_t key_stride[MTL_GROUP_MAX_KEYS];
+  uint32_t n_aggs;
+  AggSlot aggs[MTL_GROUP_MAX_AGGS];
+  uint32_t W;
+  uint32_t dense;
+  uint64_t cap;     // dense: domain size; hash: power-of-two capacity
+  uint64_t* tkeys;  // hash: cap + 1 keys ([cap] = side slot for GEMPTY)
+  uint64_t* st;     // (cap + 1) * W state words
+  uint64_t init[kMaxW];
+  uint32_t cache_slots;
+  uint32_t* err;    // 64: table full, 8: key outside its stated range
+};
+
+__device__ __forceinline__ uint64_t gfmix(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+__device__ __forceinline__ bool pack_key(const GroupArgs& a, uint64_t row, uint64_t& key) {
+  if (a.raw) {
+    key = (uint64_t)load_key(a.key_col[0], a.key_type[0], row);
+    return true;
+  }
+  uint64_t p = 0;
+  for (uint32_t i = 0; i < a.n_keys; ++i) {
+    uint64_t off = (uint64_t)(load_key(a.key_col[i], a.key_type[i], row) - a.key_min[i]);
+    if (off >= a.key_span[i]) return false;
+    p += off * a.key_stride[i];
+  }
+  key = p;
+  return true;
+}
+
+__device__ __forceinline__ double gload_f64(const void* col, int32_t type, uint64_t row) {
+  switch (type) {
+    case MTL_FLOAT: return (double)((const float*)col)[row];
+    case MTL_DOUBLE: return ((const double*)col)[row];
+    default: return (double)load_key(col, type, row);
+  }
+}
+
+__device__ __forceinline__ void add128(uint64_t* p, uint64_t lo, int64_t hi) {
+  unsigned long long old = atomicAdd((unsigned long long*)p, (unsigned long long)lo);
+  int64_t h = hi + (int64_t)(((uint64_t)old + lo) < (uint64_t)old);
+  if (h) atomicAdd((unsigned long long*)(p + 1), (unsigned long long)h);
+}
+
+// Per-row update of one group's state words (global or LDS: atomics with
+// the address space of p).
+__device__ __forceinline__ void update_row(const GroupArgs& a, uint64_t* p, uint64_t row) {
+  atomicAdd((unsigned long long*)&p[0], 1ull);
+  for (uint32_t i = 0; i < a.n_aggs; ++i) {
+    const AggSlot& s = a.aggs[i];
+    uint64_t* w = p + s.word;
+    switch (s.kind) {
+      case K_SUMI: {
+        int64_t v = load_key(s.col, s.type, row);
+        add128(w, (uint64_t)v, v < 0 ? -1 : 0);
+        break;
+      }
+      case K_SUMF:
+        atomicAdd((double*)w, gload_f64(s.col, s.type, row));
+        break;
+      case K_MINI:
+        atomicMin((long long*)w, (long long)load_key(s.col, s.type, row));
+        break;
+      case K_MAXI:
+        atomicMax((long long*)w, (long long)load_key(s.col, s.type, row));
+        break;
+      case K_MINF:
+        atomicMin((unsigned long long*)w, enc_f64(gload_f64(s.col, s.type, row)));
+        break;
+      case K_MAXF:
+        atomicMax((unsigned long long*)w, enc_f64(gload_f64(s.col, s.type, row)));
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+// Merge a partial state (from an LDS cache slot) into a global group.
+__device__ __forceinline__ void merge_state(const GroupArgs& a, uint64_t* g, const uint64_t* p) {
+  atomicAdd((unsigned long long*)&g[0], (unsigned long long)p[0]);
+  for (uint32_t i = 0; i < a.n_aggs; ++i) {
+    const AggSlot& s = a.aggs[i];
+    uint64_t* w = g + s.word;
+    const uint64_t* v = p + s.word;
+    switch (s.kind) {
+      case K_SUMI: add128(w, v[0], (int64_t)v[1]); break;
+      case K_SUMF: atomicAdd((double*)w, __longlong_as_double((long long)v[0])); break;
+      case K_MINI: atomicMin((long long*)w, (long long)v[0]); break;
+      case K_MAXI: atomicMax((long long*)w, (long long)v[0]); break;
+      case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)v[0]); break;
+      case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)v[0]); break;
+      default: break;
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t global_slot(const GroupArgs& a, uint64_t key) {
+  if (a.dense) return key;
+  if (key == GEMPTY) return a.cap;
+  uint64_t mask = a.cap - 1, s = gfmix(key) & mask;
+  for (uint64_t p = 0; p < a.cap; ++p) {
+    uint64_t k = a.tkeys[s];
+    if (k == key) return s;
+    if (k == GEMPTY) {
+      unsigned long long old = atomicCAS((unsigned long long*)&a.tkeys[s], GEMPTY, key);
+      if (old == GEMPTY || old == key) return s;
+    }
+    s = (s + 1) & mask;
+  }
+  atomicOr(a.err, 64u);
+  return GEMPTY;
+}
+
+__global__ void k_group_init(GroupArgs a, uint64_t n_groups) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_groups;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t* p = a.st + i * a.W;
+    for (uint32_t w = 0; w < a.W; ++w) p[w] = a.init[w];
+    if (!a.dense) a.tkeys[i] = GEMPTY;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_group_agg(GroupArgs a) {
+  extern __shared__ uint64_t smem[];
+  const uint32_t cs = a.cache_slots;
+  uint64_t* ckeys = smem;
+  uint64_t* cst = smem + cs;
+  for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
+    ckeys[i] = GEMPTY;
+    for (uint32_t w = 0; w < a.W; ++w) cst[i * a.W + w] = a.init[w];
+  }
+  __syncthreads();
+  const uint64_t n = a.sel ? (a.sel_count ? (uint64_t)*a.sel_count : a.sel_cap) : a.n_rows;
+  const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t start = (uint64_t)blockIdx.x * per;
+  const uint64_t end = start + per < n ? start + per : n;
+  for (uint64_t k = start + threadIdx.x; k < end; k += blockDim.x) {
+    uint64_t row = a.sel ? a.sel[k] : k;
+    uint64_t key;
+    if (!pack_key(a, row, key)) {
+      atomicOr(a.err, 8u);
+      continue;
+    }
+    int slot = -1;
+    if (cs && key != GEMPTY) {
+      uint32_t h = (uint32_t)gfmix(key) & (cs - 1);
+#pragma unroll 1
+      for (int p = 0; p < 8; ++p) {
+        uint32_t si = (h + p) & (cs - 1);
+        uint64_t ck = __hip_atomic_load(&ckeys[si], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (ck == key) {
+          slot = (int)si;
+          break;
+        }
+        if (ck == GEMPTY) {
+          unsigned long long old =
+              atomicCAS((unsigned long long*)&ckeys[si], GEMPTY, (unsigned long long)key);
+          if (old == GEMPTY || old == key) {
+            slot = (int)si;
+            break;
+          }
+        }
+      }
+    }
+    if (slot >= 0) {
+      update_row(a, cst + (uint64_t)slot * a.W, row);
+    } else {
+      uint64_t g = global_slot(a, key);
+      if (g != GEMPTY) update_row(a, a.st + g * a.W, row);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
+    uint64_t key = ckeys[i];
+    if (key == GEMPTY || cst[i * a.W] == 0) continue;
+    uint64_t g = global_slot(a, key);
+    if (g != GEMPTY) merge_state(a, a.st + g * a.W, cst + (uint64_t)i * a.W);
+  }
+}
+
+// Stable compaction of non-empty groups (count word != 0).
+constexpr int CBLOCK = 256;
+constexpr int CPER = 8;
+constexpr int CTILE = CBLOCK * CPER;
+
+__global__ __launch_bounds__(CBLOCK) void k_group_compact(const uint64_t* st, uint32_t W,
+                                                          uint64_t n, uint32_t* out,
+                                                          uint32_t* out_count,
+                                                          uint64_t* status, uint32_t* ctr,
+                                                          uint32_t* err) {
+  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ uint32_t s_wave[CBLOCK / WAVE];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  if (tid == 0) s_tile = atomicAdd(ctr, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t i0 = (uint64_t)tile * CTILE + (uint64_t)tid * CPER;
+  uint32_t flags = 0, c = 0;
+#pragma unroll
+  for (int j = 0; j < CPER; ++j)
+    if (i0 + j < n && st[(i0 + j) * W] != 0) {
+      flags |= 1u << j;
+      ++c;
+    }
+  uint32_t incl = wave_inclusive_u32(c);
+  if (lane == WAVE - 1) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t woff = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < CBLOCK / WAVE; ++w) {
+    if (w < wave) woff += s_wave[w];
+    total += s_wave[w];
+  }
+  if (wave == 0) {
+    uint32_t ex = lookback_wave(status, tile, total, err);
+    if (lane == 0) s_prefix = ex;
+  }
+  __syncthreads();
+  uint32_t pos = s_prefix + woff + incl - c;
+#pragma unroll
+  for (int j = 0; j < CPER; ++j)
+    if (flags & (1u << j)) out[pos++] = (uint32_t)(i0 + j);
+  if (tile == gridDim.x - 1 && tid == 0) *out_count = s_prefix + total;
+}
+
+struct EmitArgs {
+  const uint32_t* idx;    // group indices
+  const uint32_t* count;  // device count of idx
+  const uint64_t* st;
+  const uint64_t* tkeys;
+  uint32_t W, dense, raw, n_keys;
+  int64_t key_min[MTL_GROUP_MAX_KEYS];
+  uint64_t key_span[MTL_GROUP_MAX_KEYS];
+  uint64_t key_stride[MTL_GROUP_MAX_KEYS];
+  uint64_t cap;
+  uint32_t n_aggs;
+  AggSlot aggs[MTL_GROUP_MAX_AGGS];
+  int64_t* out_keys;   // n x n_keys
+  uint64_t* out_cnt;   // n
+  uint64_t* out_aggs;  // n x (2 words per agg)
+};
+
+__device__ __forceinline__ uint64_t group_key_of(const EmitArgs& a, uint32_t g) {
+  if (a.dense) return (uint64_t)g;
+  return g == a.cap ? GEMPTY : a.tkeys[g];
+}
+
+__global__ void k_group_emit(EmitArgs a) {
+  const uint32_t n = *a.count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t g = a.idx[i];
+    uint64_t key = group_key_of(a, g);
+    for (uint32_t j = 0; j < a.n_keys; ++j) {
+      int64_t v;
+      if (a.raw) v = (int64_t)key;
+      else v = a.key_min[j] + (int64_t)((key / a.key_stride[j]) % a.key_span[j]);
+      a.out_keys[(uint64_t)i * a.n_keys + j] = v;
+    }
+    const uint64_t* p = a.st + (uint64_t)g * a.W;
+    a.out_cnt[i] = p[0];
+    for (uint32_t k = 0; k < a.n_aggs; ++k) {
+      uint64_t* o = a.out_aggs + ((uint64_t)i * a.n_aggs + k) * 2;
+      const AggSlot& s = a.aggs[k];
+      o[0] = s.kind == K_COUNT ? p[0] : p[s.word];
+      o[1] = s.kind == K_SUMI ? p[s.word + 1] : 0;
+    }
+  }
+}
+
+// Top-K over all table entries: primary = one aggregate (two key words),
+// then the packed key ascending.
+struct GTopkArgs {
+  const uint64_t* st;
+  const uint64_t* tkeys;
+  uint32_t W, dense, raw;
+  uint64_t n;  // table entries
+  uint64_t cap;
+  uint32_t kind, word;
+  int32_t desc;
+  int32_t avg;  // order by sum / count
+  uint32_t k;
+  uint64_t* ck[4];
+  uint32_t* cid;
+  uint32_t* cn;
+  uint32_t* done;
+  uint32_t* out_idx;
+  uint32_t* out_n;
+};
+
+__device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint64_t k[4]) {
+  const uint64_t* p = a.st + g * a.W;
+  uint64_t k0, k1 = 0;
+  switch (a.kind) {
+    case K_COUNT: k0 = enc_i64((int64_t)p[0]); break;
+    case K_SUMI:
+      if (a.avg) {
+        double v = ((double)(int64_t)p[a.word + 1] * 18446744073709551616.0 +
+                    (double)p[a.word]) / (double)p[0];
+        k0 = enc_f64(v);
+      } else {
+        k0 = enc_i64((int64_t)p[a.word + 1]);
+        k1 = p[a.word];
+      }
+      break;
+    case K_SUMF: {
+      double v = __longlong_as_double((long long)p[a.word]);
+      k0 = enc_f64(a.avg ? v / (double)p[0] : v);
+      break;
+    }
+    case K_MINI:
+    case K_MAXI: k0 = enc_i64((int64_t)p[a.word]); break;
+    default: k0 = p[a.word]; break;  // MINF/MAXF already encoded
+  }
+  if (a.desc) {
+    k0 = ~k0;
+    k1 = ~k1;
+  }
+  k[0] = k0;
+  k[1] = k1;
+  uint64_t key = a.dense ? g : (g == a.cap ? GEMPTY : a.tkeys[g]);
+  k[2] = a.raw ? enc_i64((int64_t)key) : key;
+  k[3] = 0;
+}
+
+__global__ __launch_bounds__(TOPK_BLOCK) void k_group_topk(GTopkArgs a) {
+  __shared__ TopkLds s;
+  __shared__ bool s_last;
+  topk_init(s);
+  const uint64_t chunk = (a.n + gridDim.x - 1) / gridDim.x;
+  const uint64_t start = (uint64_t)blockIdx.x * chunk;
+  const uint64_t end = start + chunk < a.n ? start + chunk : a.n;
+  for (uint64_t base = start; base < end; base += TOPK_ROUND) {
+    for (int e = 0; e < 2; ++e) {
+      uint64_t g = base + e * TOPK_BLOCK + threadIdx.x;
+      if (g < end && a.st[g * a.W] != 0) {
+        uint64_t k[4];
+        gtopk_keys(a, g, k);
+        if (k[0] <= s.thr) topk_push(s, k[0], k[1], k[2], k[3], (uint32_t)g);
+      }
+    }
+    topk_maybe_compact(s, a.k);
+  }
+  __syncthreads();
+  uint32_t n = s.n;
+  __syncthreads();
+  topk_sort(s, n);
+  topk_truncate(s, a.k);
+  const uint32_t kb = s.n;
+  for (uint32_t i = threadIdx.x; i < kb; i += blockDim.x) {
+    uint64_t o = (uint64_t)blockIdx.x * a.k + i;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) a.ck[f][o] = s.k[f][i];
+    a.cid[o] = s.id[i];
+  }
+  if (threadIdx.x == 0) a.cn[blockIdx.x] = kb;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  topk_init(s);
+  for (uint32_t b = 0; b < gridDim.x; ++b) {
+    uint32_t nb = __hip_atomic_load(&a.cn[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t base = 0; base < nb; base += TOPK_ROUND) {
+      for (int e = 0; e < 2; ++e) {
+        uint32_t i = base + e * TOPK_BLOCK + threadIdx.x;
+        if (i < nb) {
+          uint64_t o = (uint64_t)b * a.k + i;
+          uint64_t k0 = a.ck[0][o];
+          if (k0 <= s.thr) topk_push(s, k0, a.ck[1][o], a.ck[2][o], a.ck[3][o], a.cid[o]);
+        }
+      }
+      topk_maybe_compact(s, a.k);
+    }
+  }
+  __syncthreads();
+  n = s.n;
+  __syncthreads();
+  topk_sort(s, n);
+  topk_truncate(s, a.k);
+  for (uint32_t i = threadIdx.x; i < s.n; i += blockDim.x) a.out_idx[i] = s.id[i];
+  if (threadIdx.x == 0) *a.out_n = s.n;
+}
+
+}  // namespace mtl
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct mtl_groups {
+  mtl_stream* s = nullptr;
+  mtl::GroupArgs args{};
+  uint64_t entries = 0;  // table entries (dense domain or cap + 1)
+  void* keys_mem = nullptr;
+  void* st_mem = nullptr;
+  int32_t agg_func[MTL_GROUP_MAX_AGGS];
+  int32_t agg_type[MTL_GROUP_MAX_AGGS];
+};
+
+namespace mtl {
+
+static bool is_float(int32_t t) { return t == MTL_FLOAT || t == MTL_DOUBLE; }
+
+static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entries) {
+  if (!d) return set_error(MTL_ERR_INVALID_ARGUMENT, "desc is NULL");
+  if (d->n_keys < 1 || d->n_keys > MTL_GROUP_MAX_KEYS)
+    return set_error(MTL_UNSUPPORTED, "GROUP BY with %u keys", d->n_keys);
+  if (d->n_aggs > MTL_GROUP_MAX_AGGS)
+    return set_error(MTL_UNSUPPORTED, "more than %d aggregates", MTL_GROUP_MAX_AGGS);
+  uint64_t rows = d->sel ? d->sel_capacity : d->n_rows;
+  if (rows > 0xFFFFFFFFull) return set_error(MTL_UNSUPPORTED, "more than 2^32-1 rows");
+  *a = GroupArgs{};
+  a->n_rows = d->n_rows;
+  a->sel = d->sel;
+  a->sel_count = d->sel ? d->sel_count : nullptr;
+  a->sel_cap = d->sel_capacity;
+  a->n_keys = d->n_keys;
+  // packed domain
+  long double dom = 1;
+  for (uint32_t i = 0; i < d->n_keys; ++i) {
+    if (!d->key_col[i] || !type_is_int(d->key_type[i]))
+      return set_error(MTL_UNSUPPORTED, "group key %u must be an integer column", i);
+    if (d->key_max[i] < d->key_min[i])
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "group key %u: max < min", i);
+    a->key_col[i] = d->key_col[i];
+    a->key_type[i] = d->key_type[i];
+    a->key_min[i] = d->key_min[i];
+    a->key_span[i] = (uint64_t)(d->key_max[i] - d->key_min[i]) + 1;
+    dom *= (a->key_span[i] == 0 ? 18446744073709551616.0L : (long double)a->key_span[i]);
+  }
+  if (dom > 4611686018427387904.0L) {  // 2^62: cannot pack
+    if (d->n_keys != 1)
+      return set_error(MTL_UNSUPPORTED, "composite group key domain exceeds 2^62");
+    a->raw = 1;
+  } else {
+    uint64_t stride = 1;
+    for (int i = (int)d->n_keys - 1; i >= 0; --i) {
+      a->key_stride[i] = stride;
+      stride *= a->key_span[i];
+    }
+  }
+  uint64_t D = a->raw ? 0 : (uint64_t)dom;
+  // aggregate state words
+  uint32_t w = 1;
+  a->init[0] = 0;
+  for (uint32_t i = 0; i < d->n_aggs; ++i) {
+    const mtl_agg_spec& sp = d->aggs[i];
+    AggSlot& s = a->aggs[i];
+    s.col = sp.values;
+    s.type = sp.type;
+    bool need_col = sp.func != MTL_AGG_COUNT;
+    if (need_col && (!sp.values || !type_width(sp.type) || sp.type == MTL_HUGEINT))
+      return set_error(MTL_UNSUPPORTED, "aggregate %u: unsupported value column", i);
+    switch (sp.func) {
+      case MTL_AGG_COUNT: s.kind = K_COUNT; s.word = 0; break;
+      case MTL_AGG_SUM:
+      case MTL_AGG_AVG:
+        if (is_float(sp.type)) {
+          s.kind = K_SUMF; s.word = w; a->init[w++] = 0;
+        } else {
+          s.kind = K_SUMI; s.word = w; a->init[w++] = 0; a->init[w++] = 0;
+        }
+        break;
+      case MTL_AGG_MIN:
+        s.word = w;
+        if (is_float(sp.type)) { s.kind = K_MINF; a->init[w++] = ~0ull; }
+        else { s.kind = K_MINI; a->init[w++] = (uint64_t)INT64_MAX; }
+        break;
+      case MTL_AGG_MAX:
+        s.word = w;
+        if (is_float(sp.type)) { s.kind = K_MAXF; a->init[w++] = 0; }
+        else { s.kind = K_MAXI; a->init[w++] = (uint64_t)INT64_MIN; }
+        break;
+      default:
+        return set_error(MTL_UNSUPPORTED, "aggregate function %d", sp.func);
+    }
+  }
+  a->n_aggs = d->n_aggs;
+  a->W = w;
+  if (!a->raw && D <= kDenseMax) {
+    a->dense = 1;
+    a->cap = D;
+    *entries = D;
+  } else {
+    a->dense = 0;
+    uint64_t want = std::min<uint64_t>(rows, a->raw ? rows : D);
+    a->cap = 16;
+    while (a->cap < want * 2) a->cap <<= 1;
+    *entries = a->cap + 1;
+  }
+  // LDS cache: <= 48 KB
+  uint32_t slot_bytes = 8 * (1 + a->W);
+  uint32_t cs = 1;
+  while ((uint64_t)cs * 2 * slot_bytes <= 48 * 1024) cs *= 2;
+  if (a->dense && (uint64_t)cs >= D) {
+    // tiny domains: the whole domain fits; use a cache at least as large
+    uint32_t c2 = 1;
+    while (c2 < D * 2) c2 *= 2;
+    cs = std::min<uint32_t>(cs, c2);
+  }
+  a->cache_slots = cs;
+  return MTL_OK;
+}
+
+}  // namespace mtl
+
+using namespace mtl;
+
+extern "C" {
+
+int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* s) {
+  if (!out) return set_error(MTL_ERR_INVALID_ARGUMENT, "out is NULL");
+  GroupArgs a;
+  uint64_t entries;
+  MTL_TRY(group_plan(desc, &a, &entries));
+  MTL_TRY(resolve_stream(s, &s));
+  std::lock_guard<std::mutex> lk(s->mu);
+  mtl_groups* g = new mtl_groups();
+  g->s = s;
+  g->entries = entries;
+  for (uint32_t i = 0; i < desc->n_aggs; ++i) {
+    g->agg_func[i] = desc->aggs[i].func;
+    g->agg_type[i] = desc->aggs[i].type;
+  }
+  hipError_t e = hipMallocAsync(&g->st_mem, entries * a.W * 8 + 8, s->stream);
+  if (e == hipSuccess && !a.dense) e = hipMallocAsync(&g->keys_mem, entries * 8, s->stream);
+  if (e != hipSuccess) {
+    if (g->st_mem) (void)hipFreeAsync(g->st_mem, s->stream);
+    delete g;
+    return set_error(MTL_ERR_PROCESSING, "processing error: group table of %llu entries: %s",
+                     (unsigned long long)entries, hipGetErrorString(e));
+  }
+  s->arena.reset();
+  void* err;
+  MTL_TRY(s->arena.alloc(256, &err));
+  ZeroList zl;
+  zl.add((uint32_t*)err, 64);
+  MTL_TRY(launch_zero(zl, s->stream));
+  a.st = (uint64_t*)g->st_mem;
+  a.tkeys = (uint64_t*)g->keys_mem;
+  a.err = (uint32_t*)err;
+  {
+    ProfScope ps(s, "k_group_init");
+    uint64_t blocks = std::min<uint64_t>(ceil_div(entries, 256), 8192);
+    hipLaunchKernelGGL(k_group_init, dim3((uint32_t)std::max<uint64_t>(blocks, 1)), dim3(256),
+                       0, s->stream, a, entries);
+  }
+  MTL_HIP(hipGetLastError());
+  uint64_t rows = desc->sel ? desc->sel_capacity : desc->n_rows;
+  if (rows) {
+    size_t lds = (size_t)a.cache_slots * 8 * (1 + a.W);
+    uint64_t grid = std::min<uint64_t>(ceil_div(rows, 4096), (uint64_t)cu_count() * 2);
+    ProfScope ps(s, "k_group_agg");
+    hipLaunchKernelGGL(k_group_agg, dim3((uint32_t)std::max<uint64_t>(grid, 1)), dim3(256),
+                       lds, s->stream, a);
+  }
+  MTL_HIP(hipGetLastError());
+  g->args = a;
+  g->args.err = nullptr;  // arena memory is not kept past this call
+  // check errors now (synchronous boundary of the aggregate)
+  void* host;
+  MTL_TRY(stream_pinned(s, 8, &host));
+  MTL_HIP(hipMemcpyAsync(host, err, 4, hipMemcpyDeviceToHost, s->stream));
+  MTL_HIP(hipStreamSynchronize(s->stream));
+  uint32_t eb = *(uint32_t*)host;
+  if (eb) {
+    (void)hipFreeAsync(g->st_mem, s->stream);
+    if (g->keys_mem) (void)hipFreeAsync(g->keys_mem, s->stream);
+    delete g;
+    if (eb & 8u) return set_error(MTL_ERR_INVALID_ARGUMENT, "a group key lies outside its stated [min, max]");
+    return set_error(MTL_ERR_PROCESSING, "processing error: group table overflow");
+  }
+  *out = g;
+  return MTL_OK;
+}
+
+int32_t mtl_groups_release(mtl_groups* g) {
+  if (!g) return MTL_OK;
+  (void)hipFreeAsync(g->st_mem, g->s->stream);
+  if (g->keys_mem) (void)hipFreeAsync(g->keys_mem, g->s->stream);
+  delete g;
+  return MTL_OK;
+}
+
+// Emit `idx` groups (device list + device count) into host buffers.
+static int32_t emit_groups(mtl_groups* g, const uint32_t* idx, const uint32_t* count,
+                           uint64_t n, int64_t* keys, void* const* agg_out) {
+  mtl_stream* s = g->s;
+  const GroupArgs& a = g->args;
+  if (n == 0) return MTL_OK;
+  void *dk, *dc, *da;
+  MTL_TRY(s->arena.alloc(n * a.n_keys * 8, &dk));
+  MTL_TRY(s->arena.alloc(n * 8, &dc));
+  MTL_TRY(s->arena.alloc(n * std::max<uint32_t>(a.n_aggs, 1) * 16, &da));
+  EmitArgs e{};
+  e.idx = idx;
+  e.count = count;
+  e.st = a.st;
+  e.tkeys = a.tkeys;
+  e.W = a.W;
+  e.dense = a.dense;
+  e.raw = a.raw;
+  e.n_keys = a.n_keys;
+  for (uint32_t j = 0; j < a.n_keys; ++j) {
+    e.key_min[j] = a.key_min[j];
+    e.key_span[j] = a.key_span[j];
+    e.key_stride[j] = a.key_stride[j];
+  }
+  e.cap = a.cap;
+  e.n_aggs = a.n_aggs;
+  for (uint32_t k = 0; k < a.n_aggs; ++k) e.aggs[k] = a.aggs[k];
+  e.out_keys = (int64_t*)dk;
+  e.out_cnt = (uint64_t*)dc;
+  e.out_aggs = (uint64_t*)da;
+  {
+    ProfScope ps(s, "k_group_emit");
+    hipLaunchKernelGGL(k_group_emit, dim3((uint32_t)std::min<uint64_t>(ceil_div(n, 256), 4096)),
+                       dim3(256), 0, s->stream, e);
+  }
+  MTL_HIP(hipGetLastError());
+  std::vector<uint64_t> cnt(n), st(n * std::max<uint32_t>(a.n_aggs, 1) * 2);
+  if (keys)
+    MTL_HIP(hipMemcpyAsync(keys, dk, n * a.n_keys * 8, hipMemcpyDeviceToHost, s->stream));
+  MTL_HIP(hipMemcpyAsync(cnt.data(), dc, n * 8, hipMemcpyDeviceToHost, s->stream));
+  MTL_HIP(hipMemcpyAsync(st.data(), da, st.size() * 8, hipMemcpyDeviceToHost, s->stream));
+  MTL_HIP(hipStreamSynchronize(s->stream));
+  if (!agg_out) return MTL_OK;
+  for (uint32_t k = 0; k < a.n_aggs; ++k) {
+    if (!agg_out[k]) continue;
+    const AggSlot& sl = a.aggs[k];
+    int32_t func = g->agg_func[k];
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t* v = &st[(i * a.n_aggs + k) * 2];
+      switch (func) {
+        case MTL_AGG_COUNT:
+          ((int64_t*)agg_out[k])[i] = (int64_t)cnt[i];
+          break;
+        case MTL_AGG_SUM:
+          if (sl.kind == K_SUMI) {
+            ((uint64_t*)agg_out[k])[2 * i] = v[0];
+            ((uint64_t*)agg_out[k])[2 * i + 1] = v[1];
+          } else {
+            double d;
+            memcpy(&d, &v[0], 8);
+            ((double*)agg_out[k])[i] = d;
+          }
+          break;
+        case MTL_AGG_AVG:
+          if (sl.kind == K_SUMI) {
+            // hugeint sum / count in long double, as DuckDB's integer AVG
+            __int128 sum = (__int128)(((unsigned __int128)v[1] << 64) | v[0]);
+            ((double*)agg_out[k])[i] = (double)((long double)sum / (long double)cnt[i]);
+          } else {
+            double d;
+            memcpy(&d, &v[0], 8);
+            ((double*)agg_out[k])[i] = d / (double)cnt[i];
+          }
+          break;
+        default:  // MIN / MAX
+          if (sl.kind == K_MINF || sl.kind == K_MAXF) {
+            uint64_t b = v[0];
+            b = (b & 0x8000000000000000ull) ? (b & ~0x8000000000000000ull) : ~b;
+            double d;
+            memcpy(&d, &b, 8);
+            ((double*)agg_out[k])[i] = d;
+          } else {
+            ((int64_t*)agg_out[k])[i] = (int64_t)v[0];
+          }
+          break;
+      }
+    }
+  }
+  return MTL_OK;
+}
+
+static int32_t compact_groups(mtl_groups* g, uint32_t** idx, uint32_t** count) {
+  mtl_stream* s = g->s;
+  const GroupArgs& a = g->args;
+  uint64_t n = g->entries;
+  uint32_t tiles = (uint32_t)ceil_div(n ? n : 1, CTILE);
+  void *list, *status, *misc;
+  MTL_TRY(s->arena.alloc((n ? n : 1) * 4, &list));
+  MTL_TRY(s->arena.alloc((uint64_t)tiles * 8, &status));
+  MTL_TRY(s->arena.alloc(256, &misc));
+  ZeroList zl;
+  zl.add((uint32_t*)status, (uint64_t)tiles * 2);
+  zl.add((uint32_t*)misc, 64);
+  MTL_TRY(launch_zero(zl, s->stream));
+  {
+    ProfScope ps(s, "k_group_compact");
+    hipLaunchKernelGGL(k_group_compact, dim3(tiles), dim3(CBLOCK), 0, s->stream, a.st, a.W, n,
+                       (uint32_t*)list, (uint32_t*)misc, (uint64_t*)status,
+                       (uint32_t*)misc + 1, (uint32_t*)misc + 2);
+  }
+  MTL_HIP(hipGetLastError());
+  *idx = (uint32_t*)list;
+  *count = (uint32_t*)misc;
+  return MTL_OK;
+}
+
+int32_t mtl_groups_count(mtl_groups* g, uint64_t* n) {
+  if (!g || !n) return set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(g->s->mu);
+  g->s->arena.reset();
+  uint32_t *idx, *count;
+  MTL_TRY(compact_groups(g, &idx, &count));
+  void* host;
+  MTL_TRY(stream_pinned(g->s, 8, &host));
+  MTL_HIP(hipMemcpyAsync(host, count, 4, hipMemcpyDeviceToHost, g->s->stream));
+  MTL_HIP(hipStreamSynchronize(g->s->stream));
+  *n = *(uint32_t*)host;
+  return MTL_OK;
+}
+
+int32_t mtl_groups_fetch(mtl_groups* g, int64_t* keys, void* const* agg_out, uint64_t capacity,
+                         uint64_t* n_out) {
+  if (!g || !n_out) return set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(g->s->mu);
+  g->s->arena.reset();
+  uint32_t *idx, *count;
+  MTL_TRY(compact_groups(g, &idx, &count));
+  void* host;
+  MTL_TRY(stream_pinned(g->s, 8, &host));
+  MTL_HIP(hipMemcpyAsync(host, count, 4, hipMemcpyDeviceToHost, g->s->stream));
+  MTL_HIP(hipStreamSynchronize(g->s->stream));
+  uint64_t n = *(uint32_t*)host;
+  *n_out = n;
+  if (n > capacity)
+    return set_error(MTL_ERR_INVALID_ARGUMENT, "%llu groups exceed the capacity %llu",
+                     (unsigned long long)n, (unsigned long long)capacity);
+  return emit_groups(g, idx, count, n, keys, agg_out);
+}
+
+int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, int64_t* keys,
+                        void* const* agg_out, uint32_t* n_out) {
+  if (!g || !n_out) return set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  const GroupArgs& a = g->args;
+  if (agg >= a.n_aggs) return set_error(MTL_ERR_INVALID_ARGUMENT, "aggregate index %u", agg);
+  if (k == 0 || k > (uint32_t)TOPK_MAX_K)
+    return set_error(MTL_UNSUPPORTED, "top-K with K=%u (supported 1..%d)", k, TOPK_MAX_K);
+  mtl_stream* s = g->s;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->arena.reset();
+  uint32_t grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(ceil_div(g->entries, 16384), (uint64_t)cu_count()));
+  void *ck[4], *cid, *cn, *misc, *oidx;
+  for (int f = 0; f < 4; ++f) MTL_TRY(s->arena.alloc((uint64_t)grid * k * 8, &ck[f]));
+  MTL_TRY(s->arena.alloc((uint64_t)grid * k * 4, &cid));
+  MTL_TRY(s->arena.alloc((uint64_t)grid * 4, &cn));
+  MTL_TRY(s->arena.alloc(256, &misc));
+  MTL_TRY(s->arena.alloc((uint64_t)k * 4, &oidx));
+  ZeroList zl;
+  zl.add((uint32_t*)misc, 64);
+  MTL_TRY(launch_zero(zl, s->stream));
+  GTopkArgs t{};
+  t.st = a.st;
+  t.tkeys = a.tkeys;
+  t.W = a.W;
+  t.dense = a.dense;
+  t.raw = a.raw;
+  t.n = g->entries;
+  t.cap = a.cap;
+  t.kind = a.aggs[agg].kind;
+  t.word = a.aggs[agg].word;
+  t.desc = desc;
+  t.avg = g->agg_func[agg] == MTL_AGG_AVG;
+  t.k = k;
+  for (int f = 0; f < 4; ++f) t.ck[f] = (uint64_t*)ck[f];
+  t.cid = (uint32_t*)cid;
+  t.cn = (uint32_t*)cn;
+  t.done = (uint32_t*)misc;
+  t.out_idx = (uint32_t*)oidx;
+  t.out_n = (uint32_t*)misc + 1;
+  {
+    ProfScope ps(s, "k_group_topk");
+    hipLaunchKernelGGL(k_group_topk, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+  }
+  MTL_HIP(hipGetLastError());
+  void* host;
+  MTL_TRY(stream_pinned(s, 8, &host));
+  MTL_HIP(hipMemcpyAsync(host, (uint32_t*)misc + 1, 4, hipMemcpyDeviceToHost, s->stream));
+  MTL_HIP(hipStreamSynchronize(s->stream));
+  uint32_t n = *(uint32_t*)host;
+  *n_out = n;
+  return emit_groups(g, (const uint32_t*)oidx, (uint32_t*)misc + 1, n, keys, agg_out);
+}
+
+}  // extern "C"

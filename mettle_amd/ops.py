@@ -124,6 +124,60 @@ def join(strategy: int, bkeys: torch.Tensor, btype: int, pkeys: torch.Tensor, pt
     return out_p[:n], out_b[:n]
 
 
+class Groups:
+    """Device-resident group table (mtl_groups handle)."""
+
+    def __init__(self, handle, n_keys, aggs):
+        self.h = handle
+        self.n_keys = n_keys
+        self.aggs = aggs  # list of (func, type)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            abi.lib().mtl_groups_release(self.h)
+            self.h = None
+
+    def count(self) -> int:
+        n = C.c_uint64()
+        abi.check(abi.lib().mtl_groups_count(self.h, C.byref(n)))
+        return n.value
+
+    def _outs(self, cap):
+        outs = [np.empty(max(cap, 1), dtype=abi.agg_result_dtype(f, t)) for f, t in self.aggs]
+        ptrs = (C.c_void_p * max(1, len(outs)))(*[o.ctypes.data for o in outs])
+        return outs, ptrs
+
+    def fetch(self):
+        cap = self.count()
+        keys = np.empty((max(cap, 1), self.n_keys), dtype=np.int64)
+        outs, ptrs = self._outs(cap)
+        n = C.c_uint64()
+        abi.check(abi.lib().mtl_groups_fetch(self.h, keys.ctypes.data, ptrs, cap, C.byref(n)))
+        return keys[:n.value], [o[:n.value] for o in outs]
+
+    def topk(self, agg: int, k: int, desc: bool = True):
+        keys = np.empty((k, self.n_keys), dtype=np.int64)
+        outs, ptrs = self._outs(k)
+        n = C.c_uint32()
+        abi.check(abi.lib().mtl_groups_topk(self.h, agg, 1 if desc else 0, k, keys.ctypes.data,
+                                            ptrs, C.byref(n)))
+        return keys[:n.value], [o[:n.value] for o in outs]
+
+
+def group_agg(keys, aggs, n_rows: int, sel: torch.Tensor | None = None,
+              sel_count: torch.Tensor | None = None) -> Groups:
+    """keys: list of (device column, type, min, max); aggs: list of
+    (func, device column | None, type)."""
+    torch.cuda.synchronize()
+    d = abi.make_group_desc(n_rows, [(ptr(c), t, mn, mx) for c, t, mn, mx in keys],
+                            [(f, ptr(c), t) for f, c, t in aggs], sel=ptr(sel),
+                            sel_count=ptr(sel_count),
+                            sel_capacity=0 if sel is None else sel.numel())
+    h = C.c_void_p()
+    abi.check(abi.lib().mtl_group_agg(C.byref(d), C.byref(h), None))
+    return Groups(h, len(keys), [(f, t) for f, _, t in aggs])
+
+
 def star_topk(desc: abi.StarDesc, stream=None):
     """Run the star pipeline synchronously; returns a list of StarRow."""
     torch.cuda.synchronize()

@@ -351,6 +351,106 @@ done:
   return status;
 }
 
+/* ---------------------------------------------------------------------- */
+/* Grouped aggregate: sort rows by key tuple, aggregate runs.              */
+/* Restates grouped_aggregate_exact_gv (aggregate_ops.rs:101-457) and the  */
+/* CPU HashMap path (aggregate.rs:128-216) with DuckDB result types:       */
+/* COUNT -> BIGINT, SUM(int) -> HUGEINT, SUM/AVG(float) -> DOUBLE          */
+/* (compensated sum), AVG(int) -> hugeint / count in long double.          */
+/* ---------------------------------------------------------------------- */
+static const mtl_group_desc *g_gdesc;
+
+static int orc_row_key_cmp(const void *pa, const void *pb) {
+  uint32_t a = *(const uint32_t *)pa, b = *(const uint32_t *)pb;
+  for (uint32_t i = 0; i < g_gdesc->n_keys; ++i) {
+    int c = cmp3_i64(orc_key(g_gdesc->key_col[i], g_gdesc->key_type[i], a),
+                     orc_key(g_gdesc->key_col[i], g_gdesc->key_type[i], b));
+    if (c) return c;
+  }
+  return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+/* Returns the number of groups written (ascending by key tuple), or -1 if
+ * more than `cap` groups exist. */
+int64_t orc_group_agg(const mtl_group_desc *d, int64_t *keys_out, void *const *agg_out,
+                      uint64_t cap) {
+  uint64_t n = d->sel ? d->sel_capacity : d->n_rows;
+  uint32_t *rows = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  for (uint64_t i = 0; i < n; ++i) rows[i] = d->sel ? d->sel[i] : (uint32_t)i;
+  g_gdesc = d;
+  qsort(rows, n, sizeof(uint32_t), orc_row_key_cmp);
+  int64_t ng = 0;
+  uint64_t i = 0;
+  while (i < n) {
+    uint64_t j = i;
+    while (j < n) {
+      int same = 1;
+      for (uint32_t k = 0; k < d->n_keys && same; ++k)
+        same = orc_key(d->key_col[k], d->key_type[k], rows[j]) ==
+               orc_key(d->key_col[k], d->key_type[k], rows[i]);
+      if (!same) break;
+      ++j;
+    }
+    if ((uint64_t)ng >= cap) {
+      free(rows);
+      return -1;
+    }
+    for (uint32_t k = 0; k < d->n_keys; ++k)
+      keys_out[ng * d->n_keys + k] = orc_key(d->key_col[k], d->key_type[k], rows[i]);
+    int64_t cnt = (int64_t)(j - i);
+    for (uint32_t a = 0; a < d->n_aggs; ++a) {
+      const mtl_agg_spec *sp = &d->aggs[a];
+      int isf = sp->type == MTL_FLOAT || sp->type == MTL_DOUBLE;
+      if (sp->func == MTL_AGG_COUNT) {
+        ((int64_t *)agg_out[a])[ng] = cnt;
+      } else if (sp->func == MTL_AGG_SUM || sp->func == MTL_AGG_AVG) {
+        if (isf) {
+          double s = 0, c = 0;
+          for (uint64_t r = i; r < j; ++r) {
+            double v = orc_f64(sp->values, sp->type, rows[r]), t = s + v;
+            if (fabs(s) >= fabs(v)) c += (s - t) + v;
+            else c += (v - t) + s;
+            s = t;
+          }
+          s += c;
+          ((double *)agg_out[a])[ng] = sp->func == MTL_AGG_SUM ? s : s / (double)cnt;
+        } else {
+          __int128 s = 0;
+          for (uint64_t r = i; r < j; ++r) s += orc_key(sp->values, sp->type, rows[r]);
+          if (sp->func == MTL_AGG_SUM) {
+            ((uint64_t *)agg_out[a])[2 * ng] = (uint64_t)s;
+            ((int64_t *)agg_out[a])[2 * ng + 1] = (int64_t)(s >> 64);
+          } else {
+            ((double *)agg_out[a])[ng] = (double)((long double)s / (long double)cnt);
+          }
+        }
+      } else {
+        int mx = sp->func == MTL_AGG_MAX;
+        if (isf) {
+          double best = orc_f64(sp->values, sp->type, rows[i]);
+          for (uint64_t r = i + 1; r < j; ++r) {
+            double v = orc_f64(sp->values, sp->type, rows[r]);
+            int c = cmp3_f64(v, best);
+            if (mx ? c > 0 : c < 0) best = v;
+          }
+          ((double *)agg_out[a])[ng] = best;
+        } else {
+          int64_t best = orc_key(sp->values, sp->type, rows[i]);
+          for (uint64_t r = i + 1; r < j; ++r) {
+            int64_t v = orc_key(sp->values, sp->type, rows[r]);
+            if (mx ? v > best : v < best) best = v;
+          }
+          ((int64_t *)agg_out[a])[ng] = best;
+        }
+      }
+    }
+    ++ng;
+    i = j;
+  }
+  free(rows);
+  return ng;
+}
+
 /* Days since 1970-01-01 for a civil date (proleptic Gregorian), the closed
  * form of parse_date_to_days_since_epoch (src/plan/parser.rs:426-449). */
 int32_t orc_date_days(int32_t y, int32_t m, int32_t d) {

@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
                                     C.POINTER(u32)]
         L.orc_date_days.restype = i32
         L.orc_date_days.argtypes = [i32, i32, i32]
+        L.orc_group_agg.restype = i64
+        L.orc_group_agg.argtypes = [C.POINTER(abi.GroupDesc), vp, C.POINTER(vp), u64]
         _lib = L
     return _lib
 
@@ -98,6 +100,24 @@ def star_topk(desc: abi.StarDesc):
     n = C.c_uint32(0)
     st = lib().orc_star_topk(C.byref(desc), out, C.byref(n))
     return st, [out[i] for i in range(n.value)]
+
+
+def group_agg(keys, aggs, sel=None):
+    """keys: list of (np column, type); aggs: list of (func, np column | None,
+    type).  Returns (keys int64 [n, n_keys], [per-agg numpy arrays]) in
+    ascending key order."""
+    n_rows = len(keys[0][0])
+    kk = [(k.ctypes.data, t, int(k.min()) if len(k) else 0, int(k.max()) if len(k) else 0)
+          for k, t in keys]
+    aa = [(f, None if c is None else c.ctypes.data, t) for f, c, t in aggs]
+    d = abi.make_group_desc(n_rows, kk, aa, sel=None if sel is None else sel.ctypes.data,
+                            sel_capacity=0 if sel is None else len(sel))
+    cap = max(1, len(sel) if sel is not None else n_rows)
+    kout = np.empty((cap, len(keys)), dtype=np.int64)
+    outs = [np.empty(cap, dtype=abi.agg_result_dtype(f, t)) for f, _, t in aggs]
+    ptrs = (C.c_void_p * max(1, len(outs)))(*[o.ctypes.data for o in outs])
+    n = lib().orc_group_agg(C.byref(d), kout.ctypes.data, ptrs, cap)
+    return kout[:n].copy(), [o[:n].copy() for o in outs]
 
 
 def date_days(y: int, m: int, d: int) -> int:
