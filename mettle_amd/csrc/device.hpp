@@ -169,6 +169,12 @@ __device__ __forceinline__ int64_t load_key(const void* col, int32_t type,
   }
 }
 
+// Offset of a key from a range minimum, in unsigned arithmetic (a signed
+// subtraction overflows for full-range BIGINT keys).
+__host__ __device__ __forceinline__ uint64_t key_off(int64_t key, int64_t min) {
+  return (uint64_t)key - (uint64_t)min;
+}
+
 // Order-preserving unsigned encodings (smaller encoded = smaller value).
 __device__ __forceinline__ uint64_t enc_i64(int64_t v) {
   return (uint64_t)v ^ 0x8000000000000000ull;

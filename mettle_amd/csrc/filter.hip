@@ -224,7 +224,7 @@ __device__ __forceinline__ uint32_t eval_lane(const FilterArgs& a, uint64_t n,
         uint32_t b = 1u << (4 * j + v);
         if (bits & b) {
           int64_t key = load_key(a.probe_key, a.probe_key_type, rid[j][v]);
-          uint64_t s = (uint64_t)(key - a.probe_min);
+          uint64_t s = key_off(key, a.probe_min);
           bool hit = s < a.probe_span &&
                      ((a.probe_bm[s >> 6] >> (s & 63)) & 1ull);
           if (!hit) bits &= ~b;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
           a.out_sel[pos] = r;
           if (a.mark_bm) {
             int64_t key = load_key(a.mark_key, a.mark_key_type, r);
-            uint64_t s = (uint64_t)(key - a.mark_min);
+            uint64_t s = key_off(key, a.mark_min);
             if (s < a.mark_span) {
               unsigned long long bit = 1ull << (s & 63);
               unsigned long long old =

@@ -87,7 +87,7 @@ __device__ __forceinline__ bool pack_key(const GroupArgs& a, uint64_t row, uint6
   }
   uint64_t p = 0;
   for (uint32_t i = 0; i < a.n_keys; ++i) {
-    uint64_t off = (uint64_t)(load_key(a.key_col[i], a.key_type[i], row) - a.key_min[i]);
+    uint64_t off = key_off(load_key(a.key_col[i], a.key_type[i], row), a.key_min[i]);
     if (off >= a.key_span[i]) return false;
     p += off * a.key_stride[i];
   }
@@ -495,7 +495,7 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
     a->key_col[i] = d->key_col[i];
     a->key_type[i] = d->key_type[i];
     a->key_min[i] = d->key_min[i];
-    a->key_span[i] = (uint64_t)(d->key_max[i] - d->key_min[i]) + 1;
+    a->key_span[i] = key_off(d->key_max[i], d->key_min[i]) + 1;  // 0 = 2^64
     dom *= (a->key_span[i] == 0 ? 18446744073709551616.0L : (long double)a->key_span[i]);
   }
   if (dom > 4611686018427387904.0L) {  // 2^62: cannot pack

@@ -80,7 +80,7 @@ __global__ void k_build(int32_t strategy, const void* keys, int32_t key_type,
     uint32_t row = sel ? sel[i] : (uint32_t)i;
     int64_t key = load_key(keys, key_type, row);
     if (strategy == MTL_JOIN_DENSE) {
-      uint64_t s = (uint64_t)(key - min_key);
+      uint64_t s = key_off(key, min_key);
       if (s >= slots) {
         atomicOr(err, 8u);
         continue;
@@ -116,7 +116,7 @@ __global__ void k_build(int32_t strategy, const void* keys, int32_t key_type,
 __device__ __forceinline__ int32_t lookup(int32_t strategy, const void* table,
                                           int64_t min_key, uint64_t slots, int64_t key) {
   if (strategy == MTL_JOIN_DENSE) {
-    uint64_t s = (uint64_t)(key - min_key);
+    uint64_t s = key_off(key, min_key);
     return s < slots ? ((const int32_t*)table)[s] : -1;
   }
   HashTable h = hash_view((void*)table, slots);

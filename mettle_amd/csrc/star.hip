@@ -56,7 +56,7 @@ __global__ void k_mark_dim(DimArgs a) {
     bool ok = true;
     for (uint32_t p = 0; p < a.n_preds && ok; ++p) ok = eval_pred_row(a.preds[p], i);
     if (!ok) continue;
-    uint64_t s = (uint64_t)(load_key(a.key, a.key_type, i) - a.min);
+    uint64_t s = key_off(load_key(a.key, a.key_type, i), a.min);
     if (s >= a.span) {
       atomicOr(a.err, 8u);
       continue;
@@ -145,7 +145,7 @@ __global__ void k_map(MapArgs a) {
     a.acc[i] = 0;
     a.cnt[i] = 0;
     uint32_t row = a.prow[i];
-    uint64_t s = (uint64_t)(load_key(a.key, a.key_type, row) - a.min);
+    uint64_t s = key_off(load_key(a.key, a.key_type, row), a.min);
     if (s >= a.span) continue;  // flagged by the parent pass
     uint32_t r = rank_of(a.bm, a.rank, s);
     a.row_of[r] = row;
@@ -227,7 +227,7 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void fact_row(const FactArgs& a, uint64_t row, int64_t key,
                                          bool& ovf, bool& sovf) {
-  uint64_t s = (uint64_t)(key - a.min);
+  uint64_t s = key_off(key, a.min);
   if (s >= a.span) return;
   uint64_t w = a.bm[s >> 6];
   if (!((w >> (s & 63)) & 1ull)) return;
@@ -479,14 +479,14 @@ static int32_t star_validate(const mtl_star_desc* d, StarPlan* sp) {
   MTL_TRY(check_preds(d->f_preds, d->f_n_preds, "fact"));
   MTL_TRY(check_expr(d->agg));
   if (d->p_key_max < d->p_key_min) return set_error(MTL_ERR_INVALID_ARGUMENT, "parent key range");
-  sp->pspan = (uint64_t)(d->p_key_max - d->p_key_min) + 1;
+  sp->pspan = key_off(d->p_key_max, d->p_key_min) + 1;
   if (sp->pspan == 0 || sp->pspan > (1ull << 36))
     return set_error(MTL_UNSUPPORTED, "parent key span %llu too wide for a bitmap",
                      (unsigned long long)sp->pspan);
   sp->pwords = ceil_div(sp->pspan, 64);
   if (d->d_rows) {
     if (d->d_key_max < d->d_key_min) return set_error(MTL_ERR_INVALID_ARGUMENT, "dim key range");
-    sp->dspan = (uint64_t)(d->d_key_max - d->d_key_min) + 1;
+    sp->dspan = key_off(d->d_key_max, d->d_key_min) + 1;
     if (sp->dspan == 0 || sp->dspan > (1ull << 36))
       return set_error(MTL_UNSUPPORTED, "dimension key span too wide for a bitmap");
     sp->dwords = ceil_div(sp->dspan, 64);

@@ -145,14 +145,14 @@ int64_t orc_join(const void *bkeys, int32_t btype, const uint32_t *bsel, uint64_
     if (k < mn) mn = k;
     if (k > mx) mx = k;
   }
-  uint64_t span = (uint64_t)(mx - mn) + 1;
+  uint64_t span = (uint64_t)mx - (uint64_t)mn + 1;
   if (span > (1ull << 31)) return -2;
   int64_t *slot = (int64_t *)malloc(span * sizeof(int64_t));
   if (!slot) return -2;
   for (uint64_t i = 0; i < span; ++i) slot[i] = -1;
   for (uint64_t i = 0; i < bn; ++i) {
     uint64_t row = bsel ? bsel[i] : i;
-    uint64_t s = (uint64_t)(orc_key(bkeys, btype, row) - mn);
+    uint64_t s = (uint64_t)orc_key(bkeys, btype, row) - (uint64_t)mn;
     if (slot[s] != -1) {
       free(slot);
       return -1;
@@ -164,7 +164,7 @@ int64_t orc_join(const void *bkeys, int32_t btype, const uint32_t *bsel, uint64_
     uint64_t row = psel ? psel[i] : i;
     int64_t k = orc_key(pkeys, ptype, row);
     if (k < mn || k > mx) continue;
-    int64_t b = slot[(uint64_t)(k - mn)];
+    int64_t b = slot[(uint64_t)k - (uint64_t)mn];
     if (b < 0) continue;
     out_probe[out] = (uint32_t)row;
     out_build[out] = (uint32_t)b;
@@ -256,11 +256,11 @@ static int orc_group_cmp(const void *pa, const void *pb) {
  * MTL_ERR_INVALID_ARGUMENT (key outside its stated range). */
 int32_t orc_star_topk(const mtl_star_desc *d, mtl_star_row *out, uint32_t *out_n) {
   uint8_t *dim_ok = NULL;
-  uint64_t dspan = 0, pspan = (uint64_t)(d->p_key_max - d->p_key_min) + 1;
+  uint64_t dspan = 0, pspan = (uint64_t)d->p_key_max - (uint64_t)d->p_key_min + 1;
   int32_t status = MTL_OK;
   *out_n = 0;
   if (d->d_rows) {
-    dspan = (uint64_t)(d->d_key_max - d->d_key_min) + 1;
+    dspan = (uint64_t)d->d_key_max - (uint64_t)d->d_key_min + 1;
     dim_ok = (uint8_t *)calloc(dspan, 1);
     for (uint64_t i = 0; i < d->d_rows; ++i) {
       int ok = 1;
@@ -296,7 +296,7 @@ int32_t orc_star_topk(const mtl_star_desc *d, mtl_star_row *out, uint32_t *out_n
     for (uint64_t i = 0; i < d->f_rows; ++i) {
       int64_t fk = orc_key(d->f_fk, d->f_fk_type, i);
       if (fk < d->p_key_min || fk > d->p_key_max) continue;
-      uint64_t s = (uint64_t)(fk - d->p_key_min);
+      uint64_t s = (uint64_t)fk - (uint64_t)d->p_key_min;
       if (prow[s] < 0) continue;
       int ok = 1;
       for (uint32_t p = 0; p < d->f_n_preds && ok; ++p) ok = orc_pred_row(&d->f_preds[p], i);
