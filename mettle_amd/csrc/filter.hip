@@ -12,6 +12,7 @@
 #include "filter.hpp"
 #include "mtl_internal.hpp"
 #include "prof.hpp"
+#include "rowtile.hpp"
 
 namespace mtl {
 using namespace dev;
@@ -41,140 +42,8 @@ int32_t launch_zero(const ZeroList& zl, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Filter + compaction.
-// Tile layout: 4 waves x FJ steps x 64 lanes x 4 consecutive rows, so lane l
-// of wave w at step j owns rows base + w*FJ*256 + j*256 + 4l + {0..3}: every
-// load instruction of a wave covers one contiguous span (16-byte lanes).
+// Filter + compaction over row tiles (rowtile.hpp).
 // ---------------------------------------------------------------------------
-constexpr int FBLOCK = 256;
-constexpr int FJ = 4;
-constexpr int FTILE = FBLOCK * FJ * 4;  // 4096 rows per tile
-constexpr int FWAVES = FBLOCK / WAVE;
-
-template <typename T>
-__device__ __forceinline__ void load4(const T* __restrict__ col, uint64_t k,
-                                      uint64_t n, bool vec_ok, T x[4]) {
-  if (vec_ok && k + 3 < n) {
-    if constexpr (sizeof(T) == 4) {
-      uint4 u = *(const uint4*)(col + k);
-      x[0] = __builtin_bit_cast(T, u.x);
-      x[1] = __builtin_bit_cast(T, u.y);
-      x[2] = __builtin_bit_cast(T, u.z);
-      x[3] = __builtin_bit_cast(T, u.w);
-    } else if constexpr (sizeof(T) == 8) {
-      ulonglong2 a = *(const ulonglong2*)(col + k);
-      ulonglong2 b = *(const ulonglong2*)(col + k + 2);
-      x[0] = __builtin_bit_cast(T, a.x);
-      x[1] = __builtin_bit_cast(T, a.y);
-      x[2] = __builtin_bit_cast(T, b.x);
-      x[3] = __builtin_bit_cast(T, b.y);
-    } else {
-      uint32_t u = *(const uint32_t*)(col + k);
-      x[0] = (T)(u & 0xff);
-      x[1] = (T)((u >> 8) & 0xff);
-      x[2] = (T)((u >> 16) & 0xff);
-      x[3] = (T)(u >> 24);
-    }
-  } else {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) x[v] = (k + v < n) ? col[k + v] : T(0);
-  }
-}
-
-template <typename T, typename Cmp>
-__device__ __forceinline__ void pred_dense(const T* __restrict__ col,
-                                           uint64_t k0, uint64_t n, bool vec_ok,
-                                           Cmp cmp, uint32_t& bits) {
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    uint32_t m = (bits >> (4 * j)) & 0xFu;
-    if (!m) continue;
-    T x[4];
-    load4<T>(col, k0 + (uint64_t)j * 256, n, vec_ok, x);
-    uint32_t r = 0;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) r |= (uint32_t)cmp(x[v]) << v;
-    bits &= ~(((~r) & 0xFu) << (4 * j));
-  }
-}
-
-template <typename T, typename Cmp>
-__device__ __forceinline__ void pred_sel(const T* __restrict__ col,
-                                         const uint32_t (&rid)[FJ][4],
-                                         Cmp cmp, uint32_t& bits) {
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      uint32_t b = 1u << (4 * j + v);
-      if ((bits & b) && !cmp(col[rid[j][v]])) bits &= ~b;
-    }
-  }
-}
-
-// Dispatch one predicate on (type, op); `f` receives a typed column pointer
-// and a comparison functor.
-template <typename F>
-__device__ __forceinline__ void with_pred(const mtl_pred& p, F&& f) {
-  const int op = p.op;
-  switch (p.type) {
-    case MTL_INTEGER:
-    case MTL_DATE:
-    case MTL_DICT32: {
-      const int32_t s = (int32_t)p.scalar;
-      const int32_t* c = (const int32_t*)p.column;
-      switch (op) {
-        case MTL_CMP_EQ: f(c, [=](int32_t x) { return x == s; }); break;
-        case MTL_CMP_LT: f(c, [=](int32_t x) { return x < s; }); break;
-        case MTL_CMP_LE: f(c, [=](int32_t x) { return x <= s; }); break;
-        case MTL_CMP_GT: f(c, [=](int32_t x) { return x > s; }); break;
-        default: f(c, [=](int32_t x) { return x >= s; }); break;
-      }
-      break;
-    }
-    case MTL_BIGINT:
-    case MTL_TIMESTAMP:
-    case MTL_DECIMAL: {
-      const int64_t s = p.scalar;
-      const int64_t* c = (const int64_t*)p.column;
-      switch (op) {
-        case MTL_CMP_EQ: f(c, [=](int64_t x) { return x == s; }); break;
-        case MTL_CMP_LT: f(c, [=](int64_t x) { return x < s; }); break;
-        case MTL_CMP_LE: f(c, [=](int64_t x) { return x <= s; }); break;
-        case MTL_CMP_GT: f(c, [=](int64_t x) { return x > s; }); break;
-        default: f(c, [=](int64_t x) { return x >= s; }); break;
-      }
-      break;
-    }
-    case MTL_DICT8: {
-      const int32_t s = (int32_t)p.scalar;
-      const uint8_t* c = (const uint8_t*)p.column;
-      switch (op) {
-        case MTL_CMP_EQ: f(c, [=](uint8_t x) { return (int32_t)x == s; }); break;
-        case MTL_CMP_LT: f(c, [=](uint8_t x) { return (int32_t)x < s; }); break;
-        case MTL_CMP_LE: f(c, [=](uint8_t x) { return (int32_t)x <= s; }); break;
-        case MTL_CMP_GT: f(c, [=](uint8_t x) { return (int32_t)x > s; }); break;
-        default: f(c, [=](uint8_t x) { return (int32_t)x >= s; }); break;
-      }
-      break;
-    }
-    case MTL_FLOAT: {
-      const float s = (float)__longlong_as_double(p.scalar);
-      const float* c = (const float*)p.column;
-      f(c, [=](float x) { return cmp_float<float>(x, s, op); });
-      break;
-    }
-    case MTL_DOUBLE: {
-      const double s = __longlong_as_double(p.scalar);
-      const double* c = (const double*)p.column;
-      f(c, [=](double x) { return cmp_float<double>(x, s, op); });
-      break;
-    }
-    default:
-      break;
-  }
-}
-
 // Evaluate all predicates (and the optional semi-join probe) for the 16 rows
 // of this lane; returns the pass bits (bit 4j+v).
 __device__ __forceinline__ uint32_t eval_lane(const FilterArgs& a, uint64_t n,
@@ -199,10 +68,14 @@ __device__ __forceinline__ uint32_t eval_lane(const FilterArgs& a, uint64_t n,
       }
     }
   } else {
+    // rows past the end read row 0 (always in bounds); their bits are clear
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) rid[j][v] = (uint32_t)(k0 + (uint64_t)j * 256 + v);
+      for (int v = 0; v < 4; ++v) {
+        uint64_t k = k0 + (uint64_t)j * 256 + v;
+        rid[j][v] = k < n ? (uint32_t)k : 0u;
+      }
   }
 
   for (int p = 0; p < a.n_preds && bits; ++p) {
@@ -215,39 +88,101 @@ __device__ __forceinline__ uint32_t eval_lane(const FilterArgs& a, uint64_t n,
       });
     }
   }
-  // Semi-join probe: key(row) - min must be a set bit of the bitmap.
+  // Semi-join probe: key(row) - min must be a set bit of the bitmap.  All
+  // 16 keys and bitmap words are loaded before any is used, so the loads
+  // overlap instead of serialising behind one another.
   if (a.probe_bm && bits) {
+    int64_t key[FJ][4];
+    load_keys16(a.probe_key, a.probe_key_type, !a.in_sel && a.probe_vec, k0, n, rid, key);
+    uint64_t w[FJ][4];
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        uint32_t b = 1u << (4 * j + v);
-        if (bits & b) {
-          int64_t key = load_key(a.probe_key, a.probe_key_type, rid[j][v]);
-          uint64_t s = key_off(key, a.probe_min);
-          bool hit = s < a.probe_span &&
-                     ((a.probe_bm[s >> 6] >> (s & 63)) & 1ull);
-          if (!hit) bits &= ~b;
-        }
+        uint64_t s = key_off(key[j][v], a.probe_min);
+        w[j][v] = a.probe_bm[s < a.probe_span ? (s >> 6) : 0];
+      }
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        uint64_t s = key_off(key[j][v], a.probe_min);
+        bool hit = s < a.probe_span && ((w[j][v] >> (s & 63)) & 1ull);
+        if (!hit) bits &= ~(1u << (4 * j + v));
       }
   }
   return bits;
 }
 
+// Words of the mark bitmap near the tile's first key are OR-ed in an LDS
+// window and flushed once per block (sorted keys: one global atomic per
+// touched word instead of one per lane-step); others go straight to HBM.
+constexpr int kMarkWin = 1024;  // words (64K keys), 8 KB of LDS
+
+__device__ __forceinline__ void mark_word(const FilterArgs& a, uint64_t* win, uint64_t wbase,
+                                          uint64_t word, uint64_t bits) {
+  uint64_t wi = word - wbase;
+  if (wi < (uint64_t)kMarkWin) atomicOr((unsigned long long*)&win[wi], bits);
+  else atomicOr((unsigned long long*)&a.mark_bm[word], bits);
+}
+
+// Set bit (key - min) of the mark bitmap for every kept row.  The (up to)
+// four rows of a lane-step usually share one 64-bit word (sorted keys):
+// their bits are OR-ed first, so one atomic covers them.  The atomics do
+// not return: a repeated key is detected afterwards by comparing the
+// bitmap's popcount with mark_count (the number of marked rows).
+__device__ __forceinline__ void mark_rows(const FilterArgs& a, uint64_t n, uint64_t k0,
+                                          const uint32_t (&rid)[FJ][4], uint32_t bits,
+                                          uint64_t* win, uint64_t wbase) {
+  int64_t key[FJ][4];
+  load_keys16(a.mark_key, a.mark_key_type, !a.in_sel && a.mark_vec, k0, n, rid, key);
+  uint32_t e = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    uint32_t m = (bits >> (4 * j)) & 0xFu;
+    if (!m) continue;
+    uint64_t wj = ~0ull, bj = 0;
+    uint32_t rest = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (!((m >> v) & 1u)) continue;
+      uint64_t s = key_off(key[j][v], a.mark_min);
+      if (s >= a.mark_span) {
+        e |= 8u;
+        continue;
+      }
+      if (wj == ~0ull) wj = s >> 6;
+      if ((s >> 6) == wj) bj |= 1ull << (s & 63);
+      else rest |= 1u << v;
+    }
+    if (bj) mark_word(a, win, wbase, wj, bj);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (!((rest >> v) & 1u)) continue;
+      uint64_t s = key_off(key[j][v], a.mark_min);
+      mark_word(a, win, wbase, s >> 6, 1ull << (s & 63));
+    }
+  }
+  if (e) atomicOr(a.err, e);
+  if (a.mark_count) {
+    uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
+    if (lane_id() == 0 && c) atomicAdd(a.mark_count, c);
+  }
+}
+
+// Pass 1 of a filter: one tile of 4096 rows per block, no inter-block
+// dependency.  Writes the tile's pass bits to sel_bm (1 bit per row, bit r
+// for row r of the input order) and its count to tile_counts[tile].
 __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
-  __shared__ uint32_t s_tile;
   __shared__ uint32_t s_wave_tot[FWAVES];
-  __shared__ uint32_t s_prefix;
   const int tid = threadIdx.x;
   const int wave = tid / WAVE;
   const int lane = tid % WAVE;
-  if (tid == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = blockIdx.x;
   const uint64_t n = a.in_sel ? (a.in_count ? (uint64_t)*a.in_count : a.n_rows)
                               : a.n_rows;
-  const uint64_t k0 = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256) +
-                      (uint64_t)lane * 4;
+  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
+  const uint64_t k0 = seg + (uint64_t)lane * 4;
   uint32_t rid[FJ][4];
   uint32_t bits = 0;
   if ((uint64_t)tile * FTILE < n) bits = eval_lane(a, n, k0, rid);
@@ -257,65 +192,121 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) rid[j][v] = 0;
   }
-
-  // per-step ballots -> ranks in row order
-  uint32_t lane_base[FJ];
-  uint32_t wave_total = 0;
-  const uint64_t lt = lanemask_lt();
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    uint32_t before = 0, tot = 0;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      uint64_t b = __ballot((bits >> (4 * j + v)) & 1u);
-      before += __popcll(b & lt);
-      tot += __popcll(b);
+  if (a.mark_bm) {
+    __shared__ uint64_t win[kMarkWin];
+    __shared__ uint64_t s_wbase;
+    for (int i = tid; i < kMarkWin; i += FBLOCK) win[i] = 0;
+    if (tid == 0) {
+      uint64_t base = 0;
+      if ((uint64_t)tile * FTILE < n) {
+        uint64_t r = a.in_sel ? a.in_sel[(uint64_t)tile * FTILE] : (uint64_t)tile * FTILE;
+        uint64_t s0 = key_off(load_key(a.mark_key, a.mark_key_type, r), a.mark_min);
+        base = s0 < a.mark_span ? (s0 >> 6) : 0;
+      }
+      s_wbase = base;
     }
-    lane_base[j] = wave_total + before;
-    wave_total += tot;
+    __syncthreads();
+    const uint64_t wbase = s_wbase;
+    mark_rows(a, n, k0, rid, bits, win, wbase);  // all lanes (wave sum)
+    __syncthreads();
+    const uint64_t words = (a.mark_span + 63) / 64;
+    for (int i = tid; i < kMarkWin; i += FBLOCK) {
+      uint64_t w = win[i];
+      if (w && wbase + i < words) atomicOr((unsigned long long*)&a.mark_bm[wbase + i], w);
+    }
   }
-  if (lane == 0) s_wave_tot[wave] = wave_total;
-  __syncthreads();
-  uint32_t wave_off = 0, tile_total = 0;
-#pragma unroll
-  for (int w = 0; w < FWAVES; ++w) {
-    uint32_t t = s_wave_tot[w];
-    if (w < wave) wave_off += t;
-    tile_total += t;
-  }
-  if (wave == 0) {
-    uint32_t ex = lookback_wave(a.status, tile, tile_total, a.err);
-    if (lane == 0) s_prefix = ex;
-  }
-  __syncthreads();
-  const uint32_t base = s_prefix + wave_off;
-  if (bits) {
+  if (a.sel_bm) {
+    // lane l's 4 bits of step j sit at bits 4(l%16).. of word l/16 of the
+    // 256-row segment: OR the 16 lanes of each quarter-wave together
 #pragma unroll
     for (int j = 0; j < FJ; ++j) {
-      uint32_t pos = base + lane_base[j];
+      uint64_t w = (uint64_t)((bits >> (4 * j)) & 0xFu) << (4 * (lane & 15));
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        if ((bits >> (4 * j + v)) & 1u) {
-          uint32_t r = rid[j][v];
-          a.out_sel[pos] = r;
-          if (a.mark_bm) {
-            int64_t key = load_key(a.mark_key, a.mark_key_type, r);
-            uint64_t s = key_off(key, a.mark_min);
-            if (s < a.mark_span) {
-              unsigned long long bit = 1ull << (s & 63);
-              unsigned long long old =
-                  atomicOr((unsigned long long*)&a.mark_bm[s >> 6], bit);
-              if (old & bit) atomicOr(a.err, 2u);  // duplicate key
-            } else {
-              atomicOr(a.err, 8u);  // key outside the stated [min, max]
-            }
-          }
-          ++pos;
-        }
-      }
+      for (int off = 1; off < 16; off <<= 1) w |= __shfl_xor(w, off, WAVE);
+      if ((lane & 15) == 0) a.sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = w;
     }
   }
-  if (tile == gridDim.x - 1 && tid == 0) *a.out_count = s_prefix + tile_total;
+  if (a.tile_counts) {
+    uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
+    if (lane == 0) s_wave_tot[wave] = c;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t t = 0;
+#pragma unroll
+      for (int w = 0; w < FWAVES; ++w) t += s_wave_tot[w];
+      a.tile_counts[tile] = t;
+    }
+  }
+}
+
+// Pass 2: exclusive scan of the tile counts in place (one block), total to
+// *total.  T is at most 2^32 / 4096 = 2^20 tiles.
+constexpr int SBLOCK = 1024;
+__global__ __launch_bounds__(SBLOCK) void k_scan_counts(uint32_t* counts, uint32_t T,
+                                                        uint32_t* total) {
+  __shared__ uint32_t s_wave[SBLOCK / WAVE];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t per = (T + SBLOCK - 1) / SBLOCK;
+  const uint32_t b = tid * per, e = min(T, b + per);
+  uint32_t sum = 0;
+  for (uint32_t i = b; i < e; ++i) sum += counts[i];
+  uint32_t incl = wave_inclusive_u32(sum);
+  if (lane == WAVE - 1) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t off = 0, all = 0;
+  for (int w = 0; w < SBLOCK / WAVE; ++w) {
+    if (w < wave) off += s_wave[w];
+    all += s_wave[w];
+  }
+  uint32_t run = off + incl - sum;
+  for (uint32_t i = b; i < e; ++i) {
+    uint32_t c = counts[i];
+    counts[i] = run;
+    run += c;
+  }
+  if (tid == 0) *total = all;
+}
+
+// Pass 3: expand the pass bits of each tile into ascending row ids at the
+// tile's offset.  Thread t of the block owns rows 16t .. 16t+15 of the tile.
+__global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ sel_bm,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ in_sel,
+                                                   uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_wave[FWAVES];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint64_t r0 = (uint64_t)blockIdx.x * FTILE + (uint64_t)tid * 16;
+  const uint32_t m = (uint32_t)(sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
+  const uint32_t c = (uint32_t)__popc(m);
+  uint32_t incl = wave_inclusive_u32(c);
+  if (lane == WAVE - 1) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t pos = offsets[blockIdx.x] + incl - c;
+#pragma unroll
+  for (int w = 0; w < FWAVES; ++w)
+    if (w < wave) pos += s_wave[w];
+  uint32_t mm = m;
+  while (mm) {
+    int i = __ffs(mm) - 1;
+    mm &= mm - 1;
+    uint64_t r = r0 + (uint64_t)i;
+    out[pos++] = in_sel ? in_sel[r] : (uint32_t)r;
+  }
+}
+
+void scan_counts(uint32_t* counts, uint32_t tiles, uint32_t* total, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SBLOCK), 0, st, counts, tiles, total);
+}
+
+int32_t launch_compact(const uint64_t* sel_bm, uint32_t* tile_counts, uint32_t tiles,
+                       const uint32_t* in_sel, uint32_t* out_sel, uint32_t* out_count,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SBLOCK), 0, st, tile_counts, tiles, out_count);
+  MTL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_expand, dim3(tiles), dim3(FBLOCK), 0, st, sel_bm, tile_counts, in_sel,
+                     out_sel);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
 }
 
 uint32_t filter_tiles(uint64_t n) { return (uint32_t)ceil_div(n ? n : 1, FTILE); }
@@ -332,6 +323,13 @@ int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st) {
   if (a.in_sel && ((uintptr_t)a.in_sel % 16) != 0)
     return set_error(MTL_ERR_INVALID_ARGUMENT,
                      "selection vectors must be 16-byte aligned");
+  auto key_vec = [](const void* col, int32_t type) -> uint32_t {
+    if (!col) return 0;
+    int w = type_width(type);
+    return ((uintptr_t)col % (w == 1 ? 4 : 16)) == 0 ? 1u : 0u;
+  };
+  a.probe_vec = key_vec(a.probe_key, a.probe_key_type);
+  a.mark_vec = key_vec(a.mark_key, a.mark_key_type);
   hipLaunchKernelGGL(k_filter, dim3(tiles), dim3(FBLOCK), 0, st, a);
   MTL_HIP(hipGetLastError());
   return MTL_OK;
@@ -444,11 +442,11 @@ int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
   std::lock_guard<std::mutex> g(s->mu);
   s->arena.reset();
   uint32_t tiles = filter_tiles(cap);
-  void *status, *ctr;
-  MTL_TRY(s->arena.alloc((size_t)tiles * 8, &status));
+  void *bm, *counts, *ctr;
+  MTL_TRY(s->arena.alloc((size_t)tiles * FTILE / 8, &bm));
+  MTL_TRY(s->arena.alloc((size_t)tiles * 4, &counts));
   MTL_TRY(s->arena.alloc(256, &ctr));
   ZeroList zl;
-  zl.add((uint32_t*)status, (uint64_t)tiles * 2);
   zl.add((uint32_t*)ctr, 64);
   MTL_TRY(launch_zero(zl, s->stream));
   FilterArgs a{};
@@ -457,13 +455,16 @@ int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
   a.n_rows = cap;
   a.in_sel = in_sel;
   a.in_count = in_sel ? in_count : nullptr;
-  a.out_sel = out_sel;
-  a.out_count = out_count;
-  a.status = (uint64_t*)status;
-  a.tile_ctr = (uint32_t*)ctr;
-  a.err = (uint32_t*)ctr + 1;
-  ProfScope ps(s, "k_filter");
-  return launch_filter(a, cap, s->stream);
+  a.sel_bm = (uint64_t*)bm;
+  a.tile_counts = (uint32_t*)counts;
+  a.err = (uint32_t*)ctr;
+  {
+    ProfScope ps(s, "k_filter");
+    MTL_TRY(launch_filter(a, cap, s->stream));
+  }
+  ProfScope ps(s, "k_compact");
+  return launch_compact((const uint64_t*)bm, (uint32_t*)counts, tiles, in_sel, out_sel,
+                        out_count, s->stream);
 }
 
 int32_t mtl_compare(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,

@@ -30,11 +30,9 @@ struct FilterArgs {
   uint64_t n_rows;  // rows (no in_sel) or capacity of in_sel
   const uint32_t* in_sel;
   const uint32_t* in_count;
-  uint32_t* out_sel;
-  uint32_t* out_count;
-  uint64_t* status;   // look-back status, one word per tile (zeroed)
-  uint32_t* tile_ctr; // ticket counter (zeroed)
-  uint32_t* err;      // bit 0: look-back spin cap, bit 1: duplicate mark key
+  uint64_t* sel_bm;       // optional: pass bits, 1 bit per row (tiles*4096)
+  uint32_t* tile_counts;  // optional: pass count per 4096-row tile
+  uint32_t* err;          // bit 3: mark key outside [min, max]
   // optional semi-join probe: keep the row only if bit (key - min) is set
   const uint64_t* probe_bm;
   const void* probe_key;
@@ -47,9 +45,22 @@ struct FilterArgs {
   int32_t mark_key_type;
   int64_t mark_min;
   uint64_t mark_span;
+  uint32_t probe_vec, mark_vec;  // key columns aligned for vector loads
+  uint32_t* mark_count;          // optional: number of marked rows (+=)
 };
+// One block per 4096-row tile, no inter-block dependency.  With sel_bm and
+// tile_counts set, launch_compact turns the pass bits into a stable
+// selection vector; with only mark_bm set it is a mark-only pass.
 
+constexpr int kTileRows = 4096;
 uint32_t filter_tiles(uint64_t n);
 int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st);
+// Exclusive scan of tile counts in place (one block); *total = sum.
+void scan_counts(uint32_t* counts, uint32_t tiles, uint32_t* total, hipStream_t st);
+// Exclusive scan of tile_counts (in place) + expansion of sel_bm into
+// ascending row ids (in_sel[k] when in_sel is given); *out_count = total.
+int32_t launch_compact(const uint64_t* sel_bm, uint32_t* tile_counts, uint32_t tiles,
+                       const uint32_t* in_sel, uint32_t* out_sel, uint32_t* out_count,
+                       hipStream_t st);
 
 }  // namespace mtl

@@ -247,49 +247,26 @@ __global__ __launch_bounds__(256) void k_group_agg(GroupArgs a) {
   }
 }
 
-// Stable compaction of non-empty groups (count word != 0).
-constexpr int CBLOCK = 256;
-constexpr int CPER = 8;
-constexpr int CTILE = CBLOCK * CPER;
-
-__global__ __launch_bounds__(CBLOCK) void k_group_compact(const uint64_t* st, uint32_t W,
-                                                          uint64_t n, uint32_t* out,
-                                                          uint32_t* out_count,
-                                                          uint64_t* status, uint32_t* ctr,
-                                                          uint32_t* err) {
-  __shared__ uint32_t s_tile, s_prefix;
-  __shared__ uint32_t s_wave[CBLOCK / WAVE];
+// Non-empty groups (count word != 0) as a bitmask + per-4096-entry counts;
+// launch_compact turns them into ascending group indices.  Thread t of a
+// block owns entries 16t .. 16t+15 of the tile.
+__global__ __launch_bounds__(256) void k_group_flags(const uint64_t* st, uint32_t W, uint64_t n,
+                                                     uint64_t* bm, uint32_t* counts) {
+  __shared__ uint32_t s_wave[4];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  if (tid == 0) s_tile = atomicAdd(ctr, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint64_t i0 = (uint64_t)tile * CTILE + (uint64_t)tid * CPER;
-  uint32_t flags = 0, c = 0;
+  const uint64_t e0 = (uint64_t)blockIdx.x * kTileRows + (uint64_t)tid * 16;
+  uint32_t m = 0;
 #pragma unroll
-  for (int j = 0; j < CPER; ++j)
-    if (i0 + j < n && st[(i0 + j) * W] != 0) {
-      flags |= 1u << j;
-      ++c;
-    }
-  uint32_t incl = wave_inclusive_u32(c);
-  if (lane == WAVE - 1) s_wave[wave] = incl;
+  for (int j = 0; j < 16; ++j)
+    if (e0 + j < n && st[(e0 + j) * W] != 0) m |= 1u << j;
+  uint64_t w = (uint64_t)m << (16 * (tid & 3));
+  w |= __shfl_xor(w, 1, WAVE);
+  w |= __shfl_xor(w, 2, WAVE);
+  if ((tid & 3) == 0) bm[e0 / 64] = w;
+  uint32_t c = wave_sum_u32((uint32_t)__popc(m));
+  if (lane == 0) s_wave[wave] = c;
   __syncthreads();
-  uint32_t woff = 0, total = 0;
-#pragma unroll
-  for (int w = 0; w < CBLOCK / WAVE; ++w) {
-    if (w < wave) woff += s_wave[w];
-    total += s_wave[w];
-  }
-  if (wave == 0) {
-    uint32_t ex = lookback_wave(status, tile, total, err);
-    if (lane == 0) s_prefix = ex;
-  }
-  __syncthreads();
-  uint32_t pos = s_prefix + woff + incl - c;
-#pragma unroll
-  for (int j = 0; j < CPER; ++j)
-    if (flags & (1u << j)) out[pos++] = (uint32_t)(i0 + j);
-  if (tile == gridDim.x - 1 && tid == 0) *out_count = s_prefix + total;
+  if (tid == 0) counts[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
 }
 
 struct EmitArgs {
@@ -393,61 +370,20 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint6
 
 __global__ __launch_bounds__(TOPK_BLOCK) void k_group_topk(GTopkArgs a) {
   __shared__ TopkLds s;
-  __shared__ bool s_last;
   topk_init(s);
   const uint64_t chunk = (a.n + gridDim.x - 1) / gridDim.x;
   const uint64_t start = (uint64_t)blockIdx.x * chunk;
   const uint64_t end = start + chunk < a.n ? start + chunk : a.n;
   for (uint64_t base = start; base < end; base += TOPK_ROUND) {
-    for (int e = 0; e < 2; ++e) {
-      uint64_t g = base + e * TOPK_BLOCK + threadIdx.x;
-      if (g < end && a.st[g * a.W] != 0) {
-        uint64_t k[4];
-        gtopk_keys(a, g, k);
-        if (k[0] <= s.thr) topk_push(s, k[0], k[1], k[2], k[3], (uint32_t)g);
-      }
+    uint64_t g = base + threadIdx.x;
+    if (g < end && a.st[g * a.W] != 0) {
+      uint64_t k[4];
+      gtopk_keys(a, g, k);
+      if (k[0] <= s.thr) topk_push(s, k[0], k[1], k[2], k[3], (uint32_t)g);
     }
     topk_maybe_compact(s, a.k);
   }
-  __syncthreads();
-  uint32_t n = s.n;
-  __syncthreads();
-  topk_sort(s, n);
-  topk_truncate(s, a.k);
-  const uint32_t kb = s.n;
-  for (uint32_t i = threadIdx.x; i < kb; i += blockDim.x) {
-    uint64_t o = (uint64_t)blockIdx.x * a.k + i;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) a.ck[f][o] = s.k[f][i];
-    a.cid[o] = s.id[i];
-  }
-  if (threadIdx.x == 0) a.cn[blockIdx.x] = kb;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  topk_init(s);
-  for (uint32_t b = 0; b < gridDim.x; ++b) {
-    uint32_t nb = __hip_atomic_load(&a.cn[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t base = 0; base < nb; base += TOPK_ROUND) {
-      for (int e = 0; e < 2; ++e) {
-        uint32_t i = base + e * TOPK_BLOCK + threadIdx.x;
-        if (i < nb) {
-          uint64_t o = (uint64_t)b * a.k + i;
-          uint64_t k0 = a.ck[0][o];
-          if (k0 <= s.thr) topk_push(s, k0, a.ck[1][o], a.ck[2][o], a.ck[3][o], a.cid[o]);
-        }
-      }
-      topk_maybe_compact(s, a.k);
-    }
-  }
-  __syncthreads();
-  n = s.n;
-  __syncthreads();
-  topk_sort(s, n);
-  topk_truncate(s, a.k);
+  if (!topk_publish_elect_merge(s, a.ck, a.cid, a.cn, a.done, a.k)) return;
   for (uint32_t i = threadIdx.x; i < s.n; i += blockDim.x) a.out_idx[i] = s.id[i];
   if (threadIdx.x == 0) *a.out_n = s.n;
 }
@@ -747,22 +683,20 @@ static int32_t compact_groups(mtl_groups* g, uint32_t** idx, uint32_t** count) {
   mtl_stream* s = g->s;
   const GroupArgs& a = g->args;
   uint64_t n = g->entries;
-  uint32_t tiles = (uint32_t)ceil_div(n ? n : 1, CTILE);
-  void *list, *status, *misc;
+  uint32_t tiles = (uint32_t)ceil_div(n ? n : 1, kTileRows);
+  void *list, *bm, *counts, *misc;
   MTL_TRY(s->arena.alloc((n ? n : 1) * 4, &list));
-  MTL_TRY(s->arena.alloc((uint64_t)tiles * 8, &status));
+  MTL_TRY(s->arena.alloc((uint64_t)tiles * kTileRows / 8, &bm));
+  MTL_TRY(s->arena.alloc((uint64_t)tiles * 4, &counts));
   MTL_TRY(s->arena.alloc(256, &misc));
-  ZeroList zl;
-  zl.add((uint32_t*)status, (uint64_t)tiles * 2);
-  zl.add((uint32_t*)misc, 64);
-  MTL_TRY(launch_zero(zl, s->stream));
   {
     ProfScope ps(s, "k_group_compact");
-    hipLaunchKernelGGL(k_group_compact, dim3(tiles), dim3(CBLOCK), 0, s->stream, a.st, a.W, n,
-                       (uint32_t*)list, (uint32_t*)misc, (uint64_t*)status,
-                       (uint32_t*)misc + 1, (uint32_t*)misc + 2);
+    hipLaunchKernelGGL(k_group_flags, dim3(tiles), dim3(256), 0, s->stream, a.st, a.W, n,
+                       (uint64_t*)bm, (uint32_t*)counts);
+    MTL_HIP(hipGetLastError());
+    MTL_TRY(launch_compact((const uint64_t*)bm, (uint32_t*)counts, tiles, nullptr,
+                           (uint32_t*)list, (uint32_t*)misc, s->stream));
   }
-  MTL_HIP(hipGetLastError());
   *idx = (uint32_t*)list;
   *count = (uint32_t*)misc;
   return MTL_OK;

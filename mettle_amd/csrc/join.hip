@@ -11,14 +11,15 @@
 //     no valid key collides with the sentinel (kernels.metal:187-193 used
 //     0xFFFFFFFF, which is also the key -1);
 //   - duplicate build keys are reported, not overwritten (:202-206);
-//   - probe + match compaction are one pass with ascending probe order
-//     (ballot ranks + decoupled look-back) instead of an atomic counter.
+//   - matches come out in ascending probe order (match bitmask + tile counts,
+//     scan, expand) instead of the atomic-counter order of compact_matches.
 #include <algorithm>
 
 #include "device.hpp"
 #include "filter.hpp"
 #include "mtl_internal.hpp"
 #include "prof.hpp"
+#include "rowtile.hpp"
 
 namespace mtl {
 using namespace dev;
@@ -131,11 +132,13 @@ __device__ __forceinline__ int32_t lookup(int32_t strategy, const void* table,
   return -1;
 }
 
-// Probe + stable compaction: tile = 4 waves x PJ steps x 64 lanes x 4 rows.
-constexpr int PBLOCK = 256;
-constexpr int PJ = 2;
-constexpr int PTILE = PBLOCK * PJ * 4;
-
+// Probe in two parallel passes (no inter-block dependency):
+//   k_probe_bits   : one 4096-row tile per block; bit k of the match mask is
+//                    set when probe position k finds its key, plus the
+//                    tile's match count;
+//   (scan of the tile counts, filter.hip)
+//   k_probe_expand : positions with a set bit look their key up again and
+//                    write (probe row, build row) at ascending offsets.
 struct ProbeArgs {
   int32_t strategy;
   const void* table;
@@ -146,82 +149,79 @@ struct ProbeArgs {
   const uint32_t* sel;
   const uint32_t* count;
   uint64_t cap_rows;
+  uint64_t* match_bm;
+  uint32_t* tile_counts;
   uint32_t* out_probe;
   uint32_t* out_build;
-  uint32_t* out_count;
-  uint64_t* status;
-  uint32_t* ctr;
-  uint32_t* err;
 };
 
-__global__ __launch_bounds__(PBLOCK) void k_probe(ProbeArgs a) {
-  __shared__ uint32_t s_tile, s_prefix;
-  __shared__ uint32_t s_wave[PBLOCK / WAVE];
+__global__ __launch_bounds__(FBLOCK) void k_probe_bits(ProbeArgs a) {
+  __shared__ uint32_t s_wave[FWAVES];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  if (tid == 0) s_tile = atomicAdd(a.ctr, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
   const uint64_t n = a.sel ? (a.count ? (uint64_t)*a.count : a.cap_rows) : a.cap_rows;
-  const uint64_t k0 = (uint64_t)tile * PTILE + (uint64_t)wave * (PJ * 256) + (uint64_t)lane * 4;
-  uint32_t rid[PJ][4];
-  int32_t bid[PJ][4];
-  uint32_t bits = 0;
+  const uint64_t seg = (uint64_t)blockIdx.x * FTILE + (uint64_t)wave * (FJ * 256);
+  const uint64_t k0 = seg + (uint64_t)lane * 4;
+  int64_t key[FJ][4];
 #pragma unroll
-  for (int j = 0; j < PJ; ++j)
+  for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       uint64_t k = k0 + (uint64_t)j * 256 + v;
-      rid[j][v] = 0;
-      bid[j][v] = -1;
-      if (k < n) {
-        uint32_t row = a.sel ? a.sel[k] : (uint32_t)k;
-        rid[j][v] = row;
-        int32_t b = lookup(a.strategy, a.table, a.min_key, a.slots,
-                           load_key(a.keys, a.key_type, row));
-        bid[j][v] = b;
-        if (b >= 0) bits |= 1u << (4 * j + v);
-      }
+      uint32_t row = k < n ? (a.sel ? a.sel[k] : (uint32_t)k) : 0u;
+      key[j][v] = load_key(a.keys, a.key_type, row);
     }
-  const uint64_t lt = lanemask_lt();
-  uint32_t lane_base[PJ], wave_total = 0;
+  uint32_t bits = 0;
 #pragma unroll
-  for (int j = 0; j < PJ; ++j) {
-    uint32_t before = 0, tot = 0;
+  for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      uint64_t b = __ballot((bits >> (4 * j + v)) & 1u);
-      before += __popcll(b & lt);
-      tot += __popcll(b);
+      uint64_t k = k0 + (uint64_t)j * 256 + v;
+      if (k < n && lookup(a.strategy, a.table, a.min_key, a.slots, key[j][v]) >= 0)
+        bits |= 1u << (4 * j + v);
     }
-    lane_base[j] = wave_total + before;
-    wave_total += tot;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    uint64_t w = (uint64_t)((bits >> (4 * j)) & 0xFu) << (4 * (lane & 15));
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) w |= __shfl_xor(w, off, WAVE);
+    if ((lane & 15) == 0) a.match_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = w;
   }
-  if (lane == 0) s_wave[wave] = wave_total;
+  uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
+  if (lane == 0) s_wave[wave] = c;
   __syncthreads();
-  uint32_t woff = 0, total = 0;
+  if (tid == 0) {
+    uint32_t t = 0;
 #pragma unroll
-  for (int w = 0; w < PBLOCK / WAVE; ++w) {
-    if (w < wave) woff += s_wave[w];
-    total += s_wave[w];
+    for (int w = 0; w < FWAVES; ++w) t += s_wave[w];
+    a.tile_counts[blockIdx.x] = t;
   }
-  if (wave == 0) {
-    uint32_t ex = lookback_wave(a.status, tile, total, a.err);
-    if (lane == 0) s_prefix = ex;
-  }
+}
+
+__global__ __launch_bounds__(FBLOCK) void k_probe_expand(ProbeArgs a) {
+  __shared__ uint32_t s_wave[FWAVES];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint64_t r0 = (uint64_t)blockIdx.x * FTILE + (uint64_t)tid * 16;
+  const uint32_t m = (uint32_t)(a.match_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
+  const uint32_t c = (uint32_t)__popc(m);
+  uint32_t incl = wave_inclusive_u32(c);
+  if (lane == WAVE - 1) s_wave[wave] = incl;
   __syncthreads();
-  const uint32_t base = s_prefix + woff;
+  uint32_t pos = a.tile_counts[blockIdx.x] + incl - c;
 #pragma unroll
-  for (int j = 0; j < PJ; ++j) {
-    uint32_t pos = base + lane_base[j];
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if ((bits >> (4 * j + v)) & 1u) {
-        a.out_probe[pos] = rid[j][v];
-        a.out_build[pos] = (uint32_t)bid[j][v];
-        ++pos;
-      }
+  for (int w = 0; w < FWAVES; ++w)
+    if (w < wave) pos += s_wave[w];
+  uint32_t mm = m;
+  while (mm) {
+    int i = __ffs(mm) - 1;
+    mm &= mm - 1;
+    uint64_t k = r0 + (uint64_t)i;
+    uint32_t row = a.sel ? a.sel[k] : (uint32_t)k;
+    int32_t b = lookup(a.strategy, a.table, a.min_key, a.slots,
+                       load_key(a.keys, a.key_type, row));
+    a.out_probe[pos] = row;
+    a.out_build[pos] = (uint32_t)b;
+    ++pos;
   }
-  if (tile == gridDim.x - 1 && tid == 0) *a.out_count = s_prefix + total;
 }
 
 static uint32_t grid_rows(uint64_t n) {
@@ -299,14 +299,10 @@ int32_t mtl_join_probe(int32_t strategy, const void* table, uint64_t table_bytes
   MTL_TRY(resolve_stream(s, &s));
   std::lock_guard<std::mutex> g(s->mu);
   s->arena.reset();
-  uint32_t tiles = (uint32_t)ceil_div(capacity ? capacity : 1, PTILE);
-  void *status, *misc;
-  MTL_TRY(s->arena.alloc((uint64_t)tiles * 8, &status));
-  MTL_TRY(s->arena.alloc(256, &misc));
-  ZeroList zl;
-  zl.add((uint32_t*)status, (uint64_t)tiles * 2);
-  zl.add((uint32_t*)misc, 64);
-  MTL_TRY(launch_zero(zl, s->stream));
+  uint32_t tiles = filter_tiles(capacity);
+  void *bm, *counts;
+  MTL_TRY(s->arena.alloc((uint64_t)tiles * kTileRows / 8, &bm));
+  MTL_TRY(s->arena.alloc((uint64_t)tiles * 4, &counts));
   ProbeArgs a{};
   a.strategy = strategy;
   a.table = table;
@@ -317,15 +313,17 @@ int32_t mtl_join_probe(int32_t strategy, const void* table, uint64_t table_bytes
   a.sel = sel;
   a.count = sel ? count : nullptr;
   a.cap_rows = capacity;
+  a.match_bm = (uint64_t*)bm;
+  a.tile_counts = (uint32_t*)counts;
   a.out_probe = out_probe;
   a.out_build = out_build;
-  a.out_count = out_count;
-  a.status = (uint64_t*)status;
-  a.ctr = (uint32_t*)misc;
-  a.err = (uint32_t*)misc + 1;
   {
     ProfScope ps(s, "k_probe");
-    hipLaunchKernelGGL(k_probe, dim3(tiles), dim3(PBLOCK), 0, s->stream, a);
+    hipLaunchKernelGGL(k_probe_bits, dim3(tiles), dim3(FBLOCK), 0, s->stream, a);
+    MTL_HIP(hipGetLastError());
+    scan_counts((uint32_t*)counts, tiles, out_count, s->stream);
+    MTL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_probe_expand, dim3(tiles), dim3(FBLOCK), 0, s->stream, a);
   }
   MTL_HIP(hipGetLastError());
   return MTL_OK;

@@ -8,14 +8,16 @@
 // join_loop.rs:27-332, aggregate.rs:19-296, materialize.rs:19-80.
 //
 // MI355X plan (all HBM-bound integer work, no MFMA):
-//   1. k_mark_dim : filter D, set bit (d_key - min) in a dimension bitmap
+//   1. k_filter   : (mark-only) filter D, set bit (d_key - min) in a bitmap
 //                   (1 bit per key: 188 KB for 1.5M keys, L2 resident).
-//   2. k_filter   : filter P + probe the D bitmap with p_fk, stable
-//                   compaction of qualifying P rows, and set bit
-//                   (p_key - min) in the parent bitmap.
+//   2. k_filter   : filter P + probe the D bitmap with p_fk, write the pass
+//                   bits (1 per P row) and set bit (p_key - min) in the
+//                   parent bitmap.  No compaction: nothing downstream needs
+//                   a selection vector of P.
 //   3. k_rank     : exclusive popcount prefix over the parent bitmap words:
 //                   rank(s) = dense accumulator index of key slot s.
-//   4. k_map      : row_of[rank] = P row, clear the accumulators.
+//   4. k_map      : walk the pass bits: row_of[rank] = P row; clear the
+//                   accumulators.
 //   5. k_fact     : stream F's foreign key, probe the parent bitmap, and only
 //                   for hits read the filter / expression columns; add the
 //                   exact int64 (or f64) value to acc[rank] with an atomic.
@@ -26,57 +28,36 @@
 // for 15M keys, so every probe of step 5 hits L2.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "device.hpp"
 #include "filter.hpp"
 #include "mtl_internal.hpp"
 #include "prof.hpp"
+#include "rowtile.hpp"
 #include "topk.hpp"
 
 namespace mtl {
 using namespace dev;
 
-struct DimArgs {
-  mtl_pred preds[MTL_MAX_PREDS];
-  uint32_t n_preds;
-  uint64_t n;
-  const void* key;
-  int32_t key_type;
-  int64_t min;
-  uint64_t span;
-  uint64_t* bm;
-  uint32_t* err;
-};
-
 // err bits: 1 look-back spin cap, 2 duplicate parent key, 4 duplicate dim key,
 // 8 key outside the stated [min, max], 16 expression overflow, 32 sum overflow
-__global__ void k_mark_dim(DimArgs a) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    bool ok = true;
-    for (uint32_t p = 0; p < a.n_preds && ok; ++p) ok = eval_pred_row(a.preds[p], i);
-    if (!ok) continue;
-    uint64_t s = key_off(load_key(a.key, a.key_type, i), a.min);
-    if (s >= a.span) {
-      atomicOr(a.err, 8u);
-      continue;
-    }
-    unsigned long long bit = 1ull << (s & 63);
-    unsigned long long old = atomicOr((unsigned long long*)&a.bm[s >> 6], bit);
-    if (old & bit) atomicOr(a.err, 4u);
-  }
-}
 
 // Exclusive popcount prefix over bitmap words (decoupled look-back).
 constexpr int RBLOCK = 256;
 constexpr int RPER = 8;
 constexpr int RTILE = RBLOCK * RPER;
 
+// Also writes the parent bitmap's total popcount (*par_total, last tile)
+// and adds the dimension bitmap's popcount into *dim_pop (grid-stride), so
+// k_map can compare them with the marked-row counts (duplicate keys).
 __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm,
                                                  uint64_t words,
                                                  uint32_t* __restrict__ rank,
                                                  uint64_t* status, uint32_t* ctr,
-                                                 uint32_t* err) {
+                                                 uint32_t* err, uint32_t* par_total,
+                                                 const uint64_t* __restrict__ dim_bm,
+                                                 uint64_t dim_words, uint32_t* dim_pop) {
   __shared__ uint32_t s_tile, s_prefix;
   __shared__ uint32_t s_wave[RBLOCK / WAVE];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
@@ -111,6 +92,15 @@ __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm
     if (w0 + i < words) rank[w0 + i] = run;
     run += c[i];
   }
+  if (tile == gridDim.x - 1 && tid == 0) *par_total = s_prefix + total;
+  if (dim_bm) {
+    uint32_t dp = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * RBLOCK + tid; i < dim_words;
+         i += (uint64_t)gridDim.x * RBLOCK)
+      dp += (uint32_t)__popcll(dim_bm[i]);
+    dp = wave_sum_u32(dp);
+    if (lane == 0 && dp) atomicAdd(dim_pop, dp);
+  }
 }
 
 __device__ __forceinline__ uint32_t rank_of(const uint64_t* bm, const uint32_t* rank,
@@ -121,8 +111,9 @@ __device__ __forceinline__ uint32_t rank_of(const uint64_t* bm, const uint32_t* 
 }
 
 struct MapArgs {
-  const uint32_t* prow;
-  const uint32_t* count;
+  const uint64_t* sel_bm;  // parent pass bits, 1 per P row
+  uint64_t n_rows;
+  const uint32_t* count;   // marked (qualifying) P rows
   const void* key;
   int32_t key_type;
   int64_t min;
@@ -132,23 +123,40 @@ struct MapArgs {
   uint32_t* row_of;
   uint64_t* acc;
   uint32_t* cnt;
+  const uint32_t* par_total;  // parent bitmap popcount = accumulator slots
+  const uint32_t* dim_pop;    // dimension bitmap popcount (NULL: no dim)
+  const uint32_t* dim_count;  // marked dimension rows
+  uint32_t* err;
 };
 
-// Slots r < (distinct qualifying keys) <= n get their P row.  Accumulators
-// are cleared by position i < n, which covers every r even when duplicate
-// keys (flagged by the parent pass) make the distinct count smaller than n;
-// slots without fact rows keep cnt == 0 and are never read by the top-K.
+// Every qualifying P row (a set pass bit) writes row_of[rank(key)].  Slots
+// r < par_total are exactly the set bits of the parent bitmap, each set by
+// at least one qualifying row, so every slot gets its row; the accumulators
+// are cleared by index.  A repeated key (par_total < count) is flagged.
+// Thread t owns 16 consecutive rows (one 16-bit piece of the pass bits).
 __global__ void k_map(MapArgs a) {
-  const uint32_t n = *a.count;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
+  const uint32_t M = *a.par_total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (M != *a.count) atomicOr(a.err, 2u);  // repeated parent key
+    if (a.dim_pop && *a.dim_pop != *a.dim_count) atomicOr(a.err, 4u);
+  }
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i = t0; i < M; i += nthreads) {
     a.acc[i] = 0;
     a.cnt[i] = 0;
-    uint32_t row = a.prow[i];
-    uint64_t s = key_off(load_key(a.key, a.key_type, row), a.min);
-    if (s >= a.span) continue;  // flagged by the parent pass
-    uint32_t r = rank_of(a.bm, a.rank, s);
-    a.row_of[r] = row;
+  }
+  for (uint64_t g = t0; g * 16 < a.n_rows; g += nthreads) {
+    const uint64_t r0 = g * 16;
+    uint32_t m = (uint32_t)(a.sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
+    while (m) {
+      int i = __ffs(m) - 1;
+      m &= m - 1;
+      uint64_t row = r0 + (uint64_t)i;
+      uint64_t s = key_off(load_key(a.key, a.key_type, row), a.min);
+      if (s >= a.span) continue;  // flagged by the parent pass
+      a.row_of[rank_of(a.bm, a.rank, s)] = (uint32_t)row;
+    }
   }
 }
 
@@ -220,69 +228,208 @@ struct FactArgs {
   uint32_t* cnt;
   uint32_t* err;
   uint32_t fk_vec;  // 1: fk is 16-byte aligned
+  unsigned long long* maxabs;  // max |value| over accumulated rows
 };
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ void fact_row(const FactArgs& a, uint64_t row, int64_t key,
-                                         bool& ovf, bool& sovf) {
-  uint64_t s = key_off(key, a.min);
-  if (s >= a.span) return;
-  uint64_t w = a.bm[s >> 6];
-  if (!((w >> (s & 63)) & 1ull)) return;
-  for (uint32_t p = 0; p < a.n_preds; ++p)
-    if (!eval_pred_row(a.preds[p], row)) return;
-  uint64_t below = (s & 63) ? (w & ((1ull << (s & 63)) - 1)) : 0ull;
-  uint32_t r = a.rank[s >> 6] + (uint32_t)__popcll(below);
-  if (a.expr.is_double) {
-    double v = expr_f64(a.expr, row);
-    atomicAdd((double*)&a.acc[r], v);
-  } else {
-    int64_t v = expr_i64(a.expr, row, ovf);
-    int64_t old = (int64_t)atomicAdd((unsigned long long*)&a.acc[r],
-                                     (unsigned long long)v);
-    int64_t t;
-    sovf |= __builtin_add_overflow(old, v, &t);
+// Expression over preloaded column values (one row).
+__device__ __forceinline__ int64_t expr_i64_vals(const mtl_expr& e, const int64_t* cv,
+                                                 bool& ovf) {
+  int64_t st[8];
+  int sp = 0;
+  for (uint32_t i = 0; i < e.n_ops; ++i) {
+    int op = e.op[i];
+    if (op == MTL_EXPR_COL) {
+      st[sp++] = cv[e.arg[i]];
+    } else if (op == MTL_EXPR_CONST) {
+      st[sp++] = e.arg[i];
+    } else {
+      int64_t b = st[--sp], a = st[--sp], r;
+      bool o;
+      if (op == MTL_EXPR_ADD) o = __builtin_add_overflow(a, b, &r);
+      else if (op == MTL_EXPR_SUB) o = __builtin_sub_overflow(a, b, &r);
+      else o = __builtin_mul_overflow(a, b, &r);
+      ovf |= o;
+      st[sp++] = r;
+    }
   }
-  atomicAdd(&a.cnt[r], 1u);
+  return sp ? st[sp - 1] : 0;
 }
 
-__global__ __launch_bounds__(256) void k_fact(FactArgs a) {
-  bool ovf = false, sovf = false;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 4;
-  for (uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-       base < a.n; base += stride) {
-    int64_t key[4];
-    if (a.fk_vec && base + 3 < a.n && a.fk_type != MTL_BIGINT &&
-        a.fk_type != MTL_TIMESTAMP && a.fk_type != MTL_DECIMAL &&
-        a.fk_type != MTL_DICT8) {
-      u32x4 u = __builtin_nontemporal_load((const u32x4*)((const int32_t*)a.fk + base));
-      key[0] = (int32_t)u.x;
-      key[1] = (int32_t)u.y;
-      key[2] = (int32_t)u.z;
-      key[3] = (int32_t)u.w;
-    } else if (a.fk_vec && base + 3 < a.n &&
-               (a.fk_type == MTL_BIGINT || a.fk_type == MTL_TIMESTAMP ||
-                a.fk_type == MTL_DECIMAL)) {
-      u64x2 u0 = __builtin_nontemporal_load((const u64x2*)((const int64_t*)a.fk + base));
-      u64x2 u1 =
-          __builtin_nontemporal_load((const u64x2*)((const int64_t*)a.fk + base + 2));
-      key[0] = (int64_t)u0.x;
-      key[1] = (int64_t)u0.y;
-      key[2] = (int64_t)u1.x;
-      key[3] = (int64_t)u1.y;
+__device__ __forceinline__ double expr_f64_vals(const mtl_expr& e, const double* cv) {
+  double st[8];
+  int sp = 0;
+  for (uint32_t i = 0; i < e.n_ops; ++i) {
+    int op = e.op[i];
+    if (op == MTL_EXPR_COL) {
+      st[sp++] = cv[e.arg[i]];
+    } else if (op == MTL_EXPR_CONST) {
+      st[sp++] = __longlong_as_double(e.arg[i]);
     } else {
+      double b = st[--sp], a = st[--sp];
+      st[sp++] = op == MTL_EXPR_ADD ? a + b : (op == MTL_EXPR_SUB ? a - b : a * b);
+    }
+  }
+  return sp ? st[sp - 1] : 0.0;
+}
+
+// Accumulator slots per block kept in LDS (48 KB with the counts).
+constexpr int kAccWin = 4096;
+
+// Accumulate the (up to 4) hits of one lane-step: the bitmap words, ranks
+// and expression columns of all hits are loaded before any is used, and
+// the atomics return nothing (an int64 SUM overflow is excluded afterwards
+// from max|value| x count, checked by the top-K kernel).
+template <bool DBL>
+__device__ __forceinline__ void fact_hits(const FactArgs& a, uint32_t m, const uint32_t (&sj)[4],
+                                          uint64_t row0, bool& ovf, uint64_t& mx,
+                                          uint64_t* wacc, uint32_t* wcnt, uint32_t wbase) {
+  uint64_t wv[4];
+  uint32_t rk[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        key[v] = base + v < a.n ? load_key(a.fk, a.fk_type, base + v) : 0;
+  for (int v = 0; v < 4; ++v)
+    if ((m >> v) & 1u) {
+      wv[v] = a.bm[sj[v] >> 6];
+      rk[v] = a.rank[sj[v] >> 6];
+    }
+  using V = typename std::conditional<DBL, double, int64_t>::type;
+  V cv[4][MTL_EXPR_MAX_COLS];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    if ((m >> v) & 1u) {
+#pragma unroll
+      for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
+        if ((uint32_t)c < a.expr.n_cols) {
+          if constexpr (DBL) cv[v][c] = load_f64(a.expr.col[c], a.expr.col_type[c], row0 + v);
+          else cv[v][c] = load_key(a.expr.col[c], a.expr.col_type[c], row0 + v);
+        }
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if (base + v < a.n) fact_row(a, base + v, key[v], ovf, sovf);
+  for (int v = 0; v < 4; ++v)
+    if ((m >> v) & 1u) {
+      const uint32_t sv = sj[v];
+      const uint64_t below = (sv & 63) ? (wv[v] & ((1ull << (sv & 63)) - 1)) : 0ull;
+      const uint32_t r = rk[v] + (uint32_t)__popcll(below);
+      // slots inside the block's window accumulate in LDS, others globally
+      const uint32_t wi = r - wbase;
+      const bool local = wi < (uint32_t)kAccWin;
+      if constexpr (DBL) {
+        double val = expr_f64_vals(a.expr, cv[v]);
+        if (local) unsafeAtomicAdd((double*)&wacc[wi], val);
+        else unsafeAtomicAdd((double*)&a.acc[r], val);
+      } else {
+        int64_t val = expr_i64_vals(a.expr, cv[v], ovf);
+        uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
+        mx = av > mx ? av : mx;
+        atomicAdd(local ? (unsigned long long*)&wacc[wi] : (unsigned long long*)&a.acc[r],
+                  (unsigned long long)val);
+      }
+      atomicAdd(local ? &wcnt[wi] : &a.cnt[r], 1u);
+    }
+}
+
+// One tile = 4096 fact rows per block, 16 per lane (rowtile.hpp layout).
+// Phase 1 loads all 16 foreign keys (16-byte vector loads) and then all 16
+// parent-bitmap words before testing any of them; phase 2 evaluates the
+// fact predicates for the hits only (their loads again issued together);
+// phase 3 evaluates the expression and accumulates.  No load waits on
+// another load of the same phase.
+template <typename KT>
+__device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32_t (&s)[FJ][4]) {
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    KT x[4];
+    load4<KT>((const KT*)a.fk, k0 + (uint64_t)j * 256, a.n, a.fk_vec != 0, x);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      uint64_t o = key_off((int64_t)x[v], a.min);
+      // 0xFFFFFFFF marks "outside the parent key range" (span < 2^32)
+      s[j][v] = o < a.span ? (uint32_t)o : 0xFFFFFFFFu;
+    }
   }
-  uint32_t e = (ovf ? 16u : 0u) | (sovf ? 32u : 0u);
-  if (__ballot(e != 0) && e) atomicOr(a.err, e);
+}
+
+// Each block owns a contiguous range of tiles.  Accumulation goes to an LDS
+// window of kAccWin slots starting at the rank of the block's first foreign
+// key: when the fact table is clustered by its foreign key (lines stored by
+// order), a block's hits fall into a narrow slot range, so nearly every
+// update is an LDS atomic and the block flushes each touched slot once with
+// one coalesced global atomic.  Hits outside the window (unclustered data)
+// fall back to global atomics; the result is the same either way.
+template <bool DBL>
+__global__ __launch_bounds__(FBLOCK) void k_fact(FactArgs a) {
+  __shared__ uint64_t wacc[kAccWin];
+  __shared__ uint32_t wcnt[kAccWin];
+  __shared__ uint32_t s_wbase;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  bool ovf = false;
+  uint64_t mx = 0;
+  const uint64_t tiles = (a.n + FTILE - 1) / FTILE;
+  const uint64_t tpb = (tiles + gridDim.x - 1) / gridDim.x;
+  const uint64_t tb = (uint64_t)blockIdx.x * tpb;
+  const uint64_t te = tb + tpb < tiles ? tb + tpb : tiles;
+  if (tb >= te) return;
+  for (int i = threadIdx.x; i < kAccWin; i += FBLOCK) {
+    wacc[i] = 0;
+    wcnt[i] = 0;
+  }
+  if (threadIdx.x == 0) {
+    int64_t k = load_key(a.fk, a.fk_type, tb * FTILE);
+    uint64_t s0 = key_off(k, a.min);
+    if (s0 >= a.span) s0 = k < a.min ? 0 : a.span - 1;
+    uint64_t w = a.bm[s0 >> 6];
+    uint64_t below = (s0 & 63) ? (w & ((1ull << (s0 & 63)) - 1)) : 0ull;
+    s_wbase = a.rank[s0 >> 6] + (uint32_t)__popcll(below);
+  }
+  __syncthreads();
+  const uint32_t wbase = s_wbase;
+  for (uint64_t t = tb; t < te; ++t) {
+    const uint64_t k0 = t * FTILE + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
+    uint32_t s[FJ][4];
+    switch (a.fk_type) {
+      case MTL_BIGINT:
+      case MTL_TIMESTAMP:
+      case MTL_DECIMAL: fact_keys<int64_t>(a, k0, s); break;
+      case MTL_DICT8: fact_keys<uint8_t>(a, k0, s); break;
+      default: fact_keys<int32_t>(a, k0, s); break;
+    }
+    uint32_t hits = 0;
+    {
+      uint64_t w[FJ][4];
+#pragma unroll
+      for (int j = 0; j < FJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[j][v] = a.bm[s[j][v] != 0xFFFFFFFFu ? (s[j][v] >> 6) : 0];
+#pragma unroll
+      for (int j = 0; j < FJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          bool in = k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu;
+          if (in && ((w[j][v] >> (s[j][v] & 63)) & 1ull)) hits |= 1u << (4 * j + v);
+        }
+    }
+    if (!hits) continue;
+    for (uint32_t p = 0; p < a.n_preds && hits; ++p)
+      with_pred(a.preds[p], [&](auto col, auto cmp) { pred_rows(col, k0, cmp, hits); });
+    if (!hits) continue;
+#pragma unroll
+    for (int j = 0; j < FJ; ++j) {
+      uint32_t m = (hits >> (4 * j)) & 0xFu;
+      if (!m) continue;
+      fact_hits<DBL>(a, m, s[j], k0 + (uint64_t)j * 256, ovf, mx, wacc, wcnt, wbase);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kAccWin; i += FBLOCK) {
+    uint32_t c = wcnt[i];
+    if (!c) continue;
+    if constexpr (DBL)
+      unsafeAtomicAdd((double*)&a.acc[wbase + i], __longlong_as_double((long long)wacc[i]));
+    else
+      atomicAdd((unsigned long long*)&a.acc[wbase + i], (unsigned long long)wacc[i]);
+    atomicAdd(&a.cnt[wbase + i], c);
+  }
+  if (ovf) atomicOr(a.err, 16u);
+  if (mx) atomicMax(a.maxabs, (unsigned long long)mx);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,7 +453,8 @@ struct TopkArgs {
   uint32_t* done;
   mtl_star_row* out;
   uint32_t* out_n;  // [0] = rows, [1] = error bits copied from err
-  const uint32_t* err;
+  uint32_t* err;
+  const unsigned long long* maxabs;
 };
 
 __device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t r, uint64_t k[4]) {
@@ -326,68 +474,49 @@ __device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t r, uint64_
 
 __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
   __shared__ TopkLds s;
-  __shared__ bool s_last;
   topk_init(s);
   const uint32_t m = *a.m_count;
   const uint32_t chunk = (m + gridDim.x - 1) / gridDim.x;
   const uint32_t start = blockIdx.x * chunk;
   const uint32_t end = min(m, start + chunk);
-  for (uint32_t base = start; base < end; base += TOPK_ROUND) {
-    for (int e = 0; e < 2; ++e) {
+  constexpr int E = 8;  // entries per thread per pass, loaded together
+  for (uint32_t base = start; base < end; base += TOPK_BLOCK * E) {
+    uint32_t cv[E];
+    uint64_t av[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
       uint32_t r = base + e * TOPK_BLOCK + threadIdx.x;
-      if (r < end && a.cnt[r] > 0) {
-        uint64_t v = a.acc[r];
-        uint64_t k0 = a.is_double ? ~enc_f64(__longlong_as_double((long long)v))
-                                  : ~enc_i64((int64_t)v);
+      cv[e] = r < end ? a.cnt[r] : 0u;
+      av[e] = r < end ? a.acc[r] : 0ull;
+    }
+    if (!a.is_double) {
+      // |SUM| <= count x max|value|: flag groups where that bound exceeds
+      // int64 (the int64 accumulator could have wrapped)
+      const unsigned long long mx = *a.maxabs;
+      bool big = false;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        unsigned long long prod;
+        big |= __builtin_mul_overflow((unsigned long long)cv[e], mx, &prod) ||
+               prod > 0x7FFFFFFFFFFFFFFFull;
+      }
+      if (big) atomicOr(a.err, 32u);
+    }
+    for (int e = 0; e < E; ++e) {
+      if (cv[e] != 0) {
+        uint64_t k0 = a.is_double ? ~enc_f64(__longlong_as_double((long long)av[e]))
+                                  : ~enc_i64((int64_t)av[e]);
         if (k0 <= s.thr) {
+          uint32_t r = base + e * TOPK_BLOCK + threadIdx.x;
           uint64_t k[4];
           star_keys(a, r, k);
           topk_push(s, k[0], k[1], k[2], k[3], r);
         }
       }
-    }
-    topk_maybe_compact(s, a.k);
-  }
-  __syncthreads();
-  uint32_t n = s.n;
-  __syncthreads();
-  topk_sort(s, n);
-  topk_truncate(s, a.k);
-  const uint32_t kb = s.n;
-  for (uint32_t i = threadIdx.x; i < kb; i += blockDim.x) {
-    uint64_t o = (uint64_t)blockIdx.x * a.k + i;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) a.ck[f][o] = s.k[f][i];
-    a.cid[o] = s.id[i];
-  }
-  if (threadIdx.x == 0) a.cn[blockIdx.x] = kb;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // merge: the same admission procedure over all block results
-  topk_init(s);
-  for (uint32_t b = 0; b < gridDim.x; ++b) {
-    uint32_t nb = __hip_atomic_load(&a.cn[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t base = 0; base < nb; base += TOPK_ROUND) {
-      for (int e = 0; e < 2; ++e) {
-        uint32_t i = base + e * TOPK_BLOCK + threadIdx.x;
-        if (i < nb) {
-          uint64_t o = (uint64_t)b * a.k + i;
-          uint64_t k0 = a.ck[0][o];
-          if (k0 <= s.thr) topk_push(s, k0, a.ck[1][o], a.ck[2][o], a.ck[3][o], a.cid[o]);
-        }
-      }
       topk_maybe_compact(s, a.k);
     }
   }
-  __syncthreads();
-  n = s.n;
-  __syncthreads();
-  topk_sort(s, n);
-  topk_truncate(s, a.k);
+  if (!topk_publish_elect_merge(s, a.ck, a.cid, a.cn, a.done, a.k)) return;
   for (uint32_t i = threadIdx.x; i < s.n; i += blockDim.x) {
     uint32_t r = s.id[i];
     mtl_star_row row;
@@ -480,14 +609,14 @@ static int32_t star_validate(const mtl_star_desc* d, StarPlan* sp) {
   MTL_TRY(check_expr(d->agg));
   if (d->p_key_max < d->p_key_min) return set_error(MTL_ERR_INVALID_ARGUMENT, "parent key range");
   sp->pspan = key_off(d->p_key_max, d->p_key_min) + 1;
-  if (sp->pspan == 0 || sp->pspan > (1ull << 36))
+  if (sp->pspan == 0 || sp->pspan >= (1ull << 32))
     return set_error(MTL_UNSUPPORTED, "parent key span %llu too wide for a bitmap",
                      (unsigned long long)sp->pspan);
   sp->pwords = ceil_div(sp->pspan, 64);
   if (d->d_rows) {
     if (d->d_key_max < d->d_key_min) return set_error(MTL_ERR_INVALID_ARGUMENT, "dim key range");
     sp->dspan = key_off(d->d_key_max, d->d_key_min) + 1;
-    if (sp->dspan == 0 || sp->dspan > (1ull << 36))
+    if (sp->dspan == 0 || sp->dspan >= (1ull << 32))
       return set_error(MTL_UNSUPPORTED, "dimension key span too wide for a bitmap");
     sp->dwords = ceil_div(sp->dspan, 64);
   }
@@ -503,20 +632,21 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   hipStream_t st = s->stream;
   Arena& ar = s->arena;
   const uint64_t prows = d->p_rows;
-  void *dim_bm = nullptr, *par_bm, *par_rank, *prow, *row_of, *acc, *cnt, *misc;
-  void *st_p, *st_r, *ck[4], *cid, *cn;
+  void *dim_bm = nullptr, *par_bm, *par_rank, *pbits, *row_of, *acc, *cnt, *misc;
+  void *st_r, *ck[4], *cid, *cn;
   if (d->d_rows) MTL_TRY(ar.alloc(sp.dwords * 8, &dim_bm));
   MTL_TRY(ar.alloc(sp.pwords * 8, &par_bm));
   MTL_TRY(ar.alloc(sp.pwords * 4, &par_rank));
-  MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &prow));
+  uint32_t ptiles = filter_tiles(prows);
+  MTL_TRY(ar.alloc((uint64_t)ptiles * kTileRows / 8, &pbits));
   MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &row_of));
   MTL_TRY(ar.alloc((prows ? prows : 1) * 8, &acc));
   MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &cnt));
-  MTL_TRY(ar.alloc(256, &misc));  // [0]=count_p [1]=tile ctr P [2]=tile ctr rank
-                                  // [3]=err [4]=done
-  uint32_t ptiles = filter_tiles(prows);
+  // misc words: [0] qualifying P rows, [2] rank ticket, [3] error bits,
+  // [4] top-K done counter, [6] parent bitmap popcount, [7] dimension bitmap
+  // popcount, [8] qualifying D rows, [10..11] max |value| (u64)
+  MTL_TRY(ar.alloc(256, &misc));
   uint32_t rtiles = (uint32_t)ceil_div(sp.pwords, RTILE);
-  MTL_TRY(ar.alloc((uint64_t)ptiles * 8, &st_p));
   MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(rtiles, 1) * 8, &st_r));
   for (int f = 0; f < 4; ++f) MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * d->k * 8, &ck[f]));
   MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * d->k * 4, &cid));
@@ -527,26 +657,25 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   if (dim_bm) zl.add((uint32_t*)dim_bm, sp.dwords * 2);
   zl.add((uint32_t*)par_bm, sp.pwords * 2);
   zl.add((uint32_t*)misc, 64);
-  zl.add((uint32_t*)st_p, (uint64_t)ptiles * 2);
   zl.add((uint32_t*)st_r, (uint64_t)std::max<uint32_t>(rtiles, 1) * 2);
   MTL_TRY(launch_zero(zl, st));
 
   // 1. dimension bitmap
   if (d->d_rows) {
-    DimArgs da{};
-    for (uint32_t p = 0; p < d->d_n_preds; ++p) da.preds[p] = d->d_preds[p];
-    da.n_preds = d->d_n_preds;
-    da.n = d->d_rows;
-    da.key = d->d_key;
-    da.key_type = d->d_key_type;
-    da.min = d->d_key_min;
-    da.span = sp.dspan;
-    da.bm = (uint64_t*)dim_bm;
-    da.err = mu + 3;
-    uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(d->d_rows, 256 * 4), 4096);
+    // mark-only filter pass: vectorised predicates, combined bitmap atomics
+    FilterArgs fa{};
+    for (uint32_t p = 0; p < d->d_n_preds; ++p) fa.preds[p] = d->d_preds[p];
+    fa.n_preds = d->d_n_preds;
+    fa.n_rows = d->d_rows;
+    fa.err = mu + 3;
+    fa.mark_bm = (uint64_t*)dim_bm;
+    fa.mark_key = d->d_key;
+    fa.mark_key_type = d->d_key_type;
+    fa.mark_min = d->d_key_min;
+    fa.mark_span = sp.dspan;
+    fa.mark_count = mu + 8;
     ProfScope ps(s, "k_mark_dim");
-    hipLaunchKernelGGL(k_mark_dim, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, da);
-    MTL_HIP(hipGetLastError());
+    MTL_TRY(launch_filter(fa, d->d_rows, st));
   }
   // 2. parent filter + semi-join probe + compaction + parent bitmap
   {
@@ -554,10 +683,8 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     for (uint32_t p = 0; p < d->p_n_preds; ++p) fa.preds[p] = d->p_preds[p];
     fa.n_preds = d->p_n_preds;
     fa.n_rows = prows;
-    fa.out_sel = (uint32_t*)prow;
-    fa.out_count = mu + 0;
-    fa.status = (uint64_t*)st_p;
-    fa.tile_ctr = mu + 1;
+    fa.sel_bm = (uint64_t*)pbits;
+    fa.mark_count = mu + 0;
     fa.err = mu + 3;
     if (d->d_rows) {
       fa.probe_bm = (const uint64_t*)dim_bm;
@@ -579,13 +706,15 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ProfScope ps(s, "k_rank");
     hipLaunchKernelGGL(k_rank, dim3(std::max<uint32_t>(rtiles, 1)), dim3(RBLOCK), 0, st,
                        (const uint64_t*)par_bm, sp.pwords, (uint32_t*)par_rank,
-                       (uint64_t*)st_r, mu + 2, mu + 3);
+                       (uint64_t*)st_r, mu + 2, mu + 3, mu + 6,
+                       (const uint64_t*)dim_bm, d->d_rows ? sp.dwords : 0, mu + 7);
   }
   MTL_HIP(hipGetLastError());
   // 4. map accumulator slots to P rows, clear accumulators
   {
     MapArgs ma{};
-    ma.prow = (const uint32_t*)prow;
+    ma.sel_bm = (const uint64_t*)pbits;
+    ma.n_rows = prows;
     ma.count = mu + 0;
     ma.key = d->p_key;
     ma.key_type = d->p_key_type;
@@ -596,7 +725,11 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ma.row_of = (uint32_t*)row_of;
     ma.acc = (uint64_t*)acc;
     ma.cnt = (uint32_t*)cnt;
-    uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 4), 2048);
+    ma.par_total = mu + 6;
+    ma.dim_pop = d->d_rows ? mu + 7 : nullptr;
+    ma.dim_count = mu + 8;
+    ma.err = mu + 3;
+    uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
     ProfScope ps(s, "k_map");
     hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
     MTL_HIP(hipGetLastError());
@@ -618,17 +751,22 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.cnt = (uint32_t*)cnt;
     fa.err = mu + 3;
     fa.fk_vec = ((uintptr_t)d->f_fk % 16) == 0;
-    uint64_t g = ceil_div(d->f_rows, 256 * 4 * 4);
-    uint64_t gmax = (uint64_t)cu_count() * 32;
+    fa.maxabs = (unsigned long long*)(mu + 10);
+    uint64_t g = ceil_div(d->f_rows, FTILE);
+    uint64_t gmax = (uint64_t)cu_count() * 3;  // persistent: 3 blocks per CU
     ProfScope ps(s, "k_fact");
-    hipLaunchKernelGGL(k_fact, dim3((uint32_t)std::max<uint64_t>(1, std::min(g, gmax))),
+    if (d->agg.is_double)
+      hipLaunchKernelGGL(k_fact<true>, dim3((uint32_t)std::max<uint64_t>(1, std::min(g, gmax))),
+                         dim3(256), 0, st, fa);
+    else
+      hipLaunchKernelGGL(k_fact<false>, dim3((uint32_t)std::max<uint64_t>(1, std::min(g, gmax))),
                        dim3(256), 0, st, fa);
     MTL_HIP(hipGetLastError());
   }
   // 6. top-K + output rows
   {
     TopkArgs ta{};
-    ta.m_count = mu + 0;
+    ta.m_count = mu + 6;
     ta.acc = (const uint64_t*)acc;
     ta.cnt = (const uint32_t*)cnt;
     ta.row_of = (const uint32_t*)row_of;
@@ -647,6 +785,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ta.out = out;
     ta.out_n = out_n;
     ta.err = mu + 3;
+    ta.maxabs = (const unsigned long long*)(mu + 10);
     ProfScope ps(s, "k_star_topk");
     hipLaunchKernelGGL(k_star_topk, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
     MTL_HIP(hipGetLastError());

@@ -18,8 +18,8 @@ namespace dev {
 
 constexpr int TOPK_BLOCK = 256;
 constexpr int TOPK_CAP = 1024;          // LDS candidate buffer entries
-constexpr int TOPK_ROUND = 2 * TOPK_BLOCK;  // entries admitted per round
-constexpr int TOPK_MAX_K = TOPK_CAP - TOPK_ROUND;  // 512
+constexpr int TOPK_ROUND = TOPK_BLOCK;  // entries admitted per round (1 per thread)
+constexpr int TOPK_MAX_K = TOPK_CAP - 2 * TOPK_ROUND;  // 512
 
 struct TopkLds {
   uint64_t k[4][TOPK_CAP];
@@ -101,15 +101,71 @@ __device__ __forceinline__ void topk_push(TopkLds& s, uint64_t k0, uint64_t k1,
   s.id[pos] = id;
 }
 
-// Called by all threads after each admission round.
+// Called by all threads after each admission round (<= TOPK_ROUND pushes).
+// Compacts as soon as the buffer holds more than max(TOPK_ROUND, k)
+// entries, so the first sort (which sets the threshold) is small; after it
+// most entries fail the threshold test and are never pushed.  Bound: after
+// a compaction n <= k, a round adds <= TOPK_ROUND, so n never exceeds
+// max(TOPK_ROUND, k) + TOPK_ROUND <= TOPK_CAP.
 __device__ __forceinline__ void topk_maybe_compact(TopkLds& s, uint32_t k) {
   __syncthreads();
-  if (s.n > (uint32_t)(TOPK_CAP - TOPK_ROUND)) {
+  const uint32_t limit = k > (uint32_t)TOPK_ROUND ? k : (uint32_t)TOPK_ROUND;
+  if (s.n > limit) {
     uint32_t n = s.n;
     __syncthreads();
     topk_sort(s, n);
     topk_truncate(s, k);
   }
+}
+
+// Sort the block's candidates, publish its best k to slot blockIdx.x of the
+// global candidate arrays (ck[f][b*k + i], cid, cn[b]), and let the last
+// block to finish merge all published candidates into s.  Returns true
+// only in that last block, with s holding the final top-k in order.  The
+// merge reads every block's slots in parallel (flat index b*k + i).
+__device__ __forceinline__ bool topk_publish_elect_merge(TopkLds& s, uint64_t* const* ck,
+                                                         uint32_t* cid, uint32_t* cn,
+                                                         uint32_t* done, uint32_t k) {
+  __shared__ bool s_last;
+  __syncthreads();
+  uint32_t n = s.n;
+  __syncthreads();
+  topk_sort(s, n);
+  topk_truncate(s, k);
+  const uint32_t kb = s.n;
+  for (uint32_t i = threadIdx.x; i < kb; i += blockDim.x) {
+    uint64_t o = (uint64_t)blockIdx.x * k + i;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) ck[f][o] = s.k[f][i];
+    cid[o] = s.id[i];
+  }
+  if (threadIdx.x == 0) cn[blockIdx.x] = kb;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  topk_init(s);
+  const uint32_t total = gridDim.x * k;
+  for (uint32_t base = 0; base < total; base += TOPK_ROUND) {
+    uint32_t i = base + threadIdx.x;
+    if (i < total) {
+      uint32_t b = i / k, j = i - b * k;
+      uint32_t nb = __hip_atomic_load(&cn[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (j < nb) {
+        uint64_t k0 = ck[0][i];
+        if (k0 <= s.thr) topk_push(s, k0, ck[1][i], ck[2][i], ck[3][i], cid[i]);
+      }
+    }
+    topk_maybe_compact(s, k);
+  }
+  __syncthreads();
+  n = s.n;
+  __syncthreads();
+  topk_sort(s, n);
+  topk_truncate(s, k);
+  return true;
 }
 
 }  // namespace dev
