@@ -1,6 +1,6 @@
 // Device-side helpers for gfx950 (CDNA4, wave64): atomics with explicit
-// agent scope, wave reductions, the decoupled look-back tile scan used by
-// every stable compaction, and typed predicate evaluation.
+// agent scope, wave reductions, a decoupled look-back tile scan (used only
+// where the tile count is small), compiled predicates and key encodings.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -99,58 +99,53 @@ __device__ __forceinline__ uint32_t lookback_wave(uint64_t* status,
 }
 
 // ---------------------------------------------------------------------------
-// Predicates.  DuckDB orders floats totally: NaN equals NaN and is greater
-// than every other value (DuckDB's Equals/GreaterThan on float/double).
+// Compiled predicates.  Column values and the literal are mapped to one
+// order-preserving 64-bit encoding (signed integers: sign bit flipped;
+// floats: DuckDB's total order with NaN largest and -0 == 0), so a single
+// code path compares every type: pass = bit (value <, ==, > literal) of
+// opmask.  Keeping one path instead of one per (type, op) keeps the scan
+// kernels inside the instruction cache.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ bool cmp_op(T a, T b, int op) {
-  switch (op) {
-    case MTL_CMP_EQ: return a == b;
-    case MTL_CMP_LT: return a < b;
-    case MTL_CMP_LE: return a <= b;
-    case MTL_CMP_GT: return a > b;
-    default: return a >= b;
+enum : uint32_t { PK_I32 = 0, PK_I64 = 1, PK_U8 = 2, PK_F32 = 3, PK_F64 = 4 };
+
+struct PredC {
+  const void* col;
+  uint32_t kind;    // PK_*
+  uint32_t opmask;  // bit 0: pass if value < literal, bit 1: ==, bit 2: >
+  uint64_t lit;     // encoded literal
+};
+
+__host__ __device__ __forceinline__ uint64_t enc_int(int64_t v) {
+  return (uint64_t)v ^ 0x8000000000000000ull;
+}
+__host__ __device__ __forceinline__ uint64_t enc_dbl(double d) {
+  if (d != d) return ~0ull;  // NaN: largest, all NaNs equal
+  if (d == 0.0) d = 0.0;     // -0 == 0
+  uint64_t b = __builtin_bit_cast(uint64_t, d);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ uint32_t pred_width(uint32_t kind) {
+  return kind == PK_U8 ? 1u : ((kind == PK_I64 || kind == PK_F64) ? 8u : 4u);
+}
+// raw: the value's bytes zero-extended to 64 bits
+__device__ __forceinline__ uint64_t enc_raw(uint64_t raw, uint32_t kind) {
+  switch (kind) {
+    case PK_I32: return enc_int((int64_t)(int32_t)(uint32_t)raw);
+    case PK_I64: return enc_int((int64_t)raw);
+    case PK_U8: return enc_int((int64_t)(raw & 0xFFu));
+    case PK_F32: return enc_dbl((double)__builtin_bit_cast(float, (uint32_t)raw));
+    default: return enc_dbl(__builtin_bit_cast(double, raw));
   }
 }
-
-template <typename F>
-__device__ __forceinline__ bool cmp_float(F a, F b, int op) {
-  bool an = a != a, bn = b != b;
-  if (!an && !bn) return cmp_op<F>(a, b, op);
-  // total order with NaN as the largest value
-  int c = (an && bn) ? 0 : (an ? 1 : -1);
-  switch (op) {
-    case MTL_CMP_EQ: return c == 0;
-    case MTL_CMP_LT: return c < 0;
-    case MTL_CMP_LE: return c <= 0;
-    case MTL_CMP_GT: return c > 0;
-    default: return c >= 0;
-  }
+__device__ __forceinline__ bool pred_pass(uint64_t x, const PredC& p) {
+  uint32_t c = x < p.lit ? 0u : (x == p.lit ? 1u : 2u);
+  return (p.opmask >> c) & 1u;
 }
-
-// Evaluate one predicate for a row.
-__device__ __forceinline__ bool eval_pred_row(const mtl_pred& p, uint64_t row) {
-  switch (p.type) {
-    case MTL_INTEGER:
-    case MTL_DATE:
-    case MTL_DICT32:
-      return cmp_op<int32_t>(((const int32_t*)p.column)[row], (int32_t)p.scalar,
-                             p.op);
-    case MTL_BIGINT:
-    case MTL_TIMESTAMP:
-    case MTL_DECIMAL:
-      return cmp_op<int64_t>(((const int64_t*)p.column)[row], p.scalar, p.op);
-    case MTL_DICT8:
-      return cmp_op<int32_t>((int32_t)((const uint8_t*)p.column)[row],
-                             (int32_t)p.scalar, p.op);
-    case MTL_FLOAT:
-      return cmp_float<float>(((const float*)p.column)[row],
-                              (float)__longlong_as_double(p.scalar), p.op);
-    case MTL_DOUBLE:
-      return cmp_float<double>(((const double*)p.column)[row],
-                               __longlong_as_double(p.scalar), p.op);
-    default:
-      return false;
+__device__ __forceinline__ uint64_t load_raw(const void* col, uint32_t width, uint64_t row) {
+  switch (width) {
+    case 1: return ((const uint8_t*)col)[row];
+    case 4: return ((const uint32_t*)col)[row];
+    default: return ((const uint64_t*)col)[row];
   }
 }
 

@@ -46,6 +46,7 @@ int32_t launch_zero(const ZeroList& zl, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Evaluate all predicates (and the optional semi-join probe) for the 16 rows
 // of this lane; returns the pass bits (bit 4j+v).
+template <bool SEL>
 __device__ __forceinline__ uint32_t eval_lane(const FilterArgs& a, uint64_t n,
                                               uint64_t k0,
                                               uint32_t (&rid)[FJ][4]) {
@@ -55,7 +56,7 @@ __device__ __forceinline__ uint32_t eval_lane(const FilterArgs& a, uint64_t n,
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
-  if (a.in_sel) {
+  if (SEL) {
 #pragma unroll
     for (int j = 0; j < FJ; ++j) {
       uint64_t k = k0 + (uint64_t)j * 256;
@@ -80,20 +81,15 @@ __device__ __forceinline__ uint32_t eval_lane(const FilterArgs& a, uint64_t n,
 
   for (int p = 0; p < a.n_preds && bits; ++p) {
     const bool vec_ok = (a.vec_ok >> p) & 1u;
-    if (a.in_sel) {
-      with_pred(a.preds[p], [&](auto col, auto cmp) { pred_sel(col, rid, cmp, bits); });
-    } else {
-      with_pred(a.preds[p], [&](auto col, auto cmp) {
-        pred_dense(col, k0, n, vec_ok, cmp, bits);
-      });
-    }
+    if (SEL) pred_sel(a.preds[p], rid, bits);
+    else pred_dense(a.preds[p], k0, n, vec_ok, bits);
   }
   // Semi-join probe: key(row) - min must be a set bit of the bitmap.  All
   // 16 keys and bitmap words are loaded before any is used, so the loads
   // overlap instead of serialising behind one another.
   if (a.probe_bm && bits) {
     int64_t key[FJ][4];
-    load_keys16(a.probe_key, a.probe_key_type, !a.in_sel && a.probe_vec, k0, n, rid, key);
+    load_keys16(a.probe_key, a.probe_key_type, !SEL && a.probe_vec, k0, n, rid, key);
     uint64_t w[FJ][4];
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
@@ -130,12 +126,13 @@ __device__ __forceinline__ void mark_word(const FilterArgs& a, uint64_t* win, ui
 // four rows of a lane-step usually share one 64-bit word (sorted keys):
 // their bits are OR-ed first, so one atomic covers them.  The atomics do
 // not return: a repeated key is detected afterwards by comparing the
-// bitmap's popcount with mark_count (the number of marked rows).
+// bitmap's popcount with the popcount of the pass bits (sel_bm).
+template <bool SEL>
 __device__ __forceinline__ void mark_rows(const FilterArgs& a, uint64_t n, uint64_t k0,
                                           const uint32_t (&rid)[FJ][4], uint32_t bits,
                                           uint64_t* win, uint64_t wbase) {
   int64_t key[FJ][4];
-  load_keys16(a.mark_key, a.mark_key_type, !a.in_sel && a.mark_vec, k0, n, rid, key);
+  load_keys16(a.mark_key, a.mark_key_type, !SEL && a.mark_vec, k0, n, rid, key);
   uint32_t e = 0;
 #pragma unroll
   for (int j = 0; j < FJ; ++j) {
@@ -164,28 +161,24 @@ __device__ __forceinline__ void mark_rows(const FilterArgs& a, uint64_t n, uint6
     }
   }
   if (e) atomicOr(a.err, e);
-  if (a.mark_count) {
-    uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
-    if (lane_id() == 0 && c) atomicAdd(a.mark_count, c);
-  }
 }
 
 // Pass 1 of a filter: one tile of 4096 rows per block, no inter-block
 // dependency.  Writes the tile's pass bits to sel_bm (1 bit per row, bit r
 // for row r of the input order) and its count to tile_counts[tile].
+template <bool SEL>
 __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
   __shared__ uint32_t s_wave_tot[FWAVES];
   const int tid = threadIdx.x;
   const int wave = tid / WAVE;
   const int lane = tid % WAVE;
   const uint32_t tile = blockIdx.x;
-  const uint64_t n = a.in_sel ? (a.in_count ? (uint64_t)*a.in_count : a.n_rows)
-                              : a.n_rows;
+  const uint64_t n = SEL ? (a.in_count ? (uint64_t)*a.in_count : a.n_rows) : a.n_rows;
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
   uint32_t rid[FJ][4];
   uint32_t bits = 0;
-  if ((uint64_t)tile * FTILE < n) bits = eval_lane(a, n, k0, rid);
+  if ((uint64_t)tile * FTILE < n) bits = eval_lane<SEL>(a, n, k0, rid);
   else {
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
@@ -199,7 +192,7 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
     if (tid == 0) {
       uint64_t base = 0;
       if ((uint64_t)tile * FTILE < n) {
-        uint64_t r = a.in_sel ? a.in_sel[(uint64_t)tile * FTILE] : (uint64_t)tile * FTILE;
+        uint64_t r = SEL ? a.in_sel[(uint64_t)tile * FTILE] : (uint64_t)tile * FTILE;
         uint64_t s0 = key_off(load_key(a.mark_key, a.mark_key_type, r), a.mark_min);
         base = s0 < a.mark_span ? (s0 >> 6) : 0;
       }
@@ -207,7 +200,7 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
     }
     __syncthreads();
     const uint64_t wbase = s_wbase;
-    mark_rows(a, n, k0, rid, bits, win, wbase);  // all lanes (wave sum)
+    mark_rows<SEL>(a, n, k0, rid, bits, win, wbase);  // all lanes (wave sum)
     __syncthreads();
     const uint64_t words = (a.mark_span + 63) / 64;
     for (int i = tid; i < kMarkWin; i += FBLOCK) {
@@ -239,32 +232,41 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
   }
 }
 
-// Pass 2: exclusive scan of the tile counts in place (one block), total to
-// *total.  T is at most 2^32 / 4096 = 2^20 tiles.
+// Pass 2: exclusive scan of the tile counts in place (one block of 1024),
+// total to *total.  Chunks of 1024 counts are loaded coalesced, 8 chunks at a
+// time (their loads overlap), and scanned in LDS with a running carry.
 constexpr int SBLOCK = 1024;
 __global__ __launch_bounds__(SBLOCK) void k_scan_counts(uint32_t* counts, uint32_t T,
                                                         uint32_t* total) {
   __shared__ uint32_t s_wave[SBLOCK / WAVE];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint32_t per = (T + SBLOCK - 1) / SBLOCK;
-  const uint32_t b = tid * per, e = min(T, b + per);
-  uint32_t sum = 0;
-  for (uint32_t i = b; i < e; ++i) sum += counts[i];
-  uint32_t incl = wave_inclusive_u32(sum);
-  if (lane == WAVE - 1) s_wave[wave] = incl;
-  __syncthreads();
-  uint32_t off = 0, all = 0;
-  for (int w = 0; w < SBLOCK / WAVE; ++w) {
-    if (w < wave) off += s_wave[w];
-    all += s_wave[w];
+  uint32_t carry = 0;
+  for (uint32_t g = 0; g < T; g += 8 * SBLOCK) {
+    uint32_t v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      uint32_t i = g + c * SBLOCK + tid;
+      v[c] = i < T ? counts[i] : 0u;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      uint32_t incl = wave_inclusive_u32(v[c]);
+      if (lane == WAVE - 1) s_wave[wave] = incl;
+      __syncthreads();
+      uint32_t off = 0, all = 0;
+#pragma unroll
+      for (int w = 0; w < SBLOCK / WAVE; ++w) {
+        uint32_t t = s_wave[w];
+        off += w < wave ? t : 0u;
+        all += t;
+      }
+      uint32_t i = g + c * SBLOCK + tid;
+      if (i < T) counts[i] = carry + off + incl - v[c];
+      carry += all;
+      __syncthreads();
+    }
   }
-  uint32_t run = off + incl - sum;
-  for (uint32_t i = b; i < e; ++i) {
-    uint32_t c = counts[i];
-    counts[i] = run;
-    run += c;
-  }
-  if (tid == 0) *total = all;
+  if (tid == 0) *total = carry;
 }
 
 // Pass 3: expand the pass bits of each tile into ascending row ids at the
@@ -315,8 +317,8 @@ int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st) {
   uint32_t tiles = filter_tiles(capacity);
   a.vec_ok = 0;
   for (uint32_t p = 0; p < a.n_preds; ++p) {
-    int w = type_width(a.preds[p].type);
-    uintptr_t addr = (uintptr_t)a.preds[p].column;
+    int w = (int)dev::pred_width(a.preds[p].kind);
+    uintptr_t addr = (uintptr_t)a.preds[p].col;
     int need = w == 1 ? 4 : 16;
     if (addr % need == 0) a.vec_ok |= 1u << p;
   }
@@ -330,7 +332,8 @@ int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st) {
   };
   a.probe_vec = key_vec(a.probe_key, a.probe_key_type);
   a.mark_vec = key_vec(a.mark_key, a.mark_key_type);
-  hipLaunchKernelGGL(k_filter, dim3(tiles), dim3(FBLOCK), 0, st, a);
+  if (a.in_sel) hipLaunchKernelGGL(k_filter<true>, dim3(tiles), dim3(FBLOCK), 0, st, a);
+  else hipLaunchKernelGGL(k_filter<false>, dim3(tiles), dim3(FBLOCK), 0, st, a);
   MTL_HIP(hipGetLastError());
   return MTL_OK;
 }
@@ -339,7 +342,7 @@ int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st) {
 // Mask output (compare_*_values counterpart).
 // ---------------------------------------------------------------------------
 struct CompareArgs {
-  mtl_pred preds[MTL_MAX_PREDS];
+  PredC preds[MTL_MAX_PREDS];
   uint32_t n_preds;
   uint64_t n;
   uint8_t* out;
@@ -349,7 +352,10 @@ __global__ void k_compare(CompareArgs a) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     bool ok = true;
-    for (uint32_t p = 0; p < a.n_preds && ok; ++p) ok = eval_pred_row(a.preds[p], i);
+    for (uint32_t p = 0; p < a.n_preds && ok; ++p) {
+      const PredC& c = a.preds[p];
+      ok = pred_pass(enc_raw(load_raw(c.col, pred_width(c.kind), i), c.kind), c);
+    }
     a.out[i] = ok ? 1 : 0;
   }
 }
@@ -450,7 +456,7 @@ int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
   zl.add((uint32_t*)ctr, 64);
   MTL_TRY(launch_zero(zl, s->stream));
   FilterArgs a{};
-  for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = preds[p];
+  for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = compile_pred(preds[p]);
   a.n_preds = n_preds;
   a.n_rows = cap;
   a.in_sel = in_sel;
@@ -473,7 +479,7 @@ int32_t mtl_compare(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
     return set_error(MTL_UNSUPPORTED, "more than %d predicates", MTL_MAX_PREDS);
   MTL_TRY(resolve_stream(s, &s));
   CompareArgs a{};
-  for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = preds[p];
+  for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = compile_pred(preds[p]);
   a.n_preds = n_preds;
   a.n = n_rows;
   a.out = out_mask;

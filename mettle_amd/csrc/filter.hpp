@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "../../include/mettle_hip.h"
+#include "device.hpp"
 
 namespace mtl {
 
@@ -23,8 +24,33 @@ struct ZeroList {
 };
 int32_t launch_zero(const ZeroList& zl, hipStream_t st);
 
+// Host: mtl_pred -> compiled predicate (encoded literal + op mask).  Integer
+// literals compare exactly against every integer width (DuckDB compares an
+// INTEGER column with a BIGINT literal in BIGINT); FLOAT literals are
+// rounded to float first.
+inline dev::PredC compile_pred(const mtl_pred& p) {
+  dev::PredC c;
+  c.col = p.column;
+  switch (p.type) {
+    case MTL_BIGINT:
+    case MTL_TIMESTAMP:
+    case MTL_DECIMAL: c.kind = dev::PK_I64; break;
+    case MTL_DICT8: c.kind = dev::PK_U8; break;
+    case MTL_FLOAT: c.kind = dev::PK_F32; break;
+    case MTL_DOUBLE: c.kind = dev::PK_F64; break;
+    default: c.kind = dev::PK_I32; break;
+  }
+  static const uint32_t masks[5] = {2u, 1u, 3u, 4u, 6u};  // EQ LT LE GT GE
+  c.opmask = masks[p.op];
+  double d = __builtin_bit_cast(double, p.scalar);
+  if (p.type == MTL_FLOAT) c.lit = dev::enc_dbl((double)(float)d);
+  else if (p.type == MTL_DOUBLE) c.lit = dev::enc_dbl(d);
+  else c.lit = dev::enc_int(p.scalar);
+  return c;
+}
+
 struct FilterArgs {
-  mtl_pred preds[MTL_MAX_PREDS];
+  dev::PredC preds[MTL_MAX_PREDS];
   uint32_t n_preds;
   uint32_t vec_ok;  // bit p: column p is aligned for vector loads
   uint64_t n_rows;  // rows (no in_sel) or capacity of in_sel
@@ -46,7 +72,6 @@ struct FilterArgs {
   int64_t mark_min;
   uint64_t mark_span;
   uint32_t probe_vec, mark_vec;  // key columns aligned for vector loads
-  uint32_t* mark_count;          // optional: number of marked rows (+=)
 };
 // One block per 4096-row tile, no inter-block dependency.  With sel_bm and
 // tile_counts set, launch_compact turns the pass bits into a stable

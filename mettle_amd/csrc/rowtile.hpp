@@ -51,124 +51,80 @@ __device__ __forceinline__ void load4(const T* __restrict__ col, uint64_t k,
   }
 }
 
-template <typename T, typename Cmp>
-__device__ __forceinline__ void pred_dense(const T* __restrict__ col,
-                                           uint64_t k0, uint64_t n, bool vec_ok,
-                                           Cmp cmp, uint32_t& bits) {
+// Raw bytes of 4 consecutive rows (zero-extended): one 16-byte load (two
+// for 8-byte values, one 4-byte load for bytes) when aligned and in range.
+__device__ __forceinline__ void load_raw4(const void* col, uint32_t width, uint64_t k,
+                                          uint64_t n, bool vec_ok, uint64_t (&x)[4]) {
+  if (vec_ok && k + 3 < n) {
+    if (width == 4) {
+      uint4 u = *(const uint4*)((const uint32_t*)col + k);
+      x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w;
+    } else if (width == 8) {
+      ulonglong2 a = *(const ulonglong2*)((const uint64_t*)col + k);
+      ulonglong2 b = *(const ulonglong2*)((const uint64_t*)col + k + 2);
+      x[0] = a.x; x[1] = a.y; x[2] = b.x; x[3] = b.y;
+    } else {
+      uint32_t u = *(const uint32_t*)((const uint8_t*)col + k);
+      x[0] = u & 0xFFu; x[1] = (u >> 8) & 0xFFu; x[2] = (u >> 16) & 0xFFu; x[3] = u >> 24;
+    }
+  } else {
 #pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    uint32_t m = (bits >> (4 * j)) & 0xFu;
-    if (!m) continue;
-    T x[4];
-    load4<T>(col, k0 + (uint64_t)j * 256, n, vec_ok, x);
-    uint32_t r = 0;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) r |= (uint32_t)cmp(x[v]) << v;
-    bits &= ~(((~r) & 0xFu) << (4 * j));
+    for (int v = 0; v < 4; ++v) x[v] = k + v < n ? load_raw(col, width, k + v) : 0ull;
   }
 }
 
-template <typename T, typename Cmp>
-__device__ __forceinline__ void pred_sel(const T* __restrict__ col,
-                                         const uint32_t (&rid)[FJ][4],
-                                         Cmp cmp, uint32_t& bits) {
-  // all loads first (they overlap), then the compares
-  T x[FJ][4];
+// Dense rows k0 + j*256 + v: clear the bits of rows failing p.  All four
+// steps are loaded before any compare: a "skip dead step" branch here would
+// depend on the bits the previous step just updated and serialise the loads.
+__device__ __forceinline__ void pred_dense(const PredC& p, uint64_t k0, uint64_t n, bool vec_ok,
+                                           uint32_t& bits) {
+  const uint32_t width = pred_width(p.kind);
+  uint64_t x[FJ][4];
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) load_raw4(p.col, width, k0 + (uint64_t)j * 256, n, vec_ok, x[j]);
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      x[j][v] = (bits >> (4 * j + v)) & 1u ? col[rid[j][v]] : T(0);
+      if (!pred_pass(enc_raw(x[j][v], p.kind), p)) bits &= ~(1u << (4 * j + v));
+}
+
+// Rows given by row id (all loads first, then the compares).
+__device__ __forceinline__ void pred_sel(const PredC& p, const uint32_t (&rid)[FJ][4],
+                                         uint32_t& bits) {
+  const uint32_t width = pred_width(p.kind);
+  uint64_t x[FJ][4];
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      x[j][v] = (bits >> (4 * j + v)) & 1u ? load_raw(p.col, width, rid[j][v]) : 0ull;
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       uint32_t b = 1u << (4 * j + v);
-      if ((bits & b) && !cmp(x[j][v])) bits &= ~b;
+      if ((bits & b) && !pred_pass(enc_raw(x[j][v], p.kind), p)) bits &= ~b;
     }
 }
 
-// pred_sel for dense rows: row of (j, v) is k0 + j*256 + v (every row with
-// a set bit is below n).
-template <typename T, typename Cmp>
-__device__ __forceinline__ void pred_rows(const T* __restrict__ col, uint64_t k0, Cmp cmp,
-                                          uint32_t& bits) {
-  T x[FJ][4];
+// pred_sel for dense rows k0 + j*256 + v (every row with a set bit is < n).
+__device__ __forceinline__ void pred_rows(const PredC& p, uint64_t k0, uint32_t& bits) {
+  const uint32_t width = pred_width(p.kind);
+  uint64_t x[FJ][4];
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      x[j][v] = (bits >> (4 * j + v)) & 1u ? col[k0 + (uint64_t)j * 256 + v] : T(0);
+      x[j][v] = (bits >> (4 * j + v)) & 1u ? load_raw(p.col, width, k0 + (uint64_t)j * 256 + v)
+                                            : 0ull;
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       uint32_t b = 1u << (4 * j + v);
-      if ((bits & b) && !cmp(x[j][v])) bits &= ~b;
+      if ((bits & b) && !pred_pass(enc_raw(x[j][v], p.kind), p)) bits &= ~b;
     }
-}
-
-// Dispatch one predicate on (type, op); `f` receives a typed column pointer
-// and a comparison functor.
-template <typename F>
-__device__ __forceinline__ void with_pred(const mtl_pred& p, F&& f) {
-  const int op = p.op;
-  switch (p.type) {
-    case MTL_INTEGER:
-    case MTL_DATE:
-    case MTL_DICT32: {
-      const int32_t s = (int32_t)p.scalar;
-      const int32_t* c = (const int32_t*)p.column;
-      switch (op) {
-        case MTL_CMP_EQ: f(c, [=](int32_t x) { return x == s; }); break;
-        case MTL_CMP_LT: f(c, [=](int32_t x) { return x < s; }); break;
-        case MTL_CMP_LE: f(c, [=](int32_t x) { return x <= s; }); break;
-        case MTL_CMP_GT: f(c, [=](int32_t x) { return x > s; }); break;
-        default: f(c, [=](int32_t x) { return x >= s; }); break;
-      }
-      break;
-    }
-    case MTL_BIGINT:
-    case MTL_TIMESTAMP:
-    case MTL_DECIMAL: {
-      const int64_t s = p.scalar;
-      const int64_t* c = (const int64_t*)p.column;
-      switch (op) {
-        case MTL_CMP_EQ: f(c, [=](int64_t x) { return x == s; }); break;
-        case MTL_CMP_LT: f(c, [=](int64_t x) { return x < s; }); break;
-        case MTL_CMP_LE: f(c, [=](int64_t x) { return x <= s; }); break;
-        case MTL_CMP_GT: f(c, [=](int64_t x) { return x > s; }); break;
-        default: f(c, [=](int64_t x) { return x >= s; }); break;
-      }
-      break;
-    }
-    case MTL_DICT8: {
-      const int32_t s = (int32_t)p.scalar;
-      const uint8_t* c = (const uint8_t*)p.column;
-      switch (op) {
-        case MTL_CMP_EQ: f(c, [=](uint8_t x) { return (int32_t)x == s; }); break;
-        case MTL_CMP_LT: f(c, [=](uint8_t x) { return (int32_t)x < s; }); break;
-        case MTL_CMP_LE: f(c, [=](uint8_t x) { return (int32_t)x <= s; }); break;
-        case MTL_CMP_GT: f(c, [=](uint8_t x) { return (int32_t)x > s; }); break;
-        default: f(c, [=](uint8_t x) { return (int32_t)x >= s; }); break;
-      }
-      break;
-    }
-    case MTL_FLOAT: {
-      const float s = (float)__longlong_as_double(p.scalar);
-      const float* c = (const float*)p.column;
-      f(c, [=](float x) { return cmp_float<float>(x, s, op); });
-      break;
-    }
-    case MTL_DOUBLE: {
-      const double s = __longlong_as_double(p.scalar);
-      const double* c = (const double*)p.column;
-      f(c, [=](double x) { return cmp_float<double>(x, s, op); });
-      break;
-    }
-    default:
-      break;
-  }
 }
 
 // Integer keys of the lane's 16 rows: 16-byte vector loads when the rows

@@ -48,16 +48,41 @@ constexpr int RBLOCK = 256;
 constexpr int RPER = 8;
 constexpr int RTILE = RBLOCK * RPER;
 
-// Also writes the parent bitmap's total popcount (*par_total, last tile)
-// and adds the dimension bitmap's popcount into *dim_pop (grid-stride), so
-// k_map can compare them with the marked-row counts (duplicate keys).
+// Also writes the parent bitmap's total popcount (*par_total, last tile) and
+// accumulates three popcounts for the duplicate-key checks of k_map, with
+// one atomic per block (grid-stride): pops[0] += |pass bits of P|,
+// pops[1] += |dimension bitmap|, pops[2] += |pass bits of D|.
+struct PopArgs {
+  const uint64_t* pbits;
+  uint64_t pbits_words;
+  const uint64_t* dim_bm;
+  uint64_t dim_words;
+  const uint64_t* dbits;
+  uint64_t dbits_words;
+  uint32_t* pops;
+};
+
+__device__ __forceinline__ uint32_t block_popcount(const uint64_t* p, uint64_t words,
+                                                   uint32_t* s_red) {
+  uint32_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * RBLOCK + threadIdx.x; i < words;
+       i += (uint64_t)gridDim.x * RBLOCK)
+    c += (uint32_t)__popcll(p[i]);
+  c = wave_sum_u32(c);
+  if (threadIdx.x % WAVE == 0) s_red[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int w = 0; w < RBLOCK / WAVE; ++w) t += s_red[w];
+  __syncthreads();
+  return t;
+}
+
 __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm,
                                                  uint64_t words,
                                                  uint32_t* __restrict__ rank,
                                                  uint64_t* status, uint32_t* ctr,
                                                  uint32_t* err, uint32_t* par_total,
-                                                 const uint64_t* __restrict__ dim_bm,
-                                                 uint64_t dim_words, uint32_t* dim_pop) {
+                                                 PopArgs pa) {
   __shared__ uint32_t s_tile, s_prefix;
   __shared__ uint32_t s_wave[RBLOCK / WAVE];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
@@ -93,13 +118,14 @@ __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm
     run += c[i];
   }
   if (tile == gridDim.x - 1 && tid == 0) *par_total = s_prefix + total;
-  if (dim_bm) {
-    uint32_t dp = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * RBLOCK + tid; i < dim_words;
-         i += (uint64_t)gridDim.x * RBLOCK)
-      dp += (uint32_t)__popcll(dim_bm[i]);
-    dp = wave_sum_u32(dp);
-    if (lane == 0 && dp) atomicAdd(dim_pop, dp);
+  __shared__ uint32_t s_red[RBLOCK / WAVE];
+  uint32_t c0 = block_popcount(pa.pbits, pa.pbits_words, s_red);
+  uint32_t c1 = pa.dim_bm ? block_popcount(pa.dim_bm, pa.dim_words, s_red) : 0u;
+  uint32_t c2 = pa.dbits ? block_popcount(pa.dbits, pa.dbits_words, s_red) : 0u;
+  if (tid == 0) {
+    if (c0) atomicAdd(&pa.pops[0], c0);
+    if (c1) atomicAdd(&pa.pops[1], c1);
+    if (c2) atomicAdd(&pa.pops[2], c2);
   }
 }
 
@@ -113,7 +139,6 @@ __device__ __forceinline__ uint32_t rank_of(const uint64_t* bm, const uint32_t* 
 struct MapArgs {
   const uint64_t* sel_bm;  // parent pass bits, 1 per P row
   uint64_t n_rows;
-  const uint32_t* count;   // marked (qualifying) P rows
   const void* key;
   int32_t key_type;
   int64_t min;
@@ -124,8 +149,8 @@ struct MapArgs {
   uint64_t* acc;
   uint32_t* cnt;
   const uint32_t* par_total;  // parent bitmap popcount = accumulator slots
-  const uint32_t* dim_pop;    // dimension bitmap popcount (NULL: no dim)
-  const uint32_t* dim_count;  // marked dimension rows
+  const uint32_t* pops;       // [0] P pass bits, [1] D bitmap, [2] D pass bits
+  uint32_t has_dim;
   uint32_t* err;
 };
 
@@ -137,8 +162,8 @@ struct MapArgs {
 __global__ void k_map(MapArgs a) {
   const uint32_t M = *a.par_total;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (M != *a.count) atomicOr(a.err, 2u);  // repeated parent key
-    if (a.dim_pop && *a.dim_pop != *a.dim_count) atomicOr(a.err, 4u);
+    if (M != a.pops[0]) atomicOr(a.err, 2u);                     // repeated parent key
+    if (a.has_dim && a.pops[1] != a.pops[2]) atomicOr(a.err, 4u);  // repeated dim key
   }
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -221,7 +246,7 @@ struct FactArgs {
   uint64_t span;
   const uint64_t* bm;
   const uint32_t* rank;
-  mtl_pred preds[MTL_MAX_PREDS];
+  PredC preds[MTL_MAX_PREDS];
   uint32_t n_preds;
   mtl_expr expr;
   uint64_t* acc;  // int64 or double bits
@@ -229,6 +254,9 @@ struct FactArgs {
   uint32_t* err;
   uint32_t fk_vec;  // 1: fk is 16-byte aligned
   unsigned long long* maxabs;  // max |value| over accumulated rows
+  uint32_t* hit_row;    // hit buffer (per-block regions of tpb*4096 rows)
+  uint32_t* hit_slot;   // key slot of each hit
+  uint32_t* hit_count;  // hits per block
 };
 
 // Expression over preloaded column values (one row).
@@ -272,67 +300,22 @@ __device__ __forceinline__ double expr_f64_vals(const mtl_expr& e, const double*
   return sp ? st[sp - 1] : 0.0;
 }
 
-// Accumulator slots per block kept in LDS (48 KB with the counts).
-constexpr int kAccWin = 4096;
+// The fact pass runs as two compact kernels over the same persistent grid
+// (each block owns a contiguous range of 4096-row tiles):
+//   k_fact_scan : stream the foreign key, probe the parent bitmap, evaluate
+//                 the fact predicates on the hits, and append the hit rows
+//                 (with their key slot) to the block's own region of the
+//                 hit buffer (region capacity = the block's row count, so
+//                 no global atomics and no inter-block dependency);
+//   k_fact_acc  : one hit per lane: rank, expression columns and the value
+//                 are loaded/computed with all lanes in parallel, then added
+//                 into an LDS window of accumulator slots that starts at the
+//                 block's first hit (facts clustered by foreign key keep a
+//                 block's slots in a narrow range); each touched slot is
+//                 flushed once with a coalesced global atomic, and slots
+//                 outside the window go straight to global atomics.
+constexpr int kAccWin = 4096;  // LDS accumulator slots (48 KB with counts)
 
-// Accumulate the (up to 4) hits of one lane-step: the bitmap words, ranks
-// and expression columns of all hits are loaded before any is used, and
-// the atomics return nothing (an int64 SUM overflow is excluded afterwards
-// from max|value| x count, checked by the top-K kernel).
-template <bool DBL>
-__device__ __forceinline__ void fact_hits(const FactArgs& a, uint32_t m, const uint32_t (&sj)[4],
-                                          uint64_t row0, bool& ovf, uint64_t& mx,
-                                          uint64_t* wacc, uint32_t* wcnt, uint32_t wbase) {
-  uint64_t wv[4];
-  uint32_t rk[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-    if ((m >> v) & 1u) {
-      wv[v] = a.bm[sj[v] >> 6];
-      rk[v] = a.rank[sj[v] >> 6];
-    }
-  using V = typename std::conditional<DBL, double, int64_t>::type;
-  V cv[4][MTL_EXPR_MAX_COLS];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-    if ((m >> v) & 1u) {
-#pragma unroll
-      for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
-        if ((uint32_t)c < a.expr.n_cols) {
-          if constexpr (DBL) cv[v][c] = load_f64(a.expr.col[c], a.expr.col_type[c], row0 + v);
-          else cv[v][c] = load_key(a.expr.col[c], a.expr.col_type[c], row0 + v);
-        }
-    }
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-    if ((m >> v) & 1u) {
-      const uint32_t sv = sj[v];
-      const uint64_t below = (sv & 63) ? (wv[v] & ((1ull << (sv & 63)) - 1)) : 0ull;
-      const uint32_t r = rk[v] + (uint32_t)__popcll(below);
-      // slots inside the block's window accumulate in LDS, others globally
-      const uint32_t wi = r - wbase;
-      const bool local = wi < (uint32_t)kAccWin;
-      if constexpr (DBL) {
-        double val = expr_f64_vals(a.expr, cv[v]);
-        if (local) unsafeAtomicAdd((double*)&wacc[wi], val);
-        else unsafeAtomicAdd((double*)&a.acc[r], val);
-      } else {
-        int64_t val = expr_i64_vals(a.expr, cv[v], ovf);
-        uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
-        mx = av > mx ? av : mx;
-        atomicAdd(local ? (unsigned long long*)&wacc[wi] : (unsigned long long*)&a.acc[r],
-                  (unsigned long long)val);
-      }
-      atomicAdd(local ? &wcnt[wi] : &a.cnt[r], 1u);
-    }
-}
-
-// One tile = 4096 fact rows per block, 16 per lane (rowtile.hpp layout).
-// Phase 1 loads all 16 foreign keys (16-byte vector loads) and then all 16
-// parent-bitmap words before testing any of them; phase 2 evaluates the
-// fact predicates for the hits only (their loads again issued together);
-// phase 3 evaluates the expression and accumulates.  No load waits on
-// another load of the same phase.
 template <typename KT>
 __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32_t (&s)[FJ][4]) {
 #pragma unroll
@@ -348,40 +331,21 @@ __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32
   }
 }
 
-// Each block owns a contiguous range of tiles.  Accumulation goes to an LDS
-// window of kAccWin slots starting at the rank of the block's first foreign
-// key: when the fact table is clustered by its foreign key (lines stored by
-// order), a block's hits fall into a narrow slot range, so nearly every
-// update is an LDS atomic and the block flushes each touched slot once with
-// one coalesced global atomic.  Hits outside the window (unclustered data)
-// fall back to global atomics; the result is the same either way.
-template <bool DBL>
-__global__ __launch_bounds__(FBLOCK) void k_fact(FactArgs a) {
-  __shared__ uint64_t wacc[kAccWin];
-  __shared__ uint32_t wcnt[kAccWin];
-  __shared__ uint32_t s_wbase;
-  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  bool ovf = false;
-  uint64_t mx = 0;
+__device__ __forceinline__ void fact_block_range(const FactArgs& a, uint64_t& tb, uint64_t& te) {
   const uint64_t tiles = (a.n + FTILE - 1) / FTILE;
   const uint64_t tpb = (tiles + gridDim.x - 1) / gridDim.x;
-  const uint64_t tb = (uint64_t)blockIdx.x * tpb;
-  const uint64_t te = tb + tpb < tiles ? tb + tpb : tiles;
-  if (tb >= te) return;
-  for (int i = threadIdx.x; i < kAccWin; i += FBLOCK) {
-    wacc[i] = 0;
-    wcnt[i] = 0;
-  }
-  if (threadIdx.x == 0) {
-    int64_t k = load_key(a.fk, a.fk_type, tb * FTILE);
-    uint64_t s0 = key_off(k, a.min);
-    if (s0 >= a.span) s0 = k < a.min ? 0 : a.span - 1;
-    uint64_t w = a.bm[s0 >> 6];
-    uint64_t below = (s0 & 63) ? (w & ((1ull << (s0 & 63)) - 1)) : 0ull;
-    s_wbase = a.rank[s0 >> 6] + (uint32_t)__popcll(below);
-  }
-  __syncthreads();
-  const uint32_t wbase = s_wbase;
+  tb = (uint64_t)blockIdx.x * tpb;
+  te = tb + tpb < tiles ? tb + tpb : tiles;
+}
+
+__global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
+  __shared__ uint32_t s_wave[FWAVES];
+  __shared__ uint32_t s_count;
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  uint64_t tb, te;
+  fact_block_range(a, tb, te);
+  if (tid == 0) s_count = 0;
+  const uint64_t region = tb * FTILE;  // this block's region of the hit buffer
   for (uint64_t t = tb; t < te; ++t) {
     const uint64_t k0 = t * FTILE + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
     uint32_t s[FJ][4];
@@ -407,16 +371,86 @@ __global__ __launch_bounds__(FBLOCK) void k_fact(FactArgs a) {
           if (in && ((w[j][v] >> (s[j][v] & 63)) & 1ull)) hits |= 1u << (4 * j + v);
         }
     }
-    if (!hits) continue;
-    for (uint32_t p = 0; p < a.n_preds && hits; ++p)
-      with_pred(a.preds[p], [&](auto col, auto cmp) { pred_rows(col, k0, cmp, hits); });
-    if (!hits) continue;
+    for (uint32_t p = 0; p < a.n_preds && hits; ++p) pred_rows(a.preds[p], k0, hits);
+    // append the tile's hits (block-local positions)
+    const uint32_t c = (uint32_t)__popc(hits);
+    const uint32_t incl = wave_inclusive_u32(c);
+    if (lane == WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t pos = s_count + incl - c, tot = 0;
 #pragma unroll
-    for (int j = 0; j < FJ; ++j) {
-      uint32_t m = (hits >> (4 * j)) & 0xFu;
-      if (!m) continue;
-      fact_hits<DBL>(a, m, s[j], k0 + (uint64_t)j * 256, ovf, mx, wacc, wcnt, wbase);
+    for (int w = 0; w < FWAVES; ++w) {
+      if (w < wave) pos += s_wave[w];
+      tot += s_wave[w];
     }
+    uint32_t h = hits;
+    while (h) {
+      int i = __ffs(h) - 1;
+      h &= h - 1;
+      const int j = i >> 2, v = i & 3;
+      a.hit_row[region + pos] = (uint32_t)(k0 + (uint64_t)j * 256 + v);
+      a.hit_slot[region + pos] = s[j][v];
+      ++pos;
+    }
+    __syncthreads();
+    if (tid == 0) s_count += tot;
+  }
+  __syncthreads();
+  if (tid == 0) a.hit_count[blockIdx.x] = s_count;
+}
+
+template <bool DBL>
+__global__ __launch_bounds__(FBLOCK) void k_fact_acc(FactArgs a) {
+  __shared__ uint64_t wacc[kAccWin];
+  __shared__ uint32_t wcnt[kAccWin];
+  __shared__ uint32_t s_wbase;
+  uint64_t tb, te;
+  fact_block_range(a, tb, te);
+  const uint32_t nh = tb < te ? a.hit_count[blockIdx.x] : 0u;
+  if (nh == 0) return;
+  const uint64_t region = tb * FTILE;
+  for (int i = threadIdx.x; i < kAccWin; i += FBLOCK) {
+    wacc[i] = 0;
+    wcnt[i] = 0;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t sv = a.hit_slot[region];
+    const uint64_t w = a.bm[sv >> 6];
+    const uint64_t below = (sv & 63) ? (w & ((1ull << (sv & 63)) - 1)) : 0ull;
+    s_wbase = a.rank[sv >> 6] + (uint32_t)__popcll(below);
+  }
+  __syncthreads();
+  const uint32_t wbase = s_wbase;
+  bool ovf = false;
+  uint64_t mx = 0;
+  using V = typename std::conditional<DBL, double, int64_t>::type;
+  for (uint32_t i = threadIdx.x; i < nh; i += FBLOCK) {
+    const uint32_t row = a.hit_row[region + i];
+    const uint32_t sv = a.hit_slot[region + i];
+    V cv[MTL_EXPR_MAX_COLS];
+#pragma unroll
+    for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
+      if ((uint32_t)c < a.expr.n_cols) {
+        if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
+        else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
+      }
+    const uint64_t w = a.bm[sv >> 6];
+    const uint64_t below = (sv & 63) ? (w & ((1ull << (sv & 63)) - 1)) : 0ull;
+    const uint32_t r = a.rank[sv >> 6] + (uint32_t)__popcll(below);
+    const uint32_t wi = r - wbase;
+    const bool local = wi < (uint32_t)kAccWin;
+    if constexpr (DBL) {
+      double val = expr_f64_vals(a.expr, cv);
+      if (local) unsafeAtomicAdd((double*)&wacc[wi], val);
+      else unsafeAtomicAdd((double*)&a.acc[r], val);
+    } else {
+      int64_t val = expr_i64_vals(a.expr, cv, ovf);
+      uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
+      mx = av > mx ? av : mx;
+      atomicAdd(local ? (unsigned long long*)&wacc[wi] : (unsigned long long*)&a.acc[r],
+                (unsigned long long)val);
+    }
+    atomicAdd(local ? &wcnt[wi] : &a.cnt[r], 1u);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kAccWin; i += FBLOCK) {
@@ -455,12 +489,17 @@ struct TopkArgs {
   uint32_t* out_n;  // [0] = rows, [1] = error bits copied from err
   uint32_t* err;
   const unsigned long long* maxabs;
+  uint64_t p_rows;  // row_of entries are checked against it before use
 };
 
 __device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t r, uint64_t k[4]) {
   uint64_t v = a.acc[r];
   k[0] = a.is_double ? ~enc_f64(__longlong_as_double((long long)v)) : ~enc_i64((int64_t)v);
   uint32_t row = a.row_of[r];
+  if (row >= a.p_rows) {  // cannot happen for consistent inputs; never read past P
+    atomicOr(a.err, 64u);
+    row = 0;
+  }
 #pragma unroll
   for (int t = 0; t < MTL_STAR_MAX_TIES; ++t) {
     if ((uint32_t)t < a.n_ties) {
@@ -632,9 +671,13 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   hipStream_t st = s->stream;
   Arena& ar = s->arena;
   const uint64_t prows = d->p_rows;
-  void *dim_bm = nullptr, *par_bm, *par_rank, *pbits, *row_of, *acc, *cnt, *misc;
+  void *dim_bm = nullptr, *dbits = nullptr, *par_bm, *par_rank, *pbits, *row_of, *acc, *cnt,
+       *misc;
   void *st_r, *ck[4], *cid, *cn;
-  if (d->d_rows) MTL_TRY(ar.alloc(sp.dwords * 8, &dim_bm));
+  if (d->d_rows) {
+    MTL_TRY(ar.alloc(sp.dwords * 8, &dim_bm));
+    MTL_TRY(ar.alloc((uint64_t)filter_tiles(d->d_rows) * kTileRows / 8, &dbits));
+  }
   MTL_TRY(ar.alloc(sp.pwords * 8, &par_bm));
   MTL_TRY(ar.alloc(sp.pwords * 4, &par_rank));
   uint32_t ptiles = filter_tiles(prows);
@@ -642,9 +685,9 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &row_of));
   MTL_TRY(ar.alloc((prows ? prows : 1) * 8, &acc));
   MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &cnt));
-  // misc words: [0] qualifying P rows, [2] rank ticket, [3] error bits,
-  // [4] top-K done counter, [6] parent bitmap popcount, [7] dimension bitmap
-  // popcount, [8] qualifying D rows, [10..11] max |value| (u64)
+  // misc words: [2] rank ticket, [3] error bits, [4] top-K done counter,
+  // [6] parent bitmap popcount, [7] P pass-bit popcount, [8] dimension
+  // bitmap popcount, [9] D pass-bit popcount, [10..11] max |value| (u64)
   MTL_TRY(ar.alloc(256, &misc));
   uint32_t rtiles = (uint32_t)ceil_div(sp.pwords, RTILE);
   MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(rtiles, 1) * 8, &st_r));
@@ -664,7 +707,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   if (d->d_rows) {
     // mark-only filter pass: vectorised predicates, combined bitmap atomics
     FilterArgs fa{};
-    for (uint32_t p = 0; p < d->d_n_preds; ++p) fa.preds[p] = d->d_preds[p];
+    for (uint32_t p = 0; p < d->d_n_preds; ++p) fa.preds[p] = compile_pred(d->d_preds[p]);
     fa.n_preds = d->d_n_preds;
     fa.n_rows = d->d_rows;
     fa.err = mu + 3;
@@ -673,18 +716,17 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.mark_key_type = d->d_key_type;
     fa.mark_min = d->d_key_min;
     fa.mark_span = sp.dspan;
-    fa.mark_count = mu + 8;
+    fa.sel_bm = (uint64_t*)dbits;
     ProfScope ps(s, "k_mark_dim");
     MTL_TRY(launch_filter(fa, d->d_rows, st));
   }
   // 2. parent filter + semi-join probe + compaction + parent bitmap
   {
     FilterArgs fa{};
-    for (uint32_t p = 0; p < d->p_n_preds; ++p) fa.preds[p] = d->p_preds[p];
+    for (uint32_t p = 0; p < d->p_n_preds; ++p) fa.preds[p] = compile_pred(d->p_preds[p]);
     fa.n_preds = d->p_n_preds;
     fa.n_rows = prows;
     fa.sel_bm = (uint64_t*)pbits;
-    fa.mark_count = mu + 0;
     fa.err = mu + 3;
     if (d->d_rows) {
       fa.probe_bm = (const uint64_t*)dim_bm;
@@ -704,10 +746,19 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   // 3. rank
   {
     ProfScope ps(s, "k_rank");
+    PopArgs pa{};
+    pa.pbits = (const uint64_t*)pbits;
+    pa.pbits_words = (uint64_t)ptiles * kTileRows / 64;
+    if (d->d_rows) {
+      pa.dim_bm = (const uint64_t*)dim_bm;
+      pa.dim_words = sp.dwords;
+      pa.dbits = (const uint64_t*)dbits;
+      pa.dbits_words = (uint64_t)filter_tiles(d->d_rows) * kTileRows / 64;
+    }
+    pa.pops = mu + 7;  // [7] P pass bits, [8] D bitmap, [9] D pass bits
     hipLaunchKernelGGL(k_rank, dim3(std::max<uint32_t>(rtiles, 1)), dim3(RBLOCK), 0, st,
                        (const uint64_t*)par_bm, sp.pwords, (uint32_t*)par_rank,
-                       (uint64_t*)st_r, mu + 2, mu + 3, mu + 6,
-                       (const uint64_t*)dim_bm, d->d_rows ? sp.dwords : 0, mu + 7);
+                       (uint64_t*)st_r, mu + 2, mu + 3, mu + 6, pa);
   }
   MTL_HIP(hipGetLastError());
   // 4. map accumulator slots to P rows, clear accumulators
@@ -715,7 +766,6 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     MapArgs ma{};
     ma.sel_bm = (const uint64_t*)pbits;
     ma.n_rows = prows;
-    ma.count = mu + 0;
     ma.key = d->p_key;
     ma.key_type = d->p_key_type;
     ma.min = d->p_key_min;
@@ -726,8 +776,8 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ma.acc = (uint64_t*)acc;
     ma.cnt = (uint32_t*)cnt;
     ma.par_total = mu + 6;
-    ma.dim_pop = d->d_rows ? mu + 7 : nullptr;
-    ma.dim_count = mu + 8;
+    ma.pops = mu + 7;
+    ma.has_dim = d->d_rows ? 1u : 0u;
     ma.err = mu + 3;
     uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
     ProfScope ps(s, "k_map");
@@ -744,7 +794,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.span = sp.pspan;
     fa.bm = (const uint64_t*)par_bm;
     fa.rank = (const uint32_t*)par_rank;
-    for (uint32_t p = 0; p < d->f_n_preds; ++p) fa.preds[p] = d->f_preds[p];
+    for (uint32_t p = 0; p < d->f_n_preds; ++p) fa.preds[p] = compile_pred(d->f_preds[p]);
     fa.n_preds = d->f_n_preds;
     fa.expr = d->agg;
     fa.acc = (uint64_t*)acc;
@@ -752,15 +802,27 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.err = mu + 3;
     fa.fk_vec = ((uintptr_t)d->f_fk % 16) == 0;
     fa.maxabs = (unsigned long long*)(mu + 10);
+    void *hrow, *hslot, *hcnt;
+    const uint64_t hcap = ceil_div(d->f_rows, FTILE) * FTILE;
+    MTL_TRY(ar.alloc(hcap * 4, &hrow));
+    MTL_TRY(ar.alloc(hcap * 4, &hslot));
+    MTL_TRY(ar.alloc((uint64_t)cu_count() * 7 * 4 + 256, &hcnt));
+    fa.hit_row = (uint32_t*)hrow;
+    fa.hit_slot = (uint32_t*)hslot;
+    fa.hit_count = (uint32_t*)hcnt;
     uint64_t g = ceil_div(d->f_rows, FTILE);
-    uint64_t gmax = (uint64_t)cu_count() * 3;  // persistent: 3 blocks per CU
-    ProfScope ps(s, "k_fact");
+    uint64_t gmax = (uint64_t)cu_count() * 7;  // persistent: 7 blocks per CU
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min(g, gmax));
+    {
+      ProfScope ps(s, "k_fact_scan");
+      hipLaunchKernelGGL(k_fact_scan, dim3(grid), dim3(FBLOCK), 0, st, fa);
+    }
+    MTL_HIP(hipGetLastError());
+    ProfScope ps(s, "k_fact_acc");
     if (d->agg.is_double)
-      hipLaunchKernelGGL(k_fact<true>, dim3((uint32_t)std::max<uint64_t>(1, std::min(g, gmax))),
-                         dim3(256), 0, st, fa);
+      hipLaunchKernelGGL(k_fact_acc<true>, dim3(grid), dim3(FBLOCK), 0, st, fa);
     else
-      hipLaunchKernelGGL(k_fact<false>, dim3((uint32_t)std::max<uint64_t>(1, std::min(g, gmax))),
-                       dim3(256), 0, st, fa);
+      hipLaunchKernelGGL(k_fact_acc<false>, dim3(grid), dim3(FBLOCK), 0, st, fa);
     MTL_HIP(hipGetLastError());
   }
   // 6. top-K + output rows
@@ -786,6 +848,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ta.out_n = out_n;
     ta.err = mu + 3;
     ta.maxabs = (const unsigned long long*)(mu + 10);
+    ta.p_rows = prows;
     ProfScope ps(s, "k_star_topk");
     hipLaunchKernelGGL(k_star_topk, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
     MTL_HIP(hipGetLastError());
@@ -795,6 +858,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
 
 static int32_t star_error(uint32_t err) {
   if (err & 1u) return set_error(MTL_ERR_PROCESSING, "processing error: look-back wait exceeded");
+  if (err & 64u) return set_error(MTL_ERR_PROCESSING, "processing error: inconsistent group map");
   if (err & 8u)
     return set_error(MTL_ERR_INVALID_ARGUMENT, "a key lies outside its stated [min, max]");
   if (err & (2u | 4u))

@@ -102,14 +102,14 @@ __device__ __forceinline__ void topk_push(TopkLds& s, uint64_t k0, uint64_t k1,
 }
 
 // Called by all threads after each admission round (<= TOPK_ROUND pushes).
-// Compacts as soon as the buffer holds more than max(TOPK_ROUND, k)
-// entries, so the first sort (which sets the threshold) is small; after it
-// most entries fail the threshold test and are never pushed.  Bound: after
-// a compaction n <= k, a round adds <= TOPK_ROUND, so n never exceeds
-// max(TOPK_ROUND, k) + TOPK_ROUND <= TOPK_CAP.
+// Compacts as soon as the buffer holds more than max(64, k) entries, so the
+// first sort (which sets the threshold) happens after the first round;
+// after it most entries fail the threshold test and are never pushed.
+// Bound: after a compaction n <= k, a round adds <= TOPK_ROUND, so n never
+// exceeds max(64, k) + TOPK_ROUND <= TOPK_CAP.
 __device__ __forceinline__ void topk_maybe_compact(TopkLds& s, uint32_t k) {
   __syncthreads();
-  const uint32_t limit = k > (uint32_t)TOPK_ROUND ? k : (uint32_t)TOPK_ROUND;
+  const uint32_t limit = k > 64u ? k : 64u;
   if (s.n > limit) {
     uint32_t n = s.n;
     __syncthreads();
@@ -147,18 +147,34 @@ __device__ __forceinline__ bool topk_publish_elect_merge(TopkLds& s, uint64_t* c
   if (!s_last) return false;
   __threadfence();
   topk_init(s);
+  // Each thread first loads up to E of the published candidates into
+  // registers (all loads in flight together), then the admission rounds run
+  // from registers: the merge costs one memory round trip per chunk of
+  // TOPK_ROUND * E candidates instead of one per round.
+  constexpr int E = 8;
   const uint32_t total = gridDim.x * k;
-  for (uint32_t base = 0; base < total; base += TOPK_ROUND) {
-    uint32_t i = base + threadIdx.x;
-    if (i < total) {
+  for (uint32_t chunk = 0; chunk < total; chunk += TOPK_ROUND * E) {
+    uint64_t c0[E], c1[E], c2[E], c3[E];
+    uint32_t ci[E];
+    bool ok[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      uint32_t i = chunk + e * TOPK_ROUND + threadIdx.x;
       uint32_t b = i / k, j = i - b * k;
-      uint32_t nb = __hip_atomic_load(&cn[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (j < nb) {
-        uint64_t k0 = ck[0][i];
-        if (k0 <= s.thr) topk_push(s, k0, ck[1][i], ck[2][i], ck[3][i], cid[i]);
-      }
+      ok[e] = i < total && j < __hip_atomic_load(&cn[b < gridDim.x ? b : 0], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t ii = ok[e] ? i : 0;
+      c0[e] = ck[0][ii];
+      c1[e] = ck[1][ii];
+      c2[e] = ck[2][ii];
+      c3[e] = ck[3][ii];
+      ci[e] = cid[ii];
     }
-    topk_maybe_compact(s, k);
+    for (int e = 0; e < E; ++e) {
+      if (chunk + e * TOPK_ROUND >= total) break;  // uniform
+      if (ok[e] && c0[e] <= s.thr) topk_push(s, c0[e], c1[e], c2[e], c3[e], ci[e]);
+      topk_maybe_compact(s, k);
+    }
   }
   __syncthreads();
   n = s.n;
