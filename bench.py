@@ -43,6 +43,37 @@ def kernel_profile(L):
     return out
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes recorded in profiles/traffic.json (already scaled by
+    the copy-kernel calibration there), or None when not recorded for this
+    kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload") != workload:
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes")
+
+
+def copy_calibration(n_bytes: int = 1 << 30, reps: int = 5) -> int:
+    """A plain device copy of known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters: run under rocprofv3 --pmc, each dispatch moves
+    n_bytes in and n_bytes out."""
+    import torch
+    src = torch.empty(n_bytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n_bytes, "reps": reps}), flush=True)
+    return 0
+
+
 def cpu_baseline(scale: float, seconds: float):
     """Oracle (C restatement, 1 thread) on a bounded sample of the workload;
     also checks the GPU result on the same sample."""
@@ -81,7 +112,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(sharding is described in DESIGN.md, not run)"}))
@@ -168,12 +203,16 @@ def main() -> int:
     per_kernel = {k: round(ms / max(c, 1) * 1e3, 2) for k, (ms, c) in prof.items()}
     dom = max(prof, key=lambda k: prof[k][0])
     dom_us = prof[dom][0] / prof[dom][1] * 1e3
-    kernel_bytes = {"k_fact": n_l * datagen.STAR_BYTES_PER_ROW["lines"],
-                    "k_filter_parent": n_o * datagen.STAR_BYTES_PER_ROW["orders"],
-                    "k_mark_dim": n_c * datagen.STAR_BYTES_PER_ROW["customers"]}
-    dom_bytes = kernel_bytes.get(dom, 0)
+    counts = datagen.star_counts(cust, orders, lines)
+    kbytes = datagen.star_kernel_bytes(n_c, n_o, n_l, counts)
+    dom_bytes = kbytes.get(dom, 0)
     achieved = dom_bytes / (dom_us * 1e-6) / 1e9
-    log(f"kernels (us/launch): {per_kernel}")
+    kernels = {k: {"us": per_kernel[k], "bytes": kbytes.get(k, 0),
+                   "GBs": round(kbytes.get(k, 0) / (per_kernel[k] * 1e-6) / 1e9, 1)
+                   if per_kernel[k] > 0 else None}
+               for k in per_kernel}
+    log(f"kernels: {kernels}")
+    traffic = measured_traffic(dom, f"lines {n_l}")
 
     result = {
         "metric": METRIC,
@@ -193,12 +232,12 @@ def main() -> int:
                    "rows": in_rows, "scale": args.scale, "k": 10},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "traffic": traffic, "algorithmic_bytes": dom_bytes,
                      "avg_launch_us": dom_us},
         "pipeline": {"algorithmic_bytes": alg_bytes, "device_ms": ms_per_step,
                      "achieved_GBs": alg_bytes / (ms_per_step * 1e-3) / 1e9,
                      "frac": alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "kernel_us": per_kernel},
+                     "kernels": kernels, "stage_rows": counts},
         "latency_ms": {"serial_p50": lat[len(lat) // 2], "serial_p99": lat[-1],
                        "queries_per_s_serial": 1e3 / (sum(lat) / len(lat)),
                        "queries_per_s_batched": args.steps / (dev_ms / 1e3)},

@@ -33,6 +33,34 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
   return v;
 }
 
+// Load at base + a 32-bit byte offset: with a wave-uniform base this is
+// one SGPR base plus a 32-bit VGPR offset per lane, instead of a 64-bit
+// VGPR address per lane (gathers keep many loads in flight at once).
+template <typename T>
+__device__ __forceinline__ T ld_off32(const T* base, uint32_t byte_off) {
+  return *(const T*)((const char*)base + byte_off);
+}
+
+// Block-wide maxima (blockDim.x <= 1024, a multiple of 64); every thread
+// of the block must call them and every thread gets the result.
+__device__ __forceinline__ uint64_t block_max_u64(uint64_t v) {
+  __shared__ uint64_t s_part[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off, WAVE);
+    v = o > v ? o : v;
+  }
+  __syncthreads();
+  if (threadIdx.x % WAVE == 0) s_part[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  uint64_t m = 0;
+  for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) m = s_part[w] > m ? s_part[w] : m;
+  return m;
+}
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v) {
+  return (uint32_t)block_max_u64(v);
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_inclusive_u32(uint32_t v) {
   int l = lane_id();

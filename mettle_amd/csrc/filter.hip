@@ -127,6 +127,136 @@ __device__ __forceinline__ void mark_word(const FilterArgs& a, uint64_t* win, ui
 // their bits are OR-ed first, so one atomic covers them.  The atomics do
 // not return: a repeated key is detected afterwards by comparing the
 // bitmap's popcount with the popcount of the pass bits (sel_bm).
+// ---------------------------------------------------------------------------
+// Narrow dense path (no input selection; at most one predicate; its column
+// and the probe and mark keys are at most 4 bytes wide, key spans < 2^32): the
+// three columns are loaded together, one round trip, into 32-bit registers,
+// and the probe bitmap is read as 32-bit words, so the kernel keeps enough
+// registers free for a high occupancy.
+// ---------------------------------------------------------------------------
+// Raw values (zero-extended) of the 16 dense rows of a lane; 0 past n.
+__device__ __forceinline__ void load_u32x16(const void* col, uint32_t width, uint64_t k0,
+                                            uint64_t n, bool vec, uint32_t (&x)[FJ][4]) {
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const uint64_t k = k0 + (uint64_t)j * 256;
+    if (vec && k + 3 < n) {
+      if (width == 4) {
+        uint4 u = *(const uint4*)((const uint32_t*)col + k);
+        x[j][0] = u.x; x[j][1] = u.y; x[j][2] = u.z; x[j][3] = u.w;
+      } else {
+        uint32_t u = *(const uint32_t*)((const uint8_t*)col + k);
+        x[j][0] = u & 0xFFu; x[j][1] = (u >> 8) & 0xFFu; x[j][2] = (u >> 16) & 0xFFu;
+        x[j][3] = u >> 24;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        x[j][v] = k + v < n ? (width == 4 ? ((const uint32_t*)col)[k + v]
+                                          : (uint32_t)((const uint8_t*)col)[k + v])
+                            : 0u;
+    }
+  }
+}
+
+// Raw narrow key -> offset from min, 0xFFFFFFFF when outside [0, span).
+__device__ __forceinline__ uint32_t narrow_off(uint32_t raw, int32_t type, int64_t min,
+                                               uint64_t span) {
+  const int64_t key = type == MTL_DICT8 ? (int64_t)raw : (int64_t)(int32_t)raw;
+  const uint64_t o = key_off(key, min);
+  return o < span ? (uint32_t)o : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t eval_narrow(const FilterArgs& a, uint64_t n, uint64_t k0,
+                                                uint32_t (&ms)[FJ][4]) {
+  uint32_t bits = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
+  uint32_t px[FJ][4], pk[FJ][4];
+  if (a.n_preds)
+    load_u32x16(a.preds[0].col, pred_width(a.preds[0].kind), k0, n, a.vec_ok & 1u, px);
+  if (a.probe_bm)
+    load_u32x16(a.probe_key, a.probe_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.probe_vec, pk);
+  if (a.mark_bm)
+    load_u32x16(a.mark_key, a.mark_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.mark_vec, ms);
+  if (a.probe_bm) {
+    const uint32_t* bm32 = (const uint32_t*)a.probe_bm;
+    uint32_t w[FJ][4];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        pk[j][v] = narrow_off(pk[j][v], a.probe_key_type, a.probe_min, a.probe_span);
+        w[j][v] = ld_off32(bm32, pk[j][v] != 0xFFFFFFFFu ? (pk[j][v] >> 5) << 2 : 0u);
+      }
+    if (a.n_preds) {
+#pragma unroll
+      for (int j = 0; j < FJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (!pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0]))
+            bits &= ~(1u << (4 * j + v));
+    }
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (pk[j][v] == 0xFFFFFFFFu || !((w[j][v] >> (pk[j][v] & 31)) & 1u))
+          bits &= ~(1u << (4 * j + v));
+  } else if (a.n_preds) {
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (!pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0]))
+          bits &= ~(1u << (4 * j + v));
+  }
+  if (a.mark_bm) {
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        ms[j][v] = narrow_off(ms[j][v], a.mark_key_type, a.mark_min, a.mark_span);
+  }
+  return bits;
+}
+
+// Mark the offsets of the kept rows (0xFFFFFFFF: key outside the range).
+__device__ __forceinline__ void mark_offsets(const FilterArgs& a, const uint32_t (&ms)[FJ][4],
+                                             uint32_t bits, uint64_t* win, uint64_t wbase) {
+  uint32_t e = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    uint32_t m = (bits >> (4 * j)) & 0xFu;
+    if (!m) continue;
+    uint64_t wj = ~0ull, bj = 0;
+    uint32_t rest = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (!((m >> v) & 1u)) continue;
+      const uint32_t s = ms[j][v];
+      if (s == 0xFFFFFFFFu) {
+        e |= 8u;
+        continue;
+      }
+      if (wj == ~0ull) wj = s >> 6;
+      if ((s >> 6) == wj) bj |= 1ull << (s & 63);
+      else rest |= 1u << v;
+    }
+    if (bj) mark_word(a, win, wbase, wj, bj);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (!((rest >> v) & 1u)) continue;
+      const uint32_t s = ms[j][v];
+      mark_word(a, win, wbase, s >> 6, 1ull << (s & 63));
+    }
+  }
+  if (e) atomicOr(a.err, e);
+}
+
 template <bool SEL>
 __device__ __forceinline__ void mark_rows(const FilterArgs& a, uint64_t n, uint64_t k0,
                                           const uint32_t (&rid)[FJ][4], uint32_t bits,
@@ -166,7 +296,7 @@ __device__ __forceinline__ void mark_rows(const FilterArgs& a, uint64_t n, uint6
 // Pass 1 of a filter: one tile of 4096 rows per block, no inter-block
 // dependency.  Writes the tile's pass bits to sel_bm (1 bit per row, bit r
 // for row r of the input order) and its count to tile_counts[tile].
-template <bool SEL>
+template <bool SEL, bool NARROW>
 __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
   __shared__ uint32_t s_wave_tot[FWAVES];
   const int tid = threadIdx.x;
@@ -176,20 +306,26 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
   const uint64_t n = SEL ? (a.in_count ? (uint64_t)*a.in_count : a.n_rows) : a.n_rows;
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
-  uint32_t rid[FJ][4];
+  uint32_t rid[FJ][4];  // row ids (generic path) or mark offsets (narrow path)
   uint32_t bits = 0;
-  if ((uint64_t)tile * FTILE < n) bits = eval_lane<SEL>(a, n, k0, rid);
-  else {
+  if ((uint64_t)tile * FTILE < n) {
+    if (NARROW) bits = eval_narrow(a, n, k0, rid);
+    else bits = eval_lane<SEL>(a, n, k0, rid);
+  } else {
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) rid[j][v] = 0;
+      for (int v = 0; v < 4; ++v) rid[j][v] = NARROW ? 0xFFFFFFFFu : 0u;
   }
   if (a.mark_bm) {
     __shared__ uint64_t win[kMarkWin];
     __shared__ uint64_t s_wbase;
     for (int i = tid; i < kMarkWin; i += FBLOCK) win[i] = 0;
-    if (tid == 0) {
+    if (NARROW) {
+      // the window starts at the word of the tile's first row (tid 0's
+      // first mark offset, already in registers)
+      if (tid == 0) s_wbase = rid[0][0] != 0xFFFFFFFFu ? (rid[0][0] >> 6) : 0;
+    } else if (tid == 0) {
       uint64_t base = 0;
       if ((uint64_t)tile * FTILE < n) {
         uint64_t r = SEL ? a.in_sel[(uint64_t)tile * FTILE] : (uint64_t)tile * FTILE;
@@ -200,7 +336,8 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
     }
     __syncthreads();
     const uint64_t wbase = s_wbase;
-    mark_rows<SEL>(a, n, k0, rid, bits, win, wbase);  // all lanes (wave sum)
+    if (NARROW) mark_offsets(a, rid, bits, win, wbase);
+    else mark_rows<SEL>(a, n, k0, rid, bits, win, wbase);
     __syncthreads();
     const uint64_t words = (a.mark_span + 63) / 64;
     for (int i = tid; i < kMarkWin; i += FBLOCK) {
@@ -332,8 +469,19 @@ int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st) {
   };
   a.probe_vec = key_vec(a.probe_key, a.probe_key_type);
   a.mark_vec = key_vec(a.mark_key, a.mark_key_type);
-  if (a.in_sel) hipLaunchKernelGGL(k_filter<true>, dim3(tiles), dim3(FBLOCK), 0, st, a);
-  else hipLaunchKernelGGL(k_filter<false>, dim3(tiles), dim3(FBLOCK), 0, st, a);
+  auto narrow_key = [](const void* col, int32_t type, uint64_t span) {
+    return !col || (type_width(type) <= 4 && span < (1ull << 32));
+  };
+  const bool narrow = !a.in_sel && a.n_preds <= 1 &&
+                      (a.n_preds == 0 || dev::pred_width(a.preds[0].kind) <= 4) &&
+                      (a.probe_bm || a.mark_bm) &&
+                      narrow_key(a.probe_bm ? a.probe_key : nullptr, a.probe_key_type,
+                                 a.probe_span) &&
+                      narrow_key(a.mark_bm ? a.mark_key : nullptr, a.mark_key_type, a.mark_span);
+  if (a.in_sel) hipLaunchKernelGGL((k_filter<true, false>), dim3(tiles), dim3(FBLOCK), 0, st, a);
+  else if (narrow)
+    hipLaunchKernelGGL((k_filter<false, true>), dim3(tiles), dim3(FBLOCK), 0, st, a);
+  else hipLaunchKernelGGL((k_filter<false, false>), dim3(tiles), dim3(FBLOCK), 0, st, a);
   MTL_HIP(hipGetLastError());
   return MTL_OK;
 }

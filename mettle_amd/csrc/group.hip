@@ -325,10 +325,8 @@ struct GTopkArgs {
   int32_t desc;
   int32_t avg;  // order by sum / count
   uint32_t k;
-  uint64_t* ck[4];
-  uint32_t* cid;
-  uint32_t* cn;
-  uint32_t* done;
+  TopkPublished pub;  // per-block lists
+  uint32_t nblocks;
   uint32_t* out_idx;
   uint32_t* out_n;
 };
@@ -368,24 +366,38 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint6
   k[3] = 0;
 }
 
+template <int R>
 __global__ __launch_bounds__(TOPK_BLOCK) void k_group_topk(GTopkArgs a) {
-  __shared__ TopkLds s;
-  topk_init(s);
+  __shared__ TopkMergeLds<R> ml;
+  WaveTopk<R> wl;
+  wl.init(a.k);
   const uint64_t chunk = (a.n + gridDim.x - 1) / gridDim.x;
   const uint64_t start = (uint64_t)blockIdx.x * chunk;
   const uint64_t end = start + chunk < a.n ? start + chunk : a.n;
-  for (uint64_t base = start; base < end; base += TOPK_ROUND) {
-    uint64_t g = base + threadIdx.x;
+  for (uint64_t base = start; base < end; base += TOPK_BLOCK) {
+    const uint64_t g = base + threadIdx.x;
+    uint64_t k[4] = {0, 0, 0, 0};
+    bool pre = false;
     if (g < end && a.st[g * a.W] != 0) {
-      uint64_t k[4];
       gtopk_keys(a, g, k);
-      if (k[0] <= s.thr) topk_push(s, k[0], k[1], k[2], k[3], (uint32_t)g);
+      pre = k[0] <= wl.thr0;
     }
-    topk_maybe_compact(s, a.k);
+    wl.offer(pre, k, (uint32_t)g);
   }
-  if (!topk_publish_elect_merge(s, a.ck, a.cid, a.cn, a.done, a.k)) return;
-  for (uint32_t i = threadIdx.x; i < s.n; i += blockDim.x) a.out_idx[i] = s.id[i];
-  if (threadIdx.x == 0) *a.out_n = s.n;
+  topk_block_merge(wl, ml);
+  topk_publish(wl, a.pub, blockIdx.x);
+}
+
+template <int R>
+__global__ __launch_bounds__(TOPK_BLOCK) void k_group_topk_merge(GTopkArgs a) {
+  __shared__ TopkMergeLds<R> ml;
+  __shared__ uint32_t ids[R * 64];
+  __shared__ uint32_t nn;
+  WaveTopk<R> wl;
+  topk_merge_published(wl, ml, a.pub, a.nblocks, a.k);
+  topk_to_lds(wl, ids, &nn);
+  for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) a.out_idx[i] = ids[i];
+  if (threadIdx.x == 0) *a.out_n = nn;
 }
 
 }  // namespace mtl
@@ -769,15 +781,30 @@ int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, i
   t.desc = desc;
   t.avg = g->agg_func[agg] == MTL_AGG_AVG;
   t.k = k;
-  for (int f = 0; f < 4; ++f) t.ck[f] = (uint64_t*)ck[f];
-  t.cid = (uint32_t*)cid;
-  t.cn = (uint32_t*)cn;
-  t.done = (uint32_t*)misc;
+  for (int f = 0; f < 4; ++f) t.pub.ck[f] = (uint64_t*)ck[f];
+  t.pub.cid = (uint32_t*)cid;
+  t.pub.cn = (uint32_t*)cn;
+  t.nblocks = grid;
   t.out_idx = (uint32_t*)oidx;
   t.out_n = (uint32_t*)misc + 1;
   {
     ProfScope ps(s, "k_group_topk");
-    hipLaunchKernelGGL(k_group_topk, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+    if (k <= 64)
+      hipLaunchKernelGGL(k_group_topk<1>, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+    else if (k <= 128)
+      hipLaunchKernelGGL(k_group_topk<2>, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+    else
+      hipLaunchKernelGGL(k_group_topk<8>, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+  }
+  MTL_HIP(hipGetLastError());
+  {
+    ProfScope ps(s, "k_group_topk_merge");
+    if (k <= 64)
+      hipLaunchKernelGGL(k_group_topk_merge<1>, dim3(1), dim3(TOPK_BLOCK), 0, s->stream, t);
+    else if (k <= 128)
+      hipLaunchKernelGGL(k_group_topk_merge<2>, dim3(1), dim3(TOPK_BLOCK), 0, s->stream, t);
+    else
+      hipLaunchKernelGGL(k_group_topk_merge<8>, dim3(1), dim3(TOPK_BLOCK), 0, s->stream, t);
   }
   MTL_HIP(hipGetLastError());
   void* host;

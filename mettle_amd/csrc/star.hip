@@ -188,54 +188,12 @@ __global__ void k_map(MapArgs a) {
 // ---------------------------------------------------------------------------
 // Expression program (postfix), evaluated only for qualifying fact rows.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int64_t expr_i64(const mtl_expr& e, uint64_t row,
-                                            bool& ovf) {
-  int64_t st[8];
-  int sp = 0;
-  for (uint32_t i = 0; i < e.n_ops; ++i) {
-    int op = e.op[i];
-    if (op == MTL_EXPR_COL) {
-      int c = (int)e.arg[i];
-      st[sp++] = load_key(e.col[c], e.col_type[c], row);
-    } else if (op == MTL_EXPR_CONST) {
-      st[sp++] = e.arg[i];
-    } else {
-      int64_t b = st[--sp], a = st[--sp], r;
-      bool o;
-      if (op == MTL_EXPR_ADD) o = __builtin_add_overflow(a, b, &r);
-      else if (op == MTL_EXPR_SUB) o = __builtin_sub_overflow(a, b, &r);
-      else o = __builtin_mul_overflow(a, b, &r);
-      ovf |= o;
-      st[sp++] = r;
-    }
-  }
-  return sp ? st[sp - 1] : 0;
-}
-
 __device__ __forceinline__ double load_f64(const void* col, int32_t type, uint64_t row) {
   switch (type) {
     case MTL_FLOAT: return (double)((const float*)col)[row];
     case MTL_DOUBLE: return ((const double*)col)[row];
     default: return (double)load_key(col, type, row);
   }
-}
-
-__device__ __forceinline__ double expr_f64(const mtl_expr& e, uint64_t row) {
-  double st[8];
-  int sp = 0;
-  for (uint32_t i = 0; i < e.n_ops; ++i) {
-    int op = e.op[i];
-    if (op == MTL_EXPR_COL) {
-      int c = (int)e.arg[i];
-      st[sp++] = load_f64(e.col[c], e.col_type[c], row);
-    } else if (op == MTL_EXPR_CONST) {
-      st[sp++] = __longlong_as_double(e.arg[i]);
-    } else {
-      double b = st[--sp], a = st[--sp];
-      st[sp++] = op == MTL_EXPR_ADD ? a + b : (op == MTL_EXPR_SUB ? a - b : a * b);
-    }
-  }
-  return sp ? st[sp - 1] : 0.0;
 }
 
 struct FactArgs {
@@ -253,68 +211,102 @@ struct FactArgs {
   uint32_t* cnt;
   uint32_t* err;
   uint32_t fk_vec;  // 1: fk is 16-byte aligned
-  unsigned long long* maxabs;  // max |value| over accumulated rows
-  uint32_t* hit_row;    // hit buffer (per-block regions of tpb*4096 rows)
+  uint32_t pred0_vec;  // 1: the first predicate's column is 16-byte aligned
+  uint32_t* hit_row;    // hit buffer: 1024-entry region per wave of each tile
   uint32_t* hit_slot;   // key slot of each hit
-  uint32_t* hit_count;  // hits per block
+  uint32_t* hit_count;  // hits per wave region
+  uint64_t n_regions;   // wave regions (tiles x 4)
+  unsigned long long* tmax;  // per k_fact_acc block: max |value| it added
 };
 
-// Expression over preloaded column values (one row).
-__device__ __forceinline__ int64_t expr_i64_vals(const mtl_expr& e, const int64_t* cv,
+// Expression over preloaded column values (one row).  The operand stack
+// lives in eight registers that shift on push and pop (the host checks
+// depth <= 8), and column values are picked with a static select, so no
+// dynamically indexed array spills to scratch memory.
+template <typename V>
+__device__ __forceinline__ V pick_col(const V (&cv)[MTL_EXPR_MAX_COLS], int64_t c) {
+  V v = cv[0];
+#pragma unroll
+  for (int q = 1; q < MTL_EXPR_MAX_COLS; ++q)
+    if (c == q) v = cv[q];
+  return v;
+}
+
+template <typename V>
+struct ShiftStack {
+  V s[8];
+  __device__ __forceinline__ void push(V v) {
+#pragma unroll
+    for (int i = 7; i > 0; --i) s[i] = s[i - 1];
+    s[0] = v;
+  }
+  // replace the top two entries (s1 op s0) by r
+  __device__ __forceinline__ void fold(V r) {
+    s[0] = r;
+#pragma unroll
+    for (int i = 1; i < 7; ++i) s[i] = s[i + 1];
+  }
+};
+
+__device__ __forceinline__ int64_t expr_i64_vals(const mtl_expr& e,
+                                                 const int64_t (&cv)[MTL_EXPR_MAX_COLS],
                                                  bool& ovf) {
-  int64_t st[8];
-  int sp = 0;
+  ShiftStack<int64_t> st;
   for (uint32_t i = 0; i < e.n_ops; ++i) {
-    int op = e.op[i];
+    const int op = e.op[i];
     if (op == MTL_EXPR_COL) {
-      st[sp++] = cv[e.arg[i]];
+      st.push(pick_col(cv, e.arg[i]));
     } else if (op == MTL_EXPR_CONST) {
-      st[sp++] = e.arg[i];
+      st.push(e.arg[i]);
     } else {
-      int64_t b = st[--sp], a = st[--sp], r;
+      const int64_t y = st.s[0], x = st.s[1];
+      int64_t r;
       bool o;
-      if (op == MTL_EXPR_ADD) o = __builtin_add_overflow(a, b, &r);
-      else if (op == MTL_EXPR_SUB) o = __builtin_sub_overflow(a, b, &r);
-      else o = __builtin_mul_overflow(a, b, &r);
+      if (op == MTL_EXPR_ADD) o = __builtin_add_overflow(x, y, &r);
+      else if (op == MTL_EXPR_SUB) o = __builtin_sub_overflow(x, y, &r);
+      else o = __builtin_mul_overflow(x, y, &r);
       ovf |= o;
-      st[sp++] = r;
+      st.fold(r);
     }
   }
-  return sp ? st[sp - 1] : 0;
+  return st.s[0];
 }
 
-__device__ __forceinline__ double expr_f64_vals(const mtl_expr& e, const double* cv) {
-  double st[8];
-  int sp = 0;
+__device__ __forceinline__ double expr_f64_vals(const mtl_expr& e,
+                                                const double (&cv)[MTL_EXPR_MAX_COLS]) {
+  ShiftStack<double> st;
   for (uint32_t i = 0; i < e.n_ops; ++i) {
-    int op = e.op[i];
+    const int op = e.op[i];
     if (op == MTL_EXPR_COL) {
-      st[sp++] = cv[e.arg[i]];
+      st.push(pick_col(cv, e.arg[i]));
     } else if (op == MTL_EXPR_CONST) {
-      st[sp++] = __longlong_as_double(e.arg[i]);
+      st.push(__longlong_as_double(e.arg[i]));
     } else {
-      double b = st[--sp], a = st[--sp];
-      st[sp++] = op == MTL_EXPR_ADD ? a + b : (op == MTL_EXPR_SUB ? a - b : a * b);
+      const double y = st.s[0], x = st.s[1];
+      st.fold(op == MTL_EXPR_ADD ? x + y : (op == MTL_EXPR_SUB ? x - y : x * y));
     }
   }
-  return sp ? st[sp - 1] : 0.0;
+  return st.s[0];
 }
 
-// The fact pass runs as two compact kernels over the same persistent grid
-// (each block owns a contiguous range of 4096-row tiles):
-//   k_fact_scan : stream the foreign key, probe the parent bitmap, evaluate
-//                 the fact predicates on the hits, and append the hit rows
-//                 (with their key slot) to the block's own region of the
-//                 hit buffer (region capacity = the block's row count, so
-//                 no global atomics and no inter-block dependency);
-//   k_fact_acc  : one hit per lane: rank, expression columns and the value
-//                 are loaded/computed with all lanes in parallel, then added
-//                 into an LDS window of accumulator slots that starts at the
-//                 block's first hit (facts clustered by foreign key keep a
-//                 block's slots in a narrow range); each touched slot is
-//                 flushed once with a coalesced global atomic, and slots
-//                 outside the window go straight to global atomics.
-constexpr int kAccWin = 4096;  // LDS accumulator slots (48 KB with counts)
+// The fact pass, two kernels:
+//   k_fact_scan (one 4096-row tile per block, no barrier): load the
+//     foreign keys and the first fact predicate's column together (dense
+//     16-byte loads: rows are read once, in one round trip, instead of a
+//     second dependent round for the rows that found a parent), probe the
+//     parent bitmap (the parent key order of the facts keeps a wave's probes
+//     within a few cache lines), evaluate the predicates, and write each
+//     wave's hits (row, key slot) to that wave's own 1024-entry region of
+//     the hit buffer with a wave-local prefix sum, plus the wave's count;
+//   k_fact_acc (kAccTiles tiles per block): the block's hits are numbered
+//     across its tiles' wave regions, one hit per thread: the slot rank
+//     (prefix popcount of the parent bitmap), the expression columns and
+//     the value are loaded and computed in parallel and added to the slot
+//     with global atomics (hits are sparse and spread over consecutive
+//     slots).  Each block writes the max |value| it added to tmax[block]
+//     (no shared counter); the top-K merge reduces it for the overflow bound.
+constexpr int kAccTiles = 8;
+constexpr int kAccRegions = kAccTiles * FWAVES;
 
 template <typename KT>
 __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32_t (&s)[FJ][4]) {
@@ -331,139 +323,113 @@ __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32
   }
 }
 
-__device__ __forceinline__ void fact_block_range(const FactArgs& a, uint64_t& tb, uint64_t& te) {
-  const uint64_t tiles = (a.n + FTILE - 1) / FTILE;
-  const uint64_t tpb = (tiles + gridDim.x - 1) / gridDim.x;
-  tb = (uint64_t)blockIdx.x * tpb;
-  te = tb + tpb < tiles ? tb + tpb : tiles;
-}
-
 __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
-  __shared__ uint32_t s_wave[FWAVES];
-  __shared__ uint32_t s_count;
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  uint64_t tb, te;
-  fact_block_range(a, tb, te);
-  if (tid == 0) s_count = 0;
-  const uint64_t region = tb * FTILE;  // this block's region of the hit buffer
-  for (uint64_t t = tb; t < te; ++t) {
-    const uint64_t k0 = t * FTILE + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
-    uint32_t s[FJ][4];
-    switch (a.fk_type) {
-      case MTL_BIGINT:
-      case MTL_TIMESTAMP:
-      case MTL_DECIMAL: fact_keys<int64_t>(a, k0, s); break;
-      case MTL_DICT8: fact_keys<uint8_t>(a, k0, s); break;
-      default: fact_keys<int32_t>(a, k0, s); break;
-    }
-    uint32_t hits = 0;
-    {
-      uint64_t w[FJ][4];
+  const uint64_t k0 = (uint64_t)blockIdx.x * FTILE + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
+  // first predicate's column, loaded with the keys
+  uint64_t px[FJ][4];
+  if (a.n_preds) {
+    const uint32_t width = pred_width(a.preds[0].kind);
 #pragma unroll
-      for (int j = 0; j < FJ; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) w[j][v] = a.bm[s[j][v] != 0xFFFFFFFFu ? (s[j][v] >> 6) : 0];
-#pragma unroll
-      for (int j = 0; j < FJ; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          bool in = k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu;
-          if (in && ((w[j][v] >> (s[j][v] & 63)) & 1ull)) hits |= 1u << (4 * j + v);
-        }
-    }
-    for (uint32_t p = 0; p < a.n_preds && hits; ++p) pred_rows(a.preds[p], k0, hits);
-    // append the tile's hits (block-local positions)
-    const uint32_t c = (uint32_t)__popc(hits);
-    const uint32_t incl = wave_inclusive_u32(c);
-    if (lane == WAVE - 1) s_wave[wave] = incl;
-    __syncthreads();
-    uint32_t pos = s_count + incl - c, tot = 0;
-#pragma unroll
-    for (int w = 0; w < FWAVES; ++w) {
-      if (w < wave) pos += s_wave[w];
-      tot += s_wave[w];
-    }
-    uint32_t h = hits;
-    while (h) {
-      int i = __ffs(h) - 1;
-      h &= h - 1;
-      const int j = i >> 2, v = i & 3;
-      a.hit_row[region + pos] = (uint32_t)(k0 + (uint64_t)j * 256 + v);
-      a.hit_slot[region + pos] = s[j][v];
-      ++pos;
-    }
-    __syncthreads();
-    if (tid == 0) s_count += tot;
+    for (int j = 0; j < FJ; ++j)
+      load_raw4(a.preds[0].col, width, k0 + (uint64_t)j * 256, a.n, a.pred0_vec != 0, px[j]);
   }
-  __syncthreads();
-  if (tid == 0) a.hit_count[blockIdx.x] = s_count;
+  uint32_t s[FJ][4];
+  switch (a.fk_type) {
+    case MTL_BIGINT:
+    case MTL_TIMESTAMP:
+    case MTL_DECIMAL: fact_keys<int64_t>(a, k0, s); break;
+    case MTL_DICT8: fact_keys<uint8_t>(a, k0, s); break;
+    default: fact_keys<int32_t>(a, k0, s); break;
+  }
+  uint32_t hits = 0;
+  {
+    uint64_t w[FJ][4];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        w[j][v] = ld_off32(a.bm, s[j][v] != 0xFFFFFFFFu ? (s[j][v] >> 6) << 3 : 0u);
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        bool in = k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu;
+        if (in && ((w[j][v] >> (s[j][v] & 63)) & 1ull)) hits |= 1u << (4 * j + v);
+      }
+  }
+  if (a.n_preds) {
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (!pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0])) hits &= ~(1u << (4 * j + v));
+  }
+  for (uint32_t p = 1; p < a.n_preds && hits; ++p) pred_rows(a.preds[p], k0, hits);
+  // the wave's hits, in row order, to the wave's region
+  const uint32_t c = (uint32_t)__popc(hits);
+  const uint32_t incl = wave_inclusive_u32(c);
+  const uint64_t region = (uint64_t)blockIdx.x * FTILE + (uint64_t)wave * (FJ * 256);
+  uint32_t p = incl - c;
+#pragma unroll
+  for (int i = 0; i < FJ * 4; ++i) {  // static indices: s stays in registers
+    if (!((hits >> i) & 1u)) continue;
+    a.hit_row[region + p] = (uint32_t)(k0 + (uint64_t)(i >> 2) * 256 + (i & 3));
+    a.hit_slot[region + p] = s[i >> 2][i & 3];
+    ++p;
+  }
+  if (lane == WAVE - 1) a.hit_count[(uint64_t)blockIdx.x * FWAVES + wave] = incl;
 }
 
 template <bool DBL>
 __global__ __launch_bounds__(FBLOCK) void k_fact_acc(FactArgs a) {
-  __shared__ uint64_t wacc[kAccWin];
-  __shared__ uint32_t wcnt[kAccWin];
-  __shared__ uint32_t s_wbase;
-  uint64_t tb, te;
-  fact_block_range(a, tb, te);
-  const uint32_t nh = tb < te ? a.hit_count[blockIdx.x] : 0u;
-  if (nh == 0) return;
-  const uint64_t region = tb * FTILE;
-  for (int i = threadIdx.x; i < kAccWin; i += FBLOCK) {
-    wacc[i] = 0;
-    wcnt[i] = 0;
-  }
-  if (threadIdx.x == 0) {
-    const uint32_t sv = a.hit_slot[region];
-    const uint64_t w = a.bm[sv >> 6];
-    const uint64_t below = (sv & 63) ? (w & ((1ull << (sv & 63)) - 1)) : 0ull;
-    s_wbase = a.rank[sv >> 6] + (uint32_t)__popcll(below);
+  __shared__ uint32_t s_off[kAccRegions + 1];
+  const int tid = threadIdx.x;
+  const uint64_t r0 = (uint64_t)blockIdx.x * kAccRegions;  // first wave region
+  if (tid < WAVE) {
+    const uint64_t r = r0 + tid;
+    const uint32_t cnt = (tid < kAccRegions && r < a.n_regions) ? a.hit_count[r] : 0u;
+    const uint32_t incl = wave_inclusive_u32(cnt);
+    if (tid <= kAccRegions) s_off[tid] = incl - cnt;
   }
   __syncthreads();
-  const uint32_t wbase = s_wbase;
+  const uint32_t total = s_off[kAccRegions];
   bool ovf = false;
   uint64_t mx = 0;
   using V = typename std::conditional<DBL, double, int64_t>::type;
-  for (uint32_t i = threadIdx.x; i < nh; i += FBLOCK) {
-    const uint32_t row = a.hit_row[region + i];
-    const uint32_t sv = a.hit_slot[region + i];
+  for (uint32_t e = tid; e < total; e += FBLOCK) {
+    uint32_t q = 0;
+#pragma unroll
+    for (int i = 1; i < kAccRegions; ++i) q += e >= s_off[i] ? 1u : 0u;
+    const uint64_t at = (r0 + q) * (FJ * 256) + (e - s_off[q]);
+    const uint32_t row = a.hit_row[at], sv = a.hit_slot[at];
     V cv[MTL_EXPR_MAX_COLS];
 #pragma unroll
     for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
       if ((uint32_t)c < a.expr.n_cols) {
         if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
         else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
+      } else {
+        cv[c] = 0;
       }
     const uint64_t w = a.bm[sv >> 6];
     const uint64_t below = (sv & 63) ? (w & ((1ull << (sv & 63)) - 1)) : 0ull;
     const uint32_t r = a.rank[sv >> 6] + (uint32_t)__popcll(below);
-    const uint32_t wi = r - wbase;
-    const bool local = wi < (uint32_t)kAccWin;
     if constexpr (DBL) {
-      double val = expr_f64_vals(a.expr, cv);
-      if (local) unsafeAtomicAdd((double*)&wacc[wi], val);
-      else unsafeAtomicAdd((double*)&a.acc[r], val);
+      unsafeAtomicAdd((double*)&a.acc[r], expr_f64_vals(a.expr, cv));
     } else {
-      int64_t val = expr_i64_vals(a.expr, cv, ovf);
-      uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
+      const int64_t val = expr_i64_vals(a.expr, cv, ovf);
+      const uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
       mx = av > mx ? av : mx;
-      atomicAdd(local ? (unsigned long long*)&wacc[wi] : (unsigned long long*)&a.acc[r],
-                (unsigned long long)val);
+      atomicAdd((unsigned long long*)&a.acc[r], (unsigned long long)val);
     }
-    atomicAdd(local ? &wcnt[wi] : &a.cnt[r], 1u);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kAccWin; i += FBLOCK) {
-    uint32_t c = wcnt[i];
-    if (!c) continue;
-    if constexpr (DBL)
-      unsafeAtomicAdd((double*)&a.acc[wbase + i], __longlong_as_double((long long)wacc[i]));
-    else
-      atomicAdd((unsigned long long*)&a.acc[wbase + i], (unsigned long long)wacc[i]);
-    atomicAdd(&a.cnt[wbase + i], c);
+    atomicAdd(&a.cnt[r], 1u);
   }
   if (ovf) atomicOr(a.err, 16u);
-  if (mx) atomicMax(a.maxabs, (unsigned long long)mx);
+  if (!DBL) {
+    mx = block_max_u64(mx);
+    if (tid == 0) a.tmax[blockIdx.x] = mx;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -480,15 +446,14 @@ struct TopkArgs {
   const void* tie_col[MTL_STAR_MAX_TIES];
   int32_t tie_type[MTL_STAR_MAX_TIES];
   int32_t tie_desc[MTL_STAR_MAX_TIES];
-  // per-block results
-  uint64_t* ck[4];
-  uint32_t* cid;
-  uint32_t* cn;
-  uint32_t* done;
+  TopkPublished pub;  // per-block lists
+  uint32_t nblocks;
   mtl_star_row* out;
   uint32_t* out_n;  // [0] = rows, [1] = error bits copied from err
   uint32_t* err;
-  const unsigned long long* maxabs;
+  const unsigned long long* tmax;  // per fact tile: max |value| added
+  uint32_t n_tmax;
+  uint32_t* bcmax;                 // per top-K block: max count of its slots
   uint64_t p_rows;  // row_of entries are checked against it before use
 };
 
@@ -511,14 +476,17 @@ __device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t r, uint64_
   }
 }
 
+template <int R>
 __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
-  __shared__ TopkLds s;
-  topk_init(s);
+  __shared__ TopkMergeLds<R> ml;
+  WaveTopk<R> wl;
+  wl.init(a.k);
   const uint32_t m = *a.m_count;
   const uint32_t chunk = (m + gridDim.x - 1) / gridDim.x;
   const uint32_t start = blockIdx.x * chunk;
   const uint32_t end = min(m, start + chunk);
   constexpr int E = 8;  // entries per thread per pass, loaded together
+  uint32_t cmax = 0;
   for (uint32_t base = start; base < end; base += TOPK_BLOCK * E) {
     uint32_t cv[E];
     uint64_t av[E];
@@ -528,36 +496,51 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
       cv[e] = r < end ? a.cnt[r] : 0u;
       av[e] = r < end ? a.acc[r] : 0ull;
     }
-    if (!a.is_double) {
-      // |SUM| <= count x max|value|: flag groups where that bound exceeds
-      // int64 (the int64 accumulator could have wrapped)
-      const unsigned long long mx = *a.maxabs;
-      bool big = false;
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        unsigned long long prod;
-        big |= __builtin_mul_overflow((unsigned long long)cv[e], mx, &prod) ||
-               prod > 0x7FFFFFFFFFFFFFFFull;
-      }
-      if (big) atomicOr(a.err, 32u);
-    }
+    for (int e = 0; e < E; ++e) cmax = cv[e] > cmax ? cv[e] : cmax;
+#pragma unroll
     for (int e = 0; e < E; ++e) {
-      if (cv[e] != 0) {
-        uint64_t k0 = a.is_double ? ~enc_f64(__longlong_as_double((long long)av[e]))
-                                  : ~enc_i64((int64_t)av[e]);
-        if (k0 <= s.thr) {
-          uint32_t r = base + e * TOPK_BLOCK + threadIdx.x;
-          uint64_t k[4];
-          star_keys(a, r, k);
-          topk_push(s, k[0], k[1], k[2], k[3], r);
-        }
-      }
-      topk_maybe_compact(s, a.k);
+      const uint32_t r = base + e * TOPK_BLOCK + threadIdx.x;
+      uint64_t k[4] = {0, 0, 0, 0};
+      k[0] = a.is_double ? ~enc_f64(__longlong_as_double((long long)av[e]))
+                         : ~enc_i64((int64_t)av[e]);
+      const bool pre = cv[e] != 0 && k[0] <= wl.thr0;
+      if (pre) star_keys(a, r, k);
+      wl.offer(pre, k, r);
     }
   }
-  if (!topk_publish_elect_merge(s, a.ck, a.cid, a.cn, a.done, a.k)) return;
-  for (uint32_t i = threadIdx.x; i < s.n; i += blockDim.x) {
-    uint32_t r = s.id[i];
+  topk_block_merge(wl, ml);
+  topk_publish(wl, a.pub, blockIdx.x);
+  cmax = block_max_u32(cmax);
+  if (threadIdx.x == 0) a.bcmax[blockIdx.x] = cmax;
+}
+
+// One block: merge the published block lists and write the result rows.
+template <int R>
+__global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk_merge(TopkArgs a) {
+  __shared__ TopkMergeLds<R> ml;
+  __shared__ uint32_t ids[R * 64];
+  __shared__ uint32_t nn;
+  WaveTopk<R> wl;
+  // |SUM| <= count x max|value| for every group: flag the run when that
+  // bound exceeds int64 (an int64 accumulator could have wrapped)
+  if (!a.is_double) {
+    uint64_t mx = 0;
+    uint32_t cm = 0;
+    for (uint32_t i = threadIdx.x; i < a.n_tmax; i += TOPK_BLOCK) mx = a.tmax[i] > mx ? a.tmax[i] : mx;
+    for (uint32_t i = threadIdx.x; i < a.nblocks; i += TOPK_BLOCK) cm = a.bcmax[i] > cm ? a.bcmax[i] : cm;
+    mx = block_max_u64(mx);
+    cm = block_max_u32(cm);
+    unsigned long long prod;
+    if (threadIdx.x == 0 &&
+        (__builtin_mul_overflow((unsigned long long)cm, (unsigned long long)mx, &prod) ||
+         prod > 0x7FFFFFFFFFFFFFFFull))
+      atomicOr(a.err, 32u);
+  }
+  topk_merge_published(wl, ml, a.pub, a.nblocks, a.k);
+  topk_to_lds(wl, ids, &nn);
+  for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) {
+    uint32_t r = ids[i];
     mtl_star_row row;
     row.p_row = a.row_of[r];
     row.count = a.cnt[r];
@@ -567,7 +550,7 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
     a.out[i] = row;
   }
   if (threadIdx.x == 0) {
-    a.out_n[0] = s.n;
+    a.out_n[0] = nn;
     a.out_n[1] = __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -673,7 +656,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   const uint64_t prows = d->p_rows;
   void *dim_bm = nullptr, *dbits = nullptr, *par_bm, *par_rank, *pbits, *row_of, *acc, *cnt,
        *misc;
-  void *st_r, *ck[4], *cid, *cn;
+  void *st_r, *ck[4], *cid, *cn, *bcmax, *tmax;
   if (d->d_rows) {
     MTL_TRY(ar.alloc(sp.dwords * 8, &dim_bm));
     MTL_TRY(ar.alloc((uint64_t)filter_tiles(d->d_rows) * kTileRows / 8, &dbits));
@@ -685,15 +668,19 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &row_of));
   MTL_TRY(ar.alloc((prows ? prows : 1) * 8, &acc));
   MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &cnt));
-  // misc words: [2] rank ticket, [3] error bits, [4] top-K done counter,
+  // misc words: [2] rank ticket, [3] error bits,
   // [6] parent bitmap popcount, [7] P pass-bit popcount, [8] dimension
-  // bitmap popcount, [9] D pass-bit popcount, [10..11] max |value| (u64)
+  // bitmap popcount, [9] D pass-bit popcount
   MTL_TRY(ar.alloc(256, &misc));
   uint32_t rtiles = (uint32_t)ceil_div(sp.pwords, RTILE);
   MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(rtiles, 1) * 8, &st_r));
   for (int f = 0; f < 4; ++f) MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * d->k * 8, &ck[f]));
   MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * d->k * 4, &cid));
   MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * 4, &cn));
+  MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * 4, &bcmax));
+  const uint32_t ftiles = d->f_rows ? filter_tiles(d->f_rows) : 0u;
+  const uint32_t acc_blocks = (uint32_t)ceil_div(ftiles, kAccTiles);
+  MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(acc_blocks, 1) * 8, &tmax));
   uint32_t* mu = (uint32_t*)misc;
 
   ZeroList zl;
@@ -801,28 +788,26 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.cnt = (uint32_t*)cnt;
     fa.err = mu + 3;
     fa.fk_vec = ((uintptr_t)d->f_fk % 16) == 0;
-    fa.maxabs = (unsigned long long*)(mu + 10);
+    fa.tmax = (unsigned long long*)tmax;
+    fa.pred0_vec = d->f_n_preds && ((uintptr_t)d->f_preds[0].column % 16) == 0;
     void *hrow, *hslot, *hcnt;
-    const uint64_t hcap = ceil_div(d->f_rows, FTILE) * FTILE;
-    MTL_TRY(ar.alloc(hcap * 4, &hrow));
-    MTL_TRY(ar.alloc(hcap * 4, &hslot));
-    MTL_TRY(ar.alloc((uint64_t)cu_count() * 7 * 4 + 256, &hcnt));
+    MTL_TRY(ar.alloc((uint64_t)ftiles * FTILE * 4, &hrow));
+    MTL_TRY(ar.alloc((uint64_t)ftiles * FTILE * 4, &hslot));
+    MTL_TRY(ar.alloc((uint64_t)ftiles * FWAVES * 4, &hcnt));
     fa.hit_row = (uint32_t*)hrow;
     fa.hit_slot = (uint32_t*)hslot;
     fa.hit_count = (uint32_t*)hcnt;
-    uint64_t g = ceil_div(d->f_rows, FTILE);
-    uint64_t gmax = (uint64_t)cu_count() * 7;  // persistent: 7 blocks per CU
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min(g, gmax));
+    fa.n_regions = (uint64_t)ftiles * FWAVES;
     {
       ProfScope ps(s, "k_fact_scan");
-      hipLaunchKernelGGL(k_fact_scan, dim3(grid), dim3(FBLOCK), 0, st, fa);
+      hipLaunchKernelGGL(k_fact_scan, dim3(ftiles), dim3(FBLOCK), 0, st, fa);
     }
     MTL_HIP(hipGetLastError());
     ProfScope ps(s, "k_fact_acc");
     if (d->agg.is_double)
-      hipLaunchKernelGGL(k_fact_acc<true>, dim3(grid), dim3(FBLOCK), 0, st, fa);
+      hipLaunchKernelGGL(k_fact_acc<true>, dim3(acc_blocks), dim3(FBLOCK), 0, st, fa);
     else
-      hipLaunchKernelGGL(k_fact_acc<false>, dim3(grid), dim3(FBLOCK), 0, st, fa);
+      hipLaunchKernelGGL(k_fact_acc<false>, dim3(acc_blocks), dim3(FBLOCK), 0, st, fa);
     MTL_HIP(hipGetLastError());
   }
   // 6. top-K + output rows
@@ -840,17 +825,34 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
       ta.tie_type[t] = d->tie_type[t];
       ta.tie_desc[t] = d->tie_desc[t];
     }
-    for (int f = 0; f < 4; ++f) ta.ck[f] = (uint64_t*)ck[f];
-    ta.cid = (uint32_t*)cid;
-    ta.cn = (uint32_t*)cn;
-    ta.done = mu + 4;
+    for (int f = 0; f < 4; ++f) ta.pub.ck[f] = (uint64_t*)ck[f];
+    ta.pub.cid = (uint32_t*)cid;
+    ta.pub.cn = (uint32_t*)cn;
+    ta.nblocks = sp.topk_grid;
     ta.out = out;
     ta.out_n = out_n;
     ta.err = mu + 3;
-    ta.maxabs = (const unsigned long long*)(mu + 10);
+    ta.tmax = (const unsigned long long*)tmax;
+    ta.n_tmax = d->agg.is_double ? 0u : acc_blocks;
+    ta.bcmax = (uint32_t*)bcmax;
     ta.p_rows = prows;
-    ProfScope ps(s, "k_star_topk");
-    hipLaunchKernelGGL(k_star_topk, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
+    {
+      ProfScope ps(s, "k_star_topk");
+      if (d->k <= 64)
+        hipLaunchKernelGGL(k_star_topk<1>, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
+      else if (d->k <= 128)
+        hipLaunchKernelGGL(k_star_topk<2>, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
+      else
+        hipLaunchKernelGGL(k_star_topk<8>, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
+    }
+    MTL_HIP(hipGetLastError());
+    ProfScope ps2(s, "k_star_topk_merge");
+    if (d->k <= 64)
+      hipLaunchKernelGGL(k_star_topk_merge<1>, dim3(1), dim3(TOPK_BLOCK), 0, st, ta);
+    else if (d->k <= 128)
+      hipLaunchKernelGGL(k_star_topk_merge<2>, dim3(1), dim3(TOPK_BLOCK), 0, st, ta);
+    else
+      hipLaunchKernelGGL(k_star_topk_merge<8>, dim3(1), dim3(TOPK_BLOCK), 0, st, ta);
     MTL_HIP(hipGetLastError());
   }
   return MTL_OK;

@@ -1,13 +1,20 @@
-// Block-level top-K over candidates keyed by up to four 64-bit words plus a
-// 32-bit id, lexicographically smaller = better.  Used by the star pipeline
-// and by the grouped-aggregate top-K.
+// Top-K over candidates keyed by four 64-bit words plus a 32-bit id,
+// lexicographically smaller = better (ids are unique, so the order is
+// total).  Used by the star pipeline and by the grouped-aggregate top-K.
 //
 // Reference: grouped_candidate_better (kernels.metal:1102-1118) compared
 // (sum desc, secondary asc, row id asc) and topk_dense_grouped_rows_parallel_v
-// (:1249-1390) ran one 128-thread group with K <= 20.  Here every block
-// keeps a candidate buffer in LDS, admits only entries whose primary key
-// does not lose to the current K-th best, and the last block to finish
-// merges all block results; K <= TOPK_MAX_K.
+// (:1249-1390) ran one 128-thread group with K <= 20.
+//
+// Here each wave keeps its own sorted list of the best K candidates in
+// registers: position p = r*64 + lane lives in row r of that lane (R rows,
+// K <= 64*R).  A batch of 64 candidates (one per lane) is first tested
+// against the K-th key word 0 (one compare per lane, one ballot); the few
+// that pass are inserted one at a time: rank = popcount of a ballot of
+// "entry better than candidate", then the tail shifts down one position
+// with lane shuffles.  No LDS and no block barrier on this path.  The waves
+// of a block then merge through LDS into wave 0, each block publishes its
+// list, and a one-block kernel merges the published lists the same way.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,171 +24,221 @@ namespace mtl {
 namespace dev {
 
 constexpr int TOPK_BLOCK = 256;
-constexpr int TOPK_CAP = 1024;          // LDS candidate buffer entries
-constexpr int TOPK_ROUND = TOPK_BLOCK;  // entries admitted per round (1 per thread)
-constexpr int TOPK_MAX_K = TOPK_CAP - 2 * TOPK_ROUND;  // 512
+constexpr int TOPK_WAVES = TOPK_BLOCK / 64;
+constexpr int TOPK_MAX_K = 512;
 
-struct TopkLds {
-  uint64_t k[4][TOPK_CAP];
-  uint32_t id[TOPK_CAP];
-  uint32_t n;
-  uint64_t thr;  // admit entries with k[0] <= thr
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
+  uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
+  uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shup64(uint64_t v) {
+  uint32_t lo = __shfl_up((uint32_t)v, 1, 64);
+  uint32_t hi = __shfl_up((uint32_t)(v >> 32), 1, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <int R>
+struct WaveTopk {
+  uint64_t k[R][4];
+  uint32_t id[R];
+  uint32_t n;     // valid entries (wave-uniform)
+  uint32_t K;
+  uint64_t thr0;  // key word 0 of the K-th entry once full, else all ones
+
+  __device__ __forceinline__ void init(uint32_t kk) {
+    n = 0;
+    K = kk;
+    thr0 = ~0ull;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) k[r][f] = ~0ull;
+      id[r] = 0xFFFFFFFFu;
+    }
+  }
+
+  // entry (row r of this lane) strictly better than c
+  __device__ __forceinline__ bool better(int r, const uint64_t (&c)[4], uint32_t cid) const {
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+      if (k[r][f] != c[f]) return k[r][f] < c[f];
+    return id[r] < cid;
+  }
+
+  // Insert one wave-uniform candidate.
+  __device__ __forceinline__ void insert(const uint64_t (&c)[4], uint32_t cid) {
+    const int lane = __lane_id();
+    uint32_t pos = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool b = (uint32_t)(r * 64 + lane) < n && better(r, c, cid);
+      pos += (uint32_t)__popcll(__ballot(b));
+    }
+    if (pos >= K) return;
+#pragma unroll
+    for (int r = R - 1; r >= 0; --r) {
+      if ((uint32_t)(r * 64) > n) continue;  // row holds nothing and receives nothing
+      uint64_t pk[4];
+      uint32_t pid;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) pk[f] = shup64(k[r][f]);
+      pid = __shfl_up(id[r], 1, 64);
+      if (r > 0) {
+        uint64_t ck[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) ck[f] = bcast64(k[r - 1][f], 63);
+        uint32_t cidr = __builtin_amdgcn_readlane((int)id[r - 1], 63);
+        if (lane == 0) {
+#pragma unroll
+          for (int f = 0; f < 4; ++f) pk[f] = ck[f];
+          pid = cidr;
+        }
+      }
+      const uint32_t p = (uint32_t)(r * 64 + lane);
+      if (p > pos) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) k[r][f] = pk[f];
+        id[r] = pid;
+      } else if (p == pos) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) k[r][f] = c[f];
+        id[r] = cid;
+      }
+    }
+    if (n < K) ++n;
+    if (n == K) {
+      const int rr = (int)((K - 1) / 64), ll = (int)((K - 1) % 64);
+      uint64_t t = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r == rr) t = bcast64(k[r][0], ll);
+      thr0 = t;
+    }
+  }
+
+  // Each lane offers at most one candidate (ok = it passed the caller's
+  // k0 <= thr0 test, keys complete).  Wave-uniform control flow required.
+  __device__ __forceinline__ void offer(bool ok, const uint64_t (&c)[4], uint32_t cid) {
+    uint64_t mask = __ballot(ok);
+    while (mask) {
+      const int src = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)mask) - 1);
+      uint64_t bc[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) bc[f] = bcast64(c[f], src);
+      const uint32_t bid = __builtin_amdgcn_readlane((int)cid, src);
+      insert(bc, bid);
+      mask &= mask - 1;
+      if (n == K) mask &= __ballot(c[0] <= thr0);
+    }
+  }
 };
 
-__device__ __forceinline__ bool cand_less(const TopkLds& s, int a, int b) {
-#pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    if (s.k[f][a] != s.k[f][b]) return s.k[f][a] < s.k[f][b];
-  }
-  return s.id[a] < s.id[b];
-}
+// LDS area for merging the waves of a block into wave 0.
+template <int R>
+struct TopkMergeLds {
+  uint64_t k[TOPK_WAVES - 1][4][R * 64];
+  uint32_t id[TOPK_WAVES - 1][R * 64];
+  uint32_t n[TOPK_WAVES - 1];
+};
 
-__device__ __forceinline__ void cand_swap(TopkLds& s, int a, int b) {
+// Merge the lists of all waves of the block into wave 0's list.  Every
+// thread of the block must call it; afterwards only wave 0's list is
+// meaningful.
+template <int R>
+__device__ __forceinline__ void topk_block_merge(WaveTopk<R>& wl, TopkMergeLds<R>& m) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (wave > 0) {
 #pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    uint64_t t = s.k[f][a];
-    s.k[f][a] = s.k[f][b];
-    s.k[f][b] = t;
-  }
-  uint32_t t = s.id[a];
-  s.id[a] = s.id[b];
-  s.id[b] = t;
-}
-
-// Sort the first n entries ascending (best first); pads to a power of two.
-// All threads of the block must call it.
-__device__ void topk_sort(TopkLds& s, uint32_t n) {
-  uint32_t p = 2;
-  while (p < n) p <<= 1;
-  for (uint32_t i = n + threadIdx.x; i < p; i += blockDim.x) {
+    for (int r = 0; r < R; ++r) {
 #pragma unroll
-    for (int f = 0; f < 4; ++f) s.k[f][i] = ~0ull;
-    s.id[i] = 0xFFFFFFFFu;
+      for (int f = 0; f < 4; ++f) m.k[wave - 1][f][r * 64 + lane] = wl.k[r][f];
+      m.id[wave - 1][r * 64 + lane] = wl.id[r];
+    }
+    if (lane == 0) m.n[wave - 1] = wl.n;
   }
   __syncthreads();
-  for (uint32_t size = 2; size <= p; size <<= 1) {
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t t = threadIdx.x; t < p / 2; t += blockDim.x) {
-        uint32_t i = 2 * t - (t & (stride - 1));
-        uint32_t j = i + stride;
-        bool up = (i & size) == 0;
-        bool worse = cand_less(s, j, i);  // a[i] > a[j]
-        if (worse == up) cand_swap(s, i, j);
+  if (wave == 0) {
+    for (int w = 0; w < TOPK_WAVES - 1; ++w) {
+      const uint32_t nw = m.n[w];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if ((uint32_t)(r * 64) >= nw) break;
+        const uint32_t p = r * 64 + lane;
+        uint64_t c[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) c[f] = m.k[w][f][p];
+        wl.offer(p < nw && c[0] <= wl.thr0, c, m.id[w][p]);
       }
-      __syncthreads();
     }
   }
 }
 
-// After a sort: keep the best k and tighten the admission threshold.
-__device__ __forceinline__ void topk_truncate(TopkLds& s, uint32_t k) {
-  if (threadIdx.x == 0) {
-    if (s.n > k) s.n = k;
-    if (s.n == k && k > 0) s.thr = s.k[0][k - 1];
-  }
-  __syncthreads();
-}
+// Published per-block lists: ck[f][b*K + i], cid[b*K + i], cn[b].
+struct TopkPublished {
+  uint64_t* ck[4];
+  uint32_t* cid;
+  uint32_t* cn;
+};
 
-__device__ __forceinline__ void topk_init(TopkLds& s) {
-  if (threadIdx.x == 0) {
-    s.n = 0;
-    s.thr = ~0ull;
-  }
-  __syncthreads();
-}
-
-// Append a candidate (caller checked k0 <= thr).
-__device__ __forceinline__ void topk_push(TopkLds& s, uint64_t k0, uint64_t k1,
-                                          uint64_t k2, uint64_t k3, uint32_t id) {
-  uint32_t pos = atomicAdd(&s.n, 1u);
-  s.k[0][pos] = k0;
-  s.k[1][pos] = k1;
-  s.k[2][pos] = k2;
-  s.k[3][pos] = k3;
-  s.id[pos] = id;
-}
-
-// Called by all threads after each admission round (<= TOPK_ROUND pushes).
-// Compacts as soon as the buffer holds more than max(64, k) entries, so the
-// first sort (which sets the threshold) happens after the first round;
-// after it most entries fail the threshold test and are never pushed.
-// Bound: after a compaction n <= k, a round adds <= TOPK_ROUND, so n never
-// exceeds max(64, k) + TOPK_ROUND <= TOPK_CAP.
-__device__ __forceinline__ void topk_maybe_compact(TopkLds& s, uint32_t k) {
-  __syncthreads();
-  const uint32_t limit = k > 64u ? k : 64u;
-  if (s.n > limit) {
-    uint32_t n = s.n;
-    __syncthreads();
-    topk_sort(s, n);
-    topk_truncate(s, k);
-  }
-}
-
-// Sort the block's candidates, publish its best k to slot blockIdx.x of the
-// global candidate arrays (ck[f][b*k + i], cid, cn[b]), and let the last
-// block to finish merge all published candidates into s.  Returns true
-// only in that last block, with s holding the final top-k in order.  The
-// merge reads every block's slots in parallel (flat index b*k + i).
-__device__ __forceinline__ bool topk_publish_elect_merge(TopkLds& s, uint64_t* const* ck,
-                                                         uint32_t* cid, uint32_t* cn,
-                                                         uint32_t* done, uint32_t k) {
-  __shared__ bool s_last;
-  __syncthreads();
-  uint32_t n = s.n;
-  __syncthreads();
-  topk_sort(s, n);
-  topk_truncate(s, k);
-  const uint32_t kb = s.n;
-  for (uint32_t i = threadIdx.x; i < kb; i += blockDim.x) {
-    uint64_t o = (uint64_t)blockIdx.x * k + i;
+// Wave 0 of block b writes its (merged) list.
+template <int R>
+__device__ __forceinline__ void topk_publish(const WaveTopk<R>& wl, const TopkPublished& p,
+                                             uint32_t b) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
 #pragma unroll
-    for (int f = 0; f < 4; ++f) ck[f][o] = s.k[f][i];
-    cid[o] = s.id[i];
+  for (int r = 0; r < R; ++r) {
+    const uint32_t i = r * 64 + lane;
+    if (i < wl.n) {
+      const uint64_t o = (uint64_t)b * wl.K + i;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) p.ck[f][o] = wl.k[r][f];
+      p.cid[o] = wl.id[r];
+    }
   }
-  if (threadIdx.x == 0) cn[blockIdx.x] = kb;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  topk_init(s);
-  // Each thread first loads up to E of the published candidates into
-  // registers (all loads in flight together), then the admission rounds run
-  // from registers: the merge costs one memory round trip per chunk of
-  // TOPK_ROUND * E candidates instead of one per round.
-  constexpr int E = 8;
-  const uint32_t total = gridDim.x * k;
-  for (uint32_t chunk = 0; chunk < total; chunk += TOPK_ROUND * E) {
-    uint64_t c0[E], c1[E], c2[E], c3[E];
+  if (lane == 0) p.cn[b] = wl.n;
+}
+
+// One block: merge nb published lists into wave 0's list (sorted, best
+// first).  Waves take the published entries round robin in batches of 64,
+// with the loads of E batches in flight together.
+template <int R>
+__device__ __forceinline__ void topk_merge_published(WaveTopk<R>& wl, TopkMergeLds<R>& m,
+                                                     const TopkPublished& p, uint32_t nb,
+                                                     uint32_t K) {
+  wl.init(K);
+  const uint32_t total = nb * K;
+  constexpr int E = 4;
+  for (uint32_t chunk = 0; chunk < total; chunk += TOPK_BLOCK * E) {
+    uint64_t c[E][4];
     uint32_t ci[E];
     bool ok[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      uint32_t i = chunk + e * TOPK_ROUND + threadIdx.x;
-      uint32_t b = i / k, j = i - b * k;
-      ok[e] = i < total && j < __hip_atomic_load(&cn[b < gridDim.x ? b : 0], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t ii = ok[e] ? i : 0;
-      c0[e] = ck[0][ii];
-      c1[e] = ck[1][ii];
-      c2[e] = ck[2][ii];
-      c3[e] = ck[3][ii];
-      ci[e] = cid[ii];
+      const uint32_t i = chunk + e * TOPK_BLOCK + threadIdx.x;
+      const uint32_t b = i / K, j = i - b * K;
+      ok[e] = i < total && j < p.cn[b < nb ? b : 0];
+      const uint32_t ii = ok[e] ? i : 0;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) c[e][f] = p.ck[f][ii];
+      ci[e] = p.cid[ii];
     }
-    for (int e = 0; e < E; ++e) {
-      if (chunk + e * TOPK_ROUND >= total) break;  // uniform
-      if (ok[e] && c0[e] <= s.thr) topk_push(s, c0[e], c1[e], c2[e], c3[e], ci[e]);
-      topk_maybe_compact(s, k);
-    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) wl.offer(ok[e] && c[e][0] <= wl.thr0, c[e], ci[e]);
+  }
+  topk_block_merge(wl, m);
+}
+
+// Broadcast wave 0's final list to LDS so any thread can read entry i.
+template <int R>
+__device__ __forceinline__ void topk_to_lds(const WaveTopk<R>& wl, uint32_t* ids, uint32_t* n) {
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) ids[r * 64 + threadIdx.x] = wl.id[r];
+    if (threadIdx.x == 0) *n = wl.n;
   }
   __syncthreads();
-  n = s.n;
-  __syncthreads();
-  topk_sort(s, n);
-  topk_truncate(s, k);
-  return true;
 }
 
 }  // namespace dev

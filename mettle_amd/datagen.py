@@ -129,3 +129,45 @@ STAR_BYTES_PER_ROW = {"customers": 4 + 1, "orders": 4 + 4 + 4 + 4, "lines": 4 + 
 def star_algorithmic_bytes(n_c: int, n_o: int, n_l: int) -> int:
     b = STAR_BYTES_PER_ROW
     return n_c * b["customers"] + n_o * b["orders"] + n_l * b["lines"]
+
+
+def star_counts(cust, orders, lines, segment: int = STAR_SEGMENT,
+                cutoff: int = STAR_CUTOFF) -> dict:
+    """Row counts at each stage of the star query (numpy, for the per-kernel
+    byte figures): qualifying customers, qualifying orders, lines whose
+    order qualifies, and lines that also pass l_ship > cutoff."""
+    c_ok = np.zeros(int(cust["c_key"].max()) + 1, dtype=bool)
+    c_ok[cust["c_key"][cust["segment"] == segment]] = True
+    o_pass = (orders["o_date"] < cutoff) & c_ok[orders["o_cust"]]
+    o_ok = np.zeros(int(orders["o_key"].max()) + 1, dtype=bool)
+    o_ok[orders["o_key"][o_pass]] = True
+    l_parent = o_ok[lines["l_order"]]
+    hits = l_parent & (lines["l_ship"] > cutoff)
+    return {"dim_pass": int(c_ok.sum()), "parent_pass": int(o_pass.sum()),
+            "fact_parent_rows": int(l_parent.sum()), "hits": int(hits.sum())}
+
+
+def star_kernel_bytes(n_c: int, n_o: int, n_l: int, counts: dict) -> dict:
+    """Algorithmic HBM bytes of each star-pipeline kernel per launch: every
+    column a kernel must read, at its declared width, over the rows it must
+    read it for, plus its outputs.  Late-materialised columns count only
+    over the rows that reach them."""
+    dp, pp = counts["dim_pass"], counts["parent_pass"]
+    fp, h = counts["fact_parent_rows"], counts["hits"]
+    return {
+        # c_key 4 + segment 1; dimension bitmap and pass bits written
+        "k_mark_dim": n_c * 5 + n_c // 8 * 2,
+        # o_date 4 + o_cust 4 + o_key 4; parent bitmap and pass bits written
+        "k_filter_parent": n_o * 12 + n_o // 8 * 2,
+        # three bitmaps read, ranks written (one u32 per 64 parent keys)
+        "k_rank": n_o // 8 * 2 + n_c // 8 + n_o // 64 * 4,
+        # pass bits read; row_of (4 B) written and accumulators (12 B) cleared
+        "k_map": n_o // 8 + pp * 16,
+        # l_order 4 + l_ship 4 over all lines (both streamed in one pass),
+        # hit row + slot (8 B) written
+        "k_fact_scan": n_l * 8 + h * 8,
+        # hit row + slot read, price 8 + discount 8, accumulators updated
+        "k_fact_acc": h * (8 + 16) + min(h, pp) * 12,
+        "k_star_topk": pp * 12,
+        "k_star_topk_merge": 0,
+    }

@@ -221,3 +221,49 @@ def test_star_duplicate_parent_key_falls_back(ops):
     d, _keep = device_desc(ops, tables, build)
     with pytest.raises(abi.Unsupported):
         ops.star_topk(d)
+
+
+def _one_order_tables(n_lines, price):
+    cust = {"c_key": np.array([1], np.int32), "segment": np.array([1], np.uint8)}
+    orders = {"o_key": np.array([10], np.int32), "o_cust": np.array([1], np.int32),
+              "o_date": np.array([1], np.int32), "o_prio": np.zeros(1, np.int32)}
+    lines = {"l_order": np.full(n_lines, 10, np.int32),
+             "price": np.full(n_lines, price, np.int64),
+             "discount": np.zeros(n_lines, np.int64),
+             "l_ship": np.full(n_lines, 50, np.int32)}
+    return cust, orders, lines
+
+
+def _one_order_desc(n_lines):
+    def build(p):
+        d = datagen.star_desc(p, 1, 1, n_lines, segment=1, cutoff=20)
+        d.d_key_min = d.d_key_max = 1
+        d.p_key_min = d.p_key_max = 10
+        return d
+    return build
+
+
+def test_star_large_sums_exact(ops):
+    """Two lines of 100 * 2^55 each: the sum (7.2e18) still fits int64."""
+    tables = _one_order_tables(2, 2 ** 55)
+    d, _keep = device_desc(ops, tables, _one_order_desc(2))
+    rows = ops.star_topk(d)
+    assert [(r.p_row, r.count, r.sum_i64) for r in rows] == [(0, 2, 2 * 100 * 2 ** 55)]
+
+
+def test_star_sum_overflow_bound_falls_back(ops):
+    """Three such lines exceed int64: the bound count x max|value| flags it."""
+    tables = _one_order_tables(3, 2 ** 55)
+    d, _keep = device_desc(ops, tables, _one_order_desc(3))
+    with pytest.raises(abi.Unsupported):
+        ops.star_topk(d)
+
+
+def test_star_expression_overflow_is_an_error(ops):
+    """price * (100 - discount) itself overflows int64: an overflow error,
+    as DuckDB raises for the same expression."""
+    tables = _one_order_tables(1, 2 ** 60)
+    d, _keep = device_desc(ops, tables, _one_order_desc(1))
+    with pytest.raises(abi.MettleError) as ei:
+        ops.star_topk(d)
+    assert ei.value.code == abi.MTL_ERR_OVERFLOW

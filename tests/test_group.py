@@ -215,3 +215,18 @@ def test_gpu_group_topk_parity(k):
     assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order]
     assert [abi.hugeint_to_int(r) for r in to[0]] == [sums[i] for i in order]
     assert to[1].tolist() == [int(ro[1][i]) for i in order]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [10, 64, 65, 100, 512])
+def test_gpu_group_topk_all_ties(k):
+    """Every group has the same COUNT: the order is decided by the key alone
+    (ascending), whatever block or wave saw each group first."""
+    ops = _gpu_ops()
+    n_keys = 5000
+    g = np.tile(np.arange(n_keys, dtype=np.int64)[::-1], 20)
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, 0, n_keys - 1)],
+                           [(F.AGG_COUNT, None, 0)], len(g))
+    tk, to = groups.topk(0, k, desc=True)
+    assert tk[:, 0].tolist() == list(range(k))
+    assert to[0].tolist() == [20] * k
