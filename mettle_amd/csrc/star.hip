@@ -397,33 +397,68 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_acc(FactArgs a) {
   bool ovf = false;
   uint64_t mx = 0;
   using V = typename std::conditional<DBL, double, int64_t>::type;
-  for (uint32_t e = tid; e < total; e += FBLOCK) {
-    uint32_t q = 0;
+  // block-uniform trip count: every lane takes part in the wave reduction
+  for (uint32_t e0 = 0; e0 < total; e0 += FBLOCK) {
+    const uint32_t e = e0 + tid;
+    const bool live = e < total;
+    uint32_t r = 0xFFFFFFFFu;
+    V val = 0;
+    if (live) {
+      uint32_t q = 0;
 #pragma unroll
-    for (int i = 1; i < kAccRegions; ++i) q += e >= s_off[i] ? 1u : 0u;
-    const uint64_t at = (r0 + q) * (FJ * 256) + (e - s_off[q]);
-    const uint32_t row = a.hit_row[at], sv = a.hit_slot[at];
-    V cv[MTL_EXPR_MAX_COLS];
+      for (int i = 1; i < kAccRegions; ++i) q += e >= s_off[i] ? 1u : 0u;
+      const uint64_t at = (r0 + q) * (FJ * 256) + (e - s_off[q]);
+      const uint32_t row = a.hit_row[at], sv = a.hit_slot[at];
+      V cv[MTL_EXPR_MAX_COLS];
 #pragma unroll
-    for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
-      if ((uint32_t)c < a.expr.n_cols) {
-        if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
-        else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
+      for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
+        if ((uint32_t)c < a.expr.n_cols) {
+          if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
+          else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
+        } else {
+          cv[c] = 0;
+        }
+      const uint64_t w = a.bm[sv >> 6];
+      const uint64_t below = (sv & 63) ? (w & ((1ull << (sv & 63)) - 1)) : 0ull;
+      r = a.rank[sv >> 6] + (uint32_t)__popcll(below);
+      if constexpr (DBL) {
+        val = expr_f64_vals(a.expr, cv);
       } else {
-        cv[c] = 0;
+        val = expr_i64_vals(a.expr, cv, ovf);
+        const uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
+        mx = av > mx ? av : mx;
       }
-    const uint64_t w = a.bm[sv >> 6];
-    const uint64_t below = (sv & 63) ? (w & ((1ull << (sv & 63)) - 1)) : 0ull;
-    const uint32_t r = a.rank[sv >> 6] + (uint32_t)__popcll(below);
-    if constexpr (DBL) {
-      unsafeAtomicAdd((double*)&a.acc[r], expr_f64_vals(a.expr, cv));
-    } else {
-      const int64_t val = expr_i64_vals(a.expr, cv, ovf);
-      const uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
-      mx = av > mx ? av : mx;
-      atomicAdd((unsigned long long*)&a.acc[r], (unsigned long long)val);
     }
-    atomicAdd(&a.cnt[r], 1u);
+    // Hits of one parent row sit in consecutive lanes (the hit lists keep
+    // row order): sum each run of equal slots across the wave and let only
+    // the run's first lane update the slot, so a slot gets one atomic per
+    // run instead of one per hit (same-address atomics serialise).
+    const int lane = tid % WAVE;
+    const uint32_t prev = __shfl_up(r, 1, WAVE);
+    const uint32_t head = (lane == 0 || prev != r) ? 1u : 0u;
+    const uint32_t seg = wave_inclusive_u32(head);
+    uint32_t n = live ? 1u : 0u;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+      const uint32_t os = __shfl_down(seg, off, WAVE);
+      const uint32_t on = __shfl_down(n, off, WAVE);
+      V ov;
+      if constexpr (DBL) {
+        ov = __shfl_down(val, off, WAVE);
+      } else {
+        ov = (int64_t)__shfl_down((unsigned long long)val, off, WAVE);
+      }
+      if (lane + off < WAVE && os == seg) {
+        if constexpr (DBL) val += ov;
+        else val = (int64_t)((uint64_t)val + (uint64_t)ov);  // wrapping, as the atomics
+        n += on;
+      }
+    }
+    if (head && r != 0xFFFFFFFFu) {
+      if constexpr (DBL) unsafeAtomicAdd((double*)&a.acc[r], val);
+      else atomicAdd((unsigned long long*)&a.acc[r], (unsigned long long)val);
+      atomicAdd(&a.cnt[r], n);
+    }
   }
   if (ovf) atomicOr(a.err, 16u);
   if (!DBL) {
@@ -457,10 +492,8 @@ struct TopkArgs {
   uint64_t p_rows;  // row_of entries are checked against it before use
 };
 
-__device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t r, uint64_t k[4]) {
-  uint64_t v = a.acc[r];
-  k[0] = a.is_double ? ~enc_f64(__longlong_as_double((long long)v)) : ~enc_i64((int64_t)v);
-  uint32_t row = a.row_of[r];
+// Tie-break words of slot r whose parent row is `row` (read with the slot).
+__device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t row, uint64_t k[4]) {
   if (row >= a.p_rows) {  // cannot happen for consistent inputs; never read past P
     atomicOr(a.err, 64u);
     row = 0;
@@ -488,13 +521,14 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
   constexpr int E = 8;  // entries per thread per pass, loaded together
   uint32_t cmax = 0;
   for (uint32_t base = start; base < end; base += TOPK_BLOCK * E) {
-    uint32_t cv[E];
+    uint32_t cv[E], rv[E];
     uint64_t av[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       uint32_t r = base + e * TOPK_BLOCK + threadIdx.x;
       cv[e] = r < end ? a.cnt[r] : 0u;
       av[e] = r < end ? a.acc[r] : 0ull;
+      rv[e] = r < end ? a.row_of[r] : 0u;
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) cmax = cv[e] > cmax ? cv[e] : cmax;
@@ -505,7 +539,7 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
       k[0] = a.is_double ? ~enc_f64(__longlong_as_double((long long)av[e]))
                          : ~enc_i64((int64_t)av[e]);
       const bool pre = cv[e] != 0 && k[0] <= wl.thr0;
-      if (pre) star_keys(a, r, k);
+      if (pre) star_keys(a, rv[e], k);
       wl.offer(pre, k, r);
     }
   }
@@ -527,7 +561,17 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk_merge(TopkArgs a) {
   if (!a.is_double) {
     uint64_t mx = 0;
     uint32_t cm = 0;
-    for (uint32_t i = threadIdx.x; i < a.n_tmax; i += TOPK_BLOCK) mx = a.tmax[i] > mx ? a.tmax[i] : mx;
+    // eight independent loads in flight per thread per round
+    for (uint32_t i0 = 0; i0 < a.n_tmax; i0 += TOPK_BLOCK * 8) {
+      uint64_t t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t i = i0 + u * TOPK_BLOCK + threadIdx.x;
+        t[u] = i < a.n_tmax ? a.tmax[i] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) mx = t[u] > mx ? t[u] : mx;
+    }
     for (uint32_t i = threadIdx.x; i < a.nblocks; i += TOPK_BLOCK) cm = a.bcmax[i] > cm ? a.bcmax[i] : cm;
     mx = block_max_u64(mx);
     cm = block_max_u32(cm);
@@ -642,8 +686,10 @@ static int32_t star_validate(const mtl_star_desc* d, StarPlan* sp) {
       return set_error(MTL_UNSUPPORTED, "dimension key span too wide for a bitmap");
     sp->dwords = ceil_div(sp->dspan, 64);
   }
-  uint64_t g = ceil_div(d->p_rows ? d->p_rows : 1, 8192);
-  sp->topk_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, (uint64_t)cu_count()));
+  // about one 2048-slot round per block (the slot count is at most P)
+  uint64_t g = ceil_div(d->p_rows ? d->p_rows : 1, 2048);
+  sp->topk_grid =
+      (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 3 * (uint64_t)cu_count()));
   return MTL_OK;
 }
 

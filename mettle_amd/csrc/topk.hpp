@@ -32,10 +32,13 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
   uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
   return ((uint64_t)hi << 32) | lo;
 }
+// Lane i receives lane i-1's value (lane 0 keeps its own): one DPP move
+// per 32 bits (wave_shr:1), no LDS crossbar round trip.
+__device__ __forceinline__ uint32_t shup32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x138, 0xF, 0xF, false);
+}
 __device__ __forceinline__ uint64_t shup64(uint64_t v) {
-  uint32_t lo = __shfl_up((uint32_t)v, 1, 64);
-  uint32_t hi = __shfl_up((uint32_t)(v >> 32), 1, 64);
-  return ((uint64_t)hi << 32) | lo;
+  return ((uint64_t)shup32((uint32_t)(v >> 32)) << 32) | shup32((uint32_t)v);
 }
 
 template <int R>
@@ -83,7 +86,7 @@ struct WaveTopk {
       uint32_t pid;
 #pragma unroll
       for (int f = 0; f < 4; ++f) pk[f] = shup64(k[r][f]);
-      pid = __shfl_up(id[r], 1, 64);
+      pid = shup32(id[r]);
       if (r > 0) {
         uint64_t ck[4];
 #pragma unroll
@@ -209,23 +212,27 @@ __device__ __forceinline__ void topk_merge_published(WaveTopk<R>& wl, TopkMergeL
                                                      uint32_t K) {
   wl.init(K);
   const uint32_t total = nb * K;
-  constexpr int E = 4;
+  // every load is issued unconditionally (entries past a block's count are
+  // read but discarded), so the counts do not delay the candidate loads
+  constexpr int E = R <= 2 ? 16 : 4;
   for (uint32_t chunk = 0; chunk < total; chunk += TOPK_BLOCK * E) {
     uint64_t c[E][4];
-    uint32_t ci[E];
-    bool ok[E];
+    uint32_t ci[E], cnb[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const uint32_t i = chunk + e * TOPK_BLOCK + threadIdx.x;
-      const uint32_t b = i / K, j = i - b * K;
-      ok[e] = i < total && j < p.cn[b < nb ? b : 0];
-      const uint32_t ii = ok[e] ? i : 0;
+      const uint32_t ii = i < total ? i : 0;
+      cnb[e] = p.cn[ii / K];
 #pragma unroll
       for (int f = 0; f < 4; ++f) c[e][f] = p.ck[f][ii];
       ci[e] = p.cid[ii];
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) wl.offer(ok[e] && c[e][0] <= wl.thr0, c[e], ci[e]);
+    for (int e = 0; e < E; ++e) {
+      const uint32_t i = chunk + e * TOPK_BLOCK + threadIdx.x;
+      const bool ok = i < total && i % K < cnb[e];
+      wl.offer(ok && c[e][0] <= wl.thr0, c[e], ci[e]);
+    }
   }
   topk_block_merge(wl, m);
 }
