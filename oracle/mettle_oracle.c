@@ -77,7 +77,9 @@ int orc_pred_row(const mtl_pred *p, uint64_t row) {
     case MTL_INTEGER:
     case MTL_DATE:
     case MTL_DICT32:
-      c = cmp3_i64(((const int32_t *)p->column)[row], (int64_t)(int32_t)p->scalar);
+      /* DuckDB compares an INTEGER column with a BIGINT literal in BIGINT:
+       * the literal is never truncated to 32 bits */
+      c = cmp3_i64(((const int32_t *)p->column)[row], p->scalar);
       break;
     default:
       c = cmp3_i64(orc_key(p->column, p->type, row), p->scalar);

@@ -66,6 +66,16 @@ def test_filter_multi_predicate_and_refine(ops):
     np.testing.assert_array_equal(ops.to_host(got2, np.uint32), ref)
 
 
+@pytest.mark.parametrize("lit", [2**40, -2**40, 2**31, -2**31 - 1])
+def test_filter_int_column_wide_literal(ops, lit):
+    v = np.array([-2**31, -5, 0, 5, 2**31 - 1] * 1000, dtype=np.int32)
+    d = ops.to_device(v)
+    for op in (abi.CMP_EQ, abi.CMP_LT, abi.CMP_LE, abi.CMP_GT, abi.CMP_GE):
+        ref = orc.filter([(v, abi.INTEGER, op, lit)], len(v))
+        got, _ = ops.filter([(d, abi.INTEGER, op, lit)], len(v))
+        np.testing.assert_array_equal(ops.to_host(got, np.uint32), ref)
+
+
 def test_filter_float_nan_semantics(ops):
     col = np.array([np.nan, 1.0, -np.inf, np.inf, np.nan, 0.0, -0.0], dtype=np.float32)
     d = ops.to_device(col)

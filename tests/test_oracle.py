@@ -85,6 +85,17 @@ def test_filter_reference_fixtures():
     assert list(got) == [1, 2]
 
 
+@pytest.mark.parametrize("op", list(NP_OPS))
+@pytest.mark.parametrize("lit", [2**40, -2**40, 2**31, -2**31 - 1])
+def test_filter_int_column_wide_literal(op, lit):
+    """An INTEGER column against a literal outside int32: compared in int64
+    (truncating the literal to 32 bits would flip these results)."""
+    v = np.array([-2**31, -5, 0, 5, 2**31 - 1], dtype=np.int32)
+    got = orc.filter([(v, abi.INTEGER, op, lit)], len(v))
+    ref = np.flatnonzero(NP_OPS[op](v.astype(np.int64), np.int64(lit))).astype(np.uint32)
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_filter_planted_op_mistake_is_caught():
     # swapping LT for LE changes the result on the boundary literal
     v = np.array([1, 2, 3], dtype=np.int32)
