@@ -182,6 +182,17 @@ __device__ __forceinline__ uint32_t eval_narrow(const FilterArgs& a, uint64_t n,
     load_u32x16(a.probe_key, a.probe_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.probe_vec, pk);
   if (a.mark_bm)
     load_u32x16(a.mark_key, a.mark_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.mark_vec, ms);
+  // the predicate first (its column arrived with the keys), so the bitmap
+  // probe below reads only rows that still pass: random probes are L2
+  // line reads, the largest cost of this pass
+  if (a.n_preds) {
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (!pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0]))
+          bits &= ~(1u << (4 * j + v));
+  }
   if (a.probe_bm) {
     const uint32_t* bm32 = (const uint32_t*)a.probe_bm;
     uint32_t w[FJ][4];
@@ -190,28 +201,14 @@ __device__ __forceinline__ uint32_t eval_narrow(const FilterArgs& a, uint64_t n,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         pk[j][v] = narrow_off(pk[j][v], a.probe_key_type, a.probe_min, a.probe_span);
-        w[j][v] = ld_off32(bm32, pk[j][v] != 0xFFFFFFFFu ? (pk[j][v] >> 5) << 2 : 0u);
+        const bool need = ((bits >> (4 * j + v)) & 1u) && pk[j][v] != 0xFFFFFFFFu;
+        w[j][v] = need ? ld_off32(bm32, (pk[j][v] >> 5) << 2) : 0u;
       }
-    if (a.n_preds) {
-#pragma unroll
-      for (int j = 0; j < FJ; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (!pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0]))
-            bits &= ~(1u << (4 * j + v));
-    }
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         if (pk[j][v] == 0xFFFFFFFFu || !((w[j][v] >> (pk[j][v] & 31)) & 1u))
-          bits &= ~(1u << (4 * j + v));
-  } else if (a.n_preds) {
-#pragma unroll
-    for (int j = 0; j < FJ; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (!pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0]))
           bits &= ~(1u << (4 * j + v));
   }
   if (a.mark_bm) {

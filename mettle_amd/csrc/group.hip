@@ -325,8 +325,8 @@ struct GTopkArgs {
   int32_t desc;
   int32_t avg;  // order by sum / count
   uint32_t k;
-  TopkPublished pub;  // per-block lists
-  uint32_t nblocks;
+  TopkStage stage;    // full keys of every wave-list entry
+  TopkLists lists0;   // per-block lists written by phase 1
   uint32_t* out_idx;
   uint32_t* out_n;
 };
@@ -367,36 +367,47 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint6
 }
 
 template <int R>
-__global__ __launch_bounds__(TOPK_BLOCK) void k_group_topk(GTopkArgs a) {
-  __shared__ TopkMergeLds<R> ml;
+__global__ __launch_bounds__(TopkShape<R>::NT) void k_group_topk(GTopkArgs a) {
+  using S = TopkShape<R>;
+  __shared__ TopkBlockLds<S::W, S::CAP> sl;
+  topk_block_init(sl);
   WaveTopk<R> wl;
   wl.init(a.k);
   const uint64_t chunk = (a.n + gridDim.x - 1) / gridDim.x;
   const uint64_t start = (uint64_t)blockIdx.x * chunk;
   const uint64_t end = start + chunk < a.n ? start + chunk : a.n;
-  for (uint64_t base = start; base < end; base += TOPK_BLOCK) {
+  for (uint64_t base = start; base < end; base += S::NT) {
+    const uint64_t bthr =
+        __hip_atomic_load(&sl.thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const uint64_t g = base + threadIdx.x;
     uint64_t k[4] = {0, 0, 0, 0};
     bool pre = false;
     if (g < end && a.st[g * a.W] != 0) {
       gtopk_keys(a, g, k);
-      pre = k[0] <= wl.thr0;
+      pre = k[0] <= wl.thr0 && k[0] <= bthr;
     }
     wl.offer(pre, k, (uint32_t)g);
+    topk_share_thr(wl, sl);
   }
-  topk_block_merge(wl, ml);
-  topk_publish(wl, a.pub, blockIdx.x);
+  topk_block_finish<S::NT>(wl, sl, a.stage, a.lists0);
 }
 
-template <int R>
-__global__ __launch_bounds__(TOPK_BLOCK) void k_group_topk_merge(GTopkArgs a) {
-  __shared__ TopkMergeLds<R> ml;
-  __shared__ uint32_t ids[R * 64];
-  __shared__ uint32_t nn;
-  WaveTopk<R> wl;
-  topk_merge_published(wl, ml, a.pub, a.nblocks, a.k);
-  topk_to_lds(wl, ids, &nn);
-  for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) a.out_idx[i] = ids[i];
+// Merge level: block b merges lists [b*F, b*F+F) of `in`; the single block
+// of the final level writes the table indices of the top K.
+__global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_group_topk_merge(GTopkArgs a,
+                                                                         TopkLists in,
+                                                                         TopkLists out,
+                                                                         uint32_t L,
+                                                                         uint32_t final_level) {
+  __shared__ TopkMergeLds s;
+  const int cur = topk_merge_lists(s, in, L, a.k, a.stage);
+  if (!final_level) {
+    topk_write_list(s, cur, out, a.k);
+    return;
+  }
+  const uint32_t nn = s.n[cur][0];
+  for (uint32_t i = threadIdx.x; i < nn; i += TOPK_MERGE_THREADS)
+    a.out_idx[i] = a.stage.cid[s.g[cur][i]];
   if (threadIdx.x == 0) *a.out_n = nn;
 }
 
@@ -757,12 +768,20 @@ int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, i
   mtl_stream* s = g->s;
   std::lock_guard<std::mutex> lk(s->mu);
   s->arena.reset();
+  const uint64_t nt = k <= 128 ? 1024 : 256;
   uint32_t grid = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>(ceil_div(g->entries, 16384), (uint64_t)cu_count()));
-  void *ck[4], *cid, *cn, *misc, *oidx;
-  for (int f = 0; f < 4; ++f) MTL_TRY(s->arena.alloc((uint64_t)grid * k * 8, &ck[f]));
-  MTL_TRY(s->arena.alloc((uint64_t)grid * k * 4, &cid));
-  MTL_TRY(s->arena.alloc((uint64_t)grid * 4, &cn));
+      1, std::min<uint64_t>(ceil_div(g->entries, nt * 16), (uint64_t)cu_count()));
+  void *ck[4], *cid, *lk0[2], *lg[2], *ln[2], *misc, *oidx;
+  const uint64_t stage_n = (uint64_t)grid * (nt / 64) * k;
+  for (int f = 0; f < 4; ++f) MTL_TRY(s->arena.alloc(stage_n * 8, &ck[f]));
+  MTL_TRY(s->arena.alloc(stage_n * 4, &cid));
+  const uint64_t la = grid, lb = ceil_div(grid, topk_fan(k));
+  MTL_TRY(s->arena.alloc(la * k * 8, &lk0[0]));
+  MTL_TRY(s->arena.alloc(la * k * 4, &lg[0]));
+  MTL_TRY(s->arena.alloc(la * 4, &ln[0]));
+  MTL_TRY(s->arena.alloc(lb * k * 8, &lk0[1]));
+  MTL_TRY(s->arena.alloc(lb * k * 4, &lg[1]));
+  MTL_TRY(s->arena.alloc(lb * 4, &ln[1]));
   MTL_TRY(s->arena.alloc(256, &misc));
   MTL_TRY(s->arena.alloc((uint64_t)k * 4, &oidx));
   ZeroList zl;
@@ -781,32 +800,40 @@ int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, i
   t.desc = desc;
   t.avg = g->agg_func[agg] == MTL_AGG_AVG;
   t.k = k;
-  for (int f = 0; f < 4; ++f) t.pub.ck[f] = (uint64_t*)ck[f];
-  t.pub.cid = (uint32_t*)cid;
-  t.pub.cn = (uint32_t*)cn;
-  t.nblocks = grid;
+  for (int f = 0; f < 4; ++f) t.stage.ck[f] = (uint64_t*)ck[f];
+  t.stage.cid = (uint32_t*)cid;
+  TopkLists lists[2];
+  for (int b = 0; b < 2; ++b)
+    lists[b] = TopkLists{(uint64_t*)lk0[b], (uint32_t*)lg[b], (uint32_t*)ln[b]};
+  t.lists0 = lists[0];
   t.out_idx = (uint32_t*)oidx;
   t.out_n = (uint32_t*)misc + 1;
   {
     ProfScope ps(s, "k_group_topk");
     if (k <= 64)
-      hipLaunchKernelGGL(k_group_topk<1>, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+      hipLaunchKernelGGL(k_group_topk<1>, dim3(grid), dim3(TopkShape<1>::NT), 0, s->stream, t);
     else if (k <= 128)
-      hipLaunchKernelGGL(k_group_topk<2>, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+      hipLaunchKernelGGL(k_group_topk<2>, dim3(grid), dim3(TopkShape<2>::NT), 0, s->stream, t);
     else
-      hipLaunchKernelGGL(k_group_topk<8>, dim3(grid), dim3(TOPK_BLOCK), 0, s->stream, t);
+      hipLaunchKernelGGL(k_group_topk<8>, dim3(grid), dim3(TopkShape<8>::NT), 0, s->stream, t);
   }
   MTL_HIP(hipGetLastError());
   {
     ProfScope ps(s, "k_group_topk_merge");
-    if (k <= 64)
-      hipLaunchKernelGGL(k_group_topk_merge<1>, dim3(1), dim3(TOPK_BLOCK), 0, s->stream, t);
-    else if (k <= 128)
-      hipLaunchKernelGGL(k_group_topk_merge<2>, dim3(1), dim3(TOPK_BLOCK), 0, s->stream, t);
-    else
-      hipLaunchKernelGGL(k_group_topk_merge<8>, dim3(1), dim3(TOPK_BLOCK), 0, s->stream, t);
+    const uint32_t fan = topk_fan(k);
+    uint32_t L = grid;
+    int in = 0;
+    while (true) {
+      const uint32_t blocks = (uint32_t)ceil_div(L, fan);
+      const uint32_t fin = blocks == 1 ? 1u : 0u;
+      hipLaunchKernelGGL(k_group_topk_merge, dim3(blocks), dim3(TOPK_MERGE_THREADS), 0,
+                         s->stream, t, lists[in], lists[in ^ 1], L, fin);
+      MTL_HIP(hipGetLastError());
+      if (fin) break;
+      L = blocks;
+      in ^= 1;
+    }
   }
-  MTL_HIP(hipGetLastError());
   void* host;
   MTL_TRY(stream_pinned(s, 8, &host));
   MTL_HIP(hipMemcpyAsync(host, (uint32_t*)misc + 1, 4, hipMemcpyDeviceToHost, s->stream));

@@ -342,6 +342,16 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
     case MTL_DICT8: fact_keys<uint8_t>(a, k0, s); break;
     default: fact_keys<int32_t>(a, k0, s); break;
   }
+  // rows in range and, when there is a predicate, passing it (its column
+  // arrived with the keys): only these probe the parent bitmap
+  uint32_t cand = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu &&
+          (!a.n_preds || pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0])))
+        cand |= 1u << (4 * j + v);
   uint32_t hits = 0;
   {
     uint64_t w[FJ][4];
@@ -349,21 +359,13 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        w[j][v] = ld_off32(a.bm, s[j][v] != 0xFFFFFFFFu ? (s[j][v] >> 6) << 3 : 0u);
-#pragma unroll
-    for (int j = 0; j < FJ; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        bool in = k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu;
-        if (in && ((w[j][v] >> (s[j][v] & 63)) & 1ull)) hits |= 1u << (4 * j + v);
-      }
-  }
-  if (a.n_preds) {
+        w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(a.bm, (s[j][v] >> 6) << 3) : 0ull;
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (!pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0])) hits &= ~(1u << (4 * j + v));
+        if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (s[j][v] & 63)) & 1ull))
+          hits |= 1u << (4 * j + v);
   }
   for (uint32_t p = 1; p < a.n_preds && hits; ++p) pred_rows(a.preds[p], k0, hits);
   // the wave's hits, in row order, to the wave's region
@@ -481,14 +483,15 @@ struct TopkArgs {
   const void* tie_col[MTL_STAR_MAX_TIES];
   int32_t tie_type[MTL_STAR_MAX_TIES];
   int32_t tie_desc[MTL_STAR_MAX_TIES];
-  TopkPublished pub;  // per-block lists
-  uint32_t nblocks;
+  TopkStage stage;    // full keys of every wave-list entry
+  TopkLists lists0;   // per-block lists written by phase 1
   mtl_star_row* out;
   uint32_t* out_n;  // [0] = rows, [1] = error bits copied from err
   uint32_t* err;
   const unsigned long long* tmax;  // per fact tile: max |value| added
   uint32_t n_tmax;
   uint32_t* bcmax;                 // per top-K block: max count of its slots
+  uint32_t n_bcmax;
   uint64_t p_rows;  // row_of entries are checked against it before use
 };
 
@@ -510,8 +513,10 @@ __device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t row, uint6
 }
 
 template <int R>
-__global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
-  __shared__ TopkMergeLds<R> ml;
+__global__ __launch_bounds__(TopkShape<R>::NT) void k_star_topk(TopkArgs a) {
+  using S = TopkShape<R>;
+  __shared__ TopkBlockLds<S::W, S::CAP> sl;
+  topk_block_init(sl);
   WaveTopk<R> wl;
   wl.init(a.k);
   const uint32_t m = *a.m_count;
@@ -520,59 +525,65 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk(TopkArgs a) {
   const uint32_t end = min(m, start + chunk);
   constexpr int E = 8;  // entries per thread per pass, loaded together
   uint32_t cmax = 0;
-  for (uint32_t base = start; base < end; base += TOPK_BLOCK * E) {
+  for (uint32_t base = start; base < end; base += S::NT * E) {
     uint32_t cv[E], rv[E];
     uint64_t av[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      uint32_t r = base + e * TOPK_BLOCK + threadIdx.x;
+      uint32_t r = base + e * S::NT + threadIdx.x;
       cv[e] = r < end ? a.cnt[r] : 0u;
       av[e] = r < end ? a.acc[r] : 0ull;
       rv[e] = r < end ? a.row_of[r] : 0u;
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) cmax = cv[e] > cmax ? cv[e] : cmax;
+    const uint64_t bthr = __hip_atomic_load(&sl.thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const uint32_t r = base + e * TOPK_BLOCK + threadIdx.x;
+      const uint32_t r = base + e * S::NT + threadIdx.x;
       uint64_t k[4] = {0, 0, 0, 0};
       k[0] = a.is_double ? ~enc_f64(__longlong_as_double((long long)av[e]))
                          : ~enc_i64((int64_t)av[e]);
-      const bool pre = cv[e] != 0 && k[0] <= wl.thr0;
+      const bool pre = cv[e] != 0 && k[0] <= wl.thr0 && k[0] <= bthr;
       if (pre) star_keys(a, rv[e], k);
       wl.offer(pre, k, r);
     }
+    topk_share_thr(wl, sl);
   }
-  topk_block_merge(wl, ml);
-  topk_publish(wl, a.pub, blockIdx.x);
+  topk_block_finish<S::NT>(wl, sl, a.stage, a.lists0);
   cmax = block_max_u32(cmax);
   if (threadIdx.x == 0) a.bcmax[blockIdx.x] = cmax;
 }
 
-// One block: merge the published block lists and write the result rows.
-template <int R>
-__global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk_merge(TopkArgs a) {
-  __shared__ TopkMergeLds<R> ml;
-  __shared__ uint32_t ids[R * 64];
-  __shared__ uint32_t nn;
-  WaveTopk<R> wl;
+// Merge level: block b merges lists [b*F, b*F+F) of `in`; the single block
+// of the final level also checks the overflow bound and writes the rows.
+__global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_star_topk_merge(TopkArgs a, TopkLists in,
+                                                                        TopkLists out, uint32_t L,
+                                                                        uint32_t final_level) {
+  __shared__ TopkMergeLds s;
+  const int cur = topk_merge_lists(s, in, L, a.k, a.stage);
+  if (!final_level) {
+    topk_write_list(s, cur, out, a.k);
+    return;
+  }
   // |SUM| <= count x max|value| for every group: flag the run when that
   // bound exceeds int64 (an int64 accumulator could have wrapped)
   if (!a.is_double) {
     uint64_t mx = 0;
     uint32_t cm = 0;
     // eight independent loads in flight per thread per round
-    for (uint32_t i0 = 0; i0 < a.n_tmax; i0 += TOPK_BLOCK * 8) {
+    for (uint32_t i0 = 0; i0 < a.n_tmax; i0 += TOPK_MERGE_THREADS * 8) {
       uint64_t t[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const uint32_t i = i0 + u * TOPK_BLOCK + threadIdx.x;
+        const uint32_t i = i0 + u * TOPK_MERGE_THREADS + threadIdx.x;
         t[u] = i < a.n_tmax ? a.tmax[i] : 0ull;
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) mx = t[u] > mx ? t[u] : mx;
     }
-    for (uint32_t i = threadIdx.x; i < a.nblocks; i += TOPK_BLOCK) cm = a.bcmax[i] > cm ? a.bcmax[i] : cm;
+    for (uint32_t i = threadIdx.x; i < a.n_bcmax; i += TOPK_MERGE_THREADS)
+      cm = a.bcmax[i] > cm ? a.bcmax[i] : cm;
     mx = block_max_u64(mx);
     cm = block_max_u32(cm);
     unsigned long long prod;
@@ -581,10 +592,9 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk_merge(TopkArgs a) {
          prod > 0x7FFFFFFFFFFFFFFFull))
       atomicOr(a.err, 32u);
   }
-  topk_merge_published(wl, ml, a.pub, a.nblocks, a.k);
-  topk_to_lds(wl, ids, &nn);
-  for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) {
-    uint32_t r = ids[i];
+  const uint32_t nn = s.n[cur][0];
+  for (uint32_t i = threadIdx.x; i < nn; i += TOPK_MERGE_THREADS) {
+    const uint32_t r = a.stage.cid[s.g[cur][i]];
     mtl_star_row row;
     row.p_row = a.row_of[r];
     row.count = a.cnt[r];
@@ -593,6 +603,7 @@ __global__ __launch_bounds__(TOPK_BLOCK) void k_star_topk_merge(TopkArgs a) {
     row.sum_f64 = a.is_double ? __longlong_as_double((long long)v) : (double)(int64_t)v;
     a.out[i] = row;
   }
+  __syncthreads();  // every thread's atomicOr above precedes the error read
   if (threadIdx.x == 0) {
     a.out_n[0] = nn;
     a.out_n[1] = __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -686,10 +697,13 @@ static int32_t star_validate(const mtl_star_desc* d, StarPlan* sp) {
       return set_error(MTL_UNSUPPORTED, "dimension key span too wide for a bitmap");
     sp->dwords = ceil_div(sp->dspan, 64);
   }
-  // about one 2048-slot round per block (the slot count is at most P)
-  uint64_t g = ceil_div(d->p_rows ? d->p_rows : 1, 2048);
-  sp->topk_grid =
-      (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 3 * (uint64_t)cu_count()));
+  // about one round of 8 slots per thread per block (the slot count is
+  // at most P): 1024-thread blocks, one per CU, for K <= 128; 256-thread
+  // blocks, up to three per CU, above
+  const uint64_t nt = d->k <= 128 ? 1024 : 256;
+  uint64_t g = ceil_div(d->p_rows ? d->p_rows : 1, nt * 8);
+  sp->topk_grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(g, (d->k <= 128 ? 1 : 3) * (uint64_t)cu_count()));
   return MTL_OK;
 }
 
@@ -702,7 +716,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   const uint64_t prows = d->p_rows;
   void *dim_bm = nullptr, *dbits = nullptr, *par_bm, *par_rank, *pbits, *row_of, *acc, *cnt,
        *misc;
-  void *st_r, *ck[4], *cid, *cn, *bcmax, *tmax;
+  void *st_r, *ck[4], *cid, *lk0[2], *lg[2], *ln[2], *bcmax, *tmax;
   if (d->d_rows) {
     MTL_TRY(ar.alloc(sp.dwords * 8, &dim_bm));
     MTL_TRY(ar.alloc((uint64_t)filter_tiles(d->d_rows) * kTileRows / 8, &dbits));
@@ -720,9 +734,18 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   MTL_TRY(ar.alloc(256, &misc));
   uint32_t rtiles = (uint32_t)ceil_div(sp.pwords, RTILE);
   MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(rtiles, 1) * 8, &st_r));
-  for (int f = 0; f < 4; ++f) MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * d->k * 8, &ck[f]));
-  MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * d->k * 4, &cid));
-  MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * 4, &cn));
+  // top-K: staged wave-list entries, then two ping-pong sets of lists
+  const uint64_t tk_waves = d->k <= 128 ? 16 : 4;
+  const uint64_t stage_n = (uint64_t)sp.topk_grid * tk_waves * d->k;
+  for (int f = 0; f < 4; ++f) MTL_TRY(ar.alloc(stage_n * 8, &ck[f]));
+  MTL_TRY(ar.alloc(stage_n * 4, &cid));
+  const uint64_t la = sp.topk_grid, lb = ceil_div(sp.topk_grid, topk_fan(d->k));
+  MTL_TRY(ar.alloc(la * d->k * 8, &lk0[0]));
+  MTL_TRY(ar.alloc(la * d->k * 4, &lg[0]));
+  MTL_TRY(ar.alloc(la * 4, &ln[0]));
+  MTL_TRY(ar.alloc(lb * d->k * 8, &lk0[1]));
+  MTL_TRY(ar.alloc(lb * d->k * 4, &lg[1]));
+  MTL_TRY(ar.alloc(lb * 4, &ln[1]));
   MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * 4, &bcmax));
   const uint32_t ftiles = d->f_rows ? filter_tiles(d->f_rows) : 0u;
   const uint32_t acc_blocks = (uint32_t)ceil_div(ftiles, kAccTiles);
@@ -871,35 +894,46 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
       ta.tie_type[t] = d->tie_type[t];
       ta.tie_desc[t] = d->tie_desc[t];
     }
-    for (int f = 0; f < 4; ++f) ta.pub.ck[f] = (uint64_t*)ck[f];
-    ta.pub.cid = (uint32_t*)cid;
-    ta.pub.cn = (uint32_t*)cn;
-    ta.nblocks = sp.topk_grid;
+    for (int f = 0; f < 4; ++f) ta.stage.ck[f] = (uint64_t*)ck[f];
+    ta.stage.cid = (uint32_t*)cid;
+    TopkLists lists[2];
+    for (int b = 0; b < 2; ++b)
+      lists[b] = TopkLists{(uint64_t*)lk0[b], (uint32_t*)lg[b], (uint32_t*)ln[b]};
+    ta.lists0 = lists[0];
     ta.out = out;
     ta.out_n = out_n;
     ta.err = mu + 3;
     ta.tmax = (const unsigned long long*)tmax;
     ta.n_tmax = d->agg.is_double ? 0u : acc_blocks;
     ta.bcmax = (uint32_t*)bcmax;
+    ta.n_bcmax = sp.topk_grid;
     ta.p_rows = prows;
     {
       ProfScope ps(s, "k_star_topk");
       if (d->k <= 64)
-        hipLaunchKernelGGL(k_star_topk<1>, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
+        hipLaunchKernelGGL(k_star_topk<1>, dim3(sp.topk_grid), dim3(TopkShape<1>::NT), 0, st, ta);
       else if (d->k <= 128)
-        hipLaunchKernelGGL(k_star_topk<2>, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
+        hipLaunchKernelGGL(k_star_topk<2>, dim3(sp.topk_grid), dim3(TopkShape<2>::NT), 0, st, ta);
       else
-        hipLaunchKernelGGL(k_star_topk<8>, dim3(sp.topk_grid), dim3(TOPK_BLOCK), 0, st, ta);
+        hipLaunchKernelGGL(k_star_topk<8>, dim3(sp.topk_grid), dim3(TopkShape<8>::NT), 0, st, ta);
     }
     MTL_HIP(hipGetLastError());
+    // merge levels: F lists per block until one list remains; the last
+    // level (one block) writes the rows
     ProfScope ps2(s, "k_star_topk_merge");
-    if (d->k <= 64)
-      hipLaunchKernelGGL(k_star_topk_merge<1>, dim3(1), dim3(TOPK_BLOCK), 0, st, ta);
-    else if (d->k <= 128)
-      hipLaunchKernelGGL(k_star_topk_merge<2>, dim3(1), dim3(TOPK_BLOCK), 0, st, ta);
-    else
-      hipLaunchKernelGGL(k_star_topk_merge<8>, dim3(1), dim3(TOPK_BLOCK), 0, st, ta);
-    MTL_HIP(hipGetLastError());
+    const uint32_t fan = topk_fan(d->k);
+    uint32_t L = sp.topk_grid;
+    int in = 0;
+    while (true) {
+      const uint32_t blocks = (uint32_t)ceil_div(L, fan);
+      const uint32_t fin = blocks == 1 ? 1u : 0u;
+      hipLaunchKernelGGL(k_star_topk_merge, dim3(blocks), dim3(TOPK_MERGE_THREADS), 0, st, ta,
+                         lists[in], lists[in ^ 1], L, fin);
+      MTL_HIP(hipGetLastError());
+      if (fin) break;
+      L = blocks;
+      in ^= 1;
+    }
   }
   return MTL_OK;
 }

@@ -9,12 +9,13 @@
 // Here each wave keeps its own sorted list of the best K candidates in
 // registers: position p = r*64 + lane lives in row r of that lane (R rows,
 // K <= 64*R).  A batch of 64 candidates (one per lane) is first tested
-// against the K-th key word 0 (one compare per lane, one ballot); the few
-// that pass are inserted one at a time: rank = popcount of a ballot of
-// "entry better than candidate", then the tail shifts down one position
-// with lane shuffles.  No LDS and no block barrier on this path.  The waves
-// of a block then merge through LDS into wave 0, each block publishes its
-// list, and a one-block kernel merges the published lists the same way.
+// against the K-th key word 0 of the wave and a block-shared bound (one
+// compare per lane, one ballot); the few that pass are inserted one at a
+// time: rank = popcount of a ballot of "entry better than candidate", then
+// the tail shifts down one position with DPP lane moves.  The wave lists of
+// a block, then the block lists, are merged by rank in LDS (below), level by
+// level, with one binary search per entry instead of one insertion per
+// candidate.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,9 +24,16 @@
 namespace mtl {
 namespace dev {
 
-constexpr int TOPK_BLOCK = 256;
-constexpr int TOPK_WAVES = TOPK_BLOCK / 64;
 constexpr int TOPK_MAX_K = 512;
+
+// Phase-1 block shape for lists of R rows: 16 waves while the per-wave
+// lists are short, 4 waves for K > 128 (LDS holds two buffers of W lists).
+template <int R>
+struct TopkShape {
+  static constexpr int NT = R <= 2 ? 1024 : 256;
+  static constexpr int W = NT / 64;
+  static constexpr int CAP = 64 * R;
+};
 
 __device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
   uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
@@ -137,115 +145,190 @@ struct WaveTopk {
   }
 };
 
-// LDS area for merging the waves of a block into wave 0.
-template <int R>
-struct TopkMergeLds {
-  uint64_t k[TOPK_WAVES - 1][4][R * 64];
-  uint32_t id[TOPK_WAVES - 1][R * 64];
-  uint32_t n[TOPK_WAVES - 1];
-};
-
-// Merge the lists of all waves of the block into wave 0's list.  Every
-// thread of the block must call it; afterwards only wave 0's list is
-// meaningful.
-template <int R>
-__device__ __forceinline__ void topk_block_merge(WaveTopk<R>& wl, TopkMergeLds<R>& m) {
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  if (wave > 0) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-#pragma unroll
-      for (int f = 0; f < 4; ++f) m.k[wave - 1][f][r * 64 + lane] = wl.k[r][f];
-      m.id[wave - 1][r * 64 + lane] = wl.id[r];
-    }
-    if (lane == 0) m.n[wave - 1] = wl.n;
-  }
-  __syncthreads();
-  if (wave == 0) {
-    for (int w = 0; w < TOPK_WAVES - 1; ++w) {
-      const uint32_t nw = m.n[w];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if ((uint32_t)(r * 64) >= nw) break;
-        const uint32_t p = r * 64 + lane;
-        uint64_t c[4];
-#pragma unroll
-        for (int f = 0; f < 4; ++f) c[f] = m.k[w][f][p];
-        wl.offer(p < nw && c[0] <= wl.thr0, c, m.id[w][p]);
-      }
-    }
-  }
-}
-
-// Published per-block lists: ck[f][b*K + i], cid[b*K + i], cn[b].
-struct TopkPublished {
+// ---------------------------------------------------------------------------
+// Merging sorted lists: merge by rank in LDS.
+//
+// Every wave list entry is staged in global memory with its full key
+// (stage.ck[f][g], stage.cid[g]); merges move only (key word 0, g).  Two
+// sorted lists A and B merge in one parallel step: entry i of A lands at
+// i + |{b in B : b < A[i]}| (a binary search in B), and symmetrically; the
+// keys are distinct (ids are unique), so the positions are a permutation and
+// entries landing at K or beyond are dropped.  L lists merge pairwise in
+// ceil(log2 L) levels with one barrier each.  A tie on word 0 reads the
+// full keys from the stage (rare: equal aggregates).
+// ---------------------------------------------------------------------------
+struct TopkStage {
   uint64_t* ck[4];
   uint32_t* cid;
-  uint32_t* cn;
 };
 
-// Wave 0 of block b writes its (merged) list.
-template <int R>
-__device__ __forceinline__ void topk_publish(const WaveTopk<R>& wl, const TopkPublished& p,
-                                             uint32_t b) {
-  if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
+__device__ __forceinline__ bool tk_less(uint64_t ak, uint32_t ag, uint64_t bk, uint32_t bg,
+                                        const TopkStage& st) {
+  if (ak != bk) return ak < bk;
+#pragma unroll
+  for (int f = 1; f < 4; ++f) {
+    const uint64_t x = st.ck[f][ag], y = st.ck[f][bg];
+    if (x != y) return x < y;
+  }
+  return st.cid[ag] < st.cid[bg];
+}
+
+// Lists li < L of length n[li] <= K at stride K in buffer `cur` (k0, g, n);
+// returns the buffer holding the merged list (list 0).  Every thread of
+// the block calls it after a barrier that published the input.
+template <int NT>
+__device__ __forceinline__ int tk_tree_merge(uint64_t* const (&k0)[2], uint32_t* const (&g)[2],
+                                             uint32_t* const (&n)[2], uint32_t L, uint32_t K,
+                                             const TopkStage& st) {
+  int cur = 0;
+  while (L > 1) {  // block-uniform
+    const uint32_t Lo = (L + 1) >> 1;
+    const uint64_t* ak = k0[cur];
+    const uint32_t* ag = g[cur];
+    const uint32_t* an = n[cur];
+    uint64_t* bk = k0[cur ^ 1];
+    uint32_t* bg = g[cur ^ 1];
+    uint32_t* bn = n[cur ^ 1];
+    for (uint32_t t = threadIdx.x; t < L * K; t += NT) {
+      const uint32_t li = t / K, i = t - li * K;
+      if (i >= an[li]) continue;
+      const uint64_t xk = ak[t];
+      const uint32_t xg = ag[t];
+      uint32_t pos = i;
+      const uint32_t pl = li ^ 1u;
+      if (pl < L) {
+        const uint32_t base = pl * K;
+        uint32_t lo = 0, hi = an[pl];
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (tk_less(ak[base + mid], ag[base + mid], xk, xg, st)) lo = mid + 1;
+          else hi = mid;
+        }
+        pos += lo;
+      }
+      if (pos < K) {
+        bk[(li >> 1) * K + pos] = xk;
+        bg[(li >> 1) * K + pos] = xg;
+      }
+    }
+    for (uint32_t q = threadIdx.x; q < Lo; q += NT) {
+      const uint32_t c = an[2 * q] + (2 * q + 1 < L ? an[2 * q + 1] : 0u);
+      bn[q] = c < K ? c : K;
+    }
+    __syncthreads();
+    cur ^= 1;
+    L = Lo;
+  }
+  return cur;
+}
+
+// A set of lists (k0, g) at stride K with their lengths (global memory).
+struct TopkLists {
+  uint64_t* k0;
+  uint32_t* g;
+  uint32_t* n;
+};
+
+// LDS of a phase-1 block: one list per wave (W waves, K <= CAP per list).
+template <int W, int CAP>
+struct TopkBlockLds {
+  uint64_t k0[2][W * CAP];
+  uint32_t g[2][W * CAP];
+  uint32_t n[2][W];
+  uint64_t thr;  // block-shared bound on key word 0 (min of full waves' K-th)
+};
+
+template <int W, int CAP>
+__device__ __forceinline__ void topk_block_init(TopkBlockLds<W, CAP>& s) {
+  if (threadIdx.x == 0) s.thr = ~0ull;
+  __syncthreads();
+}
+
+// After a batch: a wave with a full list lowers the block bound.
+template <int R, int W, int CAP>
+__device__ __forceinline__ void topk_share_thr(const WaveTopk<R>& wl, TopkBlockLds<W, CAP>& s) {
+  if (wl.n == wl.K && wl.K > 0 && __lane_id() == 0 && wl.thr0 < s.thr)
+    atomicMin((unsigned long long*)&s.thr, (unsigned long long)wl.thr0);
+}
+
+// End of phase 1: stage every wave's entries, merge the block's wave lists
+// in LDS and write the block's list to `out` (list blockIdx.x).  Every
+// thread of the block calls it.
+template <int NT, int R, int W, int CAP>
+__device__ __forceinline__ void topk_block_finish(const WaveTopk<R>& wl, TopkBlockLds<W, CAP>& s,
+                                                  const TopkStage& st, const TopkLists& out) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t K = wl.K;
+  const uint32_t gbase = ((uint32_t)blockIdx.x * W + wave) * K;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t i = r * 64 + lane;
     if (i < wl.n) {
-      const uint64_t o = (uint64_t)b * wl.K + i;
+      const uint32_t gi = gbase + i;
 #pragma unroll
-      for (int f = 0; f < 4; ++f) p.ck[f][o] = wl.k[r][f];
-      p.cid[o] = wl.id[r];
+      for (int f = 0; f < 4; ++f) st.ck[f][gi] = wl.k[r][f];
+      st.cid[gi] = wl.id[r];
+      s.k0[0][wave * K + i] = wl.k[r][0];
+      s.g[0][wave * K + i] = gi;
     }
   }
-  if (lane == 0) p.cn[b] = wl.n;
+  if (lane == 0) s.n[0][wave] = wl.n;
+  __syncthreads();  // stage and LDS lists visible to the block
+  uint64_t* const k0p[2] = {s.k0[0], s.k0[1]};
+  uint32_t* const gp[2] = {s.g[0], s.g[1]};
+  uint32_t* const np[2] = {s.n[0], s.n[1]};
+  const int cur = tk_tree_merge<NT>(k0p, gp, np, W, K, st);
+  const uint32_t nf = s.n[cur][0];
+  for (uint32_t i = threadIdx.x; i < nf; i += NT) {
+    out.k0[(uint64_t)blockIdx.x * K + i] = s.k0[cur][i];
+    out.g[(uint64_t)blockIdx.x * K + i] = s.g[cur][i];
+  }
+  if (threadIdx.x == 0) out.n[blockIdx.x] = nf;
 }
 
-// One block: merge nb published lists into wave 0's list (sorted, best
-// first).  Waves take the published entries round robin in batches of 64,
-// with the loads of E batches in flight together.
-template <int R>
-__device__ __forceinline__ void topk_merge_published(WaveTopk<R>& wl, TopkMergeLds<R>& m,
-                                                     const TopkPublished& p, uint32_t nb,
-                                                     uint32_t K) {
-  wl.init(K);
-  const uint32_t total = nb * K;
-  // every load is issued unconditionally (entries past a block's count are
-  // read but discarded), so the counts do not delay the candidate loads
-  constexpr int E = R <= 2 ? 16 : 4;
-  for (uint32_t chunk = 0; chunk < total; chunk += TOPK_BLOCK * E) {
-    uint64_t c[E][4];
-    uint32_t ci[E], cnb[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const uint32_t i = chunk + e * TOPK_BLOCK + threadIdx.x;
-      const uint32_t ii = i < total ? i : 0;
-      cnb[e] = p.cn[ii / K];
-#pragma unroll
-      for (int f = 0; f < 4; ++f) c[e][f] = p.ck[f][ii];
-      ci[e] = p.cid[ii];
-    }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const uint32_t i = chunk + e * TOPK_BLOCK + threadIdx.x;
-      const bool ok = i < total && i % K < cnb[e];
-      wl.offer(ok && c[e][0] <= wl.thr0, c[e], ci[e]);
-    }
-  }
-  topk_block_merge(wl, m);
+// Merge kernel body: block b merges lists [b*F, b*F + F) of `in` (L lists)
+// in LDS; returns the LDS buffer holding the block's merged list.
+constexpr int TOPK_MCAP = 2048;  // LDS entries per merge buffer (F * K)
+constexpr int TOPK_MFAN = 256;   // max lists per merge block
+constexpr int TOPK_MERGE_THREADS = 1024;
+
+struct TopkMergeLds {
+  uint64_t k0[2][TOPK_MCAP];
+  uint32_t g[2][TOPK_MCAP];
+  uint32_t n[2][TOPK_MFAN];
+};
+
+__host__ __device__ __forceinline__ uint32_t topk_fan(uint32_t K) {
+  uint32_t f = (uint32_t)TOPK_MCAP / K;
+  return f < (uint32_t)TOPK_MFAN ? f : (uint32_t)TOPK_MFAN;
 }
 
-// Broadcast wave 0's final list to LDS so any thread can read entry i.
-template <int R>
-__device__ __forceinline__ void topk_to_lds(const WaveTopk<R>& wl, uint32_t* ids, uint32_t* n) {
-  if (threadIdx.x < 64) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) ids[r * 64 + threadIdx.x] = wl.id[r];
-    if (threadIdx.x == 0) *n = wl.n;
+__device__ __forceinline__ int topk_merge_lists(TopkMergeLds& s, const TopkLists& in, uint32_t L,
+                                                uint32_t K, const TopkStage& st) {
+  const uint32_t F = topk_fan(K);
+  const uint32_t first = blockIdx.x * F;
+  const uint32_t nl = L - first < F ? L - first : F;
+  for (uint32_t t = threadIdx.x; t < nl * K; t += TOPK_MERGE_THREADS) {
+    const uint64_t src = (uint64_t)first * K + t;  // entries past a list's length are unused
+    s.k0[0][t] = in.k0[src];
+    s.g[0][t] = in.g[src];
   }
+  for (uint32_t li = threadIdx.x; li < nl; li += TOPK_MERGE_THREADS) s.n[0][li] = in.n[first + li];
   __syncthreads();
+  uint64_t* const k0p[2] = {s.k0[0], s.k0[1]};
+  uint32_t* const gp[2] = {s.g[0], s.g[1]};
+  uint32_t* const np[2] = {s.n[0], s.n[1]};
+  return tk_tree_merge<TOPK_MERGE_THREADS>(k0p, gp, np, nl, K, st);
+}
+
+__device__ __forceinline__ void topk_write_list(const TopkMergeLds& s, int cur, const TopkLists& out,
+                                                uint32_t K) {
+  const uint32_t nf = s.n[cur][0];
+  for (uint32_t i = threadIdx.x; i < nf; i += TOPK_MERGE_THREADS) {
+    out.k0[(uint64_t)blockIdx.x * K + i] = s.k0[cur][i];
+    out.g[(uint64_t)blockIdx.x * K + i] = s.g[cur][i];
+  }
+  if (threadIdx.x == 0) out.n[blockIdx.x] = nf;
 }
 
 }  // namespace dev
