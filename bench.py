@@ -74,6 +74,97 @@ def copy_calibration(n_bytes: int = 1 << 30, reps: int = 5) -> int:
     return 0
 
 
+def serving_benchmark(n_queries: int, slots: int, seed: int = 20261208):
+    """Serving stream: n_queries with Poisson arrivals mixing three shapes
+    uniformly -- the 1M-row plain query (value > 524288, value*10), a
+    BIGINT scan at 10% selectivity with a projection, and the star
+    join -> SUM -> top-10 -- each over a table of log-uniform size in
+    [1e4, 1e8] rows (prefixes of resident 1e8-row tables).  The same arrival
+    schedule, offered at 10x the measured serial capacity, runs once through
+    a 1-slot server (serial dispatch) and once through a `slots`-slot server
+    (continuous admission); every result is compared between the two."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from mettle_amd import abi, datagen, ops
+    L = abi.lib()
+    n_max = 100_000_000
+    log("serving: generating resident tables (1e8-row scan table, star at 1e8 lines)")
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, 1 << 20, n_max, dtype=np.int32)
+    b = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, n_max, dtype=np.int64,
+                     endpoint=True)
+    dv, db = ops.to_device(v), ops.to_device(b)
+    del v, b
+    cust, orders, lines = datagen.star_tables(scale=n_max / 60_000_000, workload=5)
+    dev = {}
+    for tname, cols in (("customers", cust), ("orders", orders), ("lines", lines)):
+        dev[tname] = {c: ops.to_device(x) for c, x in cols.items()}
+    l_order = lines["l_order"]
+    n_c, n_l_all = len(cust["c_key"]), len(l_order)
+    del cust, orders, lines
+    torch.cuda.synchronize()
+    cut_b = int(np.iinfo(np.int64).min + (1 << 64) // 10)
+    shapes = rng.integers(0, 3, n_queries)
+    sizes = np.floor(10 ** rng.uniform(4, 8, n_queries)).astype(np.int64)
+    queries, bufs = [], []
+    for sh, n in zip(shapes, sizes):
+        if sh == 0:
+            q = ops.scan_query([(dv, abi.INTEGER, abi.CMP_GT, 524288)], int(n),
+                               (dv, abi.INTEGER, abi.PROJ_MUL, 10))
+            out = None
+        elif sh == 1:
+            q = ops.scan_query([(db, abi.BIGINT, abi.CMP_LT, cut_b)], int(n),
+                               (db, abi.BIGINT, abi.PROJ_ADD, 7))
+            out = None
+        else:
+            nl = int(min(n, n_l_all))
+            n_o = int(l_order[nl - 1])  # orders 1..n_o cover the first nl lines
+            d = datagen.star_desc(lambda t, c: dev[t][c].data_ptr(), n_c, n_o, nl)
+            q, out = ops.star_query(d)
+        queries.append(q)
+        bufs.append(out)
+
+    def run(n_slots, arrivals):
+        with ops.Server(n_slots) as srv:
+            t0 = L.mtl_now_ns() + 50_000_000
+            tickets = [srv.submit(q, t0 + int(a)) for q, a in zip(queries, arrivals)]
+            res = [srv.wait(t) for t in tickets]
+        first = min(r.t_submit_ns for r in res)
+        last = max(r.t_done_ns for r in res)
+        lat = np.array([r.t_done_ns - r.t_submit_ns for r in res], dtype=np.float64) / 1e6
+        outs = []
+        for r, q, bo in zip(res, queries, bufs):
+            if q.kind == abi.Q_STAR:
+                outs.append(tuple((bo[i].p_row, bo[i].count, bo[i].sum_i64)
+                                  for i in range(r.n_rows)))
+            else:
+                outs.append((r.count, r.fingerprint))
+        return {"qps": len(res) / ((last - first) / 1e9),
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "span_s": (last - first) / 1e9}, outs
+
+    # serial capacity on a prefix, all arriving at once
+    n_cal = min(500, n_queries)
+    q_all, b_all = queries, bufs
+    queries, bufs = q_all[:n_cal], b_all[:n_cal]
+    cal, _ = run(1, np.zeros(n_cal))
+    queries, bufs = q_all, b_all
+    offered = 10 * cal["qps"]
+    arrivals = np.cumsum(rng.exponential(1e9 / offered, n_queries))
+    log(f"serving: serial capacity ~{cal['qps']:.0f} q/s; offering {offered:.0f} q/s")
+    serial, out_s = run(1, arrivals)
+    log(f"serving: serial {serial}")
+    batched, out_b = run(slots, arrivals)
+    log(f"serving: batched ({slots} slots) {batched}")
+    same = out_s == out_b
+    return {"queries": n_queries, "offered_qps": offered, "slots": slots,
+            "mix": "uniform over {1M-row plain filter+project, BIGINT 10% scan+project, "
+                   "star join->SUM->top-10}; rows log-uniform in [1e4, 1e8]",
+            "serial": serial, "batched": batched,
+            "speedup": batched["qps"] / serial["qps"], "identical_results": bool(same)}
+
+
 def cpu_baseline(scale: float, seconds: float):
     """Oracle (C restatement, 1 thread) on a bounded sample of the workload;
     also checks the GPU result on the same sample."""
@@ -112,6 +203,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=10000)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -243,6 +337,13 @@ def main() -> int:
                        "queries_per_s_batched": args.steps / (dev_ms / 1e3)},
     }
     L.mtl_stream_destroy(s)
+    del dev
+    if not args.no_serve:
+        result["serving"] = serving_benchmark(args.serve_queries, args.serve_slots)
+        result["latency_ms"]["queries_per_s_serial"] = result["serving"]["serial"]["qps"]
+        result["latency_ms"]["queries_per_s_batched"] = result["serving"]["batched"]["qps"]
+        result["latency_ms"]["serving_p50_ms"] = result["serving"]["batched"]["p50_ms"]
+        result["latency_ms"]["serving_p99_ms"] = result["serving"]["batched"]["p99_ms"]
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread)")
         result["cpu_baseline"] = cpu_baseline(args.cpu_scale, args.cpu_seconds)

@@ -197,6 +197,16 @@ int32_t mtl_project(const void *column, int32_t type, const uint32_t *sel,
                     int64_t scalar, void *out, uint32_t *overflow_flag,
                     mtl_stream *s);
 
+/* ORDER BY one key column: out_rows[0..*out_n) = the selection (sel, or
+ * rows 0..capacity-1 when sel is NULL) sorted by column[row] ascending, or
+ * descending when desc != 0, ties in selection order (stable).  FLOAT and
+ * DOUBLE follow DuckDB's total order (NaN largest, -0 == 0).  A device
+ * count is read once on the host.  out_rows holds capacity entries.
+ * Replaces the CPU sort_record_batch (helpers.rs:627-667). */
+int32_t mtl_sort(const void *column, int32_t type, const uint32_t *sel,
+                 const uint32_t *count, uint64_t capacity, int32_t desc,
+                 uint32_t *out_rows, uint32_t *out_n, mtl_stream *s);
+
 /* Bytes of the device table a join build needs (dense: slots x 4 B;
  * hash: next_pow2(max(2 x rows, 16)) x 16 B).  Mirrors the capacity rules of
  * choose_gpu_join_strategy / hash_join_capacity (join_plan.rs:29-56,
@@ -376,6 +386,66 @@ int32_t mtl_star_topk_async(const mtl_star_desc *desc, void *out_dev,
 
 /* Map the error bits of an asynchronous star run to a status (+ message). */
 int32_t mtl_star_status(uint32_t error_bits);
+
+/* ====================================================================== */
+/* Serving loop: continuous admission of queries onto concurrent streams. */
+/* Replaces the one-query-per-CALL materialize_query (processing.rs:      */
+/* 103-143) and stage_query (state.rs:42-54), which ran every query       */
+/* synchronously inside bind with no overlap and no admission control.    */
+/* ====================================================================== */
+#define MTL_Q_SCAN 0 /* filter -> optional projection (configs 1-2)    */
+#define MTL_Q_STAR 1 /* star join -> SUM -> top-K (config 3)            */
+
+typedef struct mtl_query {
+  int32_t kind; /* MTL_Q_*                                                */
+  /* MTL_Q_SCAN: rows of `n_rows` passing every predicate, then (when
+   * proj_col != NULL) proj_col[row] op scalar over those rows.  The result
+   * stays on the device in the server's slot; the host receives the row
+   * count and an order-sensitive fingerprint of (row id, projected value)
+   * pairs, so results can be compared alone and in a mix. */
+  uint64_t n_rows;
+  uint32_t n_preds;
+  mtl_pred preds[MTL_MAX_PREDS];
+  const void *proj_col;
+  int32_t proj_type;
+  int32_t proj_op;
+  int64_t proj_scalar;
+  /* MTL_Q_STAR */
+  mtl_star_desc star;
+  mtl_star_row *star_out; /* host buffer of star.k rows                  */
+} mtl_query;
+
+typedef struct mtl_query_result {
+  int32_t status;       /* MTL_OK or the query's error status             */
+  uint32_t n_rows;      /* MTL_Q_STAR: rows written to star_out           */
+  uint64_t count;       /* MTL_Q_SCAN: selected rows                      */
+  uint64_t fingerprint; /* MTL_Q_SCAN: hash of (row id, value) in order   */
+  int64_t t_submit_ns;  /* steady-clock stamps: arrival (or submit),      */
+  int64_t t_start_ns;   /* dispatch to a slot,                            */
+  int64_t t_done_ns;    /* completion observed by the host                */
+} mtl_query_result;
+
+typedef struct mtl_server mtl_server;
+
+/* n_slots concurrent streams (1 = serial dispatch); queries are admitted in
+ * arrival order while their device scratch fits scratch_budget (0 = the
+ * processing budget of gpu_buffer_init, or 16 GiB before init). */
+int32_t mtl_server_create(uint32_t n_slots, uint64_t scratch_budget,
+                          mtl_server **out);
+/* Queue a query (the descriptor is copied; columns and star_out must stay
+ * valid until it completes).  not_before_ns > 0 holds the query until that
+ * steady-clock time, which is then its arrival time. */
+int32_t mtl_server_submit(mtl_server *srv, const mtl_query *q,
+                          int64_t not_before_ns, uint64_t *ticket);
+/* Block until the query completes; releases the ticket. */
+int32_t mtl_server_wait(mtl_server *srv, uint64_t ticket,
+                        mtl_query_result *out);
+/* Steady-clock now, in the clock of the result stamps. */
+int64_t mtl_now_ns(void);
+/* Device scratch bytes a query needs at admission (its budget charge). */
+uint64_t mtl_query_scratch_bytes(const mtl_query *q);
+/* Waits for queued work, stops the workers, frees the slots. */
+int32_t mtl_server_destroy(mtl_server *srv);
 
 /* ====================================================================== */
 /* Kernel timing: hipEvent pairs around every kernel launch, resolved at  */

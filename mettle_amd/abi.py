@@ -136,6 +136,23 @@ class StarRow(C.Structure):
                 ("sum_f64", C.c_double)]
 
 
+Q_SCAN = 0
+Q_STAR = 1
+
+
+class Query(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n_rows", C.c_uint64), ("n_preds", C.c_uint32),
+                ("preds", Pred * MAX_PREDS), ("proj_col", C.c_void_p), ("proj_type", C.c_int32),
+                ("proj_op", C.c_int32), ("proj_scalar", C.c_int64), ("star", StarDesc),
+                ("star_out", C.POINTER(StarRow))]
+
+
+class QueryResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_rows", C.c_uint32), ("count", C.c_uint64),
+                ("fingerprint", C.c_uint64), ("t_submit_ns", C.c_int64),
+                ("t_start_ns", C.c_int64), ("t_done_ns", C.c_int64)]
+
+
 def scalar_bits(value, type_code: int) -> int:
     """Literal -> the int64 bit pattern the ABI expects for that column type."""
     if type_code in (FLOAT, DOUBLE):
@@ -199,6 +216,7 @@ def lib() -> C.CDLL:
         "mtl_compare": (i32, [C.POINTER(Pred), u32, u64, vp, vp]),
         "mtl_gather": (i32, [vp, u32, vp, vp, u64, vp, vp]),
         "mtl_project": (i32, [vp, i32, vp, vp, u64, i32, i64, vp, vp, vp]),
+        "mtl_sort": (i32, [vp, i32, vp, vp, u64, i32, vp, C.POINTER(u32), vp]),
         "mtl_join_table_bytes": (u64, [i32, u64, u64]),
         "mtl_join_build": (i32, [i32, vp, i32, vp, vp, u64, i64, u64, vp, u64, vp, vp]),
         "mtl_join_probe": (i32, [i32, vp, u64, i64, u64, vp, i32, vp, vp, u64, vp, vp, vp, vp]),
@@ -214,6 +232,12 @@ def lib() -> C.CDLL:
         "mtl_groups_fetch": (i32, [vp, vp, C.POINTER(vp), u64, C.POINTER(u64)]),
         "mtl_groups_topk": (i32, [vp, u32, i32, u32, vp, C.POINTER(vp), C.POINTER(u32)]),
         "mtl_groups_release": (i32, [vp]),
+        "mtl_server_create": (i32, [u32, u64, C.POINTER(vp)]),
+        "mtl_server_submit": (i32, [vp, C.POINTER(Query), i64, C.POINTER(u64)]),
+        "mtl_server_wait": (i32, [vp, u64, C.POINTER(QueryResult)]),
+        "mtl_server_destroy": (i32, [vp]),
+        "mtl_now_ns": (i64, []),
+        "mtl_query_scratch_bytes": (u64, [C.POINTER(Query)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)

@@ -62,18 +62,30 @@ struct PopArgs {
   uint32_t* pops;
 };
 
-__device__ __forceinline__ uint32_t block_popcount(const uint64_t* p, uint64_t words,
-                                                   uint32_t* s_red) {
+// This thread's part of a grid-stride popcount, eight loads in flight.
+__device__ __forceinline__ uint32_t thread_popcount(const uint64_t* p, uint64_t words) {
   uint32_t c = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * RBLOCK + threadIdx.x; i < words;
-       i += (uint64_t)gridDim.x * RBLOCK)
-    c += (uint32_t)__popcll(p[i]);
+  const uint64_t stride = (uint64_t)gridDim.x * RBLOCK;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * RBLOCK + threadIdx.x; i0 < words; i0 += stride * 8) {
+    uint64_t x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint64_t i = i0 + u * stride;
+      x[u] = i < words ? p[i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c += (uint32_t)__popcll(x[u]);
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t c, uint32_t* s_red) {
   c = wave_sum_u32(c);
+  __syncthreads();
   if (threadIdx.x % WAVE == 0) s_red[threadIdx.x / WAVE] = c;
   __syncthreads();
   uint32_t t = 0;
   for (int w = 0; w < RBLOCK / WAVE; ++w) t += s_red[w];
-  __syncthreads();
   return t;
 }
 
@@ -85,16 +97,25 @@ __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm
                                                  PopArgs pa) {
   __shared__ uint32_t s_tile, s_prefix;
   __shared__ uint32_t s_wave[RBLOCK / WAVE];
+  __shared__ uint32_t s_red[RBLOCK / WAVE];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
   if (tid == 0) s_tile = atomicAdd(ctr, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint64_t w0 = (uint64_t)tile * RTILE + (uint64_t)tid * RPER;
+  uint64_t x[RPER];
+#pragma unroll
+  for (int i = 0; i < RPER; ++i) x[i] = (w0 + i < words) ? bm[w0 + i] : 0ull;
+  // the duplicate-key popcounts are independent of the prefix: their loads
+  // go out with the bitmap words and finish while predecessors publish
+  const uint32_t p0 = thread_popcount(pa.pbits, pa.pbits_words);
+  const uint32_t p1 = pa.dim_bm ? thread_popcount(pa.dim_bm, pa.dim_words) : 0u;
+  const uint32_t p2 = pa.dbits ? thread_popcount(pa.dbits, pa.dbits_words) : 0u;
   uint32_t c[RPER];
   uint32_t sum = 0;
 #pragma unroll
   for (int i = 0; i < RPER; ++i) {
-    c[i] = (w0 + i < words) ? (uint32_t)__popcll(bm[w0 + i]) : 0u;
+    c[i] = (uint32_t)__popcll(x[i]);
     sum += c[i];
   }
   uint32_t incl = wave_inclusive_u32(sum);
@@ -118,10 +139,9 @@ __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm
     run += c[i];
   }
   if (tile == gridDim.x - 1 && tid == 0) *par_total = s_prefix + total;
-  __shared__ uint32_t s_red[RBLOCK / WAVE];
-  uint32_t c0 = block_popcount(pa.pbits, pa.pbits_words, s_red);
-  uint32_t c1 = pa.dim_bm ? block_popcount(pa.dim_bm, pa.dim_words, s_red) : 0u;
-  uint32_t c2 = pa.dbits ? block_popcount(pa.dbits, pa.dbits_words, s_red) : 0u;
+  const uint32_t c0 = block_sum(p0, s_red);
+  const uint32_t c1 = block_sum(p1, s_red);
+  const uint32_t c2 = block_sum(p2, s_red);
   if (tid == 0) {
     if (c0) atomicAdd(&pa.pops[0], c0);
     if (c1) atomicAdd(&pa.pops[1], c1);
@@ -151,6 +171,7 @@ struct MapArgs {
   const uint32_t* par_total;  // parent bitmap popcount = accumulator slots
   const uint32_t* pops;       // [0] P pass bits, [1] D bitmap, [2] D pass bits
   uint32_t has_dim;
+  uint32_t key_vec;  // key column 16-byte aligned (4-byte aligned for 1-byte keys)
   uint32_t* err;
 };
 
@@ -171,16 +192,38 @@ __global__ void k_map(MapArgs a) {
     a.acc[i] = 0;
     a.cnt[i] = 0;
   }
+  // 16 consecutive rows per thread: the keys of all 16 rows, then the
+  // bitmap word and rank of every qualifying one, are issued together (two
+  // round trips per thread, not two per qualifying row)
+  const uint32_t kw = (a.key_type == MTL_BIGINT || a.key_type == MTL_TIMESTAMP ||
+                       a.key_type == MTL_DECIMAL) ? 8u : (a.key_type == MTL_DICT8 ? 1u : 4u);
   for (uint64_t g = t0; g * 16 < a.n_rows; g += nthreads) {
     const uint64_t r0 = g * 16;
-    uint32_t m = (uint32_t)(a.sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
-    while (m) {
-      int i = __ffs(m) - 1;
-      m &= m - 1;
-      uint64_t row = r0 + (uint64_t)i;
-      uint64_t s = key_off(load_key(a.key, a.key_type, row), a.min);
-      if (s >= a.span) continue;  // flagged by the parent pass
-      a.row_of[rank_of(a.bm, a.rank, s)] = (uint32_t)row;
+    const uint32_t m = (uint32_t)(a.sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
+    if (!m) continue;
+    uint64_t raw[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) load_raw4(a.key, kw, r0 + 4 * j, a.n_rows, a.key_vec != 0, raw[j]);
+    uint32_t sv[16];
+    uint64_t w[16];
+    uint32_t rk[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const uint64_t x = raw[i >> 2][i & 3];
+      const int64_t key = kw == 8 ? (int64_t)x : (kw == 1 ? (int64_t)x : (int64_t)(int32_t)(uint32_t)x);
+      const uint64_t o = key_off(key, a.min);
+      // rows outside the key range were flagged by the parent pass
+      sv[i] = ((m >> i) & 1u) && o < a.span ? (uint32_t)o : 0xFFFFFFFFu;
+      const bool need = sv[i] != 0xFFFFFFFFu;
+      w[i] = need ? ld_off32(a.bm, (sv[i] >> 6) << 3) : 0ull;
+      rk[i] = need ? ld_off32(a.rank, (sv[i] >> 6) << 2) : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (sv[i] == 0xFFFFFFFFu) continue;
+      const uint32_t b = sv[i] & 63;
+      const uint64_t below = b ? (w[i] & ((1ull << b) - 1)) : 0ull;
+      a.row_of[rk[i] + (uint32_t)__popcll(below)] = (uint32_t)(r0 + i);
     }
   }
 }
@@ -305,8 +348,9 @@ __device__ __forceinline__ double expr_f64_vals(const mtl_expr& e,
 //     with global atomics (hits are sparse and spread over consecutive
 //     slots).  Each block writes the max |value| it added to tmax[block]
 //     (no shared counter); the top-K merge reduces it for the overflow bound.
-constexpr int kAccTiles = 8;
+constexpr int kAccTiles = 16;
 constexpr int kAccRegions = kAccTiles * FWAVES;
+static_assert(kAccRegions <= WAVE, "region offsets are one wave's prefix sum");
 
 template <typename KT>
 __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32_t (&s)[FJ][4]) {
@@ -392,7 +436,8 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_acc(FactArgs a) {
     const uint64_t r = r0 + tid;
     const uint32_t cnt = (tid < kAccRegions && r < a.n_regions) ? a.hit_count[r] : 0u;
     const uint32_t incl = wave_inclusive_u32(cnt);
-    if (tid <= kAccRegions) s_off[tid] = incl - cnt;
+    if (tid < kAccRegions) s_off[tid] = incl - cnt;
+    if (tid == kAccRegions - 1) s_off[kAccRegions] = incl;
   }
   __syncthreads();
   const uint32_t total = s_off[kAccRegions];
@@ -406,9 +451,13 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_acc(FactArgs a) {
     uint32_t r = 0xFFFFFFFFu;
     V val = 0;
     if (live) {
-      uint32_t q = 0;
-#pragma unroll
-      for (int i = 1; i < kAccRegions; ++i) q += e >= s_off[i] ? 1u : 0u;
+      // last region whose start is <= e (empty regions share a start)
+      uint32_t q = 0, hi = kAccRegions - 1;
+      while (q < hi) {
+        const uint32_t mid = (q + hi + 1) >> 1;
+        if (s_off[mid] <= e) q = mid;
+        else hi = mid - 1;
+      }
       const uint64_t at = (r0 + q) * (FJ * 256) + (e - s_off[q]);
       const uint32_t row = a.hit_row[at], sv = a.hit_slot[at];
       V cv[MTL_EXPR_MAX_COLS];
@@ -835,6 +884,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ma.pops = mu + 7;
     ma.has_dim = d->d_rows ? 1u : 0u;
     ma.err = mu + 3;
+    ma.key_vec = ((uintptr_t)d->p_key % (type_width(d->p_key_type) == 1 ? 4 : 16)) == 0;
     uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
     ProfScope ps(s, "k_map");
     hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);

@@ -185,3 +185,83 @@ def star_topk(desc: abi.StarDesc, stream=None):
     n = C.c_uint32(0)
     abi.check(abi.lib().mtl_star_topk(C.byref(desc), out, C.byref(n), stream))
     return [out[i] for i in range(n.value)]
+
+
+# ---------------------------------------------------------------------------
+# Serving loop
+# ---------------------------------------------------------------------------
+def scan_query(preds, n_rows: int, proj=None) -> abi.Query:
+    """preds: list of (device column, type, op, literal); proj: optional
+    (device column, type, op, scalar)."""
+    q = abi.Query()
+    q.kind = abi.Q_SCAN
+    q.n_rows = n_rows
+    q.n_preds = len(preds)
+    for i, (c, t, op, lit) in enumerate(preds):
+        q.preds[i] = abi.make_pred(ptr(c), t, op, lit)
+    if proj is not None:
+        c, t, op, sc = proj
+        q.proj_col = C.c_void_p(ptr(c))
+        q.proj_type = t
+        q.proj_op = op
+        q.proj_scalar = abi.scalar_bits(sc, t)
+    return q
+
+
+def star_query(desc: abi.StarDesc):
+    """Returns (query, output rows buffer); keep the buffer alive until the
+    query completes."""
+    q = abi.Query()
+    q.kind = abi.Q_STAR
+    q.star = desc
+    out = (abi.StarRow * max(1, desc.k))()
+    q.star_out = C.cast(out, C.POINTER(abi.StarRow))
+    return q, out
+
+
+class Server:
+    """mtl_server: n_slots concurrent streams with continuous admission."""
+
+    def __init__(self, n_slots: int, budget: int = 0):
+        self.h = C.c_void_p()
+        abi.check(abi.lib().mtl_server_create(n_slots, budget, C.byref(self.h)))
+
+    def submit(self, q: abi.Query, not_before_ns: int = 0) -> int:
+        t = C.c_uint64()
+        abi.check(abi.lib().mtl_server_submit(self.h, C.byref(q), not_before_ns, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> abi.QueryResult:
+        """The result record; raises MettleError if the query failed."""
+        r = abi.QueryResult()
+        abi.check(abi.lib().mtl_server_wait(self.h, ticket, C.byref(r)))
+        if r.status == abi.MTL_UNSUPPORTED:
+            raise abi.Unsupported(r.status, "query not supported on the GPU path")
+        if r.status != abi.MTL_OK:
+            raise abi.MettleError(r.status, f"query failed with status {r.status}")
+        return r
+
+    def close(self):
+        if self.h:
+            abi.check(abi.lib().mtl_server_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def sort(column: torch.Tensor, type_code: int, n: int, sel: torch.Tensor | None = None,
+         count: torch.Tensor | None = None, desc: bool = False):
+    """ORDER BY column over the selection (or rows 0..n-1); returns (device
+    row ids, count)."""
+    torch.cuda.synchronize()
+    cap = n if sel is None else sel.numel()
+    out = torch.empty(max(cap, 1), dtype=torch.int32, device="cuda")
+    got = C.c_uint32()
+    abi.check(abi.lib().mtl_sort(ptr(column), type_code, ptr(sel), ptr(count), cap,
+                                 1 if desc else 0, ptr(out), C.byref(got), None))
+    torch.cuda.synchronize()
+    return out[:got.value], got.value

@@ -10,6 +10,7 @@ import pytest
 from mettle_amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mettle_hip.h")
 
 
 def declared_functions():
@@ -88,3 +89,32 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.Pred) == 24
     assert C.sizeof(abi.StarRow) == 24
     assert C.sizeof(abi.Expr) == 4 + 4 + 64 + 128 + 4 + 4 + 32 + 16
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler
+    gives the header's struct."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"mtl_pred": abi.Pred, "mtl_expr": abi.Expr, "mtl_star_desc": abi.StarDesc,
+               "mtl_star_row": abi.StarRow, "mtl_agg_spec": abi.AggSpec,
+               "mtl_group_desc": abi.GroupDesc, "mtl_query": abi.Query,
+               "mtl_query_result": abi.QueryResult}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, value = line.split()
+        cls = structs[cname]
+        got = C.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert got == int(value), (cname, field, got, value)

@@ -277,3 +277,55 @@ def test_star_expression_overflow_is_an_error(ops):
     with pytest.raises(abi.MettleError) as ei:
         ops.star_topk(d)
     assert ei.value.code == abi.MTL_ERR_OVERFLOW
+
+
+def _order_key(col: np.ndarray, t: int) -> np.ndarray:
+    """Order-preserving uint64 encoding (DuckDB order: NaN largest, -0 == 0)."""
+    if t in (abi.FLOAT, abi.DOUBLE):
+        f = col.astype(np.float64)
+        f = np.where(f == 0, 0.0, f)
+        b = f.view(np.uint64)
+        neg = (b >> np.uint64(63)) == 1
+        e = np.where(neg, ~b, b | np.uint64(1 << 63))
+        return np.where(np.isnan(f), np.uint64(0xFFFFFFFFFFFFFFFF), e)
+    return col.astype(np.int64).view(np.uint64) ^ np.uint64(1 << 63)
+
+
+@pytest.mark.parametrize("t,dtype", [(abi.INTEGER, np.int32), (abi.BIGINT, np.int64),
+                                     (abi.DATE, np.int32), (abi.TIMESTAMP, np.int64),
+                                     (abi.FLOAT, np.float32), (abi.DOUBLE, np.float64),
+                                     (abi.DICT8, np.uint8)])
+@pytest.mark.parametrize("desc", [False, True])
+def test_sort_parity(ops, t, dtype, desc):
+    rng = np.random.default_rng(300 + t + 10 * desc)
+    n = 200_003
+    if np.issubdtype(dtype, np.floating):
+        col = rng.standard_normal(n).astype(dtype)
+        col[rng.integers(0, n, 500)] = np.nan
+        col[rng.integers(0, n, 500)] = -0.0
+        col[rng.integers(0, n, 500)] = 0.0
+    else:
+        info = np.iinfo(dtype)
+        col = rng.integers(info.min, info.max, n, dtype=dtype, endpoint=True)
+        col[rng.integers(0, n, 20000)] = col[0]  # many ties
+    dcol = ops.to_device(col)
+    # sort a selection (every third row) and the whole column
+    sel = np.arange(0, n, 3, dtype=np.uint32)
+    dsel = ops.to_device(sel.astype(np.int32))
+    cnt = torch.tensor([len(sel)], dtype=torch.int32, device="cuda")
+    for s, ds, dc in ((sel, dsel, cnt), (np.arange(n, dtype=np.uint32), None, None)):
+        k = _order_key(col[s], t)
+        if desc:
+            k = ~k
+        exp = s[np.argsort(k, kind="stable")]
+        got, m = ops.sort(dcol, t, n, sel=ds, count=dc, desc=desc)
+        assert m == len(s)
+        np.testing.assert_array_equal(ops.to_host(got, np.uint32), exp)
+
+
+def test_sort_empty_selection(ops):
+    col = ops.to_device(np.arange(10, dtype=np.int32))
+    sel = ops.to_device(np.zeros(4, dtype=np.int32))
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    got, m = ops.sort(col, abi.INTEGER, 10, sel=sel, count=cnt)
+    assert m == 0
