@@ -158,6 +158,31 @@ int32_t mtl_memcpy_d2h(void *dst_host, const void *src_dev, uint64_t bytes,
                        mtl_stream *s);
 
 /* ====================================================================== */
+/* HBM column cache (replaces column_to_gpu_buffer / cached_input_buffer, */
+/* runtime/mod.rs:398-565, and source_cache.rs:220-317).  Columns are     */
+/* keyed by (table, column) plus a caller version, charged to the cache    */
+/* budget, reference-counted while in use, evicted least-recently-used.   */
+/* Uploads stream through two pinned chunks on a side stream (the staging  */
+/* of pinned.rs:187-244).  Requires gpu_buffer_init.                       */
+/* ====================================================================== */
+/* Cached device copy of `bytes` host bytes; uploads on a miss or a version
+ * change.  Takes a reference (release it with mtl_cache_release). */
+int32_t mtl_cache_put(uint64_t table_id, uint32_t column_id, const void *host,
+                      uint64_t bytes, uint64_t version, const void **out_dev);
+/* Reference an already cached column; MTL_ERR_INVALID_ARGUMENT if absent. */
+int32_t mtl_cache_get(uint64_t table_id, uint32_t column_id,
+                      const void **out_dev, uint64_t *bytes);
+int32_t mtl_cache_release(uint64_t table_id, uint32_t column_id);
+/* Evict every column of a table (fails if one is referenced). */
+int32_t mtl_cache_drop(uint64_t table_id);
+int32_t mtl_cache_stats(uint64_t *entries, uint64_t *bytes, uint64_t *hits,
+                        uint64_t *misses, uint64_t *evictions,
+                        uint64_t *uploaded_bytes);
+/* Evict everything (fails if a column is referenced); called by
+ * mtl_shutdown. */
+int32_t mtl_cache_reset(void);
+
+/* ====================================================================== */
 /* Operators (device pointers in, device pointers out)                    */
 /* ====================================================================== */
 
@@ -423,6 +448,7 @@ typedef struct mtl_query_result {
   int64_t t_submit_ns;  /* steady-clock stamps: arrival (or submit),      */
   int64_t t_start_ns;   /* dispatch to a slot,                            */
   int64_t t_done_ns;    /* completion observed by the host                */
+  char error[160];      /* message when status != MTL_OK                 */
 } mtl_query_result;
 
 typedef struct mtl_server mtl_server;

@@ -150,7 +150,8 @@ class Query(C.Structure):
 class QueryResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("n_rows", C.c_uint32), ("count", C.c_uint64),
                 ("fingerprint", C.c_uint64), ("t_submit_ns", C.c_int64),
-                ("t_start_ns", C.c_int64), ("t_done_ns", C.c_int64)]
+                ("t_start_ns", C.c_int64), ("t_done_ns", C.c_int64),
+                ("error", C.c_char * 160)]
 
 
 def scalar_bits(value, type_code: int) -> int:
@@ -216,6 +217,12 @@ def lib() -> C.CDLL:
         "mtl_compare": (i32, [C.POINTER(Pred), u32, u64, vp, vp]),
         "mtl_gather": (i32, [vp, u32, vp, vp, u64, vp, vp]),
         "mtl_project": (i32, [vp, i32, vp, vp, u64, i32, i64, vp, vp, vp]),
+        "mtl_cache_put": (i32, [u64, u32, vp, u64, u64, C.POINTER(vp)]),
+        "mtl_cache_get": (i32, [u64, u32, C.POINTER(vp), C.POINTER(u64)]),
+        "mtl_cache_release": (i32, [u64, u32]),
+        "mtl_cache_drop": (i32, [u64]),
+        "mtl_cache_stats": (i32, [C.POINTER(u64)] * 6),
+        "mtl_cache_reset": (i32, []),
         "mtl_sort": (i32, [vp, i32, vp, vp, u64, i32, vp, C.POINTER(u32), vp]),
         "mtl_join_table_bytes": (u64, [i32, u64, u64]),
         "mtl_join_build": (i32, [i32, vp, i32, vp, vp, u64, i64, u64, vp, u64, vp, vp]),

@@ -259,6 +259,7 @@ int32_t mtl_budgets(uint64_t* cache_bytes, uint64_t* processing_bytes,
 }
 
 int32_t mtl_shutdown(void) {
+  MTL_TRY(mtl_cache_reset());
   Runtime& rt = runtime();
   std::lock_guard<std::mutex> g(rt.mu);
   if (rt.default_stream) {

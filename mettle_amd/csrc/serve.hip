@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <thread>
@@ -122,7 +123,7 @@ struct Slot {
 };
 
 static int32_t slot_reserve(Slot& sl, uint64_t rows) {
-  if (rows <= sl.cap) return MTL_OK;
+  if (sl.sel && rows <= sl.cap) return MTL_OK;
   MTL_HIP(hipStreamSynchronize(sl.s->stream));
   if (sl.sel) MTL_HIP(hipFree(sl.sel));
   if (sl.proj) MTL_HIP(hipFree(sl.proj));
@@ -226,6 +227,7 @@ static void worker(mtl_server* srv, size_t idx) {
       r.n_rows = n;
     }
     r.status = st;
+    if (st != MTL_OK) snprintf(r.error, sizeof(r.error), "%s", last_error());
     r.t_done_ns = now_ns();
     lk.lock();
     srv->inflight -= need;
@@ -294,6 +296,7 @@ int32_t mtl_server_submit(mtl_server* srv, const mtl_query* q, int64_t not_befor
   if (rq.need > srv->budget) {
     rq.done = true;
     rq.res.status = MTL_ERR_BUDGET;
+    snprintf(rq.res.error, sizeof(rq.res.error), "query needs more device scratch than the budget");
     rq.res.t_start_ns = rq.res.t_done_ns = now;
     srv->reqs.emplace(t, rq);
   } else {

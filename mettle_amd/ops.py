@@ -235,10 +235,11 @@ class Server:
         """The result record; raises MettleError if the query failed."""
         r = abi.QueryResult()
         abi.check(abi.lib().mtl_server_wait(self.h, ticket, C.byref(r)))
+        msg = r.error.decode(errors="replace")
         if r.status == abi.MTL_UNSUPPORTED:
-            raise abi.Unsupported(r.status, "query not supported on the GPU path")
+            raise abi.Unsupported(r.status, msg)
         if r.status != abi.MTL_OK:
-            raise abi.MettleError(r.status, f"query failed with status {r.status}")
+            raise abi.MettleError(r.status, msg)
         return r
 
     def close(self):
