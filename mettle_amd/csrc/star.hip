@@ -171,7 +171,6 @@ struct MapArgs {
   const uint32_t* par_total;  // parent bitmap popcount = accumulator slots
   const uint32_t* pops;       // [0] P pass bits, [1] D bitmap, [2] D pass bits
   uint32_t has_dim;
-  uint32_t key_vec;  // key column 16-byte aligned (4-byte aligned for 1-byte keys)
   uint32_t* err;
 };
 
@@ -192,38 +191,19 @@ __global__ void k_map(MapArgs a) {
     a.acc[i] = 0;
     a.cnt[i] = 0;
   }
-  // 16 consecutive rows per thread: the keys of all 16 rows, then the
-  // bitmap word and rank of every qualifying one, are issued together (two
-  // round trips per thread, not two per qualifying row)
-  const uint32_t kw = (a.key_type == MTL_BIGINT || a.key_type == MTL_TIMESTAMP ||
-                       a.key_type == MTL_DECIMAL) ? 8u : (a.key_type == MTL_DICT8 ? 1u : 4u);
+  // 16 consecutive rows per thread (one 16-bit piece of the pass bits);
+  // measured faster than issuing the key and rank loads of all qualifying
+  // rows together, which costs registers (about 10% of rows qualify)
   for (uint64_t g = t0; g * 16 < a.n_rows; g += nthreads) {
     const uint64_t r0 = g * 16;
-    const uint32_t m = (uint32_t)(a.sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
-    if (!m) continue;
-    uint64_t raw[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) load_raw4(a.key, kw, r0 + 4 * j, a.n_rows, a.key_vec != 0, raw[j]);
-    uint32_t sv[16];
-    uint64_t w[16];
-    uint32_t rk[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const uint64_t x = raw[i >> 2][i & 3];
-      const int64_t key = kw == 8 ? (int64_t)x : (kw == 1 ? (int64_t)x : (int64_t)(int32_t)(uint32_t)x);
-      const uint64_t o = key_off(key, a.min);
-      // rows outside the key range were flagged by the parent pass
-      sv[i] = ((m >> i) & 1u) && o < a.span ? (uint32_t)o : 0xFFFFFFFFu;
-      const bool need = sv[i] != 0xFFFFFFFFu;
-      w[i] = need ? ld_off32(a.bm, (sv[i] >> 6) << 3) : 0ull;
-      rk[i] = need ? ld_off32(a.rank, (sv[i] >> 6) << 2) : 0u;
-    }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      if (sv[i] == 0xFFFFFFFFu) continue;
-      const uint32_t b = sv[i] & 63;
-      const uint64_t below = b ? (w[i] & ((1ull << b) - 1)) : 0ull;
-      a.row_of[rk[i] + (uint32_t)__popcll(below)] = (uint32_t)(r0 + i);
+    uint32_t m = (uint32_t)(a.sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
+    while (m) {
+      int i = __ffs(m) - 1;
+      m &= m - 1;
+      uint64_t row = r0 + (uint64_t)i;
+      uint64_t s = key_off(load_key(a.key, a.key_type, row), a.min);
+      if (s >= a.span) continue;  // flagged by the parent pass
+      a.row_of[rank_of(a.bm, a.rank, s)] = (uint32_t)row;
     }
   }
 }
@@ -884,7 +864,6 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ma.pops = mu + 7;
     ma.has_dim = d->d_rows ? 1u : 0u;
     ma.err = mu + 3;
-    ma.key_vec = ((uintptr_t)d->p_key % (type_width(d->p_key_type) == 1 ? 4 : 16)) == 0;
     uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
     ProfScope ps(s, "k_map");
     hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
