@@ -413,6 +413,63 @@ int32_t mtl_star_topk_async(const mtl_star_desc *desc, void *out_dev,
 int32_t mtl_star_status(uint32_t error_bits);
 
 /* ====================================================================== */
+/* SQL lowering (replaces try_build_gpu_plan, parser.rs:20-120, and the    */
+/* shape routing of try_execute / native_gpu_shape, executor/mod.rs:44-138 */
+/* with its Q3 fast path, executor/fast_path/mod.rs:58-141).  Tables are   */
+/* registered in a catalog of device-resident columns; the admitted SELECT */
+/* subset lowers onto the operators above; anything else returns           */
+/* MTL_UNSUPPORTED so the host runs DuckDB's CPU path.                     */
+/* ====================================================================== */
+int32_t mtl_catalog_add_table(const char *table, uint64_t n_rows);
+/* Column stats: min/max bound every value (join and group keys need them);
+ * unique marks a key column (the "one" side of a join).  scale: DECIMAL. */
+int32_t mtl_catalog_add_column(const char *table, const char *column,
+                               int32_t type, int32_t scale, const void *dev,
+                               int64_t min_value, int64_t max_value,
+                               int32_t unique);
+/* Dictionary of a MTL_DICT8/MTL_DICT32 column: code i = values[i]. */
+int32_t mtl_catalog_set_dictionary(const char *table, const char *column,
+                                   const char *const *values, uint32_t n);
+/* Drop one table, or every table when table == NULL. */
+int32_t mtl_catalog_drop(const char *table);
+/* Plan without running: buf receives the shape ("scan", "group", "star2",
+ * "star3") or, on MTL_UNSUPPORTED, the reason. */
+int32_t mtl_sql_explain(const char *sql, char *buf, size_t len);
+/* Plan and run; the result holds host columns (mtl_result_*). */
+int32_t mtl_sql_execute(const char *sql, mtl_result **out, mtl_stream *s);
+int32_t mtl_result_shape(const mtl_result *r, uint32_t *n_cols,
+                         uint64_t *n_rows);
+/* Column i: its name, type (SUM of integers: MTL_HUGEINT; star SUM:
+ * MTL_DECIMAL with `scale`), and n_rows values of that type's width. */
+int32_t mtl_result_column(const mtl_result *r, uint32_t i, const char **name,
+                          int32_t *type, int32_t *scale, const void **data);
+int32_t mtl_result_free(mtl_result *r);
+
+/* gpu_rewrite_query(sql): the query printed back with every predicate
+ * `col op literal` written as gpu_{eq,lt,le,gt,ge}_scalar(col, literal) and
+ * every projection `col + literal` / `col * literal` as
+ * gpu_{add,mul}_scalar(col, literal).  *needed = bytes including the NUL.
+ * The rewritten text parses and runs through mtl_sql_execute unchanged.
+ * (Named by the reference's README.md:57; absent from its code.) */
+int32_t mtl_rewrite_query(const char *sql, char *buf, size_t len,
+                          size_t *needed);
+
+/* The scalar hooks over one vector of host values (a DuckDB DataChunk
+ * column): gpu_add_scalar / gpu_mul_scalar write values op scalar in the
+ * column's type (integer overflow -> MTL_ERR_OVERFLOW, as DuckDB raises);
+ * gpu_{eq,lt,le,gt,ge}_scalar write uint8 0/1 (floats in DuckDB's total
+ * order).  scalar_bits as in mtl_pred. */
+#define MTL_HOOK_ADD 0
+#define MTL_HOOK_MUL 1
+#define MTL_HOOK_EQ 2
+#define MTL_HOOK_LT 3
+#define MTL_HOOK_LE 4
+#define MTL_HOOK_GT 5
+#define MTL_HOOK_GE 6
+int32_t mtl_scalar_hook(int32_t hook, int32_t type, const void *values,
+                        uint64_t n, int64_t scalar_bits, void *out);
+
+/* ====================================================================== */
 /* Serving loop: continuous admission of queries onto concurrent streams. */
 /* Replaces the one-query-per-CALL materialize_query (processing.rs:      */
 /* 103-143) and stage_query (state.rs:42-54), which ran every query       */

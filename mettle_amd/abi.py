@@ -138,6 +138,7 @@ class StarRow(C.Structure):
 
 Q_SCAN = 0
 Q_STAR = 1
+HOOK_ADD, HOOK_MUL, HOOK_EQ, HOOK_LT, HOOK_LE, HOOK_GT, HOOK_GE = range(7)
 
 
 class Query(C.Structure):
@@ -223,6 +224,18 @@ def lib() -> C.CDLL:
         "mtl_cache_drop": (i32, [u64]),
         "mtl_cache_stats": (i32, [C.POINTER(u64)] * 6),
         "mtl_cache_reset": (i32, []),
+        "mtl_catalog_add_table": (i32, [C.c_char_p, u64]),
+        "mtl_catalog_add_column": (i32, [C.c_char_p, C.c_char_p, i32, i32, vp, i64, i64, i32]),
+        "mtl_catalog_set_dictionary": (i32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), u32]),
+        "mtl_catalog_drop": (i32, [C.c_char_p]),
+        "mtl_sql_explain": (i32, [C.c_char_p, C.c_char_p, C.c_size_t]),
+        "mtl_sql_execute": (i32, [C.c_char_p, C.POINTER(vp), vp]),
+        "mtl_result_shape": (i32, [vp, C.POINTER(u32), C.POINTER(u64)]),
+        "mtl_result_column": (i32, [vp, u32, C.POINTER(C.c_char_p), C.POINTER(i32),
+                                    C.POINTER(i32), C.POINTER(vp)]),
+        "mtl_result_free": (i32, [vp]),
+        "mtl_rewrite_query": (i32, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "mtl_scalar_hook": (i32, [i32, i32, vp, u64, i64, vp]),
         "mtl_sort": (i32, [vp, i32, vp, vp, u64, i32, vp, C.POINTER(u32), vp]),
         "mtl_join_table_bytes": (u64, [i32, u64, u64]),
         "mtl_join_build": (i32, [i32, vp, i32, vp, vp, u64, i64, u64, vp, u64, vp, vp]),

@@ -1,0 +1,1629 @@
+// SQL lowering: the admitted subset of SELECT is parsed, bound against a
+// catalog of device-resident tables, and lowered onto the operators of this
+// library; anything outside the subset returns MTL_UNSUPPORTED so the host
+// runs DuckDB's CPU path.
+//
+// Reference: try_build_gpu_plan (parser.rs:20-120) admits one SELECT with
+// comma or INNER JOIN sources, AND-ed `col op literal` / BETWEEN filters,
+// equi-join conditions, GROUP BY columns, SUM/COUNT/MIN/MAX/AVG, ORDER BY
+// and LIMIT (parse_where_clause :261-341, extract_literal :364-400, date
+// literals :403-431); native_gpu_shape / try_execute (executor/mod.rs:44-138)
+// route the plan; the Q3 fast path (executor/fast_path/mod.rs:58-141)
+// handles the star join.  The three shapes lowered here:
+//   scan  : one table, filters, column / `col op literal` projections,
+//           ORDER BY one column, LIMIT       -> mtl_filter, mtl_sort,
+//                                               mtl_gather, mtl_project
+//   group : one table, GROUP BY integer columns, aggregates over columns,
+//           optional ORDER BY one aggregate + LIMIT -> mtl_group_agg, topk
+//   star  : fact -> parent (-> dimension) equi-joins on unique keys, GROUP
+//           BY parent columns including its key, SUM(expression over fact
+//           columns), ORDER BY the sum then parent columns, LIMIT
+//                                            -> mtl_star_topk
+// Unlike the reference, literals keep the column's exact type (DATE days,
+// TIMESTAMP microseconds, DECIMAL scaled integers, dictionary codes), and
+// results carry DuckDB's types (exact integer sums, no float narrowing).
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mtl_internal.hpp"
+
+// ---------------------------------------------------------------------------
+// Results (host-side materialised columns)
+// ---------------------------------------------------------------------------
+struct mtl_result {
+  struct Column {
+    std::string name;
+    int32_t type;
+    int32_t scale = 0;  // DECIMAL scale
+    std::vector<uint8_t> data;
+  };
+  std::vector<Column> cols;
+  uint64_t rows = 0;
+  std::string shape;
+};
+
+namespace mtl {
+namespace sql {
+
+// ---------------------------------------------------------------------------
+// Lexer
+// ---------------------------------------------------------------------------
+struct Tok {
+  enum Kind { ID, NUM, STR, OP, END } k;
+  std::string s;
+};
+
+static std::string lower(std::string s) {
+  for (auto& c : s) c = (char)std::tolower((unsigned char)c);
+  return s;
+}
+
+static bool lex(const std::string& in, std::vector<Tok>& out, std::string& err) {
+  size_t i = 0;
+  while (i < in.size()) {
+    const char c = in[i];
+    if (std::isspace((unsigned char)c)) {
+      ++i;
+    } else if (std::isalpha((unsigned char)c) || c == '_') {
+      size_t j = i;
+      while (j < in.size() && (std::isalnum((unsigned char)in[j]) || in[j] == '_')) ++j;
+      out.push_back({Tok::ID, lower(in.substr(i, j - i))});
+      i = j;
+    } else if (c == '"') {
+      size_t j = in.find('"', i + 1);
+      if (j == std::string::npos) return err = "unterminated identifier", false;
+      out.push_back({Tok::ID, lower(in.substr(i + 1, j - i - 1))});
+      i = j + 1;
+    } else if (std::isdigit((unsigned char)c) ||
+               (c == '.' && i + 1 < in.size() && std::isdigit((unsigned char)in[i + 1]))) {
+      size_t j = i;
+      while (j < in.size() && (std::isdigit((unsigned char)in[j]) || in[j] == '.')) ++j;
+      if (j < in.size() && (in[j] == 'e' || in[j] == 'E')) {
+        ++j;
+        if (j < in.size() && (in[j] == '+' || in[j] == '-')) ++j;
+        while (j < in.size() && std::isdigit((unsigned char)in[j])) ++j;
+      }
+      out.push_back({Tok::NUM, in.substr(i, j - i)});
+      i = j;
+    } else if (c == '\'') {
+      std::string s;
+      size_t j = i + 1;
+      while (true) {
+        if (j >= in.size()) return err = "unterminated string", false;
+        if (in[j] == '\'') {
+          if (j + 1 < in.size() && in[j + 1] == '\'') {
+            s += '\'';
+            j += 2;
+            continue;
+          }
+          break;
+        }
+        s += in[j++];
+      }
+      out.push_back({Tok::STR, s});
+      i = j + 1;
+    } else {
+      static const char* two[] = {"<=", ">=", "<>", "!="};
+      bool done = false;
+      for (const char* t : two)
+        if (in.compare(i, 2, t) == 0) {
+          out.push_back({Tok::OP, t});
+          i += 2;
+          done = true;
+          break;
+        }
+      if (done) continue;
+      if (std::strchr("=<>+-*/(),.;", c)) {
+        out.push_back({Tok::OP, std::string(1, c)});
+        ++i;
+      } else {
+        return err = std::string("unexpected character '") + c + "'", false;
+      }
+    }
+  }
+  out.push_back({Tok::END, ""});
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// AST
+// ---------------------------------------------------------------------------
+struct Lit {
+  enum Kind { INT, FLT, STR, DATE, TS } k = INT;
+  int64_t i = 0;
+  double f = 0;
+  std::string s;
+  std::string text;  // as written (numbers), for printing
+};
+
+struct ColRef {
+  std::string rel;  // table name or alias ("" = unqualified)
+  std::string name;
+};
+
+struct Expr;
+using ExprP = std::shared_ptr<Expr>;
+struct Expr {
+  enum Kind { COL, LIT, ADD, SUB, MUL, AGG, STAR, CALL } k;
+  ColRef col;
+  Lit lit;
+  std::string fn;  // AGG: sum/count/min/max/avg; CALL: hook name
+  ExprP a, b;
+};
+
+struct Pred {
+  ColRef col;
+  int op;  // MTL_CMP_*
+  Lit lit;
+};
+
+struct JoinEq {
+  ColRef l, r;
+};
+
+struct Item {
+  ExprP e;
+  std::string alias;
+};
+
+struct OrderTerm {
+  ExprP e;
+  bool desc = false;
+};
+
+struct Query {
+  std::vector<Item> items;
+  std::vector<std::pair<std::string, std::string>> from;  // (table, alias)
+  std::vector<Pred> preds;
+  std::vector<JoinEq> joins;
+  std::vector<ColRef> group;
+  std::vector<OrderTerm> order;
+  int64_t limit = -1;
+};
+
+// ---------------------------------------------------------------------------
+// Parser (recursive descent over the admitted subset)
+// ---------------------------------------------------------------------------
+struct Parser {
+  const std::vector<Tok>& t;
+  size_t p = 0;
+  std::string err;
+  explicit Parser(const std::vector<Tok>& tt) : t(tt) {}
+
+  const Tok& cur() const { return t[p]; }
+  bool is_kw(const char* k) const { return cur().k == Tok::ID && cur().s == k; }
+  bool is_op(const char* o) const { return cur().k == Tok::OP && cur().s == o; }
+  bool accept_kw(const char* k) {
+    if (!is_kw(k)) return false;
+    ++p;
+    return true;
+  }
+  bool accept_op(const char* o) {
+    if (!is_op(o)) return false;
+    ++p;
+    return true;
+  }
+  bool fail(const std::string& m) {
+    if (err.empty()) err = m;
+    return false;
+  }
+  bool expect_kw(const char* k) { return accept_kw(k) || fail(std::string("expected ") + k); }
+  bool expect_op(const char* o) { return accept_op(o) || fail(std::string("expected '") + o + "'"); }
+
+  static bool reserved(const std::string& s) {
+    static const char* kw[] = {"select", "from", "where", "group", "by", "order", "limit",
+                               "and", "or", "not", "as", "join", "inner", "on", "asc", "desc",
+                               "between", "having", "distinct", "union", "left", "right",
+                               "full", "outer", "cross", "with", "offset", nullptr};
+    for (int i = 0; kw[i]; ++i)
+      if (s == kw[i]) return true;
+    return false;
+  }
+
+  bool literal(Lit& l) {
+    bool neg = false;
+    if (is_op("-")) {
+      ++p;
+      neg = true;
+    }
+    if (cur().k == Tok::NUM) {
+      const std::string s = cur().s;
+      ++p;
+      l.text = (neg ? "-" : "") + s;
+      if (s.find_first_of(".eE") == std::string::npos) {
+        errno = 0;
+        char* end = nullptr;
+        unsigned long long u = std::strtoull(s.c_str(), &end, 10);
+        if (errno || *end) return fail("integer literal out of range");
+        if (!neg && u > (unsigned long long)INT64_MAX) return fail("integer literal out of range");
+        if (neg && u > (unsigned long long)INT64_MAX + 1ull)
+          return fail("integer literal out of range");
+        l.k = Lit::INT;
+        l.i = neg ? (int64_t)(0ull - u) : (int64_t)u;
+      } else {
+        l.k = Lit::FLT;
+        l.f = std::strtod(s.c_str(), nullptr);
+        if (neg) l.f = -l.f;
+      }
+      return true;
+    }
+    if (neg) return fail("expected a number after '-'");
+    if (cur().k == Tok::STR) {
+      l.k = Lit::STR;
+      l.s = cur().s;
+      ++p;
+      return true;
+    }
+    if ((is_kw("date") || is_kw("timestamp")) && t[p + 1].k == Tok::STR) {
+      const bool ts = is_kw("timestamp");
+      p += 1;
+      l.k = ts ? Lit::TS : Lit::DATE;
+      l.s = cur().s;
+      ++p;
+      return true;
+    }
+    return fail("expected a literal");
+  }
+
+  bool colref(ColRef& c) {
+    if (cur().k != Tok::ID || reserved(cur().s)) return fail("expected a column");
+    std::string a = cur().s;
+    ++p;
+    if (accept_op(".")) {
+      if (cur().k != Tok::ID) return fail("expected a column after '.'");
+      c.rel = a;
+      c.name = cur().s;
+      ++p;
+    } else {
+      c.name = a;
+    }
+    return true;
+  }
+
+  bool is_literal_start() const {
+    if (cur().k == Tok::NUM || cur().k == Tok::STR) return true;
+    if (is_op("-") && t[p + 1].k == Tok::NUM) return true;
+    if ((is_kw("date") || is_kw("timestamp")) && t[p + 1].k == Tok::STR) return true;
+    return false;
+  }
+
+  bool factor(ExprP& e) {
+    e = std::make_shared<Expr>();
+    if (accept_op("(")) {
+      if (!expr(e)) return false;
+      return expect_op(")");
+    }
+    if (is_literal_start()) {
+      e->k = Expr::LIT;
+      return literal(e->lit);
+    }
+    if (cur().k == Tok::ID && t[p + 1].k == Tok::OP && t[p + 1].s == "(") {
+      const std::string fn = cur().s;
+      p += 2;
+      if (fn == "sum" || fn == "count" || fn == "min" || fn == "max" || fn == "avg") {
+        e->k = Expr::AGG;
+        e->fn = fn;
+        if (fn == "count" && accept_op("*")) {
+          e->a = nullptr;
+        } else {
+          if (accept_kw("distinct")) return fail("DISTINCT aggregates are not supported");
+          if (!expr(e->a)) return false;
+        }
+        return expect_op(")");
+      }
+      if (fn.rfind("gpu_", 0) == 0) {  // the scalar hooks
+        e->k = Expr::CALL;
+        e->fn = fn;
+        if (!expr(e->a) || !expect_op(",") || !expr(e->b)) return false;
+        return expect_op(")");
+      }
+      return fail("function " + fn + " is not supported");
+    }
+    e->k = Expr::COL;
+    return colref(e->col);
+  }
+
+  bool term(ExprP& e) {
+    if (!factor(e)) return false;
+    while (is_op("*") || is_op("/")) {
+      if (is_op("/")) return fail("division is not supported");
+      ++p;
+      auto n = std::make_shared<Expr>();
+      n->k = Expr::MUL;
+      n->a = e;
+      if (!factor(n->b)) return false;
+      e = n;
+    }
+    return true;
+  }
+
+  bool expr(ExprP& e) {
+    if (!term(e)) return false;
+    while (is_op("+") || is_op("-")) {
+      auto n = std::make_shared<Expr>();
+      n->k = is_op("+") ? Expr::ADD : Expr::SUB;
+      ++p;
+      n->a = e;
+      if (!term(n->b)) return false;
+      e = n;
+    }
+    return true;
+  }
+
+  static int cmp_op(const std::string& s) {
+    if (s == "=") return MTL_CMP_EQ;
+    if (s == "<") return MTL_CMP_LT;
+    if (s == "<=") return MTL_CMP_LE;
+    if (s == ">") return MTL_CMP_GT;
+    if (s == ">=") return MTL_CMP_GE;
+    return -1;
+  }
+  static int flip(int op) {
+    switch (op) {
+      case MTL_CMP_LT: return MTL_CMP_GT;
+      case MTL_CMP_LE: return MTL_CMP_GE;
+      case MTL_CMP_GT: return MTL_CMP_LT;
+      case MTL_CMP_GE: return MTL_CMP_LE;
+      default: return op;
+    }
+  }
+
+  // One conjunct of WHERE: col op lit, lit op col, col = col, col BETWEEN,
+  // hook call, or a parenthesised conjunction.
+  bool condition(Query& q) {
+    if (is_op("(")) {
+      size_t save = p;
+      ++p;
+      if (conjunction(q) && accept_op(")")) return true;
+      p = save;
+      err.clear();
+      return fail("unsupported parenthesised condition");
+    }
+    if (cur().k == Tok::ID && cur().s.rfind("gpu_", 0) == 0 && t[p + 1].s == "(") {
+      // gpu_<op>_scalar(col, literal) = the predicate itself
+      const std::string fn = cur().s;
+      p += 2;
+      Pred pr;
+      static const std::pair<const char*, int> hooks[] = {
+          {"gpu_eq_scalar", MTL_CMP_EQ}, {"gpu_lt_scalar", MTL_CMP_LT},
+          {"gpu_le_scalar", MTL_CMP_LE}, {"gpu_gt_scalar", MTL_CMP_GT},
+          {"gpu_ge_scalar", MTL_CMP_GE}};
+      pr.op = -1;
+      for (auto& h : hooks)
+        if (fn == h.first) pr.op = h.second;
+      if (pr.op < 0) return fail(fn + " is not a predicate hook");
+      if (!colref(pr.col) || !expect_op(",") || !literal(pr.lit) || !expect_op(")"))
+        return false;
+      q.preds.push_back(pr);
+      return true;
+    }
+    if (is_literal_start()) {
+      Pred pr;
+      if (!literal(pr.lit)) return false;
+      if (cur().k != Tok::OP || cmp_op(cur().s) < 0) return fail("expected a comparison");
+      pr.op = flip(cmp_op(cur().s));
+      ++p;
+      if (!colref(pr.col)) return false;
+      q.preds.push_back(pr);
+      return true;
+    }
+    ColRef c;
+    if (!colref(c)) return false;
+    if (accept_kw("between")) {
+      Pred lo, hi;
+      lo.col = hi.col = c;
+      lo.op = MTL_CMP_GE;
+      hi.op = MTL_CMP_LE;
+      if (!literal(lo.lit) || !expect_kw("and") || !literal(hi.lit)) return false;
+      q.preds.push_back(lo);
+      q.preds.push_back(hi);
+      return true;
+    }
+    if (cur().k != Tok::OP || cmp_op(cur().s) < 0) return fail("expected a comparison");
+    const int op = cmp_op(cur().s);
+    ++p;
+    if (is_literal_start()) {
+      Pred pr;
+      pr.col = c;
+      pr.op = op;
+      if (!literal(pr.lit)) return false;
+      q.preds.push_back(pr);
+      return true;
+    }
+    ColRef d;
+    if (!colref(d)) return false;
+    if (op != MTL_CMP_EQ) return fail("only equality between columns (joins) is supported");
+    q.joins.push_back({c, d});
+    return true;
+  }
+
+  bool conjunction(Query& q) {
+    if (!condition(q)) return false;
+    while (accept_kw("and"))
+      if (!condition(q)) return false;
+    if (is_kw("or")) return fail("OR is not supported");
+    return true;
+  }
+
+  bool table_ref(Query& q) {
+    if (cur().k != Tok::ID || reserved(cur().s)) return fail("expected a table");
+    std::string name = cur().s, alias;
+    ++p;
+    if (accept_op(".")) {  // schema.table: keep the table
+      if (cur().k != Tok::ID) return fail("expected a table");
+      name = cur().s;
+      ++p;
+    }
+    if (accept_kw("as")) {
+      if (cur().k != Tok::ID) return fail("expected an alias");
+      alias = cur().s;
+      ++p;
+    } else if (cur().k == Tok::ID && !reserved(cur().s)) {
+      alias = cur().s;
+      ++p;
+    }
+    q.from.push_back({name, alias});
+    return true;
+  }
+
+  bool query(Query& q) {
+    if (!expect_kw("select")) return false;
+    if (is_kw("distinct")) return fail("DISTINCT is not supported");
+    do {
+      Item it;
+      if (accept_op("*")) {
+        it.e = std::make_shared<Expr>();
+        it.e->k = Expr::STAR;
+      } else if (!expr(it.e)) {
+        return false;
+      }
+      if (accept_kw("as")) {
+        if (cur().k != Tok::ID) return fail("expected an alias");
+        it.alias = cur().s;
+        ++p;
+      } else if (cur().k == Tok::ID && !reserved(cur().s)) {
+        it.alias = cur().s;
+        ++p;
+      }
+      q.items.push_back(it);
+    } while (accept_op(","));
+    if (!expect_kw("from")) return false;
+    if (!table_ref(q)) return false;
+    while (true) {
+      if (accept_op(",")) {
+        if (!table_ref(q)) return false;
+      } else if (is_kw("join") || is_kw("inner")) {
+        accept_kw("inner");
+        if (!expect_kw("join") || !table_ref(q) || !expect_kw("on")) return false;
+        ColRef a, b;
+        if (!colref(a) || !expect_op("=") || !colref(b)) return false;
+        q.joins.push_back({a, b});
+        while (accept_kw("and"))
+          if (!condition(q)) return false;
+      } else {
+        break;
+      }
+    }
+    if (accept_kw("where") && !conjunction(q)) return false;
+    if (accept_kw("group")) {
+      if (!expect_kw("by")) return false;
+      do {
+        ColRef c;
+        if (!colref(c)) return false;
+        q.group.push_back(c);
+      } while (accept_op(","));
+    }
+    if (is_kw("having")) return fail("HAVING is not supported");
+    if (accept_kw("order")) {
+      if (!expect_kw("by")) return false;
+      do {
+        OrderTerm o;
+        if (!expr(o.e)) return false;
+        if (accept_kw("desc")) o.desc = true;
+        else accept_kw("asc");
+        if (is_kw("nulls")) return fail("NULLS FIRST/LAST is not supported");
+        q.order.push_back(o);
+      } while (accept_op(","));
+    }
+    if (accept_kw("limit")) {
+      if (cur().k != Tok::NUM) return fail("expected a LIMIT count");
+      q.limit = std::strtoll(cur().s.c_str(), nullptr, 10);
+      ++p;
+    }
+    if (is_kw("offset")) return fail("OFFSET is not supported");
+    accept_op(";");
+    if (cur().k != Tok::END) return fail("unexpected '" + cur().s + "'");
+    return true;
+  }
+};
+
+// Days since 1970-01-01 of a proper calendar date (proleptic Gregorian),
+// the civil-from-days inverse the planner used (parser.rs:406-431).
+static bool parse_date(const std::string& s, int64_t& days) {
+  int y, m, d;
+  char tail;
+  if (std::sscanf(s.c_str(), "%d-%d-%d%c", &y, &m, &d, &tail) != 3) return false;
+  static const int mdays[] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+  const bool leap = (y % 4 == 0 && y % 100 != 0) || y % 400 == 0;
+  if (m < 1 || m > 12 || d < 1 || d > mdays[m - 1] + (m == 2 && leap)) return false;
+  const int64_t yy = y - (m <= 2);
+  const int64_t era = (yy >= 0 ? yy : yy - 399) / 400;
+  const int64_t yoe = yy - era * 400;
+  const int64_t mp = m + (m > 2 ? -3 : 9);
+  const int64_t doy = (153 * mp + 2) / 5 + d - 1;
+  const int64_t doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
+  days = era * 146097 + doe - 719468;
+  return true;
+}
+
+static bool parse_timestamp(const std::string& s, int64_t& us) {
+  int64_t days;
+  const std::string date = s.substr(0, s.find_first_of(" T"));
+  if (!parse_date(date, days)) return false;
+  int hh = 0, mm = 0;
+  double ss = 0;
+  if (date.size() < s.size()) {
+    if (std::sscanf(s.c_str() + date.size() + 1, "%d:%d:%lf", &hh, &mm, &ss) < 2) return false;
+    if (hh < 0 || hh > 23 || mm < 0 || mm > 59 || ss < 0 || ss >= 60) return false;
+  }
+  us = ((days * 24 + hh) * 60 + mm) * 60 * 1000000ll + (int64_t)std::llround(ss * 1e6);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Catalog of device-resident tables
+// ---------------------------------------------------------------------------
+struct CatCol {
+  std::string name;
+  int32_t type = 0;
+  int32_t scale = 0;  // DECIMAL
+  const void* dev = nullptr;
+  int64_t min = 0, max = 0;
+  bool unique = false;
+  std::vector<std::string> dict;  // dictionary columns: code -> value
+};
+struct CatTable {
+  std::string name;
+  uint64_t rows = 0;
+  std::vector<CatCol> cols;
+  const CatCol* find(const std::string& n) const {
+    for (auto& c : cols)
+      if (c.name == n) return &c;
+    return nullptr;
+  }
+};
+
+static std::mutex g_cat_mu;
+static std::map<std::string, CatTable>& catalog() {
+  static std::map<std::string, CatTable> c;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Binding
+// ---------------------------------------------------------------------------
+struct Bound {
+  std::vector<const CatTable*> tabs;
+  std::vector<std::string> names;  // alias or table name, per source
+  std::string err;
+
+  int source_of(const std::string& rel) const {
+    for (size_t i = 0; i < tabs.size(); ++i)
+      if (names[i] == rel || tabs[i]->name == rel) return (int)i;
+    return -1;
+  }
+  // resolve a column to (source, column); unqualified must be unambiguous
+  bool resolve(const ColRef& c, int& src, const CatCol*& col) {
+    src = -1;
+    col = nullptr;
+    if (!c.rel.empty()) {
+      src = source_of(c.rel);
+      if (src < 0) return err = "unknown table " + c.rel, false;
+      col = tabs[src]->find(c.name);
+      if (!col) return err = "unknown column " + c.rel + "." + c.name, false;
+      return true;
+    }
+    for (size_t i = 0; i < tabs.size(); ++i)
+      if (auto* cc = tabs[i]->find(c.name)) {
+        if (col) return err = "ambiguous column " + c.name, false;
+        src = (int)i;
+        col = cc;
+      }
+    if (!col) return err = "unknown column " + c.name, false;
+    return true;
+  }
+};
+
+static bool pow10_scale(int64_t v, int sc, int64_t& out) {
+  out = v;
+  for (int i = 0; i < sc; ++i)
+    if (__builtin_mul_overflow(out, (int64_t)10, &out)) return false;
+  return true;
+}
+
+// Literal -> the int64 bits mtl_pred expects for the column.  Integer
+// literals compare exactly (DuckDB widens, never truncates).
+static bool literal_bits(const Lit& l, const CatCol& c, int op, int64_t& bits, std::string& err) {
+  switch (c.type) {
+    case MTL_FLOAT:
+    case MTL_DOUBLE: {
+      double d;
+      if (l.k == Lit::INT) d = (double)l.i;
+      else if (l.k == Lit::FLT) d = l.f;
+      else return err = "non-numeric literal for " + c.name, false;
+      std::memcpy(&bits, &d, 8);
+      return true;
+    }
+    case MTL_DATE:
+      if (l.k == Lit::DATE || l.k == Lit::STR) {
+        if (!parse_date(l.s, bits)) return err = "invalid date '" + l.s + "'", false;
+        return true;
+      }
+      return err = "DATE column compared with a non-date literal", false;
+    case MTL_TIMESTAMP:
+      if (l.k == Lit::TS || l.k == Lit::DATE || l.k == Lit::STR) {
+        if (!parse_timestamp(l.s, bits)) return err = "invalid timestamp '" + l.s + "'", false;
+        return true;
+      }
+      return err = "TIMESTAMP column compared with a non-timestamp literal", false;
+    case MTL_DECIMAL: {
+      if (l.k == Lit::INT) {
+        if (!pow10_scale(l.i, c.scale, bits)) return err = "decimal literal out of range", false;
+        return true;
+      }
+      if (l.k == Lit::FLT) {  // exact only when the literal has <= scale decimals
+        const double scaled = l.f * std::pow(10.0, c.scale);
+        if (scaled != std::nearbyint(scaled) || std::fabs(scaled) > 9.0e15)
+          return err = "decimal literal needs more scale than the column", false;
+        bits = (int64_t)std::nearbyint(scaled);
+        return true;
+      }
+      return err = "non-numeric literal for a DECIMAL column", false;
+    }
+    case MTL_DICT8:
+    case MTL_DICT32:
+      if (l.k != Lit::STR) return err = "dictionary column compared with a non-string", false;
+      if (op != MTL_CMP_EQ) return err = "only '=' on dictionary columns", false;
+      for (size_t i = 0; i < c.dict.size(); ++i)
+        if (c.dict[i] == l.s) {
+          bits = (int64_t)i;
+          return true;
+        }
+      bits = (int64_t)c.dict.size();  // matches no row
+      if (c.type == MTL_DICT8 && bits > 255) bits = 255;
+      return true;
+    default:  // INTEGER, BIGINT
+      if (l.k == Lit::INT) {
+        bits = l.i;
+        return true;
+      }
+      return err = "integer column compared with a non-integer literal", false;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Execution helpers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  int32_t alloc(uint64_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
+      return set_error(MTL_ERR_PROCESSING, "processing error: hipMalloc of %llu bytes",
+                       (unsigned long long)bytes);
+    return MTL_OK;
+  }
+};
+
+static int32_t download(std::vector<uint8_t>& dst, const void* dev, uint64_t bytes,
+                        mtl_stream* s) {
+  dst.resize(bytes);
+  if (!bytes) return MTL_OK;
+  return mtl_memcpy_d2h(dst.data(), dev, bytes, s);
+}
+
+static bool is_int_type(int32_t t) {
+  return t == MTL_INTEGER || t == MTL_BIGINT || t == MTL_DATE || t == MTL_TIMESTAMP ||
+         t == MTL_DECIMAL || t == MTL_DICT8 || t == MTL_DICT32;
+}
+
+// gpu_add_scalar / gpu_mul_scalar calls and col op literal -> projection
+static bool as_projection(const ExprP& e, ColRef& col, int& op, Lit& lit) {
+  if (e->k == Expr::CALL && (e->fn == "gpu_add_scalar" || e->fn == "gpu_mul_scalar")) {
+    if (e->a->k != Expr::COL || e->b->k != Expr::LIT) return false;
+    col = e->a->col;
+    op = e->fn == "gpu_add_scalar" ? MTL_PROJ_ADD : MTL_PROJ_MUL;
+    lit = e->b->lit;
+    return true;
+  }
+  if (e->k != Expr::ADD && e->k != Expr::SUB && e->k != Expr::MUL) return false;
+  if (e->a->k == Expr::COL && e->b->k == Expr::LIT) {
+    col = e->a->col;
+    lit = e->b->lit;
+    op = e->k == Expr::ADD ? MTL_PROJ_ADD : (e->k == Expr::MUL ? MTL_PROJ_MUL : MTL_PROJ_SUB);
+    return true;
+  }
+  if (e->a->k == Expr::LIT && e->b->k == Expr::COL) {
+    col = e->b->col;
+    lit = e->a->lit;
+    op = e->k == Expr::ADD ? MTL_PROJ_ADD : (e->k == Expr::MUL ? MTL_PROJ_MUL : MTL_PROJ_SUBFROM);
+    return true;
+  }
+  return false;
+}
+
+static std::string item_name(const Item& it, size_t i) {
+  if (!it.alias.empty()) return it.alias;
+  if (it.e->k == Expr::COL) return it.e->col.name;
+  return "col" + std::to_string(i);
+}
+
+// ---------------------------------------------------------------------------
+// Shape 1: scan (filter -> ORDER BY -> LIMIT -> projections)
+// ---------------------------------------------------------------------------
+static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
+  const CatTable& T = *b.tabs[0];
+  std::vector<mtl_pred> preds;
+  for (auto& pr : q.preds) {
+    int src;
+    const CatCol* c;
+    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    mtl_pred mp{};
+    mp.column = c->dev;
+    mp.type = c->type;
+    mp.op = pr.op;
+    std::string err;
+    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
+      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+    preds.push_back(mp);
+  }
+  if (preds.size() > MTL_MAX_PREDS) return set_error(MTL_UNSUPPORTED, "too many predicates");
+  // outputs
+  struct Out {
+    std::string name;
+    const CatCol* c;
+    int op = -1;  // projection op, -1 = plain column
+    int64_t scalar = 0;
+  };
+  std::vector<Out> outs;
+  for (size_t i = 0; i < q.items.size(); ++i) {
+    const Item& it = q.items[i];
+    if (it.e->k == Expr::STAR) {
+      for (auto& c : T.cols) outs.push_back({c.name, &c});
+      continue;
+    }
+    Out o;
+    o.name = item_name(it, i);
+    int src;
+    if (it.e->k == Expr::COL) {
+      if (!b.resolve(it.e->col, src, o.c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    } else {
+      ColRef cr;
+      Lit lit;
+      if (!as_projection(it.e, cr, o.op, lit))
+        return set_error(MTL_UNSUPPORTED, "select item %zu is not a column or col op literal", i);
+      if (!b.resolve(cr, src, o.c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+      if (o.c->type == MTL_DICT8 || o.c->type == MTL_DICT32 || o.c->type == MTL_TIMESTAMP)
+        return set_error(MTL_UNSUPPORTED, "arithmetic on column %s", o.c->name.c_str());
+      if (o.c->type == MTL_DATE && o.op != MTL_PROJ_ADD && o.op != MTL_PROJ_SUB)
+        return set_error(MTL_UNSUPPORTED, "DATE supports only + and - of days");
+      if (is_int_type(o.c->type)) {
+        if (lit.k != Lit::INT) return set_error(MTL_UNSUPPORTED, "non-integer literal");
+        o.scalar = lit.i;
+        if (o.c->type == MTL_DECIMAL && (o.op == MTL_PROJ_ADD || o.op == MTL_PROJ_SUB ||
+                                         o.op == MTL_PROJ_SUBFROM) &&
+            !pow10_scale(lit.i, o.c->scale, o.scalar))
+          return set_error(MTL_UNSUPPORTED, "decimal literal out of range");
+      } else {
+        double d = lit.k == Lit::INT ? (double)lit.i : lit.f;
+        if (lit.k != Lit::INT && lit.k != Lit::FLT)
+          return set_error(MTL_UNSUPPORTED, "non-numeric literal");
+        std::memcpy(&o.scalar, &d, 8);
+      }
+    }
+    outs.push_back(o);
+  }
+  const CatCol* okey = nullptr;
+  bool odesc = false;
+  if (q.order.size() > 1) return set_error(MTL_UNSUPPORTED, "ORDER BY more than one column");
+  if (q.order.size() == 1) {
+    const ExprP& e = q.order[0].e;
+    if (e->k != Expr::COL) return set_error(MTL_UNSUPPORTED, "ORDER BY an expression");
+    int src;
+    // an output alias of a plain column, or a table column
+    for (auto& o : outs)
+      if (e->col.rel.empty() && o.name == e->col.name && o.op < 0) okey = o.c;
+    if (!okey && !b.resolve(e->col, src, okey))
+      return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    odesc = q.order[0].desc;
+  }
+  res.shape = "scan";
+  if (dry) return MTL_OK;
+  // filter
+  DevBuf sel, cnt, perm, out, ovf;
+  MTL_TRY(sel.alloc(std::max<uint64_t>(T.rows, 1) * 4));
+  MTL_TRY(cnt.alloc(16));
+  MTL_TRY(mtl_filter(preds.data(), (uint32_t)preds.size(), T.rows, nullptr, nullptr, 0,
+                     (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
+  uint32_t n = 0;
+  const uint32_t* rows = (const uint32_t*)sel.p;
+  if (okey) {
+    MTL_TRY(perm.alloc(std::max<uint64_t>(T.rows, 1) * 4));
+    MTL_TRY(mtl_sort(okey->dev, okey->type, (const uint32_t*)sel.p, (const uint32_t*)cnt.p,
+                     T.rows, odesc ? 1 : 0, (uint32_t*)perm.p, &n, s));
+    rows = (const uint32_t*)perm.p;
+  } else {
+    MTL_TRY(mtl_memcpy_d2h(&n, cnt.p, 4, s));
+  }
+  if (q.limit >= 0 && (uint64_t)q.limit < n) n = (uint32_t)q.limit;
+  res.rows = n;
+  MTL_TRY(out.alloc((uint64_t)std::max<uint32_t>(n, 1) * 8));
+  MTL_TRY(ovf.alloc(16));
+  for (auto& o : outs) {
+    mtl_result::Column col;
+    col.name = o.name;
+    col.type = o.c->type;
+    col.scale = o.c->scale;
+    const uint32_t w = (uint32_t)type_width(o.c->type);
+    if (o.op < 0) {
+      MTL_TRY(mtl_gather(o.c->dev, w, rows, nullptr, n, out.p, s));
+    } else {
+      uint32_t zero = 0;
+      MTL_TRY(mtl_memcpy_h2d(ovf.p, &zero, 4, s));
+      MTL_TRY(mtl_project(o.c->dev, o.c->type, rows, nullptr, n, o.op, o.scalar, out.p,
+                          (uint32_t*)ovf.p, s));
+      uint32_t flag = 0;
+      MTL_TRY(mtl_memcpy_d2h(&flag, ovf.p, 4, s));
+      if (flag) return set_error(MTL_ERR_OVERFLOW, "Overflow in %s of %s",
+                                 o.op == MTL_PROJ_MUL ? "multiplication" : "addition",
+                                 o.c->name.c_str());
+    }
+    MTL_TRY(download(col.data, out.p, (uint64_t)n * w, s));
+    res.cols.push_back(std::move(col));
+  }
+  return MTL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Shape 2: single-table GROUP BY
+// ---------------------------------------------------------------------------
+static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
+  const CatTable& T = *b.tabs[0];
+  std::vector<mtl_pred> preds;
+  for (auto& pr : q.preds) {
+    int src;
+    const CatCol* c;
+    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    mtl_pred mp{};
+    mp.column = c->dev;
+    mp.type = c->type;
+    mp.op = pr.op;
+    std::string err;
+    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
+      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+    preds.push_back(mp);
+  }
+  mtl_group_desc d{};
+  d.n_rows = T.rows;
+  std::vector<const CatCol*> keys;
+  for (auto& g : q.group) {
+    int src;
+    const CatCol* c;
+    if (!b.resolve(g, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    if (!is_int_type(c->type)) return set_error(MTL_UNSUPPORTED, "GROUP BY a non-integer column");
+    keys.push_back(c);
+  }
+  if (keys.empty() || keys.size() > MTL_GROUP_MAX_KEYS)
+    return set_error(MTL_UNSUPPORTED, "GROUP BY needs 1..%d columns", MTL_GROUP_MAX_KEYS);
+  d.n_keys = (uint32_t)keys.size();
+  for (size_t i = 0; i < keys.size(); ++i) {
+    d.key_col[i] = keys[i]->dev;
+    d.key_type[i] = keys[i]->type;
+    d.key_min[i] = keys[i]->min;
+    d.key_max[i] = keys[i]->max;
+  }
+  // select items: group columns or aggregates over a column
+  struct Out {
+    std::string name;
+    int key = -1, agg = -1;
+  };
+  std::vector<Out> outs;
+  std::vector<int32_t> agg_func, agg_type;
+  std::vector<int32_t> agg_scale;
+  for (size_t i = 0; i < q.items.size(); ++i) {
+    const Item& it = q.items[i];
+    Out o;
+    o.name = item_name(it, i);
+    if (it.e->k == Expr::COL) {
+      int src;
+      const CatCol* c;
+      if (!b.resolve(it.e->col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+      for (size_t k = 0; k < keys.size(); ++k)
+        if (keys[k] == c) o.key = (int)k;
+      if (o.key < 0) return set_error(MTL_UNSUPPORTED, "%s is neither grouped nor aggregated",
+                                      c->name.c_str());
+    } else if (it.e->k == Expr::AGG) {
+      if (d.n_aggs >= MTL_GROUP_MAX_AGGS) return set_error(MTL_UNSUPPORTED, "too many aggregates");
+      mtl_agg_spec& a = d.aggs[d.n_aggs];
+      const std::string& f = it.e->fn;
+      a.func = f == "sum" ? MTL_AGG_SUM : f == "count" ? MTL_AGG_COUNT : f == "min" ? MTL_AGG_MIN
+               : f == "max" ? MTL_AGG_MAX : MTL_AGG_AVG;
+      int32_t scale = 0;
+      if (it.e->a) {
+        if (it.e->a->k != Expr::COL)
+          return set_error(MTL_UNSUPPORTED, "aggregate over an expression");
+        int src;
+        const CatCol* c;
+        if (!b.resolve(it.e->a->col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+        if (c->type == MTL_DICT8 || c->type == MTL_DICT32)
+          return set_error(MTL_UNSUPPORTED, "aggregate over a dictionary column");
+        a.values = c->dev;
+        a.type = c->type;
+        scale = c->scale;
+      } else if (a.func != MTL_AGG_COUNT) {
+        return set_error(MTL_UNSUPPORTED, "%s(*)", f.c_str());
+      }
+      agg_func.push_back(a.func);
+      agg_type.push_back(a.type);
+      agg_scale.push_back(scale);
+      o.agg = (int)d.n_aggs++;
+    } else {
+      return set_error(MTL_UNSUPPORTED, "select item %zu of a GROUP BY", i);
+    }
+    outs.push_back(o);
+  }
+  // ORDER BY one aggregate (by alias or the same call) [+ LIMIT]
+  int order_agg = -1;
+  bool odesc = true;
+  if (q.order.size() > 1) return set_error(MTL_UNSUPPORTED, "ORDER BY more than one term");
+  if (q.order.size() == 1) {
+    const ExprP& e = q.order[0].e;
+    for (size_t i = 0; i < q.items.size(); ++i) {
+      const Item& it = q.items[i];
+      if (outs[i].agg < 0) continue;
+      const bool by_alias = e->k == Expr::COL && e->col.rel.empty() && e->col.name == outs[i].name;
+      const bool same_call = e->k == Expr::AGG && e->fn == it.e->fn &&
+                             ((!e->a && !it.e->a) ||
+                              (e->a && it.e->a && e->a->k == Expr::COL &&
+                               it.e->a->k == Expr::COL && e->a->col.name == it.e->a->col.name));
+      if (by_alias || same_call) order_agg = outs[i].agg;
+    }
+    if (order_agg < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY must name an aggregate");
+    odesc = q.order[0].desc;
+    if (q.limit < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY an aggregate needs LIMIT");
+  }
+  res.shape = "group";
+  if (dry) return MTL_OK;
+  DevBuf sel, cnt;
+  if (!preds.empty()) {
+    MTL_TRY(sel.alloc(std::max<uint64_t>(T.rows, 1) * 4));
+    MTL_TRY(cnt.alloc(16));
+    MTL_TRY(mtl_filter(preds.data(), (uint32_t)preds.size(), T.rows, nullptr, nullptr, 0,
+                       (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
+    d.sel = (const uint32_t*)sel.p;
+    d.sel_count = (const uint32_t*)cnt.p;
+    d.sel_capacity = T.rows;
+  }
+  mtl_groups* g = nullptr;
+  MTL_TRY(mtl_group_agg(&d, &g, s));
+  struct Release {
+    mtl_groups* g;
+    ~Release() { mtl_groups_release(g); }
+  } rel{g};
+  uint64_t ng = 0;
+  std::vector<int64_t> kbuf;
+  std::vector<std::vector<uint8_t>> abuf(d.n_aggs);
+  std::vector<void*> aptr(d.n_aggs);
+  auto agg_width = [&](size_t a) -> uint64_t {
+    const int32_t f = agg_func[a], t = agg_type[a];
+    if (f == MTL_AGG_SUM && is_int_type(t)) return 16;
+    return 8;
+  };
+  if (order_agg >= 0) {
+    const uint32_t k = (uint32_t)q.limit;
+    kbuf.resize((uint64_t)std::max<uint32_t>(k, 1) * d.n_keys);
+    for (uint32_t a = 0; a < d.n_aggs; ++a) {
+      abuf[a].resize((uint64_t)std::max<uint32_t>(k, 1) * agg_width(a));
+      aptr[a] = abuf[a].data();
+    }
+    uint32_t n = 0;
+    if (k > 0) MTL_TRY(mtl_groups_topk(g, (uint32_t)order_agg, odesc ? 1 : 0, k, kbuf.data(),
+                                       aptr.data(), &n));
+    ng = n;
+  } else {
+    MTL_TRY(mtl_groups_count(g, &ng));
+    kbuf.resize(std::max<uint64_t>(ng, 1) * d.n_keys);
+    for (uint32_t a = 0; a < d.n_aggs; ++a) {
+      abuf[a].resize(std::max<uint64_t>(ng, 1) * agg_width(a));
+      aptr[a] = abuf[a].data();
+    }
+    uint64_t got = 0;
+    MTL_TRY(mtl_groups_fetch(g, kbuf.data(), aptr.data(), ng, &got));
+    ng = got;
+    if (q.limit >= 0 && (uint64_t)q.limit < ng) ng = (uint64_t)q.limit;
+  }
+  res.rows = ng;
+  for (auto& o : outs) {
+    mtl_result::Column col;
+    col.name = o.name;
+    if (o.key >= 0) {
+      const CatCol* c = keys[o.key];
+      col.type = c->type;
+      col.scale = c->scale;
+      const int w = type_width(c->type);
+      col.data.resize(ng * w);
+      for (uint64_t r = 0; r < ng; ++r) {
+        const int64_t v = kbuf[r * d.n_keys + o.key];
+        std::memcpy(col.data.data() + r * w, &v, w);  // little-endian narrowing
+      }
+    } else {
+      const int32_t f = agg_func[o.agg], t = agg_type[o.agg];
+      const uint64_t w = agg_width(o.agg);
+      if (f == MTL_AGG_COUNT) col.type = MTL_BIGINT;
+      else if (f == MTL_AGG_SUM && is_int_type(t)) col.type = MTL_HUGEINT;
+      else if (f == MTL_AGG_AVG || t == MTL_FLOAT || t == MTL_DOUBLE) col.type = MTL_DOUBLE;
+      else col.type = MTL_BIGINT;
+      if ((f == MTL_AGG_SUM || f == MTL_AGG_MIN || f == MTL_AGG_MAX) && t == MTL_DECIMAL)
+        col.scale = agg_scale[o.agg];
+      if (f == MTL_AGG_AVG && t == MTL_DECIMAL) col.scale = 0;  // value already divided
+      col.data.assign(abuf[o.agg].begin(), abuf[o.agg].begin() + ng * w);
+    }
+    res.cols.push_back(std::move(col));
+  }
+  return MTL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Shape 3: star join -> SUM -> top-K
+// ---------------------------------------------------------------------------
+// Postfix expression over fact columns; DECIMAL operands of + and - must
+// share a scale (constants are rescaled), * adds scales.
+static bool to_postfix(const ExprP& e, Bound& b, int fact, mtl_expr& x, int& scale,
+                       std::vector<const CatCol*>& cols, std::string& err) {
+  if (x.n_ops >= MTL_EXPR_MAX_OPS) return err = "expression too long", false;
+  switch (e->k) {
+    case Expr::COL: {
+      int src;
+      const CatCol* c;
+      if (!b.resolve(e->col, src, c)) return err = b.err, false;
+      if (src != fact) return err = "SUM over a column of another table", false;
+      if (!is_int_type(c->type) || c->type == MTL_DICT8 || c->type == MTL_DICT32 ||
+          c->type == MTL_DATE || c->type == MTL_TIMESTAMP)
+        return err = "SUM over a non-numeric integer column", false;
+      int idx = -1;
+      for (size_t i = 0; i < cols.size(); ++i)
+        if (cols[i] == c) idx = (int)i;
+      if (idx < 0) {
+        if (cols.size() >= MTL_EXPR_MAX_COLS) return err = "too many columns in SUM", false;
+        idx = (int)cols.size();
+        cols.push_back(c);
+        x.col[idx] = c->dev;
+        x.col_type[idx] = c->type;
+        x.n_cols = (uint32_t)cols.size();
+      }
+      x.op[x.n_ops] = MTL_EXPR_COL;
+      x.arg[x.n_ops++] = idx;
+      scale = c->type == MTL_DECIMAL ? c->scale : 0;
+      return true;
+    }
+    case Expr::LIT:
+      if (e->lit.k != Lit::INT) return err = "non-integer constant in SUM", false;
+      x.op[x.n_ops] = MTL_EXPR_CONST;
+      x.arg[x.n_ops++] = e->lit.i;
+      scale = -1;  // a constant adopts its partner's scale
+      return true;
+    case Expr::ADD:
+    case Expr::SUB:
+    case Expr::MUL: {
+      const uint32_t start = x.n_ops;
+      int sa, sb;
+      if (!to_postfix(e->a, b, fact, x, sa, cols, err)) return false;
+      const uint32_t mid = x.n_ops;
+      if (!to_postfix(e->b, b, fact, x, sb, cols, err)) return false;
+      if (e->k == Expr::MUL) {
+        scale = (sa < 0 ? 0 : sa) + (sb < 0 ? 0 : sb);
+      } else {
+        // rescale a bare constant operand to the other side's scale
+        auto rescale = [&](uint32_t at, int to) {
+          if (x.op[at] != MTL_EXPR_CONST || x.n_ops == 0) return true;
+          int64_t v;
+          if (!pow10_scale(x.arg[at], to, v)) return false;
+          x.arg[at] = v;
+          return true;
+        };
+        if (sa < 0 && sb >= 0 && !rescale(start, sb)) return err = "constant overflow", false;
+        if (sb < 0 && sa >= 0 && !rescale(mid, sa)) return err = "constant overflow", false;
+        if (sa >= 0 && sb >= 0 && sa != sb)
+          return err = "adding DECIMAL values of different scales", false;
+        scale = sa >= 0 ? sa : sb;
+      }
+      if (x.n_ops >= MTL_EXPR_MAX_OPS) return err = "expression too long", false;
+      x.op[x.n_ops++] = e->k == Expr::ADD ? MTL_EXPR_ADD : e->k == Expr::SUB ? MTL_EXPR_SUB
+                                                                              : MTL_EXPR_MUL;
+      return true;
+    }
+    default:
+      return err = "unsupported SUM expression", false;
+  }
+}
+
+static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
+  const int nt = (int)b.tabs.size();
+  // join edges -> (child source, fk column) -> (parent source, unique key)
+  struct Edge {
+    int child, parent;
+    const CatCol *fk, *key;
+  };
+  std::vector<Edge> edges;
+  for (auto& j : q.joins) {
+    int sa, sb;
+    const CatCol *ca, *cb;
+    if (!b.resolve(j.l, sa, ca) || !b.resolve(j.r, sb, cb))
+      return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    if (sa == sb) return set_error(MTL_UNSUPPORTED, "join within one table");
+    if (cb->unique && !ca->unique) edges.push_back({sa, sb, ca, cb});
+    else if (ca->unique && !cb->unique) edges.push_back({sb, sa, cb, ca});
+    else if (ca->unique && cb->unique)
+      return set_error(MTL_UNSUPPORTED, "one-to-one join");
+    else return set_error(MTL_UNSUPPORTED, "join without a unique key side");
+  }
+  if ((int)edges.size() != nt - 1) return set_error(MTL_UNSUPPORTED, "joins do not chain the tables");
+  // chain fact -> parent [-> dimension]
+  int fact = -1;
+  for (int i = 0; i < nt; ++i) {
+    bool is_parent = false;
+    for (auto& e : edges) is_parent |= e.parent == i;
+    if (!is_parent) {
+      if (fact >= 0) return set_error(MTL_UNSUPPORTED, "more than one fact table");
+      fact = i;
+    }
+  }
+  if (fact < 0) return set_error(MTL_UNSUPPORTED, "cyclic joins");
+  const Edge* fe = nullptr;
+  const Edge* pe = nullptr;
+  for (auto& e : edges)
+    if (e.child == fact) fe = &e;
+  if (!fe) return set_error(MTL_UNSUPPORTED, "fact table not joined");
+  const int parent = fe->parent;
+  for (auto& e : edges)
+    if (&e != fe) {
+      if (e.child != parent) return set_error(MTL_UNSUPPORTED, "join graph is not a chain");
+      pe = &e;
+    }
+  const int dim = pe ? pe->parent : -1;
+  // group by: parent columns including its key
+  const CatCol* pkey = fe->key;
+  bool has_key = false;
+  std::vector<const CatCol*> gcols;
+  for (auto& g : q.group) {
+    int src;
+    const CatCol* c;
+    if (!b.resolve(g, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    if (src == fact && c == fe->fk) c = pkey;  // l_orderkey = o_orderkey
+    else if (src != parent) return set_error(MTL_UNSUPPORTED, "GROUP BY a non-parent column");
+    has_key |= c == pkey;
+    gcols.push_back(c);
+  }
+  if (!has_key) return set_error(MTL_UNSUPPORTED, "GROUP BY must include the parent key");
+  // select items: group columns and exactly one SUM
+  mtl_star_desc d{};
+  int sum_item = -1, sum_scale = 0;
+  struct Out {
+    std::string name;
+    const CatCol* c = nullptr;  // parent column; null = the sum
+  };
+  std::vector<Out> outs;
+  for (size_t i = 0; i < q.items.size(); ++i) {
+    const Item& it = q.items[i];
+    Out o;
+    o.name = item_name(it, i);
+    if (it.e->k == Expr::COL) {
+      int src;
+      const CatCol* c;
+      if (!b.resolve(it.e->col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+      if (src == fact && c == fe->fk) c = pkey;
+      bool grouped = false;
+      for (auto* g : gcols) grouped |= g == c;
+      if (!grouped) return set_error(MTL_UNSUPPORTED, "%s is not grouped", c->name.c_str());
+      o.c = c;
+    } else if (it.e->k == Expr::AGG && it.e->fn == "sum" && sum_item < 0) {
+      std::vector<const CatCol*> cols;
+      std::string err;
+      d.agg = mtl_expr{};
+      if (!to_postfix(it.e->a, b, fact, d.agg, sum_scale, cols, err))
+        return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+      if (sum_scale < 0) sum_scale = 0;
+      sum_item = (int)i;
+    } else {
+      return set_error(MTL_UNSUPPORTED, "select item %zu of the star query", i);
+    }
+    outs.push_back(o);
+  }
+  if (sum_item < 0) return set_error(MTL_UNSUPPORTED, "no SUM in the star query");
+  // ORDER BY sum DESC, then parent columns
+  if (q.order.empty() || q.limit < 0) return set_error(MTL_UNSUPPORTED, "star query needs ORDER BY ... LIMIT");
+  {
+    const ExprP& e = q.order[0].e;
+    const Item& si = q.items[sum_item];
+    const bool by_alias = e->k == Expr::COL && e->col.rel.empty() && e->col.name == outs[sum_item].name;
+    const bool same = e->k == Expr::AGG && e->fn == "sum";
+    if (!(by_alias || same) || !q.order[0].desc)
+      return set_error(MTL_UNSUPPORTED, "the star query orders by its SUM descending first");
+    (void)si;
+  }
+  for (size_t i = 1; i < q.order.size(); ++i) {
+    const ExprP& e = q.order[i].e;
+    if (e->k != Expr::COL) return set_error(MTL_UNSUPPORTED, "ORDER BY an expression");
+    int src;
+    const CatCol* c = nullptr;
+    for (auto& o : outs)
+      if (e->col.rel.empty() && o.c && o.name == e->col.name) c = o.c;
+    if (!c) {
+      if (!b.resolve(e->col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+      if (src == fact && c == fe->fk) c = pkey;
+      else if (src != parent) return set_error(MTL_UNSUPPORTED, "ORDER BY a non-parent column");
+    }
+    if (c == pkey && !q.order[i].desc) break;  // the operator's final tie-break
+    if (d.n_ties >= MTL_STAR_MAX_TIES) return set_error(MTL_UNSUPPORTED, "too many ORDER BY terms");
+    d.tie_col[d.n_ties] = c->dev;
+    d.tie_type[d.n_ties] = c->type;
+    d.tie_desc[d.n_ties] = q.order[i].desc ? 1 : 0;
+    ++d.n_ties;
+  }
+  if (q.limit > 0xFFFFFFFFll) return set_error(MTL_UNSUPPORTED, "LIMIT %lld", (long long)q.limit);
+  d.k = (uint32_t)q.limit;  // the operator rejects K it cannot serve
+  // filters by table
+  auto add_pred = [&](int src, mtl_pred* arr, uint32_t& n, const Pred& pr,
+                      const CatCol* c) -> int32_t {
+    if (n >= MTL_MAX_PREDS) return set_error(MTL_UNSUPPORTED, "too many predicates");
+    mtl_pred mp{};
+    mp.column = c->dev;
+    mp.type = c->type;
+    mp.op = pr.op;
+    std::string err;
+    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
+      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+    arr[n++] = mp;
+    (void)src;
+    return MTL_OK;
+  };
+  for (auto& pr : q.preds) {
+    int src;
+    const CatCol* c;
+    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    if (src == fact) MTL_TRY(add_pred(src, d.f_preds, d.f_n_preds, pr, c));
+    else if (src == parent) MTL_TRY(add_pred(src, d.p_preds, d.p_n_preds, pr, c));
+    else MTL_TRY(add_pred(src, d.d_preds, d.d_n_preds, pr, c));
+  }
+  const CatTable& F = *b.tabs[fact];
+  const CatTable& P = *b.tabs[parent];
+  d.f_rows = F.rows;
+  d.f_fk = fe->fk->dev;
+  d.f_fk_type = fe->fk->type;
+  d.p_rows = P.rows;
+  d.p_key = pkey->dev;
+  d.p_key_type = pkey->type;
+  d.p_key_min = pkey->min;
+  d.p_key_max = pkey->max;
+  if (dim >= 0) {
+    const CatTable& D = *b.tabs[dim];
+    d.d_rows = D.rows;
+    d.d_key = pe->key->dev;
+    d.d_key_type = pe->key->type;
+    d.d_key_min = pe->key->min;
+    d.d_key_max = pe->key->max;
+    d.p_fk = pe->fk->dev;
+    d.p_fk_type = pe->fk->type;
+  }
+  res.shape = dim >= 0 ? "star3" : "star2";
+  if (dry) return MTL_OK;
+  std::vector<mtl_star_row> rows(std::max<uint32_t>(d.k, 1));
+  uint32_t n = 0;
+  if (d.k) MTL_TRY(mtl_star_topk(&d, rows.data(), &n, s));
+  res.rows = n;
+  // gather the group columns of the K parent rows on the device
+  DevBuf prow, out;
+  MTL_TRY(prow.alloc((uint64_t)std::max<uint32_t>(n, 1) * 4));
+  MTL_TRY(out.alloc((uint64_t)std::max<uint32_t>(n, 1) * 8));
+  std::vector<uint32_t> ids(n);
+  for (uint32_t i = 0; i < n; ++i) ids[i] = rows[i].p_row;
+  if (n) MTL_TRY(mtl_memcpy_h2d(prow.p, ids.data(), (uint64_t)n * 4, s));
+  for (auto& o : outs) {
+    mtl_result::Column col;
+    col.name = o.name;
+    if (o.c) {
+      col.type = o.c->type;
+      col.scale = o.c->scale;
+      const uint32_t w = (uint32_t)type_width(o.c->type);
+      if (n) MTL_TRY(mtl_gather(o.c->dev, w, (const uint32_t*)prow.p, nullptr, n, out.p, s));
+      MTL_TRY(download(col.data, out.p, (uint64_t)n * w, s));
+    } else {
+      col.type = d.agg.is_double ? MTL_DOUBLE : MTL_DECIMAL;
+      col.scale = sum_scale;
+      col.data.resize((uint64_t)n * 8);
+      for (uint32_t i = 0; i < n; ++i) std::memcpy(col.data.data() + i * 8, &rows[i].sum_i64, 8);
+    }
+    res.cols.push_back(std::move(col));
+  }
+  return MTL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Driver
+// ---------------------------------------------------------------------------
+static int32_t plan_and_run(const char* sql, mtl_result& res, mtl_stream* s, bool dry) {
+  if (!sql) return set_error(MTL_ERR_INVALID_ARGUMENT, "sql is NULL");
+  std::vector<Tok> toks;
+  std::string err;
+  if (!lex(sql, toks, err)) return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+  Parser ps(toks);
+  Query q;
+  if (!ps.query(q)) return set_error(MTL_UNSUPPORTED, "%s", ps.err.c_str());
+  std::lock_guard<std::mutex> lk(g_cat_mu);
+  Bound b;
+  for (auto& f : q.from) {
+    auto it = catalog().find(f.first);
+    if (it == catalog().end())
+      return set_error(MTL_UNSUPPORTED, "table %s is not registered", f.first.c_str());
+    b.tabs.push_back(&it->second);
+    b.names.push_back(f.second.empty() ? f.first : f.second);
+  }
+  bool has_agg = false;
+  for (auto& it : q.items) has_agg |= it.e->k == Expr::AGG;
+  if (b.tabs.size() == 1 && q.joins.empty()) {
+    if (q.group.empty() && !has_agg) return run_scan(q, b, res, s, dry);
+    if (!q.group.empty()) return run_group(q, b, res, s, dry);
+    return set_error(MTL_UNSUPPORTED, "aggregates without GROUP BY");
+  }
+  if ((b.tabs.size() == 2 || b.tabs.size() == 3) && !q.group.empty() && has_agg)
+    return run_star(q, b, res, s, dry);
+  return set_error(MTL_UNSUPPORTED, "query shape not handled on the GPU path");
+}
+
+// ---------------------------------------------------------------------------
+// gpu_rewrite_query: print the query back with its predicates and scalar
+// projections written as the gpu_* hook calls.
+// ---------------------------------------------------------------------------
+static std::string lit_sql(const Lit& l) {
+  switch (l.k) {
+    case Lit::INT:
+    case Lit::FLT: return l.text;
+    case Lit::DATE: return "DATE '" + l.s + "'";
+    case Lit::TS: return "TIMESTAMP '" + l.s + "'";
+    default: {
+      std::string o = "'";
+      for (char c : l.s) o += c == '\'' ? std::string("''") : std::string(1, c);
+      return o + "'";
+    }
+  }
+}
+static std::string col_sql(const ColRef& c) { return c.rel.empty() ? c.name : c.rel + "." + c.name; }
+
+static std::string expr_sql(const ExprP& e) {
+  ColRef c;
+  int op;
+  Lit l;
+  if (as_projection(e, c, op, l) && (op == MTL_PROJ_ADD || op == MTL_PROJ_MUL))
+    return std::string(op == MTL_PROJ_ADD ? "gpu_add_scalar(" : "gpu_mul_scalar(") + col_sql(c) +
+           ", " + lit_sql(l) + ")";
+  switch (e->k) {
+    case Expr::COL: return col_sql(e->col);
+    case Expr::LIT: return lit_sql(e->lit);
+    case Expr::STAR: return "*";
+    case Expr::AGG: return e->fn + "(" + (e->a ? expr_sql(e->a) : std::string("*")) + ")";
+    case Expr::CALL: return e->fn + "(" + expr_sql(e->a) + ", " + expr_sql(e->b) + ")";
+    default: {
+      const char* o = e->k == Expr::ADD ? " + " : e->k == Expr::SUB ? " - " : " * ";
+      return "(" + expr_sql(e->a) + o + expr_sql(e->b) + ")";
+    }
+  }
+}
+
+static int32_t rewrite(const char* sql, std::string& out) {
+  if (!sql) return set_error(MTL_ERR_INVALID_ARGUMENT, "sql is NULL");
+  std::vector<Tok> toks;
+  std::string err;
+  if (!lex(sql, toks, err)) return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+  Parser ps(toks);
+  Query q;
+  if (!ps.query(q)) return set_error(MTL_UNSUPPORTED, "%s", ps.err.c_str());
+  static const char* hook[] = {"gpu_eq_scalar", "gpu_lt_scalar", "gpu_le_scalar",
+                               "gpu_gt_scalar", "gpu_ge_scalar"};
+  out = "SELECT ";
+  for (size_t i = 0; i < q.items.size(); ++i) {
+    if (i) out += ", ";
+    out += expr_sql(q.items[i].e);
+    if (!q.items[i].alias.empty()) out += " AS " + q.items[i].alias;
+  }
+  out += " FROM ";
+  for (size_t i = 0; i < q.from.size(); ++i) {
+    if (i) out += ", ";
+    out += q.from[i].first;
+    if (!q.from[i].second.empty()) out += " " + q.from[i].second;
+  }
+  std::vector<std::string> conj;
+  for (auto& j : q.joins) conj.push_back(col_sql(j.l) + " = " + col_sql(j.r));
+  for (auto& p : q.preds)
+    conj.push_back(std::string(hook[p.op]) + "(" + col_sql(p.col) + ", " + lit_sql(p.lit) + ")");
+  for (size_t i = 0; i < conj.size(); ++i) out += (i ? " AND " : " WHERE ") + conj[i];
+  for (size_t i = 0; i < q.group.size(); ++i) out += (i ? ", " : " GROUP BY ") + col_sql(q.group[i]);
+  for (size_t i = 0; i < q.order.size(); ++i)
+    out += (i ? ", " : " ORDER BY ") + expr_sql(q.order[i].e) + (q.order[i].desc ? " DESC" : "");
+  if (q.limit >= 0) out += " LIMIT " + std::to_string(q.limit);
+  return MTL_OK;
+}
+
+}  // namespace sql
+}  // namespace mtl
+
+extern "C" {
+
+int32_t mtl_catalog_add_table(const char* table, uint64_t n_rows) {
+  using namespace mtl::sql;
+  if (!table) return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "table is NULL");
+  std::lock_guard<std::mutex> lk(g_cat_mu);
+  CatTable t;
+  t.name = lower(table);
+  t.rows = n_rows;
+  catalog()[t.name] = t;
+  return MTL_OK;
+}
+
+int32_t mtl_catalog_add_column(const char* table, const char* column, int32_t type,
+                               int32_t scale, const void* dev, int64_t min_value,
+                               int64_t max_value, int32_t unique) {
+  using namespace mtl::sql;
+  if (!table || !column || !dev) return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!mtl::type_width(type) || type == MTL_HUGEINT)
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "column type %d", type);
+  std::lock_guard<std::mutex> lk(g_cat_mu);
+  auto it = catalog().find(lower(table));
+  if (it == catalog().end())
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "table %s is not registered", table);
+  CatCol c;
+  c.name = lower(column);
+  c.type = type;
+  c.scale = scale;
+  c.dev = dev;
+  c.min = min_value;
+  c.max = max_value;
+  c.unique = unique != 0;
+  for (auto& old : it->second.cols)
+    if (old.name == c.name) {
+      old = c;
+      return MTL_OK;
+    }
+  it->second.cols.push_back(c);
+  return MTL_OK;
+}
+
+int32_t mtl_catalog_set_dictionary(const char* table, const char* column,
+                                   const char* const* values, uint32_t n) {
+  using namespace mtl::sql;
+  if (!table || !column || (n && !values))
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(g_cat_mu);
+  auto it = catalog().find(lower(table));
+  if (it == catalog().end())
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "table %s is not registered", table);
+  for (auto& c : it->second.cols)
+    if (c.name == lower(column)) {
+      c.dict.assign(values, values + n);
+      return MTL_OK;
+    }
+  return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "column %s is not registered", column);
+}
+
+int32_t mtl_catalog_drop(const char* table) {
+  using namespace mtl::sql;
+  std::lock_guard<std::mutex> lk(g_cat_mu);
+  if (table) catalog().erase(lower(table));
+  else catalog().clear();
+  return MTL_OK;
+}
+
+int32_t mtl_sql_explain(const char* sql, char* buf, size_t len) {
+  mtl_result res;
+  const int32_t st = mtl::sql::plan_and_run(sql, res, nullptr, true);
+  if (buf && len) std::snprintf(buf, len, "%s", st == MTL_OK ? res.shape.c_str() : mtl::last_error());
+  return st;
+}
+
+int32_t mtl_sql_execute(const char* sql, mtl_result** out, mtl_stream* s) {
+  if (!out) return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  MTL_TRY(mtl::ensure_device());
+  auto* res = new mtl_result();
+  const int32_t st = mtl::sql::plan_and_run(sql, *res, s, false);
+  if (st != MTL_OK) {
+    delete res;
+    return st;
+  }
+  *out = res;
+  return MTL_OK;
+}
+
+int32_t mtl_result_shape(const mtl_result* r, uint32_t* n_cols, uint64_t* n_rows) {
+  if (!r) return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "result is NULL");
+  if (n_cols) *n_cols = (uint32_t)r->cols.size();
+  if (n_rows) *n_rows = r->rows;
+  return MTL_OK;
+}
+
+int32_t mtl_result_column(const mtl_result* r, uint32_t i, const char** name, int32_t* type,
+                          int32_t* scale, const void** data) {
+  if (!r || i >= r->cols.size())
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "no result column %u", i);
+  const auto& c = r->cols[i];
+  if (name) *name = c.name.c_str();
+  if (type) *type = c.type;
+  if (scale) *scale = c.scale;
+  if (data) *data = c.data.data();
+  return MTL_OK;
+}
+
+int32_t mtl_result_free(mtl_result* r) {
+  delete r;
+  return MTL_OK;
+}
+
+int32_t mtl_scalar_hook(int32_t hook, int32_t type, const void* values, uint64_t n,
+                        int64_t scalar_bits, void* out) {
+  using namespace mtl;
+  if (hook < MTL_HOOK_ADD || hook > MTL_HOOK_GE)
+    return set_error(MTL_ERR_INVALID_ARGUMENT, "hook %d", hook);
+  if ((!values || !out) && n) return set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  const int w = type_width(type);
+  if (!w || type == MTL_HUGEINT) return set_error(MTL_UNSUPPORTED, "hook on type %d", type);
+  if (n == 0) return MTL_OK;
+  MTL_TRY(ensure_device());
+  sql::DevBuf in, res, flag;
+  MTL_TRY(in.alloc(n * w));
+  MTL_TRY(mtl_memcpy_h2d(in.p, values, n * w, nullptr));
+  if (hook == MTL_HOOK_ADD || hook == MTL_HOOK_MUL) {
+    MTL_TRY(res.alloc(n * w));
+    MTL_TRY(flag.alloc(16));
+    uint32_t zero = 0;
+    MTL_TRY(mtl_memcpy_h2d(flag.p, &zero, 4, nullptr));
+    MTL_TRY(mtl_project(in.p, type, nullptr, nullptr, n,
+                        hook == MTL_HOOK_ADD ? MTL_PROJ_ADD : MTL_PROJ_MUL, scalar_bits, res.p,
+                        (uint32_t*)flag.p, nullptr));
+    uint32_t f = 0;
+    MTL_TRY(mtl_memcpy_d2h(&f, flag.p, 4, nullptr));
+    if (f)
+      return set_error(MTL_ERR_OVERFLOW, "Overflow in %s",
+                       hook == MTL_HOOK_ADD ? "addition" : "multiplication");
+    return mtl_memcpy_d2h(out, res.p, n * w, nullptr);
+  }
+  mtl_pred p{};
+  p.column = in.p;
+  p.type = type;
+  p.op = hook - MTL_HOOK_EQ;  // EQ LT LE GT GE in compare-op order
+  p.scalar = scalar_bits;
+  MTL_TRY(res.alloc(n));
+  MTL_TRY(mtl_compare(&p, 1, n, (uint8_t*)res.p, nullptr));
+  return mtl_memcpy_d2h(out, res.p, n, nullptr);
+}
+
+int32_t mtl_rewrite_query(const char* sql, char* buf, size_t len, size_t* needed) {
+  std::string out;
+  const int32_t st = mtl::sql::rewrite(sql, out);
+  if (st != MTL_OK) return st;
+  if (needed) *needed = out.size() + 1;
+  if (!buf || len <= out.size())
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "buffer of %zu bytes too small", len);
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return MTL_OK;
+}
+
+}  // extern "C"
