@@ -165,6 +165,108 @@ def serving_benchmark(n_queries: int, slots: int, seed: int = 20261208):
             "speedup": batched["qps"] / serial["qps"], "identical_results": bool(same)}
 
 
+def timed(fn, steps: int, warmup: int):
+    """Wall time per call of fn, which runs synchronously (it returns after
+    its device work is done), in ms: median over steps."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        fn()
+        ts.append((time.perf_counter() - t) * 1e3)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def scan_workload(args) -> dict:
+    """100M-row scan-filter-project: predicates at 1/10/50% selectivity on
+    INTEGER and BIGINT columns with a projection, the whole operator chain
+    (filter + compaction + projection) per call; algorithmic bytes =
+    N x predicate width + S x (row id 4 + output width)."""
+    import numpy as np
+    import torch
+    from mettle_amd import abi, datagen, ops
+    n = 100_000_000
+    log("scan: generating 100M rows")
+    t = datagen.scan_table(n)
+    dev = {c: ops.to_device(t[c]) for c in ("c_i", "c_b")}
+    del t
+    sel = torch.empty(n, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    out = torch.empty(n, dtype=torch.int64, device="cuda")
+    ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
+    L = abi.lib()
+    rows = []
+    for cname, ctype, width in (("c_i", abi.INTEGER, 4), ("c_b", abi.BIGINT, 8)):
+        lo = -(1 << 31) if width == 4 else -(1 << 63)
+        span = 1 << 32 if width == 4 else 1 << 64
+        for frac in (0.01, 0.10, 0.50):
+            lit = lo + int(frac * span)
+            preds = (abi.Pred * 1)(abi.make_pred(dev[cname].data_ptr(), ctype, abi.CMP_LT, lit))
+
+            def call():
+                abi.check(L.mtl_filter(preds, 1, n, None, None, 0, sel.data_ptr(),
+                                       cnt.data_ptr(), None))
+                abi.check(L.mtl_project(dev[cname].data_ptr(), ctype, sel.data_ptr(),
+                                        cnt.data_ptr(), n, abi.PROJ_ADD, 1, out.data_ptr(),
+                                        ovf.data_ptr(), None))
+                abi.check(L.mtl_stream_sync(None))
+            ms = timed(call, args.steps, args.warmup)
+            s_rows = int(cnt.item())
+            alg = n * width + s_rows * (4 + width)
+            rows.append({"column": cname, "selectivity": frac, "selected": s_rows, "ms": ms,
+                         "rows_per_s": n / (ms * 1e-3), "algorithmic_bytes": alg,
+                         "GBs": alg / (ms * 1e-3) / 1e9,
+                         "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+            log(f"scan {cname} {frac:.0%}: {ms:.3f} ms, {rows[-1]['GBs']:.0f} GB/s")
+    return {"metric": "rows/s & % HBM roofline (scan-filter-project)",
+            "value": min(r["rows_per_s"] for r in rows), "unit": "rows/s",
+            "config": {"workload": "scan-filter-project 100M rows (config 2), "
+                                   "col < literal + projection col + 1", "rows": n},
+            "variants": rows, "dtype": "int32/int64", "data": "synthetic"}
+
+
+def group_workload(args, dense: bool) -> dict:
+    """Skewed grouped aggregate: 200M rows, g BIGINT Zipf(1.1) over 10M keys
+    (or 1000 uniform keys), SUM/COUNT/MIN/MAX(v_i), SUM/AVG(v_f), then
+    top-100 by SUM(v_i); algorithmic bytes = 200M x (8 + 8 + 4)."""
+    import ctypes as C
+    import torch
+    from mettle_amd import abi, datagen, ops
+    n = 200_000_000
+    log("group: generating 200M rows")
+    t = datagen.group_table(n, dense_keys=1000 if dense else None)
+    gmax = int(t["g"].max())
+    dg, dvi, dvf = (ops.to_device(t[c]) for c in ("g", "v_i", "v_f"))
+    del t
+    L = abi.lib()
+    aggs = [(abi.AGG_SUM, dvi, abi.BIGINT), (abi.AGG_COUNT, None, 0),
+            (abi.AGG_MIN, dvi, abi.BIGINT), (abi.AGG_MAX, dvi, abi.BIGINT),
+            (abi.AGG_SUM, dvf, abi.FLOAT), (abi.AGG_AVG, dvf, abi.FLOAT)]
+    keys = [(dg, abi.BIGINT, 0, gmax)]
+    res = {}
+
+    def call():
+        g = ops.group_agg(keys, aggs, n)
+        res["topk"] = g.topk(0, 100, desc=True)
+    ms = timed(call, args.steps, args.warmup)
+    alg = n * 20
+    gbs = alg / (ms * 1e-3) / 1e9
+    log(f"group ({'1000 keys' if dense else 'Zipf 10M keys'}): {ms:.3f} ms, {gbs:.0f} GB/s")
+    return {"metric": "rows/s & % HBM roofline (grouped aggregate + top-100)",
+            "value": n / (ms * 1e-3), "unit": "rows/s", "ms_per_step": ms,
+            "config": {"workload": "grouped aggregate 200M rows (config 4), "
+                                   + ("g uniform over 1000 keys" if dense else
+                                      "g Zipf(1.1) over 10M keys"), "rows": n},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg},
+            "dtype": "int64/float32 (exact 128-bit integer sums, float64 float sums)",
+            "data": "synthetic"}
+
+
 def cpu_baseline(scale: float, seconds: float):
     """Oracle (C restatement, 1 thread) on a bounded sample of the workload;
     also checks the GPU result on the same sample."""
@@ -206,11 +308,26 @@ def main() -> int:
     ap.add_argument("--serve-queries", type=int, default=10000)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--workload", choices=["star", "scan", "group", "group-dense"],
+                    default="star", help="star: the headline line; the others measure "
+                    "configs 2 and 4 (parity configs, reported separately)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
+    if args.workload != "star":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU"}))
+            return 2
+        if args.workload == "scan":
+            r = scan_workload(args)
+        else:
+            r = group_workload(args, args.workload == "group-dense")
+        r.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                  "higher_is_better": True, "scaling": "weak", "vs_baseline": None})
+        print(json.dumps(r), flush=True)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(sharding is described in DESIGN.md, not run)"}))

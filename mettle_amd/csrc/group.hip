@@ -17,9 +17,12 @@
 //     (Zipf heads collapse there) and flushes it to the global table once;
 //   - one pass updates all aggregates of a row (state words of a group are
 //     contiguous, one cache line for typical aggregate lists);
-//   - integer SUM is exact 128-bit: the low word is added with a returning
-//     64-bit atomic and its carry goes into the high word, which gives the
-//     exact total in any order; FLOAT/DOUBLE sums accumulate in double.
+//   - integer SUM is exact 128-bit with non-returning atomics only: the
+//     low 32 bits of each value are summed in one 64-bit word and the high
+//     32 bits (signed) in another; sum = high * 2^32 + low, exact for fewer
+//     than 2^32 rows in any order; FLOAT/DOUBLE sums accumulate in double;
+//   - each thread loads the keys and values of several rows before its
+//     first update, so the loads overlap instead of waiting behind atomics.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -44,6 +47,7 @@ struct AggSlot {
   uint32_t kind;
   uint32_t word;
   int32_t type;
+  uint32_t dup;  // shares the state words of an earlier slot (same kind and column)
   const void* col;
 };
 
@@ -68,6 +72,7 @@ struct GroupArgs {
   uint64_t* st;     // (cap + 1) * W state words
   uint64_t init[kMaxW];
   uint32_t cache_slots;
+  uint32_t cache_direct;  // dense domain that fits the cache: slot = key
   uint32_t* err;    // 64: table full, 8: key outside its stated range
 };
 
@@ -103,42 +108,51 @@ __device__ __forceinline__ double gload_f64(const void* col, int32_t type, uint6
   }
 }
 
-__device__ __forceinline__ void add128(uint64_t* p, uint64_t lo, int64_t hi) {
-  unsigned long long old = atomicAdd((unsigned long long*)p, (unsigned long long)lo);
-  int64_t h = hi + (int64_t)(((uint64_t)old + lo) < (uint64_t)old);
-  if (h) atomicAdd((unsigned long long*)(p + 1), (unsigned long long)h);
+// The exact 128-bit integer sum held as (sum of low 32-bit halves, sum of
+// high signed halves): value = hi32sum * 2^32 + lo32sum.
+__device__ __forceinline__ void sum128(const uint64_t* w, uint64_t& lo, int64_t& hi) {
+  const uint64_t L = w[0];
+  const int64_t H = (int64_t)w[1];
+  lo = ((uint64_t)H << 32) + L;
+  hi = (H >> 32) + (lo < L ? 1 : 0);
+}
+
+// The value a slot accumulates for one row: int64 bits for integer kinds,
+// double bits for K_SUMF, the order-preserving encoding for K_MINF/K_MAXF.
+__device__ __forceinline__ uint64_t agg_raw(const AggSlot& s, uint64_t row) {
+  switch (s.kind) {
+    case K_SUMI:
+    case K_MINI:
+    case K_MAXI: return (uint64_t)load_key(s.col, s.type, row);
+    case K_SUMF: return (uint64_t)__double_as_longlong(gload_f64(s.col, s.type, row));
+    case K_MINF:
+    case K_MAXF: return enc_f64(gload_f64(s.col, s.type, row));
+    default: return 0;
+  }
 }
 
 // Per-row update of one group's state words (global or LDS: atomics with
-// the address space of p).
-__device__ __forceinline__ void update_row(const GroupArgs& a, uint64_t* p, uint64_t row) {
+// the address space of p; none returns a value).
+__device__ __forceinline__ void update_vals(const GroupArgs& a, uint64_t* p,
+                                            const uint64_t (&v)[MTL_GROUP_MAX_AGGS]) {
   atomicAdd((unsigned long long*)&p[0], 1ull);
-  for (uint32_t i = 0; i < a.n_aggs; ++i) {
+#pragma unroll
+  for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i) {
+    if (i >= a.n_aggs) break;
     const AggSlot& s = a.aggs[i];
+    if (s.dup) continue;
     uint64_t* w = p + s.word;
     switch (s.kind) {
-      case K_SUMI: {
-        int64_t v = load_key(s.col, s.type, row);
-        add128(w, (uint64_t)v, v < 0 ? -1 : 0);
+      case K_SUMI:
+        atomicAdd((unsigned long long*)w, (unsigned long long)(uint32_t)v[i]);
+        atomicAdd((unsigned long long*)(w + 1), (unsigned long long)((int64_t)v[i] >> 32));
         break;
-      }
-      case K_SUMF:
-        atomicAdd((double*)w, gload_f64(s.col, s.type, row));
-        break;
-      case K_MINI:
-        atomicMin((long long*)w, (long long)load_key(s.col, s.type, row));
-        break;
-      case K_MAXI:
-        atomicMax((long long*)w, (long long)load_key(s.col, s.type, row));
-        break;
-      case K_MINF:
-        atomicMin((unsigned long long*)w, enc_f64(gload_f64(s.col, s.type, row)));
-        break;
-      case K_MAXF:
-        atomicMax((unsigned long long*)w, enc_f64(gload_f64(s.col, s.type, row)));
-        break;
-      default:
-        break;
+      case K_SUMF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)v[i])); break;
+      case K_MINI: atomicMin((long long*)w, (long long)v[i]); break;
+      case K_MAXI: atomicMax((long long*)w, (long long)v[i]); break;
+      case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)v[i]); break;
+      case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)v[i]); break;
+      default: break;
     }
   }
 }
@@ -148,11 +162,15 @@ __device__ __forceinline__ void merge_state(const GroupArgs& a, uint64_t* g, con
   atomicAdd((unsigned long long*)&g[0], (unsigned long long)p[0]);
   for (uint32_t i = 0; i < a.n_aggs; ++i) {
     const AggSlot& s = a.aggs[i];
+    if (s.dup) continue;
     uint64_t* w = g + s.word;
     const uint64_t* v = p + s.word;
     switch (s.kind) {
-      case K_SUMI: add128(w, v[0], (int64_t)v[1]); break;
-      case K_SUMF: atomicAdd((double*)w, __longlong_as_double((long long)v[0])); break;
+      case K_SUMI:
+        atomicAdd((unsigned long long*)w, (unsigned long long)v[0]);
+        atomicAdd((unsigned long long*)(w + 1), (unsigned long long)v[1]);
+        break;
+      case K_SUMF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)v[0])); break;
       case K_MINI: atomicMin((long long*)w, (long long)v[0]); break;
       case K_MAXI: atomicMax((long long*)w, (long long)v[0]); break;
       case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)v[0]); break;
@@ -188,13 +206,16 @@ __global__ void k_group_init(GroupArgs a, uint64_t n_groups) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_group_agg(GroupArgs a) {
+constexpr int kAggBlock = 512;
+constexpr int GE = 4;  // rows per thread per round, loaded together
+
+__global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
   extern __shared__ uint64_t smem[];
   const uint32_t cs = a.cache_slots;
   uint64_t* ckeys = smem;
   uint64_t* cst = smem + cs;
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
-    ckeys[i] = GEMPTY;
+    ckeys[i] = a.cache_direct ? (uint64_t)i : GEMPTY;
     for (uint32_t w = 0; w < a.W; ++w) cst[i * a.W + w] = a.init[w];
   }
   __syncthreads();
@@ -202,40 +223,60 @@ __global__ __launch_bounds__(256) void k_group_agg(GroupArgs a) {
   const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t start = (uint64_t)blockIdx.x * per;
   const uint64_t end = start + per < n ? start + per : n;
-  for (uint64_t k = start + threadIdx.x; k < end; k += blockDim.x) {
-    uint64_t row = a.sel ? a.sel[k] : k;
-    uint64_t key;
-    if (!pack_key(a, row, key)) {
-      atomicOr(a.err, 8u);
-      continue;
+  for (uint64_t base = start; base < end; base += (uint64_t)blockDim.x * GE) {
+    uint64_t row[GE], key[GE];
+    bool ok[GE];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      const uint64_t k = base + (uint64_t)e * blockDim.x + threadIdx.x;
+      ok[e] = k < end;
+      row[e] = ok[e] ? (a.sel ? (uint64_t)a.sel[k] : k) : 0;
     }
-    int slot = -1;
-    if (cs && key != GEMPTY) {
-      uint32_t h = (uint32_t)gfmix(key) & (cs - 1);
+    uint64_t v[GE][MTL_GROUP_MAX_AGGS];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      key[e] = 0;
+      if (ok[e] && !pack_key(a, row[e], key[e])) {
+        atomicOr(a.err, 8u);
+        ok[e] = false;
+      }
+#pragma unroll
+      for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i)
+        v[e][i] = (ok[e] && i < a.n_aggs && !a.aggs[i].dup) ? agg_raw(a.aggs[i], row[e]) : 0ull;
+    }
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      if (!ok[e]) continue;
+      int slot = -1;
+      if (a.cache_direct) {
+        slot = (int)key[e];  // the whole (dense) domain is cached
+      } else if (cs && key[e] != GEMPTY) {
+        uint32_t h = (uint32_t)gfmix(key[e]) & (cs - 1);
 #pragma unroll 1
-      for (int p = 0; p < 8; ++p) {
-        uint32_t si = (h + p) & (cs - 1);
-        uint64_t ck = __hip_atomic_load(&ckeys[si], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (ck == key) {
-          slot = (int)si;
-          break;
-        }
-        if (ck == GEMPTY) {
-          unsigned long long old =
-              atomicCAS((unsigned long long*)&ckeys[si], GEMPTY, (unsigned long long)key);
-          if (old == GEMPTY || old == key) {
+        for (int p = 0; p < 8; ++p) {
+          uint32_t si = (h + p) & (cs - 1);
+          uint64_t ck = __hip_atomic_load(&ckeys[si], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (ck == key[e]) {
             slot = (int)si;
             break;
           }
+          if (ck == GEMPTY) {
+            unsigned long long old = atomicCAS((unsigned long long*)&ckeys[si], GEMPTY,
+                                               (unsigned long long)key[e]);
+            if (old == GEMPTY || old == key[e]) {
+              slot = (int)si;
+              break;
+            }
+          }
         }
       }
-    }
-    if (slot >= 0) {
-      update_row(a, cst + (uint64_t)slot * a.W, row);
-    } else {
-      uint64_t g = global_slot(a, key);
-      if (g != GEMPTY) update_row(a, a.st + g * a.W, row);
+      if (slot >= 0) {
+        update_vals(a, cst + (uint64_t)slot * a.W, v[e]);
+      } else {
+        uint64_t g = global_slot(a, key[e]);
+        if (g != GEMPTY) update_vals(a, a.st + g * a.W, v[e]);
+      }
     }
   }
   __syncthreads();
@@ -307,8 +348,16 @@ __global__ void k_group_emit(EmitArgs a) {
     for (uint32_t k = 0; k < a.n_aggs; ++k) {
       uint64_t* o = a.out_aggs + ((uint64_t)i * a.n_aggs + k) * 2;
       const AggSlot& s = a.aggs[k];
-      o[0] = s.kind == K_COUNT ? p[0] : p[s.word];
-      o[1] = s.kind == K_SUMI ? p[s.word + 1] : 0;
+      if (s.kind == K_SUMI) {
+        uint64_t lo;
+        int64_t hi;
+        sum128(p + s.word, lo, hi);
+        o[0] = lo;
+        o[1] = (uint64_t)hi;
+      } else {
+        o[0] = s.kind == K_COUNT ? p[0] : p[s.word];
+        o[1] = 0;
+      }
     }
   }
 }
@@ -336,16 +385,19 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint6
   uint64_t k0, k1 = 0;
   switch (a.kind) {
     case K_COUNT: k0 = enc_i64((int64_t)p[0]); break;
-    case K_SUMI:
+    case K_SUMI: {
+      uint64_t lo;
+      int64_t hi;
+      sum128(p + a.word, lo, hi);
       if (a.avg) {
-        double v = ((double)(int64_t)p[a.word + 1] * 18446744073709551616.0 +
-                    (double)p[a.word]) / (double)p[0];
+        double v = ((double)hi * 18446744073709551616.0 + (double)lo) / (double)p[0];
         k0 = enc_f64(v);
       } else {
-        k0 = enc_i64((int64_t)p[a.word + 1]);
-        k1 = p[a.word];
+        k0 = enc_i64(hi);
+        k1 = lo;
       }
       break;
+    }
     case K_SUMF: {
       double v = __longlong_as_double((long long)p[a.word]);
       k0 = enc_f64(a.avg ? v / (double)p[0] : v);
@@ -503,6 +555,17 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
       default:
         return set_error(MTL_UNSUPPORTED, "aggregate function %d", sp.func);
     }
+    // SUM(x) and AVG(x), or a repeated aggregate, share one state
+    s.dup = 0;
+    for (uint32_t j = 0; j < i; ++j) {
+      const AggSlot& o = a->aggs[j];
+      if (!o.dup && o.kind == s.kind && s.kind != K_COUNT && o.col == s.col && o.type == s.type) {
+        w = s.word;  // give back the words just taken
+        s.word = o.word;
+        s.dup = 1;
+        break;
+      }
+    }
   }
   a->n_aggs = d->n_aggs;
   a->W = w;
@@ -517,15 +580,16 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
     while (a->cap < want * 2) a->cap <<= 1;
     *entries = a->cap + 1;
   }
-  // LDS cache: <= 48 KB
+  // LDS cache of hot groups: <= 60 KB (three 512-thread blocks per CU)
+  const uint64_t kCacheBytes = 60 * 1024;
   uint32_t slot_bytes = 8 * (1 + a->W);
   uint32_t cs = 1;
-  while ((uint64_t)cs * 2 * slot_bytes <= 48 * 1024) cs *= 2;
-  if (a->dense && (uint64_t)cs >= D) {
-    // tiny domains: the whole domain fits; use a cache at least as large
-    uint32_t c2 = 1;
-    while (c2 < D * 2) c2 *= 2;
-    cs = std::min<uint32_t>(cs, c2);
+  while ((uint64_t)cs * 2 * slot_bytes <= kCacheBytes) cs *= 2;  // load factor <= 1/2
+  a->cache_direct = 0;
+  if (a->dense && D * slot_bytes <= kCacheBytes) {
+    // the whole domain fits: index the cache by key, no probing
+    cs = (uint32_t)D;
+    a->cache_direct = 1;
   }
   a->cache_slots = cs;
   return MTL_OK;
@@ -578,10 +642,12 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   uint64_t rows = desc->sel ? desc->sel_capacity : desc->n_rows;
   if (rows) {
     size_t lds = (size_t)a.cache_slots * 8 * (1 + a.W);
-    uint64_t grid = std::min<uint64_t>(ceil_div(rows, 4096), (uint64_t)cu_count() * 2);
+    // three blocks per CU fit the LDS caches (<= 48 KB each)
+    uint64_t grid = std::min<uint64_t>(ceil_div(rows, (uint64_t)kAggBlock * GE * 4),
+                                       (uint64_t)cu_count() * 3);
     ProfScope ps(s, "k_group_agg");
-    hipLaunchKernelGGL(k_group_agg, dim3((uint32_t)std::max<uint64_t>(grid, 1)), dim3(256),
-                       lds, s->stream, a);
+    hipLaunchKernelGGL(k_group_agg, dim3((uint32_t)std::max<uint64_t>(grid, 1)),
+                       dim3(kAggBlock), lds, s->stream, a);
   }
   MTL_HIP(hipGetLastError());
   g->args = a;

@@ -171,3 +171,21 @@ def star_kernel_bytes(n_c: int, n_o: int, n_l: int, counts: dict) -> dict:
         "k_star_topk": pp * 12,
         "k_star_topk_merge": 0,
     }
+
+
+def group_table(n: int = 200_000_000, n_keys: int = 10_000_000, s: float = 1.1,
+                dense_keys: int | None = None, workload: int = 4):
+    """Skewed grouped-aggregate table: g BIGINT Zipf(s) over 1..n_keys by
+    inverse-CDF sampling of the truncated distribution (or uniform over
+    0..dense_keys-1), v_i BIGINT uniform in [-1e9, 1e9], v_f FLOAT N(0,1)."""
+    rng = rng_for(workload)
+    if dense_keys:
+        g = rng.integers(0, dense_keys, n, dtype=np.int64)
+    else:
+        w = 1.0 / np.arange(1, n_keys + 1, dtype=np.float64) ** s
+        cdf = np.cumsum(w)
+        cdf /= cdf[-1]
+        g = np.searchsorted(cdf, rng.random(n)).astype(np.int64) + 1
+    return {"g": g,
+            "v_i": rng.integers(-10**9, 10**9 + 1, n, dtype=np.int64),
+            "v_f": rng.standard_normal(n, dtype=np.float32)}
