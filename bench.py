@@ -184,8 +184,9 @@ def timed(fn, steps: int, warmup: int):
 def scan_workload(args) -> dict:
     """100M-row scan-filter-project: predicates at 1/10/50% selectivity on
     INTEGER and BIGINT columns with a projection, the whole operator chain
-    (filter + compaction + projection) per call; algorithmic bytes =
-    N x predicate width + S x (row id 4 + output width)."""
+    (filter + compaction + projection, fused in mtl_filter_project) per
+    call, with mtl_filter then mtl_project timed beside it; algorithmic
+    bytes = N x predicate width + S x (row id 4 + output width)."""
     import numpy as np
     import torch
     from mettle_amd import abi, datagen, ops
@@ -207,21 +208,31 @@ def scan_workload(args) -> dict:
             lit = lo + int(frac * span)
             preds = (abi.Pred * 1)(abi.make_pred(dev[cname].data_ptr(), ctype, abi.CMP_LT, lit))
 
-            def call():
+            def fused():
+                abi.check(L.mtl_filter_project(preds, 1, n, None, None, 0,
+                                               dev[cname].data_ptr(), ctype, abi.PROJ_ADD, 1,
+                                               sel.data_ptr(), cnt.data_ptr(), out.data_ptr(),
+                                               ovf.data_ptr(), None))
+                abi.check(L.mtl_stream_sync(None))
+
+            def separate():
                 abi.check(L.mtl_filter(preds, 1, n, None, None, 0, sel.data_ptr(),
                                        cnt.data_ptr(), None))
                 abi.check(L.mtl_project(dev[cname].data_ptr(), ctype, sel.data_ptr(),
                                         cnt.data_ptr(), n, abi.PROJ_ADD, 1, out.data_ptr(),
                                         ovf.data_ptr(), None))
                 abi.check(L.mtl_stream_sync(None))
-            ms = timed(call, args.steps, args.warmup)
+            ms_sep = timed(separate, args.steps, args.warmup)
+            ms = timed(fused, args.steps, args.warmup)
             s_rows = int(cnt.item())
             alg = n * width + s_rows * (4 + width)
             rows.append({"column": cname, "selectivity": frac, "selected": s_rows, "ms": ms,
+                         "ms_filter_then_project": ms_sep,
                          "rows_per_s": n / (ms * 1e-3), "algorithmic_bytes": alg,
                          "GBs": alg / (ms * 1e-3) / 1e9,
                          "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
-            log(f"scan {cname} {frac:.0%}: {ms:.3f} ms, {rows[-1]['GBs']:.0f} GB/s")
+            log(f"scan {cname} {frac:.0%}: fused {ms:.3f} ms ({rows[-1]['GBs']:.0f} GB/s), "
+                f"filter then project {ms_sep:.3f} ms")
     return {"metric": "rows/s & % HBM roofline (scan-filter-project)",
             "value": min(r["rows_per_s"] for r in rows), "unit": "rows/s",
             "config": {"workload": "scan-filter-project 100M rows (config 2), "

@@ -199,6 +199,20 @@ int32_t mtl_filter(const mtl_pred *preds, uint32_t n_preds, uint64_t n_rows,
                    uint64_t in_capacity, uint32_t *out_sel,
                    uint32_t *out_count, mtl_stream *s);
 
+/* mtl_filter fused with one mtl_project of the passing rows: out_sel and
+ * *out_count as mtl_filter; out_proj[i] = proj_column[out_sel[i]] op scalar
+ * for i < *out_count, with mtl_project's types, ops and overflow flag.  The
+ * projection is computed while the selection is written, so the selection
+ * is not read back.  Replaces the scan path's filter_to_indices_gv followed
+ * by project_f32_scalar_gv (gather_ops.rs:23-159, scalar_ops.rs:512-561;
+ * kernels K23/K24 then K35). */
+int32_t mtl_filter_project(const mtl_pred *preds, uint32_t n_preds, uint64_t n_rows,
+                           const uint32_t *in_sel, const uint32_t *in_count,
+                           uint64_t in_capacity, const void *proj_column,
+                           int32_t proj_type, int32_t op, int64_t scalar,
+                           uint32_t *out_sel, uint32_t *out_count, void *out_proj,
+                           uint32_t *overflow_flag, mtl_stream *s);
+
 /* Per-row mask (uint8 0/1) of a predicate conjunction over rows 0..n-1.
  * Replaces compare_*_values (metal/mod.rs:146-198; scalar_ops.rs:85-279). */
 int32_t mtl_compare(const mtl_pred *preds, uint32_t n_preds, uint64_t n_rows,

@@ -218,6 +218,8 @@ def lib() -> C.CDLL:
         "mtl_compare": (i32, [C.POINTER(Pred), u32, u64, vp, vp]),
         "mtl_gather": (i32, [vp, u32, vp, vp, u64, vp, vp]),
         "mtl_project": (i32, [vp, i32, vp, vp, u64, i32, i64, vp, vp, vp]),
+        "mtl_filter_project": (i32, [C.POINTER(Pred), u32, u64, vp, vp, u64, vp, i32, i32,
+                                     i64, vp, vp, vp, vp, vp]),
         "mtl_cache_put": (i32, [u64, u32, vp, u64, u64, C.POINTER(vp)]),
         "mtl_cache_get": (i32, [u64, u32, C.POINTER(vp), C.POINTER(u64)]),
         "mtl_cache_release": (i32, [u64, u32]),

@@ -41,6 +41,16 @@ __device__ __forceinline__ T ld_off32(const T* base, uint32_t byte_off) {
   return *(const T*)((const char*)base + byte_off);
 }
 
+// Row count of a selection whose count lives on the device.  A count above
+// the capacity the caller sized the buffers for (a failed scan reports
+// 0xFFFFFFFF) is read as 0, so it never turns into reads of unwritten
+// selection entries; the host reports the failure when it reads the count.
+__device__ __forceinline__ uint64_t dev_count(const uint32_t* count, uint64_t cap) {
+  if (!count) return cap;
+  const uint64_t c = *count;
+  return c <= cap ? c : 0;
+}
+
 // Block-wide maxima (blockDim.x <= 1024, a multiple of 64); every thread
 // of the block must call them and every thread gets the result.
 __device__ __forceinline__ uint64_t block_max_u64(uint64_t v) {
@@ -75,10 +85,11 @@ __device__ __forceinline__ uint32_t wave_inclusive_u32(uint32_t v) {
 // ---------------------------------------------------------------------------
 // Decoupled look-back (single-pass prefix over tiles).  Status word per tile:
 // low 32 bits = value, bits 32..33 = flag (0 not ready, 1 aggregate,
-// 2 inclusive prefix).  Tiles are numbered in launch order through an atomic
-// ticket, so a tile only ever waits on tiles that are already running, and
-// every wait ends.  A spin cap sets *err and stops waiting (defensive; it is
-// never reached when the status array was cleared before the launch).
+// 2 inclusive prefix).  Callers number tiles so that a tile only ever waits
+// on tiles that are already running (an atomic ticket, or blockIdx where
+// workgroups are dispatched in id order), so every wait ends.  A spin cap
+// sets *err and stops waiting (defensive; it is never reached when the
+// status array was cleared before the launch).
 // ---------------------------------------------------------------------------
 constexpr uint64_t kFlagAgg = 1ull << 32;
 constexpr uint64_t kFlagInc = 2ull << 32;
@@ -111,6 +122,7 @@ __device__ __forceinline__ uint32_t lookback_wave(uint64_t* status,
       // a closer predecessor has not published yet: wait for it
       if (++spins > kSpinCap) {
         if (lane == 0) atomicOr(err, 1u);
+        __threadfence();  // the error bit is visible before this tile's status
         break;
       }
       __builtin_amdgcn_s_sleep(1);

@@ -300,7 +300,7 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
   const int wave = tid / WAVE;
   const int lane = tid % WAVE;
   const uint32_t tile = blockIdx.x;
-  const uint64_t n = SEL ? (a.in_count ? (uint64_t)*a.in_count : a.n_rows) : a.n_rows;
+  const uint64_t n = SEL ? dev_count(a.in_count, a.n_rows) : a.n_rows;
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
   uint32_t rid[FJ][4];  // row ids (generic path) or mark offsets (narrow path)
@@ -362,71 +362,283 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
 #pragma unroll
       for (int w = 0; w < FWAVES; ++w) t += s_wave_tot[w];
       a.tile_counts[tile] = t;
+      if (a.group_sums && t) atomicAdd(&a.group_sums[(tile >> kGroupShift) * kGroupStride], t);
     }
   }
 }
 
-// Pass 2: exclusive scan of the tile counts in place (one block of 1024),
-// total to *total.  Chunks of 1024 counts are loaded coalesced, 8 chunks at a
-// time (their loads overlap), and scanned in LDS with a running carry.
+// Pass 2: exclusive scan of the tile counts in place, total to *total.  One
+// block of 1024: each thread owns a run of SRUN consecutive counts (serial
+// sums), then one block-wide scan of the thread totals per round of
+// 1024 * SRUN counts, with a running carry between rounds.
 constexpr int SBLOCK = 1024;
+constexpr int SRUN = 16;
 __global__ __launch_bounds__(SBLOCK) void k_scan_counts(uint32_t* counts, uint32_t T,
                                                         uint32_t* total) {
   __shared__ uint32_t s_wave[SBLOCK / WAVE];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
   uint32_t carry = 0;
-  for (uint32_t g = 0; g < T; g += 8 * SBLOCK) {
-    uint32_t v[8];
+  for (uint32_t g = 0; g < T; g += SBLOCK * SRUN) {
+    const uint32_t i0 = g + (uint32_t)tid * SRUN;
+    uint32_t v[SRUN];
+    uint32_t sum = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      uint32_t i = g + c * SBLOCK + tid;
-      v[c] = i < T ? counts[i] : 0u;
+    for (int k = 0; k < SRUN; ++k) {
+      v[k] = i0 + k < T ? counts[i0 + k] : 0u;
+      sum += v[k];
     }
+    const uint32_t incl = wave_inclusive_u32(sum);
+    if (lane == WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      uint32_t incl = wave_inclusive_u32(v[c]);
-      if (lane == WAVE - 1) s_wave[wave] = incl;
-      __syncthreads();
-      uint32_t off = 0, all = 0;
-#pragma unroll
-      for (int w = 0; w < SBLOCK / WAVE; ++w) {
-        uint32_t t = s_wave[w];
-        off += w < wave ? t : 0u;
-        all += t;
-      }
-      uint32_t i = g + c * SBLOCK + tid;
-      if (i < T) counts[i] = carry + off + incl - v[c];
-      carry += all;
-      __syncthreads();
+    for (int w = 0; w < SBLOCK / WAVE; ++w) {
+      const uint32_t t = s_wave[w];
+      off += w < wave ? t : 0u;
+      all += t;
     }
+    uint32_t run = carry + off + incl - sum;
+#pragma unroll
+    for (int k = 0; k < SRUN; ++k) {
+      if (i0 + k < T) counts[i0 + k] = run;
+      run += v[k];
+    }
+    carry += all;
+    __syncthreads();
   }
   if (tid == 0) *total = carry;
 }
 
-// Pass 3: expand the pass bits of each tile into ascending row ids at the
-// tile's offset.  Thread t of the block owns rows 16t .. 16t+15 of the tile.
-__global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ sel_bm,
-                                                   const uint32_t* __restrict__ offsets,
-                                                   const uint32_t* __restrict__ in_sel,
-                                                   uint32_t* __restrict__ out) {
-  __shared__ uint32_t s_wave[FWAVES];
+template <typename T>
+__device__ __forceinline__ T proj_int(T x, T s, int op, bool& ovf) {
+  T r;
+  bool o;
+  switch (op) {
+    case MTL_PROJ_ADD: o = __builtin_add_overflow(x, s, &r); break;
+    case MTL_PROJ_MUL: o = __builtin_mul_overflow(x, s, &r); break;
+    case MTL_PROJ_SUB: o = __builtin_sub_overflow(x, s, &r); break;
+    default: o = __builtin_sub_overflow(s, x, &r); break;
+  }
+  ovf |= o;
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T proj_fp(T x, T s, int op) {
+  switch (op) {
+    case MTL_PROJ_ADD: return x + s;
+    case MTL_PROJ_MUL: return x * s;
+    case MTL_PROJ_SUB: return x - s;
+    default: return s - x;
+  }
+}
+
+// Optional projection fused into the expansion: out[i] = col[row_i] op s.
+struct ExpandProj {
+  const void* col;
+  int32_t op;
+  int64_t scalar;  // int64, or double bits for FLOAT/DOUBLE
+  void* out;
+  uint32_t* ovf;
+};
+
+// Fused projection of row r: the column value (raw bits), then the result
+// written at output position o.  PK: 0 none, 1 int32, 2 int64, 3 float,
+// 4 double.
+template <int PK>
+__device__ __forceinline__ uint64_t proj_load(const ExpandProj& pj, uint32_t r) {
+  if constexpr (PK == 1 || PK == 3) return ((const uint32_t*)pj.col)[r];
+  else if constexpr (PK == 2 || PK == 4) return ((const uint64_t*)pj.col)[r];
+  else return 0;
+}
+template <int PK>
+__device__ __forceinline__ void proj_store(const ExpandProj& pj, uint64_t o, uint64_t x,
+                                           bool& ovf) {
+  if constexpr (PK == 1) {
+    ((int32_t*)pj.out)[o] = proj_int<int32_t>((int32_t)(uint32_t)x, (int32_t)pj.scalar, pj.op, ovf);
+  } else if constexpr (PK == 2) {
+    ((int64_t*)pj.out)[o] = proj_int<int64_t>((int64_t)x, pj.scalar, pj.op, ovf);
+  } else if constexpr (PK == 3) {
+    ((float*)pj.out)[o] = proj_fp<float>(__builtin_bit_cast(float, (uint32_t)x),
+                                         (float)__longlong_as_double(pj.scalar), pj.op);
+  } else if constexpr (PK == 4) {
+    ((double*)pj.out)[o] =
+        proj_fp<double>(__builtin_bit_cast(double, x), __longlong_as_double(pj.scalar), pj.op);
+  }
+}
+
+// Tiles are grouped 256 to a group: the filter pass adds each tile's count
+// to its group's sum (one non-returning atomic per tile, 256 per address,
+// one cache line per group),
+// so a tile's output offset is the sum of the groups before its own plus
+// the counts of the tiles before it in its group, at most 255 + tiles/256
+// loads spread over one block.  This replaces a scan pass (or a decoupled
+// look-back, whose depth grows with the thousands of tiles in flight).
+
+__device__ __forceinline__ uint32_t tile_offset(const uint32_t* __restrict__ counts,
+                                                const uint32_t* __restrict__ group_sums,
+                                                uint32_t tile, uint32_t* s_red) {
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint64_t r0 = (uint64_t)blockIdx.x * FTILE + (uint64_t)tid * 16;
-  const uint32_t m = (uint32_t)(sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
+  const uint32_t g = tile >> kGroupShift, first = g << kGroupShift;
+  uint32_t part = first + tid < tile ? counts[first + tid] : 0u;
+  for (uint32_t i = tid; i < g; i += FBLOCK) part += group_sums[i * kGroupStride];
+  part = wave_sum_u32(part);
+  if (lane == 0) s_red[wave] = part;
+  __syncthreads();
+  uint32_t off = 0;
+#pragma unroll
+  for (int w = 0; w < FWAVES; ++w) off += s_red[w];
+  return off;
+}
+
+// One 4096-row tile per block: the positions of the tile's pass bits are a
+// block prefix sum; the row ids go to LDS first, then to out with one
+// coalesced store per thread and position (consecutive lanes, consecutive
+// addresses), and the projection, when fused, gathers its column in row
+// order in the same loop, four positions per thread in flight.  The tile's
+// output offset is offsets[tile] (scanned counts) or, with group_sums, is
+// computed here from the raw counts; then the last tile writes *out_count.
+template <int PK>
+__global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ sel_bm,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ group_sums,
+                                                   const uint32_t* __restrict__ in_sel,
+                                                   uint32_t* __restrict__ out,
+                                                   uint32_t* __restrict__ out_count,
+                                                   ExpandProj pj) {
+  __shared__ uint32_t s_ids[FTILE];
+  __shared__ uint32_t s_wave[FWAVES];
+  __shared__ uint32_t s_red[FWAVES];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t t0 = (uint64_t)tile * FTILE;
+  const uint32_t m = (uint32_t)(sel_bm[(t0 + (uint64_t)tid * 16) / 64] >> ((tid * 16) % 64)) & 0xFFFFu;
+  const uint32_t base = group_sums ? tile_offset(counts, group_sums, tile, s_red) : counts[tile];
   const uint32_t c = (uint32_t)__popc(m);
-  uint32_t incl = wave_inclusive_u32(c);
+  const uint32_t incl = wave_inclusive_u32(c);
   if (lane == WAVE - 1) s_wave[wave] = incl;
   __syncthreads();
-  uint32_t pos = offsets[blockIdx.x] + incl - c;
+  uint32_t pos = incl - c, total = 0;
 #pragma unroll
-  for (int w = 0; w < FWAVES; ++w)
+  for (int w = 0; w < FWAVES; ++w) {
     if (w < wave) pos += s_wave[w];
+    total += s_wave[w];
+  }
   uint32_t mm = m;
   while (mm) {
-    int i = __ffs(mm) - 1;
+    const int i = __ffs(mm) - 1;
     mm &= mm - 1;
-    uint64_t r = r0 + (uint64_t)i;
-    out[pos++] = in_sel ? in_sel[r] : (uint32_t)r;
+    s_ids[pos++] = (uint32_t)tid * 16 + (uint32_t)i;  // row within the tile
+  }
+  __syncthreads();
+  bool ovf = false;
+  uint32_t i = tid;
+  for (; i + 3 * FBLOCK < total; i += 4 * FBLOCK) {
+    uint32_t r[4];
+    uint64_t x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t k = t0 + s_ids[i + u * FBLOCK];
+      r[u] = in_sel ? in_sel[k] : (uint32_t)k;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = proj_load<PK>(pj, r[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      out[base + i + u * FBLOCK] = r[u];
+      proj_store<PK>(pj, base + i + u * FBLOCK, x[u], ovf);
+    }
+  }
+  for (; i < total; i += FBLOCK) {
+    const uint64_t k = t0 + s_ids[i];
+    const uint32_t r = in_sel ? in_sel[k] : (uint32_t)k;
+    const uint64_t x = proj_load<PK>(pj, r);
+    out[base + i] = r;
+    proj_store<PK>(pj, base + i, x, ovf);
+  }
+  if constexpr (PK == 1 || PK == 2) {
+    if (__ballot(ovf) && lane == 0 && pj.ovf) atomicOr(pj.ovf, 1u);
+  }
+  if (group_sums && tile == gridDim.x - 1 && tid == 0) *out_count = base + total;
+}
+
+// ---------------------------------------------------------------------------
+// Dense filter pass (no input selection, no probe or mark), the first of the
+// two passes of a dense selection: the predicates are evaluated with loads
+// of each column's own width (no 64-bit widening, so fewer registers and a
+// higher occupancy than the general pass), the pass bits go to sel_bm, the
+// tile's count to counts[tile] and to its group's sum.
+// ---------------------------------------------------------------------------
+struct DenseArgs {
+  PredC preds[MTL_MAX_PREDS];
+  uint32_t n_preds;
+  uint32_t vec_ok;  // bit p: column p is aligned for vector loads
+  uint64_t n_rows;
+  uint64_t* sel_bm;
+  uint32_t* counts;
+  uint32_t* group_sums;
+};
+
+// Clear the bits of the lane's 16 dense rows that fail p; T is the raw
+// storage type, KIND the predicate's encoding (fixed at compile time so
+// only this type's loads and compares are emitted).
+template <typename T, uint32_t KIND>
+__device__ __forceinline__ void pred_dense_t(const PredC& p, uint64_t k0, uint64_t n, bool vec,
+                                             uint32_t& bits) {
+  T x[FJ][4];
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) load4<T>((const T*)p.col, k0 + (uint64_t)j * 256, n, vec, x[j]);
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (!pred_pass(enc_raw((uint64_t)x[j][v], KIND), p)) bits &= ~(1u << (4 * j + v));
+}
+
+__device__ __forceinline__ void pred_dense_typed(const PredC& p, uint64_t k0, uint64_t n,
+                                                 bool vec, uint32_t& bits) {
+  switch (p.kind) {
+    case PK_I32: pred_dense_t<uint32_t, PK_I32>(p, k0, n, vec, bits); break;
+    case PK_F32: pred_dense_t<uint32_t, PK_F32>(p, k0, n, vec, bits); break;
+    case PK_U8: pred_dense_t<uint8_t, PK_U8>(p, k0, n, vec, bits); break;
+    case PK_I64: pred_dense_t<uint64_t, PK_I64>(p, k0, n, vec, bits); break;
+    default: pred_dense_t<uint64_t, PK_F64>(p, k0, n, vec, bits); break;
+  }
+}
+
+__global__ __launch_bounds__(FBLOCK) void k_filter_dense(DenseArgs a) {
+  __shared__ uint32_t s_wave[FWAVES];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t n = a.n_rows;
+  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
+  const uint64_t k0 = seg + (uint64_t)lane * 4;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
+  for (uint32_t p = 0; p < a.n_preds; ++p)
+    pred_dense_typed(a.preds[p], k0, n, (a.vec_ok >> p) & 1u, bits);
+  // lane l's 4 bits of step j sit at bits 4(l%16).. of word l/16 of the
+  // 256-row segment: OR the 16 lanes of each quarter-wave together
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    uint64_t w = (uint64_t)((bits >> (4 * j)) & 0xFu) << (4 * (lane & 15));
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) w |= __shfl_xor(w, off, WAVE);
+    if ((lane & 15) == 0) a.sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = w;
+  }
+  const uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
+  if (lane == 0) s_wave[wave] = c;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < FWAVES; ++w) t += s_wave[w];
+    a.counts[tile] = t;
+    if (t) atomicAdd(&a.group_sums[(tile >> kGroupShift) * kGroupStride], t);
   }
 }
 
@@ -434,13 +646,68 @@ void scan_counts(uint32_t* counts, uint32_t tiles, uint32_t* total, hipStream_t 
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SBLOCK), 0, st, counts, tiles, total);
 }
 
+// Kernel variant of a projection: 1 int32, 2 int64, 3 float, 4 double.
+// DATE and TIMESTAMP take only + and - (of days, of microseconds); DuckDB
+// has no DATE * INTEGER.
+static int32_t proj_kind(int32_t type, int32_t op, int* pk) {
+  if (op < 0 || op > 3) return set_error(MTL_ERR_INVALID_ARGUMENT, "project op %d", op);
+  switch (type) {
+    case MTL_INTEGER: *pk = 1; return MTL_OK;
+    case MTL_BIGINT:
+    case MTL_DECIMAL: *pk = 2; return MTL_OK;
+    case MTL_DATE:
+    case MTL_TIMESTAMP:
+      if (op == MTL_PROJ_MUL || op == MTL_PROJ_SUBFROM)
+        return set_error(MTL_UNSUPPORTED, "%s supports only + and -",
+                         type == MTL_DATE ? "DATE" : "TIMESTAMP");
+      *pk = type == MTL_DATE ? 1 : 2;
+      return MTL_OK;
+    case MTL_FLOAT: *pk = 3; return MTL_OK;
+    case MTL_DOUBLE: *pk = 4; return MTL_OK;
+    default: return set_error(MTL_UNSUPPORTED, "projection on type %d", type);
+  }
+}
+
 int32_t launch_compact(const uint64_t* sel_bm, uint32_t* tile_counts, uint32_t tiles,
                        const uint32_t* in_sel, uint32_t* out_sel, uint32_t* out_count,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SBLOCK), 0, st, tile_counts, tiles, out_count);
+                       hipStream_t st, const CompactProj* proj, const uint32_t* group_sums) {
+  if (!group_sums) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SBLOCK), 0, st, tile_counts, tiles,
+                       out_count);
+    MTL_HIP(hipGetLastError());
+  }
+  ExpandProj pj{};
+  int pk = 0;
+  if (proj) {
+    pj.col = proj->col;
+    pj.op = proj->op;
+    pj.scalar = proj->scalar;
+    pj.out = proj->out;
+    pj.ovf = proj->ovf;
+    MTL_TRY(proj_kind(proj->type, proj->op, &pk));
+  }
+#define MTL_EXPAND(K)                                                                           \
+  hipLaunchKernelGGL(k_expand<K>, dim3(tiles), dim3(FBLOCK), 0, st, sel_bm, tile_counts,       \
+                     group_sums, in_sel, out_sel, out_count, pj)
+  switch (pk) {
+    case 1: MTL_EXPAND(1); break;
+    case 2: MTL_EXPAND(2); break;
+    case 3: MTL_EXPAND(3); break;
+    case 4: MTL_EXPAND(4); break;
+    default: MTL_EXPAND(0); break;
+  }
+#undef MTL_EXPAND
   MTL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_expand, dim3(tiles), dim3(FBLOCK), 0, st, sel_bm, tile_counts, in_sel,
-                     out_sel);
+  return MTL_OK;
+}
+
+static int32_t launch_filter_dense(DenseArgs a, uint32_t tiles, hipStream_t st) {
+  a.vec_ok = 0;
+  for (uint32_t p = 0; p < a.n_preds; ++p) {
+    const uint32_t w = pred_width(a.preds[p].kind);
+    if ((uintptr_t)a.preds[p].col % (w == 1 ? 4 : 16) == 0) a.vec_ok |= 1u << p;
+  }
+  hipLaunchKernelGGL(k_filter_dense, dim3(tiles), dim3(FBLOCK), 0, st, a);
   MTL_HIP(hipGetLastError());
   return MTL_OK;
 }
@@ -511,7 +778,7 @@ __global__ void k_compare(CompareArgs a) {
 template <typename T>
 __global__ void k_gather(const T* __restrict__ col, const uint32_t* __restrict__ sel,
                          const uint32_t* count, uint64_t cap, T* __restrict__ out) {
-  const uint64_t n = count ? (uint64_t)*count : cap;
+  const uint64_t n = dev_count(count, cap);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = col[sel ? sel[i] : i];
@@ -520,41 +787,35 @@ __global__ void k_gather(const T* __restrict__ col, const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // Scalar projection with DuckDB overflow semantics.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T proj_int(T x, T s, int op, bool& ovf) {
-  T r;
-  bool o;
-  switch (op) {
-    case MTL_PROJ_ADD: o = __builtin_add_overflow(x, s, &r); break;
-    case MTL_PROJ_MUL: o = __builtin_mul_overflow(x, s, &r); break;
-    case MTL_PROJ_SUB: o = __builtin_sub_overflow(x, s, &r); break;
-    default: o = __builtin_sub_overflow(s, x, &r); break;
-  }
-  ovf |= o;
-  return r;
-}
-
-template <typename T>
-__device__ __forceinline__ T proj_fp(T x, T s, int op) {
-  switch (op) {
-    case MTL_PROJ_ADD: return x + s;
-    case MTL_PROJ_MUL: return x * s;
-    case MTL_PROJ_SUB: return x - s;
-    default: return s - x;
-  }
-}
-
+// Each thread handles PU elements per grid stride, with all their selection
+// loads, then all their column gathers, issued before the first use.
+constexpr int PU = 4;
 template <typename T, bool IS_FP>
 __global__ void k_project(const T* __restrict__ col, const uint32_t* __restrict__ sel,
                           const uint32_t* count, uint64_t cap, int op, T s,
                           T* __restrict__ out, uint32_t* ovf_flag) {
-  const uint64_t n = count ? (uint64_t)*count : cap;
+  const uint64_t n = dev_count(count, cap);
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
   bool ovf = false;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    T x = col[sel ? sel[i] : i];
-    if constexpr (IS_FP) out[i] = proj_fp<T>(x, s, op);
-    else out[i] = proj_int<T>(x, s, op, ovf);
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += G * PU) {
+    uint32_t r[PU];
+    T x[PU];
+#pragma unroll
+    for (int k = 0; k < PU; ++k) {
+      const uint64_t i = i0 + k * G;
+      r[k] = i < n ? (sel ? sel[i] : (uint32_t)i) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < PU; ++k)
+      if (i0 + k * G < n) x[k] = col[r[k]];
+#pragma unroll
+    for (int k = 0; k < PU; ++k) {
+      const uint64_t i = i0 + k * G;
+      if (i < n) {
+        if constexpr (IS_FP) out[i] = proj_fp<T>(x[k], s, op);
+        else out[i] = proj_int<T>(x[k], s, op, ovf);
+      }
+    }
   }
   if constexpr (!IS_FP) {
     if (__ballot(ovf) && lane_id() == 0 && ovf_flag) atomicOr(ovf_flag, 1u);
@@ -573,10 +834,10 @@ using namespace mtl;
 
 extern "C" {
 
-int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
-                   const uint32_t* in_sel, const uint32_t* in_count,
-                   uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
-                   mtl_stream* s) {
+static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                           const uint32_t* in_sel, const uint32_t* in_count,
+                           uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
+                           const CompactProj* proj, mtl_stream* s) {
   if (n_preds > MTL_MAX_PREDS)
     return set_error(MTL_UNSUPPORTED, "more than %d predicates", MTL_MAX_PREDS);
   if (!out_sel || !out_count)
@@ -593,29 +854,66 @@ int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
   std::lock_guard<std::mutex> g(s->mu);
   s->arena.reset();
   uint32_t tiles = filter_tiles(cap);
-  void *bm, *counts, *ctr;
+  void *bm, *counts, *ctr, *gsum;
   MTL_TRY(s->arena.alloc((size_t)tiles * FTILE / 8, &bm));
   MTL_TRY(s->arena.alloc((size_t)tiles * 4, &counts));
   MTL_TRY(s->arena.alloc(256, &ctr));
+  const uint32_t groups = (uint32_t)ceil_div(tiles, 1u << kGroupShift);
+  MTL_TRY(s->arena.alloc((size_t)groups * kGroupStride * 4, &gsum));
   ZeroList zl;
   zl.add((uint32_t*)ctr, 64);
+  zl.add((uint32_t*)gsum, (uint64_t)groups * kGroupStride);
   MTL_TRY(launch_zero(zl, s->stream));
-  FilterArgs a{};
-  for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = compile_pred(preds[p]);
-  a.n_preds = n_preds;
-  a.n_rows = cap;
-  a.in_sel = in_sel;
-  a.in_count = in_sel ? in_count : nullptr;
-  a.sel_bm = (uint64_t*)bm;
-  a.tile_counts = (uint32_t*)counts;
-  a.err = (uint32_t*)ctr;
-  {
+  if (!in_sel) {
+    DenseArgs a{};
+    for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = compile_pred(preds[p]);
+    a.n_preds = n_preds;
+    a.n_rows = cap;
+    a.sel_bm = (uint64_t*)bm;
+    a.counts = (uint32_t*)counts;
+    a.group_sums = (uint32_t*)gsum;
+    ProfScope ps(s, "k_filter");
+    MTL_TRY(launch_filter_dense(a, tiles, s->stream));
+  } else {
+    FilterArgs a{};
+    for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = compile_pred(preds[p]);
+    a.n_preds = n_preds;
+    a.n_rows = cap;
+    a.in_sel = in_sel;
+    a.in_count = in_count;
+    a.sel_bm = (uint64_t*)bm;
+    a.tile_counts = (uint32_t*)counts;
+    a.group_sums = (uint32_t*)gsum;
+    a.err = (uint32_t*)ctr;
     ProfScope ps(s, "k_filter");
     MTL_TRY(launch_filter(a, cap, s->stream));
   }
   ProfScope ps(s, "k_compact");
   return launch_compact((const uint64_t*)bm, (uint32_t*)counts, tiles, in_sel, out_sel,
-                        out_count, s->stream);
+                        out_count, s->stream, proj, (const uint32_t*)gsum);
+}
+
+int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                   const uint32_t* in_sel, const uint32_t* in_count,
+                   uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
+                   mtl_stream* s) {
+  return filter_impl(preds, n_preds, n_rows, in_sel, in_count, in_capacity, out_sel,
+                     out_count, nullptr, s);
+}
+
+int32_t mtl_filter_project(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                           const uint32_t* in_sel, const uint32_t* in_count,
+                           uint64_t in_capacity, const void* proj_column, int32_t proj_type,
+                           int32_t op, int64_t scalar, uint32_t* out_sel,
+                           uint32_t* out_count, void* out_proj, uint32_t* overflow_flag,
+                           mtl_stream* s) {
+  int pk = 0;
+  MTL_TRY(proj_kind(proj_type, op, &pk));
+  if (!proj_column || !out_proj)
+    return set_error(MTL_ERR_INVALID_ARGUMENT, "proj_column/out_proj are NULL");
+  CompactProj pj{proj_column, proj_type, op, scalar, out_proj, overflow_flag};
+  return filter_impl(preds, n_preds, n_rows, in_sel, in_count, in_capacity, out_sel,
+                     out_count, &pj, s);
 }
 
 int32_t mtl_compare(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
@@ -664,44 +962,35 @@ int32_t mtl_project(const void* column, int32_t type, const uint32_t* sel,
                     const uint32_t* count, uint64_t capacity, int32_t op,
                     int64_t scalar, void* out, uint32_t* overflow_flag,
                     mtl_stream* s) {
-  if (op < 0 || op > 3) return set_error(MTL_ERR_INVALID_ARGUMENT, "project op %d", op);
+  int pk = 0;
+  MTL_TRY(proj_kind(type, op, &pk));
   MTL_TRY(resolve_stream(s, &s));
   if (capacity == 0) return MTL_OK;
+  if (!column || !out) return set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
   dim3 g(grid_for(capacity)), b(256);
-  switch (type) {
-    case MTL_INTEGER:
+  switch (pk) {
+    case 1:
       hipLaunchKernelGGL((k_project<int32_t, false>), g, b, 0, s->stream,
                          (const int32_t*)column, sel, count, capacity, op,
                          (int32_t)scalar, (int32_t*)out, overflow_flag);
       break;
-    case MTL_DATE:
-      // DATE +/- INTEGER days -> DATE (DuckDB has no DATE * INTEGER)
-      if (op == MTL_PROJ_MUL || op == MTL_PROJ_SUBFROM)
-        return set_error(MTL_UNSUPPORTED, "DATE supports only + and - of days");
-      hipLaunchKernelGGL((k_project<int32_t, false>), g, b, 0, s->stream,
-                         (const int32_t*)column, sel, count, capacity, op,
-                         (int32_t)scalar, (int32_t*)out, overflow_flag);
-      break;
-    case MTL_BIGINT:
-    case MTL_DECIMAL:
+    case 2:
       hipLaunchKernelGGL((k_project<int64_t, false>), g, b, 0, s->stream,
                          (const int64_t*)column, sel, count, capacity, op, scalar,
                          (int64_t*)out, overflow_flag);
       break;
-    case MTL_FLOAT:
+    case 3:
       hipLaunchKernelGGL((k_project<float, true>), g, b, 0, s->stream,
                          (const float*)column, sel, count, capacity, op,
                          (float)__builtin_bit_cast(double, scalar), (float*)out,
                          overflow_flag);
       break;
-    case MTL_DOUBLE:
+    default:
       hipLaunchKernelGGL((k_project<double, true>), g, b, 0, s->stream,
                          (const double*)column, sel, count, capacity, op,
                          __builtin_bit_cast(double, scalar), (double*)out,
                          overflow_flag);
       break;
-    default:
-      return set_error(MTL_UNSUPPORTED, "projection on type %d", type);
   }
   MTL_HIP(hipGetLastError());
   return MTL_OK;

@@ -58,6 +58,8 @@ struct FilterArgs {
   const uint32_t* in_count;
   uint64_t* sel_bm;       // optional: pass bits, 1 bit per row (tiles*4096)
   uint32_t* tile_counts;  // optional: pass count per 4096-row tile
+  uint32_t* group_sums;   // optional: per 256-tile group, sum of its counts
+                          // (group g at group_sums[g * kGroupStride])
   uint32_t* err;          // bit 3: mark key outside [min, max]
   // optional semi-join probe: keep the row only if bit (key - min) is set
   const uint64_t* probe_bm;
@@ -78,14 +80,31 @@ struct FilterArgs {
 // selection vector; with only mark_bm set it is a mark-only pass.
 
 constexpr int kTileRows = 4096;
+constexpr int kGroupShift = 8;  // 256 tiles to a group (see launch_compact)
+// Group sums sit 256 bytes apart: atomics to one cache line serialise, so
+// each group's 256 additions get a line of their own.
+constexpr int kGroupStride = 64;
 uint32_t filter_tiles(uint64_t n);
 int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st);
 // Exclusive scan of tile counts in place (one block); *total = sum.
 void scan_counts(uint32_t* counts, uint32_t tiles, uint32_t* total, hipStream_t st);
-// Exclusive scan of tile_counts (in place) + expansion of sel_bm into
-// ascending row ids (in_sel[k] when in_sel is given); *out_count = total.
+// Projection fused into the compaction: out[i] = col[row_i] op scalar.
+struct CompactProj {
+  const void* col;
+  int32_t type;
+  int32_t op;
+  int64_t scalar;
+  void* out;
+  uint32_t* ovf;  // device flag, set to 1 on integer overflow
+};
+// Expansion of sel_bm into ascending row ids (in_sel[k] when in_sel is
+// given); *out_count = total.  Each tile's output offset comes from an
+// exclusive scan of tile_counts (in place) or, when group_sums holds the
+// sums of each 256-tile group of the counts (kGroupStride words apart), is
+// computed by the expansion itself with no scan pass.
 int32_t launch_compact(const uint64_t* sel_bm, uint32_t* tile_counts, uint32_t tiles,
                        const uint32_t* in_sel, uint32_t* out_sel, uint32_t* out_count,
-                       hipStream_t st);
+                       hipStream_t st, const CompactProj* proj = nullptr,
+                       const uint32_t* group_sums = nullptr);
 
 }  // namespace mtl

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
     for (uint32_t w = 0; w < a.W; ++w) cst[i * a.W + w] = a.init[w];
   }
   __syncthreads();
-  const uint64_t n = a.sel ? (a.sel_count ? (uint64_t)*a.sel_count : a.sel_cap) : a.n_rows;
+  const uint64_t n = a.sel ? dev_count(a.sel_count, a.sel_cap) : a.n_rows;
   const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t start = (uint64_t)blockIdx.x * per;
   const uint64_t end = start + per < n ? start + per : n;

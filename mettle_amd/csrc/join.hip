@@ -75,7 +75,7 @@ __global__ void k_fill_table(int32_t strategy, void* table, uint64_t n) {
 __global__ void k_build(int32_t strategy, const void* keys, int32_t key_type,
                         const uint32_t* sel, const uint32_t* count, uint64_t cap_rows,
                         int64_t min_key, uint64_t slots, void* table, uint32_t* err) {
-  const uint64_t n = sel ? (count ? (uint64_t)*count : cap_rows) : cap_rows;
+  const uint64_t n = sel ? dev_count(count, cap_rows) : cap_rows;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t row = sel ? sel[i] : (uint32_t)i;
@@ -158,7 +158,7 @@ struct ProbeArgs {
 __global__ __launch_bounds__(FBLOCK) void k_probe_bits(ProbeArgs a) {
   __shared__ uint32_t s_wave[FWAVES];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint64_t n = a.sel ? (a.count ? (uint64_t)*a.count : a.cap_rows) : a.cap_rows;
+  const uint64_t n = a.sel ? dev_count(a.count, a.cap_rows) : a.cap_rows;
   const uint64_t seg = (uint64_t)blockIdx.x * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
   int64_t key[FJ][4];

@@ -45,10 +45,10 @@ constexpr int kFpMaxBlocks = 1024;
 // (mod 2^64); position-dependent, so any reordering changes it.
 __global__ __launch_bounds__(kFpBlock) void k_fingerprint(const uint32_t* __restrict__ sel,
                                                           const uint32_t* count,
-                                                          const void* vals, uint32_t width,
-                                                          uint64_t* partial) {
+                                                          uint64_t cap, const void* vals,
+                                                          uint32_t width, uint64_t* partial) {
   __shared__ uint64_t s_part[kFpBlock / WAVE];
-  const uint64_t n = *count;
+  const uint64_t n = dev_count(count, cap);
   uint64_t h = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * kFpBlock + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * kFpBlock) {
@@ -143,19 +143,22 @@ static int32_t run_scan(Slot& sl, const mtl_query& q, mtl_query_result& r) {
   ZeroList zl;
   zl.add(sl.words, 2);
   MTL_TRY(launch_zero(zl, st));
-  MTL_TRY(mtl_filter(q.preds, q.n_preds, q.n_rows, nullptr, nullptr, 0, sl.sel, sl.words,
-                     sl.s));
   const void* vals = sl.sel;
   uint32_t width = 4;
-  if (q.proj_col) {
-    MTL_TRY(mtl_project(q.proj_col, q.proj_type, sl.sel, sl.words, q.n_rows, q.proj_op,
-                        q.proj_scalar, sl.proj, sl.words + 1, sl.s));
+  if (q.proj_col) {  // projection fused into the compaction
+    MTL_TRY(mtl_filter_project(q.preds, q.n_preds, q.n_rows, nullptr, nullptr, 0, q.proj_col,
+                               q.proj_type, q.proj_op, q.proj_scalar, sl.sel, sl.words, sl.proj,
+                               sl.words + 1, sl.s));
     vals = sl.proj;
     width = (uint32_t)type_width(q.proj_type);
+  } else {
+    MTL_TRY(mtl_filter(q.preds, q.n_preds, q.n_rows, nullptr, nullptr, 0, sl.sel, sl.words,
+                       sl.s));
   }
   const uint32_t blocks = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>(ceil_div(q.n_rows ? q.n_rows : 1, kFpBlock * 16), kFpMaxBlocks));
-  hipLaunchKernelGGL(k_fingerprint, dim3(blocks), dim3(kFpBlock), 0, st, sl.sel, sl.words, vals,
+  hipLaunchKernelGGL(k_fingerprint, dim3(blocks), dim3(kFpBlock), 0, st, sl.sel, sl.words,
+                     (uint64_t)q.n_rows, vals,
                      width, sl.partial);
   hipLaunchKernelGGL(k_fingerprint_sum, dim3(1), dim3(1024), 0, st, sl.partial, blocks,
                      sl.partial + kFpMaxBlocks);
@@ -165,6 +168,7 @@ static int32_t run_scan(Slot& sl, const mtl_query& q, mtl_query_result& r) {
   MTL_HIP(hipMemcpyAsync(h + 8, sl.partial + kFpMaxBlocks, 8, hipMemcpyDeviceToHost, st));
   MTL_HIP(hipStreamSynchronize(st));
   const uint32_t* hw = (const uint32_t*)h;
+  if (hw[0] > q.n_rows) return set_error(MTL_ERR_PROCESSING, "device scan failed");
   r.count = hw[0];
   r.fingerprint = *(const uint64_t*)(h + 8);
   if (hw[1]) return set_error(MTL_ERR_OVERFLOW, "Overflow in projection");

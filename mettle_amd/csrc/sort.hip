@@ -23,7 +23,7 @@ template <typename K>
 __global__ void k_sort_keys(const void* col, int32_t type, const uint32_t* sel,
                             const uint32_t* count, uint64_t cap, int32_t desc, K* keys,
                             uint32_t* rows) {
-  const uint64_t n = count ? (uint64_t)*count : cap;
+  const uint64_t n = dev_count(count, cap);
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t r = sel ? sel[i] : (uint32_t)i;

@@ -864,6 +864,7 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     rows = (const uint32_t*)perm.p;
   } else {
     MTL_TRY(mtl_memcpy_d2h(&n, cnt.p, 4, s));
+    if (n > T.rows) return set_error(MTL_ERR_PROCESSING, "device scan failed");
   }
   if (q.limit >= 0 && (uint64_t)q.limit < n) n = (uint32_t)q.limit;
   res.rows = n;
@@ -1010,6 +1011,9 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
     MTL_TRY(cnt.alloc(16));
     MTL_TRY(mtl_filter(preds.data(), (uint32_t)preds.size(), T.rows, nullptr, nullptr, 0,
                        (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
+    uint32_t n_sel = 0;
+    MTL_TRY(mtl_memcpy_d2h(&n_sel, cnt.p, 4, s));
+    if (n_sel > T.rows) return set_error(MTL_ERR_PROCESSING, "device scan failed");
     d.sel = (const uint32_t*)sel.p;
     d.sel_count = (const uint32_t*)cnt.p;
     d.sel_capacity = T.rows;

@@ -45,6 +45,15 @@ def sync() -> None:
     abi.check(abi.lib().mtl_stream_sync(None))
 
 
+def _count(cnt: torch.Tensor, cap: int) -> int:
+    """Device row count of a selection; a count above the capacity is a
+    processing error, never a result."""
+    n = int(cnt.item()) & 0xFFFFFFFF
+    if n > cap:
+        raise abi.MettleError(abi.MTL_ERR_PROCESSING, f"selection count {n} exceeds {cap}")
+    return n
+
+
 def _preds(preds):
     arr = (abi.Pred * max(1, len(preds)))()
     for i, (col, t, op, val) in enumerate(preds):
@@ -62,8 +71,29 @@ def filter(preds, n_rows: int, in_sel: torch.Tensor | None = None,
     abi.check(abi.lib().mtl_filter(_preds(preds), len(preds), n_rows, ptr(in_sel),
                                    ptr(in_count), cap, ptr(out), ptr(cnt), None))
     sync()
-    n = int(cnt.item())
+    n = _count(cnt, cap)
     return out[:n], n
+
+
+def filter_project(preds, n_rows: int, col: torch.Tensor, t: int, op: int, value,
+                   in_sel: torch.Tensor | None = None, in_count: torch.Tensor | None = None):
+    """Fused filter + projection of the passing rows.
+
+    Returns (sel, projected values, overflow flag); the values are
+    col[sel[i]] op value, computed as the selection is written."""
+    torch.cuda.synchronize()
+    cap = in_sel.numel() if in_sel is not None else n_rows
+    out = torch.empty(max(cap, 4), dtype=torch.int32, device=device())
+    vals = torch.empty(max(cap, 4), dtype=col.dtype, device=device())
+    cnt = torch.zeros(1, dtype=torch.int32, device=device())
+    flag = torch.zeros(1, dtype=torch.int32, device=device())
+    abi.check(abi.lib().mtl_filter_project(_preds(preds), len(preds), n_rows, ptr(in_sel),
+                                           ptr(in_count), cap, ptr(col), t, op,
+                                           abi.scalar_bits(value, t), ptr(out), ptr(cnt),
+                                           ptr(vals), ptr(flag), None))
+    sync()
+    n = _count(cnt, cap)
+    return out[:n], vals[:n], bool(flag.item())
 
 
 def compare(preds, n_rows: int) -> torch.Tensor:

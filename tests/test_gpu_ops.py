@@ -76,6 +76,33 @@ def test_filter_int_column_wide_literal(ops, lit):
         np.testing.assert_array_equal(ops.to_host(got, np.uint32), ref)
 
 
+@pytest.mark.parametrize("t,dtype", [(abi.DICT8, np.uint8), (abi.INTEGER, np.int32),
+                                     (abi.BIGINT, np.int64), (abi.DOUBLE, np.float64)])
+def test_filter_unaligned_columns(ops, t, dtype):
+    """Columns that start off the vector-load alignment (scalar loads)."""
+    rng = np.random.default_rng(300 + t)
+    col = make_col(rng, dtype, 50_001)
+    lit = col[777].item()
+    d = ops.to_device(col)[1:]
+    ref = orc.filter([(col[1:], t, abi.CMP_LE, lit)], len(col) - 1)
+    got, _ = ops.filter([(d, t, abi.CMP_LE, lit)], len(col) - 1)
+    np.testing.assert_array_equal(ops.to_host(got, np.uint32), ref)
+
+
+@pytest.mark.parametrize("frac", [0.0, 0.001, 0.5, 1.0])
+def test_filter_many_tiles(ops, frac):
+    """Thousands of 4096-row tiles (look-back windows well past 64 tiles),
+    from no row to every row passing."""
+    rng = np.random.default_rng(int(frac * 1000) + 7)
+    n = 12_000_017
+    col = rng.integers(0, 1 << 30, n, dtype=np.int32)
+    lit = int(frac * (1 << 30))
+    ref = orc.filter([(col, abi.INTEGER, abi.CMP_LT, lit)], n)
+    got, cnt = ops.filter([(ops.to_device(col), abi.INTEGER, abi.CMP_LT, lit)], n)
+    assert cnt == len(ref)
+    np.testing.assert_array_equal(ops.to_host(got, np.uint32), ref)
+
+
 def test_filter_float_nan_semantics(ops):
     col = np.array([np.nan, 1.0, -np.inf, np.inf, np.nan, 0.0, -0.0], dtype=np.float32)
     d = ops.to_device(col)
@@ -125,6 +152,83 @@ def test_project_overflow_detected(ops):
     assert not ovf
     _, ovf = ops.project(ops.to_device(col), abi.INTEGER, None, abi.PROJ_ADD, 10)
     assert ovf
+
+
+@pytest.mark.parametrize("t,dtype,op", [(abi.DATE, np.int32, abi.PROJ_ADD),
+                                        (abi.DATE, np.int32, abi.PROJ_SUB),
+                                        (abi.TIMESTAMP, np.int64, abi.PROJ_ADD),
+                                        (abi.TIMESTAMP, np.int64, abi.PROJ_SUB)])
+def test_project_date_timestamp(ops, t, dtype, op):
+    rng = np.random.default_rng(60 + t + op)
+    col = (make_col(rng, dtype, 20_000) // 65536).astype(dtype)
+    s = 86_400_000_000 if t == abi.TIMESTAMP else 30
+    ref, rovf = orc.project(col, t, None, op, s)
+    got, govf = ops.project(ops.to_device(col), t, None, op, s)
+    np.testing.assert_array_equal(ops.to_host(got), ref)
+    assert govf == rovf
+
+
+@pytest.mark.parametrize("t", [abi.DATE, abi.TIMESTAMP])
+@pytest.mark.parametrize("op", [abi.PROJ_MUL, abi.PROJ_SUBFROM])
+def test_project_date_timestamp_rejects_mul(ops, t, op):
+    col = np.arange(8, dtype=np.int32 if t == abi.DATE else np.int64)
+    with pytest.raises(abi.Unsupported):
+        ops.project(ops.to_device(col), t, None, op, 2)
+    with pytest.raises(abi.Unsupported):
+        ops.filter_project([], len(col), ops.to_device(col), t, op, 2)
+
+
+FP_TYPES = [(abi.INTEGER, np.int32), (abi.BIGINT, np.int64), (abi.DECIMAL, np.int64),
+            (abi.DATE, np.int32), (abi.TIMESTAMP, np.int64), (abi.FLOAT, np.float32),
+            (abi.DOUBLE, np.float64)]
+
+
+@pytest.mark.parametrize("t,dtype", FP_TYPES)
+@pytest.mark.parametrize("n", [0, 1, 4095, 4097, 250_003])
+def test_filter_project_parity(ops, t, dtype, n):
+    """The fused entry equals mtl_filter followed by the oracle's projection."""
+    rng = np.random.default_rng(70 + t * 11 + n)
+    a = rng.integers(-1000, 1000, max(n, 1)).astype(np.int32)[:n]
+    col = make_col(rng, dtype, max(n, 1))[:n]
+    if not np.issubdtype(dtype, np.floating):
+        col = (col // 4096).astype(dtype)
+    fp = np.issubdtype(dtype, np.floating)
+    op = abi.PROJ_ADD if t in (abi.DATE, abi.TIMESTAMP) else abi.PROJ_MUL
+    s = 1.1 if fp else 3
+    preds_h = [(a, abi.INTEGER, abi.CMP_LT, 250)]
+    sel = orc.filter(preds_h, n)
+    ref, rovf = orc.project(col, t, sel, op, s)
+    da = ops.to_device(a) if n else ops.to_device(np.zeros(1, np.int32))
+    dc = ops.to_device(col) if n else ops.to_device(np.zeros(1, dtype))
+    gsel, gvals, govf = ops.filter_project([(da, abi.INTEGER, abi.CMP_LT, 250)], n, dc, t, op, s)
+    np.testing.assert_array_equal(ops.to_host(gsel, np.uint32), sel)
+    np.testing.assert_array_equal(ops.to_host(gvals).view(np.uint8), ref.view(np.uint8))
+    assert govf == rovf
+
+
+def test_filter_project_refine_and_overflow(ops):
+    """An input selection with a device count; overflow of a passing row is
+    flagged, overflow of a failing row is not."""
+    rng = np.random.default_rng(80)
+    n = 100_000
+    a = rng.integers(0, 100, n).astype(np.int32)
+    v = rng.integers(-1000, 1000, n).astype(np.int32)
+    big = np.where(a >= 50)[0][:3]
+    v[big] = 2**31 - 5
+    da, dv = ops.to_device(a), ops.to_device(v)
+    sel0, cnt0 = ops.filter([(da, abi.INTEGER, abi.CMP_GE, 10)], n)
+    cnt_t = torch.tensor([cnt0], dtype=torch.int32, device="cuda")
+    # rows with a < 50 never carry the large values: no overflow
+    s1, v1, o1 = ops.filter_project([(da, abi.INTEGER, abi.CMP_LT, 50)], n, dv, abi.INTEGER,
+                                    abi.PROJ_ADD, 10, in_sel=sel0.contiguous(), in_count=cnt_t)
+    ref_sel = orc.filter([(a, abi.INTEGER, abi.CMP_GE, 10), (a, abi.INTEGER, abi.CMP_LT, 50)], n)
+    ref_v, ref_o = orc.project(v, abi.INTEGER, ref_sel, abi.PROJ_ADD, 10)
+    np.testing.assert_array_equal(ops.to_host(s1, np.uint32), ref_sel)
+    np.testing.assert_array_equal(ops.to_host(v1), ref_v)
+    assert not o1 and not ref_o
+    _, _, o2 = ops.filter_project([(da, abi.INTEGER, abi.CMP_GE, 50)], n, dv, abi.INTEGER,
+                                  abi.PROJ_ADD, 10)
+    assert o2
 
 
 @pytest.mark.parametrize("strategy", [abi.JOIN_DENSE, abi.JOIN_HASH])
