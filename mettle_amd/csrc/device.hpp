@@ -51,6 +51,22 @@ __device__ __forceinline__ uint64_t dev_count(const uint32_t* count, uint64_t ca
   return c <= cap ? c : 0;
 }
 
+// 64-bit finaliser of MurmurHash3 (fmix64): the mixing step of result
+// fingerprints.
+__host__ __device__ __forceinline__ uint64_t fp_mix(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+// Fingerprint term of output position i holding (row, value): the sum of
+// the terms over all positions (mod 2^64) depends on the order of the rows.
+__host__ __device__ __forceinline__ uint64_t fp_term(uint64_t i, uint32_t row, uint64_t v) {
+  return fp_mix((i << 32) ^ (uint64_t)row ^ 0x9e3779b97f4a7c15ull) + fp_mix(v + i);
+}
+
 // Block-wide maxima (blockDim.x <= 1024, a multiple of 64); every thread
 // of the block must call them and every thread gets the result.
 __device__ __forceinline__ uint64_t block_max_u64(uint64_t v) {

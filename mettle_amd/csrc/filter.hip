@@ -451,19 +451,30 @@ __device__ __forceinline__ uint64_t proj_load(const ExpandProj& pj, uint32_t r) 
   else if constexpr (PK == 2 || PK == 4) return ((const uint64_t*)pj.col)[r];
   else return 0;
 }
+// Returns the stored value's bits (zero-extended), or 0 without projection.
 template <int PK>
-__device__ __forceinline__ void proj_store(const ExpandProj& pj, uint64_t o, uint64_t x,
-                                           bool& ovf) {
+__device__ __forceinline__ uint64_t proj_store(const ExpandProj& pj, uint64_t o, uint64_t x,
+                                               bool& ovf) {
   if constexpr (PK == 1) {
-    ((int32_t*)pj.out)[o] = proj_int<int32_t>((int32_t)(uint32_t)x, (int32_t)pj.scalar, pj.op, ovf);
+    const int32_t y = proj_int<int32_t>((int32_t)(uint32_t)x, (int32_t)pj.scalar, pj.op, ovf);
+    ((int32_t*)pj.out)[o] = y;
+    return (uint32_t)y;
   } else if constexpr (PK == 2) {
-    ((int64_t*)pj.out)[o] = proj_int<int64_t>((int64_t)x, pj.scalar, pj.op, ovf);
+    const int64_t y = proj_int<int64_t>((int64_t)x, pj.scalar, pj.op, ovf);
+    ((int64_t*)pj.out)[o] = y;
+    return (uint64_t)y;
   } else if constexpr (PK == 3) {
-    ((float*)pj.out)[o] = proj_fp<float>(__builtin_bit_cast(float, (uint32_t)x),
-                                         (float)__longlong_as_double(pj.scalar), pj.op);
+    const float y = proj_fp<float>(__builtin_bit_cast(float, (uint32_t)x),
+                                   (float)__longlong_as_double(pj.scalar), pj.op);
+    ((float*)pj.out)[o] = y;
+    return __builtin_bit_cast(uint32_t, y);
   } else if constexpr (PK == 4) {
-    ((double*)pj.out)[o] =
+    const double y =
         proj_fp<double>(__builtin_bit_cast(double, x), __longlong_as_double(pj.scalar), pj.op);
+    ((double*)pj.out)[o] = y;
+    return __builtin_bit_cast(uint64_t, y);
+  } else {
+    return 0;
   }
 }
 
@@ -498,14 +509,16 @@ __device__ __forceinline__ uint32_t tile_offset(const uint32_t* __restrict__ cou
 // order in the same loop, four positions per thread in flight.  The tile's
 // output offset is offsets[tile] (scanned counts) or, with group_sums, is
 // computed here from the raw counts; then the last tile writes *out_count.
-template <int PK>
+// With FP the tile adds the fingerprint terms of its positions (the value
+// is the projection, or the row id without one) to its group's sum.
+template <int PK, bool FP>
 __global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ sel_bm,
                                                    const uint32_t* __restrict__ counts,
                                                    const uint32_t* __restrict__ group_sums,
                                                    const uint32_t* __restrict__ in_sel,
                                                    uint32_t* __restrict__ out,
                                                    uint32_t* __restrict__ out_count,
-                                                   ExpandProj pj) {
+                                                   ExpandProj pj, uint64_t* fp_groups) {
   __shared__ uint32_t s_ids[FTILE];
   __shared__ uint32_t s_wave[FWAVES];
   __shared__ uint32_t s_red[FWAVES];
@@ -532,6 +545,7 @@ __global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ 
   }
   __syncthreads();
   bool ovf = false;
+  uint64_t h = 0;
   uint32_t i = tid;
   for (; i + 3 * FBLOCK < total; i += 4 * FBLOCK) {
     uint32_t r[4];
@@ -545,19 +559,37 @@ __global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ 
     for (int u = 0; u < 4; ++u) x[u] = proj_load<PK>(pj, r[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      out[base + i + u * FBLOCK] = r[u];
-      proj_store<PK>(pj, base + i + u * FBLOCK, x[u], ovf);
+      const uint64_t o = (uint64_t)base + i + u * FBLOCK;
+      out[o] = r[u];
+      const uint64_t vb = proj_store<PK>(pj, o, x[u], ovf);
+      if constexpr (FP) h += fp_term(o, r[u], PK ? vb : r[u]);
     }
   }
   for (; i < total; i += FBLOCK) {
     const uint64_t k = t0 + s_ids[i];
     const uint32_t r = in_sel ? in_sel[k] : (uint32_t)k;
     const uint64_t x = proj_load<PK>(pj, r);
-    out[base + i] = r;
-    proj_store<PK>(pj, base + i, x, ovf);
+    const uint64_t o = (uint64_t)base + i;
+    out[o] = r;
+    const uint64_t vb = proj_store<PK>(pj, o, x, ovf);
+    if constexpr (FP) h += fp_term(o, r, PK ? vb : r);
   }
   if constexpr (PK == 1 || PK == 2) {
     if (__ballot(ovf) && lane == 0 && pj.ovf) atomicOr(pj.ovf, 1u);
+  }
+  if constexpr (FP) {
+    __shared__ uint64_t s_fp[FWAVES];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off, WAVE);
+    if (lane == 0) s_fp[wave] = h;
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t t = 0;
+#pragma unroll
+      for (int w = 0; w < FWAVES; ++w) t += s_fp[w];
+      if (t) atomicAdd((unsigned long long*)&fp_groups[(tile >> kGroupShift) * kFpStride],
+                       (unsigned long long)t);
+    }
   }
   if (group_sums && tile == gridDim.x - 1 && tid == 0) *out_count = base + total;
 }
@@ -670,7 +702,8 @@ static int32_t proj_kind(int32_t type, int32_t op, int* pk) {
 
 int32_t launch_compact(const uint64_t* sel_bm, uint32_t* tile_counts, uint32_t tiles,
                        const uint32_t* in_sel, uint32_t* out_sel, uint32_t* out_count,
-                       hipStream_t st, const CompactProj* proj, const uint32_t* group_sums) {
+                       hipStream_t st, const CompactProj* proj, const uint32_t* group_sums,
+                       uint64_t* fp_groups) {
   if (!group_sums) {
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SBLOCK), 0, st, tile_counts, tiles,
                        out_count);
@@ -686,15 +719,25 @@ int32_t launch_compact(const uint64_t* sel_bm, uint32_t* tile_counts, uint32_t t
     pj.ovf = proj->ovf;
     MTL_TRY(proj_kind(proj->type, proj->op, &pk));
   }
-#define MTL_EXPAND(K)                                                                           \
-  hipLaunchKernelGGL(k_expand<K>, dim3(tiles), dim3(FBLOCK), 0, st, sel_bm, tile_counts,       \
-                     group_sums, in_sel, out_sel, out_count, pj)
-  switch (pk) {
-    case 1: MTL_EXPAND(1); break;
-    case 2: MTL_EXPAND(2); break;
-    case 3: MTL_EXPAND(3); break;
-    case 4: MTL_EXPAND(4); break;
-    default: MTL_EXPAND(0); break;
+#define MTL_EXPAND(K, F)                                                                        \
+  hipLaunchKernelGGL((k_expand<K, F>), dim3(tiles), dim3(FBLOCK), 0, st, sel_bm, tile_counts,  \
+                     group_sums, in_sel, out_sel, out_count, pj, fp_groups)
+  if (fp_groups) {
+    switch (pk) {
+      case 1: MTL_EXPAND(1, true); break;
+      case 2: MTL_EXPAND(2, true); break;
+      case 3: MTL_EXPAND(3, true); break;
+      case 4: MTL_EXPAND(4, true); break;
+      default: MTL_EXPAND(0, true); break;
+    }
+  } else {
+    switch (pk) {
+      case 1: MTL_EXPAND(1, false); break;
+      case 2: MTL_EXPAND(2, false); break;
+      case 3: MTL_EXPAND(3, false); break;
+      case 4: MTL_EXPAND(4, false); break;
+      default: MTL_EXPAND(0, false); break;
+    }
   }
 #undef MTL_EXPAND
   MTL_HIP(hipGetLastError());
@@ -837,7 +880,7 @@ extern "C" {
 static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
                            const uint32_t* in_sel, const uint32_t* in_count,
                            uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
-                           const CompactProj* proj, mtl_stream* s) {
+                           const CompactProj* proj, ScanExtras* ex, mtl_stream* s) {
   if (n_preds > MTL_MAX_PREDS)
     return set_error(MTL_UNSUPPORTED, "more than %d predicates", MTL_MAX_PREDS);
   if (!out_sel || !out_count)
@@ -860,9 +903,19 @@ static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_r
   MTL_TRY(s->arena.alloc(256, &ctr));
   const uint32_t groups = (uint32_t)ceil_div(tiles, 1u << kGroupShift);
   MTL_TRY(s->arena.alloc((size_t)groups * kGroupStride * 4, &gsum));
-  ZeroList zl;
+  ZeroList zl = ex ? ex->zero : ZeroList{};
+  if (zl.n > 5) return set_error(MTL_ERR_INVALID_ARGUMENT, "too many regions to clear");
   zl.add((uint32_t*)ctr, 64);
   zl.add((uint32_t*)gsum, (uint64_t)groups * kGroupStride);
+  uint64_t* fpg = nullptr;
+  if (ex && ex->fingerprint) {
+    void* p;
+    MTL_TRY(s->arena.alloc((size_t)groups * kFpStride * 8, &p));
+    fpg = (uint64_t*)p;
+    zl.add((uint32_t*)p, (uint64_t)groups * kFpStride * 2);
+    ex->fp_groups = fpg;
+    ex->fp_n = groups;
+  }
   MTL_TRY(launch_zero(zl, s->stream));
   if (!in_sel) {
     DenseArgs a{};
@@ -890,7 +943,7 @@ static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_r
   }
   ProfScope ps(s, "k_compact");
   return launch_compact((const uint64_t*)bm, (uint32_t*)counts, tiles, in_sel, out_sel,
-                        out_count, s->stream, proj, (const uint32_t*)gsum);
+                        out_count, s->stream, proj, (const uint32_t*)gsum, fpg);
 }
 
 int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
@@ -898,7 +951,7 @@ int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
                    uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
                    mtl_stream* s) {
   return filter_impl(preds, n_preds, n_rows, in_sel, in_count, in_capacity, out_sel,
-                     out_count, nullptr, s);
+                     out_count, nullptr, nullptr, s);
 }
 
 int32_t mtl_filter_project(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
@@ -913,7 +966,7 @@ int32_t mtl_filter_project(const mtl_pred* preds, uint32_t n_preds, uint64_t n_r
     return set_error(MTL_ERR_INVALID_ARGUMENT, "proj_column/out_proj are NULL");
   CompactProj pj{proj_column, proj_type, op, scalar, out_proj, overflow_flag};
   return filter_impl(preds, n_preds, n_rows, in_sel, in_count, in_capacity, out_sel,
-                     out_count, &pj, s);
+                     out_count, &pj, nullptr, s);
 }
 
 int32_t mtl_compare(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
@@ -997,3 +1050,20 @@ int32_t mtl_project(const void* column, int32_t type, const uint32_t* sel,
 }
 
 }  // extern "C"
+
+namespace mtl {
+
+int32_t filter_project_ex(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                          const CompactProj* proj, uint32_t* out_sel, uint32_t* out_count,
+                          ScanExtras* ex, mtl_stream* s) {
+  if (proj) {
+    int pk = 0;
+    MTL_TRY(proj_kind(proj->type, proj->op, &pk));
+    if (!proj->col || !proj->out)
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "proj_column/out_proj are NULL");
+  }
+  return filter_impl(preds, n_preds, n_rows, nullptr, nullptr, 0, out_sel, out_count, proj, ex,
+                     s);
+}
+
+}  // namespace mtl

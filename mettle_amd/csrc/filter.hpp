@@ -97,6 +97,10 @@ struct CompactProj {
   void* out;
   uint32_t* ovf;  // device flag, set to 1 on integer overflow
 };
+// Per-group fingerprint sums of a compaction's output (see fp_term), one
+// uint64 per 256-tile group, kFpStride uint64 apart (a cache line each).
+constexpr int kFpStride = 32;
+
 // Expansion of sel_bm into ascending row ids (in_sel[k] when in_sel is
 // given); *out_count = total.  Each tile's output offset comes from an
 // exclusive scan of tile_counts (in place) or, when group_sums holds the
@@ -105,6 +109,21 @@ struct CompactProj {
 int32_t launch_compact(const uint64_t* sel_bm, uint32_t* tile_counts, uint32_t tiles,
                        const uint32_t* in_sel, uint32_t* out_sel, uint32_t* out_count,
                        hipStream_t st, const CompactProj* proj = nullptr,
-                       const uint32_t* group_sums = nullptr);
+                       const uint32_t* group_sums = nullptr, uint64_t* fp_groups = nullptr);
+
+// Internal hooks of the serving loop on a filter (+ projection): extra
+// regions cleared by the filter's own clearing launch, and the output
+// fingerprint accumulated by the expansion (fp_groups / fp_n are outputs:
+// the per-group sums, cleared and filled on the stream; the caller adds
+// them up).
+struct ScanExtras {
+  ZeroList zero;
+  bool fingerprint = false;
+  uint64_t* fp_groups = nullptr;
+  uint32_t fp_n = 0;
+};
+int32_t filter_project_ex(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                          const CompactProj* proj, uint32_t* out_sel, uint32_t* out_count,
+                          ScanExtras* ex, mtl_stream* s);
 
 }  // namespace mtl

@@ -29,50 +29,12 @@
 namespace mtl {
 using namespace dev;
 
-__device__ __forceinline__ uint64_t fp_mix(uint64_t x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdull;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ull;
-  x ^= x >> 33;
-  return x;
-}
-
-constexpr int kFpBlock = 256;
-constexpr int kFpMaxBlocks = 1024;
-
-// partial[b] = sum over this block's positions i of mix(i, row_i, value_i)
-// (mod 2^64); position-dependent, so any reordering changes it.
-__global__ __launch_bounds__(kFpBlock) void k_fingerprint(const uint32_t* __restrict__ sel,
-                                                          const uint32_t* count,
-                                                          uint64_t cap, const void* vals,
-                                                          uint32_t width, uint64_t* partial) {
-  __shared__ uint64_t s_part[kFpBlock / WAVE];
-  const uint64_t n = dev_count(count, cap);
-  uint64_t h = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * kFpBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kFpBlock) {
-    uint64_t v = 0;
-    if (width == 8) v = ((const uint64_t*)vals)[i];
-    else if (width == 4) v = ((const uint32_t*)vals)[i];
-    else if (width == 1) v = ((const uint8_t*)vals)[i];
-    h += fp_mix((i << 32) ^ (uint64_t)sel[i] ^ 0x9e3779b97f4a7c15ull) + fp_mix(v + i);
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off, WAVE);
-  if (threadIdx.x % WAVE == 0) s_part[threadIdx.x / WAVE] = h;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    for (int w = 0; w < kFpBlock / WAVE; ++w) t += s_part[w];
-    partial[blockIdx.x] = t;
-  }
-}
-
-__global__ void k_fingerprint_sum(const uint64_t* partial, uint32_t n, uint64_t* out) {
+// Sum of the per-group fingerprint sums (stride apart) into *out.
+__global__ void k_fingerprint_sum(const uint64_t* partial, uint32_t n, uint32_t stride,
+                                  uint64_t* out) {
   __shared__ uint64_t s_part[16];
   uint64_t t = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) t += partial[i];
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) t += partial[(uint64_t)i * stride];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, WAVE);
   if (threadIdx.x % WAVE == 0) s_part[threadIdx.x / WAVE] = t;
@@ -94,7 +56,10 @@ uint64_t query_scratch(const mtl_query& q) {
   if (q.kind == MTL_Q_SCAN) {
     const uint64_t tiles = ceil_div(q.n_rows ? q.n_rows : 1, kTileRows);
     const uint64_t pw = q.proj_col ? (uint64_t)std::max(type_width(q.proj_type), 4) : 0;
-    return q.n_rows * (4 + pw) + tiles * (kTileRows / 8 + 4) + (1u << 16);
+    const uint64_t groups = ceil_div(tiles, 1u << kGroupShift);
+    // selection + projection, pass bits and tile counts, group sums and
+    // fingerprint sums (a cache line each per group), clearing words
+    return q.n_rows * (4 + pw) + tiles * (kTileRows / 8 + 4) + groups * 512 + (1u << 16);
   }
   const mtl_star_desc& d = q.star;
   const uint64_t pspan = key_off(d.p_key_max, d.p_key_min) + 1;
@@ -117,9 +82,8 @@ struct Slot {
   uint32_t* sel = nullptr;
   void* proj = nullptr;
   uint64_t cap = 0;  // rows the buffers hold
-  uint32_t* words = nullptr;    // [0] count, [1] overflow flag
-  uint64_t* partial = nullptr;  // kFpMaxBlocks + 1 (last: fingerprint)
-  void* host = nullptr;         // pinned: count, overflow, fingerprint
+  uint32_t* words = nullptr;  // [0] count, [1] overflow flag, [2..3] fingerprint
+  void* host = nullptr;       // pinned copy of words[0..3]
 };
 
 static int32_t slot_reserve(Slot& sl, uint64_t rows) {
@@ -137,40 +101,27 @@ static int32_t slot_reserve(Slot& sl, uint64_t rows) {
   return MTL_OK;
 }
 
+// One clearing launch (the filter's, with the slot's words added), the
+// filter pass, the expansion with the projection and the fingerprint fused,
+// the fingerprint sum, and one 16-byte copy of (count, overflow, fingerprint).
 static int32_t run_scan(Slot& sl, const mtl_query& q, mtl_query_result& r) {
   MTL_TRY(slot_reserve(sl, q.n_rows));
   hipStream_t st = sl.s->stream;
-  ZeroList zl;
-  zl.add(sl.words, 2);
-  MTL_TRY(launch_zero(zl, st));
-  const void* vals = sl.sel;
-  uint32_t width = 4;
-  if (q.proj_col) {  // projection fused into the compaction
-    MTL_TRY(mtl_filter_project(q.preds, q.n_preds, q.n_rows, nullptr, nullptr, 0, q.proj_col,
-                               q.proj_type, q.proj_op, q.proj_scalar, sl.sel, sl.words, sl.proj,
-                               sl.words + 1, sl.s));
-    vals = sl.proj;
-    width = (uint32_t)type_width(q.proj_type);
-  } else {
-    MTL_TRY(mtl_filter(q.preds, q.n_preds, q.n_rows, nullptr, nullptr, 0, sl.sel, sl.words,
-                       sl.s));
-  }
-  const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>(ceil_div(q.n_rows ? q.n_rows : 1, kFpBlock * 16), kFpMaxBlocks));
-  hipLaunchKernelGGL(k_fingerprint, dim3(blocks), dim3(kFpBlock), 0, st, sl.sel, sl.words,
-                     (uint64_t)q.n_rows, vals,
-                     width, sl.partial);
-  hipLaunchKernelGGL(k_fingerprint_sum, dim3(1), dim3(1024), 0, st, sl.partial, blocks,
-                     sl.partial + kFpMaxBlocks);
+  ScanExtras ex;
+  ex.zero.add(sl.words, 4);
+  ex.fingerprint = true;
+  CompactProj pj{q.proj_col, q.proj_type, q.proj_op, q.proj_scalar, sl.proj, sl.words + 1};
+  MTL_TRY(filter_project_ex(q.preds, q.n_preds, q.n_rows, q.proj_col ? &pj : nullptr, sl.sel,
+                            sl.words, &ex, sl.s));
+  hipLaunchKernelGGL(k_fingerprint_sum, dim3(1), dim3(1024), 0, st, ex.fp_groups, ex.fp_n,
+                     (uint32_t)kFpStride, (uint64_t*)(sl.words + 2));
   MTL_HIP(hipGetLastError());
-  char* h = (char*)sl.host;
-  MTL_HIP(hipMemcpyAsync(h, sl.words, 8, hipMemcpyDeviceToHost, st));
-  MTL_HIP(hipMemcpyAsync(h + 8, sl.partial + kFpMaxBlocks, 8, hipMemcpyDeviceToHost, st));
+  MTL_HIP(hipMemcpyAsync(sl.host, sl.words, 16, hipMemcpyDeviceToHost, st));
   MTL_HIP(hipStreamSynchronize(st));
-  const uint32_t* hw = (const uint32_t*)h;
+  const uint32_t* hw = (const uint32_t*)sl.host;
   if (hw[0] > q.n_rows) return set_error(MTL_ERR_PROCESSING, "device scan failed");
   r.count = hw[0];
-  r.fingerprint = *(const uint64_t*)(h + 8);
+  r.fingerprint = *(const uint64_t*)(hw + 2);
   if (hw[1]) return set_error(MTL_ERR_OVERFLOW, "Overflow in projection");
   return MTL_OK;
 }
@@ -266,9 +217,6 @@ int32_t mtl_server_create(uint32_t n_slots, uint64_t scratch_budget, mtl_server*
     int32_t st = mtl_stream_create(&sl.s);
     if (st == MTL_OK && hipMalloc((void**)&sl.words, 64) != hipSuccess)
       st = set_error(MTL_ERR_PROCESSING, "processing error: hipMalloc");
-    if (st == MTL_OK &&
-        hipMalloc((void**)&sl.partial, (kFpMaxBlocks + 1) * 8) != hipSuccess)
-      st = set_error(MTL_ERR_PROCESSING, "processing error: hipMalloc");
     if (st == MTL_OK && hipHostMalloc(&sl.host, 64, hipHostMallocDefault) != hipSuccess)
       st = set_error(MTL_ERR_PROCESSING, "processing error: hipHostMalloc");
     if (st != MTL_OK) {
@@ -343,7 +291,6 @@ int32_t mtl_server_destroy(mtl_server* srv) {
     if (sl.sel) hipFree(sl.sel);
     if (sl.proj) hipFree(sl.proj);
     if (sl.words) hipFree(sl.words);
-    if (sl.partial) hipFree(sl.partial);
     if (sl.host) hipHostFree(sl.host);
     if (sl.s) mtl_stream_destroy(sl.s);
   }

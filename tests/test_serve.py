@@ -53,7 +53,11 @@ def _scan_workload(ops, rng, n):
                         (db, abi.BIGINT, abi.PROJ_ADD, 7))
     sel2 = orc.filter([(b, abi.BIGINT, abi.CMP_LT, 0), (v, abi.INTEGER, abi.CMP_LE, cut)], n)
     exp2 = (len(sel2), host_fingerprint(sel2, (b[sel2] + 7).view(np.uint64)))
-    return [(q1, exp1), (q2, exp2)], (dv, db)
+    # no projection: the value of each position is its row id
+    q3 = ops.scan_query([(db, abi.BIGINT, abi.CMP_GE, -2**39)], n)
+    sel3 = orc.filter([(b, abi.BIGINT, abi.CMP_GE, -2**39)], n)
+    exp3 = (len(sel3), host_fingerprint(sel3, sel3))
+    return [(q1, exp1), (q2, exp2), (q3, exp3)], (dv, db)
 
 
 def test_scan_results_match_oracle_alone_and_in_mix(ops):
