@@ -108,23 +108,24 @@ __device__ __forceinline__ void pred_sel(const PredC& p, const uint32_t (&rid)[F
     }
 }
 
-// pred_sel for dense rows k0 + j*256 + v (every row with a set bit is < n).
+// pred_sel for dense rows k0 + j*256 + v (every row with a set bit is < n),
+// one step of four rows at a time: a later predicate of a scan that has
+// already dropped most rows, kept small in registers.
 __device__ __forceinline__ void pred_rows(const PredC& p, uint64_t k0, uint32_t& bits) {
   const uint32_t width = pred_width(p.kind);
-  uint64_t x[FJ][4];
 #pragma unroll
-  for (int j = 0; j < FJ; ++j)
+  for (int j = 0; j < FJ; ++j) {
+    const uint32_t m = (bits >> (4 * j)) & 0xFu;
+    if (!m) continue;
+    uint64_t x[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      x[j][v] = (bits >> (4 * j + v)) & 1u ? load_raw(p.col, width, k0 + (uint64_t)j * 256 + v)
-                                            : 0ull;
+      x[v] = (m >> v) & 1u ? load_raw(p.col, width, k0 + (uint64_t)j * 256 + v) : 0ull;
 #pragma unroll
-  for (int j = 0; j < FJ; ++j)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      uint32_t b = 1u << (4 * j + v);
-      if ((bits & b) && !pred_pass(enc_raw(x[j][v], p.kind), p)) bits &= ~b;
-    }
+    for (int v = 0; v < 4; ++v)
+      if (((m >> v) & 1u) && !pred_pass(enc_raw(x[v], p.kind), p))
+        bits &= ~(1u << (4 * j + v));
+  }
 }
 
 // Integer keys of the lane's 16 rows: 16-byte vector loads when the rows

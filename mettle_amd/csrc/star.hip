@@ -89,8 +89,12 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t c, uint32_t* s_red) {
   return t;
 }
 
+// Blocks past the rank tiles (tickets >= rtiles) only add popcounts: the
+// grid covers the larger of the parent bitmap and the dimension bitmaps, so
+// a small parent table with a large dimension table is not left to a few
+// blocks.
 __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm,
-                                                 uint64_t words,
+                                                 uint64_t words, uint32_t rtiles,
                                                  uint32_t* __restrict__ rank,
                                                  uint64_t* status, uint32_t* ctr,
                                                  uint32_t* err, uint32_t* par_total,
@@ -102,43 +106,46 @@ __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm
   if (tid == 0) s_tile = atomicAdd(ctr, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
+  const bool ranks = tile < rtiles;  // block-uniform
   const uint64_t w0 = (uint64_t)tile * RTILE + (uint64_t)tid * RPER;
   uint64_t x[RPER];
 #pragma unroll
-  for (int i = 0; i < RPER; ++i) x[i] = (w0 + i < words) ? bm[w0 + i] : 0ull;
+  for (int i = 0; i < RPER; ++i) x[i] = (ranks && w0 + i < words) ? bm[w0 + i] : 0ull;
   // the duplicate-key popcounts are independent of the prefix: their loads
   // go out with the bitmap words and finish while predecessors publish
   const uint32_t p0 = thread_popcount(pa.pbits, pa.pbits_words);
   const uint32_t p1 = pa.dim_bm ? thread_popcount(pa.dim_bm, pa.dim_words) : 0u;
   const uint32_t p2 = pa.dbits ? thread_popcount(pa.dbits, pa.dbits_words) : 0u;
-  uint32_t c[RPER];
-  uint32_t sum = 0;
+  if (ranks) {
+    uint32_t c[RPER];
+    uint32_t sum = 0;
 #pragma unroll
-  for (int i = 0; i < RPER; ++i) {
-    c[i] = (uint32_t)__popcll(x[i]);
-    sum += c[i];
-  }
-  uint32_t incl = wave_inclusive_u32(sum);
-  if (lane == WAVE - 1) s_wave[wave] = incl;
-  __syncthreads();
-  uint32_t woff = 0, total = 0;
+    for (int i = 0; i < RPER; ++i) {
+      c[i] = (uint32_t)__popcll(x[i]);
+      sum += c[i];
+    }
+    uint32_t incl = wave_inclusive_u32(sum);
+    if (lane == WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t woff = 0, total = 0;
 #pragma unroll
-  for (int w = 0; w < RBLOCK / WAVE; ++w) {
-    if (w < wave) woff += s_wave[w];
-    total += s_wave[w];
-  }
-  if (wave == 0) {
-    uint32_t ex = lookback_wave(status, tile, total, err);
-    if (lane == 0) s_prefix = ex;
-  }
-  __syncthreads();
-  uint32_t run = s_prefix + woff + incl - sum;
+    for (int w = 0; w < RBLOCK / WAVE; ++w) {
+      if (w < wave) woff += s_wave[w];
+      total += s_wave[w];
+    }
+    if (wave == 0) {
+      uint32_t ex = lookback_wave(status, tile, total, err);
+      if (lane == 0) s_prefix = ex;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + woff + incl - sum;
 #pragma unroll
-  for (int i = 0; i < RPER; ++i) {
-    if (w0 + i < words) rank[w0 + i] = run;
-    run += c[i];
+    for (int i = 0; i < RPER; ++i) {
+      if (w0 + i < words) rank[w0 + i] = run;
+      run += c[i];
+    }
+    if (tile == rtiles - 1 && tid == 0) *par_total = s_prefix + total;
   }
-  if (tile == gridDim.x - 1 && tid == 0) *par_total = s_prefix + total;
   const uint32_t c0 = block_sum(p0, s_red);
   const uint32_t c1 = block_sum(p1, s_red);
   const uint32_t c2 = block_sum(p2, s_red);
@@ -347,25 +354,24 @@ __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32
   }
 }
 
+// PW: byte width of the first predicate's column (0 without predicates), so
+// its values are held at their own width; bitmap words are read as 32-bit
+// words.  Both keep the kernel under 64 VGPRs (8 waves per SIMD).
+template <int PW, typename KT>
 __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
   const uint64_t k0 = (uint64_t)blockIdx.x * FTILE + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
   // first predicate's column, loaded with the keys
-  uint64_t px[FJ][4];
-  if (a.n_preds) {
-    const uint32_t width = pred_width(a.preds[0].kind);
+  using PT = typename std::conditional<
+      PW == 8, uint64_t, typename std::conditional<PW == 1, uint8_t, uint32_t>::type>::type;
+  PT px[FJ][4];
+  if constexpr (PW != 0) {
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
-      load_raw4(a.preds[0].col, width, k0 + (uint64_t)j * 256, a.n, a.pred0_vec != 0, px[j]);
+      load4<PT>((const PT*)a.preds[0].col, k0 + (uint64_t)j * 256, a.n, a.pred0_vec != 0, px[j]);
   }
   uint32_t s[FJ][4];
-  switch (a.fk_type) {
-    case MTL_BIGINT:
-    case MTL_TIMESTAMP:
-    case MTL_DECIMAL: fact_keys<int64_t>(a, k0, s); break;
-    case MTL_DICT8: fact_keys<uint8_t>(a, k0, s); break;
-    default: fact_keys<int32_t>(a, k0, s); break;
-  }
+  fact_keys<KT>(a, k0, s);
   // rows in range and, when there is a predicate, passing it (its column
   // arrived with the keys): only these probe the parent bitmap
   uint32_t cand = 0;
@@ -373,22 +379,26 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      if (k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu &&
-          (!a.n_preds || pred_pass(enc_raw(px[j][v], a.preds[0].kind), a.preds[0])))
+      if (k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu) {
+        if constexpr (PW != 0) {
+          if (!pred_pass(enc_raw((uint64_t)px[j][v], a.preds[0].kind), a.preds[0])) continue;
+        }
         cand |= 1u << (4 * j + v);
+      }
   uint32_t hits = 0;
   {
-    uint64_t w[FJ][4];
+    const uint32_t* bm32 = (const uint32_t*)a.bm;
+    uint32_t w[FJ][4];
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(a.bm, (s[j][v] >> 6) << 3) : 0ull;
+        w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(bm32, (s[j][v] >> 5) << 2) : 0u;
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (s[j][v] & 63)) & 1ull))
+        if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (s[j][v] & 31)) & 1u))
           hits |= 1u << (4 * j + v);
   }
   for (uint32_t p = 1; p < a.n_preds && hits; ++p) pred_rows(a.preds[p], k0, hits);
@@ -733,6 +743,10 @@ static int32_t star_validate(const mtl_star_desc* d, StarPlan* sp) {
   uint64_t g = ceil_div(d->p_rows ? d->p_rows : 1, nt * 8);
   sp->topk_grid = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>(g, (d->k <= 128 ? 1 : 3) * (uint64_t)cu_count()));
+  // when one merge block can take every phase-1 list while keeping at least
+  // half the CUs busy in phase 1, a single merge launch suffices
+  const uint32_t fan = topk_fan(d->k);
+  if (sp->topk_grid > fan && 2 * fan >= (uint32_t)cu_count()) sp->topk_grid = fan;
   return MTL_OK;
 }
 
@@ -841,8 +855,12 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
       pa.dbits_words = (uint64_t)filter_tiles(d->d_rows) * kTileRows / 64;
     }
     pa.pops = mu + 7;  // [7] P pass bits, [8] D bitmap, [9] D pass bits
-    hipLaunchKernelGGL(k_rank, dim3(std::max<uint32_t>(rtiles, 1)), dim3(RBLOCK), 0, st,
-                       (const uint64_t*)par_bm, sp.pwords, (uint32_t*)par_rank,
+    const uint32_t rt = std::max<uint32_t>(rtiles, 1);
+    const uint64_t pop_words = std::max(pa.pbits_words, std::max(pa.dim_words, pa.dbits_words));
+    const uint32_t pop_blocks =
+        (uint32_t)std::min<uint64_t>(ceil_div(pop_words, (uint64_t)RBLOCK * 8), 2048);
+    hipLaunchKernelGGL(k_rank, dim3(std::max(rt, pop_blocks)), dim3(RBLOCK), 0, st,
+                       (const uint64_t*)par_bm, sp.pwords, rt, (uint32_t*)par_rank,
                        (uint64_t*)st_r, mu + 2, mu + 3, mu + 6, pa);
   }
   MTL_HIP(hipGetLastError());
@@ -898,7 +916,20 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.n_regions = (uint64_t)ftiles * FWAVES;
     {
       ProfScope ps(s, "k_fact_scan");
-      hipLaunchKernelGGL(k_fact_scan, dim3(ftiles), dim3(FBLOCK), 0, st, fa);
+      const uint32_t pw = d->f_n_preds ? pred_width(fa.preds[0].kind) : 0u;
+      const int kw = d->f_fk_type == MTL_DICT8 ? 1 : type_width(d->f_fk_type) == 8 ? 8 : 4;
+#define MTL_FACT_SCAN(P, K) \
+  hipLaunchKernelGGL((k_fact_scan<P, K>), dim3(ftiles), dim3(FBLOCK), 0, st, fa)
+#define MTL_FACT_SCAN_K(P)                  \
+  if (kw == 4) MTL_FACT_SCAN(P, int32_t);   \
+  else if (kw == 8) MTL_FACT_SCAN(P, int64_t); \
+  else MTL_FACT_SCAN(P, uint8_t)
+      if (pw == 0) { MTL_FACT_SCAN_K(0); }
+      else if (pw == 1) { MTL_FACT_SCAN_K(1); }
+      else if (pw == 4) { MTL_FACT_SCAN_K(4); }
+      else { MTL_FACT_SCAN_K(8); }
+#undef MTL_FACT_SCAN_K
+#undef MTL_FACT_SCAN
     }
     MTL_HIP(hipGetLastError());
     ProfScope ps(s, "k_fact_acc");
