@@ -17,10 +17,14 @@
 //     (Zipf heads collapse there) and flushes it to the global table once;
 //   - one pass updates all aggregates of a row (state words of a group are
 //     contiguous, one cache line for typical aggregate lists);
-//   - integer SUM is exact 128-bit with non-returning atomics only: the
-//     low 32 bits of each value are summed in one 64-bit word and the high
-//     32 bits (signed) in another; sum = high * 2^32 + low, exact for fewer
-//     than 2^32 rows in any order; FLOAT/DOUBLE sums accumulate in double;
+//   - integer SUM is exact 128-bit with non-returning atomics only: each
+//     value is split as v = lo + hi * 2^32 with lo its sign-extended low
+//     half, the lo parts are summed in one signed 64-bit word and the hi
+//     parts in another; sum = high * 2^32 + low, exact for fewer than 2^32
+//     rows in any order, and values within int32 (hi = 0) skip the high
+//     add; FLOAT/DOUBLE sums accumulate in double;
+//   - in LDS, MIN/MAX read the partial state first and issue the atomic
+//     only when the row improves it (rarely, after a group's first rows);
 //   - each thread loads the keys and values of several rows before its
 //     first update, so the loads overlap instead of waiting behind atomics.
 #include <algorithm>
@@ -49,7 +53,12 @@ struct AggSlot {
   int32_t type;
   uint32_t dup;  // shares the state words of an earlier slot (same kind and column)
   const void* col;
+  uint32_t src;  // value source (see GroupArgs::src_*)
 };
+
+// How a value source reads a column: int64, double bits, or the
+// order-preserving encoding of the double (float MIN/MAX).
+enum SrcConv : uint32_t { SC_INT = 0, SC_DBL, SC_ENCF };
 
 struct GroupArgs {
   uint64_t n_rows;
@@ -65,6 +74,12 @@ struct GroupArgs {
   uint64_t key_stride[MTL_GROUP_MAX_KEYS];
   uint32_t n_aggs;
   AggSlot aggs[MTL_GROUP_MAX_AGGS];
+  // distinct (column, conversion) value sources of the aggregates: each is
+  // loaded once per row (SUM, MIN and MAX of one column share one load)
+  uint32_t n_src;
+  const void* src_col[MTL_GROUP_MAX_AGGS];
+  int32_t src_type[MTL_GROUP_MAX_AGGS];
+  uint32_t src_conv[MTL_GROUP_MAX_AGGS];
   uint32_t W;
   uint32_t dense;
   uint64_t cap;     // dense: domain size; hash: power-of-two capacity
@@ -108,67 +123,102 @@ __device__ __forceinline__ double gload_f64(const void* col, int32_t type, uint6
   }
 }
 
-// The exact 128-bit integer sum held as (sum of low 32-bit halves, sum of
-// high signed halves): value = hi32sum * 2^32 + lo32sum.
+// The exact 128-bit integer sum held as two signed 64-bit sums X (of the
+// sign-extended low halves) and Y (of the high parts): value = X + Y * 2^32,
+// returned as (low 64 bits, high 64 bits) of the 128-bit value.
 __device__ __forceinline__ void sum128(const uint64_t* w, uint64_t& lo, int64_t& hi) {
-  const uint64_t L = w[0];
-  const int64_t H = (int64_t)w[1];
-  lo = ((uint64_t)H << 32) + L;
-  hi = (H >> 32) + (lo < L ? 1 : 0);
+  const int64_t X = (int64_t)w[0], Y = (int64_t)w[1];
+  const uint64_t yl = (uint64_t)Y << 32;
+  lo = yl + (uint64_t)X;
+  hi = (Y >> 32) + (X < 0 ? -1 : 0) + (lo < yl ? 1 : 0);
 }
 
-// The value a slot accumulates for one row: int64 bits for integer kinds,
-// double bits for K_SUMF, the order-preserving encoding for K_MINF/K_MAXF.
-__device__ __forceinline__ uint64_t agg_raw(const AggSlot& s, uint64_t row) {
-  switch (s.kind) {
-    case K_SUMI:
-    case K_MINI:
-    case K_MAXI: return (uint64_t)load_key(s.col, s.type, row);
-    case K_SUMF: return (uint64_t)__double_as_longlong(gload_f64(s.col, s.type, row));
-    case K_MINF:
-    case K_MAXF: return enc_f64(gload_f64(s.col, s.type, row));
-    default: return 0;
+// v = lo + hi * 2^32 with lo = (int32)v: |lo|, |hi| <= 2^31.
+__device__ __forceinline__ void split_i64(int64_t v, int64_t& lo, int64_t& hi) {
+  lo = (int64_t)(int32_t)(uint32_t)(uint64_t)v;
+  hi = (v >> 32) + (lo < 0 ? 1 : 0);
+}
+
+// The value a source gives for one row: int64 bits (integer aggregates),
+// double bits (K_SUMF), or the order-preserving encoding (K_MINF/K_MAXF).
+__device__ __forceinline__ uint64_t src_raw(const GroupArgs& a, uint32_t j, uint64_t row) {
+  switch (a.src_conv[j]) {
+    case SC_INT: return (uint64_t)load_key(a.src_col[j], a.src_type[j], row);
+    case SC_DBL: return (uint64_t)__double_as_longlong(gload_f64(a.src_col[j], a.src_type[j], row));
+    default: return enc_f64(gload_f64(a.src_col[j], a.src_type[j], row));
   }
 }
 
-// Per-row update of one group's state words (global or LDS: atomics with
-// the address space of p; none returns a value).
-__device__ __forceinline__ void update_vals(const GroupArgs& a, uint64_t* p,
-                                            const uint64_t (&v)[MTL_GROUP_MAX_AGGS]) {
+// v[s] for a wave-uniform source index s, without indexing registers
+// dynamically.
+template <int NS>
+__device__ __forceinline__ uint64_t pick_src(const uint64_t (&v)[NS], uint32_t s) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) r |= v[j] & (0ull - (uint64_t)(s == (uint32_t)j));
+  return r;
+}
+
+// Per-row update of one group's state words (LDS or global: atomics with
+// the address space of p; none returns a value).  Word k of the group is
+// at p[k * ws]: global states are contiguous (ws = 1), LDS states are
+// stored word-major (ws = the cache size), so the lanes of a wave that
+// update random groups spread over all LDS banks.  In LDS a MIN/MAX word is
+// read first and the atomic issued only if the row improves it: the read
+// may be stale, which costs at most an unneeded (monotone) atomic.
+template <bool LDS, int NS>
+__device__ __forceinline__ void update_vals(const GroupArgs& a, uint64_t* p, uint32_t ws,
+                                            const uint64_t (&vs)[NS]) {
   atomicAdd((unsigned long long*)&p[0], 1ull);
 #pragma unroll
   for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i) {
     if (i >= a.n_aggs) break;
     const AggSlot& s = a.aggs[i];
-    if (s.dup) continue;
-    uint64_t* w = p + s.word;
+    if (s.dup || s.kind == K_COUNT) continue;
+    uint64_t* w = p + (uint64_t)s.word * ws;
+    const uint64_t v = pick_src<NS>(vs, s.src);
     switch (s.kind) {
-      case K_SUMI:
-        atomicAdd((unsigned long long*)w, (unsigned long long)(uint32_t)v[i]);
-        atomicAdd((unsigned long long*)(w + 1), (unsigned long long)((int64_t)v[i] >> 32));
+      case K_SUMI: {
+        int64_t lo, hi;
+        split_i64((int64_t)v, lo, hi);
+        atomicAdd((unsigned long long*)w, (unsigned long long)lo);
+        if (hi) atomicAdd((unsigned long long*)(w + ws), (unsigned long long)hi);
         break;
-      case K_SUMF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)v[i])); break;
-      case K_MINI: atomicMin((long long*)w, (long long)v[i]); break;
-      case K_MAXI: atomicMax((long long*)w, (long long)v[i]); break;
-      case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)v[i]); break;
-      case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)v[i]); break;
+      }
+      case K_SUMF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)v)); break;
+      case K_MINI:
+        if (!LDS || (long long)v < *(volatile long long*)w) atomicMin((long long*)w, (long long)v);
+        break;
+      case K_MAXI:
+        if (!LDS || (long long)v > *(volatile long long*)w) atomicMax((long long*)w, (long long)v);
+        break;
+      case K_MINF:
+        if (!LDS || v < *(volatile uint64_t*)w)
+          atomicMin((unsigned long long*)w, (unsigned long long)v);
+        break;
+      case K_MAXF:
+        if (!LDS || v > *(volatile uint64_t*)w)
+          atomicMax((unsigned long long*)w, (unsigned long long)v);
+        break;
       default: break;
     }
   }
 }
 
-// Merge a partial state (from an LDS cache slot) into a global group.
-__device__ __forceinline__ void merge_state(const GroupArgs& a, uint64_t* g, const uint64_t* p) {
+// Merge a partial state (an LDS cache slot, word k at p[k * ws]) into a
+// global group.
+__device__ __forceinline__ void merge_state(const GroupArgs& a, uint64_t* g, const uint64_t* p,
+                                            uint32_t ws) {
   atomicAdd((unsigned long long*)&g[0], (unsigned long long)p[0]);
   for (uint32_t i = 0; i < a.n_aggs; ++i) {
     const AggSlot& s = a.aggs[i];
     if (s.dup) continue;
     uint64_t* w = g + s.word;
-    const uint64_t* v = p + s.word;
+    const uint64_t* v = p + (uint64_t)s.word * ws;
     switch (s.kind) {
       case K_SUMI:
         atomicAdd((unsigned long long*)w, (unsigned long long)v[0]);
-        atomicAdd((unsigned long long*)(w + 1), (unsigned long long)v[1]);
+        if (v[ws]) atomicAdd((unsigned long long*)(w + 1), (unsigned long long)v[ws]);
         break;
       case K_SUMF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)v[0])); break;
       case K_MINI: atomicMin((long long*)w, (long long)v[0]); break;
@@ -206,9 +256,14 @@ __global__ void k_group_init(GroupArgs a, uint64_t n_groups) {
   }
 }
 
-constexpr int kAggBlock = 512;
+// 12 waves per block: two blocks (and their LDS caches) per CU at 6 waves
+// per SIMD, within the SGPR-bound limit of 7.
+constexpr int kAggBlock = 768;
 constexpr int GE = 4;  // rows per thread per round, loaded together
 
+// NS: compile-time bound on the number of value sources (registers for the
+// values of GE rows are sized by it, not by the aggregate count).
+template <int NS>
 __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
   extern __shared__ uint64_t smem[];
   const uint32_t cs = a.cache_slots;
@@ -216,7 +271,7 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
   uint64_t* cst = smem + cs;
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
     ckeys[i] = a.cache_direct ? (uint64_t)i : GEMPTY;
-    for (uint32_t w = 0; w < a.W; ++w) cst[i * a.W + w] = a.init[w];
+    for (uint32_t w = 0; w < a.W; ++w) cst[(uint64_t)w * cs + i] = a.init[w];
   }
   __syncthreads();
   const uint64_t n = a.sel ? dev_count(a.sel_count, a.sel_cap) : a.n_rows;
@@ -232,7 +287,14 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
       ok[e] = k < end;
       row[e] = ok[e] ? (a.sel ? (uint64_t)a.sel[k] : k) : 0;
     }
-    uint64_t v[GE][MTL_GROUP_MAX_AGGS];
+    // the values do not wait for the key range checks: keys and values of
+    // all GE rows are in flight together (one round trip, not two)
+    uint64_t v[GE][NS];
+#pragma unroll
+    for (int e = 0; e < GE; ++e)
+#pragma unroll
+      for (uint32_t j = 0; j < (uint32_t)NS; ++j)
+        v[e][j] = (ok[e] && j < a.n_src) ? src_raw(a, j, row[e]) : 0ull;
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
       key[e] = 0;
@@ -240,9 +302,6 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
         atomicOr(a.err, 8u);
         ok[e] = false;
       }
-#pragma unroll
-      for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i)
-        v[e][i] = (ok[e] && i < a.n_aggs && !a.aggs[i].dup) ? agg_raw(a.aggs[i], row[e]) : 0ull;
     }
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
@@ -272,19 +331,19 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
         }
       }
       if (slot >= 0) {
-        update_vals(a, cst + (uint64_t)slot * a.W, v[e]);
+        update_vals<true, NS>(a, cst + slot, cs, v[e]);
       } else {
         uint64_t g = global_slot(a, key[e]);
-        if (g != GEMPTY) update_vals(a, a.st + g * a.W, v[e]);
+        if (g != GEMPTY) update_vals<false, NS>(a, a.st + g * a.W, 1, v[e]);
       }
     }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
     uint64_t key = ckeys[i];
-    if (key == GEMPTY || cst[i * a.W] == 0) continue;
+    if (key == GEMPTY || cst[i] == 0) continue;  // word 0: the row count
     uint64_t g = global_slot(a, key);
-    if (g != GEMPTY) merge_state(a, a.st + g * a.W, cst + (uint64_t)i * a.W);
+    if (g != GEMPTY) merge_state(a, a.st + g * a.W, cst + i, cs);
   }
 }
 
@@ -322,6 +381,12 @@ struct EmitArgs {
   uint64_t cap;
   uint32_t n_aggs;
   AggSlot aggs[MTL_GROUP_MAX_AGGS];
+  // distinct (column, conversion) value sources of the aggregates: each is
+  // loaded once per row (SUM, MIN and MAX of one column share one load)
+  uint32_t n_src;
+  const void* src_col[MTL_GROUP_MAX_AGGS];
+  int32_t src_type[MTL_GROUP_MAX_AGGS];
+  uint32_t src_conv[MTL_GROUP_MAX_AGGS];
   int64_t* out_keys;   // n x n_keys
   uint64_t* out_cnt;   // n
   uint64_t* out_aggs;  // n x (2 words per agg)
@@ -569,6 +634,27 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
   }
   a->n_aggs = d->n_aggs;
   a->W = w;
+  // value sources: one per distinct (column, type, conversion)
+  a->n_src = 0;
+  for (uint32_t i = 0; i < d->n_aggs; ++i) {
+    AggSlot& sl = a->aggs[i];
+    sl.src = 0;
+    if (sl.kind == K_COUNT) continue;
+    const uint32_t conv = sl.kind == K_SUMF                            ? SC_DBL
+                          : (sl.kind == K_MINF || sl.kind == K_MAXF) ? SC_ENCF
+                                                                     : SC_INT;
+    uint32_t j = 0;
+    while (j < a->n_src &&
+           !(a->src_col[j] == sl.col && a->src_type[j] == sl.type && a->src_conv[j] == conv))
+      ++j;
+    if (j == a->n_src) {
+      a->src_col[j] = sl.col;
+      a->src_type[j] = sl.type;
+      a->src_conv[j] = conv;
+      ++a->n_src;
+    }
+    sl.src = j;
+  }
   if (!a->raw && D <= kDenseMax) {
     a->dense = 1;
     a->cap = D;
@@ -580,7 +666,7 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
     while (a->cap < want * 2) a->cap <<= 1;
     *entries = a->cap + 1;
   }
-  // LDS cache of hot groups: <= 60 KB (three 512-thread blocks per CU)
+  // LDS cache of hot groups: <= 60 KB (two 768-thread blocks per CU)
   const uint64_t kCacheBytes = 60 * 1024;
   uint32_t slot_bytes = 8 * (1 + a->W);
   uint32_t cs = 1;
@@ -642,12 +728,17 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   uint64_t rows = desc->sel ? desc->sel_capacity : desc->n_rows;
   if (rows) {
     size_t lds = (size_t)a.cache_slots * 8 * (1 + a.W);
-    // three blocks per CU fit the LDS caches (<= 48 KB each)
+    // as many blocks as are resident at once (LDS caches of <= 60 KB in the
+    // 160 KB of a CU; at most 2 blocks of 768 threads), so every block gets
+    // an equal share of the rows and none waits for a second round
+    const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(2, (160 * 1024) / std::max<size_t>(lds, 1)));
     uint64_t grid = std::min<uint64_t>(ceil_div(rows, (uint64_t)kAggBlock * GE * 4),
-                                       (uint64_t)cu_count() * 3);
+                                       (uint64_t)cu_count() * per_cu);
     ProfScope ps(s, "k_group_agg");
-    hipLaunchKernelGGL(k_group_agg, dim3((uint32_t)std::max<uint64_t>(grid, 1)),
-                       dim3(kAggBlock), lds, s->stream, a);
+    const dim3 gd((uint32_t)std::max<uint64_t>(grid, 1));
+    if (a.n_src <= 2) hipLaunchKernelGGL(k_group_agg<2>, gd, dim3(kAggBlock), lds, s->stream, a);
+    else if (a.n_src <= 4) hipLaunchKernelGGL(k_group_agg<4>, gd, dim3(kAggBlock), lds, s->stream, a);
+    else hipLaunchKernelGGL(k_group_agg<8>, gd, dim3(kAggBlock), lds, s->stream, a);
   }
   MTL_HIP(hipGetLastError());
   g->args = a;

@@ -155,6 +155,33 @@ def test_gpu_group_agg_parity(domain):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("domain", [5, 3_000_000])
+def test_gpu_group_agg_int64_extreme_sums(domain):
+    """Values over the whole int64 range (both halves of each value in use,
+    low halves at the sign boundary): the 128-bit SUM, AVG, MIN and MAX are
+    exact, in LDS-cached groups (5 keys) and in the global table (3M keys)."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(7 + domain)
+    n = 300_000
+    g = rng.integers(0, domain, n).astype(np.int64)
+    special = np.array([np.iinfo(np.int64).max, np.iinfo(np.int64).min, -1, 0x7FFFFFFF,
+                        -0x80000000, 0xFFFFFFFF, 0x80000000, -0x80000001,
+                        np.iinfo(np.int64).max - 0x7FFFFFFF], dtype=np.int64)
+    vi = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, n, dtype=np.int64,
+                      endpoint=True)
+    m = rng.random(n) < 0.2
+    vi[m] = special[rng.integers(0, len(special), int(m.sum()))]
+    aggs = [(F.AGG_SUM, vi, F.BIGINT), (F.AGG_COUNT, None, 0), (F.AGG_MIN, vi, F.BIGINT),
+            (F.AGG_MAX, vi, F.BIGINT), (F.AGG_AVG, vi, F.BIGINT)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs)
+    dg, dvi = ops.to_device(g), ops.to_device(vi)
+    groups = ops.group_agg([(dg, F.BIGINT, int(g.min()), int(g.max()))],
+                           [(f, None if c is None else dvi, t) for f, c, t in aggs], n)
+    gk, go = groups.fetch()
+    compare(gk, go, rk, ro, aggs)
+
+
+@pytest.mark.gpu
 def test_gpu_group_agg_raw_bigint_keys_and_skew():
     ops = _gpu_ops()
     rng = np.random.default_rng(99)
