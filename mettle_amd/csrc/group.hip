@@ -29,6 +29,8 @@
 //     first update, so the loads overlap instead of waiting behind atomics.
 #include <algorithm>
 #include <cmath>
+
+#include <rocprim/device/device_radix_sort.hpp>
 #include <cstring>
 #include <vector>
 
@@ -89,6 +91,15 @@ struct GroupArgs {
   uint32_t cache_slots;
   uint32_t cache_direct;  // dense domain that fits the cache: slot = key
   uint32_t* err;    // 64: table full, 8: key outside its stated range
+  // spill mode (large directly indexed domains): rows whose group misses
+  // the block's LDS cache are written to the block's region of the spill
+  // arrays (region of block b starts at row b * per) instead of updating
+  // the global table with device-scope atomics
+  uint32_t spill;
+  uint32_t* spill_key;   // [rows]
+  uint64_t* spill_val;   // [n_src][rows], source-major
+  uint32_t* spill_n;     // [grid]: spilled rows of each block
+  uint64_t spill_rows;   // row capacity of the spill arrays
 };
 
 __device__ __forceinline__ uint64_t gfmix(uint64_t k) {
@@ -266,9 +277,11 @@ constexpr int GE = 4;  // rows per thread per round, loaded together
 template <int NS>
 __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
   extern __shared__ uint64_t smem[];
+  __shared__ uint32_t s_spill;
   const uint32_t cs = a.cache_slots;
   uint64_t* ckeys = smem;
   uint64_t* cst = smem + cs;
+  if (threadIdx.x == 0) s_spill = 0;
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
     ckeys[i] = a.cache_direct ? (uint64_t)i : GEMPTY;
     for (uint32_t w = 0; w < a.W; ++w) cst[(uint64_t)w * cs + i] = a.init[w];
@@ -305,9 +318,9 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
     }
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
-      if (!ok[e]) continue;
       int slot = -1;
-      if (a.cache_direct) {
+      if (!ok[e]) {
+      } else if (a.cache_direct) {
         slot = (int)key[e];  // the whole (dense) domain is cached
       } else if (cs && key[e] != GEMPTY) {
         uint32_t h = (uint32_t)gfmix(key[e]) & (cs - 1);
@@ -330,20 +343,181 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
           }
         }
       }
-      if (slot >= 0) {
-        update_vals<true, NS>(a, cst + slot, cs, v[e]);
-      } else {
+      if (ok[e] && slot >= 0) update_vals<true, NS>(a, cst + slot, cs, v[e]);
+      const bool miss = ok[e] && slot < 0;
+      if (a.spill) {
+        // every lane of the wave is here: one LDS atomic per wave reserves
+        // the wave's spill entries, in lane order
+        const uint64_t mb = __ballot(miss);
+        if (mb) {
+          const int leader = __ffsll((unsigned long long)mb) - 1;
+          uint32_t at = 0;
+          if (lane_id() == leader) at = atomicAdd(&s_spill, (uint32_t)__popcll(mb));
+          at = __shfl(at, leader, WAVE) + (uint32_t)__popcll(mb & lanemask_lt());
+          if (miss) {
+            const uint64_t o = start + at;
+            a.spill_key[o] = (uint32_t)key[e];
+#pragma unroll
+            for (uint32_t j = 0; j < (uint32_t)NS; ++j)
+              if (j < a.n_src) a.spill_val[j * a.spill_rows + o] = v[e][j];
+          }
+        }
+      } else if (miss) {
         uint64_t g = global_slot(a, key[e]);
         if (g != GEMPTY) update_vals<false, NS>(a, a.st + g * a.W, 1, v[e]);
       }
     }
   }
   __syncthreads();
+  if (a.spill && threadIdx.x == 0) a.spill_n[blockIdx.x] = s_spill;
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
     uint64_t key = ckeys[i];
     if (key == GEMPTY || cst[i] == 0) continue;  // word 0: the row count
     uint64_t g = global_slot(a, key);
     if (g != GEMPTY) merge_state(a, a.st + g * a.W, cst + i, cs);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spilled rows (cache misses of a large dense domain): compacted, sorted by
+// group with rocPRIM's radix sort, then each run of one group is reduced in
+// registers and added to the global state (the LDS flushes of k_group_agg
+// are complete by then).  This replaces one device-scope atomic per
+// aggregate per missed row, the cost of the long tail of a skewed key
+// distribution.
+// ---------------------------------------------------------------------------
+template <int NS>
+struct SpillPay {
+  uint64_t v[NS];
+};
+
+// One block per k_group_agg block: its spilled rows go to [base, base + n)
+// of the compacted arrays, base = the spilled rows of the blocks before it.
+template <int NS>
+__global__ __launch_bounds__(256) void k_spill_compact(GroupArgs a, uint32_t agg_grid,
+                                                       uint32_t* ckey, SpillPay<NS>* cpay,
+                                                       uint32_t* total) {
+  __shared__ uint32_t s_red[4];
+  const uint32_t b = blockIdx.x, tid = threadIdx.x;
+  uint32_t part = 0;
+  for (uint32_t i = tid; i < b; i += 256) part += a.spill_n[i];
+  part = wave_sum_u32(part);
+  if (tid % WAVE == 0) s_red[tid / WAVE] = part;
+  __syncthreads();
+  const uint32_t base = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  const uint32_t cnt = a.spill_n[b];
+  const uint64_t n = a.sel ? dev_count(a.sel_count, a.sel_cap) : a.n_rows;
+  const uint64_t region = (uint64_t)b * ((n + agg_grid - 1) / agg_grid);
+  for (uint32_t i = tid; i < cnt; i += 256) {
+    ckey[base + i] = a.spill_key[region + i];
+    SpillPay<NS> p;
+#pragma unroll
+    for (uint32_t j = 0; j < (uint32_t)NS; ++j)
+      p.v[j] = j < a.n_src ? a.spill_val[j * a.spill_rows + region + i] : 0ull;
+    cpay[base + i] = p;
+  }
+  if (b == agg_grid - 1 && tid == 0) *total = base + cnt;
+}
+
+// Each thread reduces the runs of equal groups inside its own chunk of
+// kSpillPer sorted entries, with the partial state of every aggregate
+// (compile-time index) in registers.  A run wholly inside the chunk is the
+// whole group: it is added to the global state with plain loads and stores
+// (no other thread touches that group).  A run that touches a chunk edge may
+// continue in a neighbouring chunk: its partial is merged with device-scope
+// atomics, so a group with a long run costs one merge per chunk, not one
+// atomic per row.
+constexpr int kSpillPer = 32;
+template <int NS>
+__global__ __launch_bounds__(256) void k_spill_reduce(GroupArgs a, const uint32_t* skey,
+                                                      const SpillPay<NS>* spay, uint32_t total) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kSpillPer;
+  for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kSpillPer; i0 < total;
+       i0 += stride) {
+    const uint64_t i1 = i0 + kSpillPer < total ? i0 + kSpillPer : total;
+    const uint32_t prev_key = i0 > 0 ? skey[i0 - 1] : 0xFFFFFFFFu;
+    const uint32_t next_key = i1 < total ? skey[i1] : 0xFFFFFFFFu;
+    uint64_t i = i0;
+    while (i < i1) {
+      const uint32_t key = skey[i];
+      uint64_t acc1[MTL_GROUP_MAX_AGGS], acc2[MTL_GROUP_MAX_AGGS];
+      double accd[MTL_GROUP_MAX_AGGS];
+#pragma unroll
+      for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
+        acc1[k] = k < a.n_aggs ? a.init[a.aggs[k].word] : 0ull;
+        acc2[k] = 0;
+        accd[k] = 0.0;
+      }
+      uint64_t j = i;
+      for (; j < i1 && skey[j] == key; ++j) {
+        const SpillPay<NS> p = spay[j];
+#pragma unroll
+        for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
+          if (k >= a.n_aggs) break;
+          const AggSlot& sl = a.aggs[k];
+          if (sl.dup || sl.kind == K_COUNT) continue;
+          const uint64_t v = pick_src<NS>(p.v, sl.src);
+          switch (sl.kind) {
+            case K_SUMI: {
+              int64_t lo, hi;
+              split_i64((int64_t)v, lo, hi);
+              acc1[k] += (uint64_t)lo;
+              acc2[k] += (uint64_t)hi;
+              break;
+            }
+            case K_SUMF: accd[k] += __longlong_as_double((long long)v); break;
+            case K_MINI: acc1[k] = (int64_t)v < (int64_t)acc1[k] ? v : acc1[k]; break;
+            case K_MAXI: acc1[k] = (int64_t)v > (int64_t)acc1[k] ? v : acc1[k]; break;
+            case K_MINF: acc1[k] = v < acc1[k] ? v : acc1[k]; break;
+            case K_MAXF: acc1[k] = v > acc1[k] ? v : acc1[k]; break;
+            default: break;
+          }
+        }
+      }
+      const uint64_t cnt = j - i;
+      const bool shared = (i == i0 && key == prev_key) || (j == i1 && key == next_key);
+      uint64_t* g = a.st + (uint64_t)key * a.W;
+      if (shared) atomicAdd((unsigned long long*)g, (unsigned long long)cnt);
+      else g[0] += cnt;
+#pragma unroll
+      for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
+        if (k >= a.n_aggs) break;
+        const AggSlot& sl = a.aggs[k];
+        if (sl.dup || sl.kind == K_COUNT) continue;
+        uint64_t* w = g + sl.word;
+        if (shared) {
+          switch (sl.kind) {
+            case K_SUMI:
+              atomicAdd((unsigned long long*)w, (unsigned long long)acc1[k]);
+              if (acc2[k]) atomicAdd((unsigned long long*)(w + 1), (unsigned long long)acc2[k]);
+              break;
+            case K_SUMF: unsafeAtomicAdd((double*)w, accd[k]); break;
+            case K_MINI: atomicMin((long long*)w, (long long)acc1[k]); break;
+            case K_MAXI: atomicMax((long long*)w, (long long)acc1[k]); break;
+            case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)acc1[k]); break;
+            case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)acc1[k]); break;
+            default: break;
+          }
+        } else {
+          switch (sl.kind) {
+            case K_SUMI:
+              w[0] += acc1[k];
+              w[1] += acc2[k];
+              break;
+            case K_SUMF:
+              w[0] = (uint64_t)__double_as_longlong(__longlong_as_double((long long)w[0]) +
+                                                    accd[k]);
+              break;
+            case K_MINI: w[0] = (int64_t)acc1[k] < (int64_t)w[0] ? acc1[k] : w[0]; break;
+            case K_MAXI: w[0] = (int64_t)acc1[k] > (int64_t)w[0] ? acc1[k] : w[0]; break;
+            case K_MINF: w[0] = acc1[k] < w[0] ? acc1[k] : w[0]; break;
+            case K_MAXF: w[0] = acc1[k] > w[0] ? acc1[k] : w[0]; break;
+            default: break;
+          }
+        }
+      }
+      i = j;
+    }
   }
 }
 
@@ -681,6 +855,57 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
   return MTL_OK;
 }
 
+// Spill mode for a directly indexed domain too large for the LDS cache,
+// from this many rows (smaller inputs update the global table directly).
+constexpr uint64_t kSpillMinRows = 1ull << 20;
+
+// Compaction, sort and reduction of the spilled rows (after k_group_agg on
+// the same stream); `total` spilled rows, read by the caller.
+template <int NS>
+static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t total, mtl_stream* s) {
+  if (total == 0) return MTL_OK;
+  if (total > 0xFFFFFFFFull) return set_error(MTL_UNSUPPORTED, "more than 2^32-1 spilled rows");
+  Arena& ar = s->arena;
+  hipStream_t st = s->stream;
+  void *ckey, *cpay, *skey, *spay, *tot;
+  MTL_TRY(ar.alloc(total * 4, &ckey));
+  MTL_TRY(ar.alloc(total * sizeof(SpillPay<NS>), &cpay));
+  MTL_TRY(ar.alloc(total * 4, &skey));
+  MTL_TRY(ar.alloc(total * sizeof(SpillPay<NS>), &spay));
+  MTL_TRY(ar.alloc(16, &tot));
+  {
+    ProfScope ps(s, "k_spill_compact");
+    hipLaunchKernelGGL(k_spill_compact<NS>, dim3(agg_grid), dim3(256), 0, st, a, agg_grid,
+                       (uint32_t*)ckey, (SpillPay<NS>*)cpay, (uint32_t*)tot);
+    MTL_HIP(hipGetLastError());
+  }
+  // radix bits of the largest packed key (dense domain: keys < cap)
+  unsigned int bits = 1;
+  while (bits < 32 && (a.cap - 1) >> bits) ++bits;
+  size_t temp_bytes = 0;
+  MTL_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint32_t*)ckey, (uint32_t*)skey,
+                                    (SpillPay<NS>*)cpay, (SpillPay<NS>*)spay, (unsigned int)total,
+                                    0u, bits, st));
+  void* temp;
+  MTL_TRY(ar.alloc(temp_bytes ? temp_bytes : 1, &temp));
+  {
+    ProfScope ps(s, "spill_sort");
+    MTL_HIP(rocprim::radix_sort_pairs(temp, temp_bytes, (uint32_t*)ckey, (uint32_t*)skey,
+                                      (SpillPay<NS>*)cpay, (SpillPay<NS>*)spay,
+                                      (unsigned int)total, 0u, bits, st));
+  }
+  {
+    ProfScope ps(s, "k_spill_reduce");
+    const uint64_t blocks = std::min<uint64_t>(ceil_div(total, 256 * kSpillPer),
+                                               (uint64_t)cu_count() * 16);
+    hipLaunchKernelGGL(k_spill_reduce<NS>, dim3((uint32_t)std::max<uint64_t>(blocks, 1)),
+                       dim3(256), 0, st, a, (const uint32_t*)skey, (const SpillPay<NS>*)spay,
+                       (uint32_t)total);
+    MTL_HIP(hipGetLastError());
+  }
+  return MTL_OK;
+}
+
 }  // namespace mtl
 
 using namespace mtl;
@@ -734,15 +959,49 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
     const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(2, (160 * 1024) / std::max<size_t>(lds, 1)));
     uint64_t grid = std::min<uint64_t>(ceil_div(rows, (uint64_t)kAggBlock * GE * 4),
                                        (uint64_t)cu_count() * per_cu);
-    ProfScope ps(s, "k_group_agg");
     const dim3 gd((uint32_t)std::max<uint64_t>(grid, 1));
-    if (a.n_src <= 2) hipLaunchKernelGGL(k_group_agg<2>, gd, dim3(kAggBlock), lds, s->stream, a);
-    else if (a.n_src <= 4) hipLaunchKernelGGL(k_group_agg<4>, gd, dim3(kAggBlock), lds, s->stream, a);
-    else hipLaunchKernelGGL(k_group_agg<8>, gd, dim3(kAggBlock), lds, s->stream, a);
+    if (a.dense && !a.cache_direct && rows >= kSpillMinRows) {
+      void *sk, *sv, *sn;
+      MTL_TRY(s->arena.alloc(rows * 4, &sk));
+      MTL_TRY(s->arena.alloc(rows * 8 * std::max<uint32_t>(a.n_src, 1), &sv));
+      MTL_TRY(s->arena.alloc(gd.x * 4, &sn));
+      a.spill = 1;
+      a.spill_key = (uint32_t*)sk;
+      a.spill_val = (uint64_t*)sv;
+      a.spill_n = (uint32_t*)sn;
+      a.spill_rows = rows;
+    }
+    {
+      ProfScope ps(s, "k_group_agg");
+      if (a.n_src <= 2) hipLaunchKernelGGL(k_group_agg<2>, gd, dim3(kAggBlock), lds, s->stream, a);
+      else if (a.n_src <= 4) hipLaunchKernelGGL(k_group_agg<4>, gd, dim3(kAggBlock), lds, s->stream, a);
+      else hipLaunchKernelGGL(k_group_agg<8>, gd, dim3(kAggBlock), lds, s->stream, a);
+      MTL_HIP(hipGetLastError());
+    }
+    if (a.spill) {
+      // the spilled row count sizes the sort: read it with the error bits
+      void* hs;
+      MTL_TRY(stream_pinned(s, 8 + (size_t)gd.x * 4, &hs));
+      MTL_HIP(hipMemcpyAsync(hs, err, 4, hipMemcpyDeviceToHost, s->stream));
+      MTL_HIP(hipMemcpyAsync((char*)hs + 8, a.spill_n, (size_t)gd.x * 4, hipMemcpyDeviceToHost,
+                             s->stream));
+      MTL_HIP(hipStreamSynchronize(s->stream));
+      uint64_t total = 0;
+      for (uint32_t b = 0; b < gd.x; ++b) total += ((const uint32_t*)((char*)hs + 8))[b];
+      if (*(const uint32_t*)hs == 0) {
+        if (a.n_src <= 2) MTL_TRY(spill_finish<2>(a, gd.x, total, s));
+        else if (a.n_src <= 4) MTL_TRY(spill_finish<4>(a, gd.x, total, s));
+        else MTL_TRY(spill_finish<8>(a, gd.x, total, s));
+      }
+    }
   }
   MTL_HIP(hipGetLastError());
   g->args = a;
   g->args.err = nullptr;  // arena memory is not kept past this call
+  g->args.spill = 0;
+  g->args.spill_key = nullptr;
+  g->args.spill_val = nullptr;
+  g->args.spill_n = nullptr;
   // check errors now (synchronous boundary of the aggregate)
   void* host;
   MTL_TRY(stream_pinned(s, 8, &host));

@@ -162,10 +162,15 @@ class Groups:
         self.n_keys = n_keys
         self.aggs = aggs  # list of (func, type)
 
-    def __del__(self):
+    def release(self):
         if getattr(self, "h", None):
             abi.lib().mtl_groups_release(self.h)
             self.h = None
+
+    def __del__(self):
+        # at interpreter exit the module globals may already be gone
+        if abi is not None and getattr(abi, "lib", None) is not None:
+            self.release()
 
     def count(self) -> int:
         n = C.c_uint64()

@@ -135,11 +135,14 @@ def compare(gk, go, rk, ro, aggs, float_rel=1e-9):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("domain", [7, 1000, 70_000, 5_000_000])
-def test_gpu_group_agg_parity(domain):
+@pytest.mark.parametrize("domain,n", [(7, 400_000), (1000, 400_000), (70_000, 400_000),
+                                      (5_000_000, 400_000), (70_000, 2_000_000),
+                                      (5_000_000, 3_000_000)])
+def test_gpu_group_agg_parity(domain, n):
+    """Below 2^20 rows every cache miss updates the global table; from 2^20
+    rows on, misses of a large dense domain are spilled, sorted and reduced."""
     ops = _gpu_ops()
-    rng = np.random.default_rng(domain)
-    n = 400_000
+    rng = np.random.default_rng(domain + n)
     g = rng.integers(0, domain, n).astype(np.int64) * 3 - 17
     vi = rng.integers(-10**9, 10**9 + 1, n).astype(np.int64)
     vf = rng.standard_normal(n).astype(np.float32)
@@ -155,14 +158,14 @@ def test_gpu_group_agg_parity(domain):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("domain", [5, 3_000_000])
-def test_gpu_group_agg_int64_extreme_sums(domain):
+@pytest.mark.parametrize("domain,n", [(5, 300_000), (3_000_000, 300_000),
+                                      (3_000_000, 1_500_000)])
+def test_gpu_group_agg_int64_extreme_sums(domain, n):
     """Values over the whole int64 range (both halves of each value in use,
     low halves at the sign boundary): the 128-bit SUM, AVG, MIN and MAX are
     exact, in LDS-cached groups (5 keys) and in the global table (3M keys)."""
     ops = _gpu_ops()
-    rng = np.random.default_rng(7 + domain)
-    n = 300_000
+    rng = np.random.default_rng(7 + domain + n)
     g = rng.integers(0, domain, n).astype(np.int64)
     special = np.array([np.iinfo(np.int64).max, np.iinfo(np.int64).min, -1, 0x7FFFFFFF,
                         -0x80000000, 0xFFFFFFFF, 0x80000000, -0x80000001,
@@ -177,6 +180,30 @@ def test_gpu_group_agg_int64_extreme_sums(domain):
     dg, dvi = ops.to_device(g), ops.to_device(vi)
     groups = ops.group_agg([(dg, F.BIGINT, int(g.min()), int(g.max()))],
                            [(f, None if c is None else dvi, t) for f, c, t in aggs], n)
+    gk, go = groups.fetch()
+    compare(gk, go, rk, ro, aggs)
+
+
+@pytest.mark.gpu
+def test_gpu_group_agg_zipf_spill_with_selection():
+    """A skewed key over a large dense domain (hot keys in the LDS caches,
+    the tail spilled), through an input selection, with FLOAT MIN/MAX."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(2024)
+    n = 3_000_000
+    g = (np.minimum(rng.zipf(1.2, n), 2_000_000) + 5).astype(np.int64)
+    vi = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    vf = rng.standard_normal(n).astype(np.float32)
+    sel = np.flatnonzero(rng.random(n) < 0.7).astype(np.uint32)
+    aggs = [(F.AGG_SUM, vi, F.BIGINT), (F.AGG_COUNT, None, 0), (F.AGG_MIN, vf, F.FLOAT),
+            (F.AGG_MAX, vf, F.FLOAT), (F.AGG_AVG, vi, F.BIGINT), (F.AGG_SUM, vf, F.FLOAT)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs, sel=sel)
+    dg, dvi, dvf = ops.to_device(g), ops.to_device(vi), ops.to_device(vf)
+    dev = {id(vi): dvi, id(vf): dvf}
+    dsel = ops.to_device(sel)
+    groups = ops.group_agg([(dg, F.BIGINT, int(g.min()), int(g.max()))],
+                           [(f, None if c is None else dev[id(c)], t) for f, c, t in aggs], n,
+                           sel=dsel)
     gk, go = groups.fetch()
     compare(gk, go, rk, ro, aggs)
 
