@@ -419,104 +419,115 @@ __global__ __launch_bounds__(256) void k_spill_compact(GroupArgs a, uint32_t agg
   if (b == agg_grid - 1 && tid == 0) *total = base + cnt;
 }
 
-// Each thread reduces the runs of equal groups inside its own chunk of
-// kSpillPer sorted entries, with the partial state of every aggregate
-// (compile-time index) in registers.  A run wholly inside the chunk is the
-// whole group: it is added to the global state with plain loads and stores
-// (no other thread touches that group).  A run that touches a chunk edge may
-// continue in a neighbouring chunk: its partial is merged with device-scope
-// atomics, so a group with a long run costs one merge per chunk, not one
-// atomic per row.
-constexpr int kSpillPer = 32;
+// Partial state of one run for aggregate slot sl: (acc1, acc2) are the two
+// integer words (SUM: signed low/high sums; MIN/MAX: the value in acc1),
+// accd the double sum.
+__device__ __forceinline__ void run_combine(uint32_t kind, uint64_t& a1, uint64_t& a2,
+                                            double& ad, uint64_t b1, uint64_t b2, double bd) {
+  switch (kind) {
+    case K_SUMI: a1 += b1; a2 += b2; break;
+    case K_SUMF: ad += bd; break;
+    case K_MINI: a1 = (int64_t)b1 < (int64_t)a1 ? b1 : a1; break;
+    case K_MAXI: a1 = (int64_t)b1 > (int64_t)a1 ? b1 : a1; break;
+    case K_MINF: a1 = b1 < a1 ? b1 : a1; break;
+    case K_MAXF: a1 = b1 > a1 ? b1 : a1; break;
+    default: break;
+  }
+}
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
+  return ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, WAVE) << 32) |
+         (uint32_t)__shfl_up((int)(uint32_t)v, d, WAVE);
+}
+
+// Each wave takes 64 consecutive sorted entries at a time (one coalesced
+// load per lane), finds the runs of equal groups (a lane heads a run when
+// its key differs from the lane before it), and reduces every run with a
+// segmented inclusive scan across the lanes, per aggregate (compile-time
+// index).  The last lane of a run adds the run to the group: with plain
+// loads and stores when the run is the whole group (it does not reach the
+// ends of the 64 entries with the same group on the other side), else with
+// device-scope atomics (a group whose run crosses several windows costs
+// one merge per window, not one atomic per row).
 template <int NS>
 __global__ __launch_bounds__(256) void k_spill_reduce(GroupArgs a, const uint32_t* skey,
                                                       const SpillPay<NS>* spay, uint32_t total) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kSpillPer;
-  for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kSpillPer; i0 < total;
-       i0 += stride) {
-    const uint64_t i1 = i0 + kSpillPer < total ? i0 + kSpillPer : total;
-    const uint32_t prev_key = i0 > 0 ? skey[i0 - 1] : 0xFFFFFFFFu;
-    const uint32_t next_key = i1 < total ? skey[i1] : 0xFFFFFFFFu;
-    uint64_t i = i0;
-    while (i < i1) {
-      const uint32_t key = skey[i];
-      uint64_t acc1[MTL_GROUP_MAX_AGGS], acc2[MTL_GROUP_MAX_AGGS];
-      double accd[MTL_GROUP_MAX_AGGS];
-#pragma unroll
-      for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
-        acc1[k] = k < a.n_aggs ? a.init[a.aggs[k].word] : 0ull;
-        acc2[k] = 0;
-        accd[k] = 0.0;
-      }
-      uint64_t j = i;
-      for (; j < i1 && skey[j] == key; ++j) {
-        const SpillPay<NS> p = spay[j];
-#pragma unroll
-        for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
-          if (k >= a.n_aggs) break;
-          const AggSlot& sl = a.aggs[k];
-          if (sl.dup || sl.kind == K_COUNT) continue;
-          const uint64_t v = pick_src<NS>(p.v, sl.src);
-          switch (sl.kind) {
-            case K_SUMI: {
-              int64_t lo, hi;
-              split_i64((int64_t)v, lo, hi);
-              acc1[k] += (uint64_t)lo;
-              acc2[k] += (uint64_t)hi;
-              break;
-            }
-            case K_SUMF: accd[k] += __longlong_as_double((long long)v); break;
-            case K_MINI: acc1[k] = (int64_t)v < (int64_t)acc1[k] ? v : acc1[k]; break;
-            case K_MAXI: acc1[k] = (int64_t)v > (int64_t)acc1[k] ? v : acc1[k]; break;
-            case K_MINF: acc1[k] = v < acc1[k] ? v : acc1[k]; break;
-            case K_MAXF: acc1[k] = v > acc1[k] ? v : acc1[k]; break;
-            default: break;
-          }
-        }
-      }
-      const uint64_t cnt = j - i;
-      const bool shared = (i == i0 && key == prev_key) || (j == i1 && key == next_key);
-      uint64_t* g = a.st + (uint64_t)key * a.W;
+  const int lane = lane_id();
+  const uint64_t wave_id = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint64_t n_waves = (uint64_t)gridDim.x * blockDim.x / WAVE;
+  for (uint64_t base = wave_id * WAVE; base < total; base += n_waves * WAVE) {
+    const uint64_t i = base + lane;
+    const bool valid = i < total;
+    const uint32_t key = valid ? skey[i] : 0xFFFFFFFFu;  // keys are < 2^32 - 1
+    SpillPay<NS> p{};
+    if (valid) p = spay[i];
+    const uint32_t before = base > 0 ? skey[base - 1] : 0xFFFFFFFFu;
+    const uint32_t after = base + WAVE < total ? skey[base + WAVE] : 0xFFFFFFFFu;
+    const uint32_t up = (uint32_t)__shfl_up((int)key, 1, WAVE);
+    const uint32_t down = (uint32_t)__shfl_down((int)key, 1, WAVE);
+    const bool head = lane == 0 || key != up;
+    const bool tail = lane == WAVE - 1 || key != down;
+    const uint64_t heads = __ballot(head);
+    // lane index of this run's head: the highest head at or below this lane
+    const uint64_t le = lane == WAVE - 1 ? ~0ull : ((1ull << (lane + 1)) - 1);
+    const int first = 63 - __clzll((long long)(heads & le));
+    const bool shared = (first == 0 && key == before) || (lane == WAVE - 1 && key == after);
+    uint64_t* g = a.st + (uint64_t)key * a.W;
+    if (valid && tail) {
+      const uint64_t cnt = (uint64_t)(lane - first + 1);
       if (shared) atomicAdd((unsigned long long*)g, (unsigned long long)cnt);
       else g[0] += cnt;
+    }
 #pragma unroll
-      for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
-        if (k >= a.n_aggs) break;
-        const AggSlot& sl = a.aggs[k];
-        if (sl.dup || sl.kind == K_COUNT) continue;
+    for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
+      if (k >= a.n_aggs) break;
+      const AggSlot& sl = a.aggs[k];
+      if (sl.dup || sl.kind == K_COUNT) continue;
+      const uint64_t v = pick_src<NS>(p.v, sl.src);
+      uint64_t a1, a2 = 0;
+      double ad = 0.0;
+      if (sl.kind == K_SUMI) {
+        int64_t lo, hi;
+        split_i64((int64_t)v, lo, hi);
+        a1 = (uint64_t)lo;
+        a2 = (uint64_t)hi;
+      } else if (sl.kind == K_SUMF) {
+        a1 = 0;
+        ad = __longlong_as_double((long long)v);
+      } else {
+        a1 = v;
+      }
+#pragma unroll
+      for (int off = 1; off < WAVE; off <<= 1) {
+        const uint64_t b1 = shfl_up64(a1, off), b2 = shfl_up64(a2, off);
+        const double bd = __longlong_as_double((long long)shfl_up64(
+            (uint64_t)__double_as_longlong(ad), off));
+        if (lane - off >= first) run_combine(sl.kind, a1, a2, ad, b1, b2, bd);
+      }
+      if (valid && tail) {
         uint64_t* w = g + sl.word;
         if (shared) {
           switch (sl.kind) {
             case K_SUMI:
-              atomicAdd((unsigned long long*)w, (unsigned long long)acc1[k]);
-              if (acc2[k]) atomicAdd((unsigned long long*)(w + 1), (unsigned long long)acc2[k]);
+              atomicAdd((unsigned long long*)w, (unsigned long long)a1);
+              if (a2) atomicAdd((unsigned long long*)(w + 1), (unsigned long long)a2);
               break;
-            case K_SUMF: unsafeAtomicAdd((double*)w, accd[k]); break;
-            case K_MINI: atomicMin((long long*)w, (long long)acc1[k]); break;
-            case K_MAXI: atomicMax((long long*)w, (long long)acc1[k]); break;
-            case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)acc1[k]); break;
-            case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)acc1[k]); break;
+            case K_SUMF: unsafeAtomicAdd((double*)w, ad); break;
+            case K_MINI: atomicMin((long long*)w, (long long)a1); break;
+            case K_MAXI: atomicMax((long long*)w, (long long)a1); break;
+            case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)a1); break;
+            case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)a1); break;
             default: break;
           }
         } else {
-          switch (sl.kind) {
-            case K_SUMI:
-              w[0] += acc1[k];
-              w[1] += acc2[k];
-              break;
-            case K_SUMF:
-              w[0] = (uint64_t)__double_as_longlong(__longlong_as_double((long long)w[0]) +
-                                                    accd[k]);
-              break;
-            case K_MINI: w[0] = (int64_t)acc1[k] < (int64_t)w[0] ? acc1[k] : w[0]; break;
-            case K_MAXI: w[0] = (int64_t)acc1[k] > (int64_t)w[0] ? acc1[k] : w[0]; break;
-            case K_MINF: w[0] = acc1[k] < w[0] ? acc1[k] : w[0]; break;
-            case K_MAXF: w[0] = acc1[k] > w[0] ? acc1[k] : w[0]; break;
-            default: break;
-          }
+          uint64_t c1 = w[0], c2 = sl.kind == K_SUMI ? w[1] : 0;
+          double cd = __longlong_as_double((long long)c1);
+          run_combine(sl.kind, c1, c2, cd, a1, a2, ad);
+          if (sl.kind == K_SUMF) c1 = (uint64_t)__double_as_longlong(cd);
+          w[0] = c1;
+          if (sl.kind == K_SUMI) w[1] = c2;
         }
       }
-      i = j;
     }
   }
 }
@@ -896,8 +907,7 @@ static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t total, mtl
   }
   {
     ProfScope ps(s, "k_spill_reduce");
-    const uint64_t blocks = std::min<uint64_t>(ceil_div(total, 256 * kSpillPer),
-                                               (uint64_t)cu_count() * 16);
+    const uint64_t blocks = std::min<uint64_t>(ceil_div(total, 256), (uint64_t)cu_count() * 16);
     hipLaunchKernelGGL(k_spill_reduce<NS>, dim3((uint32_t)std::max<uint64_t>(blocks, 1)),
                        dim3(256), 0, st, a, (const uint32_t*)skey, (const SpillPay<NS>*)spay,
                        (uint32_t)total);
