@@ -206,6 +206,34 @@ def test_filter_project_parity(ops, t, dtype, n):
     assert govf == rovf
 
 
+@pytest.mark.parametrize("t,dtype", FP_TYPES)
+@pytest.mark.parametrize("n", [0, 1, 4097, 1_000_003])
+@pytest.mark.parametrize("second", [False, True])
+def test_filter_project_same_column(ops, t, dtype, n, second):
+    """Projection of the first predicate's column: the filter pass stages the
+    passing values for the expansion (optionally with a second predicate on
+    another column)."""
+    rng = np.random.default_rng(90 + t * 5 + n + second)
+    col = make_col(rng, dtype, max(n, 1))[:n]
+    fp = np.issubdtype(dtype, np.floating)
+    if not fp:
+        col = (col // 4096).astype(dtype)
+    b = rng.integers(0, 100, max(n, 1)).astype(np.int32)[:n]
+    lit = col[n // 4].item() if n else 0
+    op = abi.PROJ_SUB if t in (abi.DATE, abi.TIMESTAMP) else abi.PROJ_ADD
+    s_ = 2.5 if fp else 11
+    preds_h = [(col, t, abi.CMP_GE, lit)] + ([(b, abi.INTEGER, abi.CMP_LT, 60)] if second else [])
+    sel = orc.filter(preds_h, n)
+    ref, rovf = orc.project(col, t, sel, op, s_)
+    dc = ops.to_device(col) if n else ops.to_device(np.zeros(1, dtype))
+    db = ops.to_device(b) if n else ops.to_device(np.zeros(1, np.int32))
+    preds_d = [(dc, t, abi.CMP_GE, lit)] + ([(db, abi.INTEGER, abi.CMP_LT, 60)] if second else [])
+    gsel, gvals, govf = ops.filter_project(preds_d, n, dc, t, op, s_)
+    np.testing.assert_array_equal(ops.to_host(gsel, np.uint32), sel)
+    np.testing.assert_array_equal(ops.to_host(gvals).view(np.uint8), ref.view(np.uint8))
+    assert govf == rovf
+
+
 def test_filter_project_refine_and_overflow(ops):
     """An input selection with a device count; overflow of a passing row is
     flagged, overflow of a failing row is not."""
