@@ -630,17 +630,33 @@ struct GTopkArgs {
   uint32_t* out_n;
 };
 
-__device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint64_t k[4]) {
+// The words a group's top-K key needs: count, the ordering aggregate's
+// one or two state words, and the stored key (hash table).
+struct GRaw {
+  uint64_t cnt, w0, w1, key;
+};
+
+__device__ __forceinline__ GRaw gtopk_load(const GTopkArgs& a, uint64_t g) {
   const uint64_t* p = a.st + g * a.W;
+  GRaw r;
+  r.cnt = p[0];
+  r.w0 = p[a.word];
+  r.w1 = a.kind == K_SUMI ? p[a.word + 1] : 0ull;
+  r.key = a.dense ? g : (g == a.cap ? GEMPTY : a.tkeys[g]);
+  return r;
+}
+
+__device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, const GRaw& r, uint64_t k[4]) {
   uint64_t k0, k1 = 0;
   switch (a.kind) {
-    case K_COUNT: k0 = enc_i64((int64_t)p[0]); break;
+    case K_COUNT: k0 = enc_i64((int64_t)r.cnt); break;
     case K_SUMI: {
+      const uint64_t w[2] = {r.w0, r.w1};
       uint64_t lo;
       int64_t hi;
-      sum128(p + a.word, lo, hi);
+      sum128(w, lo, hi);
       if (a.avg) {
-        double v = ((double)hi * 18446744073709551616.0 + (double)lo) / (double)p[0];
+        double v = ((double)hi * 18446744073709551616.0 + (double)lo) / (double)r.cnt;
         k0 = enc_f64(v);
       } else {
         k0 = enc_i64(hi);
@@ -649,13 +665,13 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint6
       break;
     }
     case K_SUMF: {
-      double v = __longlong_as_double((long long)p[a.word]);
-      k0 = enc_f64(a.avg ? v / (double)p[0] : v);
+      double v = __longlong_as_double((long long)r.w0);
+      k0 = enc_f64(a.avg ? v / (double)r.cnt : v);
       break;
     }
     case K_MINI:
-    case K_MAXI: k0 = enc_i64((int64_t)p[a.word]); break;
-    default: k0 = p[a.word]; break;  // MINF/MAXF already encoded
+    case K_MAXI: k0 = enc_i64((int64_t)r.w0); break;
+    default: k0 = r.w0; break;  // MINF/MAXF already encoded
   }
   if (a.desc) {
     k0 = ~k0;
@@ -663,11 +679,13 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, uint64_t g, uint6
   }
   k[0] = k0;
   k[1] = k1;
-  uint64_t key = a.dense ? g : (g == a.cap ? GEMPTY : a.tkeys[g]);
-  k[2] = a.raw ? enc_i64((int64_t)key) : key;
+  k[2] = a.raw ? enc_i64((int64_t)r.key) : r.key;
   k[3] = 0;
 }
 
+// Phase 1: GE_TK groups per thread per round, their words loaded together
+// (one round trip per round instead of a count load and then a key load).
+constexpr int GE_TK = 4;
 template <int R>
 __global__ __launch_bounds__(TopkShape<R>::NT) void k_group_topk(GTopkArgs a) {
   using S = TopkShape<R>;
@@ -678,17 +696,26 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_group_topk(GTopkArgs a) {
   const uint64_t chunk = (a.n + gridDim.x - 1) / gridDim.x;
   const uint64_t start = (uint64_t)blockIdx.x * chunk;
   const uint64_t end = start + chunk < a.n ? start + chunk : a.n;
-  for (uint64_t base = start; base < end; base += S::NT) {
+  for (uint64_t base = start; base < end; base += (uint64_t)S::NT * GE_TK) {
+    GRaw r[GE_TK];
+#pragma unroll
+    for (int e = 0; e < GE_TK; ++e) {
+      const uint64_t g = base + (uint64_t)e * S::NT + threadIdx.x;
+      r[e] = g < end ? gtopk_load(a, g) : GRaw{0, 0, 0, 0};
+    }
     const uint64_t bthr =
         __hip_atomic_load(&sl.thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    const uint64_t g = base + threadIdx.x;
-    uint64_t k[4] = {0, 0, 0, 0};
-    bool pre = false;
-    if (g < end && a.st[g * a.W] != 0) {
-      gtopk_keys(a, g, k);
-      pre = k[0] <= wl.thr0 && k[0] <= bthr;
+#pragma unroll
+    for (int e = 0; e < GE_TK; ++e) {
+      const uint64_t g = base + (uint64_t)e * S::NT + threadIdx.x;
+      uint64_t k[4] = {0, 0, 0, 0};
+      bool pre = false;
+      if (g < end && r[e].cnt != 0) {
+        gtopk_keys(a, r[e], k);
+        pre = k[0] <= wl.thr0 && k[0] <= bthr;
+      }
+      wl.offer(pre, k, (uint32_t)g);
     }
-    wl.offer(pre, k, (uint32_t)g);
     topk_share_thr(wl, sl);
   }
   topk_block_finish<S::NT>(wl, sl, a.stage, a.lists0);
