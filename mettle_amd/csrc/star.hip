@@ -402,16 +402,23 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
           hits |= 1u << (4 * j + v);
   }
   for (uint32_t p = 1; p < a.n_preds && hits; ++p) pred_rows(a.preds[p], k0, hits);
-  // the wave's hits, in row order, to the wave's region
+  // the wave's hits, lane by lane, to the wave's region.  Hits are sparse:
+  // the loop runs once per hit of the lane (not 16 times with masked
+  // stores), and a hit's slot is picked from s by a select chain (static
+  // indices, so s stays in registers).
   const uint32_t c = (uint32_t)__popc(hits);
   const uint32_t incl = wave_inclusive_u32(c);
   const uint64_t region = (uint64_t)blockIdx.x * FTILE + (uint64_t)wave * (FJ * 256);
   uint32_t p = incl - c;
+  uint32_t h = hits;
+  while (h) {
+    const int i = __ffs(h) - 1;
+    h &= h - 1;
+    uint32_t sv = s[0][0];
 #pragma unroll
-  for (int i = 0; i < FJ * 4; ++i) {  // static indices: s stays in registers
-    if (!((hits >> i) & 1u)) continue;
+    for (int q = 1; q < FJ * 4; ++q) sv = i == q ? s[q >> 2][q & 3] : sv;
     a.hit_row[region + p] = (uint32_t)(k0 + (uint64_t)(i >> 2) * 256 + (i & 3));
-    a.hit_slot[region + p] = s[i >> 2][i & 3];
+    a.hit_slot[region + p] = sv;
     ++p;
   }
   if (lane == WAVE - 1) a.hit_count[(uint64_t)blockIdx.x * FWAVES + wave] = incl;
