@@ -703,18 +703,26 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_group_topk(GTopkArgs a) {
       const uint64_t g = base + (uint64_t)e * S::NT + threadIdx.x;
       r[e] = g < end ? gtopk_load(a, g) : GRaw{0, 0, 0, 0};
     }
+    uint64_t kk[GE_TK][4];
+    uint64_t m1 = ~0ull, m2 = ~0ull;
+#pragma unroll
+    for (int e = 0; e < GE_TK; ++e) {
+      const uint64_t g = base + (uint64_t)e * S::NT + threadIdx.x;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) kk[e][f] = 0;
+      if (g < end && r[e].cnt != 0) {
+        gtopk_keys(a, r[e], kk[e]);
+        topk_min2(kk[e][0], m1, m2);
+      }
+    }
+    if (R <= 2 && base == start) topk_seed_thr(m1, m2, a.k, sl);  // bound from round 1
     const uint64_t bthr =
         __hip_atomic_load(&sl.thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
     for (int e = 0; e < GE_TK; ++e) {
       const uint64_t g = base + (uint64_t)e * S::NT + threadIdx.x;
-      uint64_t k[4] = {0, 0, 0, 0};
-      bool pre = false;
-      if (g < end && r[e].cnt != 0) {
-        gtopk_keys(a, r[e], k);
-        pre = k[0] <= wl.thr0 && k[0] <= bthr;
-      }
-      wl.offer(pre, k, (uint32_t)g);
+      const bool pre = g < end && r[e].cnt != 0 && kk[e][0] <= wl.thr0 && kk[e][0] <= bthr;
+      wl.offer(pre, kk[e], (uint32_t)g);
     }
     topk_share_thr(wl, sl);
   }

@@ -583,6 +583,16 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_star_topk(TopkArgs a) {
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) cmax = cv[e] > cmax ? cv[e] : cmax;
+    if (R <= 2 && base == start) {  // seed the block bound from the first round
+      uint64_t m1 = ~0ull, m2 = ~0ull;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const uint64_t k0 = a.is_double ? ~enc_f64(__longlong_as_double((long long)av[e]))
+                                        : ~enc_i64((int64_t)av[e]);
+        if (cv[e] != 0) topk_min2(k0, m1, m2);
+      }
+      topk_seed_thr(m1, m2, a.k, sl);
+    }
     const uint64_t bthr = __hip_atomic_load(&sl.thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
     for (int e = 0; e < E; ++e) {

@@ -244,6 +244,51 @@ __device__ __forceinline__ void topk_block_init(TopkBlockLds<W, CAP>& s) {
   __syncthreads();
 }
 
+// Seed the block bound before any insertion (K <= 128).  Every lane passes
+// the smallest and second-smallest key word 0 among the candidates it holds
+// (lmin1 <= lmin2, all ones when absent).  For K <= 64 a wave sorts its 64
+// lane minima (bitonic network over the lanes) and takes the K-th smallest:
+// K lanes hold a candidate at or below it.  For K <= 128 it sorts the lanes'
+// second minima and takes the ceil(K/2)-th smallest: ceil(K/2) lanes hold two
+// candidates at or below it.  Either way no candidate above the bound can
+// reach the top K, so the block bound is lowered to it.  With the bound known
+// from the start, only a few candidates per block are ever inserted, instead
+// of every candidate that beats a still-filling list.  Every thread of the
+// block calls it (it ends with a barrier).
+template <int W, int CAP>
+__device__ __forceinline__ void topk_seed_thr(uint64_t lmin1, uint64_t lmin2, uint32_t K,
+                                              TopkBlockLds<W, CAP>& s) {
+  const int lane = __lane_id();
+  uint64_t v = K <= 64 ? lmin1 : lmin2;
+  const int at = K <= 64 ? (int)K - 1 : (int)((K + 1) / 2) - 1;
+#pragma unroll
+  for (int kk = 2; kk <= 64; kk <<= 1)
+#pragma unroll
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      const uint64_t o =
+          ((uint64_t)(uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), jj, 64) << 32) |
+          (uint32_t)__shfl_xor((int)(uint32_t)v, jj, 64);
+      const bool asc = (lane & kk) == 0, lower = (lane & jj) == 0;
+      v = (asc == lower) ? (o < v ? o : v) : (o > v ? o : v);
+    }
+  if (K >= 1 && K <= 128) {
+    const uint64_t t = bcast64(v, at);
+    if (lane == 0 && t != ~0ull && t < s.thr)
+      atomicMin((unsigned long long*)&s.thr, (unsigned long long)t);
+  }
+  __syncthreads();
+}
+
+// Running smallest and second-smallest of a lane's candidate keys.
+__device__ __forceinline__ void topk_min2(uint64_t k0, uint64_t& m1, uint64_t& m2) {
+  if (k0 < m1) {
+    m2 = m1;
+    m1 = k0;
+  } else if (k0 < m2) {
+    m2 = k0;
+  }
+}
+
 // After a batch: a wave with a full list lowers the block bound.
 template <int R, int W, int CAP>
 __device__ __forceinline__ void topk_share_thr(const WaveTopk<R>& wl, TopkBlockLds<W, CAP>& s) {
