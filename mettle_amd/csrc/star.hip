@@ -589,7 +589,9 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_star_topk(TopkArgs a) {
       for (int e = 0; e < E; ++e) {
         const uint64_t k0 = a.is_double ? ~enc_f64(__longlong_as_double((long long)av[e]))
                                         : ~enc_i64((int64_t)av[e]);
-        if (cv[e] != 0) topk_min2(k0, m1, m2);
+        if (cv[e] == 0) continue;
+        if (R == 1) m1 = k0 < m1 ? k0 : m1;  // K <= 64 needs the minimum only
+        else topk_min2(k0, m1, m2);
       }
       topk_seed_thr(m1, m2, a.k, sl);
     }
@@ -617,16 +619,11 @@ __global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_star_topk_merge(TopkArgs
                                                                         TopkLists out, uint32_t L,
                                                                         uint32_t final_level) {
   __shared__ TopkMergeLds s;
-  const int cur = topk_merge_lists(s, in, L, a.k, a.stage);
-  if (!final_level) {
-    topk_write_list(s, cur, out, a.k);
-    return;
-  }
-  // |SUM| <= count x max|value| for every group: flag the run when that
-  // bound exceeds int64 (an int64 accumulator could have wrapped)
-  if (!a.is_double) {
-    uint64_t mx = 0;
-    uint32_t cm = 0;
+  // the overflow bound's inputs are independent of the merge: on the final
+  // level their loads go out first and arrive while the lists merge
+  uint64_t mx = 0;
+  uint32_t cm = 0;
+  if (final_level && !a.is_double) {
     // eight independent loads in flight per thread per round
     for (uint32_t i0 = 0; i0 < a.n_tmax; i0 += TOPK_MERGE_THREADS * 8) {
       uint64_t t[8];
@@ -640,6 +637,15 @@ __global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_star_topk_merge(TopkArgs
     }
     for (uint32_t i = threadIdx.x; i < a.n_bcmax; i += TOPK_MERGE_THREADS)
       cm = a.bcmax[i] > cm ? a.bcmax[i] : cm;
+  }
+  const int cur = topk_merge_lists(s, in, L, a.k, a.stage);
+  if (!final_level) {
+    topk_write_list(s, cur, out, a.k);
+    return;
+  }
+  // |SUM| <= count x max|value| for every group: flag the run when that
+  // bound exceeds int64 (an int64 accumulator could have wrapped)
+  if (!a.is_double) {
     mx = block_max_u64(mx);
     cm = block_max_u32(cm);
     unsigned long long prod;
