@@ -890,7 +890,9 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
   const uint64_t kCacheBytes = 60 * 1024;
   uint32_t slot_bytes = 8 * (1 + a->W);
   uint32_t cs = 1;
-  while ((uint64_t)cs * 2 * slot_bytes <= kCacheBytes) cs *= 2;  // load factor <= 1/2
+  // the largest power of two of slots that fits (1024 for six state words);
+  // first come, first cached, up to 8 probes per key
+  while ((uint64_t)cs * 2 * slot_bytes <= kCacheBytes) cs *= 2;
   a->cache_direct = 0;
   if (a->dense && D * slot_bytes <= kCacheBytes) {
     // the whole domain fits: index the cache by key, no probing
