@@ -90,6 +90,7 @@ struct GroupArgs {
   uint64_t init[kMaxW];
   uint32_t cache_slots;
   uint32_t cache_direct;  // dense domain that fits the cache: slot = key
+  uint32_t door;          // hashed cache admits a group on its second sighting
   uint32_t* err;    // 64: table full, 8: key outside its stated range
   // spill mode (large directly indexed domains): rows whose group misses
   // the block's LDS cache are written to the block's region of the spill
@@ -270,6 +271,7 @@ __global__ void k_group_init(GroupArgs a, uint64_t n_groups) {
 // 12 waves per block: two blocks (and their LDS caches) per CU at 6 waves
 // per SIMD, within the SGPR-bound limit of 7.
 constexpr int kAggBlock = 768;
+constexpr uint32_t kDoorBits = 1u << 16;  // doorkeeper bitset of the hashed cache (8 KB)
 constexpr int GE = 4;  // rows per thread per round, loaded together
 
 // NS: compile-time bound on the number of value sources (registers for the
@@ -281,7 +283,10 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
   const uint32_t cs = a.cache_slots;
   uint64_t* ckeys = smem;
   uint64_t* cst = smem + cs;
+  uint32_t* door = (uint32_t*)(cst + (uint64_t)cs * a.W);  // kDoorBits bits when a.door
   if (threadIdx.x == 0) s_spill = 0;
+  if (a.door)
+    for (uint32_t i = threadIdx.x; i < kDoorBits / 32; i += blockDim.x) door[i] = 0;
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
     ckeys[i] = a.cache_direct ? (uint64_t)i : GEMPTY;
     for (uint32_t w = 0; w < a.W; ++w) cst[(uint64_t)w * cs + i] = a.init[w];
@@ -323,7 +328,8 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
       } else if (a.cache_direct) {
         slot = (int)key[e];  // the whole (dense) domain is cached
       } else if (cs && key[e] != GEMPTY) {
-        uint32_t h = (uint32_t)gfmix(key[e]) & (cs - 1);
+        const uint64_t hm = gfmix(key[e]);
+        uint32_t h = (uint32_t)hm & (cs - 1);
 #pragma unroll 1
         for (int p = 0; p < 8; ++p) {
           uint32_t si = (h + p) & (cs - 1);
@@ -334,6 +340,14 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
             break;
           }
           if (ck == GEMPTY) {
+            if (a.door) {
+              // doorkeeper: a group is cached on its second sighting in this
+              // block, so the one-off keys of a long tail do not take the
+              // slots of hot groups (first come, first cached otherwise)
+              const uint32_t b = (uint32_t)(hm >> 32) & (kDoorBits - 1);
+              const uint32_t was = atomicOr(&door[b >> 5], 1u << (b & 31));
+              if (!((was >> (b & 31)) & 1u)) break;
+            }
             unsigned long long old = atomicCAS((unsigned long long*)&ckeys[si], GEMPTY,
                                                (unsigned long long)key[e]);
             if (old == GEMPTY || old == key[e]) {
@@ -999,7 +1013,8 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   MTL_HIP(hipGetLastError());
   uint64_t rows = desc->sel ? desc->sel_capacity : desc->n_rows;
   if (rows) {
-    size_t lds = (size_t)a.cache_slots * 8 * (1 + a.W);
+    a.door = (!a.cache_direct && a.cache_slots && rows >= kSpillMinRows) ? 1u : 0u;
+    size_t lds = (size_t)a.cache_slots * 8 * (1 + a.W) + (a.door ? kDoorBits / 8 : 0);
     // as many blocks as are resident at once (LDS caches of <= 60 KB in the
     // 160 KB of a CU; at most 2 blocks of 768 threads), so every block gets
     // an equal share of the rows and none waits for a second round
