@@ -209,6 +209,52 @@ def test_gpu_group_agg_zipf_spill_with_selection():
 
 
 @pytest.mark.gpu
+def test_gpu_group_agg_composite_keys_spill():
+    """Two packed keys whose domain (about 1.4M groups) exceeds the LDS cache,
+    2M rows: the spill path with composite keys, DOUBLE MIN/MAX and AVG."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(77)
+    n = 2_000_000
+    k1 = rng.integers(-300, 400, n).astype(np.int32)
+    k2 = rng.integers(18000, 20000, n).astype(np.int32)  # DATE days
+    vd = rng.standard_normal(n) * 1e6
+    vi = rng.integers(-2**40, 2**40, n).astype(np.int64)
+    aggs = [(F.AGG_COUNT, None, 0), (F.AGG_MIN, vd, F.DOUBLE), (F.AGG_MAX, vd, F.DOUBLE),
+            (F.AGG_AVG, vd, F.DOUBLE), (F.AGG_SUM, vi, F.BIGINT)]
+    rk, ro = orc.group_agg([(k1, F.INTEGER), (k2, F.DATE)], aggs)
+    dev = {id(vd): ops.to_device(vd), id(vi): ops.to_device(vi)}
+    groups = ops.group_agg([(ops.to_device(k1), F.INTEGER, int(k1.min()), int(k1.max())),
+                            (ops.to_device(k2), F.DATE, int(k2.min()), int(k2.max()))],
+                           [(f, None if c is None else dev[id(c)], t) for f, c, t in aggs], n)
+    gk, go = groups.fetch()
+    compare(gk, go, rk, ro, aggs)
+
+
+@pytest.mark.gpu
+def test_gpu_group_agg_raw_keys_large():
+    """Full-range BIGINT keys (hash table, no spill) over 1.5M rows with a
+    skewed head: the hashed LDS cache with its doorkeeper, misses into the
+    global hash table."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(101)
+    n = 1_500_000
+    pool = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, 200_000, dtype=np.int64)
+    g = pool[np.minimum(rng.zipf(1.15, n) - 1, len(pool) - 1)]
+    v = rng.integers(-10**6, 10**6, n).astype(np.int64)
+    aggs = [(F.AGG_SUM, v, F.BIGINT), (F.AGG_COUNT, None, 0), (F.AGG_MIN, v, F.BIGINT)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs)
+    dv = ops.to_device(v)
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, np.iinfo(np.int64).min,
+                             np.iinfo(np.int64).max)],
+                           [(f, None if c is None else dv, t) for f, c, t in aggs], n)
+    gk, go = groups.fetch()
+    order = np.lexsort([gk[:, 0]])
+    gk = gk[order]
+    go = [o[order] for o in go]
+    compare(gk, go, rk, ro, aggs)
+
+
+@pytest.mark.gpu
 def test_gpu_group_agg_raw_bigint_keys_and_skew():
     ops = _gpu_ops()
     rng = np.random.default_rng(99)
