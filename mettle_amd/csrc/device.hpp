@@ -33,6 +33,20 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
   return v;
 }
 
+// Lane i of each 16-lane row receives lane i - OFF of the same row (DPP
+// row_shr); the first OFF lanes of a row receive 0.  One VALU move, no LDS
+// crossbar round trip.
+template <int OFF>
+__device__ __forceinline__ uint32_t row_shr(uint32_t v) {
+  static_assert(OFF >= 1 && OFF <= 15, "row_shr takes 1..15");
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + OFF, 0xF, 0xF, false);
+}
+// Lane i of each 16-lane row receives lane i + 1 (DPP row_shl:1); the last
+// lane of a row receives 0.
+__device__ __forceinline__ uint32_t row_shl1(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xF, 0xF, false);
+}
+
 // Load at base + a 32-bit byte offset: with a wave-uniform base this is
 // one SGPR base plus a 32-bit VGPR offset per lane, instead of a 64-bit
 // VGPR address per lane (gathers keep many loads in flight at once).

@@ -290,6 +290,26 @@ __device__ __forceinline__ void mark_rows(const FilterArgs& a, uint64_t n, uint6
   if (e) atomicOr(a.err, e);
 }
 
+// The pass bits of one 1024-row wave segment: lane l's 4 bits of step j sit
+// at bits 4(l%16).. of word l/16 of the step's 256 rows.  An OR scan over
+// each 16-lane row (DPP row shifts) leaves the word in the row's last lane,
+// which stores it.
+__device__ __forceinline__ void store_pass_bits(uint64_t* sel_bm, uint64_t seg, uint32_t bits,
+                                                int lane) {
+  const uint32_t r = (uint32_t)lane & 15u;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const uint32_t nib = (bits >> (4 * j)) & 0xFu;
+    uint32_t lo = r < 8 ? nib << (4 * r) : 0u;
+    uint32_t hi = r < 8 ? 0u : nib << (4 * (r - 8));
+    lo |= row_shr<1>(lo); hi |= row_shr<1>(hi);
+    lo |= row_shr<2>(lo); hi |= row_shr<2>(hi);
+    lo |= row_shr<4>(lo); hi |= row_shr<4>(hi);
+    lo |= row_shr<8>(lo); hi |= row_shr<8>(hi);
+    if (r == 15) sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = ((uint64_t)hi << 32) | lo;
+  }
+}
+
 // Pass 1 of a filter: one tile of 4096 rows per block, no inter-block
 // dependency.  Writes the tile's pass bits to sel_bm (1 bit per row, bit r
 // for row r of the input order) and its count to tile_counts[tile].
@@ -342,17 +362,7 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
       if (w && wbase + i < words) atomicOr((unsigned long long*)&a.mark_bm[wbase + i], w);
     }
   }
-  if (a.sel_bm) {
-    // lane l's 4 bits of step j sit at bits 4(l%16).. of word l/16 of the
-    // 256-row segment: OR the 16 lanes of each quarter-wave together
-#pragma unroll
-    for (int j = 0; j < FJ; ++j) {
-      uint64_t w = (uint64_t)((bits >> (4 * j)) & 0xFu) << (4 * (lane & 15));
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) w |= __shfl_xor(w, off, WAVE);
-      if ((lane & 15) == 0) a.sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = w;
-    }
-  }
+  if (a.sel_bm) store_pass_bits(a.sel_bm, seg, bits, lane);
   if (a.tile_counts) {
     uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
     if (lane == 0) s_wave_tot[wave] = c;
@@ -364,6 +374,225 @@ __global__ __launch_bounds__(FBLOCK) void k_filter(FilterArgs a) {
       a.tile_counts[tile] = t;
       if (a.group_sums && t) atomicAdd(&a.group_sums[(tile >> kGroupShift) * kGroupStride], t);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent probe pass.  A narrow pass with a semi-join probe and no tile
+// counts (the parent pass of a star join) is bound by its probes: a probe of
+// a random key is a divergent load, one L2 request per lane, and at about
+// half the rows of a 15M-row parent they cost as much as streaming its three
+// columns.  Here one block of 1024 threads per CU copies the low words of the
+// probe bitmap (up to 144 KB: 1,179,648 keys) into LDS once; its waves then
+// walk 1024-row segments round-robin, and probes of keys in the copied part
+// read LDS.  Marks are OR-combined across the lanes of a wave that hit the
+// same 64-bit word (a segmented OR scan), so a run of sorted keys costs one
+// atomic per word, and no barrier follows the copy.
+// ---------------------------------------------------------------------------
+constexpr int PBLOCK = 1024;
+constexpr uint32_t kLdsProbeWords = 36864;  // 32-bit words, 144 KB
+
+// Mark the offsets of the kept rows of one wave segment.  Per step, each
+// lane folds its rows that share the word of its first kept row into one
+// (word, bits) pair; a segmented OR scan over each 16-lane row (DPP row
+// shifts) merges the pairs of lanes with equal words (OR is idempotent, so
+// merging equal words that are not adjacent is harmless), and the last lane
+// of each run of equal words in a row issues the atomic.  With dense sorted
+// keys a row's 64 rows are exactly one bitmap word: one atomic per word.  A
+// lane's rows with another word are marked one by one.
+// One step of the segmented OR scan: take the bits of lane r - OFF of the
+// row when it holds the same word.
+template <int OFF>
+__device__ __forceinline__ void row_seg_or(uint32_t r, uint32_t w, uint32_t& lo, uint32_t& hi) {
+  const uint32_t ow = row_shr<OFF>(w), ol = row_shr<OFF>(lo), oh = row_shr<OFF>(hi);
+  if (r >= (uint32_t)OFF && ow == w) {
+    lo |= ol;
+    hi |= oh;
+  }
+}
+
+__device__ __forceinline__ void mark_offsets_wave(const FilterArgs& a,
+                                                  const uint32_t (&ms)[FJ][4], uint32_t bits,
+                                                  int lane) {
+  uint32_t e = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const uint32_t m = (bits >> (4 * j)) & 0xFu;
+    if (__ballot(m != 0) == 0) continue;  // wave-uniform: nothing kept at this step
+    // branch-free fold: the word of the first kept row (selects from the
+    // last row back), then the bits of the kept rows in that word
+    uint32_t wj = 0xFFFFFFFFu, rest = 0, lo = 0, hi = 0;
+#pragma unroll
+    for (int v = 3; v >= 0; --v) {
+      const bool k = ((m >> v) & 1u) && ms[j][v] != 0xFFFFFFFFu;
+      wj = k ? ms[j][v] >> 6 : wj;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t sv = ms[j][v];
+      const bool sel = (m >> v) & 1u;
+      e |= (sel && sv == 0xFFFFFFFFu) ? 8u : 0u;
+      const bool k = sel && sv != 0xFFFFFFFFu;
+      const bool same = k && (sv >> 6) == wj;
+      const uint32_t b = 1u << (sv & 31);
+      lo |= (same && !(sv & 32)) ? b : 0u;
+      hi |= (same && (sv & 32)) ? b : 0u;
+      rest |= (k && !same) ? 1u << v : 0u;
+    }
+    const uint32_t r = (uint32_t)lane & 15u;
+    row_seg_or<1>(r, wj, lo, hi);
+    row_seg_or<2>(r, wj, lo, hi);
+    row_seg_or<4>(r, wj, lo, hi);
+    row_seg_or<8>(r, wj, lo, hi);
+    const uint32_t nw = row_shl1(wj);
+    if ((lo | hi) && (r == 15 || nw != wj))
+      atomicOr((unsigned long long*)&a.mark_bm[wj], ((uint64_t)hi << 32) | lo);
+    if (__ballot(rest != 0) == 0) continue;  // sorted keys: a lane's rows share a word
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (!((rest >> v) & 1u)) continue;
+      const uint32_t sv = ms[j][v];
+      atomicOr((unsigned long long*)&a.mark_bm[sv >> 6], 1ull << (sv & 63));
+    }
+  }
+  if (e) atomicOr(a.err, e);
+}
+
+// The 16 values of lane l of a 1024-row wave segment of a 4-byte column
+// (rows k0 + 256 j + v, k0 = seg + 4 l).  FULL: the segment lies inside the
+// table and the column is 16-byte aligned, so each step is one dwordx4 with
+// no branch, and the loads of all columns go out back to back.
+template <bool FULL>
+__device__ __forceinline__ void seg_load4(const uint32_t* col, uint64_t k0, uint64_t n,
+                                          uint32_t (&x)[FJ][4]) {
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const uint64_t k = k0 + (uint64_t)j * 256;
+    if (FULL) {
+      const uint4 u = *(const uint4*)(col + k);
+      x[j][0] = u.x; x[j][1] = u.y; x[j][2] = u.z; x[j][3] = u.w;
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[j][v] = k + v < n ? col[k + v] : 0u;
+    }
+  }
+}
+
+// Offset of a 4-byte integer key from min, 0xFFFFFFFF outside [0, span).
+// K32 (the host checked 0 <= min and min + span <= 2^31): one 32-bit
+// subtraction, since a negative key or one below min wraps to at least
+// 2^31 - min >= span.
+template <bool K32>
+__device__ __forceinline__ uint32_t key_off4(uint32_t raw, int64_t min, uint64_t span) {
+  if (K32) {
+    const uint32_t o = raw - (uint32_t)min;
+    return o < (uint32_t)span ? o : 0xFFFFFFFFu;
+  }
+  return narrow_off(raw, MTL_INTEGER, min, span);
+}
+
+// One wave segment of the probe pass: returns the pass bits (bit 4j+v) and
+// leaves the mark offsets in ms (0xFFFFFFFF: key outside the range).
+// Predicate (of kind PK; none when PK < 0) first, then the probe of the
+// surviving rows: offsets
+// below lds_keys read the LDS copy, the rest the bitmap in HBM.  Every probe
+// load is issued unconditionally (a lane with nothing to probe reads word 0)
+// and the results are selected afterwards, so no branch separates the loads.
+template <int PK, bool K32, bool FULL>
+__device__ __forceinline__ uint32_t probe_segment(const FilterArgs& a, uint64_t n, uint64_t k0,
+                                                  const uint32_t* lds, uint32_t lds_keys,
+                                                  uint32_t (&ms)[FJ][4]) {
+  uint32_t bits = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (FULL || k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
+  uint32_t px[FJ][4], pk[FJ][4];
+  if (PK >= 0) seg_load4<FULL>((const uint32_t*)a.preds[0].col, k0, n, px);
+  seg_load4<FULL>((const uint32_t*)a.probe_key, k0, n, pk);
+  seg_load4<FULL>((const uint32_t*)a.mark_key, k0, n, ms);
+  // keep the scheduler from trading the loads of later steps for registers:
+  // all twelve go out before any of them is used
+  __builtin_amdgcn_sched_barrier(0);
+  if (PK >= 0) {
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (!pred_pass(enc_raw(px[j][v], (uint32_t)PK), a.preds[0]))
+          bits &= ~(1u << (4 * j + v));
+  }
+  const uint32_t* bm32 = (const uint32_t*)a.probe_bm;
+  uint32_t wg[FJ][4];
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t o = key_off4<K32>(pk[j][v], a.probe_min, a.probe_span);
+      pk[j][v] = ((bits >> (4 * j + v)) & 1u) ? o : 0xFFFFFFFFu;
+      const bool in_hbm = pk[j][v] >= lds_keys && pk[j][v] != 0xFFFFFFFFu;
+      wg[j][v] = ld_off32(bm32, in_hbm ? (pk[j][v] >> 5) << 2 : 0u);
+    }
+  __builtin_amdgcn_sched_barrier(0);
+  // the LDS reads go out while the HBM probes are in flight
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t o = pk[j][v];
+      const uint32_t wl = lds[o < lds_keys ? o >> 5 : 0u];
+      const uint32_t w = o < lds_keys ? wl : wg[j][v];
+      const uint32_t hit = (uint32_t)(o != 0xFFFFFFFFu) & (w >> (o & 31));
+      bits &= ~((~hit & 1u) << (4 * j + v));
+      ms[j][v] = key_off4<K32>(ms[j][v], a.mark_min, a.mark_span);
+    }
+  return bits;
+}
+
+// lds_words: 32-bit words of the probe bitmap to copy (a multiple of 4, at
+// most kLdsProbeWords and at most the bitmap's size).  The host sends only
+// passes whose probe and mark keys are 4-byte integers, with one predicate
+// of kind PK (PK_I32 or PK_F32) or none (PK < 0), and a mark bitmap.
+template <int PK, bool K32>
+__global__ __launch_bounds__(PBLOCK) void k_filter_probe(FilterArgs a, uint32_t lds_words) {
+  __shared__ uint32_t s_bm[kLdsProbeWords];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t* bm32 = (const uint32_t*)a.probe_bm;
+  {
+    // all nine 16-byte loads of a thread in flight together
+    constexpr int kCopy = kLdsProbeWords / (PBLOCK * 4);
+    static_assert(kCopy * PBLOCK * 4 == kLdsProbeWords, "copy covers the LDS words");
+    uint4 t[kCopy];
+#pragma unroll
+    for (int c = 0; c < kCopy; ++c) {
+      const uint32_t i = ((uint32_t)c * PBLOCK + (uint32_t)tid) * 4;
+      t[c] = *(const uint4*)&bm32[i < lds_words ? i : 0u];  // unconditional: no branch
+    }
+#pragma unroll
+    for (int c = 0; c < kCopy; ++c) {
+      const uint32_t i = ((uint32_t)c * PBLOCK + (uint32_t)tid) * 4;
+      if (i < lds_words) *(uint4*)&s_bm[i] = t[c];
+    }
+  }
+  __syncthreads();
+  const uint32_t lds_keys = lds_words * 32;
+  const uint64_t n = a.n_rows;
+  const bool vec = a.probe_vec && a.mark_vec && (PK < 0 || (a.vec_ok & 1u));
+  // every segment of the 4096-row tiles, so the pass bits of rows past n up
+  // to the tile end are written (as zeros) just as the tiled kernel does
+  const uint64_t segs = (n + FTILE - 1) / FTILE * (FTILE / (FJ * 256));
+  const uint64_t nwaves = (uint64_t)gridDim.x * (PBLOCK / WAVE);
+  for (uint64_t sg = (uint64_t)blockIdx.x * (PBLOCK / WAVE) + wave; sg < segs; sg += nwaves) {
+    const uint64_t seg = sg * (FJ * 256);
+    const uint64_t k0 = seg + (uint64_t)lane * 4;
+    uint32_t ms[FJ][4];
+    uint32_t bits;
+    if (vec && seg + FJ * 256 <= n)
+      bits = probe_segment<PK, K32, true>(a, n, k0, s_bm, lds_keys, ms);
+    else bits = probe_segment<PK, K32, false>(a, n, k0, s_bm, lds_keys, ms);
+    mark_offsets_wave(a, ms, bits, lane);
+    store_pass_bits(a.sel_bm, seg, bits, lane);
   }
 }
 
@@ -653,15 +882,7 @@ __global__ __launch_bounds__(FBLOCK) void k_filter_dense(DenseArgs a) {
       if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
   for (uint32_t p = 0; p < a.n_preds; ++p)
     pred_dense_typed(a.preds[p], k0, n, (a.vec_ok >> p) & 1u, bits);
-  // lane l's 4 bits of step j sit at bits 4(l%16).. of word l/16 of the
-  // 256-row segment: OR the 16 lanes of each quarter-wave together
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    uint64_t w = (uint64_t)((bits >> (4 * j)) & 0xFu) << (4 * (lane & 15));
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) w |= __shfl_xor(w, off, WAVE);
-    if ((lane & 15) == 0) a.sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = w;
-  }
+  store_pass_bits(a.sel_bm, seg, bits, lane);
   const uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
   if (lane == 0) s_wave[wave] = c;
   __syncthreads();
@@ -785,8 +1006,37 @@ int32_t launch_filter(FilterArgs a, uint64_t capacity, hipStream_t st) {
                       narrow_key(a.probe_bm ? a.probe_key : nullptr, a.probe_key_type,
                                  a.probe_span) &&
                       narrow_key(a.mark_bm ? a.mark_key : nullptr, a.mark_key_type, a.mark_span);
-  if (a.in_sel) hipLaunchKernelGGL((k_filter<true, false>), dim3(tiles), dim3(FBLOCK), 0, st, a);
-  else if (narrow)
+  // probe-bound passes (a large table, a probe bitmap of 64K keys or more,
+  // pass bits and marks only) take the persistent kernel with the bitmap's
+  // low words in LDS
+  const int pk = a.n_preds ? (int)a.preds[0].kind : -1;
+  const bool lds_probe = narrow && a.probe_bm && a.mark_bm && a.sel_bm && !a.tile_counts &&
+                         !a.group_sums && a.n_rows >= (1u << 20) &&
+                         a.probe_span >= (1u << 16) && ((uintptr_t)a.probe_bm % 16) == 0 &&
+                         type_width(a.probe_key_type) == 4 &&
+                         type_width(a.mark_key_type) == 4 &&
+                         (pk < 0 || pk == (int)PK_I32 || pk == (int)PK_F32);
+  if (lds_probe) {
+    const uint64_t words32 = ceil_div(a.probe_span, 64) * 2;
+    const uint32_t lds_words = (uint32_t)std::min<uint64_t>(kLdsProbeWords, words32 & ~3ull);
+    const uint64_t segs = (uint64_t)tiles * (FTILE / (FJ * 256));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(cu_count(), ceil_div(segs, PBLOCK / WAVE));
+    auto fits32 = [](int64_t min, uint64_t span) {
+      return min >= 0 && (uint64_t)min + span <= (1ull << 31);
+    };
+    const bool k32 = fits32(a.probe_min, a.probe_span) && fits32(a.mark_min, a.mark_span);
+#define MTL_PROBE(PK, K32) \
+  hipLaunchKernelGGL((k_filter_probe<PK, K32>), dim3(grid), dim3(PBLOCK), 0, st, a, lds_words)
+#define MTL_PROBE_K(PK) \
+  if (k32) MTL_PROBE(PK, true); else MTL_PROBE(PK, false)
+    if (pk == (int)PK_I32) { MTL_PROBE_K((int)PK_I32); }
+    else if (pk == (int)PK_F32) { MTL_PROBE_K((int)PK_F32); }
+    else { MTL_PROBE_K(-1); }
+#undef MTL_PROBE_K
+#undef MTL_PROBE
+  } else if (a.in_sel) {
+    hipLaunchKernelGGL((k_filter<true, false>), dim3(tiles), dim3(FBLOCK), 0, st, a);
+  } else if (narrow)
     hipLaunchKernelGGL((k_filter<false, true>), dim3(tiles), dim3(FBLOCK), 0, st, a);
   else hipLaunchKernelGGL((k_filter<false, false>), dim3(tiles), dim3(FBLOCK), 0, st, a);
   MTL_HIP(hipGetLastError());

@@ -324,18 +324,48 @@ def test_star_reference_q3_fixture(ops, decimal, boundary):
     assert got == golden.Q3_EXPECTED
 
 
-@pytest.mark.parametrize("scale,k", [(0.002, 10), (0.01, 10), (0.01, 100), (0.05, 512)])
-def test_star_parity_random(ops, scale, k):
-    tables = datagen.star_tables(scale=scale, workload=30)
-    n_c, n_o, n_l = (len(tables[0]["c_key"]), len(tables[1]["o_key"]),
-                     len(tables[2]["l_order"]))
-    build = lambda p: datagen.star_desc(p, n_c, n_o, n_l, k=k)  # noqa: E731
+def _star_parity(ops, tables, build):
     st, ref = orc.star_topk(host_desc(tables, build))
     assert st == abi.MTL_OK
     d, _keep = device_desc(ops, tables, build)
     rows = ops.star_topk(d)
     assert [(r.p_row, r.count, r.sum_i64) for r in rows] == \
         [(r.p_row, r.count, r.sum_i64) for r in ref]
+    return rows
+
+
+@pytest.mark.parametrize("scale,k", [(0.002, 10), (0.01, 10), (0.01, 100), (0.05, 512),
+                                     (0.1, 10), (1.0, 10)])
+def test_star_parity_random(ops, scale, k):
+    """Scales 0.1 and 1.0 have parent tables of 1.5M and 15M rows, which take
+    the persistent probe pass (dimension spans of 150K and 1.5M keys: all of
+    the first and 1,179,648 keys of the second are probed in LDS)."""
+    tables = datagen.star_tables(scale=scale, workload=30)
+    n_c, n_o, n_l = (len(tables[0]["c_key"]), len(tables[1]["o_key"]),
+                     len(tables[2]["l_order"]))
+    _star_parity(ops, tables, lambda p: datagen.star_desc(p, n_c, n_o, n_l, k=k))
+
+
+@pytest.mark.parametrize("shuffle_orders", [False, True])
+def test_star_parity_wide_dimension_span(ops, shuffle_orders):
+    """Customer keys stretched x20 (span 3M): probes below 1,179,648 read the
+    LDS copy of the bitmap, the rest read it in HBM.  Shuffled orders make
+    the parent's mark keys unsorted, so few lanes share a bitmap word."""
+    cust, orders, lines = datagen.star_tables(scale=0.1, workload=32)
+    cust["c_key"] = cust["c_key"] * 20
+    orders["o_cust"] = orders["o_cust"] * 20
+    if shuffle_orders:
+        perm = np.random.default_rng(7).permutation(len(orders["o_key"]))
+        orders = {c: np.ascontiguousarray(v[perm]) for c, v in orders.items()}
+    tables = (cust, orders, lines)
+    n_c, n_o, n_l = len(cust["c_key"]), len(orders["o_key"]), len(lines["l_order"])
+
+    def build(p):
+        d = datagen.star_desc(p, n_c, n_o, n_l, k=50)
+        d.d_key_min, d.d_key_max = 20, 20 * n_c
+        return d
+    rows = _star_parity(ops, tables, build)
+    assert len(rows) == 50
 
 
 def test_star_no_qualifying_rows(ops):
