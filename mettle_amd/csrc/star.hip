@@ -339,12 +339,12 @@ constexpr int kAccTiles = 16;
 constexpr int kAccRegions = kAccTiles * FWAVES;
 static_assert(kAccRegions <= WAVE, "region offsets are one wave's prefix sum");
 
-template <typename KT>
+template <typename KT, bool FULL>
 __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32_t (&s)[FJ][4]) {
 #pragma unroll
   for (int j = 0; j < FJ; ++j) {
     KT x[4];
-    load4<KT>((const KT*)a.fk, k0 + (uint64_t)j * 256, a.n, a.fk_vec != 0, x);
+    load4<KT, FULL>((const KT*)a.fk, k0 + (uint64_t)j * 256, a.n, a.fk_vec != 0, x);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       uint64_t o = key_off((int64_t)x[v], a.min);
@@ -357,8 +357,11 @@ __device__ __forceinline__ void fact_keys(const FactArgs& a, uint64_t k0, uint32
 // PW: byte width of the first predicate's column (0 without predicates), so
 // its values are held at their own width; bitmap words are read as 32-bit
 // words.  Both keep the kernel under 64 VGPRs (8 waves per SIMD).
-template <int PW, typename KT>
-__global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
+// FULL: the tile lies inside the table and its columns are aligned, so
+// every column load is a straight-line vector load and all of them are in
+// flight together.
+template <int PW, typename KT, bool FULL>
+__device__ __forceinline__ void fact_scan_tile(const FactArgs& a) {
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
   const uint64_t k0 = (uint64_t)blockIdx.x * FTILE + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
   // first predicate's column, loaded with the keys
@@ -368,10 +371,11 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
   if constexpr (PW != 0) {
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
-      load4<PT>((const PT*)a.preds[0].col, k0 + (uint64_t)j * 256, a.n, a.pred0_vec != 0, px[j]);
+      load4<PT, FULL>((const PT*)a.preds[0].col, k0 + (uint64_t)j * 256, a.n, a.pred0_vec != 0,
+                      px[j]);
   }
   uint32_t s[FJ][4];
-  fact_keys<KT>(a, k0, s);
+  fact_keys<KT, FULL>(a, k0, s);
   // rows in range and, when there is a predicate, passing it (its column
   // arrived with the keys): only these probe the parent bitmap
   uint32_t cand = 0;
@@ -379,7 +383,7 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      if (k0 + (uint64_t)j * 256 + v < a.n && s[j][v] != 0xFFFFFFFFu) {
+      if ((FULL || k0 + (uint64_t)j * 256 + v < a.n) && s[j][v] != 0xFFFFFFFFu) {
         if constexpr (PW != 0) {
           if (!pred_pass(enc_raw((uint64_t)px[j][v], a.preds[0].kind), a.preds[0])) continue;
         }
@@ -422,6 +426,14 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
     ++p;
   }
   if (lane == WAVE - 1) a.hit_count[(uint64_t)blockIdx.x * FWAVES + wave] = incl;
+}
+
+template <int PW, typename KT>
+__global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
+  const bool full = a.fk_vec && (PW == 0 || a.pred0_vec) &&
+                    (uint64_t)(blockIdx.x + 1) * FTILE <= a.n;  // block-uniform
+  if (full) fact_scan_tile<PW, KT, true>(a);
+  else fact_scan_tile<PW, KT, false>(a);
 }
 
 template <bool DBL>
