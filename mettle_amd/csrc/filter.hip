@@ -460,8 +460,8 @@ __device__ __forceinline__ void mark_offsets_wave(const FilterArgs& a,
 
 // The 16 values of lane l of a 1024-row wave segment of a 4-byte column
 // (rows k0 + 256 j + v, k0 = seg + 4 l).  FULL: the segment lies inside the
-// table and the column is 16-byte aligned, so each step is one dwordx4 with
-// no branch, and the loads of all columns go out back to back.
+// table and the column is 16-byte aligned, so each step is one streaming
+// dwordx4 with no branch, and the loads of all columns go out back to back.
 template <bool FULL>
 __device__ __forceinline__ void seg_load4(const uint32_t* col, uint64_t k0, uint64_t n,
                                           uint32_t (&x)[FJ][4]) {
@@ -469,7 +469,7 @@ __device__ __forceinline__ void seg_load4(const uint32_t* col, uint64_t k0, uint
   for (int j = 0; j < FJ; ++j) {
     const uint64_t k = k0 + (uint64_t)j * 256;
     if (FULL) {
-      const uint4 u = *(const uint4*)(col + k);
+      const uint4 u = ld_stream(col + k);
       x[j][0] = u.x; x[j][1] = u.y; x[j][2] = u.z; x[j][3] = u.w;
     } else {
 #pragma unroll
@@ -843,12 +843,13 @@ struct DenseArgs {
 // Clear the bits of the lane's 16 dense rows that fail p; T is the raw
 // storage type, KIND the predicate's encoding (fixed at compile time so
 // only this type's loads and compares are emitted).
-template <typename T, uint32_t KIND>
+template <typename T, uint32_t KIND, bool FULL>
 __device__ __forceinline__ void pred_dense_t(const PredC& p, uint64_t k0, uint64_t n, bool vec,
                                              uint32_t& bits) {
   T x[FJ][4];
 #pragma unroll
-  for (int j = 0; j < FJ; ++j) load4<T>((const T*)p.col, k0 + (uint64_t)j * 256, n, vec, x[j]);
+  for (int j = 0; j < FJ; ++j)
+    load4<T, FULL>((const T*)p.col, k0 + (uint64_t)j * 256, n, vec, x[j]);
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
@@ -856,14 +857,15 @@ __device__ __forceinline__ void pred_dense_t(const PredC& p, uint64_t k0, uint64
       if (!pred_pass(enc_raw((uint64_t)x[j][v], KIND), p)) bits &= ~(1u << (4 * j + v));
 }
 
+template <bool FULL>
 __device__ __forceinline__ void pred_dense_typed(const PredC& p, uint64_t k0, uint64_t n,
                                                  bool vec, uint32_t& bits) {
   switch (p.kind) {
-    case PK_I32: pred_dense_t<uint32_t, PK_I32>(p, k0, n, vec, bits); break;
-    case PK_F32: pred_dense_t<uint32_t, PK_F32>(p, k0, n, vec, bits); break;
-    case PK_U8: pred_dense_t<uint8_t, PK_U8>(p, k0, n, vec, bits); break;
-    case PK_I64: pred_dense_t<uint64_t, PK_I64>(p, k0, n, vec, bits); break;
-    default: pred_dense_t<uint64_t, PK_F64>(p, k0, n, vec, bits); break;
+    case PK_I32: pred_dense_t<uint32_t, PK_I32, FULL>(p, k0, n, vec, bits); break;
+    case PK_F32: pred_dense_t<uint32_t, PK_F32, FULL>(p, k0, n, vec, bits); break;
+    case PK_U8: pred_dense_t<uint8_t, PK_U8, FULL>(p, k0, n, vec, bits); break;
+    case PK_I64: pred_dense_t<uint64_t, PK_I64, FULL>(p, k0, n, vec, bits); break;
+    default: pred_dense_t<uint64_t, PK_F64, FULL>(p, k0, n, vec, bits); break;
   }
 }
 
@@ -880,8 +882,14 @@ __global__ __launch_bounds__(FBLOCK) void k_filter_dense(DenseArgs a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
-  for (uint32_t p = 0; p < a.n_preds; ++p)
-    pred_dense_typed(a.preds[p], k0, n, (a.vec_ok >> p) & 1u, bits);
+  // a tile inside the table with every column aligned: straight-line
+  // streaming loads (block-uniform choice)
+  const bool full = (uint64_t)(tile + 1) * FTILE <= n &&
+                    a.vec_ok == (a.n_preds >= 32 ? ~0u : (1u << a.n_preds) - 1u);
+  for (uint32_t p = 0; p < a.n_preds; ++p) {
+    if (full) pred_dense_typed<true>(a.preds[p], k0, n, true, bits);
+    else pred_dense_typed<false>(a.preds[p], k0, n, (a.vec_ok >> p) & 1u, bits);
+  }
   store_pass_bits(a.sel_bm, seg, bits, lane);
   const uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
   if (lane == 0) s_wave[wave] = c;
