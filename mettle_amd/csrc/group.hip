@@ -82,6 +82,11 @@ struct GroupArgs {
   const void* src_col[MTL_GROUP_MAX_AGGS];
   int32_t src_type[MTL_GROUP_MAX_AGGS];
   uint32_t src_conv[MTL_GROUP_MAX_AGGS];
+  // byte width of each source's values and of the first key's; sources past
+  // n_src read the first key column (always present) so every load of a
+  // batch can be issued without a guard
+  uint32_t src_w[MTL_GROUP_MAX_AGGS];
+  uint32_t key_w0;
   uint32_t W;
   uint32_t dense;
   uint64_t cap;     // dense: domain size; hash: power-of-two capacity
@@ -112,21 +117,6 @@ __device__ __forceinline__ uint64_t gfmix(uint64_t k) {
   return k;
 }
 
-__device__ __forceinline__ bool pack_key(const GroupArgs& a, uint64_t row, uint64_t& key) {
-  if (a.raw) {
-    key = (uint64_t)load_key(a.key_col[0], a.key_type[0], row);
-    return true;
-  }
-  uint64_t p = 0;
-  for (uint32_t i = 0; i < a.n_keys; ++i) {
-    uint64_t off = key_off(load_key(a.key_col[i], a.key_type[i], row), a.key_min[i]);
-    if (off >= a.key_span[i]) return false;
-    p += off * a.key_stride[i];
-  }
-  key = p;
-  return true;
-}
-
 __device__ __forceinline__ double gload_f64(const void* col, int32_t type, uint64_t row) {
   switch (type) {
     case MTL_FLOAT: return (double)((const float*)col)[row];
@@ -151,14 +141,75 @@ __device__ __forceinline__ void split_i64(int64_t v, int64_t& lo, int64_t& hi) {
   hi = (v >> 32) + (lo < 0 ? 1 : 0);
 }
 
-// The value a source gives for one row: int64 bits (integer aggregates),
-// double bits (K_SUMF), or the order-preserving encoding (K_MINF/K_MAXF).
-__device__ __forceinline__ uint64_t src_raw(const GroupArgs& a, uint32_t j, uint64_t row) {
-  switch (a.src_conv[j]) {
-    case SC_INT: return (uint64_t)load_key(a.src_col[j], a.src_type[j], row);
-    case SC_DBL: return (uint64_t)__double_as_longlong(gload_f64(a.src_col[j], a.src_type[j], row));
-    default: return enc_f64(gload_f64(a.src_col[j], a.src_type[j], row));
+// Raw bits of GE rows of a column of `width`-byte values.  The width switch
+// sits outside the row loop and nothing converts the loaded registers here,
+// so the GE loads go out back to back (a conversion right after each load
+// would make the compiler wait for it before issuing the next).
+template <int GE_>
+__device__ __forceinline__ void load_bits(const void* col, uint32_t width,
+                                          const uint64_t (&row)[GE_], uint64_t (&b)[GE_]) {
+  if (width == 8) {
+#pragma unroll
+    for (int e = 0; e < GE_; ++e) b[e] = ((const uint64_t*)col)[row[e]];
+  } else if (width == 4) {
+#pragma unroll
+    for (int e = 0; e < GE_; ++e) b[e] = ((const uint32_t*)col)[row[e]];
+  } else {
+#pragma unroll
+    for (int e = 0; e < GE_; ++e) b[e] = ((const uint8_t*)col)[row[e]];
   }
+}
+
+// load_key's value from the raw bits of a column of type t.
+__device__ __forceinline__ int64_t bits_key(uint64_t b, int32_t t) {
+  switch (t) {
+    case MTL_BIGINT:
+    case MTL_TIMESTAMP:
+    case MTL_DECIMAL:
+      return (int64_t)b;
+    case MTL_DICT8:
+      return (int64_t)(b & 0xFFu);
+    default:
+      return (int64_t)(int32_t)(uint32_t)b;
+  }
+}
+
+// gload_f64's value from the raw bits of a column of type t.
+__device__ __forceinline__ double bits_f64(uint64_t b, int32_t t) {
+  switch (t) {
+    case MTL_FLOAT: return (double)__uint_as_float((uint32_t)b);
+    case MTL_DOUBLE: return __longlong_as_double((long long)b);
+    default: return (double)bits_key(b, t);
+  }
+}
+
+// src_raw's value from the raw bits of source j.
+__device__ __forceinline__ uint64_t src_bits(const GroupArgs& a, uint32_t j, uint64_t b) {
+  switch (a.src_conv[j]) {
+    case SC_INT: return (uint64_t)bits_key(b, a.src_type[j]);
+    case SC_DBL: return (uint64_t)__double_as_longlong(bits_f64(b, a.src_type[j]));
+    default: return enc_f64(bits_f64(b, a.src_type[j]));
+  }
+}
+
+// pack_key with the first key column's raw bits already loaded (the other
+// key columns, if any, are loaded here).
+__device__ __forceinline__ bool pack_key_bits(const GroupArgs& a, uint64_t row, uint64_t b0,
+                                              uint64_t& key) {
+  if (a.raw) {
+    key = (uint64_t)bits_key(b0, a.key_type[0]);
+    return true;
+  }
+  uint64_t off = key_off(bits_key(b0, a.key_type[0]), a.key_min[0]);
+  if (off >= a.key_span[0]) return false;
+  uint64_t p = off * a.key_stride[0];
+  for (uint32_t i = 1; i < a.n_keys; ++i) {
+    off = key_off(load_key(a.key_col[i], a.key_type[i], row), a.key_min[i]);
+    if (off >= a.key_span[i]) return false;
+    p += off * a.key_stride[i];
+  }
+  key = p;
+  return true;
 }
 
 // v[s] for a wave-uniform source index s, without indexing registers
@@ -305,18 +356,22 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
       ok[e] = k < end;
       row[e] = ok[e] ? (a.sel ? (uint64_t)a.sel[k] : k) : 0;
     }
-    // the values do not wait for the key range checks: keys and values of
-    // all GE rows are in flight together (one round trip, not two)
+    // raw bits of the first key column and of every source for all GE rows
+    // in flight together (rows past the end read row 0), then conversions
+    uint64_t kb[GE], vb[NS][GE];
+    load_bits<GE>(a.key_col[0], a.key_w0, row, kb);
+#pragma unroll
+    for (uint32_t j = 0; j < (uint32_t)NS; ++j) load_bits<GE>(a.src_col[j], a.src_w[j], row, vb[j]);
     uint64_t v[GE][NS];
 #pragma unroll
     for (int e = 0; e < GE; ++e)
 #pragma unroll
       for (uint32_t j = 0; j < (uint32_t)NS; ++j)
-        v[e][j] = (ok[e] && j < a.n_src) ? src_raw(a, j, row[e]) : 0ull;
+        v[e][j] = (ok[e] && j < a.n_src) ? src_bits(a, j, vb[j][e]) : 0ull;
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
       key[e] = 0;
-      if (ok[e] && !pack_key(a, row[e], key[e])) {
+      if (ok[e] && !pack_key_bits(a, row[e], kb[e], key[e])) {
         atomicOr(a.err, 8u);
         ok[e] = false;
       }
@@ -888,6 +943,14 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
       ++a->n_src;
     }
     sl.src = j;
+  }
+  a->key_w0 = (uint32_t)type_width(a->key_type[0]);
+  for (uint32_t j = 0; j < MTL_GROUP_MAX_AGGS; ++j) {
+    if (j >= a->n_src) {  // unused: a valid column, never converted
+      a->src_col[j] = a->key_col[0];
+      a->src_type[j] = a->key_type[0];
+    }
+    a->src_w[j] = (uint32_t)type_width(a->src_type[j]);
   }
   if (!a->raw && D <= kDenseMax) {
     a->dense = 1;
