@@ -429,7 +429,7 @@ __device__ __forceinline__ void fact_scan_tile(const FactArgs& a) {
 }
 
 template <int PW, typename KT>
-__global__ __launch_bounds__(FBLOCK) void k_fact_scan(FactArgs a) {
+__global__ __launch_bounds__(FBLOCK, 8) void k_fact_scan(FactArgs a) {
   const bool full = a.fk_vec && (PW == 0 || a.pred0_vec) &&
                     (uint64_t)(blockIdx.x + 1) * FTILE <= a.n;  // block-uniform
   if (full) fact_scan_tile<PW, KT, true>(a);
