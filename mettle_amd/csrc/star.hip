@@ -390,19 +390,48 @@ __device__ __forceinline__ void fact_scan_tile(const FactArgs& a) {
         cand |= 1u << (4 * j + v);
       }
   uint32_t hits = 0;
-  {
-    const uint32_t* bm32 = (const uint32_t*)a.bm;
+  const uint32_t* bm32 = (const uint32_t*)a.bm;
+  uint32_t far = cand;  // candidates still to probe one by one
+  if (FULL) {
+    // Clustered keys (a fact table ordered by its parent): a lane's four
+    // consecutive rows of a step name one or two parents, so one two-word
+    // window at the word of the step's first candidate covers them all: two
+    // loads per step instead of four (the bitmap has a zero word past its
+    // end).  Rows outside the window fall back to their own load below.
+    uint32_t wb[FJ], lo[FJ], hi[FJ];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j) {
+      uint32_t s0 = 0;
+#pragma unroll
+      for (int v = 3; v >= 0; --v) s0 = ((cand >> (4 * j + v)) & 1u) ? s[j][v] : s0;
+      wb[j] = s0 >> 5;
+      lo[j] = ld_off32(bm32, wb[j] << 2);
+      hi[j] = ld_off32(bm32, (wb[j] + 1) << 2);
+    }
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t b = 1u << (4 * j + v);
+        const uint32_t d = s[j][v] - (wb[j] << 5);
+        if ((cand & b) && d < 64) {
+          far &= ~b;
+          if (((d < 32 ? lo[j] : hi[j]) >> (d & 31)) & 1u) hits |= b;
+        }
+      }
+  }
+  if (__ballot(far != 0)) {  // wave-uniform
     uint32_t w[FJ][4];
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(bm32, (s[j][v] >> 5) << 2) : 0u;
+        w[j][v] = ((far >> (4 * j + v)) & 1u) ? ld_off32(bm32, (s[j][v] >> 5) << 2) : 0u;
 #pragma unroll
     for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (s[j][v] & 31)) & 1u))
+        if (((far >> (4 * j + v)) & 1u) && ((w[j][v] >> (s[j][v] & 31)) & 1u))
           hits |= 1u << (4 * j + v);
   }
   for (uint32_t p = 1; p < a.n_preds && hits; ++p) pred_rows(a.preds[p], k0, hits);
@@ -799,7 +828,8 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     MTL_TRY(ar.alloc(sp.dwords * 8, &dim_bm));
     MTL_TRY(ar.alloc((uint64_t)filter_tiles(d->d_rows) * kTileRows / 8, &dbits));
   }
-  MTL_TRY(ar.alloc(sp.pwords * 8, &par_bm));
+  // one zero word past the end: the fact scan reads two-word windows
+  MTL_TRY(ar.alloc((sp.pwords + 1) * 8, &par_bm));
   MTL_TRY(ar.alloc(sp.pwords * 4, &par_rank));
   uint32_t ptiles = filter_tiles(prows);
   MTL_TRY(ar.alloc((uint64_t)ptiles * kTileRows / 8, &pbits));
@@ -832,7 +862,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
 
   ZeroList zl;
   if (dim_bm) zl.add((uint32_t*)dim_bm, sp.dwords * 2);
-  zl.add((uint32_t*)par_bm, sp.pwords * 2);
+  zl.add((uint32_t*)par_bm, (sp.pwords + 1) * 2);
   zl.add((uint32_t*)misc, 64);
   zl.add((uint32_t*)st_r, (uint64_t)std::max<uint32_t>(rtiles, 1) * 2);
   MTL_TRY(launch_zero(zl, st));

@@ -346,17 +346,23 @@ def test_star_parity_random(ops, scale, k):
     _star_parity(ops, tables, lambda p: datagen.star_desc(p, n_c, n_o, n_l, k=k))
 
 
-@pytest.mark.parametrize("shuffle_orders", [False, True])
-def test_star_parity_wide_dimension_span(ops, shuffle_orders):
+@pytest.mark.parametrize("shuffle_orders,shuffle_lines", [(False, False), (True, False),
+                                                           (False, True)])
+def test_star_parity_wide_dimension_span(ops, shuffle_orders, shuffle_lines):
     """Customer keys stretched x20 (span 3M): probes below 1,179,648 read the
     LDS copy of the bitmap, the rest read it in HBM.  Shuffled orders make
-    the parent's mark keys unsorted, so few lanes share a bitmap word."""
+    the parent's mark keys unsorted, so few lanes share a bitmap word;
+    shuffled lines make the fact scan's foreign keys unclustered, so its
+    two-word probe windows miss and rows probe one by one."""
     cust, orders, lines = datagen.star_tables(scale=0.1, workload=32)
     cust["c_key"] = cust["c_key"] * 20
     orders["o_cust"] = orders["o_cust"] * 20
     if shuffle_orders:
         perm = np.random.default_rng(7).permutation(len(orders["o_key"]))
         orders = {c: np.ascontiguousarray(v[perm]) for c, v in orders.items()}
+    if shuffle_lines:
+        perm = np.random.default_rng(8).permutation(len(lines["l_order"]))
+        lines = {c: np.ascontiguousarray(v[perm]) for c, v in lines.items()}
     tables = (cust, orders, lines)
     n_c, n_o, n_l = len(cust["c_key"]), len(orders["o_key"]), len(lines["l_order"])
 
