@@ -237,6 +237,7 @@ struct FactArgs {
   PredC preds[MTL_MAX_PREDS];
   uint32_t n_preds;
   mtl_expr expr;
+  uint32_t expr_w8;  // every expression column is 8 bytes wide (unused ones name column 0)
   uint64_t* acc;  // int64 or double bits
   uint32_t* cnt;
   uint32_t* err;
@@ -499,17 +500,39 @@ __global__ __launch_bounds__(FBLOCK) void k_fact_acc(FactArgs a) {
       const uint64_t at = (r0 + q) * (FJ * 256) + (e - s_off[q]);
       const uint32_t row = a.hit_row[at], sv = a.hit_slot[at];
       V cv[MTL_EXPR_MAX_COLS];
+      uint64_t w;
+      uint32_t rk;
+      if (a.expr_w8) {
+        // every column 8 bytes wide: the column loads and the slot lookup go
+        // out together, one round trip, converted afterwards
+        uint64_t raw[MTL_EXPR_MAX_COLS];
 #pragma unroll
-      for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
-        if ((uint32_t)c < a.expr.n_cols) {
-          if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
-          else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
-        } else {
-          cv[c] = 0;
+        for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c) raw[c] = ((const uint64_t*)a.expr.col[c])[row];
+        w = a.bm[sv >> 6];
+        rk = a.rank[sv >> 6];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c) {
+          if constexpr (DBL)
+            cv[c] = a.expr.col_type[c] == MTL_DOUBLE ? __longlong_as_double((long long)raw[c])
+                                                     : (double)(int64_t)raw[c];
+          else cv[c] = (int64_t)raw[c];
+          if ((uint32_t)c >= a.expr.n_cols) cv[c] = 0;
         }
-      const uint64_t w = a.bm[sv >> 6];
+      } else {
+#pragma unroll
+        for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
+          if ((uint32_t)c < a.expr.n_cols) {
+            if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
+            else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
+          } else {
+            cv[c] = 0;
+          }
+        w = a.bm[sv >> 6];
+        rk = a.rank[sv >> 6];
+      }
       const uint64_t below = (sv & 63) ? (w & ((1ull << (sv & 63)) - 1)) : 0ull;
-      r = a.rank[sv >> 6] + (uint32_t)__popcll(below);
+      r = rk + (uint32_t)__popcll(below);
       if constexpr (DBL) {
         val = expr_f64_vals(a.expr, cv);
       } else {
@@ -965,6 +988,14 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     for (uint32_t p = 0; p < d->f_n_preds; ++p) fa.preds[p] = compile_pred(d->f_preds[p]);
     fa.n_preds = d->f_n_preds;
     fa.expr = d->agg;
+    fa.expr_w8 = d->agg.n_cols > 0;
+    for (uint32_t c = 0; c < d->agg.n_cols && c < MTL_EXPR_MAX_COLS; ++c)
+      fa.expr_w8 &= type_width(d->agg.col_type[c]) == 8 ? 1u : 0u;
+    if (fa.expr_w8)
+      for (uint32_t c = d->agg.n_cols; c < MTL_EXPR_MAX_COLS; ++c) {
+        fa.expr.col[c] = d->agg.col[0];
+        fa.expr.col_type[c] = d->agg.col_type[0];
+      }
     fa.acc = (uint64_t*)acc;
     fa.cnt = (uint32_t*)cnt;
     fa.err = mu + 3;
