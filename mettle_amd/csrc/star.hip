@@ -215,6 +215,72 @@ __global__ void k_map(MapArgs a) {
   }
 }
 
+// 4-byte keys aligned for vector loads: a thread's 16 keys are one 64-byte
+// line, loaded whole (4 vector loads) only when one of its rows qualifies;
+// the slot lookups of its qualifying rows then go out four at a time (rows
+// picked from the key registers with a select chain, unused lookups read
+// word 0), so a thread waits about 1 + ceil(q / 4) round trips for q
+// qualifying rows instead of 2 q.
+__global__ __launch_bounds__(256) void k_map4(MapArgs a) {
+  const uint32_t M = *a.par_total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (M != a.pops[0]) atomicOr(a.err, 2u);                     // repeated parent key
+    if (a.has_dim && a.pops[1] != a.pops[2]) atomicOr(a.err, 4u);  // repeated dim key
+  }
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i = t0; i < M; i += nthreads) {
+    a.acc[i] = 0;
+    a.cnt[i] = 0;
+  }
+  const uint32_t* key = (const uint32_t*)a.key;
+  for (uint64_t g = t0; g * 16 < a.n_rows; g += nthreads) {
+    const uint64_t r0 = g * 16;
+    uint32_t m = (uint32_t)(a.sel_bm[r0 / 64] >> (r0 % 64)) & 0xFFFFu;
+    if (!m) continue;
+    uint32_t kv[16];
+    if (r0 + 16 <= a.n_rows) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint4 u = *(const uint4*)(key + r0 + 4 * q);
+        kv[4 * q] = u.x; kv[4 * q + 1] = u.y; kv[4 * q + 2] = u.z; kv[4 * q + 3] = u.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) kv[i] = r0 + i < a.n_rows ? key[r0 + i] : 0u;
+    }
+    while (m) {
+      uint32_t sl[4], rw[4], rk[4];
+      uint64_t bw[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        sl[q] = 0xFFFFFFFFu;
+        rw[q] = 0;
+        if (m) {
+          const int i = __ffs(m) - 1;
+          m &= m - 1;
+          uint32_t kk = kv[0];
+#pragma unroll
+          for (int t = 1; t < 16; ++t) kk = i == t ? kv[t] : kk;
+          const uint64_t o = key_off((int64_t)(int32_t)kk, a.min);
+          if (o < a.span) sl[q] = (uint32_t)o;  // else flagged by the parent pass
+          rw[q] = (uint32_t)(r0 + (uint64_t)i);
+        }
+        const uint32_t wi = sl[q] != 0xFFFFFFFFu ? sl[q] >> 6 : 0u;
+        bw[q] = a.bm[wi];
+        rk[q] = a.rank[wi];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (sl[q] == 0xFFFFFFFFu) continue;
+        const uint32_t o = sl[q];
+        const uint64_t below = (o & 63) ? (bw[q] & ((1ull << (o & 63)) - 1)) : 0ull;
+        a.row_of[rk[q] + (uint32_t)__popcll(below)] = rw[q];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Expression program (postfix), evaluated only for qualifying fact rows.
 // ---------------------------------------------------------------------------
@@ -972,7 +1038,11 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ma.err = mu + 3;
     uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
     ProfScope ps(s, "k_map");
-    hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
+    if (type_width(d->p_key_type) == 4 && d->p_key_type != MTL_FLOAT &&
+        ((uintptr_t)d->p_key % 16) == 0)
+      hipLaunchKernelGGL(k_map4, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
+    else
+      hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
     MTL_HIP(hipGetLastError());
   }
   // 5. fact pass
