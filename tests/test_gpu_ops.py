@@ -374,6 +374,39 @@ def test_star_parity_wide_dimension_span(ops, shuffle_orders, shuffle_lines):
     assert len(rows) == 50
 
 
+@pytest.mark.parametrize("variant", ["float_pred", "no_pred", "negative_keys"])
+def test_star_parity_parent_pass_variants(ops, variant):
+    """The persistent parent pass's other instantiations (1.5M-row parents):
+    a FLOAT parent predicate, no parent predicate, and keys below zero (the
+    64-bit key-offset path instead of the 32-bit one)."""
+    cust, orders, lines = datagen.star_tables(scale=0.1, workload=33)
+    shift = 0
+    if variant == "float_pred":
+        orders["o_df"] = orders["o_date"].astype(np.float32) + np.float32(0.25)
+    if variant == "negative_keys":
+        shift = -1_000_000_000
+        cust["c_key"] = cust["c_key"] + shift
+        orders["o_cust"] = orders["o_cust"] + shift
+        orders["o_key"] = orders["o_key"] + shift
+        lines["l_order"] = lines["l_order"] + shift
+    tables = (cust, orders, lines)
+    n_c, n_o, n_l = len(cust["c_key"]), len(orders["o_key"]), len(lines["l_order"])
+
+    def build(p):
+        d = datagen.star_desc(p, n_c, n_o, n_l, k=20)
+        if variant == "float_pred":
+            d.p_preds[0] = abi.make_pred(p("orders", "o_df"), abi.FLOAT, abi.CMP_LT,
+                                         float(datagen.STAR_CUTOFF))
+        elif variant == "no_pred":
+            d.p_n_preds = 0
+        else:
+            d.d_key_min, d.d_key_max = 1 + shift, n_c + shift
+            d.p_key_min, d.p_key_max = 1 + shift, n_o + shift
+        return d
+    rows = _star_parity(ops, tables, build)
+    assert len(rows) == 20
+
+
 def test_star_no_qualifying_rows(ops):
     tables = datagen.star_tables(scale=0.001, workload=31)
     n_c, n_o, n_l = (len(tables[0]["c_key"]), len(tables[1]["o_key"]),
