@@ -183,7 +183,16 @@ struct PredC {
   uint32_t kind;    // PK_*
   uint32_t opmask;  // bit 0: pass if value < literal, bit 1: ==, bit 2: >
   uint64_t lit;     // encoded literal
+  // PK_I32: the same test as an int32 interval, pass iff
+  // (x - lo32 <= hi32 - lo32, unsigned) != neg32 (an empty interval is the
+  // full one negated)
+  int32_t lo32, hi32;
+  uint32_t neg32;
 };
+
+__device__ __forceinline__ bool pred_pass_i32(uint32_t x, const PredC& p) {
+  return ((x - (uint32_t)p.lo32) <= ((uint32_t)p.hi32 - (uint32_t)p.lo32)) != (p.neg32 != 0);
+}
 
 __host__ __device__ __forceinline__ uint64_t enc_int(int64_t v) {
   return (uint64_t)v ^ 0x8000000000000000ull;

@@ -46,6 +46,28 @@ inline dev::PredC compile_pred(const mtl_pred& p) {
   if (p.type == MTL_FLOAT) c.lit = dev::enc_dbl((double)(float)d);
   else if (p.type == MTL_DOUBLE) c.lit = dev::enc_dbl(d);
   else c.lit = dev::enc_int(p.scalar);
+  // int32 interval of the passing values (EQ [L, L], LT [min, L-1], LE
+  // [min, L], GT [L+1, max], GE [L, max]), clamped to int32
+  const int64_t L = p.scalar, I32MIN = INT32_MIN, I32MAX = INT32_MAX;
+  int64_t lo = I32MIN, hi = I32MAX;
+  switch (p.op) {
+    case 0: lo = L; hi = L; break;
+    case 1: hi = L == INT64_MIN ? INT64_MIN : L - 1; break;
+    case 2: hi = L; break;
+    case 3: lo = L == INT64_MAX ? INT64_MAX : L + 1; break;
+    default: lo = L; break;
+  }
+  lo = lo < I32MIN ? I32MIN : lo;
+  hi = hi > I32MAX ? I32MAX : hi;
+  if (lo > hi) {  // nothing passes
+    c.lo32 = INT32_MIN;
+    c.hi32 = INT32_MAX;
+    c.neg32 = 1;
+  } else {
+    c.lo32 = (int32_t)lo;
+    c.hi32 = (int32_t)hi;
+    c.neg32 = 0;
+  }
   return c;
 }
 
