@@ -117,14 +117,6 @@ __device__ __forceinline__ uint64_t gfmix(uint64_t k) {
   return k;
 }
 
-__device__ __forceinline__ double gload_f64(const void* col, int32_t type, uint64_t row) {
-  switch (type) {
-    case MTL_FLOAT: return (double)((const float*)col)[row];
-    case MTL_DOUBLE: return ((const double*)col)[row];
-    default: return (double)load_key(col, type, row);
-  }
-}
-
 // The exact 128-bit integer sum held as two signed 64-bit sums X (of the
 // sign-extended low halves) and Y (of the high parts): value = X + Y * 2^32,
 // returned as (low 64 bits, high 64 bits) of the 128-bit value.
@@ -160,7 +152,7 @@ __device__ __forceinline__ void load_bits(const void* col, uint32_t width,
   }
 }
 
-// load_key's value from the raw bits of a column of type t.
+// The int64 value of the raw bits of an integer column of type t.
 __device__ __forceinline__ int64_t bits_key(uint64_t b, int32_t t) {
   switch (t) {
     case MTL_BIGINT:
@@ -174,7 +166,7 @@ __device__ __forceinline__ int64_t bits_key(uint64_t b, int32_t t) {
   }
 }
 
-// gload_f64's value from the raw bits of a column of type t.
+// A double from the raw bits of a column of type t (FLOAT, DOUBLE or integer).
 __device__ __forceinline__ double bits_f64(uint64_t b, int32_t t) {
   switch (t) {
     case MTL_FLOAT: return (double)__uint_as_float((uint32_t)b);
@@ -183,7 +175,9 @@ __device__ __forceinline__ double bits_f64(uint64_t b, int32_t t) {
   }
 }
 
-// src_raw's value from the raw bits of source j.
+// The value source j gives for a row, from its raw bits: int64 bits
+// (integer aggregates), double bits (K_SUMF), or the order-preserving
+// encoding (K_MINF/K_MAXF).
 __device__ __forceinline__ uint64_t src_bits(const GroupArgs& a, uint32_t j, uint64_t b) {
   switch (a.src_conv[j]) {
     case SC_INT: return (uint64_t)bits_key(b, a.src_type[j]);
@@ -192,8 +186,10 @@ __device__ __forceinline__ uint64_t src_bits(const GroupArgs& a, uint32_t j, uin
   }
 }
 
-// pack_key with the first key column's raw bits already loaded (the other
-// key columns, if any, are loaded here).
+// The group key of a row: the key columns packed by their [min, max] ranges
+// (a single raw key is used as is); false when a key lies outside its range.
+// The first key column's raw bits come loaded; other key columns, if any,
+// are loaded here.
 __device__ __forceinline__ bool pack_key_bits(const GroupArgs& a, uint64_t row, uint64_t b0,
                                               uint64_t& key) {
   if (a.raw) {
