@@ -1096,8 +1096,8 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
       const uint32_t pw = d->f_n_preds ? pred_width(fa.preds[0].kind) : 0u;
       const int kw = d->f_fk_type == MTL_DICT8 ? 1 : type_width(d->f_fk_type) == 8 ? 8 : 4;
       // 4-byte keys in [0, 2^31) and an integer 4-byte predicate (or none):
-      // the 32-bit instantiation (about a fifth fewer VALU instructions per
-      // row; the pass is VALU-heavy)
+      // the 32-bit instantiation (32-bit key offsets and predicate compares;
+      // 58 VGPRs without spills against 64 with)
       const bool fast = kw == 4 && d->p_key_min >= 0 &&
                         (uint64_t)d->p_key_min + sp.pspan <= (1ull << 31) &&
                         (pw == 0 || (pw == 4 && fa.preds[0].kind == PK_I32));
