@@ -711,8 +711,14 @@ __device__ __forceinline__ GRaw gtopk_load(const GTopkArgs& a, uint64_t g) {
   return r;
 }
 
+// Key words: k0 carries the order on its own wherever it can (the lists and
+// their bounds compare k0 first, and a tie on k0 reads the other words from
+// global memory), k1 and k2 complete it, k3 is the group key (ascending).
+// An exact 128-bit SUM is (int64 saturation of the value, high 64 bits,
+// low 64 bits): the saturated word orders every sum that fits int64, and the
+// two exact words order the rest.
 __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, const GRaw& r, uint64_t k[4]) {
-  uint64_t k0, k1 = 0;
+  uint64_t k0, k1 = 0, k2 = 0;
   switch (a.kind) {
     case K_COUNT: k0 = enc_i64((int64_t)r.cnt); break;
     case K_SUMI: {
@@ -724,8 +730,11 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, const GRaw& r, ui
         double v = ((double)hi * 18446744073709551616.0 + (double)lo) / (double)r.cnt;
         k0 = enc_f64(v);
       } else {
-        k0 = enc_i64(hi);
-        k1 = lo;
+        const bool fits = hi == ((int64_t)lo >> 63);  // hi is lo's sign extension
+        const int64_t sat = fits ? (int64_t)lo : (hi < 0 ? INT64_MIN : INT64_MAX);
+        k0 = enc_i64(sat);
+        k1 = enc_i64(hi);
+        k2 = lo;
       }
       break;
     }
@@ -741,11 +750,12 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, const GRaw& r, ui
   if (a.desc) {
     k0 = ~k0;
     k1 = ~k1;
+    k2 = ~k2;
   }
   k[0] = k0;
   k[1] = k1;
-  k[2] = a.raw ? enc_i64((int64_t)r.key) : r.key;
-  k[3] = 0;
+  k[2] = k2;
+  k[3] = a.raw ? enc_i64((int64_t)r.key) : r.key;
 }
 
 // Phase 1: GE_TK groups per thread per round, their words loaded together
@@ -780,7 +790,17 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_group_topk(GTopkArgs a) {
         topk_min2(kk[e][0], m1, m2);
       }
     }
-    if (R <= 2 && base == start) topk_seed_thr(m1, m2, a.k, sl);  // bound from round 1
+    if (base == start) {  // the bound from round 1: the block's exact K-th key
+      __shared__ TopkSelectLds sq;
+      uint64_t k0s[GE_TK];
+      bool val[GE_TK];
+#pragma unroll
+      for (int e = 0; e < GE_TK; ++e) {
+        k0s[e] = kk[e][0];
+        val[e] = base + (uint64_t)e * S::NT + threadIdx.x < end && r[e].cnt != 0;
+      }
+      topk_seed_select<S::NT>(k0s, val, a.k, sq, sl);
+    }
     const uint64_t bthr =
         __hip_atomic_load(&sl.thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll

@@ -279,6 +279,74 @@ __device__ __forceinline__ void topk_seed_thr(uint64_t lmin1, uint64_t lmin2, ui
   __syncthreads();
 }
 
+// Seed the block bound with the exact K-th smallest key word 0 among the
+// candidates the block holds (v[e] where valid[e]): since those are K keys
+// no worse than it, nothing above it can reach the top K.  A tighter bound
+// than topk_seed_thr's (lane minima), for scans of many candidates per
+// block, where each candidate that passes a loose bound costs a wave-serial
+// insertion.  MSD radix select, 8 digits of 8 bits: per digit a 256-bin LDS
+// histogram of the candidates still matching the chosen prefix, and one
+// wave finds the digit that holds the rank.  Every thread of the block calls
+// it (it ends with a barrier).
+struct TopkSelectLds {
+  uint32_t hist[256];
+  uint32_t digit, rank, total;
+};
+
+template <int NT, int GE, int W, int CAP>
+__device__ __forceinline__ void topk_seed_select(const uint64_t (&v)[GE], const bool (&valid)[GE],
+                                                 uint32_t K, TopkSelectLds& q,
+                                                 TopkBlockLds<W, CAP>& s) {
+  const int tid = threadIdx.x, lane = __lane_id();
+  uint64_t prefix = 0;
+  uint32_t rank = K;  // 1-based rank still to find among the matching keys
+  for (int d = 7; d >= 0; --d) {
+    const int sh = d * 8;
+    const uint64_t hi_mask = d == 7 ? 0ull : (~0ull << (sh + 8));
+    for (int i = tid; i < 256; i += NT) q.hist[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < GE; ++e)
+      if (valid[e] && (v[e] & hi_mask) == prefix) atomicAdd(&q.hist[(v[e] >> sh) & 255u], 1u);
+    __syncthreads();
+    if (tid < 64) {
+      // lane l holds bins 4l .. 4l+3: inclusive prefix over the lanes
+      uint32_t c[4], sum = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        c[b] = q.hist[4 * lane + b];
+        sum += c[b];
+      }
+      uint32_t incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+      }
+      if (lane == 63) q.total = incl;
+      uint32_t before = incl - sum;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (before < rank && rank <= before + c[b]) {
+          q.digit = (uint32_t)(4 * lane + b);
+          q.rank = rank - before;
+        }
+        before += c[b];
+      }
+    }
+    __syncthreads();
+    if (q.total < rank) {  // fewer than K candidates: no bound (block-uniform)
+      __syncthreads();
+      return;
+    }
+    prefix |= (uint64_t)q.digit << sh;
+    rank = q.rank;
+    __syncthreads();  // q is rewritten by the next digit
+  }
+  if (tid == 0 && prefix < s.thr) atomicMin((unsigned long long*)&s.thr, (unsigned long long)prefix);
+  __syncthreads();
+}
+
 // Running smallest and second-smallest of a lane's candidate keys.
 __device__ __forceinline__ void topk_min2(uint64_t k0, uint64_t& m1, uint64_t& m2) {
   if (k0 < m1) {
