@@ -235,7 +235,7 @@ def scan_workload(args) -> dict:
                 f"filter then project {ms_sep:.3f} ms")
     return {"metric": "rows/s & % HBM roofline (scan-filter-project)",
             "value": min(r["rows_per_s"] for r in rows), "unit": "rows/s",
-            "config": {"workload": "scan-filter-project 100M rows (config 2), "
+            "config": {"workload": "scan-filter-project 100M rows, "
                                    "col < literal + projection col + 1", "rows": n},
             "variants": rows, "dtype": "int32/int64", "data": "synthetic"}
 
@@ -269,7 +269,7 @@ def group_workload(args, dense: bool) -> dict:
     log(f"group ({'1000 keys' if dense else 'Zipf 10M keys'}): {ms:.3f} ms, {gbs:.0f} GB/s")
     return {"metric": "rows/s & % HBM roofline (grouped aggregate + top-100)",
             "value": n / (ms * 1e-3), "unit": "rows/s", "ms_per_step": ms,
-            "config": {"workload": "grouped aggregate 200M rows (config 4), "
+            "config": {"workload": "grouped aggregate 200M rows, "
                                    + ("g uniform over 1000 keys" if dense else
                                       "g Zipf(1.1) over 10M keys"), "rows": n},
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -329,7 +329,7 @@ def main() -> int:
         return copy_calibration()
     if args.workload != "star":
         if args.gpus != 1:
-            print(json.dumps({"error": "this build runs on one GPU"}))
+            print(json.dumps({"error": "this library runs on one GPU"}))
             return 2
         if args.workload == "scan":
             r = scan_workload(args)
@@ -340,8 +340,8 @@ def main() -> int:
         print(json.dumps(r), flush=True)
         return 0
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "this library runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is designed but not run)"}))
         return 2
 
     import torch
@@ -450,7 +450,7 @@ def main() -> int:
         "dtype": "int64",
         "data": "synthetic (numpy PCG64 seed 20261203+3), columns resident in HBM",
         "config": {"workload": "join->group-agg->top-10: customers 1.5M, orders 15M, "
-                               f"lines {n_l} (config 3)",
+                               f"lines {n_l}",
                    "rows": in_rows, "scale": args.scale, "k": 10},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

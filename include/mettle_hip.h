@@ -489,8 +489,8 @@ int32_t mtl_scalar_hook(int32_t hook, int32_t type, const void *values,
 /* 103-143) and stage_query (state.rs:42-54), which ran every query       */
 /* synchronously inside bind with no overlap and no admission control.    */
 /* ====================================================================== */
-#define MTL_Q_SCAN 0 /* filter -> optional projection (configs 1-2)    */
-#define MTL_Q_STAR 1 /* star join -> SUM -> top-K (config 3)            */
+#define MTL_Q_SCAN 0 /* filter -> optional projection (plain scans)    */
+#define MTL_Q_STAR 1 /* star join -> SUM -> top-K (TPC-H Q3 shape)     */
 
 typedef struct mtl_query {
   int32_t kind; /* MTL_Q_*                                                */

@@ -790,7 +790,7 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_group_topk(GTopkArgs a) {
         topk_min2(kk[e][0], m1, m2);
       }
     }
-    if (base == start) {  // the bound from round 1: the block's exact K-th key
+    if (base == start) {  // the bound from the first pass: the block's exact K-th key
       __shared__ TopkSelectLds sq;
       uint64_t k0s[GE_TK];
       bool val[GE_TK];

@@ -1,6 +1,6 @@
 // Star join -> grouped SUM -> top-K, device resident end to end.
 //
-// The shape is TPC-H Q3 (and BASELINE config 3): a dimension D filtered to a
+// The shape is TPC-H Q3 (customers / orders / lines): a dimension D filtered to a
 // key set, a parent P filtered and semi-joined to D, a fact F filtered and
 // joined to P, SUM(expr) grouped by the P row, ORDER BY sum DESC + ties,
 // LIMIT K.  Reference pipeline: execute_gpu_native

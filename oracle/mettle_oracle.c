@@ -1,7 +1,7 @@
 /*
  * mettle_oracle.c — CPU restatement of the mettle hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library:
+ * TEST INFRASTRUCTURE ONLY.  This file computes expected results for the HIP library:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it, and only to compute expected results or a CPU timing.  The
  * product path (libmettle_hip.so and mettle_amd/) never calls it.

@@ -1,7 +1,7 @@
 """Python access to the CPU oracle (oracle/mettle_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker.  Nothing under
+cpu_baseline leg of bench.py, always as the source of expected results.  Nothing under
 mettle_amd/ imports this module.
 
 All pointers handed to the oracle are host (numpy) pointers.

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker.
+"""Pins the CPU oracle before its results are trusted as expected values.
 
 Each function of oracle/mettle_oracle.c is checked against something it does
 not restate: the reference's own fixtures (DuckDB-produced rows in
