@@ -292,6 +292,9 @@ typedef struct mtl_agg_spec {
   int32_t func;       /* MTL_AGG_*                                        */
   int32_t type;       /* value column type (ignored when values == NULL)  */
   const void *values; /* device column; NULL = COUNT(*)                   */
+  int32_t scale;      /* MTL_DECIMAL: digits after the point, 0..18; AVG  */
+                      /* divides by count * 10^scale (DuckDB's DECIMAL    */
+                      /* AVG -> DOUBLE); ignored for other types          */
 } mtl_agg_spec;
 
 typedef struct mtl_group_desc {

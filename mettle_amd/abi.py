@@ -82,7 +82,8 @@ AGG_SUM, AGG_COUNT, AGG_MIN, AGG_MAX, AGG_AVG = range(5)
 
 
 class AggSpec(C.Structure):
-    _fields_ = [("func", C.c_int32), ("type", C.c_int32), ("values", C.c_void_p)]
+    _fields_ = [("func", C.c_int32), ("type", C.c_int32), ("values", C.c_void_p),
+                ("scale", C.c_int32)]
 
 
 class GroupDesc(C.Structure):
@@ -109,7 +110,8 @@ def agg_result_dtype(func: int, type_code: int):
 
 
 def make_group_desc(n_rows, keys, aggs, sel=None, sel_count=None, sel_capacity=0):
-    """keys: list of (ptr, type, min, max); aggs: list of (func, ptr, type)."""
+    """keys: list of (ptr, type, min, max); aggs: list of (func, ptr, type)
+    or (func, ptr, type, scale) -- scale: DECIMAL digits after the point."""
     d = GroupDesc()
     d.n_rows = n_rows
     d.sel = sel
@@ -122,8 +124,9 @@ def make_group_desc(n_rows, keys, aggs, sel=None, sel_count=None, sel_capacity=0
         d.key_min[i] = int(mn)
         d.key_max[i] = int(mx)
     d.n_aggs = len(aggs)
-    for i, (f, p, t) in enumerate(aggs):
-        d.aggs[i] = AggSpec(f, t, p)
+    for i, a in enumerate(aggs):
+        f, p, t = a[:3]
+        d.aggs[i] = AggSpec(f, t, p, a[3] if len(a) > 3 else 0)
     return d
 
 

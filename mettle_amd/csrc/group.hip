@@ -687,7 +687,8 @@ struct GTopkArgs {
   uint64_t cap;
   uint32_t kind, word;
   int32_t desc;
-  int32_t avg;  // order by sum / count
+  int32_t avg;         // order by sum / count
+  uint64_t avg_scale;  // AVG(DECIMAL): the count's factor 10^scale (else 1)
   uint32_t k;
   TopkStage stage;    // full keys of every wave-list entry
   TopkLists lists0;   // per-block lists written by phase 1
@@ -711,6 +712,70 @@ __device__ __forceinline__ GRaw gtopk_load(const GTopkArgs& a, uint64_t g) {
   return r;
 }
 
+// DuckDB's AVG over an integer column (IntegerAverageOperationHugeint): the
+// 128-bit sum converted to long double (64-bit significand, round to nearest
+// even), divided by the long double divisor (the count, times 10^scale for
+// a DECIMAL), the quotient rounded to a 64-bit significand and then to
+// double.  Emulated here with integer arithmetic, so the top-K key of a
+// group is bit for bit the AVG the host emits with x87 long double (the
+// two roundings included); div = count * 10^scale < 2^64 is exact.
+__device__ __forceinline__ double avg_ld(uint64_t lo, int64_t hi, uint64_t div) {
+  const bool neg = hi < 0;
+  uint64_t al = lo, ah = (uint64_t)hi;
+  if (neg) {  // |sum| as two words
+    al = ~lo + 1;
+    ah = ~(uint64_t)hi + (al == 0 ? 1 : 0);
+  }
+  if ((ah | al) == 0) return 0.0;
+  // |sum| rounded to a 64-bit significand: m * 2^e
+  uint64_t m;
+  int e;
+  if (ah == 0) {
+    m = al;
+    e = 0;
+  } else {
+    const int sh = 64 - __clzll((long long)ah);  // bits of ah, 1..64
+    m = sh == 64 ? ah : ((ah << (64 - sh)) | (al >> sh));
+    const uint64_t drop = sh == 64 ? al : (al & ((1ull << sh) - 1));
+    const uint64_t half = 1ull << (sh - 1);
+    e = sh;
+    if (drop > half || (drop == half && (m & 1))) {
+      if (++m == 0) {
+        m = 1ull << 63;
+        ++e;
+      }
+    }
+  }
+  // quotient m / div to a 64-bit significand, a round bit and a sticky bit
+  uint64_t acc = m / div, r = m % div;
+  int x = e;
+  auto next_bit = [&]() -> uint64_t {
+    // r < div < 2^64: 2r may carry out of 64 bits, and is then >= div
+    const bool carry = (r >> 63) != 0;
+    const uint64_t r2 = r << 1;
+    const uint64_t bit = (carry || r2 >= div) ? 1u : 0u;
+    r = bit ? r2 - div : r2;
+    return bit;
+  };
+  while (!(acc >> 63)) {
+    acc = (acc << 1) | next_bit();
+    --x;
+  }
+  const uint64_t g = next_bit();
+  if (g && (r != 0 || (acc & 1))) {
+    if (++acc == 0) {
+      acc = 1ull << 63;
+      ++x;
+    }
+  }
+  // long double -> double: round the 64-bit significand to 53 bits
+  uint64_t d = acc >> 11;
+  const uint64_t low = acc & 0x7FF;
+  if (low > 0x400 || (low == 0x400 && (d & 1))) ++d;
+  const double v = ldexp((double)d, x + 11);
+  return neg ? -v : v;
+}
+
 // Key words: k0 carries the order on its own wherever it can (the lists and
 // their bounds compare k0 first, and a tie on k0 reads the other words from
 // global memory), k1 and k2 complete it, k3 is the group key (ascending).
@@ -727,8 +792,7 @@ __device__ __forceinline__ void gtopk_keys(const GTopkArgs& a, const GRaw& r, ui
       int64_t hi;
       sum128(w, lo, hi);
       if (a.avg) {
-        double v = ((double)hi * 18446744073709551616.0 + (double)lo) / (double)r.cnt;
-        k0 = enc_f64(v);
+        k0 = enc_f64(avg_ld(lo, hi, r.cnt * a.avg_scale));
       } else {
         const bool fits = hi == ((int64_t)lo >> 63);  // hi is lo's sign extension
         const int64_t sat = fits ? (int64_t)lo : (hi < 0 ? INT64_MIN : INT64_MAX);
@@ -846,11 +910,19 @@ struct mtl_groups {
   void* st_mem = nullptr;
   int32_t agg_func[MTL_GROUP_MAX_AGGS];
   int32_t agg_type[MTL_GROUP_MAX_AGGS];
+  int32_t agg_scale[MTL_GROUP_MAX_AGGS];  // DECIMAL digits (AVG divides by 10^scale)
 };
 
 namespace mtl {
 
 static bool is_float(int32_t t) { return t == MTL_FLOAT || t == MTL_DOUBLE; }
+
+// 10^s as DuckDB's AverageDecimalBindData holds it: a double, widened.
+static long double pow10_ld(int32_t s) {
+  double p = 1.0;
+  for (int32_t i = 0; i < s; ++i) p *= 10.0;
+  return (long double)p;
+}
 
 static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entries) {
   if (!d) return set_error(MTL_ERR_INVALID_ARGUMENT, "desc is NULL");
@@ -1065,6 +1137,7 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   for (uint32_t i = 0; i < desc->n_aggs; ++i) {
     g->agg_func[i] = desc->aggs[i].func;
     g->agg_type[i] = desc->aggs[i].type;
+    g->agg_scale[i] = desc->aggs[i].type == MTL_DECIMAL ? desc->aggs[i].scale : 0;
   }
   hipError_t e = hipMallocAsync(&g->st_mem, entries * a.W * 8 + 8, s->stream);
   if (e == hipSuccess && !a.dense) e = hipMallocAsync(&g->keys_mem, entries * 8, s->stream);
@@ -1233,9 +1306,11 @@ static int32_t emit_groups(mtl_groups* g, const uint32_t* idx, const uint32_t* c
           break;
         case MTL_AGG_AVG:
           if (sl.kind == K_SUMI) {
-            // hugeint sum / count in long double, as DuckDB's integer AVG
+            // hugeint sum / (count * 10^scale) in long double, as DuckDB's
+            // integer and DECIMAL AVG
             __int128 sum = (__int128)(((unsigned __int128)v[1] << 64) | v[0]);
-            ((double*)agg_out[k])[i] = (double)((long double)sum / (long double)cnt[i]);
+            ((double*)agg_out[k])[i] = (double)((long double)sum /
+                                                ((long double)cnt[i] * pow10_ld(g->agg_scale[k])));
           } else {
             double d;
             memcpy(&d, &v[0], 8);
@@ -1356,6 +1431,15 @@ int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, i
   t.word = a.aggs[agg].word;
   t.desc = desc;
   t.avg = g->agg_func[agg] == MTL_AGG_AVG;
+  t.avg_scale = 1;
+  if (t.avg && t.kind == K_SUMI) {
+    // the divisor count * 10^scale must stay an exact integer below 2^64
+    // (count < 2^32): DECIMAL scales up to 9
+    if (g->agg_scale[agg] < 0 || g->agg_scale[agg] > 9)
+      return set_error(MTL_UNSUPPORTED, "ORDER BY AVG of a DECIMAL with scale %d",
+                       g->agg_scale[agg]);
+    for (int i = 0; i < g->agg_scale[agg]; ++i) t.avg_scale *= 10;
+  }
   t.k = k;
   for (int f = 0; f < 4; ++f) t.stage.ck[f] = (uint64_t*)ck[f];
   t.stage.cid = (uint32_t*)cid;

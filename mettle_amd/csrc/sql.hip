@@ -971,6 +971,7 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
         a.values = c->dev;
         a.type = c->type;
         scale = c->scale;
+        a.scale = c->type == MTL_DECIMAL ? c->scale : 0;
       } else if (a.func != MTL_AGG_COUNT) {
         return set_error(MTL_UNSUPPORTED, "%s(*)", f.c_str());
       }
@@ -1002,6 +1003,8 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
     if (order_agg < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY must name an aggregate");
     odesc = q.order[0].desc;
     if (q.limit < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY an aggregate needs LIMIT");
+    if (q.limit > 0xFFFFFFFFll)
+      return set_error(MTL_UNSUPPORTED, "LIMIT %lld exceeds 2^32-1", (long long)q.limit);
   }
   res.shape = "group";
   if (dry) return MTL_OK;
@@ -1079,7 +1082,7 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
       else col.type = MTL_BIGINT;
       if ((f == MTL_AGG_SUM || f == MTL_AGG_MIN || f == MTL_AGG_MAX) && t == MTL_DECIMAL)
         col.scale = agg_scale[o.agg];
-      if (f == MTL_AGG_AVG && t == MTL_DECIMAL) col.scale = 0;  // value already divided
+      if (f == MTL_AGG_AVG) col.scale = 0;  // a DOUBLE: the group engine divided by 10^scale
       col.data.assign(abuf[o.agg].begin(), abuf[o.agg].begin() + ng * w);
     }
     res.cols.push_back(std::move(col));

@@ -202,15 +202,15 @@ class Groups:
 def group_agg(keys, aggs, n_rows: int, sel: torch.Tensor | None = None,
               sel_count: torch.Tensor | None = None) -> Groups:
     """keys: list of (device column, type, min, max); aggs: list of
-    (func, device column | None, type)."""
+    (func, device column | None, type[, DECIMAL scale])."""
     torch.cuda.synchronize()
     d = abi.make_group_desc(n_rows, [(ptr(c), t, mn, mx) for c, t, mn, mx in keys],
-                            [(f, ptr(c), t) for f, c, t in aggs], sel=ptr(sel),
+                            [(a[0], ptr(a[1])) + tuple(a[2:]) for a in aggs], sel=ptr(sel),
                             sel_count=ptr(sel_count),
                             sel_capacity=0 if sel is None else sel.numel())
     h = C.c_void_p()
     abi.check(abi.lib().mtl_group_agg(C.byref(d), C.byref(h), None))
-    return Groups(h, len(keys), [(f, t) for f, _, t in aggs])
+    return Groups(h, len(keys), [(a[0], a[2]) for a in aggs])
 
 
 def star_topk(desc: abi.StarDesc, stream=None):

@@ -423,7 +423,14 @@ int64_t orc_group_agg(const mtl_group_desc *d, int64_t *keys_out, void *const *a
             ((uint64_t *)agg_out[a])[2 * ng] = (uint64_t)s;
             ((int64_t *)agg_out[a])[2 * ng + 1] = (int64_t)(s >> 64);
           } else {
-            ((double *)agg_out[a])[ng] = (double)((long double)s / (long double)cnt);
+            /* avg.cpp GetAverageDivident: count * 10^scale for DECIMAL */
+            long double div = (long double)cnt;
+            if (sp->type == MTL_DECIMAL) {
+              double p10 = 1.0;
+              for (int32_t q = 0; q < sp->scale; ++q) p10 *= 10.0;
+              div *= (long double)p10;
+            }
+            ((double *)agg_out[a])[ng] = (double)((long double)s / div);
           }
         }
       } else {

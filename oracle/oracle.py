@@ -104,17 +104,17 @@ def star_topk(desc: abi.StarDesc):
 
 def group_agg(keys, aggs, sel=None):
     """keys: list of (np column, type); aggs: list of (func, np column | None,
-    type).  Returns (keys int64 [n, n_keys], [per-agg numpy arrays]) in
-    ascending key order."""
+    type[, DECIMAL scale]).  Returns (keys int64 [n, n_keys], [per-agg numpy
+    arrays]) in ascending key order."""
     n_rows = len(keys[0][0])
     kk = [(k.ctypes.data, t, int(k.min()) if len(k) else 0, int(k.max()) if len(k) else 0)
           for k, t in keys]
-    aa = [(f, None if c is None else c.ctypes.data, t) for f, c, t in aggs]
+    aa = [(a[0], None if a[1] is None else a[1].ctypes.data) + tuple(a[2:]) for a in aggs]
     d = abi.make_group_desc(n_rows, kk, aa, sel=None if sel is None else sel.ctypes.data,
                             sel_capacity=0 if sel is None else len(sel))
     cap = max(1, len(sel) if sel is not None else n_rows)
     kout = np.empty((cap, len(keys)), dtype=np.int64)
-    outs = [np.empty(cap, dtype=abi.agg_result_dtype(f, t)) for f, _, t in aggs]
+    outs = [np.empty(cap, dtype=abi.agg_result_dtype(a[0], a[2])) for a in aggs]
     ptrs = (C.c_void_p * max(1, len(outs)))(*[o.ctypes.data for o in outs])
     n = lib().orc_group_agg(C.byref(d), kout.ctypes.data, ptrs, cap)
     return kout[:n].copy(), [o[:n].copy() for o in outs]

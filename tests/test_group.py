@@ -122,14 +122,38 @@ def _gpu_ops():
     return ops
 
 
-def compare(gk, go, rk, ro, aggs, float_rel=1e-9):
+# FLOAT/DOUBLE SUM and AVG tolerance: the device adds in double with
+# atomics, so the result depends on the order of the additions.  A sum is
+# accepted when |got - exact| <= FLOAT_REL * sum(|v|) over the group's rows
+# (AVG: both sides divided by the count), against the oracle's compensated
+# double sum.  Measuring against sum(|v|) keeps cancellation from inflating
+# the relative error of a sum near zero.
+FLOAT_REL = 1e-6
+
+
+def float_abs_sums(keys, col, sel=None):
+    """sum(|v|) per group, in the oracle's group order."""
+    _, outs = orc.group_agg(keys, [(F.AGG_SUM, np.abs(col.astype(np.float64)), F.DOUBLE),
+                                   (F.AGG_COUNT, None, 0)], sel=sel)
+    return outs[0], outs[1]
+
+
+def assert_float_agg(got, exp, abs_sums, counts, func):
+    for i, (x, y) in enumerate(zip(got, exp)):
+        bound = FLOAT_REL * abs_sums[i] / (counts[i] if func == F.AGG_AVG else 1)
+        assert abs(x - y) <= bound + 1e-300, (i, x, y, bound)
+
+
+def compare(gk, go, rk, ro, aggs, keys=None, sel=None):
+    """keys: the (np column, type) list the oracle grouped by, needed for the
+    tolerance of FLOAT/DOUBLE sums."""
     assert gk.shape == rk.shape
     np.testing.assert_array_equal(gk, rk)
-    for a, (f, _, t) in enumerate(aggs):
+    for a, (f, c, t) in enumerate(aggs):
         g, r = as_py(go[a], f, t), as_py(ro[a], f, t)
         if t in (F.FLOAT, F.DOUBLE) and f in (F.AGG_SUM, F.AGG_AVG):
-            for x, y in zip(g, r):
-                assert math.isclose(x, y, rel_tol=float_rel, abs_tol=1e-9), (x, y)
+            abs_sums, counts = float_abs_sums(keys, c, sel)
+            assert_float_agg(g, r, abs_sums, counts, f)
         else:
             assert g == r
 
@@ -154,7 +178,7 @@ def test_gpu_group_agg_parity(domain, n):
     groups = ops.group_agg([(dg, F.BIGINT, int(g.min()), int(g.max()))],
                            [(f, None if c is None else dev[id(c)], t) for f, c, t in aggs], n)
     gk, go = groups.fetch()
-    compare(gk, go, rk, ro, aggs)
+    compare(gk, go, rk, ro, aggs, keys=[(g, F.BIGINT)])
 
 
 @pytest.mark.gpu
@@ -205,7 +229,7 @@ def test_gpu_group_agg_zipf_spill_with_selection():
                            [(f, None if c is None else dev[id(c)], t) for f, c, t in aggs], n,
                            sel=dsel)
     gk, go = groups.fetch()
-    compare(gk, go, rk, ro, aggs)
+    compare(gk, go, rk, ro, aggs, keys=[(g, F.BIGINT)], sel=sel)
 
 
 @pytest.mark.gpu
@@ -227,7 +251,7 @@ def test_gpu_group_agg_composite_keys_spill():
                             (ops.to_device(k2), F.DATE, int(k2.min()), int(k2.max()))],
                            [(f, None if c is None else dev[id(c)], t) for f, c, t in aggs], n)
     gk, go = groups.fetch()
-    compare(gk, go, rk, ro, aggs)
+    compare(gk, go, rk, ro, aggs, keys=[(k1, F.INTEGER), (k2, F.DATE)])
 
 
 @pytest.mark.gpu
@@ -330,3 +354,200 @@ def test_gpu_group_topk_all_ties(k):
     tk, to = groups.topk(0, k, desc=True)
     assert tk[:, 0].tolist() == list(range(k))
     assert to[0].tolist() == [20] * k
+
+
+def test_oracle_avg_decimal_divides_by_scale():
+    """AVG over DECIMAL(15,2) stored as cents: DuckDB divides the hugeint sum
+    by count * 10^scale (avg.cpp GetAverageDivident), so the mean of 123.45
+    and 200.00 is 161.725, not 16172.5."""
+    key = np.zeros(2, np.int32)
+    cents = np.array([12345, 20000], np.int64)
+    _, outs = orc.group_agg([(key, F.INTEGER)], [(F.AGG_AVG, cents, F.DECIMAL, 2),
+                                                 (F.AGG_AVG, cents, F.BIGINT)])
+    assert outs[0][0] == 161.725
+    assert outs[1][0] == 16172.5
+
+
+def test_oracle_avg_matches_exact_quotient():
+    """The oracle's long double AVG against the exact rational mean: equal
+    to within one ulp for sums within and beyond int64 (brute force with
+    Python fractions)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    n = 4000
+    g = rng.integers(0, 40, n).astype(np.int32)
+    v = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+    keys, outs = orc.group_agg([(g, F.INTEGER)], [(F.AGG_AVG, v, F.BIGINT)])
+    for i, k in enumerate(keys[:, 0]):
+        vals = v[g == k]
+        exact = Fraction(sum(int(x) for x in vals), len(vals))
+        got = outs[0][i]
+        assert abs(Fraction(got) - exact) <= abs(exact) * Fraction(1, 2**52)
+
+
+def _topk_expected(rk, vals, k, desc):
+    """Indices of the top k groups under (value, key ascending); vals are
+    exact Python numbers."""
+    sign = -1 if desc else 1
+    return sorted(range(len(rk)), key=lambda i: (sign * vals[i], int(rk[i, 0])))[:k]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("desc", [True, False])
+def test_gpu_group_topk_avg_mixed_sign(desc):
+    """ORDER BY AVG(BIGINT): sums of both signs, small negative sums (whose
+    high word is -1), sums beyond +-2^64, and groups with equal averages;
+    the device key must order exactly like the AVG the oracle emits."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(31 + desc)
+    n_keys = 3000
+    rows = []
+    for key in range(n_keys):
+        kind = key % 6
+        cnt = int(rng.integers(1, 9))
+        if kind == 0:  # small negative sums in (-1024, 0)
+            vals = rng.integers(-300, 100, cnt)
+        elif kind == 1:  # huge magnitudes: sums beyond 2^64
+            vals = rng.choice([2**62 + 1, -(2**62) - 3, 2**63 - 1, -(2**63)], cnt)
+        elif kind == 2:  # equal averages: -3.5 or 7.25, different counts
+            vals = [-3, -4] * (cnt // 2 + 1) if key % 12 == 2 else [7, 7, 7, 8]
+        else:
+            vals = rng.integers(-10**12, 10**12, cnt)
+        rows += [(key, int(x)) for x in vals]
+    rows = np.array(rows, dtype=object)
+    g = rows[:, 0].astype(np.int64)
+    v = np.array([int(x) for x in rows[:, 1]], dtype=np.int64)
+    perm = rng.permutation(len(g))
+    g, v = g[perm], v[perm]
+    aggs = [(F.AGG_AVG, v, F.BIGINT), (F.AGG_COUNT, None, 0), (F.AGG_SUM, v, F.BIGINT)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs)
+    avgs = [float(x) for x in ro[0]]
+    dv = ops.to_device(v)
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, 0, n_keys - 1)],
+                           [(F.AGG_AVG, dv, F.BIGINT), (F.AGG_COUNT, None, 0),
+                            (F.AGG_SUM, dv, F.BIGINT)], len(g))
+    for k in (10, 129):
+        order = _topk_expected(rk, avgs, k, desc)
+        tk, to = groups.topk(0, k, desc=desc)
+        assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order]
+        assert to[0].tolist() == [avgs[i] for i in order]
+        assert [abi.hugeint_to_int(r) for r in to[2]] == \
+            [abi.hugeint_to_int(ro[2][i]) for i in order]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("desc", [True, False])
+def test_gpu_group_topk_avg_decimal(desc):
+    """ORDER BY AVG(DECIMAL(15,2)): the key and the value divide by
+    count * 100."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(8)
+    n = 200_000
+    g = rng.integers(0, 5000, n).astype(np.int64)
+    cents = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    aggs = [(F.AGG_AVG, cents, F.DECIMAL, 2), (F.AGG_COUNT, None, 0)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs)
+    avgs = [float(x) for x in ro[0]]
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, 0, 4999)],
+                           [(F.AGG_AVG, ops.to_device(cents), F.DECIMAL, 2),
+                            (F.AGG_COUNT, None, 0)], n)
+    order = _topk_expected(rk, avgs, 64, desc)
+    tk, to = groups.topk(0, 64, desc=desc)
+    assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order]
+    assert to[0].tolist() == [avgs[i] for i in order]
+    gk, go = groups.fetch()
+    np.testing.assert_array_equal(gk, rk)
+    assert go[0].tolist() == avgs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("desc", [True, False])
+def test_gpu_group_topk_sums_straddling_int64(desc):
+    """Sums just beyond INT64_MAX and INT64_MIN share the saturated first key
+    word; the exact high and low words must order them."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(3)
+    imax, imin = np.iinfo(np.int64).max, np.iinfo(np.int64).min
+    g, v = [], []
+    for j in range(1, 61):  # sums imax + j and imin - j
+        g += [j, j, 100 + j, 100 + j]
+        v += [imax, j, imin, -j]
+    for key in range(200, 2200):  # ordinary groups in between
+        g.append(key)
+        v.append(int(rng.integers(-10**15, 10**15)))
+    g = np.array(g, np.int64)
+    v = np.array(v, np.int64)
+    perm = rng.permutation(len(g))
+    g, v = g[perm], v[perm]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], [(F.AGG_SUM, v, F.BIGINT)])
+    sums = [abi.hugeint_to_int(r) for r in ro[0]]
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, int(g.min()), int(g.max()))],
+                           [(F.AGG_SUM, ops.to_device(v), F.BIGINT)], len(g))
+    for k in (20, 65):
+        order = _topk_expected(rk, sums, k, desc)
+        tk, to = groups.topk(0, k, desc=desc)
+        assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order]
+        assert [abi.hugeint_to_int(r) for r in to[0]] == [sums[i] for i in order]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("func,vtype,desc", [(F.AGG_MIN, F.BIGINT, True),
+                                             (F.AGG_MIN, F.BIGINT, False),
+                                             (F.AGG_MAX, F.INTEGER, False),
+                                             (F.AGG_SUM, F.DOUBLE, True),
+                                             (F.AGG_SUM, F.DOUBLE, False),
+                                             (F.AGG_MAX, F.DOUBLE, True),
+                                             (F.AGG_AVG, F.DOUBLE, False),
+                                             (F.AGG_COUNT, None, False),
+                                             (F.AGG_SUM, F.BIGINT, False)])
+@pytest.mark.parametrize("k", [64, 65, 128, 129])
+def test_gpu_group_topk_orders_and_k(func, vtype, desc, k):
+    """Top-K by MIN, MAX, COUNT, integer and DOUBLE sums and averages, both
+    directions, at K around the one- and two-row wave-list boundaries.
+    DOUBLE values are small integers, so every double sum is exact in any
+    order and the expected order is exact too."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(k * 7 + func)
+    n, n_keys = 300_000, 12_000
+    g = rng.integers(0, n_keys, n).astype(np.int64)
+    if vtype == F.DOUBLE:
+        col = rng.integers(-1000, 1000, n).astype(np.float64)
+    elif vtype == F.INTEGER:
+        col = rng.integers(-2**31, 2**31, n).astype(np.int32)
+    else:
+        col = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    aggs = [(func, None if func == F.AGG_COUNT else col, vtype or 0), (F.AGG_COUNT, None, 0)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs)
+    if func == F.AGG_SUM and vtype != F.DOUBLE:
+        vals = [abi.hugeint_to_int(r) for r in ro[0]]
+    else:
+        vals = [x.item() for x in ro[0]]
+    dcol = None if func == F.AGG_COUNT else ops.to_device(col)
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, 0, n_keys - 1)],
+                           [(func, dcol, vtype or 0), (F.AGG_COUNT, None, 0)], n)
+    order = _topk_expected(rk, vals, k, desc)
+    tk, to = groups.topk(0, k, desc=desc)
+    assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order]
+    got = ([abi.hugeint_to_int(r) for r in to[0]] if func == F.AGG_SUM and vtype != F.DOUBLE
+           else [x.item() for x in to[0]])
+    assert got == [vals[i] for i in order]
+
+
+@pytest.mark.gpu
+def test_gpu_group_topk_many_rounds_per_block():
+    """6M groups: each top-K block walks about 23K entries in several rounds
+    after its seeded bound."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(606)
+    n, n_keys = 6_500_000, 6_000_000
+    g = rng.integers(0, n_keys, n).astype(np.int64)
+    v = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    rk, ro = orc.group_agg([(g, F.BIGINT)], [(F.AGG_SUM, v, F.BIGINT)])
+    sums = np.array([abi.hugeint_to_int(r) for r in ro[0]], dtype=np.int64)
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, 0, n_keys - 1)],
+                           [(F.AGG_SUM, ops.to_device(v), F.BIGINT)], n)
+    for desc in (True, False):
+        order = np.lexsort((rk[:, 0], -sums if desc else sums))[:100]
+        tk, to = groups.topk(0, 100, desc=desc)
+        assert tk[:, 0].tolist() == rk[order, 0].tolist()
+        assert [abi.hugeint_to_int(r) for r in to[0]] == sums[order].tolist()

@@ -116,6 +116,35 @@ def test_group_topk_matches_oracle(ops):
     assert out["s"][0] == abi.HUGEINT
 
 
+@pytest.mark.parametrize("order", ["", " ORDER BY a DESC LIMIT 40", " ORDER BY a ASC LIMIT 40"])
+def test_group_avg_decimal(ops, order):
+    """AVG over DECIMAL(15,2): a DOUBLE divided by count * 100, as DuckDB
+    returns it, with and without ordering by it."""
+    rng = np.random.default_rng(4)
+    n = 100_000
+    g = rng.integers(0, 700, n).astype(np.int64)
+    cents = rng.integers(-10**8, 10**9, n).astype(np.int64)
+    dg, dc = ops.to_device(g), ops.to_device(cents)
+    S.register_table("t", n, [("g", abi.BIGINT, dg.data_ptr(), 0, 699, False),
+                              ("price", abi.DECIMAL, dc.data_ptr(), -10**8, 10**9, False, 2)])
+    out = S.execute(f"SELECT g, AVG(price) AS a, SUM(price) AS s FROM t GROUP BY g{order}")
+    rk, ro = orc.group_agg([(g, abi.BIGINT)], [(abi.AGG_AVG, cents, abi.DECIMAL, 2),
+                                                (abi.AGG_SUM, cents, abi.DECIMAL, 2)])
+    avgs = [float(x) for x in ro[0]]
+    idx = list(range(len(rk)))
+    if order:
+        sign = -1 if "DESC" in order else 1
+        idx = sorted(idx, key=lambda i: (sign * avgs[i], int(rk[i, 0])))[:40]
+    assert out["a"][:2] == (abi.DOUBLE, 0)
+    assert out["s"][:2] == (abi.HUGEINT, 2)
+    assert out["g"][2].tolist() == [int(rk[i, 0]) for i in idx]
+    assert out["a"][2].tolist() == [avgs[i] for i in idx]
+    assert out["s"][2] == [abi.hugeint_to_int(ro[1][i]) for i in idx]
+    # the value itself: the mean in currency units, not in cents
+    k0 = int(rk[idx[0], 0])
+    assert abs(out["a"][2][0] - cents[g == k0].mean() / 100) < 1e-6
+
+
 def test_projection_overflow_and_unsupported(ops):
     v = np.array([2**30, 5, -7], dtype=np.int32)
     dv = ops.to_device(v)

@@ -78,6 +78,10 @@ def test_admitted_shapes(sql, shape):
     ("SELECT o.o_key, SUM(l.price) AS r FROM orders o, lines l WHERE l.l_order = o.o_key "
      "GROUP BY o.o_date ORDER BY r DESC LIMIT 3", "parent key"),
     ("SELECT c_b FROM t WHERE c_b = 1; SELECT 1", "unexpected"),
+    ("SELECT g, SUM(c_b) AS s FROM t GROUP BY g ORDER BY s DESC LIMIT 4294967296",
+     "exceeds 2^32-1"),
+    ("SELECT g, SUM(c_b) AS s FROM t GROUP BY g ORDER BY s DESC LIMIT 4294967297",
+     "exceeds 2^32-1"),
 ])
 def test_refused_with_reason(sql, why):
     with pytest.raises(abi.Unsupported) as ei:
