@@ -149,9 +149,16 @@ int32_t mtl_stream_sync(mtl_stream *s); /* = end_batch's wait              */
 /* The hipStream_t behind a handle (for callers that share streams). */
 void *mtl_stream_native(mtl_stream *s);
 
-/* Device memory from the processing budget (GpuBuffer::new / upload).    */
+/* Device memory from the processing budget (GpuBuffer::new / upload).
+ * After gpu_buffer_init every device allocation of the library (these, the
+ * streams' scratch arenas, group tables, SQL buffers, the serving slots) is
+ * charged to the processing budget; one that would exceed it fails with
+ * MTL_ERR_BUDGET (the reference's budgets bounded only SQL text length:
+ * runtime/mod.rs:339-351). */
 int32_t mtl_alloc(uint64_t bytes, void **out_dev);
 int32_t mtl_free(void *dev);
+/* Device bytes charged to the processing budget now. */
+int32_t mtl_processing_in_use(uint64_t *bytes);
 int32_t mtl_memcpy_h2d(void *dst_dev, const void *src_host, uint64_t bytes,
                        mtl_stream *s);
 int32_t mtl_memcpy_d2h(void *dst_host, const void *src_dev, uint64_t bytes,
@@ -492,8 +499,10 @@ int32_t mtl_scalar_hook(int32_t hook, int32_t type, const void *values,
 /* 103-143) and stage_query (state.rs:42-54), which ran every query       */
 /* synchronously inside bind with no overlap and no admission control.    */
 /* ====================================================================== */
-#define MTL_Q_SCAN 0 /* filter -> optional projection (plain scans)    */
-#define MTL_Q_STAR 1 /* star join -> SUM -> top-K (TPC-H Q3 shape)     */
+#define MTL_Q_SCAN 0  /* filter -> optional projection (plain scans)   */
+#define MTL_Q_STAR 1  /* star join -> SUM -> top-K (TPC-H Q3 shape)    */
+#define MTL_Q_GROUP 2 /* grouped aggregate (+ top-K of the groups)     */
+#define MTL_Q_SQL 3   /* SQL text, lowered and run like mtl_sql_execute */
 
 typedef struct mtl_query {
   int32_t kind; /* MTL_Q_*                                                */
@@ -512,31 +521,63 @@ typedef struct mtl_query {
   /* MTL_Q_STAR */
   mtl_star_desc star;
   mtl_star_row *star_out; /* host buffer of star.k rows                  */
+  /* MTL_Q_GROUP: mtl_group_agg over `group`; with group_k > 0 the top
+   * group_k groups by aggregate group_agg (group_desc != 0: largest first),
+   * else every group (at most group_capacity), written to the host buffers
+   * in mtl_groups_fetch / mtl_groups_topk layout. */
+  mtl_group_desc group;
+  uint32_t group_agg;
+  int32_t group_desc;
+  uint32_t group_k;
+  uint64_t group_capacity;
+  int64_t *group_keys;                  /* capacity x n_keys            */
+  void *group_out[MTL_GROUP_MAX_AGGS];  /* one buffer per aggregate     */
+  /* MTL_Q_SQL: the statement (copied at submit) */
+  const char *sql;
 } mtl_query;
 
 typedef struct mtl_query_result {
   int32_t status;       /* MTL_OK or the query's error status             */
-  uint32_t n_rows;      /* MTL_Q_STAR: rows written to star_out           */
+  uint32_t n_rows;      /* MTL_Q_STAR: rows written to star_out;          */
+                        /* MTL_Q_GROUP: groups written to the buffers     */
   uint64_t count;       /* MTL_Q_SCAN: selected rows                      */
   uint64_t fingerprint; /* MTL_Q_SCAN: hash of (row id, value) in order   */
   int64_t t_submit_ns;  /* steady-clock stamps: arrival (or submit),      */
   int64_t t_start_ns;   /* dispatch to a slot,                            */
   int64_t t_done_ns;    /* completion observed by the host                */
   char error[160];      /* message when status != MTL_OK                 */
+  uint64_t groups;      /* MTL_Q_GROUP: number of groups                  */
+  mtl_result *result;   /* MTL_Q_SQL: the rows (free: mtl_result_free)    */
+  uint32_t batch;       /* queries of the batch it ran in (1: alone)      */
 } mtl_query_result;
 
 typedef struct mtl_server mtl_server;
 
-/* n_slots concurrent streams (1 = serial dispatch); queries are admitted in
- * arrival order while their device scratch fits scratch_budget (0 = the
- * processing budget of gpu_buffer_init, or 16 GiB before init). */
-int32_t mtl_server_create(uint32_t n_slots, uint64_t scratch_budget,
-                          mtl_server **out);
+/* n_slots concurrent streams, each with a worker thread.  A worker takes
+ * every query that has arrived, in arrival order, up to max_batch, while
+ * their device scratch fits scratch_budget (0 = the processing budget of
+ * gpu_buffer_init, or 16 GiB before init), and runs them as one batch:
+ * scans with one predicate on the same column and star queries over the
+ * same tables share their table passes (each tile is read once for the
+ * whole batch), with one host synchronisation for the batch; group-by and
+ * SQL queries run one after another on the same stream.  n_slots = 1 and
+ * max_batch = 1 is serial one-query-at-a-time dispatch, the reference's
+ * one query per CALL (processing.rs:103-143). */
+int32_t mtl_server_create(uint32_t n_slots, uint32_t max_batch,
+                          uint64_t scratch_budget, mtl_server **out);
 /* Queue a query (the descriptor is copied; columns and star_out must stay
  * valid until it completes).  not_before_ns > 0 holds the query until that
  * steady-clock time, which is then its arrival time. */
 int32_t mtl_server_submit(mtl_server *srv, const mtl_query *q,
                           int64_t not_before_ns, uint64_t *ticket);
+/* Queue a SQL statement (the text is copied); it is lowered at submit
+ * (MTL_UNSUPPORTED statements complete at once with that status, the
+ * signal for DuckDB's CPU path) and runs like mtl_sql_execute.  The result
+ * handle arrives in mtl_query_result.result.  Replaces gpu_processing's
+ * materialize_query / stage_query (processing.rs:103-143, state.rs:42-54)
+ * as a request loop. */
+int32_t mtl_server_submit_sql(mtl_server *srv, const char *sql,
+                              int64_t not_before_ns, uint64_t *ticket);
 /* Block until the query completes; releases the ticket. */
 int32_t mtl_server_wait(mtl_server *srv, uint64_t ticket,
                         mtl_query_result *out);

@@ -141,6 +141,8 @@ class StarRow(C.Structure):
 
 Q_SCAN = 0
 Q_STAR = 1
+Q_GROUP = 2
+Q_SQL = 3
 HOOK_ADD, HOOK_MUL, HOOK_EQ, HOOK_LT, HOOK_LE, HOOK_GT, HOOK_GE = range(7)
 
 
@@ -148,14 +150,19 @@ class Query(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n_rows", C.c_uint64), ("n_preds", C.c_uint32),
                 ("preds", Pred * MAX_PREDS), ("proj_col", C.c_void_p), ("proj_type", C.c_int32),
                 ("proj_op", C.c_int32), ("proj_scalar", C.c_int64), ("star", StarDesc),
-                ("star_out", C.POINTER(StarRow))]
+                ("star_out", C.POINTER(StarRow)),
+                ("group", GroupDesc), ("group_agg", C.c_uint32), ("group_desc", C.c_int32),
+                ("group_k", C.c_uint32), ("group_capacity", C.c_uint64),
+                ("group_keys", C.c_void_p), ("group_out", C.c_void_p * GROUP_MAX_AGGS),
+                ("sql", C.c_char_p)]
 
 
 class QueryResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("n_rows", C.c_uint32), ("count", C.c_uint64),
                 ("fingerprint", C.c_uint64), ("t_submit_ns", C.c_int64),
                 ("t_start_ns", C.c_int64), ("t_done_ns", C.c_int64),
-                ("error", C.c_char * 160)]
+                ("error", C.c_char * 160), ("groups", C.c_uint64), ("result", C.c_void_p),
+                ("batch", C.c_uint32)]
 
 
 def scalar_bits(value, type_code: int) -> int:
@@ -257,8 +264,10 @@ def lib() -> C.CDLL:
         "mtl_groups_fetch": (i32, [vp, vp, C.POINTER(vp), u64, C.POINTER(u64)]),
         "mtl_groups_topk": (i32, [vp, u32, i32, u32, vp, C.POINTER(vp), C.POINTER(u32)]),
         "mtl_groups_release": (i32, [vp]),
-        "mtl_server_create": (i32, [u32, u64, C.POINTER(vp)]),
+        "mtl_server_create": (i32, [u32, u32, u64, C.POINTER(vp)]),
         "mtl_server_submit": (i32, [vp, C.POINTER(Query), i64, C.POINTER(u64)]),
+        "mtl_server_submit_sql": (i32, [vp, C.c_char_p, i64, C.POINTER(u64)]),
+        "mtl_processing_in_use": (i32, [C.POINTER(u64)]),
         "mtl_server_wait": (i32, [vp, u64, C.POINTER(QueryResult)]),
         "mtl_server_destroy": (i32, [vp]),
         "mtl_now_ns": (i64, []),

@@ -134,39 +134,6 @@ __device__ __forceinline__ void mark_word(const FilterArgs& a, uint64_t* win, ui
 // and the probe bitmap is read as 32-bit words, so the kernel keeps enough
 // registers free for a high occupancy.
 // ---------------------------------------------------------------------------
-// Raw values (zero-extended) of the 16 dense rows of a lane; 0 past n.
-__device__ __forceinline__ void load_u32x16(const void* col, uint32_t width, uint64_t k0,
-                                            uint64_t n, bool vec, uint32_t (&x)[FJ][4]) {
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    const uint64_t k = k0 + (uint64_t)j * 256;
-    if (vec && k + 3 < n) {
-      if (width == 4) {
-        uint4 u = *(const uint4*)((const uint32_t*)col + k);
-        x[j][0] = u.x; x[j][1] = u.y; x[j][2] = u.z; x[j][3] = u.w;
-      } else {
-        uint32_t u = *(const uint32_t*)((const uint8_t*)col + k);
-        x[j][0] = u & 0xFFu; x[j][1] = (u >> 8) & 0xFFu; x[j][2] = (u >> 16) & 0xFFu;
-        x[j][3] = u >> 24;
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        x[j][v] = k + v < n ? (width == 4 ? ((const uint32_t*)col)[k + v]
-                                          : (uint32_t)((const uint8_t*)col)[k + v])
-                            : 0u;
-    }
-  }
-}
-
-// Raw narrow key -> offset from min, 0xFFFFFFFF when outside [0, span).
-__device__ __forceinline__ uint32_t narrow_off(uint32_t raw, int32_t type, int64_t min,
-                                               uint64_t span) {
-  const int64_t key = type == MTL_DICT8 ? (int64_t)raw : (int64_t)(int32_t)raw;
-  const uint64_t o = key_off(key, min);
-  return o < span ? (uint32_t)o : 0xFFFFFFFFu;
-}
-
 __device__ __forceinline__ uint32_t eval_narrow(const FilterArgs& a, uint64_t n, uint64_t k0,
                                                 uint32_t (&ms)[FJ][4]) {
   uint32_t bits = 0;
@@ -288,26 +255,6 @@ __device__ __forceinline__ void mark_rows(const FilterArgs& a, uint64_t n, uint6
     }
   }
   if (e) atomicOr(a.err, e);
-}
-
-// The pass bits of one 1024-row wave segment: lane l's 4 bits of step j sit
-// at bits 4(l%16).. of word l/16 of the step's 256 rows.  An OR scan over
-// each 16-lane row (DPP row shifts) leaves the word in the row's last lane,
-// which stores it.
-__device__ __forceinline__ void store_pass_bits(uint64_t* sel_bm, uint64_t seg, uint32_t bits,
-                                                int lane) {
-  const uint32_t r = (uint32_t)lane & 15u;
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    const uint32_t nib = (bits >> (4 * j)) & 0xFu;
-    uint32_t lo = r < 8 ? nib << (4 * r) : 0u;
-    uint32_t hi = r < 8 ? 0u : nib << (4 * (r - 8));
-    lo |= row_shr<1>(lo); hi |= row_shr<1>(hi);
-    lo |= row_shr<2>(lo); hi |= row_shr<2>(hi);
-    lo |= row_shr<4>(lo); hi |= row_shr<4>(hi);
-    lo |= row_shr<8>(lo); hi |= row_shr<8>(hi);
-    if (r == 15) sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = ((uint64_t)hi << 32) | lo;
-  }
 }
 
 // Pass 1 of a filter: one tile of 4096 rows per block, no inter-block
@@ -1322,6 +1269,303 @@ int32_t filter_project_ex(const mtl_pred* preds, uint32_t n_preds, uint64_t n_ro
   }
   return filter_impl(preds, n_preds, n_rows, nullptr, nullptr, 0, out_sel, out_count, proj, ex,
                      s);
+}
+
+}  // namespace mtl
+
+// ---------------------------------------------------------------------------
+// Shared scan of a batch of queries.  k_mq_filter loads each tile of the
+// shared predicate column once and, query by query, writes the pass bits,
+// the tile count and the group sum (the same layout as the single-query
+// filter pass); k_mq_expand stages the tile's projection column in LDS once
+// (when enough of its rows pass, summed over the queries) and expands every
+// query's selection, projection and fingerprint from it, with each query's
+// output offset computed from its own counts and group sums.
+// ---------------------------------------------------------------------------
+namespace mtl {
+
+struct MQScanQ {
+  PredC pred;
+  uint64_t n;
+  uint32_t tiles;
+  int32_t op;
+  int64_t scalar;
+  uint32_t* out_sel;
+  void* out_proj;
+  uint64_t* bm;
+  uint32_t* counts;
+  uint32_t* gsums;
+  uint64_t* fp;
+  uint32_t* words;  // [0] count, [1] overflow, [2..3] fingerprint
+};
+
+struct MQScanArgs {
+  const void* col;   // shared predicate column
+  const void* pcol;  // shared projection column (nullptr: none)
+  uint64_t n_max;    // largest row count of the group
+  uint32_t nq;
+  uint32_t vec;      // col aligned for vector loads
+  const MQScanQ* q;
+};
+
+template <typename T, uint32_t KIND>
+__global__ __launch_bounds__(FBLOCK) void k_mq_filter(MQScanArgs a) {
+  __shared__ uint32_t s_cnt[kMQMax * FWAVES];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
+  const uint64_t k0 = seg + (uint64_t)lane * 4;
+  T x[FJ][4];
+  if (a.vec && (uint64_t)(tile + 1) * FTILE <= a.n_max) {  // block-uniform
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+      load4<T, true>((const T*)a.col, k0 + (uint64_t)j * 256, a.n_max, true, x[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+      load4<T, false>((const T*)a.col, k0 + (uint64_t)j * 256, a.n_max, a.vec != 0, x[j]);
+  }
+  for (uint32_t q = 0; q < a.nq; ++q) {
+    const MQScanQ& Q = a.q[q];
+    const uint64_t n = Q.n;
+    uint32_t bits = 0;
+    if ((uint64_t)tile * FTILE < n) {  // block-uniform
+#pragma unroll
+      for (int j = 0; j < FJ; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (k0 + (uint64_t)j * 256 + v < n &&
+              pred_pass(enc_raw((uint64_t)x[j][v], KIND), Q.pred))
+            bits |= 1u << (4 * j + v);
+      store_pass_bits(Q.bm, seg, bits, lane);
+    }
+    const uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
+    if (lane == 0) s_cnt[q * FWAVES + wave] = c;
+  }
+  __syncthreads();
+  if ((uint32_t)tid < a.nq) {
+    const MQScanQ& Q = a.q[tid];
+    if ((uint64_t)tile * FTILE < Q.n) {
+      uint32_t t = 0;
+#pragma unroll
+      for (int w = 0; w < FWAVES; ++w) t += s_cnt[tid * FWAVES + w];
+      Q.counts[tile] = t;
+      if (t) atomicAdd(&Q.gsums[(tile >> kGroupShift) * kGroupStride], t);
+    }
+  }
+}
+
+template <int PK>
+__global__ __launch_bounds__(FBLOCK) void k_mq_expand(MQScanArgs a) {
+  using VT = typename std::conditional<PK == 2 || PK == 4, uint64_t, uint32_t>::type;
+  __shared__ uint32_t s_ids[FTILE];
+  __shared__ VT s_val[PK ? FTILE : 1];
+  __shared__ uint32_t s_wave[FWAVES];
+  __shared__ uint32_t s_red[FWAVES];
+  __shared__ uint64_t s_fp[FWAVES];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t t0 = (uint64_t)tile * FTILE;
+  // stage the tile's projection values once when the queries' passing rows
+  // of this tile cover a good part of it (else each query gathers its own)
+  bool staged = false;
+  if constexpr (PK != 0) {
+    uint32_t passing = 0;
+    for (uint32_t q = 0; q < a.nq; ++q)
+      if (tile < a.q[q].tiles) passing += a.q[q].counts[tile];
+    staged = passing >= FTILE / 8;  // block-uniform
+    if (staged) {
+      const VT* col = (const VT*)a.pcol;
+#pragma unroll 4
+      for (uint32_t i = tid; i < FTILE; i += FBLOCK) {
+        const uint64_t r = t0 + i;
+        s_val[i] = r < a.n_max ? col[r] : VT(0);
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t q = 0; q < a.nq; ++q) {
+    const MQScanQ& Q = a.q[q];
+    if (tile >= Q.tiles) continue;  // block-uniform
+    const uint32_t m =
+        (uint32_t)(Q.bm[(t0 + (uint64_t)tid * 16) / 64] >> ((tid * 16) % 64)) & 0xFFFFu;
+    const uint32_t base = tile_offset(Q.counts, Q.gsums, tile, s_red);
+    const uint32_t c = (uint32_t)__popc(m);
+    const uint32_t incl = wave_inclusive_u32(c);
+    if (lane == WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t pos = incl - c, total = 0;
+#pragma unroll
+    for (int w = 0; w < FWAVES; ++w) {
+      if (w < wave) pos += s_wave[w];
+      total += s_wave[w];
+    }
+    uint32_t mm = m;
+    while (mm) {
+      const int i = __ffs(mm) - 1;
+      mm &= mm - 1;
+      s_ids[pos++] = (uint32_t)tid * 16 + (uint32_t)i;
+    }
+    __syncthreads();
+    ExpandProj pj{a.pcol, Q.op, Q.scalar, Q.out_proj, nullptr};
+    bool ovf = false;
+    uint64_t h = 0;
+    for (uint32_t i = tid; i < total; i += FBLOCK) {
+      const uint32_t li = s_ids[i];
+      const uint32_t r = (uint32_t)(t0 + li);
+      uint64_t x = 0;
+      if constexpr (PK != 0) x = staged ? (uint64_t)s_val[li] : proj_load<PK>(pj, r);
+      const uint64_t o = (uint64_t)base + i;
+      Q.out_sel[o] = r;
+      const uint64_t vb = proj_store<PK>(pj, o, x, ovf);
+      h += fp_term(o, r, PK ? vb : r);
+    }
+    if constexpr (PK == 1 || PK == 2) {
+      if (__ballot(ovf) && lane == 0) atomicOr(&Q.words[1], 1u);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off, WAVE);
+    if (lane == 0) s_fp[wave] = h;
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t t = 0;
+#pragma unroll
+      for (int w = 0; w < FWAVES; ++w) t += s_fp[w];
+      if (t) atomicAdd((unsigned long long*)&Q.fp[(tile >> kGroupShift) * kFpStride],
+                       (unsigned long long)t);
+      if (tile == Q.tiles - 1) Q.words[0] = base + total;
+    }
+    __syncthreads();  // s_ids, s_wave, s_red and s_fp are reused by the next query
+  }
+}
+
+// Block q adds up query q's per-group fingerprint sums.
+__global__ __launch_bounds__(1024) void k_mq_fp_sum(MQScanArgs a) {
+  __shared__ uint64_t s_part[1024 / WAVE];
+  const MQScanQ& Q = a.q[blockIdx.x];
+  const uint32_t groups = (Q.tiles + (1u << kGroupShift) - 1) >> kGroupShift;
+  uint64_t t = 0;
+  for (uint32_t i = threadIdx.x; i < groups; i += blockDim.x) t += Q.fp[(uint64_t)i * kFpStride];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, WAVE);
+  if (threadIdx.x % WAVE == 0) s_part[threadIdx.x / WAVE] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t s = 0;
+    for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) s += s_part[w];
+    Q.words[2] = (uint32_t)s;
+    Q.words[3] = (uint32_t)(s >> 32);
+  }
+}
+
+bool scan_batchable(const mtl_query& q) {
+  if (q.kind != MTL_Q_SCAN || q.n_preds != 1 || q.n_rows == 0 || q.n_rows > 0xFFFFFFFFull)
+    return false;
+  const mtl_pred& p = q.preds[0];
+  if (!p.column || !type_width(p.type) || p.type == MTL_HUGEINT || p.op < 0 || p.op > 4)
+    return false;
+  if (q.proj_col) {
+    int pk = 0;
+    if (proj_kind(q.proj_type, q.proj_op, &pk) != MTL_OK) return false;
+  }
+  return true;
+}
+
+bool scan_shareable(const mtl_query& a, const mtl_query& b) {
+  return a.preds[0].column == b.preds[0].column && a.preds[0].type == b.preds[0].type &&
+         a.proj_col == b.proj_col && (!a.proj_col || a.proj_type == b.proj_type);
+}
+
+size_t scan_batch_stage_bytes(uint32_t n) { return (size_t)n * sizeof(MQScanQ) + 256; }
+
+int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* words,
+                           void* host_stage, mtl_stream* s) {
+  if (n == 0) return MTL_OK;
+  if (n > kMQMax) return set_error(MTL_ERR_INVALID_ARGUMENT, "more than %u queries", kMQMax);
+  const mtl_query& q0 = *items[0].q;
+  for (uint32_t i = 0; i < n; ++i)
+    if (!scan_batchable(*items[i].q) || !scan_shareable(q0, *items[i].q))
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "query %u cannot join the shared scan", i);
+  int pk = 0;
+  if (q0.proj_col) MTL_TRY(proj_kind(q0.proj_type, q0.proj_op, &pk));
+  Arena& ar = s->arena;
+  // per-query scratch; the regions to clear (group sums, fingerprint sums)
+  // are carved from one block so one clearing launch covers them
+  uint64_t n_max = 0, clear_words = 0;
+  std::vector<uint32_t> tiles(n), groups(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    n_max = std::max<uint64_t>(n_max, items[i].q->n_rows);
+    tiles[i] = filter_tiles(items[i].q->n_rows);
+    groups[i] = (uint32_t)ceil_div(tiles[i], 1u << kGroupShift);
+    clear_words += (uint64_t)groups[i] * (kGroupStride + 2 * kFpStride);
+  }
+  void* clr;
+  MTL_TRY(ar.alloc(clear_words * 4, &clr));
+  MQScanQ* hq = (MQScanQ*)host_stage;
+  uint32_t* cw = (uint32_t*)clr;
+  for (uint32_t i = 0; i < n; ++i) {
+    const mtl_query& q = *items[i].q;
+    MQScanQ& Q = hq[i];
+    Q = MQScanQ{};
+    Q.pred = compile_pred(q.preds[0]);
+    Q.n = q.n_rows;
+    Q.tiles = tiles[i];
+    Q.op = q.proj_op;
+    Q.scalar = q.proj_scalar;
+    Q.out_sel = items[i].out_sel;
+    Q.out_proj = items[i].out_proj;
+    void *bm, *cnt;
+    MTL_TRY(ar.alloc((uint64_t)tiles[i] * FTILE / 8, &bm));
+    MTL_TRY(ar.alloc((uint64_t)tiles[i] * 4, &cnt));
+    Q.bm = (uint64_t*)bm;
+    Q.counts = (uint32_t*)cnt;
+    Q.fp = (uint64_t*)cw;  // 8-byte aligned: every carve is a multiple of 64 words
+    cw += (uint64_t)groups[i] * 2 * kFpStride;
+    Q.gsums = cw;
+    cw += (uint64_t)groups[i] * kGroupStride;
+    Q.words = words + 4 * i;
+  }
+  void* dq;
+  MTL_TRY(ar.alloc((size_t)n * sizeof(MQScanQ), &dq));
+  hipStream_t st = s->stream;
+  MTL_HIP(hipMemcpyAsync(dq, hq, (size_t)n * sizeof(MQScanQ), hipMemcpyHostToDevice, st));
+  ZeroList zl;
+  zl.add((uint32_t*)clr, clear_words);
+  zl.add(words, (uint64_t)n * 4);
+  MTL_TRY(launch_zero(zl, st));
+  MQScanArgs a{};
+  a.col = q0.preds[0].column;
+  a.pcol = q0.proj_col;
+  a.n_max = n_max;
+  a.nq = n;
+  a.vec = ((uintptr_t)a.col % (type_width(q0.preds[0].type) == 1 ? 4 : 16)) == 0;
+  a.q = (const MQScanQ*)dq;
+  const dim3 grid(filter_tiles(n_max));
+  {
+    ProfScope ps(s, "k_mq_filter");
+    switch (hq[0].pred.kind) {
+      case PK_I32: hipLaunchKernelGGL((k_mq_filter<uint32_t, PK_I32>), grid, dim3(FBLOCK), 0, st, a); break;
+      case PK_F32: hipLaunchKernelGGL((k_mq_filter<uint32_t, PK_F32>), grid, dim3(FBLOCK), 0, st, a); break;
+      case PK_U8: hipLaunchKernelGGL((k_mq_filter<uint8_t, PK_U8>), grid, dim3(FBLOCK), 0, st, a); break;
+      case PK_I64: hipLaunchKernelGGL((k_mq_filter<uint64_t, PK_I64>), grid, dim3(FBLOCK), 0, st, a); break;
+      default: hipLaunchKernelGGL((k_mq_filter<uint64_t, PK_F64>), grid, dim3(FBLOCK), 0, st, a); break;
+    }
+    MTL_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(s, "k_mq_expand");
+    switch (pk) {
+      case 1: hipLaunchKernelGGL(k_mq_expand<1>, grid, dim3(FBLOCK), 0, st, a); break;
+      case 2: hipLaunchKernelGGL(k_mq_expand<2>, grid, dim3(FBLOCK), 0, st, a); break;
+      case 3: hipLaunchKernelGGL(k_mq_expand<3>, grid, dim3(FBLOCK), 0, st, a); break;
+      case 4: hipLaunchKernelGGL(k_mq_expand<4>, grid, dim3(FBLOCK), 0, st, a); break;
+      default: hipLaunchKernelGGL(k_mq_expand<0>, grid, dim3(FBLOCK), 0, st, a); break;
+    }
+    MTL_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_mq_fp_sum, dim3(n), dim3(1024), 0, st, a);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
 }
 
 }  // namespace mtl

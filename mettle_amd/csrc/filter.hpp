@@ -148,4 +148,32 @@ int32_t filter_project_ex(const mtl_pred* preds, uint32_t n_preds, uint64_t n_ro
                           const CompactProj* proj, uint32_t* out_sel, uint32_t* out_count,
                           ScanExtras* ex, mtl_stream* s);
 
+// ---------------------------------------------------------------------------
+// Shared scans of a batch of queries (the serving loop's continuous
+// batching).  Queries with one predicate on the same column, projecting the
+// same column (or nothing), form a share group: each tile of the column is
+// loaded once for all of them.  Each query keeps its own row count (a prefix
+// of the resident table), operator, literal and outputs.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMQMax = 32;  // queries per share group
+struct ScanBatchItem {
+  const mtl_query* q;
+  uint32_t* out_sel;  // q->n_rows row ids
+  void* out_proj;     // q->n_rows projected values (nullptr without projection)
+};
+// A query the shared scan can take: one predicate, rows < 2^32, a valid
+// projection.
+bool scan_batchable(const mtl_query& q);
+// Same share group: same predicate column and type, same projection column
+// and type.
+bool scan_shareable(const mtl_query& a, const mtl_query& b);
+// Pinned host bytes the enqueue stages its per-query arguments in.
+size_t scan_batch_stage_bytes(uint32_t n);
+// Enqueue the shared scan of n <= kMQMax shareable queries on s.  words: n x
+// 4 uint32 per query (count, overflow flag, 64-bit fingerprint), cleared and
+// filled on the stream; the fingerprint is the one the single-query serving
+// path computes.  Scratch comes from the stream's arena (not reset here).
+int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* words,
+                           void* host_stage, mtl_stream* s);
+
 }  // namespace mtl

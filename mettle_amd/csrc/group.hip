@@ -908,6 +908,7 @@ struct mtl_groups {
   uint64_t entries = 0;  // table entries (dense domain or cap + 1)
   void* keys_mem = nullptr;
   void* st_mem = nullptr;
+  uint64_t bytes = 0;  // charged to the processing budget
   int32_t agg_func[MTL_GROUP_MAX_AGGS];
   int32_t agg_type[MTL_GROUP_MAX_AGGS];
   int32_t agg_scale[MTL_GROUP_MAX_AGGS];  // DECIMAL digits (AVG divides by 10^scale)
@@ -1118,6 +1119,18 @@ static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t total, mtl
   return MTL_OK;
 }
 
+// Device bytes a grouped aggregate may take at most: the table, the spill
+// arrays of a large dense domain, and the top-K and emission scratch (an
+// upper bound for admission; 0 entries when the plan is refused).
+uint64_t group_scratch_bytes(const mtl_group_desc& d) {
+  GroupArgs a;
+  uint64_t entries = 0;
+  if (group_plan(&d, &a, &entries) != MTL_OK) return 1u << 20;
+  const uint64_t rows = d.sel ? d.sel_capacity : d.n_rows;
+  return entries * (a.W + (a.dense ? 0 : 1)) * 8 + rows * (4 + 8 * (uint64_t)a.n_src) * 2 +
+         entries * (16 + 8 * (uint64_t)a.n_aggs) + (64u << 20);
+}
+
 }  // namespace mtl
 
 using namespace mtl;
@@ -1139,14 +1152,31 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
     g->agg_type[i] = desc->aggs[i].type;
     g->agg_scale[i] = desc->aggs[i].type == MTL_DECIMAL ? desc->aggs[i].scale : 0;
   }
+  g->bytes = entries * a.W * 8 + 8 + (a.dense ? 0 : entries * 8);
+  if (int32_t cs = charge_processing(g->bytes, "group table")) {
+    delete g;
+    return cs;
+  }
   hipError_t e = hipMallocAsync(&g->st_mem, entries * a.W * 8 + 8, s->stream);
   if (e == hipSuccess && !a.dense) e = hipMallocAsync(&g->keys_mem, entries * 8, s->stream);
   if (e != hipSuccess) {
     if (g->st_mem) (void)hipFreeAsync(g->st_mem, s->stream);
+    release_processing(g->bytes);
     delete g;
     return set_error(MTL_ERR_PROCESSING, "processing error: group table of %llu entries: %s",
                      (unsigned long long)entries, hipGetErrorString(e));
   }
+  // every error return from here on releases the table
+  struct Owner {
+    mtl_groups* g;
+    ~Owner() {
+      if (!g) return;
+      (void)hipFreeAsync(g->st_mem, g->s->stream);
+      if (g->keys_mem) (void)hipFreeAsync(g->keys_mem, g->s->stream);
+      release_processing(g->bytes);
+      delete g;
+    }
+  } owner{g};
   s->arena.reset();
   void* err;
   MTL_TRY(s->arena.alloc(256, &err));
@@ -1223,12 +1253,10 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   MTL_HIP(hipStreamSynchronize(s->stream));
   uint32_t eb = *(uint32_t*)host;
   if (eb) {
-    (void)hipFreeAsync(g->st_mem, s->stream);
-    if (g->keys_mem) (void)hipFreeAsync(g->keys_mem, s->stream);
-    delete g;
     if (eb & 8u) return set_error(MTL_ERR_INVALID_ARGUMENT, "a group key lies outside its stated [min, max]");
     return set_error(MTL_ERR_PROCESSING, "processing error: group table overflow");
   }
+  owner.g = nullptr;
   *out = g;
   return MTL_OK;
 }
@@ -1237,6 +1265,7 @@ int32_t mtl_groups_release(mtl_groups* g) {
   if (!g) return MTL_OK;
   (void)hipFreeAsync(g->st_mem, g->s->stream);
   if (g->keys_mem) (void)hipFreeAsync(g->keys_mem, g->s->stream);
+  release_processing(g->bytes);
   delete g;
   return MTL_OK;
 }

@@ -97,9 +97,18 @@ struct Runtime {
   uint64_t processing_bytes = 0;
   uint64_t pinned_bytes = 0;
   mtl_stream* default_stream = nullptr;
-  std::atomic<uint64_t> device_bytes_in_use{0};
+  std::atomic<uint64_t> device_bytes_in_use{0};  // charged to the processing budget
   std::atomic<uint64_t> pinned_bytes_in_use{0};
 };
+
+// Processing-budget accounting: every device allocation of the library
+// outside the column cache is charged here.  Before gpu_buffer_init there
+// is no limit (the charge is still counted).
+int32_t charge_processing(uint64_t bytes, const char* what);
+void release_processing(uint64_t bytes);
+// hipMalloc charged to the processing budget; dev_free releases the charge.
+int32_t dev_alloc(uint64_t bytes, void** out, const char* what);
+void dev_free(void* p, uint64_t bytes);
 
 Runtime& runtime();
 // Ensures a device context exists (HIP init + default stream) without

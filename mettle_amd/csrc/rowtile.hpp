@@ -199,5 +199,58 @@ __device__ __forceinline__ void load_keys16(const void* col, int32_t type, bool 
   }
 }
 
+// Raw values (zero-extended) of the 16 dense rows of a lane; 0 past n.
+__device__ __forceinline__ void load_u32x16(const void* col, uint32_t width, uint64_t k0,
+                                            uint64_t n, bool vec, uint32_t (&x)[FJ][4]) {
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const uint64_t k = k0 + (uint64_t)j * 256;
+    if (vec && k + 3 < n) {
+      if (width == 4) {
+        uint4 u = *(const uint4*)((const uint32_t*)col + k);
+        x[j][0] = u.x; x[j][1] = u.y; x[j][2] = u.z; x[j][3] = u.w;
+      } else {
+        uint32_t u = *(const uint32_t*)((const uint8_t*)col + k);
+        x[j][0] = u & 0xFFu; x[j][1] = (u >> 8) & 0xFFu; x[j][2] = (u >> 16) & 0xFFu;
+        x[j][3] = u >> 24;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        x[j][v] = k + v < n ? (width == 4 ? ((const uint32_t*)col)[k + v]
+                                          : (uint32_t)((const uint8_t*)col)[k + v])
+                            : 0u;
+    }
+  }
+}
+
+// Raw narrow key -> offset from min, 0xFFFFFFFF when outside [0, span).
+__device__ __forceinline__ uint32_t narrow_off(uint32_t raw, int32_t type, int64_t min,
+                                               uint64_t span) {
+  const int64_t key = type == MTL_DICT8 ? (int64_t)raw : (int64_t)(int32_t)raw;
+  const uint64_t o = key_off(key, min);
+  return o < span ? (uint32_t)o : 0xFFFFFFFFu;
+}
+
+// The pass bits of one 1024-row wave segment: lane l's 4 bits of step j sit
+// at bits 4(l%16).. of word l/16 of the step's 256 rows.  An OR scan over
+// each 16-lane row (DPP row shifts) leaves the word in the row's last lane,
+// which stores it.
+__device__ __forceinline__ void store_pass_bits(uint64_t* sel_bm, uint64_t seg, uint32_t bits,
+                                                int lane) {
+  const uint32_t r = (uint32_t)lane & 15u;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const uint32_t nib = (bits >> (4 * j)) & 0xFu;
+    uint32_t lo = r < 8 ? nib << (4 * r) : 0u;
+    uint32_t hi = r < 8 ? 0u : nib << (4 * (r - 8));
+    lo |= row_shr<1>(lo); hi |= row_shr<1>(hi);
+    lo |= row_shr<2>(lo); hi |= row_shr<2>(hi);
+    lo |= row_shr<4>(lo); hi |= row_shr<4>(hi);
+    lo |= row_shr<8>(lo); hi |= row_shr<8>(hi);
+    if (r == 15) sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = ((uint64_t)hi << 32) | lo;
+  }
+}
+
 }  // namespace dev
 }  // namespace mtl

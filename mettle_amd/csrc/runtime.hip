@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 
 #include "mtl_internal.hpp"
 #include "prof.hpp"
@@ -34,19 +35,61 @@ Runtime& runtime() {
   return *rt;
 }
 
+int32_t charge_processing(uint64_t bytes, const char* what) {
+  Runtime& rt = runtime();
+  const uint64_t limit = rt.initialized ? rt.processing_bytes : 0;
+  uint64_t cur = rt.device_bytes_in_use.load();
+  while (true) {
+    if (limit && (cur + bytes < cur || cur + bytes > limit))
+      return set_error(MTL_ERR_BUDGET,
+                       "processing budget exceeded: %s needs %llu bytes with %llu of %llu "
+                       "in use (gpu_buffer_init)",
+                       what, (unsigned long long)bytes, (unsigned long long)cur,
+                       (unsigned long long)limit);
+    if (rt.device_bytes_in_use.compare_exchange_weak(cur, cur + bytes)) return MTL_OK;
+  }
+}
+
+void release_processing(uint64_t bytes) { runtime().device_bytes_in_use -= bytes; }
+
+int32_t dev_alloc(uint64_t bytes, void** out, const char* what) {
+  MTL_TRY(charge_processing(bytes, what));
+  hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    release_processing(bytes);
+    *out = nullptr;
+    return set_error(MTL_ERR_PROCESSING, "processing error: %s hipMalloc(%llu): %s", what,
+                     (unsigned long long)bytes, hipGetErrorString(e));
+  }
+  return MTL_OK;
+}
+
+void dev_free(void* p, uint64_t bytes) {
+  if (!p) return;
+  hipFree(p);
+  release_processing(bytes);
+}
+
 int32_t Arena::alloc(size_t bytes, void** out) {
   size_t need = (bytes + 255) & ~size_t(255);
   if (need == 0) need = 256;
   if (chunks.empty() || off + need > chunks.back().cap) {
+    // doubling chunks, or exactly what is needed when the doubled chunk
+    // would not fit the processing budget
     size_t cap = chunks.empty() ? (size_t(64) << 20) : chunks.back().cap * 2;
     while (cap < need) cap *= 2;
+    if (charge_processing(cap, "scratch arena") != MTL_OK) {
+      cap = (need + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+      MTL_TRY(charge_processing(cap, "scratch arena"));
+    }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, cap);
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
+      release_processing(cap);
       return set_error(MTL_ERR_PROCESSING,
                        "processing error: scratch arena hipMalloc(%zu): %s",
                        cap, hipGetErrorString(e));
-    runtime().device_bytes_in_use += cap;
+    }
     chunks.push_back({(char*)p, cap});
     off = 0;
   }
@@ -58,17 +101,13 @@ int32_t Arena::alloc(size_t bytes, void** out) {
 
 void Arena::release_retired() {
   while (chunks.size() > 1) {
-    hipFree(chunks.front().base);
-    runtime().device_bytes_in_use -= chunks.front().cap;
+    dev_free(chunks.front().base, chunks.front().cap);
     chunks.erase(chunks.begin());
   }
 }
 
 void Arena::release_all() {
-  for (auto& c : chunks) {
-    hipFree(c.base);
-    runtime().device_bytes_in_use -= c.cap;
-  }
+  for (auto& c : chunks) dev_free(c.base, c.cap);
   chunks.clear();
   off = 0;
 }
@@ -323,16 +362,38 @@ void* mtl_stream_native(mtl_stream* s) {
   return (void*)s->stream;
 }
 
+namespace {
+std::mutex g_alloc_mu;
+std::unordered_map<void*, uint64_t> g_allocs;  // mtl_alloc pointer -> bytes charged
+}  // namespace
+
 int32_t mtl_alloc(uint64_t bytes, void** out_dev) {
   if (!out_dev) return set_error(MTL_ERR_INVALID_ARGUMENT, "out_dev is NULL");
   MTL_TRY(ensure_device());
-  MTL_HIP(hipMalloc(out_dev, bytes ? bytes : 1));
-  runtime().device_bytes_in_use += bytes;
+  MTL_TRY(dev_alloc(bytes, out_dev, "mtl_alloc"));
+  std::lock_guard<std::mutex> g(g_alloc_mu);
+  g_allocs[*out_dev] = bytes;
   return MTL_OK;
 }
 
 int32_t mtl_free(void* dev) {
-  if (dev) MTL_HIP(hipFree(dev));
+  if (!dev) return MTL_OK;
+  uint64_t bytes = 0;
+  {
+    std::lock_guard<std::mutex> g(g_alloc_mu);
+    auto it = g_allocs.find(dev);
+    if (it == g_allocs.end())
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "mtl_free of a pointer mtl_alloc did not return");
+    bytes = it->second;
+    g_allocs.erase(it);
+  }
+  dev_free(dev, bytes);
+  return MTL_OK;
+}
+
+int32_t mtl_processing_in_use(uint64_t* bytes) {
+  if (!bytes) return set_error(MTL_ERR_INVALID_ARGUMENT, "bytes is NULL");
+  *bytes = runtime().device_bytes_in_use.load();
   return MTL_OK;
 }
 

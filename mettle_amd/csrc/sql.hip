@@ -33,6 +33,7 @@
 #include <string>
 #include <vector>
 
+#include "filter.hpp"
 #include "mtl_internal.hpp"
 
 // ---------------------------------------------------------------------------
@@ -48,9 +49,12 @@ struct mtl_result {
   std::vector<Column> cols;
   uint64_t rows = 0;
   std::string shape;
+  uint64_t scratch = 0;  // planning only: device bytes the run may take
 };
 
 namespace mtl {
+uint64_t group_scratch_bytes(const mtl_group_desc& d);  // group.hip
+uint64_t query_scratch(const mtl_query& q);             // serve.hip
 namespace sql {
 
 // ---------------------------------------------------------------------------
@@ -712,15 +716,15 @@ static bool literal_bits(const Lit& l, const CatCol& c, int op, int64_t& bits, s
 // ---------------------------------------------------------------------------
 // Execution helpers
 // ---------------------------------------------------------------------------
+// Device buffer charged to the processing budget (MTL_ERR_BUDGET when it
+// would exceed it).
 struct DevBuf {
   void* p = nullptr;
-  ~DevBuf() {
-    if (p) hipFree(p);
-  }
+  uint64_t n = 0;
+  ~DevBuf() { dev_free(p, n); }
   int32_t alloc(uint64_t bytes) {
-    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
-      return set_error(MTL_ERR_PROCESSING, "processing error: hipMalloc of %llu bytes",
-                       (unsigned long long)bytes);
+    MTL_TRY(dev_alloc(bytes, &p, "SQL buffer"));
+    n = bytes;
     return MTL_OK;
   }
 };
@@ -848,6 +852,8 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     odesc = q.order[0].desc;
   }
   res.shape = "scan";
+  // selection, count, sort permutation and one output column at a time
+  res.scratch = T.rows * (4 + 4 + 8 + 8) + (1u << 20);
   if (dry) return MTL_OK;
   // filter
   DevBuf sel, cnt, perm, out, ovf;
@@ -1007,6 +1013,7 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
       return set_error(MTL_UNSUPPORTED, "LIMIT %lld exceeds 2^32-1", (long long)q.limit);
   }
   res.shape = "group";
+  res.scratch = (preds.empty() ? 0 : T.rows * 4) + group_scratch_bytes(d);
   if (dry) return MTL_OK;
   DevBuf sel, cnt;
   if (!preds.empty()) {
@@ -1333,6 +1340,12 @@ static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     d.p_fk_type = pe->fk->type;
   }
   res.shape = dim >= 0 ? "star3" : "star2";
+  {
+    mtl_query sq{};
+    sq.kind = MTL_Q_STAR;
+    sq.star = d;
+    res.scratch = query_scratch(sq);
+  }
   if (dry) return MTL_OK;
   std::vector<mtl_star_row> rows(std::max<uint32_t>(d.k, 1));
   uint32_t n = 0;
@@ -1473,6 +1486,18 @@ static int32_t rewrite(const char* sql, std::string& out) {
 }  // namespace sql
 }  // namespace mtl
 
+namespace mtl {
+// Admission size of a statement for the serving loop: its plan's device
+// scratch (MTL_UNSUPPORTED and the reason when it does not lower).
+int32_t sql_scratch_bytes(const char* sql, uint64_t* bytes) {
+  if (!sql) return set_error(MTL_ERR_INVALID_ARGUMENT, "sql is NULL");
+  mtl_result res;
+  MTL_TRY(sql::plan_and_run(sql, res, nullptr, true));
+  *bytes = res.scratch;
+  return MTL_OK;
+}
+}  // namespace mtl
+
 extern "C" {
 
 int32_t mtl_catalog_add_table(const char* table, uint64_t n_rows) {
@@ -1539,6 +1564,7 @@ int32_t mtl_catalog_drop(const char* table) {
   return MTL_OK;
 }
 
+
 int32_t mtl_sql_explain(const char* sql, char* buf, size_t len) {
   mtl_result res;
   const int32_t st = mtl::sql::plan_and_run(sql, res, nullptr, true);
@@ -1584,6 +1610,46 @@ int32_t mtl_result_free(mtl_result* r) {
   return MTL_OK;
 }
 
+namespace {
+// Per-thread buffers of the scalar hooks: a stream, device memory and
+// pinned staging, grown on demand and reused by every call of the thread
+// (DuckDB calls a hook once per vector of 2048 values), so a call makes no
+// allocation and synchronises once.  They are held until the process ends.
+struct HookBufs {
+  mtl_stream* s = nullptr;
+  void* dev = nullptr;
+  uint64_t dev_cap = 0;
+  void* host = nullptr;
+  uint64_t host_cap = 0;
+};
+thread_local HookBufs t_hook;
+
+int32_t hook_reserve(uint64_t dev_bytes, uint64_t host_bytes) {
+  using namespace mtl;
+  HookBufs& h = t_hook;
+  if (!h.s) MTL_TRY(mtl_stream_create(&h.s));
+  if (dev_bytes > h.dev_cap) {
+    uint64_t cap = 1 << 16;
+    while (cap < dev_bytes) cap *= 2;
+    void* p = nullptr;
+    MTL_TRY(dev_alloc(cap, &p, "scalar hook buffer"));
+    dev_free(h.dev, h.dev_cap);
+    h.dev = p;
+    h.dev_cap = cap;
+  }
+  if (host_bytes > h.host_cap) {
+    uint64_t cap = 1 << 16;
+    while (cap < host_bytes) cap *= 2;
+    void* p = nullptr;
+    MTL_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
+    if (h.host) hipHostFree(h.host);
+    h.host = p;
+    h.host_cap = cap;
+  }
+  return MTL_OK;
+}
+}  // namespace
+
 int32_t mtl_scalar_hook(int32_t hook, int32_t type, const void* values, uint64_t n,
                         int64_t scalar_bits, void* out) {
   using namespace mtl;
@@ -1594,32 +1660,44 @@ int32_t mtl_scalar_hook(int32_t hook, int32_t type, const void* values, uint64_t
   if (!w || type == MTL_HUGEINT) return set_error(MTL_UNSUPPORTED, "hook on type %d", type);
   if (n == 0) return MTL_OK;
   MTL_TRY(ensure_device());
-  sql::DevBuf in, res, flag;
-  MTL_TRY(in.alloc(n * w));
-  MTL_TRY(mtl_memcpy_h2d(in.p, values, n * w, nullptr));
-  if (hook == MTL_HOOK_ADD || hook == MTL_HOOK_MUL) {
-    MTL_TRY(res.alloc(n * w));
-    MTL_TRY(flag.alloc(16));
-    uint32_t zero = 0;
-    MTL_TRY(mtl_memcpy_h2d(flag.p, &zero, 4, nullptr));
-    MTL_TRY(mtl_project(in.p, type, nullptr, nullptr, n,
-                        hook == MTL_HOOK_ADD ? MTL_PROJ_ADD : MTL_PROJ_MUL, scalar_bits, res.p,
-                        (uint32_t*)flag.p, nullptr));
-    uint32_t f = 0;
-    MTL_TRY(mtl_memcpy_d2h(&f, flag.p, 4, nullptr));
-    if (f)
-      return set_error(MTL_ERR_OVERFLOW, "Overflow in %s",
-                       hook == MTL_HOOK_ADD ? "addition" : "multiplication");
-    return mtl_memcpy_d2h(out, res.p, n * w, nullptr);
+  const bool proj = hook == MTL_HOOK_ADD || hook == MTL_HOOK_MUL;
+  const uint64_t in_b = (n * w + 255) & ~uint64_t(255);
+  const uint64_t out_w = proj ? (uint64_t)w : 1u;
+  const uint64_t res_b = (n * out_w + 255) & ~uint64_t(255);
+  MTL_TRY(hook_reserve(in_b + res_b + 256, in_b + res_b + 256));
+  HookBufs& h = t_hook;
+  hipStream_t st = h.s->stream;
+  char* din = (char*)h.dev;
+  char* dres = din + in_b;
+  uint32_t* dflag = (uint32_t*)(dres + res_b);
+  char* hin = (char*)h.host;
+  char* hres = hin + in_b;
+  const uint32_t* hflag = (const uint32_t*)(hres + res_b);
+  std::memcpy(hin, values, n * w);
+  MTL_HIP(hipMemcpyAsync(din, hin, n * w, hipMemcpyHostToDevice, st));
+  if (proj) {
+    ZeroList zl;
+    zl.add(dflag, 4);
+    MTL_TRY(launch_zero(zl, st));
+    MTL_TRY(mtl_project(din, type, nullptr, nullptr, n,
+                        hook == MTL_HOOK_ADD ? MTL_PROJ_ADD : MTL_PROJ_MUL, scalar_bits, dres,
+                        dflag, h.s));
+    MTL_HIP(hipMemcpyAsync(hres, dres, res_b + 16, hipMemcpyDeviceToHost, st));
+  } else {
+    mtl_pred p{};
+    p.column = din;
+    p.type = type;
+    p.op = hook - MTL_HOOK_EQ;  // EQ LT LE GT GE in compare-op order
+    p.scalar = scalar_bits;
+    MTL_TRY(mtl_compare(&p, 1, n, (uint8_t*)dres, h.s));
+    MTL_HIP(hipMemcpyAsync(hres, dres, n, hipMemcpyDeviceToHost, st));
   }
-  mtl_pred p{};
-  p.column = in.p;
-  p.type = type;
-  p.op = hook - MTL_HOOK_EQ;  // EQ LT LE GT GE in compare-op order
-  p.scalar = scalar_bits;
-  MTL_TRY(res.alloc(n));
-  MTL_TRY(mtl_compare(&p, 1, n, (uint8_t*)res.p, nullptr));
-  return mtl_memcpy_d2h(out, res.p, n, nullptr);
+  MTL_HIP(hipStreamSynchronize(st));
+  if (proj && hflag[0])
+    return set_error(MTL_ERR_OVERFLOW, "Overflow in %s",
+                     hook == MTL_HOOK_ADD ? "addition" : "multiplication");
+  std::memcpy(out, hres, n * out_w);
+  return MTL_OK;
 }
 
 int32_t mtl_rewrite_query(const char* sql, char* buf, size_t len, size_t* needed) {

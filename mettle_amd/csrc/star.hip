@@ -35,6 +35,7 @@
 #include "mtl_internal.hpp"
 #include "prof.hpp"
 #include "rowtile.hpp"
+#include "star.hpp"
 #include "topk.hpp"
 
 namespace mtl {
@@ -914,6 +915,151 @@ static int32_t star_validate(const mtl_star_desc* d, StarPlan* sp) {
   return MTL_OK;
 }
 
+// One star query's device scratch (star_enqueue and the shared batch).
+struct StarBufs {
+  void *dim_bm = nullptr, *dbits = nullptr, *par_bm, *par_rank, *pbits, *row_of, *acc, *cnt,
+       *misc;
+  void *st_r, *ck[4], *cid, *lk0[2], *lg[2], *ln[2], *bcmax, *tmax;
+  uint32_t rtiles = 0;
+  uint32_t n_tmax = 0;  // entries of tmax (the overflow bound's per-block maxima)
+};
+
+// Steps 3 and 4: prefix popcounts of the parent bitmap, then the slot map
+// (row_of) and the cleared accumulators.
+static int32_t launch_rank_map(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b,
+                               mtl_stream* s) {
+  hipStream_t st = s->stream;
+  const uint64_t prows = d->p_rows;
+  const uint32_t ptiles = filter_tiles(prows);
+  uint32_t* mu = (uint32_t*)b.misc;
+  {
+    ProfScope ps(s, "k_rank");
+    PopArgs pa{};
+    pa.pbits = (const uint64_t*)b.pbits;
+    pa.pbits_words = (uint64_t)ptiles * kTileRows / 64;
+    if (d->d_rows) {
+      pa.dim_bm = (const uint64_t*)b.dim_bm;
+      pa.dim_words = sp.dwords;
+      pa.dbits = (const uint64_t*)b.dbits;
+      pa.dbits_words = (uint64_t)filter_tiles(d->d_rows) * kTileRows / 64;
+    }
+    pa.pops = mu + 7;  // [7] P pass bits, [8] D bitmap, [9] D pass bits
+    const uint32_t rt = std::max<uint32_t>(b.rtiles, 1);
+    const uint64_t pop_words = std::max(pa.pbits_words, std::max(pa.dim_words, pa.dbits_words));
+    const uint32_t pop_blocks =
+        (uint32_t)std::min<uint64_t>(ceil_div(pop_words, (uint64_t)RBLOCK * 8), 2048);
+    hipLaunchKernelGGL(k_rank, dim3(std::max(rt, pop_blocks)), dim3(RBLOCK), 0, st,
+                       (const uint64_t*)b.par_bm, sp.pwords, rt, (uint32_t*)b.par_rank,
+                       (uint64_t*)b.st_r, mu + 2, mu + 3, mu + 6, pa);
+  }
+  MTL_HIP(hipGetLastError());
+  MapArgs ma{};
+  ma.sel_bm = (const uint64_t*)b.pbits;
+  ma.n_rows = prows;
+  ma.key = d->p_key;
+  ma.key_type = d->p_key_type;
+  ma.min = d->p_key_min;
+  ma.span = sp.pspan;
+  ma.bm = (const uint64_t*)b.par_bm;
+  ma.rank = (const uint32_t*)b.par_rank;
+  ma.row_of = (uint32_t*)b.row_of;
+  ma.acc = (uint64_t*)b.acc;
+  ma.cnt = (uint32_t*)b.cnt;
+  ma.par_total = mu + 6;
+  ma.pops = mu + 7;
+  ma.has_dim = d->d_rows ? 1u : 0u;
+  ma.err = mu + 3;
+  uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
+  ProfScope ps(s, "k_map");
+  if (type_width(d->p_key_type) == 4 && d->p_key_type != MTL_FLOAT &&
+      ((uintptr_t)d->p_key % 16) == 0)
+    hipLaunchKernelGGL(k_map4, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
+  else
+    hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
+}
+
+// Step 6: top-K over the accumulators and the output rows.
+static int32_t launch_star_topk(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b,
+                                mtl_star_row* out, uint32_t* out_n, mtl_stream* s) {
+  hipStream_t st = s->stream;
+  uint32_t* mu = (uint32_t*)b.misc;
+  TopkArgs ta{};
+  ta.m_count = mu + 6;
+  ta.acc = (const uint64_t*)b.acc;
+  ta.cnt = (const uint32_t*)b.cnt;
+  ta.row_of = (const uint32_t*)b.row_of;
+  ta.is_double = d->agg.is_double;
+  ta.k = d->k;
+  ta.n_ties = d->n_ties;
+  for (uint32_t t = 0; t < d->n_ties; ++t) {
+    ta.tie_col[t] = d->tie_col[t];
+    ta.tie_type[t] = d->tie_type[t];
+    ta.tie_desc[t] = d->tie_desc[t];
+  }
+  for (int f = 0; f < 4; ++f) ta.stage.ck[f] = (uint64_t*)b.ck[f];
+  ta.stage.cid = (uint32_t*)b.cid;
+  TopkLists lists[2];
+  for (int i = 0; i < 2; ++i)
+    lists[i] = TopkLists{(uint64_t*)b.lk0[i], (uint32_t*)b.lg[i], (uint32_t*)b.ln[i]};
+  ta.lists0 = lists[0];
+  ta.out = out;
+  ta.out_n = out_n;
+  ta.err = mu + 3;
+  ta.tmax = (const unsigned long long*)b.tmax;
+  ta.n_tmax = d->agg.is_double ? 0u : b.n_tmax;
+  ta.bcmax = (uint32_t*)b.bcmax;
+  ta.n_bcmax = sp.topk_grid;
+  ta.p_rows = d->p_rows;
+  {
+    ProfScope ps(s, "k_star_topk");
+    if (d->k <= 64)
+      hipLaunchKernelGGL(k_star_topk<1>, dim3(sp.topk_grid), dim3(TopkShape<1>::NT), 0, st, ta);
+    else if (d->k <= 128)
+      hipLaunchKernelGGL(k_star_topk<2>, dim3(sp.topk_grid), dim3(TopkShape<2>::NT), 0, st, ta);
+    else
+      hipLaunchKernelGGL(k_star_topk<8>, dim3(sp.topk_grid), dim3(TopkShape<8>::NT), 0, st, ta);
+  }
+  MTL_HIP(hipGetLastError());
+  // merge levels: F lists per block until one list remains; the last
+  // level (one block) writes the rows
+  ProfScope ps2(s, "k_star_topk_merge");
+  const uint32_t fan = topk_fan(d->k);
+  uint32_t L = sp.topk_grid;
+  int in = 0;
+  while (true) {
+    const uint32_t blocks = (uint32_t)ceil_div(L, fan);
+    const uint32_t fin = blocks == 1 ? 1u : 0u;
+    hipLaunchKernelGGL(k_star_topk_merge, dim3(blocks), dim3(TOPK_MERGE_THREADS), 0, st, ta,
+                       lists[in], lists[in ^ 1], L, fin);
+    MTL_HIP(hipGetLastError());
+    if (fin) break;
+    L = blocks;
+    in ^= 1;
+  }
+  return MTL_OK;
+}
+
+// Top-K scratch of one query (staged wave-list entries, two ping-pong sets
+// of lists, per-block count maxima) from the arena.
+static int32_t alloc_topk_bufs(const mtl_star_desc* d, const StarPlan& sp, Arena& ar,
+                               StarBufs& b) {
+  const uint64_t tk_waves = d->k <= 128 ? 16 : 4;
+  const uint64_t stage_n = (uint64_t)sp.topk_grid * tk_waves * d->k;
+  for (int f = 0; f < 4; ++f) MTL_TRY(ar.alloc(stage_n * 8, &b.ck[f]));
+  MTL_TRY(ar.alloc(stage_n * 4, &b.cid));
+  const uint64_t la = sp.topk_grid, lb = ceil_div(sp.topk_grid, topk_fan(d->k));
+  MTL_TRY(ar.alloc(la * d->k * 8, &b.lk0[0]));
+  MTL_TRY(ar.alloc(la * d->k * 4, &b.lg[0]));
+  MTL_TRY(ar.alloc(la * 4, &b.ln[0]));
+  MTL_TRY(ar.alloc(lb * d->k * 8, &b.lk0[1]));
+  MTL_TRY(ar.alloc(lb * d->k * 4, &b.lg[1]));
+  MTL_TRY(ar.alloc(lb * 4, &b.ln[1]));
+  MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * 4, &b.bcmax));
+  return MTL_OK;
+}
+
 // Enqueue the whole pipeline on s.  out (device, k rows) and out_n (device,
 // 2 x uint32: rows, error bits) receive the result.
 static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stream* s,
@@ -921,50 +1067,37 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
   hipStream_t st = s->stream;
   Arena& ar = s->arena;
   const uint64_t prows = d->p_rows;
-  void *dim_bm = nullptr, *dbits = nullptr, *par_bm, *par_rank, *pbits, *row_of, *acc, *cnt,
-       *misc;
-  void *st_r, *ck[4], *cid, *lk0[2], *lg[2], *ln[2], *bcmax, *tmax;
+  StarBufs b;
   if (d->d_rows) {
-    MTL_TRY(ar.alloc(sp.dwords * 8, &dim_bm));
-    MTL_TRY(ar.alloc((uint64_t)filter_tiles(d->d_rows) * kTileRows / 8, &dbits));
+    MTL_TRY(ar.alloc(sp.dwords * 8, &b.dim_bm));
+    MTL_TRY(ar.alloc((uint64_t)filter_tiles(d->d_rows) * kTileRows / 8, &b.dbits));
   }
   // one zero word past the end: the fact scan reads two-word windows
-  MTL_TRY(ar.alloc((sp.pwords + 1) * 8, &par_bm));
-  MTL_TRY(ar.alloc(sp.pwords * 4, &par_rank));
-  uint32_t ptiles = filter_tiles(prows);
-  MTL_TRY(ar.alloc((uint64_t)ptiles * kTileRows / 8, &pbits));
-  MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &row_of));
-  MTL_TRY(ar.alloc((prows ? prows : 1) * 8, &acc));
-  MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &cnt));
+  MTL_TRY(ar.alloc((sp.pwords + 1) * 8, &b.par_bm));
+  MTL_TRY(ar.alloc(sp.pwords * 4, &b.par_rank));
+  const uint32_t ptiles = filter_tiles(prows);
+  MTL_TRY(ar.alloc((uint64_t)ptiles * kTileRows / 8, &b.pbits));
+  MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &b.row_of));
+  MTL_TRY(ar.alloc((prows ? prows : 1) * 8, &b.acc));
+  MTL_TRY(ar.alloc((prows ? prows : 1) * 4, &b.cnt));
   // misc words: [2] rank ticket, [3] error bits,
   // [6] parent bitmap popcount, [7] P pass-bit popcount, [8] dimension
   // bitmap popcount, [9] D pass-bit popcount
-  MTL_TRY(ar.alloc(256, &misc));
-  uint32_t rtiles = (uint32_t)ceil_div(sp.pwords, RTILE);
-  MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(rtiles, 1) * 8, &st_r));
-  // top-K: staged wave-list entries, then two ping-pong sets of lists
-  const uint64_t tk_waves = d->k <= 128 ? 16 : 4;
-  const uint64_t stage_n = (uint64_t)sp.topk_grid * tk_waves * d->k;
-  for (int f = 0; f < 4; ++f) MTL_TRY(ar.alloc(stage_n * 8, &ck[f]));
-  MTL_TRY(ar.alloc(stage_n * 4, &cid));
-  const uint64_t la = sp.topk_grid, lb = ceil_div(sp.topk_grid, topk_fan(d->k));
-  MTL_TRY(ar.alloc(la * d->k * 8, &lk0[0]));
-  MTL_TRY(ar.alloc(la * d->k * 4, &lg[0]));
-  MTL_TRY(ar.alloc(la * 4, &ln[0]));
-  MTL_TRY(ar.alloc(lb * d->k * 8, &lk0[1]));
-  MTL_TRY(ar.alloc(lb * d->k * 4, &lg[1]));
-  MTL_TRY(ar.alloc(lb * 4, &ln[1]));
-  MTL_TRY(ar.alloc((uint64_t)sp.topk_grid * 4, &bcmax));
+  MTL_TRY(ar.alloc(256, &b.misc));
+  b.rtiles = (uint32_t)ceil_div(sp.pwords, RTILE);
+  MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(b.rtiles, 1) * 8, &b.st_r));
+  MTL_TRY(alloc_topk_bufs(d, sp, ar, b));
   const uint32_t ftiles = d->f_rows ? filter_tiles(d->f_rows) : 0u;
   const uint32_t acc_blocks = (uint32_t)ceil_div(ftiles, kAccTiles);
-  MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(acc_blocks, 1) * 8, &tmax));
-  uint32_t* mu = (uint32_t*)misc;
+  MTL_TRY(ar.alloc((uint64_t)std::max<uint32_t>(acc_blocks, 1) * 8, &b.tmax));
+  b.n_tmax = acc_blocks;
+  uint32_t* mu = (uint32_t*)b.misc;
 
   ZeroList zl;
-  if (dim_bm) zl.add((uint32_t*)dim_bm, sp.dwords * 2);
-  zl.add((uint32_t*)par_bm, (sp.pwords + 1) * 2);
-  zl.add((uint32_t*)misc, 64);
-  zl.add((uint32_t*)st_r, (uint64_t)std::max<uint32_t>(rtiles, 1) * 2);
+  if (b.dim_bm) zl.add((uint32_t*)b.dim_bm, sp.dwords * 2);
+  zl.add((uint32_t*)b.par_bm, (sp.pwords + 1) * 2);
+  zl.add((uint32_t*)b.misc, 64);
+  zl.add((uint32_t*)b.st_r, (uint64_t)std::max<uint32_t>(b.rtiles, 1) * 2);
   MTL_TRY(launch_zero(zl, st));
 
   // 1. dimension bitmap
@@ -975,12 +1108,12 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.n_preds = d->d_n_preds;
     fa.n_rows = d->d_rows;
     fa.err = mu + 3;
-    fa.mark_bm = (uint64_t*)dim_bm;
+    fa.mark_bm = (uint64_t*)b.dim_bm;
     fa.mark_key = d->d_key;
     fa.mark_key_type = d->d_key_type;
     fa.mark_min = d->d_key_min;
     fa.mark_span = sp.dspan;
-    fa.sel_bm = (uint64_t*)dbits;
+    fa.sel_bm = (uint64_t*)b.dbits;
     ProfScope ps(s, "k_mark_dim");
     MTL_TRY(launch_filter(fa, d->d_rows, st));
   }
@@ -990,16 +1123,16 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     for (uint32_t p = 0; p < d->p_n_preds; ++p) fa.preds[p] = compile_pred(d->p_preds[p]);
     fa.n_preds = d->p_n_preds;
     fa.n_rows = prows;
-    fa.sel_bm = (uint64_t*)pbits;
+    fa.sel_bm = (uint64_t*)b.pbits;
     fa.err = mu + 3;
     if (d->d_rows) {
-      fa.probe_bm = (const uint64_t*)dim_bm;
+      fa.probe_bm = (const uint64_t*)b.dim_bm;
       fa.probe_key = d->p_fk;
       fa.probe_key_type = d->p_fk_type;
       fa.probe_min = d->d_key_min;
       fa.probe_span = sp.dspan;
     }
-    fa.mark_bm = (uint64_t*)par_bm;
+    fa.mark_bm = (uint64_t*)b.par_bm;
     fa.mark_key = d->p_key;
     fa.mark_key_type = d->p_key_type;
     fa.mark_min = d->p_key_min;
@@ -1007,55 +1140,8 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     ProfScope ps(s, "k_filter_parent");
     MTL_TRY(launch_filter(fa, prows, st));
   }
-  // 3. rank
-  {
-    ProfScope ps(s, "k_rank");
-    PopArgs pa{};
-    pa.pbits = (const uint64_t*)pbits;
-    pa.pbits_words = (uint64_t)ptiles * kTileRows / 64;
-    if (d->d_rows) {
-      pa.dim_bm = (const uint64_t*)dim_bm;
-      pa.dim_words = sp.dwords;
-      pa.dbits = (const uint64_t*)dbits;
-      pa.dbits_words = (uint64_t)filter_tiles(d->d_rows) * kTileRows / 64;
-    }
-    pa.pops = mu + 7;  // [7] P pass bits, [8] D bitmap, [9] D pass bits
-    const uint32_t rt = std::max<uint32_t>(rtiles, 1);
-    const uint64_t pop_words = std::max(pa.pbits_words, std::max(pa.dim_words, pa.dbits_words));
-    const uint32_t pop_blocks =
-        (uint32_t)std::min<uint64_t>(ceil_div(pop_words, (uint64_t)RBLOCK * 8), 2048);
-    hipLaunchKernelGGL(k_rank, dim3(std::max(rt, pop_blocks)), dim3(RBLOCK), 0, st,
-                       (const uint64_t*)par_bm, sp.pwords, rt, (uint32_t*)par_rank,
-                       (uint64_t*)st_r, mu + 2, mu + 3, mu + 6, pa);
-  }
-  MTL_HIP(hipGetLastError());
-  // 4. map accumulator slots to P rows, clear accumulators
-  {
-    MapArgs ma{};
-    ma.sel_bm = (const uint64_t*)pbits;
-    ma.n_rows = prows;
-    ma.key = d->p_key;
-    ma.key_type = d->p_key_type;
-    ma.min = d->p_key_min;
-    ma.span = sp.pspan;
-    ma.bm = (const uint64_t*)par_bm;
-    ma.rank = (const uint32_t*)par_rank;
-    ma.row_of = (uint32_t*)row_of;
-    ma.acc = (uint64_t*)acc;
-    ma.cnt = (uint32_t*)cnt;
-    ma.par_total = mu + 6;
-    ma.pops = mu + 7;
-    ma.has_dim = d->d_rows ? 1u : 0u;
-    ma.err = mu + 3;
-    uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
-    ProfScope ps(s, "k_map");
-    if (type_width(d->p_key_type) == 4 && d->p_key_type != MTL_FLOAT &&
-        ((uintptr_t)d->p_key % 16) == 0)
-      hipLaunchKernelGGL(k_map4, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
-    else
-      hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
-    MTL_HIP(hipGetLastError());
-  }
+  // 3-4. rank, slot map
+  MTL_TRY(launch_rank_map(d, sp, b, s));
   // 5. fact pass
   if (d->f_rows) {
     FactArgs fa{};
@@ -1064,8 +1150,8 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     fa.fk_type = d->f_fk_type;
     fa.min = d->p_key_min;
     fa.span = sp.pspan;
-    fa.bm = (const uint64_t*)par_bm;
-    fa.rank = (const uint32_t*)par_rank;
+    fa.bm = (const uint64_t*)b.par_bm;
+    fa.rank = (const uint32_t*)b.par_rank;
     for (uint32_t p = 0; p < d->f_n_preds; ++p) fa.preds[p] = compile_pred(d->f_preds[p]);
     fa.n_preds = d->f_n_preds;
     fa.expr = d->agg;
@@ -1077,11 +1163,11 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
         fa.expr.col[c] = d->agg.col[0];
         fa.expr.col_type[c] = d->agg.col_type[0];
       }
-    fa.acc = (uint64_t*)acc;
-    fa.cnt = (uint32_t*)cnt;
+    fa.acc = (uint64_t*)b.acc;
+    fa.cnt = (uint32_t*)b.cnt;
     fa.err = mu + 3;
     fa.fk_vec = ((uintptr_t)d->f_fk % 16) == 0;
-    fa.tmax = (unsigned long long*)tmax;
+    fa.tmax = (unsigned long long*)b.tmax;
     fa.pred0_vec = d->f_n_preds && ((uintptr_t)d->f_preds[0].column % 16) == 0;
     void *hrow, *hslot, *hcnt;
     MTL_TRY(ar.alloc((uint64_t)ftiles * FTILE * 4, &hrow));
@@ -1127,62 +1213,7 @@ static int32_t star_enqueue(const mtl_star_desc* d, const StarPlan& sp, mtl_stre
     MTL_HIP(hipGetLastError());
   }
   // 6. top-K + output rows
-  {
-    TopkArgs ta{};
-    ta.m_count = mu + 6;
-    ta.acc = (const uint64_t*)acc;
-    ta.cnt = (const uint32_t*)cnt;
-    ta.row_of = (const uint32_t*)row_of;
-    ta.is_double = d->agg.is_double;
-    ta.k = d->k;
-    ta.n_ties = d->n_ties;
-    for (uint32_t t = 0; t < d->n_ties; ++t) {
-      ta.tie_col[t] = d->tie_col[t];
-      ta.tie_type[t] = d->tie_type[t];
-      ta.tie_desc[t] = d->tie_desc[t];
-    }
-    for (int f = 0; f < 4; ++f) ta.stage.ck[f] = (uint64_t*)ck[f];
-    ta.stage.cid = (uint32_t*)cid;
-    TopkLists lists[2];
-    for (int b = 0; b < 2; ++b)
-      lists[b] = TopkLists{(uint64_t*)lk0[b], (uint32_t*)lg[b], (uint32_t*)ln[b]};
-    ta.lists0 = lists[0];
-    ta.out = out;
-    ta.out_n = out_n;
-    ta.err = mu + 3;
-    ta.tmax = (const unsigned long long*)tmax;
-    ta.n_tmax = d->agg.is_double ? 0u : acc_blocks;
-    ta.bcmax = (uint32_t*)bcmax;
-    ta.n_bcmax = sp.topk_grid;
-    ta.p_rows = prows;
-    {
-      ProfScope ps(s, "k_star_topk");
-      if (d->k <= 64)
-        hipLaunchKernelGGL(k_star_topk<1>, dim3(sp.topk_grid), dim3(TopkShape<1>::NT), 0, st, ta);
-      else if (d->k <= 128)
-        hipLaunchKernelGGL(k_star_topk<2>, dim3(sp.topk_grid), dim3(TopkShape<2>::NT), 0, st, ta);
-      else
-        hipLaunchKernelGGL(k_star_topk<8>, dim3(sp.topk_grid), dim3(TopkShape<8>::NT), 0, st, ta);
-    }
-    MTL_HIP(hipGetLastError());
-    // merge levels: F lists per block until one list remains; the last
-    // level (one block) writes the rows
-    ProfScope ps2(s, "k_star_topk_merge");
-    const uint32_t fan = topk_fan(d->k);
-    uint32_t L = sp.topk_grid;
-    int in = 0;
-    while (true) {
-      const uint32_t blocks = (uint32_t)ceil_div(L, fan);
-      const uint32_t fin = blocks == 1 ? 1u : 0u;
-      hipLaunchKernelGGL(k_star_topk_merge, dim3(blocks), dim3(TOPK_MERGE_THREADS), 0, st, ta,
-                         lists[in], lists[in ^ 1], L, fin);
-      MTL_HIP(hipGetLastError());
-      if (fin) break;
-      L = blocks;
-      in ^= 1;
-    }
-  }
-  return MTL_OK;
+  return launch_star_topk(d, sp, b, out, out_n, s);
 }
 
 static int32_t star_error(uint32_t err) {
@@ -1245,3 +1276,469 @@ int32_t mtl_star_topk(const mtl_star_desc* desc, mtl_star_row* out_rows, uint32_
 extern "C" int32_t mtl_star_status(uint32_t error_bits) {
   return mtl::star_check_error(error_bits);
 }
+
+// ---------------------------------------------------------------------------
+// Shared passes of a batch of star queries (star.hpp).  Each pass streams
+// its table once: every 4096-row tile's key and predicate columns are loaded
+// together, then each query of the batch evaluates its own predicate and
+// row count on the loaded values and writes its own pass bits and bitmap
+// marks.  The fact pass adds each hit straight into its query's accumulator
+// (rank, expression columns, value): there are no hit lists, so a batch
+// needs no scratch proportional to the fact table.
+// ---------------------------------------------------------------------------
+namespace mtl {
+
+struct MQStarQ {
+  uint64_t d_rows, p_rows, f_rows;
+  PredC dpred, ppred, fpred;
+  uint64_t pspan;  // parent key span of this query
+  uint64_t* dim_bm;
+  uint64_t* dbits;
+  uint64_t* pbits;
+  uint64_t* par_bm;
+  const uint32_t* rank;
+  uint64_t* acc;
+  uint32_t* cnt;
+  uint32_t* err;
+  unsigned long long* tmax;  // max |value| added (overflow bound)
+};
+
+struct MQStarArgs {
+  uint32_t nq;
+  const MQStarQ* q;
+  uint64_t n_max;  // rows of the streamed table (largest over the batch)
+  const void* d_key;
+  int32_t d_key_type;
+  uint32_t d_kvec;
+  const void* d_pcol;
+  uint32_t d_pw, d_pvec;  // predicate column width (0: none), alignment
+  int64_t d_min;
+  uint64_t dspan;
+  const void* p_key;
+  int32_t p_key_type;
+  uint32_t p_kvec;
+  const void* p_fk;
+  int32_t p_fk_type;
+  uint32_t p_fvec;
+  const void* p_pcol;
+  uint32_t p_pw, p_pvec;
+  int64_t p_min;
+  const void* f_fk;
+  int32_t f_fk_type;
+  uint32_t f_fvec;
+  const void* f_pcol;
+  uint32_t f_pw, f_pvec;
+  mtl_expr expr;
+};
+
+// Bits of the lane's 16 rows that lie below n.
+__device__ __forceinline__ uint32_t rows_below(uint64_t k0, uint64_t n) {
+  uint32_t bits = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
+  return bits;
+}
+
+__device__ __forceinline__ void pred_u32x16(const PredC& p, const uint32_t (&x)[FJ][4],
+                                            uint32_t& bits) {
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (!pred_pass(enc_raw(x[j][v], p.kind), p)) bits &= ~(1u << (4 * j + v));
+}
+
+// Set bit off[j][v] of bm for every row with a set bit (0xFFFFFFFF: key
+// outside its range, error bit 8).  A lane's rows of a step share a word
+// when the keys are sorted; the lanes of a 16-lane row whose word is the
+// row's last lane's word OR their bits together first, so sorted keys cost
+// one atomic per 64 keys.
+__device__ __forceinline__ void mark_global(uint64_t* bm, const uint32_t (&off)[FJ][4],
+                                            uint32_t bits, uint32_t* err, int lane) {
+  uint32_t e = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const uint32_t m = (bits >> (4 * j)) & 0xFu;
+    uint32_t w = 0xFFFFFFFFu, rest = 0;
+    uint64_t b = 0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      if (!((m >> v) & 1u)) continue;
+      const uint32_t s = off[j][v];
+      if (s == 0xFFFFFFFFu) {
+        e |= 8u;
+        continue;
+      }
+      if (w == 0xFFFFFFFFu) w = s >> 6;
+      if ((s >> 6) == w) b |= 1ull << (s & 63);
+      else rest |= 1u << v;
+    }
+    const uint32_t wl = (uint32_t)__shfl((int)w, lane | 15, WAVE);
+    const bool join = b != 0 && w == wl;
+    unsigned long long jb = join ? b : 0ull;
+    jb |= __shfl_xor(jb, 1, WAVE);
+    jb |= __shfl_xor(jb, 2, WAVE);
+    jb |= __shfl_xor(jb, 4, WAVE);
+    jb |= __shfl_xor(jb, 8, WAVE);
+    if ((lane & 15) == 15 && jb) atomicOr((unsigned long long*)&bm[wl], jb);
+    if (b && !join) atomicOr((unsigned long long*)&bm[w], (unsigned long long)b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if ((rest >> v) & 1u) {
+        const uint32_t s = off[j][v];
+        atomicOr((unsigned long long*)&bm[s >> 6], 1ull << (s & 63));
+      }
+  }
+  if (e) atomicOr(err, e);
+}
+
+// Dimension pass: each query's pass bits and key bitmap.
+__global__ __launch_bounds__(FBLOCK) void k_mq_dim(MQStarArgs a) {
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
+  const uint64_t k0 = seg + (uint64_t)lane * 4;
+  uint32_t px[FJ][4], ko[FJ][4];
+  if (a.d_pw) load_u32x16(a.d_pcol, a.d_pw, k0, a.n_max, a.d_pvec, px);
+  load_u32x16(a.d_key, a.d_key_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.d_kvec, ko);
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ko[j][v] = narrow_off(ko[j][v], a.d_key_type, a.d_min, a.dspan);
+  for (uint32_t q = 0; q < a.nq; ++q) {
+    const MQStarQ& Q = a.q[q];
+    if ((uint64_t)tile * FTILE >= Q.d_rows) continue;  // block-uniform
+    uint32_t bits = rows_below(k0, Q.d_rows);
+    if (a.d_pw) pred_u32x16(Q.dpred, px, bits);
+    store_pass_bits(Q.dbits, seg, bits, lane);
+    mark_global(Q.dim_bm, ko, bits, Q.err, lane);
+  }
+}
+
+// Parent pass: predicate, semi-join probe of the query's dimension bitmap,
+// pass bits and parent key bitmap.
+__global__ __launch_bounds__(FBLOCK) void k_mq_parent(MQStarArgs a) {
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
+  const uint64_t k0 = seg + (uint64_t)lane * 4;
+  uint32_t px[FJ][4], fo[FJ][4], ko[FJ][4];
+  if (a.p_pw) load_u32x16(a.p_pcol, a.p_pw, k0, a.n_max, a.p_pvec, px);
+  load_u32x16(a.p_fk, a.p_fk_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.p_fvec, fo);
+  load_u32x16(a.p_key, a.p_key_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.p_kvec, ko);
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      fo[j][v] = narrow_off(fo[j][v], a.p_fk_type, a.d_min, a.dspan);
+      ko[j][v] = narrow_off(ko[j][v], a.p_key_type, a.p_min, 1ull << 32);
+    }
+  for (uint32_t q = 0; q < a.nq; ++q) {
+    const MQStarQ& Q = a.q[q];
+    if ((uint64_t)tile * FTILE >= Q.p_rows) continue;  // block-uniform
+    uint32_t bits = rows_below(k0, Q.p_rows);
+    if (a.p_pw) pred_u32x16(Q.ppred, px, bits);
+    const uint32_t* bm32 = (const uint32_t*)Q.dim_bm;
+    uint32_t w[FJ][4];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const bool need = ((bits >> (4 * j + v)) & 1u) && fo[j][v] != 0xFFFFFFFFu;
+        w[j][v] = need ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
+      }
+    uint32_t mo[FJ][4];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if (fo[j][v] == 0xFFFFFFFFu || !((w[j][v] >> (fo[j][v] & 31)) & 1u))
+          bits &= ~(1u << (4 * j + v));
+        mo[j][v] = ko[j][v] < Q.pspan ? ko[j][v] : 0xFFFFFFFFu;
+      }
+    store_pass_bits(Q.pbits, seg, bits, lane);
+    mark_global(Q.par_bm, mo, bits, Q.err, lane);
+  }
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor((unsigned long long)v, off, WAVE);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Fact pass: predicate and parent probe per query; each hit's slot (rank),
+// expression columns and value, summed per lane over runs of one slot and
+// added to the query's accumulator.
+template <bool DBL>
+__global__ __launch_bounds__(FBLOCK) void k_mq_fact(MQStarArgs a) {
+  using V = typename std::conditional<DBL, double, int64_t>::type;
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
+  const uint64_t k0 = seg + (uint64_t)lane * 4;
+  uint32_t px[FJ][4], fo[FJ][4];
+  if (a.f_pw) load_u32x16(a.f_pcol, a.f_pw, k0, a.n_max, a.f_pvec, px);
+  load_u32x16(a.f_fk, a.f_fk_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.f_fvec, fo);
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) fo[j][v] = narrow_off(fo[j][v], a.f_fk_type, a.p_min, 1ull << 32);
+  for (uint32_t q = 0; q < a.nq; ++q) {
+    const MQStarQ& Q = a.q[q];
+    if ((uint64_t)tile * FTILE >= Q.f_rows) continue;  // block-uniform
+    uint32_t cand = rows_below(k0, Q.f_rows);
+    if (a.f_pw) pred_u32x16(Q.fpred, px, cand);
+    const uint32_t* bm32 = (const uint32_t*)Q.par_bm;
+    uint32_t w[FJ][4];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if (fo[j][v] >= Q.pspan) cand &= ~(1u << (4 * j + v));
+        w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
+      }
+    uint32_t hits = 0;
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (fo[j][v] & 31)) & 1u))
+          hits |= 1u << (4 * j + v);
+    if (!__ballot(hits != 0)) continue;  // wave-uniform: most waves of a sparse join
+    uint32_t cur = 0xFFFFFFFFu, cn = 0;
+    V cs = 0;
+    uint64_t mx = 0;
+    bool ovf = false;
+    uint32_t h = hits;
+    while (h) {
+      const int i = __ffs(h) - 1;
+      h &= h - 1;
+      uint32_t sv = fo[0][0];
+#pragma unroll
+      for (int t = 1; t < FJ * 4; ++t) sv = i == t ? fo[t >> 2][t & 3] : sv;
+      const uint64_t row = k0 + (uint64_t)(i >> 2) * 256 + (uint64_t)(i & 3);
+      const uint64_t wd = Q.par_bm[sv >> 6];
+      const uint32_t rk = Q.rank[sv >> 6];
+      V cv[MTL_EXPR_MAX_COLS];
+#pragma unroll
+      for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c) {
+        if ((uint32_t)c < a.expr.n_cols) {
+          if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
+          else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
+        } else {
+          cv[c] = 0;
+        }
+      }
+      const uint64_t below = (sv & 63) ? (wd & ((1ull << (sv & 63)) - 1)) : 0ull;
+      const uint32_t r = rk + (uint32_t)__popcll(below);
+      V val;
+      if constexpr (DBL) {
+        val = expr_f64_vals(a.expr, cv);
+      } else {
+        val = expr_i64_vals(a.expr, cv, ovf);
+        const uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
+        mx = av > mx ? av : mx;
+      }
+      if (r != cur) {
+        if (cur != 0xFFFFFFFFu) {
+          if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
+          else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
+          atomicAdd(&Q.cnt[cur], cn);
+        }
+        cur = r;
+        cs = 0;
+        cn = 0;
+      }
+      if constexpr (DBL) cs += val;
+      else cs = (int64_t)((uint64_t)cs + (uint64_t)val);  // wrapping, as the atomics
+      ++cn;
+    }
+    if (cur != 0xFFFFFFFFu) {
+      if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
+      else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
+      atomicAdd(&Q.cnt[cur], cn);
+    }
+    if (ovf) atomicOr(Q.err, 16u);
+    if constexpr (!DBL) {
+      mx = wave_max_u64(mx);
+      if (lane == 0 && mx) atomicMax(Q.tmax, (unsigned long long)mx);
+    }
+  }
+}
+
+static bool narrow_key_type(int32_t t) {
+  return t == MTL_INTEGER || t == MTL_DATE || t == MTL_DICT32 || t == MTL_DICT8;
+}
+
+static bool narrow_pred(const mtl_pred* p, uint32_t n) {
+  return n == 0 || (n == 1 && type_width(p[0].type) >= 1 && type_width(p[0].type) <= 4);
+}
+
+bool star_batchable(const mtl_star_desc& d) {
+  StarPlan sp;
+  if (star_validate(&d, &sp) != MTL_OK) return false;
+  return d.d_rows && narrow_key_type(d.d_key_type) && narrow_key_type(d.p_key_type) &&
+         narrow_key_type(d.p_fk_type) && narrow_key_type(d.f_fk_type) &&
+         narrow_pred(d.d_preds, d.d_n_preds) && narrow_pred(d.p_preds, d.p_n_preds) &&
+         narrow_pred(d.f_preds, d.f_n_preds);
+}
+
+static bool same_pred_col(const mtl_pred* a, uint32_t na, const mtl_pred* b, uint32_t nb) {
+  return na == nb && (na == 0 || (a[0].column == b[0].column && a[0].type == b[0].type));
+}
+
+bool star_shareable(const mtl_star_desc& a, const mtl_star_desc& b) {
+  return a.d_key == b.d_key && a.d_key_type == b.d_key_type && a.d_key_min == b.d_key_min &&
+         a.d_key_max == b.d_key_max && a.p_key == b.p_key && a.p_key_type == b.p_key_type &&
+         a.p_key_min == b.p_key_min && a.p_fk == b.p_fk && a.p_fk_type == b.p_fk_type &&
+         a.f_fk == b.f_fk && a.f_fk_type == b.f_fk_type &&
+         same_pred_col(a.d_preds, a.d_n_preds, b.d_preds, b.d_n_preds) &&
+         same_pred_col(a.p_preds, a.p_n_preds, b.p_preds, b.p_n_preds) &&
+         same_pred_col(a.f_preds, a.f_n_preds, b.f_preds, b.f_n_preds) &&
+         std::memcmp(&a.agg, &b.agg, sizeof(mtl_expr)) == 0;
+}
+
+size_t star_batch_stage_bytes(uint32_t n) { return (size_t)n * sizeof(MQStarQ) + 256; }
+
+int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_stage,
+                           mtl_stream* s) {
+  if (n == 0) return MTL_OK;
+  if (n > kStarMQMax) return set_error(MTL_ERR_INVALID_ARGUMENT, "more than %u queries", kStarMQMax);
+  const mtl_star_desc& d0 = *items[0].d;
+  std::vector<StarPlan> sp(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    MTL_TRY(star_validate(items[i].d, &sp[i]));
+    if (!star_batchable(*items[i].d) || !star_shareable(d0, *items[i].d))
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "query %u cannot join the shared passes", i);
+  }
+  Arena& ar = s->arena;
+  hipStream_t st = s->stream;
+  // regions to clear, carved from one block (each a multiple of 64 words)
+  auto pad = [](uint64_t w) { return (w + 63) & ~uint64_t(63); };
+  std::vector<StarBufs> b(n);
+  uint64_t clear_words = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    b[i].rtiles = (uint32_t)ceil_div(sp[i].pwords, RTILE);
+    clear_words += pad(sp[i].dwords * 2) + pad((sp[i].pwords + 1) * 2) + 64 +
+                   pad((uint64_t)std::max<uint32_t>(b[i].rtiles, 1) * 2) + 64;
+  }
+  void* clr;
+  MTL_TRY(ar.alloc(clear_words * 4, &clr));
+  uint32_t* cw = (uint32_t*)clr;
+  MQStarQ* hq = (MQStarQ*)host_stage;
+  uint64_t d_max = 0, p_max = 0, f_max = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const mtl_star_desc& d = *items[i].d;
+    StarBufs& bi = b[i];
+    bi.dim_bm = cw;
+    cw += pad(sp[i].dwords * 2);
+    bi.par_bm = cw;
+    cw += pad((sp[i].pwords + 1) * 2);
+    bi.misc = cw;
+    cw += 64;
+    bi.st_r = cw;
+    cw += pad((uint64_t)std::max<uint32_t>(bi.rtiles, 1) * 2);
+    bi.tmax = cw;
+    cw += 64;
+    bi.n_tmax = 1;
+    const uint64_t prows = d.p_rows ? d.p_rows : 1;
+    MTL_TRY(ar.alloc((uint64_t)filter_tiles(d.d_rows) * kTileRows / 8, &bi.dbits));
+    MTL_TRY(ar.alloc(sp[i].pwords * 4, &bi.par_rank));
+    MTL_TRY(ar.alloc((uint64_t)filter_tiles(d.p_rows) * kTileRows / 8, &bi.pbits));
+    MTL_TRY(ar.alloc(prows * 4, &bi.row_of));
+    MTL_TRY(ar.alloc(prows * 8, &bi.acc));
+    MTL_TRY(ar.alloc(prows * 4, &bi.cnt));
+    MTL_TRY(alloc_topk_bufs(&d, sp[i], ar, bi));
+    MQStarQ& Q = hq[i];
+    Q = MQStarQ{};
+    Q.d_rows = d.d_rows;
+    Q.p_rows = d.p_rows;
+    Q.f_rows = d.f_rows;
+    if (d.d_n_preds) Q.dpred = compile_pred(d.d_preds[0]);
+    if (d.p_n_preds) Q.ppred = compile_pred(d.p_preds[0]);
+    if (d.f_n_preds) Q.fpred = compile_pred(d.f_preds[0]);
+    Q.pspan = sp[i].pspan;
+    Q.dim_bm = (uint64_t*)bi.dim_bm;
+    Q.dbits = (uint64_t*)bi.dbits;
+    Q.pbits = (uint64_t*)bi.pbits;
+    Q.par_bm = (uint64_t*)bi.par_bm;
+    Q.rank = (const uint32_t*)bi.par_rank;
+    Q.acc = (uint64_t*)bi.acc;
+    Q.cnt = (uint32_t*)bi.cnt;
+    Q.err = (uint32_t*)bi.misc + 3;
+    Q.tmax = (unsigned long long*)bi.tmax;
+    d_max = std::max(d_max, d.d_rows);
+    p_max = std::max(p_max, d.p_rows);
+    f_max = std::max(f_max, d.f_rows);
+  }
+  void* dq;
+  MTL_TRY(ar.alloc((size_t)n * sizeof(MQStarQ), &dq));
+  MTL_HIP(hipMemcpyAsync(dq, hq, (size_t)n * sizeof(MQStarQ), hipMemcpyHostToDevice, st));
+  ZeroList zl;
+  zl.add((uint32_t*)clr, clear_words);
+  MTL_TRY(launch_zero(zl, st));
+  auto vec = [](const void* p, uint32_t w) { return ((uintptr_t)p % (w == 1 ? 4 : 16)) == 0 ? 1u : 0u; };
+  auto kw = [](int32_t t) { return t == MTL_DICT8 ? 1u : 4u; };
+  MQStarArgs a{};
+  a.nq = n;
+  a.q = (const MQStarQ*)dq;
+  a.d_key = d0.d_key;
+  a.d_key_type = d0.d_key_type;
+  a.d_kvec = vec(d0.d_key, kw(d0.d_key_type));
+  a.d_pw = d0.d_n_preds ? (uint32_t)type_width(d0.d_preds[0].type) : 0u;
+  a.d_pcol = d0.d_n_preds ? d0.d_preds[0].column : nullptr;
+  a.d_pvec = a.d_pw ? vec(a.d_pcol, a.d_pw) : 0u;
+  a.d_min = d0.d_key_min;
+  a.dspan = sp[0].dspan;
+  a.p_key = d0.p_key;
+  a.p_key_type = d0.p_key_type;
+  a.p_kvec = vec(d0.p_key, kw(d0.p_key_type));
+  a.p_fk = d0.p_fk;
+  a.p_fk_type = d0.p_fk_type;
+  a.p_fvec = vec(d0.p_fk, kw(d0.p_fk_type));
+  a.p_pw = d0.p_n_preds ? (uint32_t)type_width(d0.p_preds[0].type) : 0u;
+  a.p_pcol = d0.p_n_preds ? d0.p_preds[0].column : nullptr;
+  a.p_pvec = a.p_pw ? vec(a.p_pcol, a.p_pw) : 0u;
+  a.p_min = d0.p_key_min;
+  a.f_fk = d0.f_fk;
+  a.f_fk_type = d0.f_fk_type;
+  a.f_fvec = vec(d0.f_fk, kw(d0.f_fk_type));
+  a.f_pw = d0.f_n_preds ? (uint32_t)type_width(d0.f_preds[0].type) : 0u;
+  a.f_pcol = d0.f_n_preds ? d0.f_preds[0].column : nullptr;
+  a.f_pvec = a.f_pw ? vec(a.f_pcol, a.f_pw) : 0u;
+  a.expr = d0.agg;
+  {
+    ProfScope ps(s, "k_mq_dim");
+    a.n_max = d_max;
+    hipLaunchKernelGGL(k_mq_dim, dim3(filter_tiles(d_max)), dim3(FBLOCK), 0, st, a);
+    MTL_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(s, "k_mq_parent");
+    a.n_max = p_max;
+    hipLaunchKernelGGL(k_mq_parent, dim3(filter_tiles(p_max)), dim3(FBLOCK), 0, st, a);
+    MTL_HIP(hipGetLastError());
+  }
+  for (uint32_t i = 0; i < n; ++i) MTL_TRY(launch_rank_map(items[i].d, sp[i], b[i], s));
+  if (f_max) {
+    ProfScope ps(s, "k_mq_fact");
+    a.n_max = f_max;
+    if (d0.agg.is_double)
+      hipLaunchKernelGGL(k_mq_fact<true>, dim3(filter_tiles(f_max)), dim3(FBLOCK), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_mq_fact<false>, dim3(filter_tiles(f_max)), dim3(FBLOCK), 0, st, a);
+    MTL_HIP(hipGetLastError());
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    MTL_TRY(launch_star_topk(items[i].d, sp[i], b[i], items[i].out, items[i].out_n, s));
+  return MTL_OK;
+}
+
+}  // namespace mtl

@@ -254,28 +254,67 @@ def star_query(desc: abi.StarDesc):
     return q, out
 
 
-class Server:
-    """mtl_server: n_slots concurrent streams with continuous admission."""
+def group_query(keys, aggs, n_rows: int, k: int = 0, agg: int = 0, desc: bool = True,
+                capacity: int = 0, sel: torch.Tensor | None = None):
+    """A grouped-aggregate query (keys, aggs as for group_agg): the top k
+    groups by aggregate `agg`, or every group (at most `capacity`) when k is
+    0.  Returns (query, (keys array, [per-aggregate arrays])); keep both
+    alive until the query completes."""
+    q = abi.Query()
+    q.kind = abi.Q_GROUP
+    q.group = abi.make_group_desc(n_rows, [(ptr(c), t, mn, mx) for c, t, mn, mx in keys],
+                                  [(a[0], ptr(a[1])) + tuple(a[2:]) for a in aggs],
+                                  sel=ptr(sel), sel_capacity=0 if sel is None else sel.numel())
+    cap = k if k else capacity
+    q.group_k = k
+    q.group_agg = agg
+    q.group_desc = 1 if desc else 0
+    q.group_capacity = cap
+    kout = np.zeros((max(cap, 1), len(keys)), dtype=np.int64)
+    outs = [np.zeros(max(cap, 1), dtype=abi.agg_result_dtype(a[0], a[2])) for a in aggs]
+    q.group_keys = kout.ctypes.data
+    for i, o in enumerate(outs):
+        q.group_out[i] = o.ctypes.data
+    return q, (kout, outs)
 
-    def __init__(self, n_slots: int, budget: int = 0):
+
+class Server:
+    """mtl_server: n_slots worker streams; each worker runs the queries that
+    have arrived in batches of up to max_batch (shared scans), and
+    n_slots = max_batch = 1 is serial dispatch."""
+
+    def __init__(self, n_slots: int, budget: int = 0, max_batch: int = 1):
         self.h = C.c_void_p()
-        abi.check(abi.lib().mtl_server_create(n_slots, budget, C.byref(self.h)))
+        abi.check(abi.lib().mtl_server_create(n_slots, max_batch, budget, C.byref(self.h)))
 
     def submit(self, q: abi.Query, not_before_ns: int = 0) -> int:
         t = C.c_uint64()
         abi.check(abi.lib().mtl_server_submit(self.h, C.byref(q), not_before_ns, C.byref(t)))
         return t.value
 
+    def submit_sql(self, sql: str, not_before_ns: int = 0) -> int:
+        t = C.c_uint64()
+        abi.check(abi.lib().mtl_server_submit_sql(self.h, sql.encode(), not_before_ns,
+                                                  C.byref(t)))
+        return t.value
+
     def wait(self, ticket: int) -> abi.QueryResult:
         """The result record; raises MettleError if the query failed."""
         r = abi.QueryResult()
-        abi.check(abi.lib().mtl_server_wait(self.h, ticket, C.byref(r)))
+        st = abi.lib().mtl_server_wait(self.h, ticket, C.byref(r))
         msg = r.error.decode(errors="replace")
         if r.status == abi.MTL_UNSUPPORTED:
             raise abi.Unsupported(r.status, msg)
         if r.status != abi.MTL_OK:
             raise abi.MettleError(r.status, msg)
+        abi.check(st)
         return r
+
+    def wait_sql(self, ticket: int):
+        """A SQL query's rows, as mettle_amd.sql.execute returns them."""
+        from mettle_amd import sql as S
+        r = self.wait(ticket)
+        return S.take_result(r.result)
 
     def close(self):
         if self.h:

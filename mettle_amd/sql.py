@@ -67,6 +67,14 @@ def execute(sql: str):
     L = abi.lib()
     h = C.c_void_p()
     abi.check(L.mtl_sql_execute(sql.encode(), C.byref(h), None))
+    return take_result(h)
+
+
+def take_result(h):
+    """Convert a mtl_result handle (then freed) to {column name: (type,
+    scale, values)}."""
+    L = abi.lib()
+    h = C.c_void_p(h.value if isinstance(h, C.c_void_p) else h)
     try:
         nc, nr = C.c_uint32(), C.c_uint64()
         abi.check(L.mtl_result_shape(h, C.byref(nc), C.byref(nr)))
