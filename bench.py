@@ -74,16 +74,21 @@ def copy_calibration(n_bytes: int = 1 << 30, reps: int = 5) -> int:
     return 0
 
 
-def serving_benchmark(n_queries: int, slots: int, seed: int = 20261208):
+def serving_benchmark(n_queries: int, slots: int, max_batch: int, seed: int = 20261208):
     """Serving stream: n_queries with Poisson arrivals mixing three shapes
-    uniformly -- the 1M-row plain query (value > 524288, value*10), a
-    BIGINT scan at 10% selectivity with a projection, and the star
-    join -> SUM -> top-10 -- each over a table of log-uniform size in
-    [1e4, 1e8] rows (prefixes of resident 1e8-row tables).  The same arrival
-    schedule, offered at 10x the measured serial capacity, runs once through
-    a 1-slot server (serial dispatch) and once through a `slots`-slot server
-    (continuous admission); every result is compared between the two."""
-    import ctypes as C
+    uniformly -- the 1M-row plain query's shape (value > literal, value*10),
+    a BIGINT scan at 5-15% selectivity with a projection, and the star
+    join -> SUM -> top-10 with its own segment and cutoff -- each over a
+    table of log-uniform size in [1e4, 1e8] rows (prefixes of resident
+    1e8-row tables), every query with its own random literals.  The same
+    arrival schedule, offered at 10x the measured serial capacity, runs once
+    through serial dispatch (1 slot, batches of 1: the single-query
+    operators) and once through continuous admission (`slots` workers,
+    batches of up to `max_batch`), and once more through shared-pass batches
+    (4 workers, batches of up to 32 whose scans and star joins read each
+    table tile once); every result is compared between the runs.  Latency is
+    also measured at an offered rate below serial capacity (0.8x), where
+    neither queue builds up."""
     import numpy as np
     import torch
     from mettle_amd import abi, datagen, ops
@@ -104,65 +109,97 @@ def serving_benchmark(n_queries: int, slots: int, seed: int = 20261208):
     n_c, n_l_all = len(cust["c_key"]), len(l_order)
     del cust, orders, lines
     torch.cuda.synchronize()
-    cut_b = int(np.iinfo(np.int64).min + (1 << 64) // 10)
     shapes = rng.integers(0, 3, n_queries)
     sizes = np.floor(10 ** rng.uniform(4, 8, n_queries)).astype(np.int64)
     queries, bufs = [], []
     for sh, n in zip(shapes, sizes):
         if sh == 0:
-            q = ops.scan_query([(dv, abi.INTEGER, abi.CMP_GT, 524288)], int(n),
+            cut = int(rng.integers(0, 1 << 20))
+            q = ops.scan_query([(dv, abi.INTEGER, abi.CMP_GT, cut)], int(n),
                                (dv, abi.INTEGER, abi.PROJ_MUL, 10))
             out = None
         elif sh == 1:
+            cut_b = int(np.iinfo(np.int64).min + int(rng.uniform(0.05, 0.15) * (1 << 64)))
             q = ops.scan_query([(db, abi.BIGINT, abi.CMP_LT, cut_b)], int(n),
                                (db, abi.BIGINT, abi.PROJ_ADD, 7))
             out = None
         else:
             nl = int(min(n, n_l_all))
             n_o = int(l_order[nl - 1])  # orders 1..n_o cover the first nl lines
-            d = datagen.star_desc(lambda t, c: dev[t][c].data_ptr(), n_c, n_o, nl)
+            cut = datagen.DAYS_1992_01_01 + int(rng.integers(datagen.SPAN_7Y // 4,
+                                                             3 * datagen.SPAN_7Y // 4))
+            d = datagen.star_desc(lambda t, c: dev[t][c].data_ptr(), n_c, n_o, nl,
+                                  segment=int(rng.integers(0, 5)), cutoff=cut)
             q, out = ops.star_query(d)
         queries.append(q)
         bufs.append(out)
 
-    def run(n_slots, arrivals):
-        with ops.Server(n_slots) as srv:
-            t0 = L.mtl_now_ns() + 50_000_000
-            tickets = [srv.submit(q, t0 + int(a)) for q, a in zip(queries, arrivals)]
-            res = [srv.wait(t) for t in tickets]
-        first = min(r.t_submit_ns for r in res)
-        last = max(r.t_done_ns for r in res)
-        lat = np.array([r.t_done_ns - r.t_submit_ns for r in res], dtype=np.float64) / 1e6
+    def outputs(res, qs, bs):
         outs = []
-        for r, q, bo in zip(res, queries, bufs):
+        for r, q, bo in zip(res, qs, bs):
             if q.kind == abi.Q_STAR:
                 outs.append(tuple((bo[i].p_row, bo[i].count, bo[i].sum_i64)
                                   for i in range(r.n_rows)))
             else:
                 outs.append((r.count, r.fingerprint))
-        return {"qps": len(res) / ((last - first) / 1e9),
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "span_s": (last - first) / 1e9}, outs
+        return outs
+
+    def run(n_slots, batch, arrivals, count=None):
+        qs = queries if count is None else queries[:count]
+        bs = bufs if count is None else bufs[:count]
+        with ops.Server(n_slots, max_batch=batch) as srv:
+            # warm-up: the largest queries, arriving together, grow the
+            # workers' scratch arenas and pinned buffers before timing
+            big = sorted(range(len(qs)), key=lambda i: -sizes[i])[:4 * n_slots * batch]
+            t0 = L.mtl_now_ns() + 10_000_000
+            for t in [srv.submit(qs[i], t0) for i in big]:
+                srv.wait(t)
+            t0 = L.mtl_now_ns() + 50_000_000
+            tickets = [srv.submit(q, t0 + int(a)) for q, a in zip(qs, arrivals)]
+            res = [srv.wait(t) for t in tickets]
+        first = min(r.t_submit_ns for r in res)
+        last = max(r.t_done_ns for r in res)
+        lat = np.array([r.t_done_ns - r.t_submit_ns for r in res], dtype=np.float64) / 1e6
+        stats = {"qps": len(res) / ((last - first) / 1e9),
+                 "p50_ms": float(np.percentile(lat, 50)),
+                 "p99_ms": float(np.percentile(lat, 99)), "span_s": (last - first) / 1e9,
+                 "mean_batch": float(np.mean([r.batch for r in res]))}
+        return stats, outputs(res, qs, bs)
 
     # serial capacity on a prefix, all arriving at once
     n_cal = min(500, n_queries)
-    q_all, b_all = queries, bufs
-    queries, bufs = q_all[:n_cal], b_all[:n_cal]
-    cal, _ = run(1, np.zeros(n_cal))
-    queries, bufs = q_all, b_all
+    cal, _ = run(1, 1, np.zeros(n_cal), count=n_cal)
     offered = 10 * cal["qps"]
     arrivals = np.cumsum(rng.exponential(1e9 / offered, n_queries))
     log(f"serving: serial capacity ~{cal['qps']:.0f} q/s; offering {offered:.0f} q/s")
-    serial, out_s = run(1, arrivals)
+    serial, out_s = run(1, 1, arrivals)
     log(f"serving: serial {serial}")
-    batched, out_b = run(slots, arrivals)
-    log(f"serving: batched ({slots} slots) {batched}")
-    same = out_s == out_b
+    batched, out_b = run(slots, max_batch, arrivals)
+    log(f"serving: batched ({slots} slots, batches <= {max_batch}) {batched}")
+    # the shared-pass batches (scans and star joins of a batch read each
+    # table tile once), on the same schedule
+    shared, out_sh = run(4, 32, arrivals)
+    log(f"serving: shared-pass batches (4 slots, batches <= 32) {shared}")
+    # latency below serial capacity: neither queue builds up
+    n_low = min(2000, n_queries)
+    low_rate = 0.8 * serial["qps"]
+    low_arr = np.cumsum(rng.exponential(1e9 / low_rate, n_low))
+    low_s, out_ls = run(1, 1, low_arr, count=n_low)
+    low_b, out_lb = run(slots, max_batch, low_arr, count=n_low)
+    log(f"serving at {low_rate:.0f} q/s: serial {low_s}; batched {low_b}")
+    same = out_s == out_b == out_sh and out_ls == out_lb == out_s[:n_low]
     return {"queries": n_queries, "offered_qps": offered, "slots": slots,
-            "mix": "uniform over {1M-row plain filter+project, BIGINT 10% scan+project, "
-                   "star join->SUM->top-10}; rows log-uniform in [1e4, 1e8]",
+            "max_batch": max_batch,
+            "mix": "uniform over {INTEGER value > literal + value*10, BIGINT 5-15% scan + 7, "
+                   "star join->SUM->top-10 (random segment, cutoff)}; rows log-uniform in "
+                   "[1e4, 1e8]; random literals per query",
             "serial": serial, "batched": batched,
-            "speedup": batched["qps"] / serial["qps"], "identical_results": bool(same)}
+            "speedup": batched["qps"] / serial["qps"],
+            "shared_pass_batches": dict(shared, slots=4, max_batch=32,
+                                        speedup=shared["qps"] / serial["qps"]),
+            "below_capacity": {"offered_qps": low_rate, "queries": n_low, "serial": low_s,
+                               "batched": low_b},
+            "identical_results": bool(same)}
 
 
 def timed(fn, steps: int, warmup: int):
@@ -318,6 +355,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=10000)
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-batch", type=int, default=1)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--workload", choices=["star", "scan", "group", "group-dense"],
                     default="star", help="star: the headline line; the others measure "
@@ -467,7 +505,8 @@ def main() -> int:
     L.mtl_stream_destroy(s)
     del dev
     if not args.no_serve:
-        result["serving"] = serving_benchmark(args.serve_queries, args.serve_slots)
+        result["serving"] = serving_benchmark(args.serve_queries, args.serve_slots,
+                                              args.serve_batch)
         result["latency_ms"]["queries_per_s_serial"] = result["serving"]["serial"]["qps"]
         result["latency_ms"]["queries_per_s_batched"] = result["serving"]["batched"]["qps"]
         result["latency_ms"]["serving_p50_ms"] = result["serving"]["batched"]["p50_ms"]

@@ -688,18 +688,18 @@ __device__ __forceinline__ uint32_t tile_offset(const uint32_t* __restrict__ cou
 // With FP the tile adds the fingerprint terms of its positions (the value
 // is the projection, or the row id without one) to its group's sum.
 template <int PK, bool FP>
-__global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ sel_bm,
-                                                   const uint32_t* __restrict__ counts,
-                                                   const uint32_t* __restrict__ group_sums,
-                                                   const uint32_t* __restrict__ in_sel,
-                                                   uint32_t* __restrict__ out,
-                                                   uint32_t* __restrict__ out_count,
-                                                   ExpandProj pj, uint64_t* fp_groups) {
+__device__ __forceinline__ void expand_tile(const uint64_t* __restrict__ sel_bm,
+                                            const uint32_t* __restrict__ counts,
+                                            const uint32_t* __restrict__ group_sums,
+                                            const uint32_t* __restrict__ in_sel,
+                                            uint32_t* __restrict__ out,
+                                            uint32_t* __restrict__ out_count,
+                                            const ExpandProj& pj, uint64_t* fp_groups,
+                                            uint32_t tile, bool last) {
   __shared__ uint32_t s_ids[FTILE];
   __shared__ uint32_t s_wave[FWAVES];
   __shared__ uint32_t s_red[FWAVES];
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint32_t tile = blockIdx.x;
   const uint64_t t0 = (uint64_t)tile * FTILE;
   const uint32_t m = (uint32_t)(sel_bm[(t0 + (uint64_t)tid * 16) / 64] >> ((tid * 16) % 64)) & 0xFFFFu;
   const uint32_t base = group_sums ? tile_offset(counts, group_sums, tile, s_red) : counts[tile];
@@ -767,7 +767,19 @@ __global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ 
                        (unsigned long long)t);
     }
   }
-  if (group_sums && tile == gridDim.x - 1 && tid == 0) *out_count = base + total;
+  if (group_sums && last && tid == 0) *out_count = base + total;
+}
+
+template <int PK, bool FP>
+__global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ sel_bm,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ group_sums,
+                                                   const uint32_t* __restrict__ in_sel,
+                                                   uint32_t* __restrict__ out,
+                                                   uint32_t* __restrict__ out_count,
+                                                   ExpandProj pj, uint64_t* fp_groups) {
+  expand_tile<PK, FP>(sel_bm, counts, group_sums, in_sel, out, out_count, pj, fp_groups,
+                      blockIdx.x, blockIdx.x == gridDim.x - 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1293,8 +1305,7 @@ struct MQScanQ {
   uint32_t* out_sel;
   void* out_proj;
   uint64_t* bm;
-  uint32_t* counts;
-  uint32_t* gsums;
+  uint32_t* counts;  // per tile; then, scanned, each tile's output offset
   uint64_t* fp;
   uint32_t* words;  // [0] count, [1] overflow, [2..3] fingerprint
 };
@@ -1302,147 +1313,172 @@ struct MQScanQ {
 struct MQScanArgs {
   const void* col;   // shared predicate column
   const void* pcol;  // shared projection column (nullptr: none)
-  uint64_t n_max;    // largest row count of the group
+  uint64_t n_max;    // rows of the largest query
   uint32_t nq;
-  uint32_t vec;      // col aligned for vector loads
   const MQScanQ* q;
 };
 
-template <typename T, uint32_t KIND>
-__global__ __launch_bounds__(FBLOCK) void k_mq_filter(MQScanArgs a) {
-  __shared__ uint32_t s_cnt[kMQMax * FWAVES];
-  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint32_t tile = blockIdx.x;
-  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
-  const uint64_t k0 = seg + (uint64_t)lane * 4;
-  T x[FJ][4];
-  if (a.vec && (uint64_t)(tile + 1) * FTILE <= a.n_max) {  // block-uniform
-#pragma unroll
-    for (int j = 0; j < FJ; ++j)
-      load4<T, true>((const T*)a.col, k0 + (uint64_t)j * 256, a.n_max, true, x[j]);
-  } else {
-#pragma unroll
-    for (int j = 0; j < FJ; ++j)
-      load4<T, false>((const T*)a.col, k0 + (uint64_t)j * 256, a.n_max, a.vec != 0, x[j]);
-  }
-  for (uint32_t q = 0; q < a.nq; ++q) {
-    const MQScanQ& Q = a.q[q];
-    const uint64_t n = Q.n;
-    uint32_t bits = 0;
-    if ((uint64_t)tile * FTILE < n) {  // block-uniform
-#pragma unroll
-      for (int j = 0; j < FJ; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (k0 + (uint64_t)j * 256 + v < n &&
-              pred_pass(enc_raw((uint64_t)x[j][v], KIND), Q.pred))
-            bits |= 1u << (4 * j + v);
-      store_pass_bits(Q.bm, seg, bits, lane);
-    }
-    const uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
-    if (lane == 0) s_cnt[q * FWAVES + wave] = c;
-  }
-  __syncthreads();
-  if ((uint32_t)tid < a.nq) {
-    const MQScanQ& Q = a.q[tid];
-    if ((uint64_t)tile * FTILE < Q.n) {
-      uint32_t t = 0;
-#pragma unroll
-      for (int w = 0; w < FWAVES; ++w) t += s_cnt[tid * FWAVES + w];
-      Q.counts[tile] = t;
-      if (t) atomicAdd(&Q.gsums[(tile >> kGroupShift) * kGroupStride], t);
-    }
+// Shared passes: one block per tile, the tile's column staged in LDS once;
+// each wave takes one query at a time, so the queries of a tile run in
+// parallel on the block's waves with no block barrier between them.
+// Within a wave, lane l covers rows 64 i + l of strip i (i = 0..63), so an
+// LDS read of a strip is conflict-free and a ballot over the strip is the
+// strip's 64-bit pass-bit word, stored by lane i.
+constexpr int MQ_BLOCK = 1024;
+constexpr int MQ_WAVES = MQ_BLOCK / WAVE;
+
+// The tile's values of a column of T into LDS (rows past n read as 0).
+template <typename T>
+__device__ __forceinline__ void mq_stage(T* s_val, const T* col, uint64_t t0, uint64_t n) {
+  for (uint32_t i = threadIdx.x; i < FTILE; i += MQ_BLOCK) {
+    const uint64_t r = t0 + i;
+    s_val[i] = r < n ? col[r] : T(0);
   }
 }
 
-template <int PK>
-__global__ __launch_bounds__(FBLOCK) void k_mq_expand(MQScanArgs a) {
-  using VT = typename std::conditional<PK == 2 || PK == 4, uint64_t, uint32_t>::type;
-  __shared__ uint32_t s_ids[FTILE];
-  __shared__ VT s_val[PK ? FTILE : 1];
-  __shared__ uint32_t s_wave[FWAVES];
-  __shared__ uint32_t s_red[FWAVES];
-  __shared__ uint64_t s_fp[FWAVES];
-  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+template <typename T, uint32_t KIND>
+__global__ __launch_bounds__(MQ_BLOCK) void k_mq_filter(MQScanArgs a) {
+  __shared__ T s_val[FTILE];
   const uint32_t tile = blockIdx.x;
   const uint64_t t0 = (uint64_t)tile * FTILE;
-  // stage the tile's projection values once when the queries' passing rows
-  // of this tile cover a good part of it (else each query gathers its own)
-  bool staged = false;
-  if constexpr (PK != 0) {
-    uint32_t passing = 0;
-    for (uint32_t q = 0; q < a.nq; ++q)
-      if (tile < a.q[q].tiles) passing += a.q[q].counts[tile];
-    staged = passing >= FTILE / 8;  // block-uniform
-    if (staged) {
-      const VT* col = (const VT*)a.pcol;
-#pragma unroll 4
-      for (uint32_t i = tid; i < FTILE; i += FBLOCK) {
-        const uint64_t r = t0 + i;
-        s_val[i] = r < a.n_max ? col[r] : VT(0);
+  mq_stage<T>(s_val, (const T*)a.col, t0, a.n_max);
+  __syncthreads();
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  for (uint32_t q = wave; q < a.nq; q += MQ_WAVES) {
+    const MQScanQ& Q = a.q[q];
+    if (tile >= Q.tiles) continue;  // wave-uniform
+    const uint64_t n = Q.n;
+    const PredC p = Q.pred;  // in registers for the strip loop
+    // rows of this lane below n: strips i < full are whole, strip `full`
+    // is partial
+    const uint64_t rows = n - t0 < (uint64_t)FTILE ? n - t0 : (uint64_t)FTILE;
+    uint64_t mine = 0;
+    uint32_t cnt = 0;
+#pragma unroll 1
+    for (int i0 = 0; i0 < 64; i0 += 16) {
+      // sixteen strips' values read before any is compared: no branch
+      // between the LDS reads, so they are all in flight together
+      T v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = s_val[(uint32_t)(i0 + u) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const uint32_t li = (uint32_t)(i0 + u) * 64 + lane;
+        bool ok;
+        if constexpr (KIND == PK_I32) ok = pred_pass_i32((uint32_t)v[u], p);
+        else ok = pred_pass(enc_raw((uint64_t)v[u], KIND), p);
+        const uint64_t w = __ballot((li < rows) & ok);
+        cnt += (uint32_t)__popcll(w);
+        mine = lane == i0 + u ? w : mine;
       }
     }
+    Q.bm[(uint64_t)tile * 64 + lane] = mine;
+    if (lane == 0) Q.counts[tile] = cnt;
   }
+}
+
+// Block q scans query q's tile counts in place into exclusive output
+// offsets and writes the total (the query's row count) to words[0].
+__global__ __launch_bounds__(1024) void k_mq_offsets(const MQScanQ* qs) {
+  __shared__ uint32_t s_wave[1024 / WAVE];
+  const MQScanQ& Q = qs[blockIdx.x];
+  const uint32_t T = Q.tiles;
+  const uint32_t per = (T + 1023) / 1024;
+  const uint32_t b0 = threadIdx.x * per, b1 = min(T, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t i = b0; i < b1; ++i) sum += Q.counts[i];
+  const uint32_t incl = wave_inclusive_u32(sum);
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  if (lane == WAVE - 1) s_wave[wave] = incl;
   __syncthreads();
-  for (uint32_t q = 0; q < a.nq; ++q) {
-    const MQScanQ& Q = a.q[q];
-    if (tile >= Q.tiles) continue;  // block-uniform
-    const uint32_t m =
-        (uint32_t)(Q.bm[(t0 + (uint64_t)tid * 16) / 64] >> ((tid * 16) % 64)) & 0xFFFFu;
-    const uint32_t base = tile_offset(Q.counts, Q.gsums, tile, s_red);
-    const uint32_t c = (uint32_t)__popc(m);
-    const uint32_t incl = wave_inclusive_u32(c);
-    if (lane == WAVE - 1) s_wave[wave] = incl;
+  uint32_t run = incl - sum, total = 0;
+  for (int w = 0; w < 1024 / WAVE; ++w) {
+    if (w < wave) run += s_wave[w];
+    total += s_wave[w];
+  }
+  for (uint32_t i = b0; i < b1; ++i) {
+    const uint32_t c = Q.counts[i];
+    Q.counts[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == 0) Q.words[0] = total;
+}
+
+template <int PK>
+__global__ __launch_bounds__(MQ_BLOCK) void k_mq_expand(MQScanArgs a) {
+  using VT = typename std::conditional<PK == 2 || PK == 4, uint64_t, uint32_t>::type;
+  __shared__ VT s_val[PK ? FTILE : 1];
+  const uint32_t tile = blockIdx.x;
+  const uint64_t t0 = (uint64_t)tile * FTILE;
+  if constexpr (PK != 0) {
+    mq_stage<VT>(s_val, (const VT*)a.pcol, t0, a.n_max);
     __syncthreads();
-    uint32_t pos = incl - c, total = 0;
+  }
+  const int wave = threadIdx.x / WAVE, lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  // the pass words and output offsets of this wave's queries (at most
+  // kMQMax / MQ_WAVES), loaded together before any is used
+  constexpr int QW = (int)((kMQMax + MQ_WAVES - 1) / MQ_WAVES);
+  uint64_t words_of[QW];
+  uint32_t off_of[QW];
 #pragma unroll
-    for (int w = 0; w < FWAVES; ++w) {
-      if (w < wave) pos += s_wave[w];
-      total += s_wave[w];
+  for (int j = 0; j < QW; ++j) {
+    const uint32_t q = wave + j * MQ_WAVES;
+    words_of[j] = 0;
+    off_of[j] = 0;
+    if (q < a.nq && tile < a.q[q].tiles) {
+      words_of[j] = a.q[q].bm[(uint64_t)tile * 64 + lane];
+      off_of[j] = a.q[q].counts[tile];
     }
-    uint32_t mm = m;
-    while (mm) {
-      const int i = __ffs(mm) - 1;
-      mm &= mm - 1;
-      s_ids[pos++] = (uint32_t)tid * 16 + (uint32_t)i;
-    }
-    __syncthreads();
-    ExpandProj pj{a.pcol, Q.op, Q.scalar, Q.out_proj, nullptr};
+  }
+#pragma unroll
+  for (int j = 0; j < QW; ++j) {
+    const uint32_t q = wave + j * MQ_WAVES;
+    if (q >= a.nq) break;
+    const MQScanQ& Q = a.q[q];
+    if (tile >= Q.tiles) continue;  // wave-uniform
+    const uint64_t mine = words_of[j];
+    const uint32_t off = off_of[j];
+    const uint32_t c = (uint32_t)__popcll(mine);
+    const uint32_t incl = wave_inclusive_u32(c);
+    const uint32_t excl = incl - c;
+    const ExpandProj pj{a.pcol, Q.op, Q.scalar, Q.out_proj, nullptr};
+    uint32_t* const out_sel = Q.out_sel;
     bool ovf = false;
     uint64_t h = 0;
-    for (uint32_t i = tid; i < total; i += FBLOCK) {
-      const uint32_t li = s_ids[i];
-      const uint32_t r = (uint32_t)(t0 + li);
+    const uint32_t mlo = (uint32_t)mine, mhi = (uint32_t)(mine >> 32);
+    for (int i = 0; i < 64; ++i) {
+      // strip i's word and output base from lane i (scalar reads)
+      const uint64_t w = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, i) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int)mlo, i);
+      if (!w) continue;  // uniform
+      const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)excl, i);
+      const uint32_t li = (uint32_t)i * 64 + lane;
       uint64_t x = 0;
-      if constexpr (PK != 0) x = staged ? (uint64_t)s_val[li] : proj_load<PK>(pj, r);
-      const uint64_t o = (uint64_t)base + i;
-      Q.out_sel[o] = r;
-      const uint64_t vb = proj_store<PK>(pj, o, x, ovf);
-      h += fp_term(o, r, PK ? vb : r);
+      if constexpr (PK != 0) x = (uint64_t)s_val[li];  // before the branch: no wait inside it
+      if ((w >> lane) & 1ull) {
+        const uint32_t r = (uint32_t)(t0 + li);
+        const uint64_t o = (uint64_t)off + base + (uint32_t)__popcll(w & lt);
+        out_sel[o] = r;
+        const uint64_t vb = proj_store<PK>(pj, o, x, ovf);
+        h += fp_term(o, r, PK ? vb : r);
+      }
     }
     if constexpr (PK == 1 || PK == 2) {
       if (__ballot(ovf) && lane == 0) atomicOr(&Q.words[1], 1u);
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) h += __shfl_xor(h, off, WAVE);
-    if (lane == 0) s_fp[wave] = h;
-    __syncthreads();
-    if (tid == 0) {
-      uint64_t t = 0;
-#pragma unroll
-      for (int w = 0; w < FWAVES; ++w) t += s_fp[w];
-      if (t) atomicAdd((unsigned long long*)&Q.fp[(tile >> kGroupShift) * kFpStride],
-                       (unsigned long long)t);
-      if (tile == Q.tiles - 1) Q.words[0] = base + total;
-    }
-    __syncthreads();  // s_ids, s_wave, s_red and s_fp are reused by the next query
+    for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, WAVE);
+    if (lane == 0 && h)
+      atomicAdd((unsigned long long*)&Q.fp[(tile >> kGroupShift) * kFpStride],
+                (unsigned long long)h);
   }
 }
 
 // Block q adds up query q's per-group fingerprint sums.
-__global__ __launch_bounds__(1024) void k_mq_fp_sum(MQScanArgs a) {
+__global__ __launch_bounds__(1024) void k_mq_fp_sum(const MQScanQ* qs) {
   __shared__ uint64_t s_part[1024 / WAVE];
-  const MQScanQ& Q = a.q[blockIdx.x];
+  const MQScanQ& Q = qs[blockIdx.x];
   const uint32_t groups = (Q.tiles + (1u << kGroupShift) - 1) >> kGroupShift;
   uint64_t t = 0;
   for (uint32_t i = threadIdx.x; i < groups; i += blockDim.x) t += Q.fp[(uint64_t)i * kFpStride];
@@ -1456,6 +1492,20 @@ __global__ __launch_bounds__(1024) void k_mq_fp_sum(MQScanArgs a) {
     Q.words[2] = (uint32_t)s;
     Q.words[3] = (uint32_t)(s >> 32);
   }
+}
+
+uint32_t mq_map(const uint32_t* tiles, uint32_t n, MQMap* m) {
+  *m = MQMap{};
+  m->nq = n;
+  for (uint32_t i = 0; i < n; ++i) m->order[i] = i;
+  std::stable_sort(m->order, m->order + n,
+                   [&](uint32_t x, uint32_t y) { return tiles[x] > tiles[y]; });
+  uint64_t blocks = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    m->chunks[i] = (uint32_t)ceil_div(tiles[m->order[i]], kMQChunk);
+    blocks += (uint64_t)m->chunks[i] * kMQChunk;
+  }
+  return (uint32_t)blocks;
 }
 
 bool scan_batchable(const mtl_query& q) {
@@ -1491,13 +1541,12 @@ int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* wor
   Arena& ar = s->arena;
   // per-query scratch; the regions to clear (group sums, fingerprint sums)
   // are carved from one block so one clearing launch covers them
-  uint64_t n_max = 0, clear_words = 0;
+  uint64_t clear_words = 0;
   std::vector<uint32_t> tiles(n), groups(n);
   for (uint32_t i = 0; i < n; ++i) {
-    n_max = std::max<uint64_t>(n_max, items[i].q->n_rows);
     tiles[i] = filter_tiles(items[i].q->n_rows);
     groups[i] = (uint32_t)ceil_div(tiles[i], 1u << kGroupShift);
-    clear_words += (uint64_t)groups[i] * (kGroupStride + 2 * kFpStride);
+    clear_words += (uint64_t)groups[i] * 2 * kFpStride;
   }
   void* clr;
   MTL_TRY(ar.alloc(clear_words * 4, &clr));
@@ -1521,8 +1570,6 @@ int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* wor
     Q.counts = (uint32_t*)cnt;
     Q.fp = (uint64_t*)cw;  // 8-byte aligned: every carve is a multiple of 64 words
     cw += (uint64_t)groups[i] * 2 * kFpStride;
-    Q.gsums = cw;
-    cw += (uint64_t)groups[i] * kGroupStride;
     Q.words = words + 4 * i;
   }
   void* dq;
@@ -1536,34 +1583,39 @@ int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* wor
   MQScanArgs a{};
   a.col = q0.preds[0].column;
   a.pcol = q0.proj_col;
-  a.n_max = n_max;
   a.nq = n;
-  a.vec = ((uintptr_t)a.col % (type_width(q0.preds[0].type) == 1 ? 4 : 16)) == 0;
   a.q = (const MQScanQ*)dq;
-  const dim3 grid(filter_tiles(n_max));
+  uint32_t tmax = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    tmax = std::max(tmax, tiles[i]);
+    a.n_max = std::max<uint64_t>(a.n_max, items[i].q->n_rows);
+  }
+  const dim3 grid(tmax);
   {
     ProfScope ps(s, "k_mq_filter");
     switch (hq[0].pred.kind) {
-      case PK_I32: hipLaunchKernelGGL((k_mq_filter<uint32_t, PK_I32>), grid, dim3(FBLOCK), 0, st, a); break;
-      case PK_F32: hipLaunchKernelGGL((k_mq_filter<uint32_t, PK_F32>), grid, dim3(FBLOCK), 0, st, a); break;
-      case PK_U8: hipLaunchKernelGGL((k_mq_filter<uint8_t, PK_U8>), grid, dim3(FBLOCK), 0, st, a); break;
-      case PK_I64: hipLaunchKernelGGL((k_mq_filter<uint64_t, PK_I64>), grid, dim3(FBLOCK), 0, st, a); break;
-      default: hipLaunchKernelGGL((k_mq_filter<uint64_t, PK_F64>), grid, dim3(FBLOCK), 0, st, a); break;
+      case PK_I32: hipLaunchKernelGGL((k_mq_filter<uint32_t, PK_I32>), grid, dim3(MQ_BLOCK), 0, st, a); break;
+      case PK_F32: hipLaunchKernelGGL((k_mq_filter<uint32_t, PK_F32>), grid, dim3(MQ_BLOCK), 0, st, a); break;
+      case PK_U8: hipLaunchKernelGGL((k_mq_filter<uint8_t, PK_U8>), grid, dim3(MQ_BLOCK), 0, st, a); break;
+      case PK_I64: hipLaunchKernelGGL((k_mq_filter<uint64_t, PK_I64>), grid, dim3(MQ_BLOCK), 0, st, a); break;
+      default: hipLaunchKernelGGL((k_mq_filter<uint64_t, PK_F64>), grid, dim3(MQ_BLOCK), 0, st, a); break;
     }
     MTL_HIP(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_mq_offsets, dim3(n), dim3(1024), 0, st, a.q);
+  MTL_HIP(hipGetLastError());
   {
     ProfScope ps(s, "k_mq_expand");
     switch (pk) {
-      case 1: hipLaunchKernelGGL(k_mq_expand<1>, grid, dim3(FBLOCK), 0, st, a); break;
-      case 2: hipLaunchKernelGGL(k_mq_expand<2>, grid, dim3(FBLOCK), 0, st, a); break;
-      case 3: hipLaunchKernelGGL(k_mq_expand<3>, grid, dim3(FBLOCK), 0, st, a); break;
-      case 4: hipLaunchKernelGGL(k_mq_expand<4>, grid, dim3(FBLOCK), 0, st, a); break;
-      default: hipLaunchKernelGGL(k_mq_expand<0>, grid, dim3(FBLOCK), 0, st, a); break;
+      case 1: hipLaunchKernelGGL(k_mq_expand<1>, grid, dim3(MQ_BLOCK), 0, st, a); break;
+      case 2: hipLaunchKernelGGL(k_mq_expand<2>, grid, dim3(MQ_BLOCK), 0, st, a); break;
+      case 3: hipLaunchKernelGGL(k_mq_expand<3>, grid, dim3(MQ_BLOCK), 0, st, a); break;
+      case 4: hipLaunchKernelGGL(k_mq_expand<4>, grid, dim3(MQ_BLOCK), 0, st, a); break;
+      default: hipLaunchKernelGGL(k_mq_expand<0>, grid, dim3(MQ_BLOCK), 0, st, a); break;
     }
     MTL_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_mq_fp_sum, dim3(n), dim3(1024), 0, st, a);
+  hipLaunchKernelGGL(k_mq_fp_sum, dim3(n), dim3(1024), 0, st, a.q);
   MTL_HIP(hipGetLastError());
   return MTL_OK;
 }

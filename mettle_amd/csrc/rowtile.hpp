@@ -44,28 +44,30 @@ __device__ __forceinline__ uint32_t ld_stream1(const void* p) {
 }
 
 // FULL: the caller knows the four rows are in range and the column is
-// aligned, so the load is one straight-line vector load with the streaming
-// hint.  (A branch around each load makes the compiler wait for it before
+// aligned, so the load is one straight-line vector load, with the streaming
+// hint unless NT is false (a tile several blocks read, which should stay in
+// L2).  (A branch around each load makes the compiler wait for it before
 // the next is issued.)
-template <typename T, bool FULL = false>
+template <typename T, bool FULL = false, bool NT = true>
 __device__ __forceinline__ void load4(const T* __restrict__ col, uint64_t k,
                                       uint64_t n, bool vec_ok, T x[4]) {
+  constexpr bool S = FULL && NT;
   if (FULL || (vec_ok && k + 3 < n)) {
     if constexpr (sizeof(T) == 4) {
-      uint4 u = FULL ? ld_stream(col + k) : *(const uint4*)(col + k);
+      uint4 u = S ? ld_stream(col + k) : *(const uint4*)(col + k);
       x[0] = __builtin_bit_cast(T, u.x);
       x[1] = __builtin_bit_cast(T, u.y);
       x[2] = __builtin_bit_cast(T, u.z);
       x[3] = __builtin_bit_cast(T, u.w);
     } else if constexpr (sizeof(T) == 8) {
-      ulonglong2 a = FULL ? ld_stream2(col + k) : *(const ulonglong2*)(col + k);
-      ulonglong2 b = FULL ? ld_stream2(col + k + 2) : *(const ulonglong2*)(col + k + 2);
+      ulonglong2 a = S ? ld_stream2(col + k) : *(const ulonglong2*)(col + k);
+      ulonglong2 b = S ? ld_stream2(col + k + 2) : *(const ulonglong2*)(col + k + 2);
       x[0] = __builtin_bit_cast(T, a.x);
       x[1] = __builtin_bit_cast(T, a.y);
       x[2] = __builtin_bit_cast(T, b.x);
       x[3] = __builtin_bit_cast(T, b.y);
     } else {
-      uint32_t u = FULL ? ld_stream1(col + k) : *(const uint32_t*)(col + k);
+      uint32_t u = S ? ld_stream1(col + k) : *(const uint32_t*)(col + k);
       x[0] = (T)(u & 0xff);
       x[1] = (T)((u >> 8) & 0xff);
       x[2] = (T)((u >> 16) & 0xff);
@@ -250,6 +252,46 @@ __device__ __forceinline__ void store_pass_bits(uint64_t* sel_bm, uint64_t seg, 
     lo |= row_shr<8>(lo); hi |= row_shr<8>(hi);
     if (r == 15) sel_bm[(seg + (uint64_t)j * 256) / 64 + (lane >> 4)] = ((uint64_t)hi << 32) | lo;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Block -> (query, tile) of a pass shared by a batch of queries over one
+// table.  Each block does one tile of one query, like the single-query
+// kernels.  The queries are ordered by tile count, descending (order[i]
+// is the query index of rank i); tiles are taken in chunks of kMQChunk,
+// and within a chunk the blocks of every query that covers it follow one
+// another query by query.  The blocks that read one tile for different
+// queries are thus dispatched close together and, kMQChunk being a
+// multiple of the 8 XCDs that blocks are dealt to in turn, on the same
+// XCD: the tile comes from HBM once and from that XCD's L2 after.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMQQueries = 32;  // queries per shared pass
+constexpr uint32_t kMQChunk = 64;    // tiles per chunk (a multiple of 8)
+struct MQMap {
+  uint32_t nq;
+  uint32_t chunks[kMQQueries];  // chunks of each ranked query (descending)
+  uint32_t order[kMQQueries];   // query index of each rank
+};
+
+// Blocks before chunk c: kMQChunk * sum over queries of min(c, chunks).
+__device__ __forceinline__ uint64_t mq_before(const MQMap& m, uint32_t c) {
+  uint64_t s = 0;
+  for (uint32_t i = 0; i < m.nq; ++i) s += c < m.chunks[i] ? c : m.chunks[i];
+  return s * kMQChunk;
+}
+
+// The query index and tile of block b (uniform, scalar work).
+__device__ __forceinline__ void mq_block(const MQMap& m, uint32_t b, uint32_t& q,
+                                         uint32_t& tile) {
+  uint32_t lo = 0, hi = m.chunks[0];
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (mq_before(m, mid) <= b) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t r = (uint32_t)(b - mq_before(m, lo));
+  q = m.order[r / kMQChunk];
+  tile = lo * kMQChunk + r % kMQChunk;
 }
 
 }  // namespace dev

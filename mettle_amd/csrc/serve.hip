@@ -262,10 +262,12 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
                       ? 16
                       : (16 + (uint64_t)qs[i].star.k * sizeof(mtl_star_row) + 15) & ~uint64_t(15);
       }
-    for (size_t g = 0; g < groups.size(); ++g)
-      stage_bytes += (kind[g] == MTL_Q_SCAN ? scan_batch_stage_bytes(groups[g].size())
-                                            : star_batch_stage_bytes(groups[g].size()) + 255) &
-                     ~size_t(255);
+    auto stage_of = [&](size_t g) -> size_t {
+      const size_t b = kind[g] == MTL_Q_SCAN ? scan_batch_stage_bytes(groups[g].size())
+                                             : star_batch_stage_bytes(groups[g].size());
+      return (b + 255) & ~size_t(255);
+    };
+    for (size_t g = 0; g < groups.size(); ++g) stage_bytes += stage_of(g);
     {
       std::lock_guard<std::mutex> lk(sl.s->mu);
       Arena& ar = sl.s->arena;
@@ -296,7 +298,7 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
           if (st == MTL_OK)
             st = scan_batch_enqueue(items.data(), (uint32_t)items.size(),
                                     (uint32_t*)((char*)dres + roff[mem[0]]), stage, sl.s);
-          stage += (scan_batch_stage_bytes(mem.size()) + 255) & ~size_t(255);
+          stage += stage_of(g);
         } else {
           std::vector<StarBatchItem> items(mem.size());
           for (size_t j = 0; j < mem.size(); ++j) {
@@ -306,7 +308,7 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
             items[j].out = (mtl_star_row*)(at + 16);
           }
           st = star_batch_enqueue(items.data(), (uint32_t)items.size(), stage, sl.s);
-          stage += (star_batch_stage_bytes(mem.size()) + 255) & ~size_t(255);
+          stage += stage_of(g);
         }
       }
       if (st == MTL_OK)

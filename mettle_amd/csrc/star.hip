@@ -64,10 +64,11 @@ struct PopArgs {
 };
 
 // This thread's part of a grid-stride popcount, eight loads in flight.
-__device__ __forceinline__ uint32_t thread_popcount(const uint64_t* p, uint64_t words) {
+__device__ __forceinline__ uint32_t thread_popcount(const uint64_t* p, uint64_t words,
+                                                    uint32_t bid, uint32_t nb) {
   uint32_t c = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * RBLOCK;
-  for (uint64_t i0 = (uint64_t)blockIdx.x * RBLOCK + threadIdx.x; i0 < words; i0 += stride * 8) {
+  const uint64_t stride = (uint64_t)nb * RBLOCK;
+  for (uint64_t i0 = (uint64_t)bid * RBLOCK + threadIdx.x; i0 < words; i0 += stride * 8) {
     uint64_t x[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -94,12 +95,11 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t c, uint32_t* s_red) {
 // grid covers the larger of the parent bitmap and the dimension bitmaps, so
 // a small parent table with a large dimension table is not left to a few
 // blocks.
-__global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm,
-                                                 uint64_t words, uint32_t rtiles,
-                                                 uint32_t* __restrict__ rank,
-                                                 uint64_t* status, uint32_t* ctr,
-                                                 uint32_t* err, uint32_t* par_total,
-                                                 PopArgs pa) {
+__device__ __forceinline__ void rank_body(const uint64_t* __restrict__ bm, uint64_t words,
+                                          uint32_t rtiles, uint32_t* __restrict__ rank,
+                                          uint64_t* status, uint32_t* ctr, uint32_t* err,
+                                          uint32_t* par_total, const PopArgs& pa, uint32_t bid,
+                                          uint32_t nb) {
   __shared__ uint32_t s_tile, s_prefix;
   __shared__ uint32_t s_wave[RBLOCK / WAVE];
   __shared__ uint32_t s_red[RBLOCK / WAVE];
@@ -114,9 +114,9 @@ __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm
   for (int i = 0; i < RPER; ++i) x[i] = (ranks && w0 + i < words) ? bm[w0 + i] : 0ull;
   // the duplicate-key popcounts are independent of the prefix: their loads
   // go out with the bitmap words and finish while predecessors publish
-  const uint32_t p0 = thread_popcount(pa.pbits, pa.pbits_words);
-  const uint32_t p1 = pa.dim_bm ? thread_popcount(pa.dim_bm, pa.dim_words) : 0u;
-  const uint32_t p2 = pa.dbits ? thread_popcount(pa.dbits, pa.dbits_words) : 0u;
+  const uint32_t p0 = thread_popcount(pa.pbits, pa.pbits_words, bid, nb);
+  const uint32_t p1 = pa.dim_bm ? thread_popcount(pa.dim_bm, pa.dim_words, bid, nb) : 0u;
+  const uint32_t p2 = pa.dbits ? thread_popcount(pa.dbits, pa.dbits_words, bid, nb) : 0u;
   if (ranks) {
     uint32_t c[RPER];
     uint32_t sum = 0;
@@ -157,6 +157,15 @@ __global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm
   }
 }
 
+__global__ __launch_bounds__(RBLOCK) void k_rank(const uint64_t* __restrict__ bm,
+                                                 uint64_t words, uint32_t rtiles,
+                                                 uint32_t* __restrict__ rank,
+                                                 uint64_t* status, uint32_t* ctr,
+                                                 uint32_t* err, uint32_t* par_total,
+                                                 PopArgs pa) {
+  rank_body(bm, words, rtiles, rank, status, ctr, err, par_total, pa, blockIdx.x, gridDim.x);
+}
+
 __device__ __forceinline__ uint32_t rank_of(const uint64_t* bm, const uint32_t* rank,
                                             uint64_t s) {
   uint64_t w = bm[s >> 6];
@@ -180,6 +189,7 @@ struct MapArgs {
   const uint32_t* pops;       // [0] P pass bits, [1] D bitmap, [2] D pass bits
   uint32_t has_dim;
   uint32_t* err;
+  uint32_t grid;  // blocks of this map (shared multi-query launches)
 };
 
 // Every qualifying P row (a set pass bit) writes row_of[rank(key)].  Slots
@@ -187,14 +197,14 @@ struct MapArgs {
 // at least one qualifying row, so every slot gets its row; the accumulators
 // are cleared by index.  A repeated key (par_total < count) is flagged.
 // Thread t owns 16 consecutive rows (one 16-bit piece of the pass bits).
-__global__ void k_map(MapArgs a) {
+__device__ __forceinline__ void map_body(const MapArgs& a, uint32_t bid, uint32_t nb) {
   const uint32_t M = *a.par_total;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     if (M != a.pops[0]) atomicOr(a.err, 2u);                     // repeated parent key
     if (a.has_dim && a.pops[1] != a.pops[2]) atomicOr(a.err, 4u);  // repeated dim key
   }
-  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nthreads = (uint64_t)nb * blockDim.x;
+  const uint64_t t0 = (uint64_t)bid * blockDim.x + threadIdx.x;
   for (uint64_t i = t0; i < M; i += nthreads) {
     a.acc[i] = 0;
     a.cnt[i] = 0;
@@ -216,20 +226,22 @@ __global__ void k_map(MapArgs a) {
   }
 }
 
+__global__ void k_map(MapArgs a) { map_body(a, blockIdx.x, gridDim.x); }
+
 // 4-byte keys aligned for vector loads: a thread's 16 keys are one 64-byte
 // line, loaded whole (4 vector loads) only when one of its rows qualifies;
 // the slot lookups of its qualifying rows then go out four at a time (rows
 // picked from the key registers with a select chain, unused lookups read
 // word 0), so a thread waits about 1 + ceil(q / 4) round trips for q
 // qualifying rows instead of 2 q.
-__global__ __launch_bounds__(256) void k_map4(MapArgs a) {
+__device__ __forceinline__ void map4_body(const MapArgs& a, uint32_t bid, uint32_t nb) {
   const uint32_t M = *a.par_total;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     if (M != a.pops[0]) atomicOr(a.err, 2u);                     // repeated parent key
     if (a.has_dim && a.pops[1] != a.pops[2]) atomicOr(a.err, 4u);  // repeated dim key
   }
-  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nthreads = (uint64_t)nb * blockDim.x;
+  const uint64_t t0 = (uint64_t)bid * blockDim.x + threadIdx.x;
   for (uint64_t i = t0; i < M; i += nthreads) {
     a.acc[i] = 0;
     a.cnt[i] = 0;
@@ -280,6 +292,17 @@ __global__ __launch_bounds__(256) void k_map4(MapArgs a) {
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_map4(MapArgs a) { map4_body(a, blockIdx.x, gridDim.x); }
+
+// The maps of a batch of queries: blockIdx.y is the query.
+template <bool K4>
+__global__ __launch_bounds__(256) void k_mq_map(const MapArgs* qs) {
+  const MapArgs& a = qs[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  if constexpr (K4) map4_body(a, blockIdx.x, a.grid);
+  else map_body(a, blockIdx.x, a.grid);
 }
 
 // ---------------------------------------------------------------------------
@@ -681,6 +704,7 @@ struct TopkArgs {
   uint32_t* bcmax;                 // per top-K block: max count of its slots
   uint32_t n_bcmax;
   uint64_t p_rows;  // row_of entries are checked against it before use
+  uint32_t grid;    // phase-1 blocks (shared multi-query launches)
 };
 
 // Tie-break words of slot r whose parent row is `row` (read with the slot).
@@ -701,15 +725,15 @@ __device__ __forceinline__ void star_keys(const TopkArgs& a, uint32_t row, uint6
 }
 
 template <int R>
-__global__ __launch_bounds__(TopkShape<R>::NT) void k_star_topk(TopkArgs a) {
+__device__ __forceinline__ void star_topk_body(const TopkArgs& a, uint32_t bid, uint32_t nb) {
   using S = TopkShape<R>;
   __shared__ TopkBlockLds<S::W, S::CAP> sl;
   topk_block_init(sl);
   WaveTopk<R> wl;
   wl.init(a.k);
   const uint32_t m = *a.m_count;
-  const uint32_t chunk = (m + gridDim.x - 1) / gridDim.x;
-  const uint32_t start = blockIdx.x * chunk;
+  const uint32_t chunk = (m + nb - 1) / nb;
+  const uint32_t start = bid * chunk;
   const uint32_t end = min(m, start + chunk);
   constexpr int E = 8;  // entries per thread per pass, loaded together
   uint32_t cmax = 0;
@@ -750,16 +774,29 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_star_topk(TopkArgs a) {
     }
     topk_share_thr(wl, sl);
   }
-  topk_block_finish<S::NT>(wl, sl, a.stage, a.lists0);
+  topk_block_finish<S::NT>(wl, sl, a.stage, a.lists0, bid);
   cmax = block_max_u32(cmax);
-  if (threadIdx.x == 0) a.bcmax[blockIdx.x] = cmax;
+  if (threadIdx.x == 0) a.bcmax[bid] = cmax;
+}
+
+template <int R>
+__global__ __launch_bounds__(TopkShape<R>::NT) void k_star_topk(TopkArgs a) {
+  star_topk_body<R>(a, blockIdx.x, gridDim.x);
+}
+
+// Phase 1 of a batch of queries' top-K: blockIdx.y is the query.
+template <int R>
+__global__ __launch_bounds__(TopkShape<R>::NT) void k_mq_star_topk(const TopkArgs* qs) {
+  const TopkArgs& a = qs[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  star_topk_body<R>(a, blockIdx.x, a.grid);
 }
 
 // Merge level: block b merges lists [b*F, b*F+F) of `in`; the single block
 // of the final level also checks the overflow bound and writes the rows.
-__global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_star_topk_merge(TopkArgs a, TopkLists in,
-                                                                        TopkLists out, uint32_t L,
-                                                                        uint32_t final_level) {
+__device__ __forceinline__ void star_merge_body(const TopkArgs& a, const TopkLists& in,
+                                                const TopkLists& out, uint32_t L,
+                                                uint32_t final_level, uint32_t bid) {
   __shared__ TopkMergeLds s;
   // the overflow bound's inputs are independent of the merge: on the final
   // level their loads go out first and arrive while the lists merge
@@ -780,9 +817,9 @@ __global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_star_topk_merge(TopkArgs
     for (uint32_t i = threadIdx.x; i < a.n_bcmax; i += TOPK_MERGE_THREADS)
       cm = a.bcmax[i] > cm ? a.bcmax[i] : cm;
   }
-  const int cur = topk_merge_lists(s, in, L, a.k, a.stage);
+  const int cur = topk_merge_lists(s, in, L, a.k, a.stage, bid);
   if (!final_level) {
-    topk_write_list(s, cur, out, a.k);
+    topk_write_list(s, cur, out, a.k, bid);
     return;
   }
   // |SUM| <= count x max|value| for every group: flag the run when that
@@ -812,6 +849,26 @@ __global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_star_topk_merge(TopkArgs
     a.out_n[0] = nn;
     a.out_n[1] = __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+__global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_star_topk_merge(TopkArgs a, TopkLists in,
+                                                                        TopkLists out, uint32_t L,
+                                                                        uint32_t final_level) {
+  star_merge_body(a, in, out, L, final_level, blockIdx.x);
+}
+
+// One merge level of a batch of queries: blockIdx.y selects the query's
+// level (its top-K arguments, input and output lists, list count).
+struct MQMergeQ {
+  const TopkArgs* ta;
+  TopkLists in, out;
+  uint32_t L, final_level, blocks;
+};
+
+__global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_mq_star_merge(const MQMergeQ* qs) {
+  const MQMergeQ& m = qs[blockIdx.y];
+  if (blockIdx.x >= m.blocks) return;
+  star_merge_body(*m.ta, m.in, m.out, m.L, m.final_level, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -924,38 +981,68 @@ struct StarBufs {
   uint32_t n_tmax = 0;  // entries of tmax (the overflow bound's per-block maxima)
 };
 
-// Steps 3 and 4: prefix popcounts of the parent bitmap, then the slot map
-// (row_of) and the cleared accumulators.
-static int32_t launch_rank_map(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b,
-                               mtl_stream* s) {
-  hipStream_t st = s->stream;
-  const uint64_t prows = d->p_rows;
-  const uint32_t ptiles = filter_tiles(prows);
+// Arguments of one query's rank kernel (k_rank's parameters).
+struct RankLaunch {
+  const uint64_t* bm;
+  uint64_t words;
+  uint32_t rtiles;
+  uint32_t* rank;
+  uint64_t* status;
+  uint32_t* ctr;
+  uint32_t* err;
+  uint32_t* par_total;
+  PopArgs pa;
+  uint32_t grid;
+};
+
+__global__ __launch_bounds__(RBLOCK) void k_mq_rank(const RankLaunch* qs) {
+  const RankLaunch& a = qs[blockIdx.y];
+  if (blockIdx.x >= a.grid) return;
+  rank_body(a.bm, a.words, a.rtiles, a.rank, a.status, a.ctr, a.err, a.par_total, a.pa,
+            blockIdx.x, a.grid);
+}
+
+static RankLaunch make_rank(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b) {
+  const uint32_t ptiles = filter_tiles(d->p_rows);
   uint32_t* mu = (uint32_t*)b.misc;
-  {
-    ProfScope ps(s, "k_rank");
-    PopArgs pa{};
-    pa.pbits = (const uint64_t*)b.pbits;
-    pa.pbits_words = (uint64_t)ptiles * kTileRows / 64;
-    if (d->d_rows) {
-      pa.dim_bm = (const uint64_t*)b.dim_bm;
-      pa.dim_words = sp.dwords;
-      pa.dbits = (const uint64_t*)b.dbits;
-      pa.dbits_words = (uint64_t)filter_tiles(d->d_rows) * kTileRows / 64;
-    }
-    pa.pops = mu + 7;  // [7] P pass bits, [8] D bitmap, [9] D pass bits
-    const uint32_t rt = std::max<uint32_t>(b.rtiles, 1);
-    const uint64_t pop_words = std::max(pa.pbits_words, std::max(pa.dim_words, pa.dbits_words));
-    const uint32_t pop_blocks =
-        (uint32_t)std::min<uint64_t>(ceil_div(pop_words, (uint64_t)RBLOCK * 8), 2048);
-    hipLaunchKernelGGL(k_rank, dim3(std::max(rt, pop_blocks)), dim3(RBLOCK), 0, st,
-                       (const uint64_t*)b.par_bm, sp.pwords, rt, (uint32_t*)b.par_rank,
-                       (uint64_t*)b.st_r, mu + 2, mu + 3, mu + 6, pa);
+  RankLaunch r{};
+  r.pa.pbits = (const uint64_t*)b.pbits;
+  r.pa.pbits_words = (uint64_t)ptiles * kTileRows / 64;
+  if (d->d_rows) {
+    r.pa.dim_bm = (const uint64_t*)b.dim_bm;
+    r.pa.dim_words = sp.dwords;
+    r.pa.dbits = (const uint64_t*)b.dbits;
+    r.pa.dbits_words = (uint64_t)filter_tiles(d->d_rows) * kTileRows / 64;
   }
-  MTL_HIP(hipGetLastError());
+  r.pa.pops = mu + 7;  // [7] P pass bits, [8] D bitmap, [9] D pass bits
+  const uint32_t rt = std::max<uint32_t>(b.rtiles, 1);
+  const uint64_t pop_words =
+      std::max(r.pa.pbits_words, std::max(r.pa.dim_words, r.pa.dbits_words));
+  const uint32_t pop_blocks =
+      (uint32_t)std::min<uint64_t>(ceil_div(pop_words, (uint64_t)RBLOCK * 8), 2048);
+  r.bm = (const uint64_t*)b.par_bm;
+  r.words = sp.pwords;
+  r.rtiles = rt;
+  r.rank = (uint32_t*)b.par_rank;
+  r.status = (uint64_t*)b.st_r;
+  r.ctr = mu + 2;
+  r.err = mu + 3;
+  r.par_total = mu + 6;
+  r.grid = std::max(rt, pop_blocks);
+  return r;
+}
+
+// k_map4: 4-byte parent keys aligned for vector loads.
+static bool map_k4(const mtl_star_desc* d) {
+  return type_width(d->p_key_type) == 4 && d->p_key_type != MTL_FLOAT &&
+         ((uintptr_t)d->p_key % 16) == 0;
+}
+
+static MapArgs make_map(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b) {
+  uint32_t* mu = (uint32_t*)b.misc;
   MapArgs ma{};
   ma.sel_bm = (const uint64_t*)b.pbits;
-  ma.n_rows = prows;
+  ma.n_rows = d->p_rows;
   ma.key = d->p_key;
   ma.key_type = d->p_key_type;
   ma.min = d->p_key_min;
@@ -969,21 +1056,33 @@ static int32_t launch_rank_map(const mtl_star_desc* d, const StarPlan& sp, const
   ma.pops = mu + 7;
   ma.has_dim = d->d_rows ? 1u : 0u;
   ma.err = mu + 3;
-  uint32_t g = (uint32_t)std::min<uint64_t>(ceil_div(prows ? prows : 1, 256 * 16), 4096);
+  ma.grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(ceil_div(d->p_rows ? d->p_rows : 1, 256 * 16), 4096));
+  return ma;
+}
+
+// Steps 3 and 4: prefix popcounts of the parent bitmap, then the slot map
+// (row_of) and the cleared accumulators.
+static int32_t launch_rank_map(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b,
+                               mtl_stream* s) {
+  hipStream_t st = s->stream;
+  {
+    ProfScope ps(s, "k_rank");
+    const RankLaunch r = make_rank(d, sp, b);
+    hipLaunchKernelGGL(k_rank, dim3(r.grid), dim3(RBLOCK), 0, st, r.bm, r.words, r.rtiles,
+                       r.rank, r.status, r.ctr, r.err, r.par_total, r.pa);
+  }
+  MTL_HIP(hipGetLastError());
+  const MapArgs ma = make_map(d, sp, b);
   ProfScope ps(s, "k_map");
-  if (type_width(d->p_key_type) == 4 && d->p_key_type != MTL_FLOAT &&
-      ((uintptr_t)d->p_key % 16) == 0)
-    hipLaunchKernelGGL(k_map4, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
-  else
-    hipLaunchKernelGGL(k_map, dim3(std::max<uint32_t>(g, 1)), dim3(256), 0, st, ma);
+  if (map_k4(d)) hipLaunchKernelGGL(k_map4, dim3(ma.grid), dim3(256), 0, st, ma);
+  else hipLaunchKernelGGL(k_map, dim3(ma.grid), dim3(256), 0, st, ma);
   MTL_HIP(hipGetLastError());
   return MTL_OK;
 }
 
-// Step 6: top-K over the accumulators and the output rows.
-static int32_t launch_star_topk(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b,
-                                mtl_star_row* out, uint32_t* out_n, mtl_stream* s) {
-  hipStream_t st = s->stream;
+static TopkArgs make_topk(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b,
+                          mtl_star_row* out, uint32_t* out_n) {
   uint32_t* mu = (uint32_t*)b.misc;
   TopkArgs ta{};
   ta.m_count = mu + 6;
@@ -1000,10 +1099,7 @@ static int32_t launch_star_topk(const mtl_star_desc* d, const StarPlan& sp, cons
   }
   for (int f = 0; f < 4; ++f) ta.stage.ck[f] = (uint64_t*)b.ck[f];
   ta.stage.cid = (uint32_t*)b.cid;
-  TopkLists lists[2];
-  for (int i = 0; i < 2; ++i)
-    lists[i] = TopkLists{(uint64_t*)b.lk0[i], (uint32_t*)b.lg[i], (uint32_t*)b.ln[i]};
-  ta.lists0 = lists[0];
+  ta.lists0 = TopkLists{(uint64_t*)b.lk0[0], (uint32_t*)b.lg[0], (uint32_t*)b.ln[0]};
   ta.out = out;
   ta.out_n = out_n;
   ta.err = mu + 3;
@@ -1012,6 +1108,18 @@ static int32_t launch_star_topk(const mtl_star_desc* d, const StarPlan& sp, cons
   ta.bcmax = (uint32_t*)b.bcmax;
   ta.n_bcmax = sp.topk_grid;
   ta.p_rows = d->p_rows;
+  ta.grid = sp.topk_grid;
+  return ta;
+}
+
+// Step 6: top-K over the accumulators and the output rows.
+static int32_t launch_star_topk(const mtl_star_desc* d, const StarPlan& sp, const StarBufs& b,
+                                mtl_star_row* out, uint32_t* out_n, mtl_stream* s) {
+  hipStream_t st = s->stream;
+  const TopkArgs ta = make_topk(d, sp, b, out, out_n);
+  TopkLists lists[2];
+  for (int i = 0; i < 2; ++i)
+    lists[i] = TopkLists{(uint64_t*)b.lk0[i], (uint32_t*)b.lg[i], (uint32_t*)b.ln[i]};
   {
     ProfScope ps(s, "k_star_topk");
     if (d->k <= 64)
@@ -1278,13 +1386,17 @@ extern "C" int32_t mtl_star_status(uint32_t error_bits) {
 }
 
 // ---------------------------------------------------------------------------
-// Shared passes of a batch of star queries (star.hpp).  Each pass streams
-// its table once: every 4096-row tile's key and predicate columns are loaded
-// together, then each query of the batch evaluates its own predicate and
-// row count on the loaded values and writes its own pass bits and bitmap
-// marks.  The fact pass adds each hit straight into its query's accumulator
-// (rank, expression columns, value): there are no hit lists, so a batch
-// needs no scratch proportional to the fact table.
+// Shared passes of a batch of star queries (star.hpp).  Each of the three
+// table passes is one launch for the whole batch with one block per (tile,
+// query) (rowtile.hpp MQMap): the blocks that read one tile for different
+// queries run together on one XCD, so the tile comes from HBM once.  A
+// block loads its tile's key and predicate columns, applies its query's
+// predicate and row count, and writes that query's pass bits and bitmap
+// marks.  The fact pass adds each hit straight into its query's
+// accumulator (rank, expression columns, value): there are no hit lists, so
+// a batch needs no scratch proportional to the fact table.  The per-query
+// steps between the passes (ranks, slot map, top-K and its merge levels)
+// are one launch each for the batch, blockIdx.y selecting the query.
 // ---------------------------------------------------------------------------
 namespace mtl {
 
@@ -1304,9 +1416,8 @@ struct MQStarQ {
 };
 
 struct MQStarArgs {
-  uint32_t nq;
   const MQStarQ* q;
-  uint64_t n_max;  // rows of the streamed table (largest over the batch)
+  MQMap map;  // blocks of the pass being launched
   const void* d_key;
   int32_t d_key_type;
   uint32_t d_kvec;
@@ -1395,73 +1506,67 @@ __device__ __forceinline__ void mark_global(uint64_t* bm, const uint32_t (&off)[
   if (e) atomicOr(err, e);
 }
 
-// Dimension pass: each query's pass bits and key bitmap.
+// Dimension pass: one query's pass bits and key bitmap for one tile.
 __global__ __launch_bounds__(FBLOCK) void k_mq_dim(MQStarArgs a) {
-  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint32_t tile = blockIdx.x;
+  uint32_t qi, tile;
+  mq_block(a.map, blockIdx.x, qi, tile);
+  const MQStarQ& Q = a.q[qi];
+  const uint64_t n = Q.d_rows;
+  if ((uint64_t)tile * FTILE >= n) return;  // block-uniform
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
   uint32_t px[FJ][4], ko[FJ][4];
-  if (a.d_pw) load_u32x16(a.d_pcol, a.d_pw, k0, a.n_max, a.d_pvec, px);
-  load_u32x16(a.d_key, a.d_key_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.d_kvec, ko);
+  if (a.d_pw) load_u32x16(a.d_pcol, a.d_pw, k0, n, a.d_pvec, px);
+  load_u32x16(a.d_key, a.d_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.d_kvec, ko);
+  uint32_t bits = rows_below(k0, n);
+  if (a.d_pw) pred_u32x16(Q.dpred, px, bits);
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) ko[j][v] = narrow_off(ko[j][v], a.d_key_type, a.d_min, a.dspan);
-  for (uint32_t q = 0; q < a.nq; ++q) {
-    const MQStarQ& Q = a.q[q];
-    if ((uint64_t)tile * FTILE >= Q.d_rows) continue;  // block-uniform
-    uint32_t bits = rows_below(k0, Q.d_rows);
-    if (a.d_pw) pred_u32x16(Q.dpred, px, bits);
-    store_pass_bits(Q.dbits, seg, bits, lane);
-    mark_global(Q.dim_bm, ko, bits, Q.err, lane);
-  }
+  store_pass_bits(Q.dbits, seg, bits, lane);
+  mark_global(Q.dim_bm, ko, bits, Q.err, lane);
 }
 
 // Parent pass: predicate, semi-join probe of the query's dimension bitmap,
-// pass bits and parent key bitmap.
+// pass bits and parent key bitmap for one tile of one query.
 __global__ __launch_bounds__(FBLOCK) void k_mq_parent(MQStarArgs a) {
-  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint32_t tile = blockIdx.x;
+  uint32_t qi, tile;
+  mq_block(a.map, blockIdx.x, qi, tile);
+  const MQStarQ& Q = a.q[qi];
+  const uint64_t n = Q.p_rows;
+  if ((uint64_t)tile * FTILE >= n) return;  // block-uniform
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
   uint32_t px[FJ][4], fo[FJ][4], ko[FJ][4];
-  if (a.p_pw) load_u32x16(a.p_pcol, a.p_pw, k0, a.n_max, a.p_pvec, px);
-  load_u32x16(a.p_fk, a.p_fk_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.p_fvec, fo);
-  load_u32x16(a.p_key, a.p_key_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.p_kvec, ko);
+  if (a.p_pw) load_u32x16(a.p_pcol, a.p_pw, k0, n, a.p_pvec, px);
+  load_u32x16(a.p_fk, a.p_fk_type == MTL_DICT8 ? 1u : 4u, k0, n, a.p_fvec, fo);
+  load_u32x16(a.p_key, a.p_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.p_kvec, ko);
+  uint32_t bits = rows_below(k0, n);
+  if (a.p_pw) pred_u32x16(Q.ppred, px, bits);
+  const uint32_t* bm32 = (const uint32_t*)Q.dim_bm;
+  uint32_t w[FJ][4];
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       fo[j][v] = narrow_off(fo[j][v], a.p_fk_type, a.d_min, a.dspan);
-      ko[j][v] = narrow_off(ko[j][v], a.p_key_type, a.p_min, 1ull << 32);
+      const bool need = ((bits >> (4 * j + v)) & 1u) && fo[j][v] != 0xFFFFFFFFu;
+      w[j][v] = need ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
     }
-  for (uint32_t q = 0; q < a.nq; ++q) {
-    const MQStarQ& Q = a.q[q];
-    if ((uint64_t)tile * FTILE >= Q.p_rows) continue;  // block-uniform
-    uint32_t bits = rows_below(k0, Q.p_rows);
-    if (a.p_pw) pred_u32x16(Q.ppred, px, bits);
-    const uint32_t* bm32 = (const uint32_t*)Q.dim_bm;
-    uint32_t w[FJ][4];
 #pragma unroll
-    for (int j = 0; j < FJ; ++j)
+  for (int j = 0; j < FJ; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const bool need = ((bits >> (4 * j + v)) & 1u) && fo[j][v] != 0xFFFFFFFFu;
-        w[j][v] = need ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
-      }
-    uint32_t mo[FJ][4];
-#pragma unroll
-    for (int j = 0; j < FJ; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        if (fo[j][v] == 0xFFFFFFFFu || !((w[j][v] >> (fo[j][v] & 31)) & 1u))
-          bits &= ~(1u << (4 * j + v));
-        mo[j][v] = ko[j][v] < Q.pspan ? ko[j][v] : 0xFFFFFFFFu;
-      }
-    store_pass_bits(Q.pbits, seg, bits, lane);
-    mark_global(Q.par_bm, mo, bits, Q.err, lane);
-  }
+    for (int v = 0; v < 4; ++v) {
+      if (fo[j][v] == 0xFFFFFFFFu || !((w[j][v] >> (fo[j][v] & 31)) & 1u))
+        bits &= ~(1u << (4 * j + v));
+      const uint32_t o = narrow_off(ko[j][v], a.p_key_type, a.p_min, 1ull << 32);
+      ko[j][v] = o < Q.pspan ? o : 0xFFFFFFFFu;
+    }
+  store_pass_bits(Q.pbits, seg, bits, lane);
+  mark_global(Q.par_bm, ko, bits, Q.err, lane);
 }
 
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
@@ -1473,103 +1578,100 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
   return v;
 }
 
-// Fact pass: predicate and parent probe per query; each hit's slot (rank),
-// expression columns and value, summed per lane over runs of one slot and
-// added to the query's accumulator.
+// Fact pass, one tile of one query: predicate and parent probe; each hit's
+// slot (rank), expression columns and value, summed per lane over runs of
+// one slot and added to the query's accumulator.
 template <bool DBL>
 __global__ __launch_bounds__(FBLOCK) void k_mq_fact(MQStarArgs a) {
   using V = typename std::conditional<DBL, double, int64_t>::type;
-  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
-  const uint32_t tile = blockIdx.x;
+  uint32_t qi, tile;
+  mq_block(a.map, blockIdx.x, qi, tile);
+  const MQStarQ& Q = a.q[qi];
+  const uint64_t n = Q.f_rows;
+  if ((uint64_t)tile * FTILE >= n) return;  // block-uniform
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
   uint32_t px[FJ][4], fo[FJ][4];
-  if (a.f_pw) load_u32x16(a.f_pcol, a.f_pw, k0, a.n_max, a.f_pvec, px);
-  load_u32x16(a.f_fk, a.f_fk_type == MTL_DICT8 ? 1u : 4u, k0, a.n_max, a.f_fvec, fo);
+  if (a.f_pw) load_u32x16(a.f_pcol, a.f_pw, k0, n, a.f_pvec, px);
+  load_u32x16(a.f_fk, a.f_fk_type == MTL_DICT8 ? 1u : 4u, k0, n, a.f_fvec, fo);
+  uint32_t cand = rows_below(k0, n);
+  if (a.f_pw) pred_u32x16(Q.fpred, px, cand);
+  const uint32_t* bm32 = (const uint32_t*)Q.par_bm;
+  uint32_t w[FJ][4];
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) fo[j][v] = narrow_off(fo[j][v], a.f_fk_type, a.p_min, 1ull << 32);
-  for (uint32_t q = 0; q < a.nq; ++q) {
-    const MQStarQ& Q = a.q[q];
-    if ((uint64_t)tile * FTILE >= Q.f_rows) continue;  // block-uniform
-    uint32_t cand = rows_below(k0, Q.f_rows);
-    if (a.f_pw) pred_u32x16(Q.fpred, px, cand);
-    const uint32_t* bm32 = (const uint32_t*)Q.par_bm;
-    uint32_t w[FJ][4];
+    for (int v = 0; v < 4; ++v) {
+      fo[j][v] = narrow_off(fo[j][v], a.f_fk_type, a.p_min, 1ull << 32);
+      if (fo[j][v] >= Q.pspan) cand &= ~(1u << (4 * j + v));
+      w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
+    }
+  uint32_t hits = 0;
 #pragma unroll
-    for (int j = 0; j < FJ; ++j)
+  for (int j = 0; j < FJ; ++j)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        if (fo[j][v] >= Q.pspan) cand &= ~(1u << (4 * j + v));
-        w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
-      }
-    uint32_t hits = 0;
+    for (int v = 0; v < 4; ++v)
+      if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (fo[j][v] & 31)) & 1u))
+        hits |= 1u << (4 * j + v);
+  if (!__ballot(hits != 0)) return;  // wave-uniform: most waves of a sparse join
+  uint32_t cur = 0xFFFFFFFFu, cn = 0;
+  V cs = 0;
+  uint64_t mx = 0;
+  bool ovf = false;
+  uint32_t h = hits;
+  while (h) {
+    const int i = __ffs(h) - 1;
+    h &= h - 1;
+    uint32_t sv = fo[0][0];
 #pragma unroll
-    for (int j = 0; j < FJ; ++j)
+    for (int t = 1; t < FJ * 4; ++t) sv = i == t ? fo[t >> 2][t & 3] : sv;
+    const uint64_t row = k0 + (uint64_t)(i >> 2) * 256 + (uint64_t)(i & 3);
+    const uint64_t wd = Q.par_bm[sv >> 6];
+    const uint32_t rk = Q.rank[sv >> 6];
+    V cv[MTL_EXPR_MAX_COLS];
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (fo[j][v] & 31)) & 1u))
-          hits |= 1u << (4 * j + v);
-    if (!__ballot(hits != 0)) continue;  // wave-uniform: most waves of a sparse join
-    uint32_t cur = 0xFFFFFFFFu, cn = 0;
-    V cs = 0;
-    uint64_t mx = 0;
-    bool ovf = false;
-    uint32_t h = hits;
-    while (h) {
-      const int i = __ffs(h) - 1;
-      h &= h - 1;
-      uint32_t sv = fo[0][0];
-#pragma unroll
-      for (int t = 1; t < FJ * 4; ++t) sv = i == t ? fo[t >> 2][t & 3] : sv;
-      const uint64_t row = k0 + (uint64_t)(i >> 2) * 256 + (uint64_t)(i & 3);
-      const uint64_t wd = Q.par_bm[sv >> 6];
-      const uint32_t rk = Q.rank[sv >> 6];
-      V cv[MTL_EXPR_MAX_COLS];
-#pragma unroll
-      for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c) {
-        if ((uint32_t)c < a.expr.n_cols) {
-          if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
-          else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
-        } else {
-          cv[c] = 0;
-        }
-      }
-      const uint64_t below = (sv & 63) ? (wd & ((1ull << (sv & 63)) - 1)) : 0ull;
-      const uint32_t r = rk + (uint32_t)__popcll(below);
-      V val;
-      if constexpr (DBL) {
-        val = expr_f64_vals(a.expr, cv);
+    for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c) {
+      if ((uint32_t)c < a.expr.n_cols) {
+        if constexpr (DBL) cv[c] = load_f64(a.expr.col[c], a.expr.col_type[c], row);
+        else cv[c] = load_key(a.expr.col[c], a.expr.col_type[c], row);
       } else {
-        val = expr_i64_vals(a.expr, cv, ovf);
-        const uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
-        mx = av > mx ? av : mx;
+        cv[c] = 0;
       }
-      if (r != cur) {
-        if (cur != 0xFFFFFFFFu) {
-          if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
-          else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
-          atomicAdd(&Q.cnt[cur], cn);
-        }
-        cur = r;
-        cs = 0;
-        cn = 0;
+    }
+    const uint64_t below = (sv & 63) ? (wd & ((1ull << (sv & 63)) - 1)) : 0ull;
+    const uint32_t r = rk + (uint32_t)__popcll(below);
+    V val;
+    if constexpr (DBL) {
+      val = expr_f64_vals(a.expr, cv);
+    } else {
+      val = expr_i64_vals(a.expr, cv, ovf);
+      const uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
+      mx = av > mx ? av : mx;
+    }
+    if (r != cur) {
+      if (cur != 0xFFFFFFFFu) {
+        if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
+        else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
+        atomicAdd(&Q.cnt[cur], cn);
       }
-      if constexpr (DBL) cs += val;
-      else cs = (int64_t)((uint64_t)cs + (uint64_t)val);  // wrapping, as the atomics
-      ++cn;
+      cur = r;
+      cs = 0;
+      cn = 0;
     }
-    if (cur != 0xFFFFFFFFu) {
-      if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
-      else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
-      atomicAdd(&Q.cnt[cur], cn);
-    }
-    if (ovf) atomicOr(Q.err, 16u);
-    if constexpr (!DBL) {
-      mx = wave_max_u64(mx);
-      if (lane == 0 && mx) atomicMax(Q.tmax, (unsigned long long)mx);
-    }
+    if constexpr (DBL) cs += val;
+    else cs = (int64_t)((uint64_t)cs + (uint64_t)val);  // wrapping, as the atomics
+    ++cn;
+  }
+  if (cur != 0xFFFFFFFFu) {
+    if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
+    else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
+    atomicAdd(&Q.cnt[cur], cn);
+  }
+  if (ovf) atomicOr(Q.err, 16u);
+  if constexpr (!DBL) {
+    mx = wave_max_u64(mx);
+    if (lane == 0 && mx) atomicMax(Q.tmax, (unsigned long long)mx);
   }
 }
 
@@ -1605,7 +1707,24 @@ bool star_shareable(const mtl_star_desc& a, const mtl_star_desc& b) {
          std::memcmp(&a.agg, &b.agg, sizeof(mtl_expr)) == 0;
 }
 
-size_t star_batch_stage_bytes(uint32_t n) { return (size_t)n * sizeof(MQStarQ) + 256; }
+// Merge levels of one query's top-K: lists per level, from its phase-1 grid.
+static uint32_t merge_levels(uint32_t grid, uint32_t k) {
+  const uint32_t fan = topk_fan(k);
+  uint32_t L = grid, lv = 0;
+  while (true) {
+    ++lv;
+    const uint32_t blocks = (uint32_t)ceil_div(L, fan);
+    if (blocks == 1) return lv;
+    L = blocks;
+  }
+}
+
+size_t star_batch_stage_bytes(uint32_t n) {
+  // per query: its pass arguments, rank, map and top-K arguments, and up to
+  // 8 merge levels
+  return (size_t)n * (sizeof(MQStarQ) + sizeof(RankLaunch) + sizeof(MapArgs) + sizeof(TopkArgs) +
+                      8 * sizeof(MQMergeQ)) + 1024;
+}
 
 int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_stage,
                            mtl_stream* s) {
@@ -1613,10 +1732,14 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
   if (n > kStarMQMax) return set_error(MTL_ERR_INVALID_ARGUMENT, "more than %u queries", kStarMQMax);
   const mtl_star_desc& d0 = *items[0].d;
   std::vector<StarPlan> sp(n);
+  uint32_t levels = 0;
   for (uint32_t i = 0; i < n; ++i) {
     MTL_TRY(star_validate(items[i].d, &sp[i]));
     if (!star_batchable(*items[i].d) || !star_shareable(d0, *items[i].d))
       return set_error(MTL_ERR_INVALID_ARGUMENT, "query %u cannot join the shared passes", i);
+    const uint32_t lv = merge_levels(sp[i].topk_grid, items[i].d->k);
+    if (lv > 8) return set_error(MTL_UNSUPPORTED, "top-K of %u merge levels", lv);
+    levels += lv;
   }
   Arena& ar = s->arena;
   hipStream_t st = s->stream;
@@ -1632,8 +1755,24 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
   void* clr;
   MTL_TRY(ar.alloc(clear_words * 4, &clr));
   uint32_t* cw = (uint32_t*)clr;
-  MQStarQ* hq = (MQStarQ*)host_stage;
-  uint64_t d_max = 0, p_max = 0, f_max = 0;
+  // staged arguments: [MQStarQ n][RankLaunch n][MapArgs n][TopkArgs n, by
+  // top-K shape][MQMergeQ, by level]
+  const size_t o_r = (size_t)n * sizeof(MQStarQ);
+  const size_t o_m = o_r + (size_t)n * sizeof(RankLaunch);
+  const size_t o_t = o_m + (size_t)n * sizeof(MapArgs);
+  const size_t o_g = o_t + (size_t)n * sizeof(TopkArgs);
+  const size_t stage_bytes = o_g + (size_t)levels * sizeof(MQMergeQ);
+  void* dq;
+  MTL_TRY(ar.alloc(stage_bytes, &dq));
+  char* hs = (char*)host_stage;
+  MQStarQ* hq = (MQStarQ*)hs;
+  RankLaunch* hr = (RankLaunch*)(hs + o_r);
+  MapArgs* hm = (MapArgs*)(hs + o_m);
+  TopkArgs* ht = (TopkArgs*)(hs + o_t);
+  MQMergeQ* hg = (MQMergeQ*)(hs + o_g);
+  std::vector<uint32_t> dt(n), pt(n), ft(n);
+  std::vector<TopkArgs> ta(n);
+  uint32_t rgrid = 1, mgrid = 1;
   for (uint32_t i = 0; i < n; ++i) {
     const mtl_star_desc& d = *items[i].d;
     StarBufs& bi = b[i];
@@ -1674,20 +1813,77 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
     Q.cnt = (uint32_t*)bi.cnt;
     Q.err = (uint32_t*)bi.misc + 3;
     Q.tmax = (unsigned long long*)bi.tmax;
-    d_max = std::max(d_max, d.d_rows);
-    p_max = std::max(p_max, d.p_rows);
-    f_max = std::max(f_max, d.f_rows);
+    hr[i] = make_rank(&d, sp[i], bi);
+    hm[i] = make_map(&d, sp[i], bi);
+    ta[i] = make_topk(&d, sp[i], bi, items[i].out, items[i].out_n);
+    rgrid = std::max(rgrid, hr[i].grid);
+    mgrid = std::max(mgrid, hm[i].grid);
+    dt[i] = filter_tiles(d.d_rows);
+    pt[i] = filter_tiles(d.p_rows);
+    ft[i] = d.f_rows ? filter_tiles(d.f_rows) : 0u;
   }
-  void* dq;
-  MTL_TRY(ar.alloc((size_t)n * sizeof(MQStarQ), &dq));
-  MTL_HIP(hipMemcpyAsync(dq, hq, (size_t)n * sizeof(MQStarQ), hipMemcpyHostToDevice, st));
+  // top-K arguments grouped by wave-list shape (K <= 64, <= 128, above)
+  auto shape = [](uint32_t k) { return k <= 64 ? 0 : k <= 128 ? 1 : 2; };
+  std::vector<uint32_t> tpos(n);  // query -> position in ht
+  uint32_t cls_off[4] = {0, 0, 0, 0}, cls_grid[3] = {1, 1, 1};
+  {
+    uint32_t at = 0;
+    for (int c = 0; c < 3; ++c) {
+      cls_off[c] = at;
+      for (uint32_t i = 0; i < n; ++i)
+        if (shape(items[i].d->k) == c) {
+          tpos[i] = at;
+          ht[at++] = ta[i];
+          cls_grid[c] = std::max(cls_grid[c], ta[i].grid);
+        }
+    }
+    cls_off[3] = at;
+  }
+  // merge levels, level-major: every query's level j before any level j+1
+  std::vector<uint32_t> lvl_off, lvl_grid;
+  {
+    std::vector<uint32_t> L(n), in(n, 0), left(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      L[i] = sp[i].topk_grid;
+      left[i] = merge_levels(sp[i].topk_grid, items[i].d->k);
+    }
+    uint32_t at = 0;
+    while (true) {
+      const uint32_t start = at;
+      uint32_t g = 1;
+      for (uint32_t i = 0; i < n; ++i) {
+        if (!left[i]) continue;
+        const uint32_t fan = topk_fan(items[i].d->k);
+        const uint32_t blocks = (uint32_t)ceil_div(L[i], fan);
+        MQMergeQ& m = hg[at++];
+        m.ta = (const TopkArgs*)((char*)dq + o_t) + tpos[i];
+        m.in = TopkLists{(uint64_t*)b[i].lk0[in[i]], (uint32_t*)b[i].lg[in[i]],
+                         (uint32_t*)b[i].ln[in[i]]};
+        m.out = TopkLists{(uint64_t*)b[i].lk0[in[i] ^ 1], (uint32_t*)b[i].lg[in[i] ^ 1],
+                          (uint32_t*)b[i].ln[in[i] ^ 1]};
+        m.L = L[i];
+        m.final_level = blocks == 1 ? 1u : 0u;
+        m.blocks = blocks;
+        g = std::max(g, blocks);
+        L[i] = blocks;
+        in[i] ^= 1;
+        --left[i];
+      }
+      if (at == start) break;
+      lvl_off.push_back(start);
+      lvl_off.push_back(at);
+      lvl_grid.push_back(g);
+    }
+  }
+  MTL_HIP(hipMemcpyAsync(dq, hs, stage_bytes, hipMemcpyHostToDevice, st));
   ZeroList zl;
   zl.add((uint32_t*)clr, clear_words);
   MTL_TRY(launch_zero(zl, st));
-  auto vec = [](const void* p, uint32_t w) { return ((uintptr_t)p % (w == 1 ? 4 : 16)) == 0 ? 1u : 0u; };
+  auto vec = [](const void* p, uint32_t w) {
+    return ((uintptr_t)p % (w == 1 ? 4 : 16)) == 0 ? 1u : 0u;
+  };
   auto kw = [](int32_t t) { return t == MTL_DICT8 ? 1u : 4u; };
   MQStarArgs a{};
-  a.nq = n;
   a.q = (const MQStarQ*)dq;
   a.d_key = d0.d_key;
   a.d_key_type = d0.d_key_type;
@@ -1716,28 +1912,64 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
   a.expr = d0.agg;
   {
     ProfScope ps(s, "k_mq_dim");
-    a.n_max = d_max;
-    hipLaunchKernelGGL(k_mq_dim, dim3(filter_tiles(d_max)), dim3(FBLOCK), 0, st, a);
+    const uint32_t grid = mq_map(dt.data(), n, &a.map);
+    hipLaunchKernelGGL(k_mq_dim, dim3(grid), dim3(FBLOCK), 0, st, a);
     MTL_HIP(hipGetLastError());
   }
   {
     ProfScope ps(s, "k_mq_parent");
-    a.n_max = p_max;
-    hipLaunchKernelGGL(k_mq_parent, dim3(filter_tiles(p_max)), dim3(FBLOCK), 0, st, a);
+    const uint32_t grid = mq_map(pt.data(), n, &a.map);
+    hipLaunchKernelGGL(k_mq_parent, dim3(grid), dim3(FBLOCK), 0, st, a);
     MTL_HIP(hipGetLastError());
   }
-  for (uint32_t i = 0; i < n; ++i) MTL_TRY(launch_rank_map(items[i].d, sp[i], b[i], s));
-  if (f_max) {
+  {
+    ProfScope ps(s, "k_mq_rank");
+    hipLaunchKernelGGL(k_mq_rank, dim3(rgrid, n), dim3(RBLOCK), 0, st,
+                       (const RankLaunch*)((char*)dq + o_r));
+    MTL_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(s, "k_mq_map");
+    const MapArgs* dm = (const MapArgs*)((char*)dq + o_m);
+    if (map_k4(&d0)) hipLaunchKernelGGL(k_mq_map<true>, dim3(mgrid, n), dim3(256), 0, st, dm);
+    else hipLaunchKernelGGL(k_mq_map<false>, dim3(mgrid, n), dim3(256), 0, st, dm);
+    MTL_HIP(hipGetLastError());
+  }
+  uint32_t fmax = 0;
+  for (uint32_t i = 0; i < n; ++i) fmax = std::max(fmax, ft[i]);
+  if (fmax) {
     ProfScope ps(s, "k_mq_fact");
-    a.n_max = f_max;
-    if (d0.agg.is_double)
-      hipLaunchKernelGGL(k_mq_fact<true>, dim3(filter_tiles(f_max)), dim3(FBLOCK), 0, st, a);
-    else
-      hipLaunchKernelGGL(k_mq_fact<false>, dim3(filter_tiles(f_max)), dim3(FBLOCK), 0, st, a);
+    const uint32_t grid = mq_map(ft.data(), n, &a.map);
+    if (d0.agg.is_double) hipLaunchKernelGGL(k_mq_fact<true>, dim3(grid), dim3(FBLOCK), 0, st, a);
+    else hipLaunchKernelGGL(k_mq_fact<false>, dim3(grid), dim3(FBLOCK), 0, st, a);
     MTL_HIP(hipGetLastError());
   }
-  for (uint32_t i = 0; i < n; ++i)
-    MTL_TRY(launch_star_topk(items[i].d, sp[i], b[i], items[i].out, items[i].out_n, s));
+  {
+    ProfScope ps(s, "k_mq_star_topk");
+    const TopkArgs* dt_ = (const TopkArgs*)((char*)dq + o_t);
+    for (int c = 0; c < 3; ++c) {
+      const uint32_t cnt = cls_off[c + 1] - cls_off[c];
+      if (!cnt) continue;
+      const dim3 g(cls_grid[c], cnt);
+      if (c == 0)
+        hipLaunchKernelGGL(k_mq_star_topk<1>, g, dim3(TopkShape<1>::NT), 0, st, dt_ + cls_off[c]);
+      else if (c == 1)
+        hipLaunchKernelGGL(k_mq_star_topk<2>, g, dim3(TopkShape<2>::NT), 0, st, dt_ + cls_off[c]);
+      else
+        hipLaunchKernelGGL(k_mq_star_topk<8>, g, dim3(TopkShape<8>::NT), 0, st, dt_ + cls_off[c]);
+      MTL_HIP(hipGetLastError());
+    }
+  }
+  {
+    ProfScope ps(s, "k_mq_star_merge");
+    const MQMergeQ* dg = (const MQMergeQ*)((char*)dq + o_g);
+    for (size_t j = 0; j < lvl_grid.size(); ++j) {
+      const uint32_t lo = lvl_off[2 * j], hi = lvl_off[2 * j + 1];
+      hipLaunchKernelGGL(k_mq_star_merge, dim3(lvl_grid[j], hi - lo), dim3(TOPK_MERGE_THREADS),
+                         0, st, dg + lo);
+      MTL_HIP(hipGetLastError());
+    }
+  }
   return MTL_OK;
 }
 

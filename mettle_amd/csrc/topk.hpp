@@ -369,10 +369,11 @@ __device__ __forceinline__ void topk_share_thr(const WaveTopk<R>& wl, TopkBlockL
 // thread of the block calls it.
 template <int NT, int R, int W, int CAP>
 __device__ __forceinline__ void topk_block_finish(const WaveTopk<R>& wl, TopkBlockLds<W, CAP>& s,
-                                                  const TopkStage& st, const TopkLists& out) {
+                                                  const TopkStage& st, const TopkLists& out,
+                                                  uint32_t bid) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const uint32_t K = wl.K;
-  const uint32_t gbase = ((uint32_t)blockIdx.x * W + wave) * K;
+  const uint32_t gbase = (bid * W + wave) * K;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t i = r * 64 + lane;
@@ -393,10 +394,17 @@ __device__ __forceinline__ void topk_block_finish(const WaveTopk<R>& wl, TopkBlo
   const int cur = tk_tree_merge<NT>(k0p, gp, np, W, K, st);
   const uint32_t nf = s.n[cur][0];
   for (uint32_t i = threadIdx.x; i < nf; i += NT) {
-    out.k0[(uint64_t)blockIdx.x * K + i] = s.k0[cur][i];
-    out.g[(uint64_t)blockIdx.x * K + i] = s.g[cur][i];
+    out.k0[(uint64_t)bid * K + i] = s.k0[cur][i];
+    out.g[(uint64_t)bid * K + i] = s.g[cur][i];
   }
-  if (threadIdx.x == 0) out.n[blockIdx.x] = nf;
+  if (threadIdx.x == 0) out.n[bid] = nf;
+}
+
+// The same for a kernel whose block index is its list index.
+template <int NT, int R, int W, int CAP>
+__device__ __forceinline__ void topk_block_finish(const WaveTopk<R>& wl, TopkBlockLds<W, CAP>& s,
+                                                  const TopkStage& st, const TopkLists& out) {
+  topk_block_finish<NT>(wl, s, st, out, (uint32_t)blockIdx.x);
 }
 
 // Merge kernel body: block b merges lists [b*F, b*F + F) of `in` (L lists)
@@ -417,9 +425,10 @@ __host__ __device__ __forceinline__ uint32_t topk_fan(uint32_t K) {
 }
 
 __device__ __forceinline__ int topk_merge_lists(TopkMergeLds& s, const TopkLists& in, uint32_t L,
-                                                uint32_t K, const TopkStage& st) {
+                                                uint32_t K, const TopkStage& st,
+                                                uint32_t bid = 0xFFFFFFFFu) {
   const uint32_t F = topk_fan(K);
-  const uint32_t first = blockIdx.x * F;
+  const uint32_t first = (bid == 0xFFFFFFFFu ? (uint32_t)blockIdx.x : bid) * F;
   const uint32_t nl = L - first < F ? L - first : F;
   for (uint32_t t = threadIdx.x; t < nl * K; t += TOPK_MERGE_THREADS) {
     const uint64_t src = (uint64_t)first * K + t;  // entries past a list's length are unused
@@ -435,13 +444,14 @@ __device__ __forceinline__ int topk_merge_lists(TopkMergeLds& s, const TopkLists
 }
 
 __device__ __forceinline__ void topk_write_list(const TopkMergeLds& s, int cur, const TopkLists& out,
-                                                uint32_t K) {
+                                                uint32_t K, uint32_t bid = 0xFFFFFFFFu) {
+  const uint32_t b = bid == 0xFFFFFFFFu ? (uint32_t)blockIdx.x : bid;
   const uint32_t nf = s.n[cur][0];
   for (uint32_t i = threadIdx.x; i < nf; i += TOPK_MERGE_THREADS) {
-    out.k0[(uint64_t)blockIdx.x * K + i] = s.k0[cur][i];
-    out.g[(uint64_t)blockIdx.x * K + i] = s.g[cur][i];
+    out.k0[(uint64_t)b * K + i] = s.k0[cur][i];
+    out.g[(uint64_t)b * K + i] = s.g[cur][i];
   }
-  if (threadIdx.x == 0) out.n[blockIdx.x] = nf;
+  if (threadIdx.x == 0) out.n[b] = nf;
 }
 
 }  // namespace dev
