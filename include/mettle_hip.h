@@ -206,6 +206,23 @@ int32_t mtl_filter(const mtl_pred *preds, uint32_t n_preds, uint64_t n_rows,
                    uint64_t in_capacity, uint32_t *out_sel,
                    uint32_t *out_count, mtl_stream *s);
 
+/* mtl_filter with no allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * filter's scratch (pass bits, tile counts, group sums) lives in a 16 MiB
+ * static device array, one per GPU, held until the process ends; the call
+ * makes no hipMalloc, hipFree or other allocation.  Same arguments and
+ * results as mtl_filter, at most 100,000,000 rows per call (more returns
+ * MTL_ERR_INVALID_ARGUMENT).  s == NULL uses the default stream, which
+ * must exist (mtl_init or an earlier call created it).  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (concurrent
+ * streams or threads) race; serialise them. */
+int32_t mtl_filter_noalloc(const mtl_pred *preds, uint32_t n_preds,
+                           uint64_t n_rows, const uint32_t *in_sel,
+                           const uint32_t *in_count, uint64_t in_capacity,
+                           uint32_t *out_sel, uint32_t *out_count,
+                           mtl_stream *s);
+
 /* mtl_filter fused with one mtl_project of the passing rows: out_sel and
  * *out_count as mtl_filter; out_proj[i] = proj_column[out_sel[i]] op scalar
  * for i < *out_count, with mtl_project's types, ops and overflow flag.  The

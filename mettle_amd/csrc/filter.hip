@@ -1094,10 +1094,9 @@ using namespace mtl;
 
 extern "C" {
 
-static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
-                           const uint32_t* in_sel, const uint32_t* in_count,
-                           uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
-                           const CompactProj* proj, ScanExtras* ex, mtl_stream* s) {
+static int32_t filter_check(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                            const uint32_t* in_sel, uint64_t in_capacity, uint32_t* out_sel,
+                            uint32_t* out_count, uint64_t* cap) {
   if (n_preds > MTL_MAX_PREDS)
     return set_error(MTL_UNSUPPORTED, "more than %d predicates", MTL_MAX_PREDS);
   if (!out_sel || !out_count)
@@ -1107,42 +1106,40 @@ static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_r
     if (!type_width(preds[p].type) || preds[p].type == MTL_HUGEINT ||
         preds[p].op < 0 || preds[p].op > 4 || !preds[p].column)
       return set_error(MTL_UNSUPPORTED, "predicate %u: unsupported type/op", p);
-  uint64_t cap = in_sel ? in_capacity : n_rows;
-  if (cap > 0xFFFFFFFFull)
+  *cap = in_sel ? in_capacity : n_rows;
+  if (*cap > 0xFFFFFFFFull)
     return set_error(MTL_UNSUPPORTED, "more than 2^32-1 rows");
-  MTL_TRY(resolve_stream(s, &s));
-  std::lock_guard<std::mutex> g(s->mu);
-  s->arena.reset();
-  uint32_t tiles = filter_tiles(cap);
+  return MTL_OK;
+}
+
+// Device scratch of one filter over `tiles` tiles: pass bits, tile counts,
+// a clearing word block, group sums (and fingerprint sums).
+struct FilterScratch {
   void *bm, *counts, *ctr, *gsum;
-  MTL_TRY(s->arena.alloc((size_t)tiles * FTILE / 8, &bm));
-  MTL_TRY(s->arena.alloc((size_t)tiles * 4, &counts));
-  MTL_TRY(s->arena.alloc(256, &ctr));
-  const uint32_t groups = (uint32_t)ceil_div(tiles, 1u << kGroupShift);
-  MTL_TRY(s->arena.alloc((size_t)groups * kGroupStride * 4, &gsum));
+};
+
+// The filter pass and the expansion on s with the given scratch.  prof:
+// record kernel timings (creates events when profiling is on).
+static int32_t filter_enqueue(const mtl_pred* preds, uint32_t n_preds, uint64_t cap,
+                              const uint32_t* in_sel, const uint32_t* in_count,
+                              uint32_t* out_sel, uint32_t* out_count, const CompactProj* proj,
+                              ScanExtras* ex, const FilterScratch& sc, uint64_t* fpg,
+                              uint32_t tiles, uint32_t groups, bool prof, mtl_stream* s) {
   ZeroList zl = ex ? ex->zero : ZeroList{};
   if (zl.n > 5) return set_error(MTL_ERR_INVALID_ARGUMENT, "too many regions to clear");
-  zl.add((uint32_t*)ctr, 64);
-  zl.add((uint32_t*)gsum, (uint64_t)groups * kGroupStride);
-  uint64_t* fpg = nullptr;
-  if (ex && ex->fingerprint) {
-    void* p;
-    MTL_TRY(s->arena.alloc((size_t)groups * kFpStride * 8, &p));
-    fpg = (uint64_t*)p;
-    zl.add((uint32_t*)p, (uint64_t)groups * kFpStride * 2);
-    ex->fp_groups = fpg;
-    ex->fp_n = groups;
-  }
+  zl.add((uint32_t*)sc.ctr, 64);
+  zl.add((uint32_t*)sc.gsum, (uint64_t)groups * kGroupStride);
+  if (fpg) zl.add((uint32_t*)fpg, (uint64_t)groups * kFpStride * 2);
   MTL_TRY(launch_zero(zl, s->stream));
   if (!in_sel) {
     DenseArgs a{};
     for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = compile_pred(preds[p]);
     a.n_preds = n_preds;
     a.n_rows = cap;
-    a.sel_bm = (uint64_t*)bm;
-    a.counts = (uint32_t*)counts;
-    a.group_sums = (uint32_t*)gsum;
-    ProfScope ps(s, "k_filter");
+    a.sel_bm = (uint64_t*)sc.bm;
+    a.counts = (uint32_t*)sc.counts;
+    a.group_sums = (uint32_t*)sc.gsum;
+    ProfScope ps(s, "k_filter", prof);
     MTL_TRY(launch_filter_dense(a, tiles, s->stream));
   } else {
     FilterArgs a{};
@@ -1151,16 +1148,121 @@ static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_r
     a.n_rows = cap;
     a.in_sel = in_sel;
     a.in_count = in_count;
-    a.sel_bm = (uint64_t*)bm;
-    a.tile_counts = (uint32_t*)counts;
-    a.group_sums = (uint32_t*)gsum;
-    a.err = (uint32_t*)ctr;
-    ProfScope ps(s, "k_filter");
+    a.sel_bm = (uint64_t*)sc.bm;
+    a.tile_counts = (uint32_t*)sc.counts;
+    a.group_sums = (uint32_t*)sc.gsum;
+    a.err = (uint32_t*)sc.ctr;
+    ProfScope ps(s, "k_filter", prof);
     MTL_TRY(launch_filter(a, cap, s->stream));
   }
-  ProfScope ps(s, "k_compact");
-  return launch_compact((const uint64_t*)bm, (uint32_t*)counts, tiles, in_sel, out_sel,
-                        out_count, s->stream, proj, (const uint32_t*)gsum, fpg);
+  ProfScope ps(s, "k_compact", prof);
+  return launch_compact((const uint64_t*)sc.bm, (uint32_t*)sc.counts, tiles, in_sel, out_sel,
+                        out_count, s->stream, proj, (const uint32_t*)sc.gsum, fpg);
+}
+
+static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                           const uint32_t* in_sel, const uint32_t* in_count,
+                           uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
+                           const CompactProj* proj, ScanExtras* ex, mtl_stream* s) {
+  uint64_t cap;
+  MTL_TRY(filter_check(preds, n_preds, n_rows, in_sel, in_capacity, out_sel, out_count, &cap));
+  MTL_TRY(resolve_stream(s, &s));
+  std::lock_guard<std::mutex> g(s->mu);
+  s->arena.reset();
+  const uint32_t tiles = filter_tiles(cap);
+  const uint32_t groups = (uint32_t)ceil_div(tiles, 1u << kGroupShift);
+  FilterScratch sc;
+  MTL_TRY(s->arena.alloc((size_t)tiles * FTILE / 8, &sc.bm));
+  MTL_TRY(s->arena.alloc((size_t)tiles * 4, &sc.counts));
+  MTL_TRY(s->arena.alloc(256, &sc.ctr));
+  MTL_TRY(s->arena.alloc((size_t)groups * kGroupStride * 4, &sc.gsum));
+  uint64_t* fpg = nullptr;
+  if (ex && ex->fingerprint) {
+    void* p;
+    MTL_TRY(s->arena.alloc((size_t)groups * kFpStride * 8, &p));
+    fpg = (uint64_t*)p;
+    ex->fp_groups = fpg;
+    ex->fp_n = groups;
+  }
+  return filter_enqueue(preds, n_preds, cap, in_sel, in_count, out_sel, out_count, proj, ex,
+                        sc, fpg, tiles, groups, true, s);
+}
+
+}  // extern "C"
+
+namespace mtl {
+
+// ---------------------------------------------------------------------------
+// The filter without allocation (mtl_filter_noalloc): its pass bits, tile
+// counts, clearing words and group sums live in this file-scope device
+// array, one per GPU the library runs on, held until the process ends.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
+constexpr uint64_t kNoAllocMaxRows = 100000000ull;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The device address of the current GPU's `scratch`, looked up once per
+// device (in host code the array's name is not a device address).
+static int32_t noalloc_scratch(void** out) {
+  static std::mutex mu;
+  static void* ptr[64] = {};
+  int dev = 0;
+  MTL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_error(MTL_ERR_DEVICE_UNAVAILABLE, "device %d", dev);
+  std::lock_guard<std::mutex> g(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(MTL_ERR_PROCESSING, "processing error: hipGetSymbolAddress(scratch): %s",
+                       hipGetErrorString(e));
+    ptr[dev] = p;
+  }
+  *out = ptr[dev];
+  return MTL_OK;
+}
+
+}  // namespace mtl
+
+extern "C" {
+
+int32_t mtl_filter_noalloc(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
+                           const uint32_t* in_sel, const uint32_t* in_count,
+                           uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
+                           mtl_stream* s) {
+  uint64_t cap;
+  MTL_TRY(filter_check(preds, n_preds, n_rows, in_sel, in_capacity, out_sel, out_count, &cap));
+  if (cap > kNoAllocMaxRows)
+    return set_error(MTL_ERR_INVALID_ARGUMENT,
+                     "mtl_filter_noalloc takes at most %llu rows (%llu given): use mtl_filter",
+                     (unsigned long long)kNoAllocMaxRows, (unsigned long long)cap);
+  if (!s) {
+    s = runtime().default_stream;  // creating it would allocate
+    if (!s)
+      return set_error(MTL_ERR_NOT_INITIALIZED,
+                       "mtl_filter_noalloc: pass a stream, or call mtl_init first");
+  }
+  void* base;
+  MTL_TRY(noalloc_scratch(&base));
+  const uint32_t tiles = filter_tiles(cap);
+  const uint32_t groups = (uint32_t)ceil_div(tiles, 1u << kGroupShift);
+  auto up = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+  FilterScratch sc;
+  char* at = (char*)base;
+  sc.bm = at;
+  at += up((uint64_t)tiles * FTILE / 8);
+  sc.counts = at;
+  at += up((uint64_t)tiles * 4);
+  sc.ctr = at;
+  at += 256;
+  sc.gsum = at;
+  at += up((uint64_t)groups * kGroupStride * 4);
+  if ((uint64_t)(at - (char*)base) > kNoAllocScratchBytes)
+    return set_error(MTL_ERR_INVALID_ARGUMENT, "mtl_filter_noalloc: %llu rows exceed the scratch",
+                     (unsigned long long)cap);
+  std::lock_guard<std::mutex> g(s->mu);
+  return filter_enqueue(preds, n_preds, cap, in_sel, in_count, out_sel, out_count, nullptr,
+                        nullptr, sc, nullptr, tiles, groups, false, s);
 }
 
 int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,

@@ -24,7 +24,9 @@ void prof_collect(mtl_stream* s);
 struct ProfScope {
   mtl_stream* s;
   bool on;
-  ProfScope(mtl_stream* s_, const char* name) : s(s_), on(prof_enabled()) {
+  // allow = false: never record (a path that must not create events)
+  ProfScope(mtl_stream* s_, const char* name, bool allow = true)
+      : s(s_), on(allow && prof_enabled()) {
     if (on) prof_begin(s, name);
   }
   ~ProfScope() {
