@@ -30,7 +30,6 @@
 #include <algorithm>
 #include <cmath>
 
-#include <rocprim/device/device_radix_sort.hpp>
 #include <cstring>
 #include <vector>
 
@@ -46,6 +45,7 @@ using namespace dev;
 constexpr uint64_t GEMPTY = ~0ull;
 constexpr uint64_t kDenseMax = 1ull << 26;  // directly indexed packed domain
 constexpr int kMaxW = 1 + 2 * MTL_GROUP_MAX_AGGS;
+constexpr int kMaxPlanes = 1 + 2 * MTL_GROUP_MAX_AGGS;
 
 enum Kind : uint32_t { K_COUNT = 0, K_SUMI, K_SUMF, K_MINI, K_MAXI, K_MINF, K_MAXF };
 
@@ -76,6 +76,11 @@ struct GroupArgs {
   uint64_t key_stride[MTL_GROUP_MAX_KEYS];
   uint32_t n_aggs;
   AggSlot aggs[MTL_GROUP_MAX_AGGS];
+  // the state updates of a row besides its count: one per aggregate that
+  // owns state words (not COUNT, not a duplicate), packed as kind | word << 4
+  // | source << 12
+  uint32_t n_ops;
+  uint32_t ops[MTL_GROUP_MAX_AGGS];
   // distinct (column, conversion) value sources of the aggregates: each is
   // loaded once per row (SUM, MIN and MAX of one column share one load)
   uint32_t n_src;
@@ -102,10 +107,23 @@ struct GroupArgs {
   // arrays (region of block b starts at row b * per) instead of updating
   // the global table with device-scope atomics
   uint32_t spill;
-  uint32_t* spill_key;   // [rows]
-  uint64_t* spill_val;   // [n_src][rows], source-major
-  uint32_t* spill_n;     // [grid]: spilled rows of each block
+  // spilled rows are 32-bit planes: plane 0 the packed key, then each
+  // source's value in one plane (a source of <= 4 bytes, narrowed exactly,
+  // see narrow_src) or two (low and high halves); plane_hi == plane_lo marks
+  // a narrowed source, and sources past n_src name plane 0
+  uint32_t n_planes;
+  uint32_t plane_lo[MTL_GROUP_MAX_AGGS];
+  uint32_t plane_hi[MTL_GROUP_MAX_AGGS];
+  uint32_t* spill_pl[kMaxPlanes];  // [rows] each
+  uint32_t* spill_n;     // [grid + 1]: spilled rows of each block, then the
+                         // region length (rows per block)
   uint64_t spill_rows;   // row capacity of the spill arrays
+  // spill mode: each block's LDS cache is staged here at the end (slot i of
+  // block b at b * cache_slots + i: its key, GEMPTY when unused, and its W
+  // state words) and merged into the table after the spilled rows are
+  // aggregated, since that aggregation writes whole table entries
+  uint64_t* hot_key;
+  uint64_t* hot_st;
 };
 
 __device__ __forceinline__ uint64_t gfmix(uint64_t k) {
@@ -152,6 +170,18 @@ __device__ __forceinline__ void load_bits(const void* col, uint32_t width,
   }
 }
 
+// Register type holding a raw value of a WIDTH-byte column in flight (a
+// 4-byte value is widened only when used, so no copy waits for its load).
+template <int WIDTH> struct RawT { using type = uint64_t; };
+template <> struct RawT<4> { using type = uint32_t; };
+template <int GE_, int WIDTH>
+__device__ __forceinline__ void load_raw(const void* col, const uint64_t (&row)[GE_],
+                                         typename RawT<WIDTH>::type (&b)[GE_]) {
+  using T = typename RawT<WIDTH>::type;
+#pragma unroll
+  for (int e = 0; e < GE_; ++e) b[e] = ((const T*)col)[row[e]];
+}
+
 // The int64 value of the raw bits of an integer column of type t.
 __device__ __forceinline__ int64_t bits_key(uint64_t b, int32_t t) {
   switch (t) {
@@ -183,6 +213,27 @@ __device__ __forceinline__ uint64_t src_bits(const GroupArgs& a, uint32_t j, uin
     case SC_INT: return (uint64_t)bits_key(b, a.src_type[j]);
     case SC_DBL: return (uint64_t)__double_as_longlong(bits_f64(b, a.src_type[j]));
     default: return enc_f64(bits_f64(b, a.src_type[j]));
+  }
+}
+
+// A converted source value of a column of <= 4 bytes in 32 bits, exactly:
+// an integer value fits int32, a FLOAT's double (or its order-preserving
+// encoding) is a float.  widen_src inverts it.
+__device__ __forceinline__ uint32_t narrow_src(const GroupArgs& a, uint32_t j, uint64_t x) {
+  switch (a.src_conv[j]) {
+    case SC_INT: return (uint32_t)x;
+    case SC_DBL: return __float_as_uint((float)__longlong_as_double((long long)x));
+    default: {  // SC_ENCF: decode the order-preserving encoding first
+      const uint64_t b = (x & 0x8000000000000000ull) ? (x & ~0x8000000000000000ull) : ~x;
+      return __float_as_uint((float)__longlong_as_double((long long)b));
+    }
+  }
+}
+__device__ __forceinline__ uint64_t widen_src(const GroupArgs& a, uint32_t j, uint32_t u) {
+  switch (a.src_conv[j]) {
+    case SC_INT: return (uint64_t)(int64_t)(int32_t)u;
+    case SC_DBL: return (uint64_t)__double_as_longlong((double)__uint_as_float(u));
+    default: return enc_f64((double)__uint_as_float(u));
   }
 }
 
@@ -218,6 +269,13 @@ __device__ __forceinline__ uint64_t pick_src(const uint64_t (&v)[NS], uint32_t s
   return r;
 }
 
+// A possibly stale read of an LDS state word.  A relaxed workgroup-scope
+// atomic load becomes an LDS read; a volatile read would stay a generic
+// (flat) load, which also waits for every global load in flight.
+__device__ __forceinline__ uint64_t lds_peek(uint64_t* w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Per-row update of one group's state words (LDS or global: atomics with
 // the address space of p; none returns a value).  Word k of the group is
 // at p[k * ws]: global states are contiguous (ws = 1), LDS states are
@@ -246,18 +304,16 @@ __device__ __forceinline__ void update_vals(const GroupArgs& a, uint64_t* p, uin
       }
       case K_SUMF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)v)); break;
       case K_MINI:
-        if (!LDS || (long long)v < *(volatile long long*)w) atomicMin((long long*)w, (long long)v);
+        if (!LDS || (long long)v < (long long)lds_peek(w)) atomicMin((long long*)w, (long long)v);
         break;
       case K_MAXI:
-        if (!LDS || (long long)v > *(volatile long long*)w) atomicMax((long long*)w, (long long)v);
+        if (!LDS || (long long)v > (long long)lds_peek(w)) atomicMax((long long*)w, (long long)v);
         break;
       case K_MINF:
-        if (!LDS || v < *(volatile uint64_t*)w)
-          atomicMin((unsigned long long*)w, (unsigned long long)v);
+        if (!LDS || v < lds_peek(w)) atomicMin((unsigned long long*)w, (unsigned long long)v);
         break;
       case K_MAXF:
-        if (!LDS || v > *(volatile uint64_t*)w)
-          atomicMax((unsigned long long*)w, (unsigned long long)v);
+        if (!LDS || v > lds_peek(w)) atomicMax((unsigned long long*)w, (unsigned long long)v);
         break;
       default: break;
     }
@@ -315,6 +371,84 @@ __global__ void k_group_init(GroupArgs a, uint64_t n_groups) {
   }
 }
 
+// The LDS cache updates of a round of GE rows, aggregate-major: the kind of
+// each update is decided once per round, not once per row.  Word k of cache
+// slot i is at cst[k * ws + i]; rows with slot < 0 are skipped, or, with
+// SPARE, sent to the spare column ws - 1 (never read back), which avoids a
+// branch per row where skipped rows are rare.
+template <int NS, int GE_, bool SPARE>
+__device__ __forceinline__ void cache_update(const GroupArgs& a, uint64_t* cst, uint32_t ws,
+                                             const int (&slot)[GE_],
+                                             const uint64_t (&v)[GE_][NS]) {
+  int sl[GE_];
+  bool on[GE_];
+#pragma unroll
+  for (int e = 0; e < GE_; ++e) {
+    on[e] = SPARE || slot[e] >= 0;
+    sl[e] = slot[e] >= 0 ? slot[e] : (int)ws - 1;
+  }
+#pragma unroll
+  for (int e = 0; e < GE_; ++e)
+    if (on[e]) atomicAdd((unsigned long long*)&cst[sl[e]], 1ull);
+#pragma unroll
+  for (uint32_t o = 0; o < MTL_GROUP_MAX_AGGS; ++o) {
+    if (o >= a.n_ops) break;
+    const uint32_t op = a.ops[o], kind = op & 15u, src = op >> 12;
+    uint64_t* w = cst + (uint64_t)((op >> 4) & 255u) * ws;
+    switch (kind) {
+      case K_SUMI:
+#pragma unroll
+        for (int e = 0; e < GE_; ++e) {
+          int64_t lo, hi;
+          split_i64((int64_t)pick_src<NS>(v[e], src), lo, hi);
+          if (on[e]) {
+            atomicAdd((unsigned long long*)&w[sl[e]], (unsigned long long)lo);
+            if (hi) atomicAdd((unsigned long long*)&w[ws + sl[e]], (unsigned long long)hi);
+          }
+        }
+        break;
+      case K_SUMF:
+#pragma unroll
+        for (int e = 0; e < GE_; ++e)
+          if (on[e])
+            unsafeAtomicAdd((double*)&w[sl[e]],
+                            __longlong_as_double((long long)pick_src<NS>(v[e], src)));
+        break;
+      case K_MINI:
+#pragma unroll
+        for (int e = 0; e < GE_; ++e) {
+          const long long x = (long long)pick_src<NS>(v[e], src);
+          if (on[e] && x < (long long)lds_peek(&w[sl[e]])) atomicMin((long long*)&w[sl[e]], x);
+        }
+        break;
+      case K_MAXI:
+#pragma unroll
+        for (int e = 0; e < GE_; ++e) {
+          const long long x = (long long)pick_src<NS>(v[e], src);
+          if (on[e] && x > (long long)lds_peek(&w[sl[e]])) atomicMax((long long*)&w[sl[e]], x);
+        }
+        break;
+      case K_MINF:
+#pragma unroll
+        for (int e = 0; e < GE_; ++e) {
+          const uint64_t x = pick_src<NS>(v[e], src);
+          if (on[e] && x < lds_peek(&w[sl[e]]))
+            atomicMin((unsigned long long*)&w[sl[e]], (unsigned long long)x);
+        }
+        break;
+      case K_MAXF:
+#pragma unroll
+        for (int e = 0; e < GE_; ++e) {
+          const uint64_t x = pick_src<NS>(v[e], src);
+          if (on[e] && x > lds_peek(&w[sl[e]]))
+            atomicMax((unsigned long long*)&w[sl[e]], (unsigned long long)x);
+        }
+        break;
+      default: break;
+    }
+  }
+}
+
 // 12 waves per block: two blocks (and their LDS caches) per CU at 6 waves
 // per SIMD, within the SGPR-bound limit of 7.
 constexpr int kAggBlock = 768;
@@ -323,41 +457,89 @@ constexpr int GE = 4;  // rows per thread per round, loaded together
 
 // NS: compile-time bound on the number of value sources (registers for the
 // values of GE rows are sized by it, not by the aggregate count).
-template <int NS>
-__global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
+// Specialised variants fix the shape at compile time: KW != 0 means no
+// selection vector and one packed key column of KW bytes, SW0 / SW1 the
+// byte widths of the two sources, MODE 1 the directly indexed cache and
+// MODE 2 the hashed cache with spilling (0: all of it read from the
+// arguments).  With no runtime branch around the column loads, the loads of
+// a round go out back to back and stay in flight across the updates.
+template <int NS, int KW = 0, int SW0 = 0, int SW1 = 0, int MODE = 0>
+__global__ __launch_bounds__(kAggBlock, NS <= 4 ? 6 : 4) void k_group_agg(GroupArgs a) {
+  static_assert(KW == 0 || NS == 2, "specialised variants take two sources");
+  const uint32_t* sel = KW ? nullptr : a.sel;
+  const bool direct = MODE ? MODE == 1 : a.cache_direct != 0;
+  const bool spilling = MODE ? MODE == 2 : a.spill != 0;
   extern __shared__ uint64_t smem[];
   __shared__ uint32_t s_spill;
   const uint32_t cs = a.cache_slots;
+  const uint32_t ws = cs + 1;  // word stride of the cache states: cs slots and a spare
   uint64_t* ckeys = smem;
   uint64_t* cst = smem + cs;
-  uint32_t* door = (uint32_t*)(cst + (uint64_t)cs * a.W);  // kDoorBits bits when a.door
+  uint32_t* door = (uint32_t*)(cst + (uint64_t)ws * a.W);  // kDoorBits bits when a.door
   if (threadIdx.x == 0) s_spill = 0;
   if (a.door)
     for (uint32_t i = threadIdx.x; i < kDoorBits / 32; i += blockDim.x) door[i] = 0;
-  for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
-    ckeys[i] = a.cache_direct ? (uint64_t)i : GEMPTY;
-    for (uint32_t w = 0; w < a.W; ++w) cst[(uint64_t)w * cs + i] = a.init[w];
+  for (uint32_t i = threadIdx.x; i < ws; i += blockDim.x) {
+    if (i < cs) ckeys[i] = direct ? (uint64_t)i : GEMPTY;
+    for (uint32_t w = 0; w < a.W; ++w) cst[(uint64_t)w * ws + i] = a.init[w];
   }
   __syncthreads();
-  const uint64_t n = a.sel ? dev_count(a.sel_count, a.sel_cap) : a.n_rows;
+  const uint64_t n = sel ? dev_count(a.sel_count, a.sel_cap) : a.n_rows;
   const uint64_t per = (n + gridDim.x - 1) / gridDim.x;
   const uint64_t start = (uint64_t)blockIdx.x * per;
   const uint64_t end = start + per < n ? start + per : n;
-  for (uint64_t base = start; base < end; base += (uint64_t)blockDim.x * GE) {
+  const uint64_t step = (uint64_t)blockDim.x * GE;
+  // software pipeline: the raw bits of the first key column and of every
+  // source for the next round are loaded while this round updates the
+  // cache, so the column loads of a wave stay in flight behind its atomics
+  // (rows past the end read row 0)
+  uint64_t nkb[GE], nvb[NS][GE];  // general variants
+  typename RawT<KW>::type fk[GE];  // specialised variants
+  typename RawT<SW0>::type fv0[GE];
+  typename RawT<SW1>::type fv1[GE];
+  auto fetch = [&](uint64_t b0) {
+    uint64_t r[GE];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      const uint64_t k = b0 + (uint64_t)e * blockDim.x + threadIdx.x;
+      r[e] = k < end ? (sel ? (uint64_t)sel[k] : k) : 0;
+    }
+    if constexpr (KW != 0) {
+      load_raw<GE, KW>(a.key_col[0], r, fk);
+      load_raw<GE, SW0>(a.src_col[0], r, fv0);
+      load_raw<GE, SW1>(a.src_col[1], r, fv1);
+    } else {
+      load_bits<GE>(a.key_col[0], a.key_w0, r, nkb);
+#pragma unroll
+      for (uint32_t j = 0; j < (uint32_t)NS; ++j)
+        load_bits<GE>(a.src_col[j], a.src_w[j], r, nvb[j]);
+    }
+  };
+  // with more sources the registers of a second round in flight would cost
+  // occupancy: those variants load at the top of each round
+  constexpr bool kPipe = NS <= 2;
+  if (kPipe && start < end) fetch(start);
+  for (uint64_t base = start; base < end; base += step) {
     uint64_t row[GE], key[GE];
     bool ok[GE];
+    uint64_t kb[GE], vb[NS][GE];
+    if (!kPipe) fetch(base);
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
       const uint64_t k = base + (uint64_t)e * blockDim.x + threadIdx.x;
       ok[e] = k < end;
-      row[e] = ok[e] ? (a.sel ? (uint64_t)a.sel[k] : k) : 0;
-    }
-    // raw bits of the first key column and of every source for all GE rows
-    // in flight together (rows past the end read row 0), then conversions
-    uint64_t kb[GE], vb[NS][GE];
-    load_bits<GE>(a.key_col[0], a.key_w0, row, kb);
+      row[e] = ok[e] ? (sel ? (uint64_t)sel[k] : k) : 0;
+      if constexpr (KW != 0) {
+        kb[e] = fk[e];
+        vb[0][e] = fv0[e];
+        vb[NS - 1][e] = fv1[e];
+      } else {
+        kb[e] = nkb[e];
 #pragma unroll
-    for (uint32_t j = 0; j < (uint32_t)NS; ++j) load_bits<GE>(a.src_col[j], a.src_w[j], row, vb[j]);
+        for (uint32_t j = 0; j < (uint32_t)NS; ++j) vb[j][e] = nvb[j][e];
+      }
+    }
+    if (kPipe && base + step < end) fetch(base + step);
     uint64_t v[GE][NS];
 #pragma unroll
     for (int e = 0; e < GE; ++e)
@@ -367,17 +549,27 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
       key[e] = 0;
-      if (ok[e] && !pack_key_bits(a, row[e], kb[e], key[e])) {
+      bool in_range;
+      if constexpr (KW != 0) {
+        // one packed key of KW bytes: its offset from the minimum
+        const int64_t kv = KW == 8 ? (int64_t)kb[e] : (int64_t)(int32_t)(uint32_t)kb[e];
+        key[e] = key_off(kv, a.key_min[0]);
+        in_range = key[e] < a.key_span[0];
+      } else {
+        in_range = pack_key_bits(a, row[e], kb[e], key[e]);
+      }
+      if (ok[e] && !in_range) {
         atomicOr(a.err, 8u);
         ok[e] = false;
       }
     }
+    int slot[GE];
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
-      int slot = -1;
+      slot[e] = -1;
       if (!ok[e]) {
-      } else if (a.cache_direct) {
-        slot = (int)key[e];  // the whole (dense) domain is cached
+      } else if (direct) {
+        slot[e] = (int)key[e];  // the whole (dense) domain is cached
       } else if (cs && key[e] != GEMPTY) {
         const uint64_t hm = gfmix(key[e]);
         uint32_t h = (uint32_t)hm & (cs - 1);
@@ -387,7 +579,7 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
           uint64_t ck = __hip_atomic_load(&ckeys[si], __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_WORKGROUP);
           if (ck == key[e]) {
-            slot = (int)si;
+            slot[e] = (int)si;
             break;
           }
           if (ck == GEMPTY) {
@@ -402,15 +594,19 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
             unsigned long long old = atomicCAS((unsigned long long*)&ckeys[si], GEMPTY,
                                                (unsigned long long)key[e]);
             if (old == GEMPTY || old == key[e]) {
-              slot = (int)si;
+              slot[e] = (int)si;
               break;
             }
           }
         }
       }
-      if (ok[e] && slot >= 0) update_vals<true, NS>(a, cst + slot, cs, v[e]);
-      const bool miss = ok[e] && slot < 0;
-      if (a.spill) {
+    }
+    cache_update<NS, GE, MODE == 1>(a, cst, ws, slot, v);
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      if constexpr (MODE == 1) break;  // every row in range is cached
+      const bool miss = ok[e] && slot[e] < 0;
+      if (spilling) {
         // every lane of the wave is here: one LDS atomic per wave reserves
         // the wave's spill entries, in lane order
         const uint64_t mb = __ballot(miss);
@@ -421,179 +617,581 @@ __global__ __launch_bounds__(kAggBlock) void k_group_agg(GroupArgs a) {
           at = __shfl(at, leader, WAVE) + (uint32_t)__popcll(mb & lanemask_lt());
           if (miss) {
             const uint64_t o = start + at;
-            a.spill_key[o] = (uint32_t)key[e];
+            a.spill_pl[0][o] = (uint32_t)key[e];
 #pragma unroll
-            for (uint32_t j = 0; j < (uint32_t)NS; ++j)
-              if (j < a.n_src) a.spill_val[j * a.spill_rows + o] = v[e][j];
+            for (uint32_t j = 0; j < (uint32_t)NS; ++j) {
+              if (j >= a.n_src) break;
+              const bool wide = KW ? (j == 0 ? SW0 : SW1) == 8 : a.plane_hi[j] != a.plane_lo[j];
+              if (wide) {
+                a.spill_pl[a.plane_lo[j]][o] = (uint32_t)v[e][j];
+                a.spill_pl[a.plane_hi[j]][o] = (uint32_t)(v[e][j] >> 32);
+              } else {
+                a.spill_pl[a.plane_lo[j]][o] = narrow_src(a, j, v[e][j]);
+              }
+            }
           }
         }
-      } else if (miss) {
+      } else if (MODE != 1 && miss) {
         uint64_t g = global_slot(a, key[e]);
         if (g != GEMPTY) update_vals<false, NS>(a, a.st + g * a.W, 1, v[e]);
       }
     }
   }
   __syncthreads();
-  if (a.spill && threadIdx.x == 0) a.spill_n[blockIdx.x] = s_spill;
+  if (spilling) {
+    if (threadIdx.x == 0) a.spill_n[blockIdx.x] = s_spill;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.spill_n[gridDim.x] = (uint32_t)per;
+    const uint64_t h0 = (uint64_t)blockIdx.x * cs;
+    for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
+      const uint64_t key = ckeys[i];
+      a.hot_key[h0 + i] = (key == GEMPTY || cst[i] == 0) ? GEMPTY : key;
+    }
+    // state words slot-major, consecutive threads on consecutive words
+    for (uint32_t t = threadIdx.x; t < cs * a.W; t += blockDim.x)
+      a.hot_st[h0 * a.W + t] = cst[(uint64_t)(t % a.W) * ws + t / a.W];
+    return;
+  }
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
     uint64_t key = ckeys[i];
     if (key == GEMPTY || cst[i] == 0) continue;  // word 0: the row count
     uint64_t g = global_slot(a, key);
-    if (g != GEMPTY) merge_state(a, a.st + g * a.W, cst + i, cs);
+    if (g != GEMPTY) merge_state(a, a.st + g * a.W, cst + i, ws);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Spilled rows (cache misses of a large dense domain): compacted, sorted by
-// group with rocPRIM's radix sort, then each run of one group is reduced in
-// registers and added to the global state (the LDS flushes of k_group_agg
-// are complete by then).  This replaces one device-scope atomic per
-// aggregate per missed row, the cost of the long tail of a skewed key
-// distribution.
+// Spilled rows (cache misses of a large dense domain) are partitioned by
+// packed key into ranges of 2^ps keys (partition p = key >> ps), most
+// significant digits first, at most 7 bits of the partition number per
+// scatter level; then one block per partition aggregates its rows in LDS
+// and writes the partition's table entries whole (no table clearing pass,
+// no device-scope atomics per row).  The staged LDS caches of k_group_agg
+// are merged in last.  Records are structure-of-arrays: the u32 packed key
+// and the 64-bit value sources (source j of record i at val[j * stride + i]).
 // ---------------------------------------------------------------------------
-template <int NS>
-struct SpillPay {
-  uint64_t v[NS];
+constexpr int kPartThreads = 1024;       // partition counting and scan blocks
+constexpr int kScatThreads = 512;        // scatter blocks
+constexpr int kPartChunk = 2048;         // rows a scatter block stages in LDS at a time
+constexpr uint32_t kPartTile = 8192;     // rows per scatter tile (one block)
+constexpr uint32_t kPartHistLds = 16384;  // partitions counted in LDS
+constexpr uint64_t kPartAggLds = 64 * 1024;  // LDS state of one partition
+constexpr int kPartAggThreads = 512;
+constexpr uint32_t kPartAggRows = 65536;  // rows per aggregation block
+constexpr int kPartLevelBits = 7;
+constexpr uint32_t kMaxFan = 1u << kPartLevelBits;  // buckets per scatter level
+
+// Exclusive scan of one value per thread over a block of 64-lane waves;
+// s_w holds blockDim.x / 64 words.  Every thread of the block calls it.
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* s_w) {
+  const uint32_t lane = lane_id(), wave = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, d, WAVE);
+    if ((int)lane >= d) x += y;
+  }
+  if (lane == WAVE - 1) s_w[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t t = lane < nw ? s_w[lane] : 0u, u = t;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)u, d, WAVE);
+      if ((int)lane >= d) u += y;
+    }
+    if (lane < nw) s_w[lane] = u - t;
+  }
+  __syncthreads();
+  const uint32_t r = x - v + s_w[wave];
+  __syncthreads();  // s_w may be reused by the caller
+  return r;
+}
+
+// The lanes of the wave whose row has the same digit as this lane's
+// (digits below 2^bits; lanes without a row match none): one ballot per
+// digit bit.  Lets one lane per distinct digit stand for all of them, so a
+// skewed digit costs one LDS atomic per wave, not one per lane.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, uint32_t bits) {
+  uint64_t m = __ballot(valid);
+  for (uint32_t k = 0; k < bits; ++k) {
+    const bool bit = (d >> k) & 1u;
+    const uint64_t b = __ballot(valid && bit);
+    m &= bit ? b : ~b;
+  }
+  return valid ? m : 0ull;
+}
+
+// This lane's rank among the wave's rows with its digit, after the rows of
+// that digit counted in h before (h[d] grows by the wave's count).
+__device__ __forceinline__ uint32_t wave_rank(uint32_t* h, uint32_t d, bool valid,
+                                              uint32_t bits) {
+  const uint64_t m = match_digit(d, valid, bits);
+  const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
+  uint32_t old = 0;
+  if (valid && lane_id() == leader) old = atomicAdd(&h[d], (uint32_t)__popcll(m));
+  old = (uint32_t)__shfl((int)old, leader, WAVE);
+  return old + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+// Rows of each partition: one block per spill region (the region of block r
+// of k_group_agg starts at r * seg_n[grid] and holds seg_n[r] rows); counted in
+// LDS when the partitions fit, else with device atomics per row.
+__global__ __launch_bounds__(kPartThreads) void k_part_hist(const uint32_t* key,
+                                                            const uint32_t* seg_n, uint32_t ps,
+                                                            uint32_t np, uint32_t* count) {
+  extern __shared__ uint32_t h[];
+  const bool lds = np <= kPartHistLds;
+  if (lds)
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint64_t r0 = (uint64_t)blockIdx.x * seg_n[gridDim.x];
+  const uint32_t n = seg_n[blockIdx.x];
+  for (uint32_t i0 = 0; i0 < n; i0 += 4 * blockDim.x) {
+    uint32_t k[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t i = i0 + e * blockDim.x + threadIdx.x;
+      k[e] = i < n ? key[r0 + i] : 0u;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (i0 + e * blockDim.x + threadIdx.x >= n) continue;
+      const uint32_t p = k[e] >> ps;
+      if (lds) atomicAdd(&h[p], 1u);
+      else atomicAdd(&count[p], 1u);
+    }
+  }
+  __syncthreads();
+  if (lds)
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x)
+      if (h[i]) atomicAdd(&count[i], h[i]);
+}
+
+// off[p] = rows of the partitions before p, off[np] = all rows; coff the
+// same over the aggregation blocks of each partition (one per kPartAggRows
+// rows, at least one).  One block.
+__device__ __forceinline__ uint32_t part_chunks(uint32_t c) {
+  return c ? (c + kPartAggRows - 1) / kPartAggRows : 1u;
+}
+__global__ __launch_bounds__(kPartThreads) void k_part_scan(const uint32_t* count, uint32_t np,
+                                                            uint32_t* off, uint32_t* coff) {
+  __shared__ uint32_t s_w[kPartThreads / WAVE];
+  const uint32_t per = (np + blockDim.x - 1) / blockDim.x;
+  const uint32_t b = threadIdx.x * per, e = b + per < np ? b + per : np;
+  uint32_t sum = 0, chunks = 0;
+  for (uint32_t i = b; i < e; ++i) {
+    sum += count[i];
+    chunks += part_chunks(count[i]);
+  }
+  uint32_t run = block_scan_excl(sum, s_w);
+  uint32_t crun = block_scan_excl(chunks, s_w);
+  for (uint32_t i = b; i < e; ++i) {
+    off[i] = run;
+    coff[i] = crun;
+    run += count[i];
+    crun += part_chunks(count[i]);
+  }
+  if (threadIdx.x == blockDim.x - 1) {
+    off[np] = run;
+    coff[np] = crun;
+  }
+}
+
+// One scatter level moves every row to the output region of its bucket
+// (key >> shift; bucket b's region starts at part_off[b << ushift]).  The
+// input is cut into tiles of at most kPartTile rows: on the first level the
+// tiles of each spill region (tile r * tiles_per_seg + c is rows
+// [c * kPartTile, ...) of region r), on later levels the tiles of each
+// coarse bucket's rows, which the previous level made contiguous (tiles
+// tile_first[s] .. tile_first[s + 1] - 1 of coarse bucket s).  A level is
+// three launches: the digit counts of every tile (k_part_count), their
+// scan into per-tile output offsets in tile order (k_part_offs), and the
+// scatter (k_part_scatter), which carries each digit's cursor in LDS and
+// needs no device atomics, however skewed the keys.  A tile's rows keep
+// their order within each digit only up to the LDS ranking: order within
+// a bucket is not kept.
+struct PartLevel {
+  const uint32_t* in[kMaxPlanes];  // plane 0: the packed key
+  uint32_t* out[kMaxPlanes];
+  uint32_t n_planes;
+  uint32_t first;           // first level: tiles of the spill regions
+  const uint32_t* seg_n;    // first level: rows of each region; seg_n[n_seg]: region length
+  uint32_t n_seg, tiles_per_seg;
+  const uint32_t* tile_first;  // later levels: [n_groups + 1]
+  uint32_t n_groups;           // later levels: coarse buckets
+  const uint32_t* part_off;
+  uint32_t np;
+  uint32_t* offs;   // [tile][fan]: digit counts, then the output offsets
+  uint32_t shift;   // bucket = key >> shift
+  uint32_t ushift;  // bucket b's first partition = b << ushift
+  uint32_t uprev;   // later levels: coarse bucket s's first partition = s << uprev
+  uint32_t fan, bits;  // digits of this level (first level: the buckets), bits to tell them apart
+  uint32_t nb;         // buckets of this level
 };
 
-// One block per k_group_agg block: its spilled rows go to [base, base + n)
-// of the compacted arrays, base = the spilled rows of the blocks before it.
-template <int NS>
-__global__ __launch_bounds__(256) void k_spill_compact(GroupArgs a, uint32_t agg_grid,
-                                                       uint32_t* ckey, SpillPay<NS>* cpay,
-                                                       uint32_t* total) {
-  __shared__ uint32_t s_red[4];
-  const uint32_t b = blockIdx.x, tid = threadIdx.x;
-  uint32_t part = 0;
-  for (uint32_t i = tid; i < b; i += 256) part += a.spill_n[i];
-  part = wave_sum_u32(part);
-  if (tid % WAVE == 0) s_red[tid / WAVE] = part;
+__device__ __forceinline__ uint32_t part_digit(const PartLevel& a, uint32_t key) {
+  const uint32_t b = key >> a.shift;
+  return a.first ? b : b & (a.fan - 1);
+}
+
+// Rows [*r0, *r0 + *n) and scan group *g (first level: 0; later: the coarse
+// bucket) of tile t; false for an empty or nonexistent tile.
+__device__ __forceinline__ bool part_tile(const PartLevel& a, uint32_t t, uint64_t* r0,
+                                          uint32_t* n, uint32_t* g) {
+  if (a.first) {
+    const uint32_t r = t / a.tiles_per_seg, c = t % a.tiles_per_seg;
+    if (r >= a.n_seg) return false;
+    const uint32_t rows = a.seg_n[r], f = c * kPartTile;
+    if (f >= rows) return false;
+    *r0 = (uint64_t)r * a.seg_n[a.n_seg] + f;
+    *n = rows - f < kPartTile ? rows - f : kPartTile;
+    *g = 0;
+    return true;
+  }
+  if (t >= a.tile_first[a.n_groups]) return false;
+  uint32_t lo = 0, hi = a.n_groups - 1;  // last s with tile_first[s] <= t
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (a.tile_first[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t s0 = a.part_off[lo << a.uprev];
+  const uint64_t pe = (uint64_t)(lo + 1) << a.uprev;
+  const uint32_t s1 = a.part_off[pe < a.np ? pe : a.np];
+  const uint32_t f = (t - a.tile_first[lo]) * kPartTile;
+  *r0 = (uint64_t)s0 + f;
+  *n = s1 - s0 - f < kPartTile ? s1 - s0 - f : kPartTile;
+  *g = lo;
+  return true;
+}
+
+// Tiles of each coarse bucket of a later level (one block):
+// tile_first[s] = tiles of the buckets before s, tile_first[n] = all.
+__global__ __launch_bounds__(kPartThreads) void k_part_tiles(const uint32_t* part_off,
+                                                             uint32_t np, uint32_t uprev,
+                                                             uint32_t n, uint32_t* tile_first) {
+  __shared__ uint32_t s_w[kPartThreads / WAVE];
+  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
+  auto tiles = [&](uint32_t s) {
+    const uint64_t p1 = (uint64_t)(s + 1) << uprev;
+    const uint32_t len = part_off[p1 < np ? p1 : np] - part_off[s << uprev];
+    return (len + kPartTile - 1) / kPartTile;
+  };
+  uint32_t sum = 0;
+  for (uint32_t i = b; i < e; ++i) sum += tiles(i);
+  uint32_t run = block_scan_excl(sum, s_w);
+  for (uint32_t i = b; i < e; ++i) {
+    tile_first[i] = run;
+    run += tiles(i);
+  }
+  if (threadIdx.x == blockDim.x - 1) tile_first[n] = run;
+}
+
+// Digit counts of each tile (zeros for an empty first-level tile, which the
+// first level's scan runs over).
+__global__ __launch_bounds__(256) void k_part_count(PartLevel a) {
+  __shared__ uint32_t h[kMaxFan];
+  __shared__ uint64_t s_r0;
+  __shared__ uint32_t s_n, s_ok;
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  if (tid < a.fan) h[tid] = 0;
+  if (tid == 0) {
+    uint64_t r0 = 0;
+    uint32_t n = 0, g;
+    s_ok = part_tile(a, t, &r0, &n, &g) ? 1u : 0u;
+    s_r0 = r0;
+    s_n = n;
+  }
   __syncthreads();
-  const uint32_t base = s_red[0] + s_red[1] + s_red[2] + s_red[3];
-  const uint32_t cnt = a.spill_n[b];
-  const uint64_t n = a.sel ? dev_count(a.sel_count, a.sel_cap) : a.n_rows;
-  const uint64_t region = (uint64_t)b * ((n + agg_grid - 1) / agg_grid);
-  for (uint32_t i = tid; i < cnt; i += 256) {
-    ckey[base + i] = a.spill_key[region + i];
-    SpillPay<NS> p;
+  if (!s_ok && !a.first) return;
+  const uint64_t r0 = s_r0;
+  const uint32_t n = s_ok ? s_n : 0u;
+  const uint32_t* key = a.in[0];
+  for (uint32_t i0 = 0; i0 < n; i0 += 4 * blockDim.x) {
+    uint32_t k[4];
 #pragma unroll
-    for (uint32_t j = 0; j < (uint32_t)NS; ++j)
-      p.v[j] = j < a.n_src ? a.spill_val[j * a.spill_rows + region + i] : 0ull;
-    cpay[base + i] = p;
-  }
-  if (b == agg_grid - 1 && tid == 0) *total = base + cnt;
-}
-
-// Partial state of one run for aggregate slot sl: (acc1, acc2) are the two
-// integer words (SUM: signed low/high sums; MIN/MAX: the value in acc1),
-// accd the double sum.
-__device__ __forceinline__ void run_combine(uint32_t kind, uint64_t& a1, uint64_t& a2,
-                                            double& ad, uint64_t b1, uint64_t b2, double bd) {
-  switch (kind) {
-    case K_SUMI: a1 += b1; a2 += b2; break;
-    case K_SUMF: ad += bd; break;
-    case K_MINI: a1 = (int64_t)b1 < (int64_t)a1 ? b1 : a1; break;
-    case K_MAXI: a1 = (int64_t)b1 > (int64_t)a1 ? b1 : a1; break;
-    case K_MINF: a1 = b1 < a1 ? b1 : a1; break;
-    case K_MAXF: a1 = b1 > a1 ? b1 : a1; break;
-    default: break;
-  }
-}
-
-__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
-  return ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d, WAVE) << 32) |
-         (uint32_t)__shfl_up((int)(uint32_t)v, d, WAVE);
-}
-
-// Each wave takes 64 consecutive sorted entries at a time (one coalesced
-// load per lane), finds the runs of equal groups (a lane heads a run when
-// its key differs from the lane before it), and reduces every run with a
-// segmented inclusive scan across the lanes, per aggregate (compile-time
-// index).  The last lane of a run adds the run to the group: with plain
-// loads and stores when the run is the whole group (it does not reach the
-// ends of the 64 entries with the same group on the other side), else with
-// device-scope atomics (a group whose run crosses several windows costs
-// one merge per window, not one atomic per row).
-template <int NS>
-__global__ __launch_bounds__(256) void k_spill_reduce(GroupArgs a, const uint32_t* skey,
-                                                      const SpillPay<NS>* spay, uint32_t total) {
-  const int lane = lane_id();
-  const uint64_t wave_id = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-  const uint64_t n_waves = (uint64_t)gridDim.x * blockDim.x / WAVE;
-  for (uint64_t base = wave_id * WAVE; base < total; base += n_waves * WAVE) {
-    const uint64_t i = base + lane;
-    const bool valid = i < total;
-    const uint32_t key = valid ? skey[i] : 0xFFFFFFFFu;  // keys are < 2^32 - 1
-    SpillPay<NS> p{};
-    if (valid) p = spay[i];
-    const uint32_t before = base > 0 ? skey[base - 1] : 0xFFFFFFFFu;
-    const uint32_t after = base + WAVE < total ? skey[base + WAVE] : 0xFFFFFFFFu;
-    const uint32_t up = (uint32_t)__shfl_up((int)key, 1, WAVE);
-    const uint32_t down = (uint32_t)__shfl_down((int)key, 1, WAVE);
-    const bool head = lane == 0 || key != up;
-    const bool tail = lane == WAVE - 1 || key != down;
-    const uint64_t heads = __ballot(head);
-    // lane index of this run's head: the highest head at or below this lane
-    const uint64_t le = lane == WAVE - 1 ? ~0ull : ((1ull << (lane + 1)) - 1);
-    const int first = 63 - __clzll((long long)(heads & le));
-    const bool shared = (first == 0 && key == before) || (lane == WAVE - 1 && key == after);
-    uint64_t* g = a.st + (uint64_t)key * a.W;
-    if (valid && tail) {
-      const uint64_t cnt = (uint64_t)(lane - first + 1);
-      if (shared) atomicAdd((unsigned long long*)g, (unsigned long long)cnt);
-      else g[0] += cnt;
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t i = i0 + e * blockDim.x + tid;
+      k[e] = i < n ? key[r0 + i] : 0u;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < MTL_GROUP_MAX_AGGS; ++k) {
-      if (k >= a.n_aggs) break;
-      const AggSlot& sl = a.aggs[k];
-      if (sl.dup || sl.kind == K_COUNT) continue;
-      const uint64_t v = pick_src<NS>(p.v, sl.src);
-      uint64_t a1, a2 = 0;
-      double ad = 0.0;
-      if (sl.kind == K_SUMI) {
-        int64_t lo, hi;
-        split_i64((int64_t)v, lo, hi);
-        a1 = (uint64_t)lo;
-        a2 = (uint64_t)hi;
-      } else if (sl.kind == K_SUMF) {
-        a1 = 0;
-        ad = __longlong_as_double((long long)v);
-      } else {
-        a1 = v;
-      }
+    for (int e = 0; e < 4; ++e) {
+      const bool valid = i0 + e * blockDim.x + tid < n;
+      (void)wave_rank(h, valid ? part_digit(a, k[e]) : 0u, valid, a.bits);
+    }
+  }
+  __syncthreads();
+  if (tid < a.fan) a.offs[(uint64_t)t * a.fan + tid] = h[tid];
+}
+
+// Digit d's counts over the tiles of scan group g, in tile order, into
+// output offsets: part_off of the digit's bucket plus the rows of that
+// digit in the group's tiles before.  Block (d, g).
+__global__ __launch_bounds__(256) void k_part_offs(PartLevel a, uint32_t n_tiles) {
+  __shared__ uint32_t s_w[256 / WAVE];
+  __shared__ uint32_t s_carry;
+  const uint32_t d = blockIdx.x, g = blockIdx.y;
+  const uint32_t bkt = a.first ? d : (g << a.bits) | d;
+  if (bkt >= a.nb) return;
+  const uint32_t t0 = a.first ? 0u : a.tile_first[g];
+  const uint32_t t1 = a.first ? n_tiles : a.tile_first[g + 1];
+  if (threadIdx.x == 0) s_carry = a.part_off[bkt << a.ushift];
+  __syncthreads();
+  for (uint32_t c = t0; c < t1; c += blockDim.x) {
+    const uint32_t t = c + threadIdx.x;
+    const uint32_t v = t < t1 ? a.offs[(uint64_t)t * a.fan + d] : 0u;
+    const uint32_t carry = s_carry;
+    const uint32_t ex = block_scan_excl(v, s_w);
+    if (t < t1) a.offs[(uint64_t)t * a.fan + d] = carry + ex;
+    if (threadIdx.x == blockDim.x - 1) s_carry = carry + ex + v;
+    __syncthreads();
+  }
+}
+
+// NPL: compile-time bound on the planes (registers hold a chunk's planes,
+// loaded together before the ranking).  Chunk by chunk, a tile ranks its
+// rows per digit in LDS, stages every plane in digit order and writes it
+// with consecutive threads on consecutive addresses (runs of one bucket).
+template <int NPL>
+__global__ __launch_bounds__(kScatThreads) void k_part_scatter(PartLevel a) {
+  __shared__ uint32_t cur[kMaxFan];   // next output row of each digit
+  __shared__ uint32_t hist[kMaxFan];  // rows of each digit in the chunk
+  __shared__ uint32_t loff[kMaxFan];  // their first staged position
+  __shared__ uint32_t dest[kPartChunk];
+  __shared__ uint32_t stage[kPartChunk];
+  __shared__ uint32_t s_w[kScatThreads / WAVE];
+  __shared__ uint64_t s_r0;
+  __shared__ uint32_t s_n, s_ok;
+  constexpr int R = kPartChunk / kScatThreads;
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  if (tid == 0) {
+    uint64_t r0 = 0;
+    uint32_t n = 0, g;
+    s_ok = part_tile(a, t, &r0, &n, &g) ? 1u : 0u;
+    s_r0 = r0;
+    s_n = n;
+  }
+  __syncthreads();
+  if (!s_ok) return;
+  const uint64_t r0 = s_r0;
+  const uint32_t n = s_n;
+  if (tid < a.fan) cur[tid] = a.offs[(uint64_t)t * a.fan + tid];
+  for (uint32_t c = 0; c < n; c += kPartChunk) {
+    const uint32_t cn = n - c < (uint32_t)kPartChunk ? n - c : (uint32_t)kPartChunk;
+    uint32_t pv[NPL][R], dg[R], rank[R];
 #pragma unroll
-      for (int off = 1; off < WAVE; off <<= 1) {
-        const uint64_t b1 = shfl_up64(a1, off), b2 = shfl_up64(a2, off);
-        const double bd = __longlong_as_double((long long)shfl_up64(
-            (uint64_t)__double_as_longlong(ad), off));
-        if (lane - off >= first) run_combine(sl.kind, a1, a2, ad, b1, b2, bd);
+    for (int p = 0; p < NPL; ++p)
+#pragma unroll
+      for (int e = 0; e < R; ++e) {
+        const uint32_t i = e * kScatThreads + tid;
+        pv[p][e] = (i < cn && (uint32_t)p < a.n_planes) ? a.in[p][r0 + c + i] : 0u;
       }
-      if (valid && tail) {
-        uint64_t* w = g + sl.word;
-        if (shared) {
-          switch (sl.kind) {
-            case K_SUMI:
-              atomicAdd((unsigned long long*)w, (unsigned long long)a1);
-              if (a2) atomicAdd((unsigned long long*)(w + 1), (unsigned long long)a2);
-              break;
-            case K_SUMF: unsafeAtomicAdd((double*)w, ad); break;
-            case K_MINI: atomicMin((long long*)w, (long long)a1); break;
-            case K_MAXI: atomicMax((long long*)w, (long long)a1); break;
-            case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)a1); break;
-            case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)a1); break;
-            default: break;
-          }
-        } else {
-          uint64_t c1 = w[0], c2 = sl.kind == K_SUMI ? w[1] : 0;
-          double cd = __longlong_as_double((long long)c1);
-          run_combine(sl.kind, c1, c2, cd, a1, a2, ad);
-          if (sl.kind == K_SUMF) c1 = (uint64_t)__double_as_longlong(cd);
-          w[0] = c1;
-          if (sl.kind == K_SUMI) w[1] = c2;
+    if (tid < a.fan) hist[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < R; ++e) {
+      const bool valid = e * kScatThreads + tid < cn;
+      dg[e] = valid ? part_digit(a, pv[0][e]) : 0u;
+      rank[e] = wave_rank(hist, dg[e], valid, a.bits);
+    }
+    __syncthreads();
+    const uint32_t hc = tid < a.fan ? hist[tid] : 0u;
+    const uint32_t ex = block_scan_excl(hc, s_w);
+    if (tid < a.fan) loff[tid] = ex;
+    __syncthreads();
+    uint32_t si[R];
+#pragma unroll
+    for (int e = 0; e < R; ++e) {
+      si[e] = ~0u;
+      if (e * kScatThreads + tid >= cn) continue;
+      si[e] = loff[dg[e]] + rank[e];
+      dest[si[e]] = cur[dg[e]] + rank[e];
+    }
+    __syncthreads();
+    if (tid < a.fan) cur[tid] += hc;
+#pragma unroll
+    for (int p = 0; p < NPL; ++p) {
+      if ((uint32_t)p >= a.n_planes) break;
+      if (p) __syncthreads();  // the previous plane's stage is read out
+#pragma unroll
+      for (int e = 0; e < R; ++e)
+        if (si[e] != ~0u) stage[si[e]] = pv[p][e];
+      __syncthreads();
+      uint32_t* out = a.out[p];
+      for (uint32_t i = tid; i < cn; i += kScatThreads) out[dest[i]] = stage[i];
+    }
+    __syncthreads();  // dest and stage are rewritten by the next chunk
+  }
+}
+
+struct PartPlanes {
+  const uint32_t* p[kMaxPlanes];
+};
+template <int NS>
+__global__ __launch_bounds__(kPartAggThreads) void k_part_agg(GroupArgs a, PartPlanes pl,
+                                                              const uint32_t* part_off,
+                                                              const uint32_t* coff, uint32_t np,
+                                                              uint32_t ps, uint64_t* xkey,
+                                                              uint64_t* xst) {
+  extern __shared__ uint64_t cst[];
+  __shared__ uint32_t s_p;
+  const uint32_t bi = blockIdx.x;
+  if (bi >= coff[np]) return;
+  if (threadIdx.x == 0) {  // the partition of this block: last p with coff[p] <= bi
+    uint32_t lo = 0, hi = np - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (coff[mid] <= bi) lo = mid;
+      else hi = mid - 1;
+    }
+    s_p = lo;
+  }
+  const uint32_t ks = 1u << ps, W = a.W, ws = ks + 1;  // ks groups and a spare column
+  for (uint32_t t = threadIdx.x; t < ws * W; t += blockDim.x) cst[t] = a.init[t / ws];
+  __syncthreads();
+  const uint32_t p = s_p, c = bi - coff[p];
+  const uint64_t g0 = (uint64_t)p << ps;
+  const uint32_t nk = a.cap - g0 < ks ? (uint32_t)(a.cap - g0) : ks;
+  const uint32_t pe = part_off[p + 1];
+  const uint32_t b = part_off[p] + c * kPartAggRows;
+  const uint32_t e = pe - b < kPartAggRows ? pe : b + kPartAggRows;
+  constexpr int R = 4;
+  for (uint32_t i0 = b; i0 < e; i0 += R * blockDim.x) {
+    int k[R];
+    uint64_t v[R][NS];
+    uint32_t lo[R][NS], hi[R][NS];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t i = i0 + r * blockDim.x + threadIdx.x;
+      const uint32_t ii = i < e ? i : b;  // rows past the end read row b
+      k[r] = i < e ? (int)(pl.p[0][ii] - (uint32_t)g0) : -1;
+#pragma unroll
+      for (uint32_t j = 0; j < (uint32_t)NS; ++j) {  // unused sources read plane 0
+        lo[r][j] = pl.p[a.plane_lo[j]][ii];
+        hi[r][j] = pl.p[a.plane_hi[j]][ii];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (uint32_t j = 0; j < (uint32_t)NS; ++j)
+        v[r][j] = a.plane_hi[j] != a.plane_lo[j] ? ((uint64_t)hi[r][j] << 32 | lo[r][j])
+                                                  : widen_src(a, j, lo[r][j]);
+    cache_update<NS, R, true>(a, cst, ws, k, v);  // rows past the end: spare column
+  }
+  __syncthreads();
+  if (c == 0) {
+    uint64_t* out = a.st + g0 * W;
+    for (uint32_t t = threadIdx.x; t < nk * W; t += blockDim.x) out[t] = cst[(t % W) * ws + t / W];
+  } else {
+    const uint64_t x0 = (uint64_t)(bi - p - 1) * ks;
+    for (uint32_t i = threadIdx.x; i < ks; i += blockDim.x)
+      xkey[x0 + i] = (i < nk && cst[i] != 0) ? g0 + i : GEMPTY;  // word 0: the row count
+    for (uint32_t t = threadIdx.x; t < ks * W; t += blockDim.x)
+      xst[x0 * W + t] = cst[(t % W) * ws + t / W];
+  }
+}
+
+// Combine partial state q (word k at q[k]) into the register copy r.
+__device__ __forceinline__ void combine_state(const GroupArgs& a, uint64_t (&r)[kMaxW],
+                                              const uint64_t* q) {
+  r[0] += q[0];
+#pragma unroll
+  for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i) {
+    if (i >= a.n_aggs) break;
+    const AggSlot& sl = a.aggs[i];
+    if (sl.dup || sl.kind == K_COUNT) continue;
+    const uint32_t w = sl.word;
+    // registers are indexed by a runtime word: select over the (unrolled)
+    // candidates instead of indexing the array dynamically
+#pragma unroll
+    for (uint32_t k = 1; k < (uint32_t)kMaxW; ++k) {
+      if (k == w) {
+        switch (sl.kind) {
+          case K_SUMI: r[k] += q[k]; break;
+          case K_SUMF:
+            r[k] = (uint64_t)__double_as_longlong(__longlong_as_double((long long)r[k]) +
+                                                  __longlong_as_double((long long)q[k]));
+            break;
+          case K_MINI: r[k] = (int64_t)q[k] < (int64_t)r[k] ? q[k] : r[k]; break;
+          case K_MAXI: r[k] = (int64_t)q[k] > (int64_t)r[k] ? q[k] : r[k]; break;
+          case K_MINF: r[k] = q[k] < r[k] ? q[k] : r[k]; break;
+          case K_MAXF: r[k] = q[k] > r[k] ? q[k] : r[k]; break;
+          default: break;
         }
       }
+      if (sl.kind == K_SUMI && k == w + 1) r[k] += q[k];
     }
+  }
+}
+
+// merge_state from a register copy (word k in r[k]).
+__device__ __forceinline__ void merge_regs(const GroupArgs& a, uint64_t* g,
+                                           const uint64_t (&r)[kMaxW]) {
+  atomicAdd((unsigned long long*)&g[0], (unsigned long long)r[0]);
+#pragma unroll
+  for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i) {
+    if (i >= a.n_aggs) break;
+    const AggSlot& sl = a.aggs[i];
+    if (sl.dup || sl.kind == K_COUNT) continue;
+    uint64_t v0 = 0, v1 = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < (uint32_t)kMaxW; ++k) {
+      if (k == sl.word) v0 = r[k];
+      if (k == sl.word + 1) v1 = r[k];
+    }
+    uint64_t* w = g + sl.word;
+    switch (sl.kind) {
+      case K_SUMI:
+        atomicAdd((unsigned long long*)w, (unsigned long long)v0);
+        if (v1) atomicAdd((unsigned long long*)(w + 1), (unsigned long long)v1);
+        break;
+      case K_SUMF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)v0)); break;
+      case K_MINI: atomicMin((long long*)w, (long long)v0); break;
+      case K_MAXI: atomicMax((long long*)w, (long long)v0); break;
+      case K_MINF: atomicMin((unsigned long long*)w, (unsigned long long)v0); break;
+      case K_MAXF: atomicMax((unsigned long long*)w, (unsigned long long)v0); break;
+      default: break;
+    }
+  }
+}
+
+// Staged partial states (slot i of stage block s: key skey[s * ks + i],
+// GEMPTY when unused, W words at sst[(s * ks + i) * W]) into the finished
+// table.  Block (x, y) takes stage blocks [y * G, y * G + G) and one thread a
+// slot column: runs of one key down the column are combined in registers
+// and merged with one set of device atomics, so a group staged by many
+// blocks (a hot key in every LDS cache, a large partition's extra blocks)
+// costs G times fewer contended atomics.  With coff set, only stage blocks
+// below coff[np] - np are live.
+__global__ __launch_bounds__(256) void k_stage_reduce(GroupArgs a, const uint64_t* skey,
+                                                      const uint64_t* sst, uint64_t n_stages,
+                                                      uint32_t ks, uint32_t G,
+                                                      const uint32_t* coff, uint32_t np) {
+  const uint64_t live = coff ? (uint64_t)(coff[np] - np) : n_stages;
+  const uint64_t s0 = (uint64_t)blockIdx.y * G;
+  if (s0 >= live) return;
+  const uint64_t s1 = s0 + G < live ? s0 + G : live;
+  const uint32_t W = a.W;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ks; i += gridDim.x * blockDim.x) {
+    uint64_t cur = GEMPTY;
+    uint64_t r[kMaxW];
+    for (uint64_t s = s0; s < s1; ++s) {
+      const uint64_t slot = s * ks + i;
+      const uint64_t key = skey[slot];
+      if (key == GEMPTY) continue;
+      const uint64_t* q = sst + slot * W;
+      if (key != cur) {
+        if (cur != GEMPTY) merge_regs(a, a.st + cur * W, r);
+        cur = key;
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)kMaxW; ++k) r[k] = k < W ? q[k] : 0ull;
+      } else {
+        combine_state(a, r, q);
+      }
+    }
+    if (cur != GEMPTY) merge_regs(a, a.st + cur * W, r);
   }
 }
 
@@ -1033,6 +1631,20 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
     }
     sl.src = j;
   }
+  // spill planes: the key, then one or two per source (see GroupArgs)
+  a->n_planes = 1;
+  for (uint32_t j = 0; j < MTL_GROUP_MAX_AGGS; ++j) {
+    a->plane_lo[j] = a->plane_hi[j] = 0;
+    if (j >= a->n_src) continue;
+    a->plane_lo[j] = a->plane_hi[j] = a->n_planes++;
+    if (type_width(a->src_type[j]) == 8) a->plane_hi[j] = a->n_planes++;
+  }
+  a->n_ops = 0;
+  for (uint32_t i = 0; i < d->n_aggs; ++i) {
+    const AggSlot& sl = a->aggs[i];
+    if (sl.dup || sl.kind == K_COUNT) continue;
+    a->ops[a->n_ops++] = sl.kind | sl.word << 4 | sl.src << 12;
+  }
   a->key_w0 = (uint32_t)type_width(a->key_type[0]);
   for (uint32_t j = 0; j < MTL_GROUP_MAX_AGGS; ++j) {
     if (j >= a->n_src) {  // unused: a valid column, never converted
@@ -1073,48 +1685,179 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
 // from this many rows (smaller inputs update the global table directly).
 constexpr uint64_t kSpillMinRows = 1ull << 20;
 
-// Compaction, sort and reduction of the spilled rows (after k_group_agg on
-// the same stream); `total` spilled rows, read by the caller.
+// The specialised k_group_agg for the shape of `a`, if it has one (see
+// k_group_agg); false: the caller launches the general variant.
+template <int KW, int S0, int S1>
+static void launch_agg_fixed(const GroupArgs& a, dim3 gd, size_t lds, hipStream_t st) {
+  if (a.cache_direct) hipLaunchKernelGGL((k_group_agg<2, KW, S0, S1, 1>), gd, dim3(kAggBlock), lds, st, a);
+  else hipLaunchKernelGGL((k_group_agg<2, KW, S0, S1, 2>), gd, dim3(kAggBlock), lds, st, a);
+}
+static bool launch_agg_fast(const GroupArgs& a, dim3 gd, size_t lds, hipStream_t st) {
+  if (a.sel || a.n_keys != 1 || a.raw || a.n_src > 2 || !(a.cache_direct || a.spill)) return false;
+  auto fits = [](uint32_t w) { return w == 4 || w == 8; };
+  if (!fits(a.key_w0) || !fits(a.src_w[0]) || !fits(a.src_w[1])) return false;
+  switch ((a.key_w0 == 8) * 4 + (a.src_w[0] == 8) * 2 + (a.src_w[1] == 8)) {
+    case 0: launch_agg_fixed<4, 4, 4>(a, gd, lds, st); break;
+    case 1: launch_agg_fixed<4, 4, 8>(a, gd, lds, st); break;
+    case 2: launch_agg_fixed<4, 8, 4>(a, gd, lds, st); break;
+    case 3: launch_agg_fixed<4, 8, 8>(a, gd, lds, st); break;
+    case 4: launch_agg_fixed<8, 4, 4>(a, gd, lds, st); break;
+    case 5: launch_agg_fixed<8, 4, 8>(a, gd, lds, st); break;
+    case 6: launch_agg_fixed<8, 8, 4>(a, gd, lds, st); break;
+    default: launch_agg_fixed<8, 8, 8>(a, gd, lds, st); break;
+  }
+  return true;
+}
+
+// Partitioning and aggregation of the spilled rows, then the merge of the
+// staged LDS caches (after k_group_agg on the same stream, which left
+// `total` rows in its per-block regions of at most seg_max rows).  Writes every
+// table entry: no k_group_init before it.
 template <int NS>
-static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t total, mtl_stream* s) {
-  if (total == 0) return MTL_OK;
+static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t seg_max, uint64_t total,
+                            mtl_stream* s) {
   if (total > 0xFFFFFFFFull) return set_error(MTL_UNSUPPORTED, "more than 2^32-1 spilled rows");
   Arena& ar = s->arena;
   hipStream_t st = s->stream;
-  void *ckey, *cpay, *skey, *spay, *tot;
-  MTL_TRY(ar.alloc(total * 4, &ckey));
-  MTL_TRY(ar.alloc(total * sizeof(SpillPay<NS>), &cpay));
-  MTL_TRY(ar.alloc(total * 4, &skey));
-  MTL_TRY(ar.alloc(total * sizeof(SpillPay<NS>), &spay));
-  MTL_TRY(ar.alloc(16, &tot));
-  {
-    ProfScope ps(s, "k_spill_compact");
-    hipLaunchKernelGGL(k_spill_compact<NS>, dim3(agg_grid), dim3(256), 0, st, a, agg_grid,
-                       (uint32_t*)ckey, (SpillPay<NS>*)cpay, (uint32_t*)tot);
+  // partition size: the largest power of two of groups whose state fits
+  uint32_t ps = 0;
+  while ((2ull << ps) * a.W * 8 <= kPartAggLds) ++ps;
+  const uint64_t np64 = (a.cap + (1ull << ps) - 1) >> ps;
+  if (np64 > 0xFFFFFFFFull) return set_error(MTL_UNSUPPORTED, "too many group partitions");
+  const uint32_t np = (uint32_t)np64;
+  uint32_t pb = 0;
+  while ((1ull << pb) < np) ++pb;
+  const uint32_t levels = pb == 0 ? 1 : (pb + kPartLevelBits - 1) / kPartLevelBits;
+  const uint32_t lb = (pb + levels - 1) / levels;
+  uint32_t ush[8];  // bucket of level l = partition >> ush[l]
+  uint64_t nb[8];
+  for (uint32_t l = 0; l < levels; ++l) {
+    ush[l] = pb > (l + 1) * lb ? pb - (l + 1) * lb : 0;
+    nb[l] = (np + (1ull << ush[l]) - 1) >> ush[l];
+  }
+  // tiles of each level (first: the spill regions'; later: a bound) and
+  // their digit offsets
+  const uint32_t tiles_per_seg = (uint32_t)ceil_div(seg_max, kPartTile);
+  uint64_t offs_words = (uint64_t)agg_grid * tiles_per_seg * nb[0], max_groups = 1;
+  for (uint32_t l = 1; l < levels; ++l) {
+    offs_words = std::max<uint64_t>(offs_words, (ceil_div(total, kPartTile) + nb[l - 1]) *
+                                                    (1ull << (ush[l - 1] - ush[l])));
+    max_groups = std::max<uint64_t>(max_groups, nb[l - 1]);
+  }
+  // aggregation blocks beyond the first of each partition (an upper bound)
+  // and their staged states
+  const uint64_t ks = 1ull << ps, extra = ceil_div(total, kPartAggRows);
+  void *cnt, *off, *coff, *offs, *tfirst, *xk = nullptr, *xs = nullptr;
+  MTL_TRY(ar.alloc((uint64_t)np * 4, &cnt));
+  MTL_TRY(ar.alloc((uint64_t)np * 4 + 4, &off));
+  MTL_TRY(ar.alloc((uint64_t)np * 4 + 4, &coff));
+  MTL_TRY(ar.alloc(offs_words * 4, &offs));
+  MTL_TRY(ar.alloc(max_groups * 4 + 4, &tfirst));
+  if (extra) {
+    MTL_TRY(ar.alloc(extra * ks * 8, &xk));
+    MTL_TRY(ar.alloc(extra * ks * a.W * 8, &xs));
+  }
+  // the second set of planes (levels alternate between it and the spill
+  // planes, contiguous from the second level on)
+  uint32_t* pb_[kMaxPlanes] = {};
+  for (uint32_t q = 0; total && q < a.n_planes; ++q) {
+    void* m;
+    MTL_TRY(ar.alloc(total * 4, &m));
+    pb_[q] = (uint32_t*)m;
+  }
+  ZeroList zl;
+  zl.add((uint32_t*)cnt, np);
+  MTL_TRY(launch_zero(zl, st));
+  if (total) {
+    ProfScope p1(s, "k_part_hist");
+    hipLaunchKernelGGL(k_part_hist, dim3(agg_grid), dim3(kPartThreads),
+                       np <= kPartHistLds ? (size_t)np * 4 : 0, st, a.spill_pl[0], a.spill_n,
+                       ps, np, (uint32_t*)cnt);
     MTL_HIP(hipGetLastError());
   }
-  // radix bits of the largest packed key (dense domain: keys < cap)
-  unsigned int bits = 1;
-  while (bits < 32 && (a.cap - 1) >> bits) ++bits;
-  size_t temp_bytes = 0;
-  MTL_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint32_t*)ckey, (uint32_t*)skey,
-                                    (SpillPay<NS>*)cpay, (SpillPay<NS>*)spay, (unsigned int)total,
-                                    0u, bits, st));
-  void* temp;
-  MTL_TRY(ar.alloc(temp_bytes ? temp_bytes : 1, &temp));
-  {
-    ProfScope ps(s, "spill_sort");
-    MTL_HIP(rocprim::radix_sort_pairs(temp, temp_bytes, (uint32_t*)ckey, (uint32_t*)skey,
-                                      (SpillPay<NS>*)cpay, (SpillPay<NS>*)spay,
-                                      (unsigned int)total, 0u, bits, st));
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartThreads), 0, st, (const uint32_t*)cnt, np,
+                     (uint32_t*)off, (uint32_t*)coff);
+  MTL_HIP(hipGetLastError());
+  PartPlanes fin{};
+  for (uint32_t q = 0; q < a.n_planes; ++q) fin.p[q] = a.spill_pl[q];
+  for (uint32_t l = 0; total && l < levels; ++l) {
+    PartLevel lv{};
+    const bool to_b = (l % 2) == 0;
+    for (uint32_t q = 0; q < a.n_planes; ++q) {
+      lv.in[q] = fin.p[q];
+      lv.out[q] = to_b ? pb_[q] : a.spill_pl[q];
+    }
+    lv.n_planes = a.n_planes;
+    lv.part_off = (const uint32_t*)off;
+    lv.np = np;
+    lv.offs = (uint32_t*)offs;
+    lv.shift = ps + ush[l];
+    lv.ushift = ush[l];
+    lv.nb = (uint32_t)nb[l];
+    uint64_t tiles;
+    uint32_t groups;
+    if (l == 0) {
+      lv.first = 1;
+      lv.seg_n = a.spill_n;
+      lv.n_seg = agg_grid;
+      lv.tiles_per_seg = tiles_per_seg;
+      lv.fan = (uint32_t)nb[0];
+      lv.bits = 0;
+      while ((1u << lv.bits) < lv.fan) ++lv.bits;
+      tiles = (uint64_t)agg_grid * tiles_per_seg;
+      groups = 1;
+    } else {
+      lv.first = 0;
+      lv.uprev = ush[l - 1];
+      lv.bits = ush[l - 1] - ush[l];
+      lv.fan = 1u << lv.bits;
+      lv.n_groups = (uint32_t)nb[l - 1];
+      lv.tile_first = (const uint32_t*)tfirst;
+      hipLaunchKernelGGL(k_part_tiles, dim3(1), dim3(kPartThreads), 0, st, (const uint32_t*)off,
+                         np, ush[l - 1], lv.n_groups, (uint32_t*)tfirst);
+      MTL_HIP(hipGetLastError());
+      tiles = ceil_div(total, kPartTile) + lv.n_groups;
+      groups = lv.n_groups;
+    }
+    {
+      ProfScope p2(s, "k_part_scatter");
+      hipLaunchKernelGGL(k_part_count, dim3((uint32_t)tiles), dim3(256), 0, st, lv);
+      MTL_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_part_offs, dim3(lv.fan, groups), dim3(256), 0, st, lv,
+                         (uint32_t)tiles);
+      MTL_HIP(hipGetLastError());
+      const dim3 gd((uint32_t)tiles), bd(kScatThreads);
+      if (a.n_planes <= 5) hipLaunchKernelGGL(k_part_scatter<5>, gd, bd, 0, st, lv);
+      else if (a.n_planes <= 9) hipLaunchKernelGGL(k_part_scatter<9>, gd, bd, 0, st, lv);
+      else hipLaunchKernelGGL(k_part_scatter<kMaxPlanes>, gd, bd, 0, st, lv);
+      MTL_HIP(hipGetLastError());
+    }
+    for (uint32_t q = 0; q < a.n_planes; ++q) fin.p[q] = lv.out[q];
   }
   {
-    ProfScope ps(s, "k_spill_reduce");
-    const uint64_t blocks = std::min<uint64_t>(ceil_div(total, 256), (uint64_t)cu_count() * 16);
-    hipLaunchKernelGGL(k_spill_reduce<NS>, dim3((uint32_t)std::max<uint64_t>(blocks, 1)),
-                       dim3(256), 0, st, a, (const uint32_t*)skey, (const SpillPay<NS>*)spay,
-                       (uint32_t)total);
+    ProfScope p3(s, "k_part_agg");
+    hipLaunchKernelGGL(k_part_agg<NS>, dim3((uint32_t)(np + extra)), dim3(kPartAggThreads),
+                       (size_t)(ks + 1) * a.W * 8, st, a, fin, (const uint32_t*)off,
+                       (const uint32_t*)coff, np, ps, (uint64_t*)xk, (uint64_t*)xs);
     MTL_HIP(hipGetLastError());
+  }
+  {
+    ProfScope p4(s, "k_stage_reduce");
+    const uint32_t cs = a.cache_slots;
+    // hot caches: the same head keys in every block, so long columns
+    const uint32_t gh = 64, gx = 16;
+    hipLaunchKernelGGL(k_stage_reduce,
+                       dim3((uint32_t)ceil_div(cs, 256), (uint32_t)ceil_div(agg_grid, gh)),
+                       dim3(256), 0, st, a, a.hot_key, a.hot_st, (uint64_t)agg_grid, cs, gh,
+                       nullptr, 0u);
+    MTL_HIP(hipGetLastError());
+    if (extra) {
+      hipLaunchKernelGGL(k_stage_reduce,
+                         dim3((uint32_t)ceil_div(ks, 256), (uint32_t)ceil_div(extra, gx)),
+                         dim3(256), 0, st, a, (const uint64_t*)xk, (const uint64_t*)xs, extra,
+                         (uint32_t)ks, gx, (const uint32_t*)coff, np);
+      MTL_HIP(hipGetLastError());
+    }
   }
   return MTL_OK;
 }
@@ -1127,7 +1870,11 @@ uint64_t group_scratch_bytes(const mtl_group_desc& d) {
   uint64_t entries = 0;
   if (group_plan(&d, &a, &entries) != MTL_OK) return 1u << 20;
   const uint64_t rows = d.sel ? d.sel_capacity : d.n_rows;
-  return entries * (a.W + (a.dense ? 0 : 1)) * 8 + rows * (4 + 8 * (uint64_t)a.n_src) * 2 +
+  // staged states: the LDS caches of k_group_agg and the partial states of
+  // the extra aggregation blocks of large partitions (<= 64 KB per block)
+  const uint64_t hot = (uint64_t)cu_count() * 2 * a.cache_slots * (1 + a.W) * 8 +
+                       ceil_div(rows, kPartAggRows) * (kPartAggLds + kPartAggLds / a.W);
+  return entries * (a.W + (a.dense ? 0 : 1)) * 8 + rows * 4 * (uint64_t)a.n_planes * 2 + hot +
          entries * (16 + 8 * (uint64_t)a.n_aggs) + (64u << 20);
 }
 
@@ -1186,17 +1933,21 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   a.st = (uint64_t*)g->st_mem;
   a.tkeys = (uint64_t*)g->keys_mem;
   a.err = (uint32_t*)err;
-  {
+  uint64_t rows = desc->sel ? desc->sel_capacity : desc->n_rows;
+  // spill mode (a directly indexed domain too large for the LDS cache):
+  // the partitioned aggregation of the spilled rows writes every entry
+  const bool spill = rows && a.dense && !a.cache_direct && rows >= kSpillMinRows;
+  if (!spill) {
     ProfScope ps(s, "k_group_init");
     uint64_t blocks = std::min<uint64_t>(ceil_div(entries, 256), 8192);
     hipLaunchKernelGGL(k_group_init, dim3((uint32_t)std::max<uint64_t>(blocks, 1)), dim3(256),
                        0, s->stream, a, entries);
   }
   MTL_HIP(hipGetLastError());
-  uint64_t rows = desc->sel ? desc->sel_capacity : desc->n_rows;
   if (rows) {
     a.door = (!a.cache_direct && a.cache_slots && rows >= kSpillMinRows) ? 1u : 0u;
-    size_t lds = (size_t)a.cache_slots * 8 * (1 + a.W) + (a.door ? kDoorBits / 8 : 0);
+    size_t lds = (size_t)a.cache_slots * 8 + (size_t)(a.cache_slots + 1) * 8 * a.W +
+                 (a.door ? kDoorBits / 8 : 0);
     // as many blocks as are resident at once (LDS caches of <= 60 KB in the
     // 160 KB of a CU; at most 2 blocks of 768 threads), so every block gets
     // an equal share of the rows and none waits for a second round
@@ -1204,26 +1955,34 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
     uint64_t grid = std::min<uint64_t>(ceil_div(rows, (uint64_t)kAggBlock * GE * 4),
                                        (uint64_t)cu_count() * per_cu);
     const dim3 gd((uint32_t)std::max<uint64_t>(grid, 1));
-    if (a.dense && !a.cache_direct && rows >= kSpillMinRows) {
-      void *sk, *sv, *sn;
-      MTL_TRY(s->arena.alloc(rows * 4, &sk));
-      MTL_TRY(s->arena.alloc(rows * 8 * std::max<uint32_t>(a.n_src, 1), &sv));
-      MTL_TRY(s->arena.alloc(gd.x * 4, &sn));
+    if (spill) {
+      void *sn, *hk, *hv;
+      const uint64_t hot = (uint64_t)gd.x * a.cache_slots;
+      for (uint32_t q = 0; q < a.n_planes; ++q) {
+        void* m;
+        MTL_TRY(s->arena.alloc(rows * 4, &m));
+        a.spill_pl[q] = (uint32_t*)m;
+      }
+      MTL_TRY(s->arena.alloc(gd.x * 4 + 4, &sn));
+      MTL_TRY(s->arena.alloc(hot * 8, &hk));
+      MTL_TRY(s->arena.alloc(hot * a.W * 8, &hv));
       a.spill = 1;
-      a.spill_key = (uint32_t*)sk;
-      a.spill_val = (uint64_t*)sv;
       a.spill_n = (uint32_t*)sn;
       a.spill_rows = rows;
+      a.hot_key = (uint64_t*)hk;
+      a.hot_st = (uint64_t*)hv;
     }
     {
       ProfScope ps(s, "k_group_agg");
-      if (a.n_src <= 2) hipLaunchKernelGGL(k_group_agg<2>, gd, dim3(kAggBlock), lds, s->stream, a);
+      if (launch_agg_fast(a, gd, lds, s->stream)) {
+      } else if (a.n_src <= 2) hipLaunchKernelGGL(k_group_agg<2>, gd, dim3(kAggBlock), lds, s->stream, a);
       else if (a.n_src <= 4) hipLaunchKernelGGL(k_group_agg<4>, gd, dim3(kAggBlock), lds, s->stream, a);
       else hipLaunchKernelGGL(k_group_agg<8>, gd, dim3(kAggBlock), lds, s->stream, a);
       MTL_HIP(hipGetLastError());
     }
     if (a.spill) {
-      // the spilled row count sizes the sort: read it with the error bits
+      // the spilled row count sizes the partition buffers: read it with
+      // the error bits
       void* hs;
       MTL_TRY(stream_pinned(s, 8 + (size_t)gd.x * 4, &hs));
       MTL_HIP(hipMemcpyAsync(hs, err, 4, hipMemcpyDeviceToHost, s->stream));
@@ -1232,10 +1991,11 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
       MTL_HIP(hipStreamSynchronize(s->stream));
       uint64_t total = 0;
       for (uint32_t b = 0; b < gd.x; ++b) total += ((const uint32_t*)((char*)hs + 8))[b];
+      const uint64_t seg_max = ceil_div(rows, gd.x);
       if (*(const uint32_t*)hs == 0) {
-        if (a.n_src <= 2) MTL_TRY(spill_finish<2>(a, gd.x, total, s));
-        else if (a.n_src <= 4) MTL_TRY(spill_finish<4>(a, gd.x, total, s));
-        else MTL_TRY(spill_finish<8>(a, gd.x, total, s));
+        if (a.n_src <= 2) MTL_TRY(spill_finish<2>(a, gd.x, seg_max, total, s));
+        else if (a.n_src <= 4) MTL_TRY(spill_finish<4>(a, gd.x, seg_max, total, s));
+        else MTL_TRY(spill_finish<8>(a, gd.x, seg_max, total, s));
       }
     }
   }
@@ -1243,9 +2003,10 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   g->args = a;
   g->args.err = nullptr;  // arena memory is not kept past this call
   g->args.spill = 0;
-  g->args.spill_key = nullptr;
-  g->args.spill_val = nullptr;
+  for (uint32_t q = 0; q < kMaxPlanes; ++q) g->args.spill_pl[q] = nullptr;
   g->args.spill_n = nullptr;
+  g->args.hot_key = nullptr;
+  g->args.hot_st = nullptr;
   // check errors now (synchronous boundary of the aggregate)
   void* host;
   MTL_TRY(stream_pinned(s, 8, &host));
