@@ -45,7 +45,7 @@ using namespace dev;
 constexpr uint64_t GEMPTY = ~0ull;
 constexpr uint64_t kDenseMax = 1ull << 26;  // directly indexed packed domain
 constexpr int kMaxW = 1 + 2 * MTL_GROUP_MAX_AGGS;
-constexpr int kMaxPlanes = 1 + 2 * MTL_GROUP_MAX_AGGS;
+constexpr int kMaxPay = 2 * MTL_GROUP_MAX_AGGS;  // payload words of a spilled row
 
 enum Kind : uint32_t { K_COUNT = 0, K_SUMI, K_SUMF, K_MINI, K_MAXI, K_MINF, K_MAXF };
 
@@ -80,7 +80,8 @@ struct GroupArgs {
   // owns state words (not COUNT, not a duplicate), packed as kind | word << 4
   // | source << 12
   uint32_t n_ops;
-  uint32_t ops[MTL_GROUP_MAX_AGGS];
+  uint32_t ops[MTL_GROUP_MAX_AGGS];  // ordered by source
+  uint32_t op_beg[MTL_GROUP_MAX_AGGS + 1];  // ops of source j: [op_beg[j], op_beg[j + 1])
   // distinct (column, conversion) value sources of the aggregates: each is
   // loaded once per row (SUM, MIN and MAX of one column share one load)
   uint32_t n_src;
@@ -107,14 +108,16 @@ struct GroupArgs {
   // arrays (region of block b starts at row b * per) instead of updating
   // the global table with device-scope atomics
   uint32_t spill;
-  // spilled rows are 32-bit planes: plane 0 the packed key, then each
-  // source's value in one plane (a source of <= 4 bytes, narrowed exactly,
-  // see narrow_src) or two (low and high halves); plane_hi == plane_lo marks
-  // a narrowed source, and sources past n_src name plane 0
-  uint32_t n_planes;
-  uint32_t plane_lo[MTL_GROUP_MAX_AGGS];
-  uint32_t plane_hi[MTL_GROUP_MAX_AGGS];
-  uint32_t* spill_pl[kMaxPlanes];  // [rows] each
+  // a spilled row is its packed key (spill_key[o]) and a payload record of
+  // n_pay 32-bit words (spill_pay[o * n_pay ...]): each source's value in
+  // one word (a source of <= 4 bytes, narrowed exactly, see narrow_src) or
+  // two (its low and high halves, words pay_lo[j] and pay_hi[j]); pay_hi ==
+  // pay_lo marks a narrowed source, and sources past n_src name word 0
+  uint32_t n_pay;
+  uint32_t pay_lo[MTL_GROUP_MAX_AGGS];
+  uint32_t pay_hi[MTL_GROUP_MAX_AGGS];
+  uint32_t* spill_key;  // [rows]
+  uint32_t* spill_pay;  // [rows * n_pay]
   uint32_t* spill_n;     // [grid + 1]: spilled rows of each block, then the
                          // region length (rows per block)
   uint64_t spill_rows;   // row capacity of the spill arrays
@@ -362,13 +365,52 @@ __device__ __forceinline__ uint64_t global_slot(const GroupArgs& a, uint64_t key
   return GEMPTY;
 }
 
+// Table initialisation; block 0 also clears the 64 error words (a.err).
 __global__ void k_group_init(GroupArgs a, uint64_t n_groups) {
+  if (blockIdx.x == 0 && threadIdx.x < 64) a.err[threadIdx.x] = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_groups;
        i += (uint64_t)gridDim.x * blockDim.x) {
     uint64_t* p = a.st + i * a.W;
     for (uint32_t w = 0; w < a.W; ++w) p[w] = a.init[w];
     if (!a.dense) a.tkeys[i] = GEMPTY;
   }
+}
+
+// The cache of the spill mode: 32-bit keys in sets of kWays consecutive
+// slots (set = multiplicative hash of the key), two 16-byte LDS reads per
+// lookup.  A key is inserted into the first empty way of its set (after
+// the doorkeeper, when on) with a CAS; ways are only ever filled, so a key
+// holds at most one way and a stale read costs at most a failed CAS.  The
+// slot, or -1 when the row is not cached (its set is full of other keys,
+// or the doorkeeper has not seen it before).
+constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
+constexpr uint32_t kWays = 8;
+__device__ __forceinline__ int set_lookup(uint32_t* ck32, uint32_t cs, uint32_t* door,
+                                          uint32_t use_door, uint32_t k) {
+  const uint32_t sets = cs / kWays;  // a power of two, at least 16 (cache of >= 128 slots)
+  const uint32_t s0 = (k * 0x9E3779B1u) >> (32 - (31 - __clz((int)sets)));
+  uint32_t* set = ck32 + s0 * kWays;
+  const uint4 w0 = ((const uint4*)set)[0], w1 = ((const uint4*)set)[1];
+  const uint32_t w[kWays] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+  int first_empty = -1;
+#pragma unroll
+  for (int i = kWays - 1; i >= 0; --i) {
+    if (w[i] == k) return (int)(s0 * kWays + i);
+    if (w[i] == kEmpty32) first_empty = i;
+  }
+  if (first_empty < 0) return -1;
+  if (use_door) {
+    // doorkeeper: a group is cached on its second sighting in this block,
+    // so the one-off keys of a long tail do not take the ways of hot groups
+    const uint32_t b = (k * 0x85EBCA6Bu) >> (32 - 16);
+    const uint32_t was = atomicOr(&door[b >> 5], 1u << (b & 31));
+    if (!((was >> (b & 31)) & 1u)) return -1;
+  }
+  for (uint32_t i = (uint32_t)first_empty; i < kWays; ++i) {
+    const uint32_t old = atomicCAS(&set[i], kEmpty32, k);
+    if (old == kEmpty32 || old == k) return (int)(s0 * kWays + i);
+  }
+  return -1;
 }
 
 // The LDS cache updates of a round of GE rows, aggregate-major: the kind of
@@ -390,61 +432,65 @@ __device__ __forceinline__ void cache_update(const GroupArgs& a, uint64_t* cst, 
 #pragma unroll
   for (int e = 0; e < GE_; ++e)
     if (on[e]) atomicAdd((unsigned long long*)&cst[sl[e]], 1ull);
+  // sources outer (compile-time: each op reads its source's registers
+  // directly), the source's ops inner
 #pragma unroll
-  for (uint32_t o = 0; o < MTL_GROUP_MAX_AGGS; ++o) {
-    if (o >= a.n_ops) break;
-    const uint32_t op = a.ops[o], kind = op & 15u, src = op >> 12;
-    uint64_t* w = cst + (uint64_t)((op >> 4) & 255u) * ws;
-    switch (kind) {
-      case K_SUMI:
+  for (uint32_t j = 0; j < (uint32_t)NS; ++j) {
+    if (j >= a.n_src) break;
 #pragma unroll
-        for (int e = 0; e < GE_; ++e) {
-          int64_t lo, hi;
-          split_i64((int64_t)pick_src<NS>(v[e], src), lo, hi);
-          if (on[e]) {
-            atomicAdd((unsigned long long*)&w[sl[e]], (unsigned long long)lo);
-            if (hi) atomicAdd((unsigned long long*)&w[ws + sl[e]], (unsigned long long)hi);
+    for (uint32_t o = 0; o < MTL_GROUP_MAX_AGGS; ++o) {
+      if (a.op_beg[j] + o >= a.op_beg[j + 1]) break;
+      const uint32_t op = a.ops[a.op_beg[j] + o], kind = op & 15u;
+      uint64_t* w = cst + (uint64_t)((op >> 4) & 255u) * ws;
+      switch (kind) {
+        case K_SUMI:
+#pragma unroll
+          for (int e = 0; e < GE_; ++e) {
+            int64_t lo, hi;
+            split_i64((int64_t)v[e][j], lo, hi);
+            if (on[e]) {
+              atomicAdd((unsigned long long*)&w[sl[e]], (unsigned long long)lo);
+              if (hi) atomicAdd((unsigned long long*)&w[ws + sl[e]], (unsigned long long)hi);
+            }
           }
-        }
-        break;
-      case K_SUMF:
+          break;
+        case K_SUMF:
 #pragma unroll
-        for (int e = 0; e < GE_; ++e)
-          if (on[e])
-            unsafeAtomicAdd((double*)&w[sl[e]],
-                            __longlong_as_double((long long)pick_src<NS>(v[e], src)));
-        break;
-      case K_MINI:
+          for (int e = 0; e < GE_; ++e)
+            if (on[e]) unsafeAtomicAdd((double*)&w[sl[e]], __longlong_as_double((long long)v[e][j]));
+          break;
+        case K_MINI:
 #pragma unroll
-        for (int e = 0; e < GE_; ++e) {
-          const long long x = (long long)pick_src<NS>(v[e], src);
-          if (on[e] && x < (long long)lds_peek(&w[sl[e]])) atomicMin((long long*)&w[sl[e]], x);
-        }
-        break;
-      case K_MAXI:
+          for (int e = 0; e < GE_; ++e) {
+            const long long x = (long long)v[e][j];
+            if (on[e] && x < (long long)lds_peek(&w[sl[e]])) atomicMin((long long*)&w[sl[e]], x);
+          }
+          break;
+        case K_MAXI:
 #pragma unroll
-        for (int e = 0; e < GE_; ++e) {
-          const long long x = (long long)pick_src<NS>(v[e], src);
-          if (on[e] && x > (long long)lds_peek(&w[sl[e]])) atomicMax((long long*)&w[sl[e]], x);
-        }
-        break;
-      case K_MINF:
+          for (int e = 0; e < GE_; ++e) {
+            const long long x = (long long)v[e][j];
+            if (on[e] && x > (long long)lds_peek(&w[sl[e]])) atomicMax((long long*)&w[sl[e]], x);
+          }
+          break;
+        case K_MINF:
 #pragma unroll
-        for (int e = 0; e < GE_; ++e) {
-          const uint64_t x = pick_src<NS>(v[e], src);
-          if (on[e] && x < lds_peek(&w[sl[e]]))
-            atomicMin((unsigned long long*)&w[sl[e]], (unsigned long long)x);
-        }
-        break;
-      case K_MAXF:
+          for (int e = 0; e < GE_; ++e) {
+            const uint64_t x = v[e][j];
+            if (on[e] && x < lds_peek(&w[sl[e]]))
+              atomicMin((unsigned long long*)&w[sl[e]], (unsigned long long)x);
+          }
+          break;
+        case K_MAXF:
 #pragma unroll
-        for (int e = 0; e < GE_; ++e) {
-          const uint64_t x = pick_src<NS>(v[e], src);
-          if (on[e] && x > lds_peek(&w[sl[e]]))
-            atomicMax((unsigned long long*)&w[sl[e]], (unsigned long long)x);
-        }
-        break;
-      default: break;
+          for (int e = 0; e < GE_; ++e) {
+            const uint64_t x = v[e][j];
+            if (on[e] && x > lds_peek(&w[sl[e]]))
+              atomicMax((unsigned long long*)&w[sl[e]], (unsigned long long)x);
+          }
+          break;
+        default: break;
+      }
     }
   }
 }
@@ -453,6 +499,7 @@ __device__ __forceinline__ void cache_update(const GroupArgs& a, uint64_t* cst, 
 // per SIMD, within the SGPR-bound limit of 7.
 constexpr int kAggBlock = 768;
 constexpr uint32_t kDoorBits = 1u << 16;  // doorkeeper bitset of the hashed cache (8 KB)
+static_assert(kDoorBits == 1u << 16, "set_lookup takes 16 bits of hash for the doorkeeper");
 constexpr int GE = 4;  // rows per thread per round, loaded together
 
 // NS: compile-time bound on the number of value sources (registers for the
@@ -479,8 +526,14 @@ __global__ __launch_bounds__(kAggBlock, NS <= 4 ? 6 : 4) void k_group_agg(GroupA
   if (threadIdx.x == 0) s_spill = 0;
   if (a.door)
     for (uint32_t i = threadIdx.x; i < kDoorBits / 32; i += blockDim.x) door[i] = 0;
+  // spill mode: packed keys below 2^32 - 1 in a set-associative cache of
+  // 32-bit keys (the first half of the key region)
+  uint32_t* ck32 = (uint32_t*)ckeys;
   for (uint32_t i = threadIdx.x; i < ws; i += blockDim.x) {
-    if (i < cs) ckeys[i] = direct ? (uint64_t)i : GEMPTY;
+    if (i < cs) {
+      if (MODE == 2) ck32[i] = kEmpty32;
+      else ckeys[i] = direct ? (uint64_t)i : GEMPTY;
+    }
     for (uint32_t w = 0; w < a.W; ++w) cst[(uint64_t)w * ws + i] = a.init[w];
   }
   __syncthreads();
@@ -570,6 +623,8 @@ __global__ __launch_bounds__(kAggBlock, NS <= 4 ? 6 : 4) void k_group_agg(GroupA
       if (!ok[e]) {
       } else if (direct) {
         slot[e] = (int)key[e];  // the whole (dense) domain is cached
+      } else if constexpr (MODE == 2) {
+        slot[e] = set_lookup(ck32, cs, door, a.door, (uint32_t)key[e]);
       } else if (cs && key[e] != GEMPTY) {
         const uint64_t hm = gfmix(key[e]);
         uint32_t h = (uint32_t)hm & (cs - 1);
@@ -617,16 +672,17 @@ __global__ __launch_bounds__(kAggBlock, NS <= 4 ? 6 : 4) void k_group_agg(GroupA
           at = __shfl(at, leader, WAVE) + (uint32_t)__popcll(mb & lanemask_lt());
           if (miss) {
             const uint64_t o = start + at;
-            a.spill_pl[0][o] = (uint32_t)key[e];
+            a.spill_key[o] = (uint32_t)key[e];
+            uint32_t* pay = a.spill_pay + o * a.n_pay;
 #pragma unroll
             for (uint32_t j = 0; j < (uint32_t)NS; ++j) {
               if (j >= a.n_src) break;
-              const bool wide = KW ? (j == 0 ? SW0 : SW1) == 8 : a.plane_hi[j] != a.plane_lo[j];
+              const bool wide = KW ? (j == 0 ? SW0 : SW1) == 8 : a.pay_hi[j] != a.pay_lo[j];
               if (wide) {
-                a.spill_pl[a.plane_lo[j]][o] = (uint32_t)v[e][j];
-                a.spill_pl[a.plane_hi[j]][o] = (uint32_t)(v[e][j] >> 32);
+                pay[a.pay_lo[j]] = (uint32_t)v[e][j];
+                pay[a.pay_hi[j]] = (uint32_t)(v[e][j] >> 32);
               } else {
-                a.spill_pl[a.plane_lo[j]][o] = narrow_src(a, j, v[e][j]);
+                pay[a.pay_lo[j]] = narrow_src(a, j, v[e][j]);
               }
             }
           }
@@ -643,7 +699,7 @@ __global__ __launch_bounds__(kAggBlock, NS <= 4 ? 6 : 4) void k_group_agg(GroupA
     if (blockIdx.x == 0 && threadIdx.x == 0) a.spill_n[gridDim.x] = (uint32_t)per;
     const uint64_t h0 = (uint64_t)blockIdx.x * cs;
     for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
-      const uint64_t key = ckeys[i];
+      const uint64_t key = MODE == 2 ? (ck32[i] == kEmpty32 ? GEMPTY : (uint64_t)ck32[i]) : ckeys[i];
       a.hot_key[h0 + i] = (key == GEMPTY || cst[i] == 0) ? GEMPTY : key;
     }
     // state words slot-major, consecutive threads on consecutive words
@@ -669,16 +725,15 @@ __global__ __launch_bounds__(kAggBlock, NS <= 4 ? 6 : 4) void k_group_agg(GroupA
 // are merged in last.  Records are structure-of-arrays: the u32 packed key
 // and the 64-bit value sources (source j of record i at val[j * stride + i]).
 // ---------------------------------------------------------------------------
-constexpr int kPartThreads = 1024;       // partition counting and scan blocks
-constexpr int kScatThreads = 512;        // scatter blocks
-constexpr int kPartChunk = 2048;         // rows a scatter block stages in LDS at a time
-constexpr uint32_t kPartTile = 8192;     // rows per scatter tile (one block)
+constexpr int kPartThreads = 1024;  // partition counting, scan and scatter blocks
+constexpr int kPartChunk = 4096;    // rows per scatter block
+constexpr int kPartLocal = 1024;    // buckets one chunk reserves through LDS
+constexpr uint32_t kLocalBits = 10;
 constexpr uint32_t kPartHistLds = 16384;  // partitions counted in LDS
 constexpr uint64_t kPartAggLds = 64 * 1024;  // LDS state of one partition
 constexpr int kPartAggThreads = 512;
 constexpr uint32_t kPartAggRows = 65536;  // rows per aggregation block
 constexpr int kPartLevelBits = 7;
-constexpr uint32_t kMaxFan = 1u << kPartLevelBits;  // buckets per scatter level
 
 // Exclusive scan of one value per thread over a block of 64-lane waves;
 // s_w holds blockDim.x / 64 words.  Every thread of the block calls it.
@@ -797,239 +852,151 @@ __global__ __launch_bounds__(kPartThreads) void k_part_scan(const uint32_t* coun
   }
 }
 
-// One scatter level moves every row to the output region of its bucket
-// (key >> shift; bucket b's region starts at part_off[b << ushift]).  The
-// input is cut into tiles of at most kPartTile rows: on the first level the
-// tiles of each spill region (tile r * tiles_per_seg + c is rows
-// [c * kPartTile, ...) of region r), on later levels the tiles of each
-// coarse bucket's rows, which the previous level made contiguous (tiles
-// tile_first[s] .. tile_first[s + 1] - 1 of coarse bucket s).  A level is
-// three launches: the digit counts of every tile (k_part_count), their
-// scan into per-tile output offsets in tile order (k_part_offs), and the
-// scatter (k_part_scatter), which carries each digit's cursor in LDS and
-// needs no device atomics, however skewed the keys.  A tile's rows keep
-// their order within each digit only up to the LDS ranking: order within
-// a bucket is not kept.
+// One scatter level: input rows go to the output region of their bucket
+// (key >> shift; bucket b's region starts at part_off[b << ushift]) in chunks
+// of kPartChunk rows.  A chunk ranks its rows per bucket in LDS (lanes of a
+// wave with the same bucket share one atomic), reserves each bucket's run
+// with one device atomic, stages the keys and then the payload records in
+// LDS in bucket order and writes them with consecutive threads on
+// consecutive rows, so each run of a bucket is one stream of records.  The
+// first level reads the spill regions (seg_n set); later levels read the
+// previous level's output, already grouped by bucket >> coarse, so a chunk
+// spans few coarse buckets (buckets past kPartLocal from the chunk's first
+// coarse bucket, rare, take one device atomic per row).  Order within a
+// bucket is not kept.
 struct PartLevel {
-  const uint32_t* in[kMaxPlanes];  // plane 0: the packed key
-  uint32_t* out[kMaxPlanes];
-  uint32_t n_planes;
-  uint32_t first;           // first level: tiles of the spill regions
-  const uint32_t* seg_n;    // first level: rows of each region; seg_n[n_seg]: region length
-  uint32_t n_seg, tiles_per_seg;
-  const uint32_t* tile_first;  // later levels: [n_groups + 1]
-  uint32_t n_groups;           // later levels: coarse buckets
+  const uint32_t* in_key;
+  const uint32_t* in_pay;
+  uint32_t* out_key;
+  uint32_t* out_pay;
+  uint32_t pw;            // payload words per row
+  const uint32_t* seg_n;  // segmented input (spill regions: seg_n[n_seg] is
+                          // their length), else contiguous
+  uint32_t n_seg;
+  uint32_t chunks_per_seg;  // bound on the chunks of one region
+  uint64_t n_in;
   const uint32_t* part_off;
-  uint32_t np;
-  uint32_t* offs;   // [tile][fan]: digit counts, then the output offsets
-  uint32_t shift;   // bucket = key >> shift
-  uint32_t ushift;  // bucket b's first partition = b << ushift
-  uint32_t uprev;   // later levels: coarse bucket s's first partition = s << uprev
-  uint32_t fan, bits;  // digits of this level (first level: the buckets), bits to tell them apart
-  uint32_t nb;         // buckets of this level
+  uint32_t* used;  // rows placed so far per bucket (cleared before the level)
+  uint32_t shift, ushift, coarse, sorted;
 };
 
-__device__ __forceinline__ uint32_t part_digit(const PartLevel& a, uint32_t key) {
-  const uint32_t b = key >> a.shift;
-  return a.first ? b : b & (a.fan - 1);
-}
+// A payload record of PW words, moved as one vector access.
+template <int PW>
+struct alignas(PW == 2 ? 8 : PW == 4 ? 16 : 4) PayRec {
+  uint32_t w[PW];
+};
 
-// Rows [*r0, *r0 + *n) and scan group *g (first level: 0; later: the coarse
-// bucket) of tile t; false for an empty or nonexistent tile.
-__device__ __forceinline__ bool part_tile(const PartLevel& a, uint32_t t, uint64_t* r0,
-                                          uint32_t* n, uint32_t* g) {
-  if (a.first) {
-    const uint32_t r = t / a.tiles_per_seg, c = t % a.tiles_per_seg;
-    if (r >= a.n_seg) return false;
-    const uint32_t rows = a.seg_n[r], f = c * kPartTile;
-    if (f >= rows) return false;
-    *r0 = (uint64_t)r * a.seg_n[a.n_seg] + f;
-    *n = rows - f < kPartTile ? rows - f : kPartTile;
-    *g = 0;
-    return true;
-  }
-  if (t >= a.tile_first[a.n_groups]) return false;
-  uint32_t lo = 0, hi = a.n_groups - 1;  // last s with tile_first[s] <= t
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) / 2;
-    if (a.tile_first[mid] <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  const uint32_t s0 = a.part_off[lo << a.uprev];
-  const uint64_t pe = (uint64_t)(lo + 1) << a.uprev;
-  const uint32_t s1 = a.part_off[pe < a.np ? pe : a.np];
-  const uint32_t f = (t - a.tile_first[lo]) * kPartTile;
-  *r0 = (uint64_t)s0 + f;
-  *n = s1 - s0 - f < kPartTile ? s1 - s0 - f : kPartTile;
-  *g = lo;
-  return true;
-}
-
-// Tiles of each coarse bucket of a later level (one block):
-// tile_first[s] = tiles of the buckets before s, tile_first[n] = all.
-__global__ __launch_bounds__(kPartThreads) void k_part_tiles(const uint32_t* part_off,
-                                                             uint32_t np, uint32_t uprev,
-                                                             uint32_t n, uint32_t* tile_first) {
-  __shared__ uint32_t s_w[kPartThreads / WAVE];
-  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-  const uint32_t b = threadIdx.x * per, e = b + per < n ? b + per : n;
-  auto tiles = [&](uint32_t s) {
-    const uint64_t p1 = (uint64_t)(s + 1) << uprev;
-    const uint32_t len = part_off[p1 < np ? p1 : np] - part_off[s << uprev];
-    return (len + kPartTile - 1) / kPartTile;
-  };
-  uint32_t sum = 0;
-  for (uint32_t i = b; i < e; ++i) sum += tiles(i);
-  uint32_t run = block_scan_excl(sum, s_w);
-  for (uint32_t i = b; i < e; ++i) {
-    tile_first[i] = run;
-    run += tiles(i);
-  }
-  if (threadIdx.x == blockDim.x - 1) tile_first[n] = run;
-}
-
-// Digit counts of each tile (zeros for an empty first-level tile, which the
-// first level's scan runs over).
-__global__ __launch_bounds__(256) void k_part_count(PartLevel a) {
-  __shared__ uint32_t h[kMaxFan];
-  __shared__ uint64_t s_r0;
-  __shared__ uint32_t s_n, s_ok;
-  const uint32_t t = blockIdx.x, tid = threadIdx.x;
-  if (tid < a.fan) h[tid] = 0;
-  if (tid == 0) {
-    uint64_t r0 = 0;
-    uint32_t n = 0, g;
-    s_ok = part_tile(a, t, &r0, &n, &g) ? 1u : 0u;
-    s_r0 = r0;
-    s_n = n;
-  }
-  __syncthreads();
-  if (!s_ok && !a.first) return;
-  const uint64_t r0 = s_r0;
-  const uint32_t n = s_ok ? s_n : 0u;
-  const uint32_t* key = a.in[0];
-  for (uint32_t i0 = 0; i0 < n; i0 += 4 * blockDim.x) {
-    uint32_t k[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint32_t i = i0 + e * blockDim.x + tid;
-      k[e] = i < n ? key[r0 + i] : 0u;
-    }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const bool valid = i0 + e * blockDim.x + tid < n;
-      (void)wave_rank(h, valid ? part_digit(a, k[e]) : 0u, valid, a.bits);
-    }
-  }
-  __syncthreads();
-  if (tid < a.fan) a.offs[(uint64_t)t * a.fan + tid] = h[tid];
-}
-
-// Digit d's counts over the tiles of scan group g, in tile order, into
-// output offsets: part_off of the digit's bucket plus the rows of that
-// digit in the group's tiles before.  Block (d, g).
-__global__ __launch_bounds__(256) void k_part_offs(PartLevel a, uint32_t n_tiles) {
-  __shared__ uint32_t s_w[256 / WAVE];
-  __shared__ uint32_t s_carry;
-  const uint32_t d = blockIdx.x, g = blockIdx.y;
-  const uint32_t bkt = a.first ? d : (g << a.bits) | d;
-  if (bkt >= a.nb) return;
-  const uint32_t t0 = a.first ? 0u : a.tile_first[g];
-  const uint32_t t1 = a.first ? n_tiles : a.tile_first[g + 1];
-  if (threadIdx.x == 0) s_carry = a.part_off[bkt << a.ushift];
-  __syncthreads();
-  for (uint32_t c = t0; c < t1; c += blockDim.x) {
-    const uint32_t t = c + threadIdx.x;
-    const uint32_t v = t < t1 ? a.offs[(uint64_t)t * a.fan + d] : 0u;
-    const uint32_t carry = s_carry;
-    const uint32_t ex = block_scan_excl(v, s_w);
-    if (t < t1) a.offs[(uint64_t)t * a.fan + d] = carry + ex;
-    if (threadIdx.x == blockDim.x - 1) s_carry = carry + ex + v;
-    __syncthreads();
-  }
-}
-
-// NPL: compile-time bound on the planes (registers hold a chunk's planes,
-// loaded together before the ranking).  Chunk by chunk, a tile ranks its
-// rows per digit in LDS, stages every plane in digit order and writes it
-// with consecutive threads on consecutive addresses (runs of one bucket).
-template <int NPL>
-__global__ __launch_bounds__(kScatThreads) void k_part_scatter(PartLevel a) {
-  __shared__ uint32_t cur[kMaxFan];   // next output row of each digit
-  __shared__ uint32_t hist[kMaxFan];  // rows of each digit in the chunk
-  __shared__ uint32_t loff[kMaxFan];  // their first staged position
+// PW: payload words (0 to 4: one vector access per record; kMaxPay: any
+// count up to it, word by word, four at a time).
+template <int PW>
+__global__ __launch_bounds__(kPartThreads) void k_part_scatter(PartLevel a) {
+  constexpr bool kAny = PW > 4;
+  constexpr int SW = kAny ? 4 : (PW > 0 ? PW : 1);  // payload words staged at a time
+  constexpr int R = kPartChunk / kPartThreads;
+  static_assert(kPartLocal == kPartThreads, "one local bucket per thread");
+  __shared__ uint32_t hist[kPartLocal];
+  __shared__ uint32_t res[kPartLocal];
   __shared__ uint32_t dest[kPartChunk];
-  __shared__ uint32_t stage[kPartChunk];
-  __shared__ uint32_t s_w[kScatThreads / WAVE];
-  __shared__ uint64_t s_r0;
-  __shared__ uint32_t s_n, s_ok;
-  constexpr int R = kPartChunk / kScatThreads;
-  const uint32_t t = blockIdx.x, tid = threadIdx.x;
-  if (tid == 0) {
-    uint64_t r0 = 0;
-    uint32_t n = 0, g;
-    s_ok = part_tile(a, t, &r0, &n, &g) ? 1u : 0u;
-    s_r0 = r0;
-    s_n = n;
+  __shared__ uint32_t stage[SW][kPartChunk];
+  __shared__ uint32_t s_w[kPartThreads / WAVE];
+  __shared__ uint32_t s_tot;
+  uint64_t c0, cn;
+  if (a.seg_n) {
+    const uint32_t seg = blockIdx.x / a.chunks_per_seg, c = blockIdx.x % a.chunks_per_seg;
+    const uint64_t n = a.seg_n[seg], first = (uint64_t)c * kPartChunk;
+    if (first >= n) return;
+    c0 = (uint64_t)seg * a.seg_n[a.n_seg] + first;
+    cn = n - first < (uint64_t)kPartChunk ? n - first : (uint64_t)kPartChunk;
+  } else {
+    c0 = (uint64_t)blockIdx.x * kPartChunk;
+    if (c0 >= a.n_in) return;
+    cn = a.n_in - c0 < (uint64_t)kPartChunk ? a.n_in - c0 : (uint64_t)kPartChunk;
+  }
+  const uint32_t tid = threadIdx.x;
+  const uint32_t base = a.sorted ? ((a.in_key[c0] >> a.shift) >> a.coarse) << a.coarse : 0u;
+  hist[tid] = 0;
+  uint32_t key[R], loc[R], rank[R];
+  PayRec<SW> pay[R];  // fixed payloads: the whole record, loaded with the keys
+#pragma unroll
+  for (int e = 0; e < R; ++e) {
+    const uint32_t i = e * kPartThreads + tid;
+    key[e] = i < cn ? a.in_key[c0 + i] : 0u;
+    if constexpr (!kAny && PW > 0)
+      if (i < cn) pay[e] = ((const PayRec<SW>*)a.in_pay)[c0 + i];
   }
   __syncthreads();
-  if (!s_ok) return;
-  const uint64_t r0 = s_r0;
-  const uint32_t n = s_n;
-  if (tid < a.fan) cur[tid] = a.offs[(uint64_t)t * a.fan + tid];
-  for (uint32_t c = 0; c < n; c += kPartChunk) {
-    const uint32_t cn = n - c < (uint32_t)kPartChunk ? n - c : (uint32_t)kPartChunk;
-    uint32_t pv[NPL][R], dg[R], rank[R];
 #pragma unroll
-    for (int p = 0; p < NPL; ++p)
+  for (int e = 0; e < R; ++e) {
+    const uint32_t i = e * kPartThreads + tid;
+    loc[e] = (key[e] >> a.shift) - base;
+    const bool local = i < cn && loc[e] < (uint32_t)kPartLocal;
+    rank[e] = wave_rank(hist, local ? loc[e] : 0u, local, kLocalBits);
+    if (!local) rank[e] = ~0u;
+  }
+  __syncthreads();
+  const uint32_t hc = hist[tid];
+  const uint32_t off = block_scan_excl(hc, s_w);
+  if (hc) res[tid] = a.part_off[(base + tid) << a.ushift] + atomicAdd(&a.used[base + tid], hc);
+  hist[tid] = off;
+  if (tid == kPartThreads - 1) s_tot = off + hc;
+  __syncthreads();
+  uint32_t si[R];
+#pragma unroll
+  for (int e = 0; e < R; ++e) {
+    const uint32_t i = e * kPartThreads + tid;
+    si[e] = ~0u;
+    if (i >= cn) continue;
+    if (rank[e] != ~0u) {
+      si[e] = hist[loc[e]] + rank[e];
+      dest[si[e]] = res[loc[e]] + rank[e];
+    } else {
+      const uint32_t bk = key[e] >> a.shift;
+      const uint32_t d = a.part_off[bk << a.ushift] + atomicAdd(&a.used[bk], 1u);
+      a.out_key[d] = key[e];
+      for (uint32_t w = 0; w < a.pw; ++w)
+        a.out_pay[(uint64_t)d * a.pw + w] = a.in_pay[(c0 + i) * a.pw + w];
+    }
+  }
+  const uint32_t nst = s_tot;  // written before the barrier above
+#pragma unroll
+  for (int e = 0; e < R; ++e)
+    if (si[e] != ~0u) stage[0][si[e]] = key[e];
+  __syncthreads();
+  for (uint32_t i = tid; i < nst; i += kPartThreads) a.out_key[dest[i]] = stage[0][i];
+  if constexpr (!kAny && PW > 0) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < R; ++e)
+      if (si[e] != ~0u)
+#pragma unroll
+        for (int w = 0; w < PW; ++w) stage[w][si[e]] = pay[e].w[w];
+    __syncthreads();
+    for (uint32_t i = tid; i < nst; i += kPartThreads) {
+      PayRec<PW> r;
+#pragma unroll
+      for (int w = 0; w < PW; ++w) r.w[w] = stage[w][i];
+      ((PayRec<PW>*)a.out_pay)[dest[i]] = r;
+    }
+  } else if constexpr (kAny) {
+    for (uint32_t w0 = 0; w0 < a.pw; w0 += SW) {
+      const uint32_t nw = a.pw - w0 < (uint32_t)SW ? a.pw - w0 : (uint32_t)SW;
+      __syncthreads();
 #pragma unroll
       for (int e = 0; e < R; ++e) {
-        const uint32_t i = e * kScatThreads + tid;
-        pv[p][e] = (i < cn && (uint32_t)p < a.n_planes) ? a.in[p][r0 + c + i] : 0u;
+        if (si[e] == ~0u) continue;
+        const uint64_t row = c0 + e * kPartThreads + tid;
+        for (uint32_t w = 0; w < nw; ++w) stage[w][si[e]] = a.in_pay[row * a.pw + w0 + w];
       }
-    if (tid < a.fan) hist[tid] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < R; ++e) {
-      const bool valid = e * kScatThreads + tid < cn;
-      dg[e] = valid ? part_digit(a, pv[0][e]) : 0u;
-      rank[e] = wave_rank(hist, dg[e], valid, a.bits);
-    }
-    __syncthreads();
-    const uint32_t hc = tid < a.fan ? hist[tid] : 0u;
-    const uint32_t ex = block_scan_excl(hc, s_w);
-    if (tid < a.fan) loff[tid] = ex;
-    __syncthreads();
-    uint32_t si[R];
-#pragma unroll
-    for (int e = 0; e < R; ++e) {
-      si[e] = ~0u;
-      if (e * kScatThreads + tid >= cn) continue;
-      si[e] = loff[dg[e]] + rank[e];
-      dest[si[e]] = cur[dg[e]] + rank[e];
-    }
-    __syncthreads();
-    if (tid < a.fan) cur[tid] += hc;
-#pragma unroll
-    for (int p = 0; p < NPL; ++p) {
-      if ((uint32_t)p >= a.n_planes) break;
-      if (p) __syncthreads();  // the previous plane's stage is read out
-#pragma unroll
-      for (int e = 0; e < R; ++e)
-        if (si[e] != ~0u) stage[si[e]] = pv[p][e];
       __syncthreads();
-      uint32_t* out = a.out[p];
-      for (uint32_t i = tid; i < cn; i += kScatThreads) out[dest[i]] = stage[i];
+      for (uint32_t i = tid; i < nst; i += kPartThreads)
+        for (uint32_t w = 0; w < nw; ++w) a.out_pay[(uint64_t)dest[i] * a.pw + w0 + w] = stage[w][i];
     }
-    __syncthreads();  // dest and stage are rewritten by the next chunk
   }
 }
 
-struct PartPlanes {
-  const uint32_t* p[kMaxPlanes];
-};
 template <int NS>
-__global__ __launch_bounds__(kPartAggThreads) void k_part_agg(GroupArgs a, PartPlanes pl,
+__global__ __launch_bounds__(kPartAggThreads) void k_part_agg(GroupArgs a, const uint32_t* pkey,
+                                                              const uint32_t* ppay,
                                                               const uint32_t* part_off,
                                                               const uint32_t* coff, uint32_t np,
                                                               uint32_t ps, uint64_t* xkey,
@@ -1057,27 +1024,36 @@ __global__ __launch_bounds__(kPartAggThreads) void k_part_agg(GroupArgs a, PartP
   const uint32_t b = part_off[p] + c * kPartAggRows;
   const uint32_t e = pe - b < kPartAggRows ? pe : b + kPartAggRows;
   constexpr int R = 4;
-  for (uint32_t i0 = b; i0 < e; i0 += R * blockDim.x) {
-    int k[R];
-    uint64_t v[R][NS];
-    uint32_t lo[R][NS], hi[R][NS];
+  // software pipeline: the next round's keys and payload words are loaded
+  // while this round updates the LDS copy (rows past the end read row b)
+  uint32_t fk[R], nlo[R][NS], nhi[R][NS];
+  auto fetch = [&](uint32_t i0) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint32_t i = i0 + r * blockDim.x + threadIdx.x;
-      const uint32_t ii = i < e ? i : b;  // rows past the end read row b
-      k[r] = i < e ? (int)(pl.p[0][ii] - (uint32_t)g0) : -1;
+      const uint32_t ii = i < e ? i : b;
+      fk[r] = pkey[ii];
+      const uint32_t* pr = ppay + (uint64_t)ii * a.n_pay;
 #pragma unroll
-      for (uint32_t j = 0; j < (uint32_t)NS; ++j) {  // unused sources read plane 0
-        lo[r][j] = pl.p[a.plane_lo[j]][ii];
-        hi[r][j] = pl.p[a.plane_hi[j]][ii];
+      for (uint32_t j = 0; j < (uint32_t)NS; ++j) {  // unused sources read word 0
+        nlo[r][j] = a.n_pay ? pr[a.pay_lo[j]] : 0u;
+        nhi[r][j] = a.n_pay ? pr[a.pay_hi[j]] : 0u;
       }
     }
+  };
+  if (b < e) fetch(b);
+  for (uint32_t i0 = b; i0 < e; i0 += R * blockDim.x) {
+    int k[R];
+    uint64_t v[R][NS];
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+    for (int r = 0; r < R; ++r) {
+      k[r] = i0 + r * blockDim.x + threadIdx.x < e ? (int)(fk[r] - (uint32_t)g0) : -1;
 #pragma unroll
       for (uint32_t j = 0; j < (uint32_t)NS; ++j)
-        v[r][j] = a.plane_hi[j] != a.plane_lo[j] ? ((uint64_t)hi[r][j] << 32 | lo[r][j])
-                                                  : widen_src(a, j, lo[r][j]);
+        v[r][j] = a.pay_hi[j] != a.pay_lo[j] ? ((uint64_t)nhi[r][j] << 32 | nlo[r][j])
+                                              : widen_src(a, j, nlo[r][j]);
+    }
+    if (i0 + R * blockDim.x < e) fetch(i0 + R * blockDim.x);
     cache_update<NS, R, true>(a, cst, ws, k, v);  // rows past the end: spare column
   }
   __syncthreads();
@@ -1093,9 +1069,43 @@ __global__ __launch_bounds__(kPartAggThreads) void k_part_agg(GroupArgs a, PartP
   }
 }
 
-// Combine partial state q (word k at q[k]) into the register copy r.
-__device__ __forceinline__ void combine_state(const GroupArgs& a, uint64_t (&r)[kMaxW],
-                                              const uint64_t* q) {
+// The staged LDS caches of the spill mode, by cache set: block (s, y)
+// takes the ways of set s in stage blocks [32y, 32y + 32), one entry per
+// thread, and combines them in an LDS table of their distinct keys (a hot
+// key sits in the same set of every block, not always in the same way);
+// each key is then merged into the table once per block.
+constexpr uint32_t kSetMergeStages = 32;
+__global__ __launch_bounds__(256) void k_hot_set_merge(GroupArgs a, uint32_t n_stages,
+                                                       uint32_t cs) {
+  static_assert(kSetMergeStages * kWays == 256, "one entry per thread");
+  extern __shared__ uint64_t sm[];
+  uint64_t* tk = sm;        // 256 keys
+  uint64_t* ts = sm + 256;  // their states, W words each (slot-major)
+  const uint32_t W = a.W, t = threadIdx.x;
+  tk[t] = GEMPTY;
+  for (uint32_t w = 0; w < W; ++w) ts[(uint64_t)t * W + w] = a.init[w];
+  const uint32_t stage = blockIdx.y * kSetMergeStages + t / kWays;
+  const uint64_t slot = (uint64_t)stage * cs + blockIdx.x * kWays + t % kWays;
+  const uint64_t key = stage < n_stages ? a.hot_key[slot] : GEMPTY;
+  __syncthreads();
+  if (key != GEMPTY) {
+    uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 56);
+    for (;; h = (h + 1) & 255u) {  // 256 slots for at most 256 keys: ends
+      const unsigned long long old =
+          atomicCAS((unsigned long long*)&tk[h], GEMPTY, (unsigned long long)key);
+      if (old == GEMPTY || old == key) break;
+    }
+    merge_state(a, ts + (uint64_t)h * W, a.hot_st + slot * W, 1);
+  }
+  __syncthreads();
+  if (tk[t] != GEMPTY) merge_state(a, a.st + tk[t] * W, ts + (uint64_t)t * W, 1);
+}
+
+// Combine partial state q into the register copy r (word k in r[k], q[k];
+// WM: compile-time bound on the state words).
+template <int WM>
+__device__ __forceinline__ void combine_state(const GroupArgs& a, uint64_t (&r)[WM],
+                                              const uint64_t (&q)[WM]) {
   r[0] += q[0];
 #pragma unroll
   for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i) {
@@ -1106,7 +1116,7 @@ __device__ __forceinline__ void combine_state(const GroupArgs& a, uint64_t (&r)[
     // registers are indexed by a runtime word: select over the (unrolled)
     // candidates instead of indexing the array dynamically
 #pragma unroll
-    for (uint32_t k = 1; k < (uint32_t)kMaxW; ++k) {
+    for (uint32_t k = 1; k < (uint32_t)WM; ++k) {
       if (k == w) {
         switch (sl.kind) {
           case K_SUMI: r[k] += q[k]; break;
@@ -1127,8 +1137,9 @@ __device__ __forceinline__ void combine_state(const GroupArgs& a, uint64_t (&r)[
 }
 
 // merge_state from a register copy (word k in r[k]).
+template <int WM>
 __device__ __forceinline__ void merge_regs(const GroupArgs& a, uint64_t* g,
-                                           const uint64_t (&r)[kMaxW]) {
+                                           const uint64_t (&r)[WM]) {
   atomicAdd((unsigned long long*)&g[0], (unsigned long long)r[0]);
 #pragma unroll
   for (uint32_t i = 0; i < MTL_GROUP_MAX_AGGS; ++i) {
@@ -1137,7 +1148,7 @@ __device__ __forceinline__ void merge_regs(const GroupArgs& a, uint64_t* g,
     if (sl.dup || sl.kind == K_COUNT) continue;
     uint64_t v0 = 0, v1 = 0;
 #pragma unroll
-    for (uint32_t k = 1; k < (uint32_t)kMaxW; ++k) {
+    for (uint32_t k = 1; k < (uint32_t)WM; ++k) {
       if (k == sl.word) v0 = r[k];
       if (k == sl.word + 1) v1 = r[k];
     }
@@ -1163,8 +1174,11 @@ __device__ __forceinline__ void merge_regs(const GroupArgs& a, uint64_t* g,
 // slot column: runs of one key down the column are combined in registers
 // and merged with one set of device atomics, so a group staged by many
 // blocks (a hot key in every LDS cache, a large partition's extra blocks)
-// costs G times fewer contended atomics.  With coff set, only stage blocks
-// below coff[np] - np are live.
+// costs G times fewer contended atomics.  The next stage's key and words
+// are loaded before the current one is combined or merged.  With coff set,
+// only stage blocks below coff[np] - np are live.  WM: compile-time bound
+// on the state words.
+template <int WM>
 __global__ __launch_bounds__(256) void k_stage_reduce(GroupArgs a, const uint64_t* skey,
                                                       const uint64_t* sst, uint64_t n_stages,
                                                       uint32_t ks, uint32_t G,
@@ -1176,22 +1190,32 @@ __global__ __launch_bounds__(256) void k_stage_reduce(GroupArgs a, const uint64_
   const uint32_t W = a.W;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ks; i += gridDim.x * blockDim.x) {
     uint64_t cur = GEMPTY;
-    uint64_t r[kMaxW];
+    uint64_t r[WM], nq[WM];
+    uint64_t nk = skey[s0 * ks + i];
+#pragma unroll
+    for (uint32_t k = 0; k < (uint32_t)WM; ++k) nq[k] = k < W ? sst[(s0 * ks + i) * W + k] : 0ull;
     for (uint64_t s = s0; s < s1; ++s) {
-      const uint64_t slot = s * ks + i;
-      const uint64_t key = skey[slot];
+      const uint64_t key = nk;
+      uint64_t q[WM];
+#pragma unroll
+      for (uint32_t k = 0; k < (uint32_t)WM; ++k) q[k] = nq[k];
+      if (s + 1 < s1) {
+        const uint64_t slot = (s + 1) * ks + i;
+        nk = skey[slot];
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)WM; ++k) nq[k] = k < W ? sst[slot * W + k] : 0ull;
+      }
       if (key == GEMPTY) continue;
-      const uint64_t* q = sst + slot * W;
       if (key != cur) {
-        if (cur != GEMPTY) merge_regs(a, a.st + cur * W, r);
+        if (cur != GEMPTY) merge_regs<WM>(a, a.st + cur * W, r);
         cur = key;
 #pragma unroll
-        for (uint32_t k = 0; k < (uint32_t)kMaxW; ++k) r[k] = k < W ? q[k] : 0ull;
+        for (uint32_t k = 0; k < (uint32_t)WM; ++k) r[k] = q[k];
       } else {
-        combine_state(a, r, q);
+        combine_state<WM>(a, r, q);
       }
     }
-    if (cur != GEMPTY) merge_regs(a, a.st + cur * W, r);
+    if (cur != GEMPTY) merge_regs<WM>(a, a.st + cur * W, r);
   }
 }
 
@@ -1631,19 +1655,23 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
     }
     sl.src = j;
   }
-  // spill planes: the key, then one or two per source (see GroupArgs)
-  a->n_planes = 1;
+  // payload words of a spilled row: one or two per source (see GroupArgs)
+  a->n_pay = 0;
   for (uint32_t j = 0; j < MTL_GROUP_MAX_AGGS; ++j) {
-    a->plane_lo[j] = a->plane_hi[j] = 0;
+    a->pay_lo[j] = a->pay_hi[j] = 0;
     if (j >= a->n_src) continue;
-    a->plane_lo[j] = a->plane_hi[j] = a->n_planes++;
-    if (type_width(a->src_type[j]) == 8) a->plane_hi[j] = a->n_planes++;
+    a->pay_lo[j] = a->pay_hi[j] = a->n_pay++;
+    if (type_width(a->src_type[j]) == 8) a->pay_hi[j] = a->n_pay++;
   }
   a->n_ops = 0;
-  for (uint32_t i = 0; i < d->n_aggs; ++i) {
-    const AggSlot& sl = a->aggs[i];
-    if (sl.dup || sl.kind == K_COUNT) continue;
-    a->ops[a->n_ops++] = sl.kind | sl.word << 4 | sl.src << 12;
+  for (uint32_t j = 0; j <= MTL_GROUP_MAX_AGGS; ++j) {
+    a->op_beg[j] = a->n_ops;
+    if (j == MTL_GROUP_MAX_AGGS) break;
+    for (uint32_t i = 0; i < d->n_aggs; ++i) {
+      const AggSlot& sl = a->aggs[i];
+      if (sl.dup || sl.kind == K_COUNT || sl.src != j) continue;
+      a->ops[a->n_ops++] = sl.kind | sl.word << 4 | sl.src << 12;
+    }
   }
   a->key_w0 = (uint32_t)type_width(a->key_type[0]);
   for (uint32_t j = 0; j < MTL_GROUP_MAX_AGGS; ++j) {
@@ -1735,127 +1763,108 @@ static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t seg_max, u
     ush[l] = pb > (l + 1) * lb ? pb - (l + 1) * lb : 0;
     nb[l] = (np + (1ull << ush[l]) - 1) >> ush[l];
   }
-  // tiles of each level (first: the spill regions'; later: a bound) and
-  // their digit offsets
-  const uint32_t tiles_per_seg = (uint32_t)ceil_div(seg_max, kPartTile);
-  uint64_t offs_words = (uint64_t)agg_grid * tiles_per_seg * nb[0], max_groups = 1;
-  for (uint32_t l = 1; l < levels; ++l) {
-    offs_words = std::max<uint64_t>(offs_words, (ceil_div(total, kPartTile) + nb[l - 1]) *
-                                                    (1ull << (ush[l - 1] - ush[l])));
-    max_groups = std::max<uint64_t>(max_groups, nb[l - 1]);
-  }
+  uint64_t used_words = 0;
+  for (uint32_t l = 0; l < levels; ++l) used_words += nb[l];
   // aggregation blocks beyond the first of each partition (an upper bound)
   // and their staged states
   const uint64_t ks = 1ull << ps, extra = ceil_div(total, kPartAggRows);
-  void *cnt, *off, *coff, *offs, *tfirst, *xk = nullptr, *xs = nullptr;
+  void *cnt, *off, *coff, *used, *xk = nullptr, *xs = nullptr, *bk = nullptr, *bp = nullptr;
   MTL_TRY(ar.alloc((uint64_t)np * 4, &cnt));
   MTL_TRY(ar.alloc((uint64_t)np * 4 + 4, &off));
   MTL_TRY(ar.alloc((uint64_t)np * 4 + 4, &coff));
-  MTL_TRY(ar.alloc(offs_words * 4, &offs));
-  MTL_TRY(ar.alloc(max_groups * 4 + 4, &tfirst));
+  MTL_TRY(ar.alloc(used_words * 4, &used));
   if (extra) {
     MTL_TRY(ar.alloc(extra * ks * 8, &xk));
     MTL_TRY(ar.alloc(extra * ks * a.W * 8, &xs));
   }
-  // the second set of planes (levels alternate between it and the spill
-  // planes, contiguous from the second level on)
-  uint32_t* pb_[kMaxPlanes] = {};
-  for (uint32_t q = 0; total && q < a.n_planes; ++q) {
-    void* m;
-    MTL_TRY(ar.alloc(total * 4, &m));
-    pb_[q] = (uint32_t*)m;
+  // the second key plane and payload (levels alternate between them and the
+  // spill arrays, contiguous from the second level on)
+  if (total) {
+    MTL_TRY(ar.alloc(total * 4, &bk));
+    MTL_TRY(ar.alloc(total * 4 * std::max<uint32_t>(a.n_pay, 1), &bp));
   }
   ZeroList zl;
   zl.add((uint32_t*)cnt, np);
+  zl.add((uint32_t*)used, used_words);
   MTL_TRY(launch_zero(zl, st));
   if (total) {
     ProfScope p1(s, "k_part_hist");
     hipLaunchKernelGGL(k_part_hist, dim3(agg_grid), dim3(kPartThreads),
-                       np <= kPartHistLds ? (size_t)np * 4 : 0, st, a.spill_pl[0], a.spill_n,
+                       np <= kPartHistLds ? (size_t)np * 4 : 0, st, a.spill_key, a.spill_n,
                        ps, np, (uint32_t*)cnt);
     MTL_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kPartThreads), 0, st, (const uint32_t*)cnt, np,
                      (uint32_t*)off, (uint32_t*)coff);
   MTL_HIP(hipGetLastError());
-  PartPlanes fin{};
-  for (uint32_t q = 0; q < a.n_planes; ++q) fin.p[q] = a.spill_pl[q];
+  const uint32_t* fk = a.spill_key;
+  const uint32_t* fp = a.spill_pay;
+  uint32_t* used_l = (uint32_t*)used;
   for (uint32_t l = 0; total && l < levels; ++l) {
     PartLevel lv{};
     const bool to_b = (l % 2) == 0;
-    for (uint32_t q = 0; q < a.n_planes; ++q) {
-      lv.in[q] = fin.p[q];
-      lv.out[q] = to_b ? pb_[q] : a.spill_pl[q];
-    }
-    lv.n_planes = a.n_planes;
+    lv.in_key = fk;
+    lv.in_pay = fp;
+    lv.out_key = to_b ? (uint32_t*)bk : a.spill_key;
+    lv.out_pay = to_b ? (uint32_t*)bp : a.spill_pay;
+    lv.pw = a.n_pay;
+    lv.n_in = total;
     lv.part_off = (const uint32_t*)off;
-    lv.np = np;
-    lv.offs = (uint32_t*)offs;
+    lv.used = used_l;
     lv.shift = ps + ush[l];
     lv.ushift = ush[l];
-    lv.nb = (uint32_t)nb[l];
-    uint64_t tiles;
-    uint32_t groups;
+    lv.coarse = l ? ush[l - 1] - ush[l] : 0;
+    lv.sorted = l ? 1u : 0u;
+    uint64_t grid;
     if (l == 0) {
-      lv.first = 1;
       lv.seg_n = a.spill_n;
       lv.n_seg = agg_grid;
-      lv.tiles_per_seg = tiles_per_seg;
-      lv.fan = (uint32_t)nb[0];
-      lv.bits = 0;
-      while ((1u << lv.bits) < lv.fan) ++lv.bits;
-      tiles = (uint64_t)agg_grid * tiles_per_seg;
-      groups = 1;
+      lv.chunks_per_seg = (uint32_t)ceil_div(seg_max, kPartChunk);
+      grid = (uint64_t)agg_grid * lv.chunks_per_seg;
     } else {
-      lv.first = 0;
-      lv.uprev = ush[l - 1];
-      lv.bits = ush[l - 1] - ush[l];
-      lv.fan = 1u << lv.bits;
-      lv.n_groups = (uint32_t)nb[l - 1];
-      lv.tile_first = (const uint32_t*)tfirst;
-      hipLaunchKernelGGL(k_part_tiles, dim3(1), dim3(kPartThreads), 0, st, (const uint32_t*)off,
-                         np, ush[l - 1], lv.n_groups, (uint32_t*)tfirst);
-      MTL_HIP(hipGetLastError());
-      tiles = ceil_div(total, kPartTile) + lv.n_groups;
-      groups = lv.n_groups;
+      grid = ceil_div(total, kPartChunk);
     }
     {
       ProfScope p2(s, "k_part_scatter");
-      hipLaunchKernelGGL(k_part_count, dim3((uint32_t)tiles), dim3(256), 0, st, lv);
-      MTL_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_part_offs, dim3(lv.fan, groups), dim3(256), 0, st, lv,
-                         (uint32_t)tiles);
-      MTL_HIP(hipGetLastError());
-      const dim3 gd((uint32_t)tiles), bd(kScatThreads);
-      if (a.n_planes <= 5) hipLaunchKernelGGL(k_part_scatter<5>, gd, bd, 0, st, lv);
-      else if (a.n_planes <= 9) hipLaunchKernelGGL(k_part_scatter<9>, gd, bd, 0, st, lv);
-      else hipLaunchKernelGGL(k_part_scatter<kMaxPlanes>, gd, bd, 0, st, lv);
+      const dim3 gd((uint32_t)grid), bd(kPartThreads);
+      switch (a.n_pay) {
+        case 0: hipLaunchKernelGGL(k_part_scatter<0>, gd, bd, 0, st, lv); break;
+        case 1: hipLaunchKernelGGL(k_part_scatter<1>, gd, bd, 0, st, lv); break;
+        case 2: hipLaunchKernelGGL(k_part_scatter<2>, gd, bd, 0, st, lv); break;
+        case 3: hipLaunchKernelGGL(k_part_scatter<3>, gd, bd, 0, st, lv); break;
+        case 4: hipLaunchKernelGGL(k_part_scatter<4>, gd, bd, 0, st, lv); break;
+        default: hipLaunchKernelGGL(k_part_scatter<kMaxPay>, gd, bd, 0, st, lv); break;
+      }
       MTL_HIP(hipGetLastError());
     }
-    for (uint32_t q = 0; q < a.n_planes; ++q) fin.p[q] = lv.out[q];
+    fk = lv.out_key;
+    fp = lv.out_pay;
+    used_l += nb[l];
   }
   {
     ProfScope p3(s, "k_part_agg");
     hipLaunchKernelGGL(k_part_agg<NS>, dim3((uint32_t)(np + extra)), dim3(kPartAggThreads),
-                       (size_t)(ks + 1) * a.W * 8, st, a, fin, (const uint32_t*)off,
+                       (size_t)(ks + 1) * a.W * 8, st, a, fk, fp, (const uint32_t*)off,
                        (const uint32_t*)coff, np, ps, (uint64_t*)xk, (uint64_t*)xs);
     MTL_HIP(hipGetLastError());
   }
   {
     ProfScope p4(s, "k_stage_reduce");
     const uint32_t cs = a.cache_slots;
-    // hot caches: the same head keys in every block, so long columns
-    const uint32_t gh = 64, gx = 16;
-    hipLaunchKernelGGL(k_stage_reduce,
-                       dim3((uint32_t)ceil_div(cs, 256), (uint32_t)ceil_div(agg_grid, gh)),
-                       dim3(256), 0, st, a, a.hot_key, a.hot_st, (uint64_t)agg_grid, cs, gh,
-                       nullptr, 0u);
+    // extra blocks of large partitions: groups of 16 stage blocks
+    const uint32_t gx = 16;
+    auto reduce = [&](dim3 gd, const uint64_t* k, const uint64_t* v, uint64_t n, uint32_t w,
+                      uint32_t g, const uint32_t* co) {
+      if (a.W <= 8) hipLaunchKernelGGL(k_stage_reduce<8>, gd, dim3(256), 0, st, a, k, v, n, w, g, co, np);
+      else hipLaunchKernelGGL(k_stage_reduce<kMaxW>, gd, dim3(256), 0, st, a, k, v, n, w, g, co, np);
+    };
+    hipLaunchKernelGGL(k_hot_set_merge,
+                       dim3(cs / kWays, (uint32_t)ceil_div(agg_grid, kSetMergeStages)), dim3(256),
+                       (size_t)256 * 8 * (1 + a.W), st, a, agg_grid, cs);
     MTL_HIP(hipGetLastError());
     if (extra) {
-      hipLaunchKernelGGL(k_stage_reduce,
-                         dim3((uint32_t)ceil_div(ks, 256), (uint32_t)ceil_div(extra, gx)),
-                         dim3(256), 0, st, a, (const uint64_t*)xk, (const uint64_t*)xs, extra,
-                         (uint32_t)ks, gx, (const uint32_t*)coff, np);
+      reduce(dim3((uint32_t)ceil_div(ks, 256), (uint32_t)ceil_div(extra, gx)), (const uint64_t*)xk,
+             (const uint64_t*)xs, extra, (uint32_t)ks, gx, (const uint32_t*)coff);
       MTL_HIP(hipGetLastError());
     }
   }
@@ -1874,7 +1883,7 @@ uint64_t group_scratch_bytes(const mtl_group_desc& d) {
   // the extra aggregation blocks of large partitions (<= 64 KB per block)
   const uint64_t hot = (uint64_t)cu_count() * 2 * a.cache_slots * (1 + a.W) * 8 +
                        ceil_div(rows, kPartAggRows) * (kPartAggLds + kPartAggLds / a.W);
-  return entries * (a.W + (a.dense ? 0 : 1)) * 8 + rows * 4 * (uint64_t)a.n_planes * 2 + hot +
+  return entries * (a.W + (a.dense ? 0 : 1)) * 8 + rows * 4 * (1 + (uint64_t)a.n_pay) * 2 + hot +
          entries * (16 + 8 * (uint64_t)a.n_aggs) + (64u << 20);
 }
 
@@ -1927,17 +1936,19 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   s->arena.reset();
   void* err;
   MTL_TRY(s->arena.alloc(256, &err));
-  ZeroList zl;
-  zl.add((uint32_t*)err, 64);
-  MTL_TRY(launch_zero(zl, s->stream));
   a.st = (uint64_t*)g->st_mem;
   a.tkeys = (uint64_t*)g->keys_mem;
   a.err = (uint32_t*)err;
   uint64_t rows = desc->sel ? desc->sel_capacity : desc->n_rows;
   // spill mode (a directly indexed domain too large for the LDS cache):
   // the partitioned aggregation of the spilled rows writes every entry
-  const bool spill = rows && a.dense && !a.cache_direct && rows >= kSpillMinRows;
-  if (!spill) {
+  const bool spill = rows && a.dense && !a.cache_direct && rows >= kSpillMinRows &&
+                     a.cache_slots >= 16 * kWays;
+  if (spill) {
+    ZeroList zl;
+    zl.add((uint32_t*)err, 64);
+    MTL_TRY(launch_zero(zl, s->stream));
+  } else {  // k_group_init also clears the error words
     ProfScope ps(s, "k_group_init");
     uint64_t blocks = std::min<uint64_t>(ceil_div(entries, 256), 8192);
     hipLaunchKernelGGL(k_group_init, dim3((uint32_t)std::max<uint64_t>(blocks, 1)), dim3(256),
@@ -1952,17 +1963,19 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
     // 160 KB of a CU; at most 2 blocks of 768 threads), so every block gets
     // an equal share of the rows and none waits for a second round
     const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(2, (160 * 1024) / std::max<size_t>(lds, 1)));
+    // (a directly indexed cache: twice that many, run in two waves of
+    // blocks, which streams the columns faster)
     uint64_t grid = std::min<uint64_t>(ceil_div(rows, (uint64_t)kAggBlock * GE * 4),
-                                       (uint64_t)cu_count() * per_cu);
+                                       (uint64_t)cu_count() * per_cu * (a.cache_direct ? 2 : 1));
     const dim3 gd((uint32_t)std::max<uint64_t>(grid, 1));
     if (spill) {
       void *sn, *hk, *hv;
       const uint64_t hot = (uint64_t)gd.x * a.cache_slots;
-      for (uint32_t q = 0; q < a.n_planes; ++q) {
-        void* m;
-        MTL_TRY(s->arena.alloc(rows * 4, &m));
-        a.spill_pl[q] = (uint32_t*)m;
-      }
+      void *sk, *sp;
+      MTL_TRY(s->arena.alloc(rows * 4, &sk));
+      MTL_TRY(s->arena.alloc(rows * 4 * std::max<uint32_t>(a.n_pay, 1), &sp));
+      a.spill_key = (uint32_t*)sk;
+      a.spill_pay = (uint32_t*)sp;
       MTL_TRY(s->arena.alloc(gd.x * 4 + 4, &sn));
       MTL_TRY(s->arena.alloc(hot * 8, &hk));
       MTL_TRY(s->arena.alloc(hot * a.W * 8, &hv));
@@ -1988,7 +2001,7 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
       MTL_HIP(hipMemcpyAsync(hs, err, 4, hipMemcpyDeviceToHost, s->stream));
       MTL_HIP(hipMemcpyAsync((char*)hs + 8, a.spill_n, (size_t)gd.x * 4, hipMemcpyDeviceToHost,
                              s->stream));
-      MTL_HIP(hipStreamSynchronize(s->stream));
+      MTL_HIP(stream_wait(s->stream));
       uint64_t total = 0;
       for (uint32_t b = 0; b < gd.x; ++b) total += ((const uint32_t*)((char*)hs + 8))[b];
       const uint64_t seg_max = ceil_div(rows, gd.x);
@@ -2003,7 +2016,8 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   g->args = a;
   g->args.err = nullptr;  // arena memory is not kept past this call
   g->args.spill = 0;
-  for (uint32_t q = 0; q < kMaxPlanes; ++q) g->args.spill_pl[q] = nullptr;
+  g->args.spill_key = nullptr;
+  g->args.spill_pay = nullptr;
   g->args.spill_n = nullptr;
   g->args.hot_key = nullptr;
   g->args.hot_st = nullptr;
@@ -2011,7 +2025,7 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   void* host;
   MTL_TRY(stream_pinned(s, 8, &host));
   MTL_HIP(hipMemcpyAsync(host, err, 4, hipMemcpyDeviceToHost, s->stream));
-  MTL_HIP(hipStreamSynchronize(s->stream));
+  MTL_HIP(stream_wait(s->stream));
   uint32_t eb = *(uint32_t*)host;
   if (eb) {
     if (eb & 8u) return set_error(MTL_ERR_INVALID_ARGUMENT, "a group key lies outside its stated [min, max]");
@@ -2031,16 +2045,23 @@ int32_t mtl_groups_release(mtl_groups* g) {
   return MTL_OK;
 }
 
-// Emit `idx` groups (device list + device count) into host buffers.
+// Emit `idx` groups (device list + device count) into host buffers: n
+// groups, or, with n_real set, at most n, the count read back with the rows
+// in the same round trip (*n_real).
 static int32_t emit_groups(mtl_groups* g, const uint32_t* idx, const uint32_t* count,
-                           uint64_t n, int64_t* keys, void* const* agg_out) {
+                           uint64_t n, int64_t* keys, void* const* agg_out,
+                           uint32_t* n_real = nullptr) {
   mtl_stream* s = g->s;
   const GroupArgs& a = g->args;
   if (n == 0) return MTL_OK;
-  void *dk, *dc, *da;
-  MTL_TRY(s->arena.alloc(n * a.n_keys * 8, &dk));
-  MTL_TRY(s->arena.alloc(n * 8, &dc));
-  MTL_TRY(s->arena.alloc(n * std::max<uint32_t>(a.n_aggs, 1) * 16, &da));
+  // keys, counts and aggregate words in one device block, so a small result
+  // comes back in one copy through the stream's pinned staging
+  const uint64_t kb = n * a.n_keys * 8, cb = n * 8, ab = n * std::max<uint32_t>(a.n_aggs, 1) * 16;
+  void* blob;
+  MTL_TRY(s->arena.alloc(kb + cb + ab, &blob));
+  void* dk = blob;
+  void* dc = (char*)blob + kb;
+  void* da = (char*)blob + kb + cb;
   EmitArgs e{};
   e.idx = idx;
   e.count = count;
@@ -2068,11 +2089,29 @@ static int32_t emit_groups(mtl_groups* g, const uint32_t* idx, const uint32_t* c
   }
   MTL_HIP(hipGetLastError());
   std::vector<uint64_t> cnt(n), st(n * std::max<uint32_t>(a.n_aggs, 1) * 2);
-  if (keys)
-    MTL_HIP(hipMemcpyAsync(keys, dk, n * a.n_keys * 8, hipMemcpyDeviceToHost, s->stream));
-  MTL_HIP(hipMemcpyAsync(cnt.data(), dc, n * 8, hipMemcpyDeviceToHost, s->stream));
-  MTL_HIP(hipMemcpyAsync(st.data(), da, st.size() * 8, hipMemcpyDeviceToHost, s->stream));
-  MTL_HIP(hipStreamSynchronize(s->stream));
+  constexpr uint64_t kStagedMax = 16ull << 20;
+  if (kb + cb + ab <= kStagedMax) {
+    void* h0;
+    MTL_TRY(stream_pinned(s, 256 + kb + cb + ab, &h0));
+    char* h = (char*)h0 + 256;
+    if (n_real) MTL_HIP(hipMemcpyAsync(h0, count, 4, hipMemcpyDeviceToHost, s->stream));
+    MTL_HIP(hipMemcpyAsync(h, blob, kb + cb + ab, hipMemcpyDeviceToHost, s->stream));
+    MTL_HIP(stream_wait(s->stream));
+    if (n_real) {
+      *n_real = *(const uint32_t*)h0;
+      if (*n_real < n) n = *n_real;
+    }
+    if (keys) memcpy(keys, h, n * a.n_keys * 8);
+    memcpy(cnt.data(), h + kb, n * 8);
+    memcpy(st.data(), h + kb + cb, n * std::max<uint32_t>(a.n_aggs, 1) * 16);
+  } else {
+    if (n_real) return set_error(MTL_ERR_INVALID_ARGUMENT, "emission of %llu groups",
+                                 (unsigned long long)n);
+    if (keys) MTL_HIP(hipMemcpyAsync(keys, dk, kb, hipMemcpyDeviceToHost, s->stream));
+    MTL_HIP(hipMemcpyAsync(cnt.data(), dc, cb, hipMemcpyDeviceToHost, s->stream));
+    MTL_HIP(hipMemcpyAsync(st.data(), da, ab, hipMemcpyDeviceToHost, s->stream));
+    MTL_HIP(stream_wait(s->stream));
+  }
   if (!agg_out) return MTL_OK;
   for (uint32_t k = 0; k < a.n_aggs; ++k) {
     if (!agg_out[k]) continue;
@@ -2156,7 +2195,7 @@ int32_t mtl_groups_count(mtl_groups* g, uint64_t* n) {
   void* host;
   MTL_TRY(stream_pinned(g->s, 8, &host));
   MTL_HIP(hipMemcpyAsync(host, count, 4, hipMemcpyDeviceToHost, g->s->stream));
-  MTL_HIP(hipStreamSynchronize(g->s->stream));
+  MTL_HIP(stream_wait(g->s->stream));
   *n = *(uint32_t*)host;
   return MTL_OK;
 }
@@ -2171,7 +2210,7 @@ int32_t mtl_groups_fetch(mtl_groups* g, int64_t* keys, void* const* agg_out, uin
   void* host;
   MTL_TRY(stream_pinned(g->s, 8, &host));
   MTL_HIP(hipMemcpyAsync(host, count, 4, hipMemcpyDeviceToHost, g->s->stream));
-  MTL_HIP(hipStreamSynchronize(g->s->stream));
+  MTL_HIP(stream_wait(g->s->stream));
   uint64_t n = *(uint32_t*)host;
   *n_out = n;
   if (n > capacity)
@@ -2265,13 +2304,8 @@ int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, i
       in ^= 1;
     }
   }
-  void* host;
-  MTL_TRY(stream_pinned(s, 8, &host));
-  MTL_HIP(hipMemcpyAsync(host, (uint32_t*)misc + 1, 4, hipMemcpyDeviceToHost, s->stream));
-  MTL_HIP(hipStreamSynchronize(s->stream));
-  uint32_t n = *(uint32_t*)host;
-  *n_out = n;
-  return emit_groups(g, (const uint32_t*)oidx, (uint32_t*)misc + 1, n, keys, agg_out);
+  // the rows of all k candidates come back with the count in one round trip
+  return emit_groups(g, (const uint32_t*)oidx, (uint32_t*)misc + 1, k, keys, agg_out, n_out);
 }
 
 }  // extern "C"

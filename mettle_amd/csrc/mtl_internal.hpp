@@ -118,6 +118,15 @@ int32_t ensure_device();
 int32_t resolve_stream(mtl_stream* s, mtl_stream** out);
 // Pinned staging for small synchronous copies.
 int32_t stream_pinned(mtl_stream* s, size_t bytes, void** out);
+// Wait for a stream's work by polling it: a blocking synchronise can return
+// tens of microseconds after the work ends, which the synchronous entry
+// points (a query's result read back) would pay on every call.
+inline hipError_t stream_wait(hipStream_t st) {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(st);
+    if (e != hipErrorNotReady) return e;
+  }
+}
 
 // Device count of CUs (for grid sizing of persistent/top-k kernels).
 inline int cu_count() { return runtime().cu_count > 0 ? runtime().cu_count : 256; }

@@ -128,6 +128,15 @@ int32_t ensure_device() {
   rt.device_name = prop.name;
   rt.arch_name = prop.gcnArchName;
   rt.cu_count = prop.multiProcessorCount;
+  // stream-ordered allocations (group tables) stay in the device's pool when
+  // freed instead of being unmapped at the next synchronisation, so a table
+  // of hundreds of MB costs no new mapping per query; the processing budget
+  // still accounts for every live byte
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, rt.device) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   mtl_stream* s = new mtl_stream();
   e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -351,7 +360,7 @@ int32_t mtl_stream_destroy(mtl_stream* s) {
 
 int32_t mtl_stream_sync(mtl_stream* s) {
   MTL_TRY(resolve_stream(s, &s));
-  MTL_HIP(hipStreamSynchronize(s->stream));
+  MTL_HIP(stream_wait(s->stream));
   prof_collect(s);
   s->arena.release_retired();
   return MTL_OK;
@@ -402,7 +411,7 @@ int32_t mtl_memcpy_h2d(void* dst_dev, const void* src_host, uint64_t bytes,
   MTL_TRY(resolve_stream(s, &s));
   MTL_HIP(hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice,
                          s->stream));
-  MTL_HIP(hipStreamSynchronize(s->stream));
+  MTL_HIP(stream_wait(s->stream));
   return MTL_OK;
 }
 
@@ -411,7 +420,7 @@ int32_t mtl_memcpy_d2h(void* dst_host, const void* src_dev, uint64_t bytes,
   MTL_TRY(resolve_stream(s, &s));
   MTL_HIP(hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost,
                          s->stream));
-  MTL_HIP(hipStreamSynchronize(s->stream));
+  MTL_HIP(stream_wait(s->stream));
   return MTL_OK;
 }
 

@@ -89,7 +89,7 @@ extern "C" int32_t mtl_sort(const void* column, int32_t type, const uint32_t* se
     void* host;
     MTL_TRY(stream_pinned(s, 8, &host));
     MTL_HIP(hipMemcpyAsync(host, count, 4, hipMemcpyDeviceToHost, s->stream));
-    MTL_HIP(hipStreamSynchronize(s->stream));
+    MTL_HIP(stream_wait(s->stream));
     n = *(const uint32_t*)host;
     if (n > capacity) return set_error(MTL_ERR_INVALID_ARGUMENT, "count exceeds capacity");
   }

@@ -1692,7 +1692,7 @@ int32_t mtl_scalar_hook(int32_t hook, int32_t type, const void* values, uint64_t
     MTL_TRY(mtl_compare(&p, 1, n, (uint8_t*)dres, h.s));
     MTL_HIP(hipMemcpyAsync(hres, dres, n, hipMemcpyDeviceToHost, st));
   }
-  MTL_HIP(hipStreamSynchronize(st));
+  MTL_HIP(stream_wait(st));
   if (proj && hflag[0])
     return set_error(MTL_ERR_OVERFLOW, "Overflow in %s",
                      hook == MTL_HOOK_ADD ? "addition" : "multiplication");

@@ -1371,7 +1371,7 @@ int32_t mtl_star_topk(const mtl_star_desc* desc, mtl_star_row* out_rows, uint32_
   MTL_TRY(stream_pinned(s, rows_bytes + 16, &host));
   MTL_HIP(hipMemcpyAsync(host, dn, 8, hipMemcpyDeviceToHost, s->stream));
   MTL_HIP(hipMemcpyAsync((char*)host + 16, dout, rows_bytes, hipMemcpyDeviceToHost, s->stream));
-  MTL_HIP(hipStreamSynchronize(s->stream));
+  MTL_HIP(stream_wait(s->stream));
   uint32_t n = ((uint32_t*)host)[0], err = ((uint32_t*)host)[1];
   MTL_TRY(star_error(err));
   memcpy(out_rows, (char*)host + 16, (size_t)n * sizeof(mtl_star_row));
