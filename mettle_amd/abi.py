@@ -206,6 +206,14 @@ def lib() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C mettle_amd)")
+    # torch bundles its own HIP runtime. When this library's runtime was loaded
+    # first and torch's second, this library's hipGetDeviceCount reported no
+    # device on the MI355X box; loading torch first gives the order in which
+    # both runtimes see the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     i32, u32, u64, i64, vp = C.c_int32, C.c_uint32, C.c_uint64, C.c_int64, C.c_void_p
     sig = {
