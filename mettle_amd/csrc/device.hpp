@@ -77,8 +77,16 @@ __host__ __device__ __forceinline__ uint64_t fp_mix(uint64_t x) {
 }
 // Fingerprint term of output position i holding (row, value): the sum of
 // the terms over all positions (mod 2^64) depends on the order of the rows.
+// 32-bit multiply-xorshift steps (a handful of VALU operations per output
+// row, against about 45 for two 64-bit finalisers).
 __host__ __device__ __forceinline__ uint64_t fp_term(uint64_t i, uint32_t row, uint64_t v) {
-  return fp_mix((i << 32) ^ (uint64_t)row ^ 0x9e3779b97f4a7c15ull) + fp_mix(v + i);
+  uint32_t x = (uint32_t)i * 0x9E3779B1u + row;
+  x ^= (uint32_t)v * 0x85EBCA77u;
+  x ^= (uint32_t)(v >> 32) * 0xC2B2AE3Du;
+  x ^= x >> 15;
+  x *= 0x2C1B3C6Du;
+  x ^= x >> 13;
+  return x;
 }
 
 // Block-wide maxima (blockDim.x <= 1024, a multiple of 64); every thread

@@ -1426,12 +1426,24 @@ struct MQScanArgs {
 // Within a wave, lane l covers rows 64 i + l of strip i (i = 0..63), so an
 // LDS read of a strip is conflict-free and a ballot over the strip is the
 // strip's 64-bit pass-bit word, stored by lane i.
-constexpr int MQ_BLOCK = 1024;
+constexpr int MQ_BLOCK = 256;
 constexpr int MQ_WAVES = MQ_BLOCK / WAVE;
 
-// The tile's values of a column of T into LDS (rows past n read as 0).
+// The tile's values of a column of T into LDS (rows past n read as 0): a
+// full tile of an aligned column with 16-byte loads, all in flight together.
 template <typename T>
 __device__ __forceinline__ void mq_stage(T* s_val, const T* col, uint64_t t0, uint64_t n) {
+  constexpr uint32_t PER = 16 / sizeof(T);  // values per 16-byte load
+  constexpr uint32_t NV = FTILE / PER / MQ_BLOCK;  // loads per thread
+  if (t0 + FTILE <= n && ((uintptr_t)col % 16) == 0) {
+    uint4 v[NV];
+#pragma unroll
+    for (uint32_t u = 0; u < NV; ++u)
+      v[u] = ld_stream((const uint4*)(col + t0) + threadIdx.x + u * MQ_BLOCK);
+#pragma unroll
+    for (uint32_t u = 0; u < NV; ++u) ((uint4*)s_val)[threadIdx.x + u * MQ_BLOCK] = v[u];
+    return;
+  }
   for (uint32_t i = threadIdx.x; i < FTILE; i += MQ_BLOCK) {
     const uint64_t r = t0 + i;
     s_val[i] = r < n ? col[r] : T(0);
@@ -1440,7 +1452,7 @@ __device__ __forceinline__ void mq_stage(T* s_val, const T* col, uint64_t t0, ui
 
 template <typename T, uint32_t KIND>
 __global__ __launch_bounds__(MQ_BLOCK) void k_mq_filter(MQScanArgs a) {
-  __shared__ T s_val[FTILE];
+  __shared__ __align__(16) T s_val[FTILE];
   const uint32_t tile = blockIdx.x;
   const uint64_t t0 = (uint64_t)tile * FTILE;
   mq_stage<T>(s_val, (const T*)a.col, t0, a.n_max);
@@ -1509,7 +1521,7 @@ __global__ __launch_bounds__(1024) void k_mq_offsets(const MQScanQ* qs) {
 template <int PK>
 __global__ __launch_bounds__(MQ_BLOCK) void k_mq_expand(MQScanArgs a) {
   using VT = typename std::conditional<PK == 2 || PK == 4, uint64_t, uint32_t>::type;
-  __shared__ VT s_val[PK ? FTILE : 1];
+  __shared__ __align__(16) VT s_val[PK ? FTILE : 2];
   const uint32_t tile = blockIdx.x;
   const uint64_t t0 = (uint64_t)tile * FTILE;
   if constexpr (PK != 0) {
@@ -1518,29 +1530,26 @@ __global__ __launch_bounds__(MQ_BLOCK) void k_mq_expand(MQScanArgs a) {
   }
   const int wave = threadIdx.x / WAVE, lane = lane_id();
   const uint64_t lt = lanemask_lt();
-  // the pass words and output offsets of this wave's queries (at most
-  // kMQMax / MQ_WAVES), loaded together before any is used
-  constexpr int QW = (int)((kMQMax + MQ_WAVES - 1) / MQ_WAVES);
-  uint64_t words_of[QW];
-  uint32_t off_of[QW];
-#pragma unroll
-  for (int j = 0; j < QW; ++j) {
-    const uint32_t q = wave + j * MQ_WAVES;
-    words_of[j] = 0;
-    off_of[j] = 0;
-    if (q < a.nq && tile < a.q[q].tiles) {
-      words_of[j] = a.q[q].bm[(uint64_t)tile * 64 + lane];
-      off_of[j] = a.q[q].counts[tile];
-    }
+  // each wave takes every MQ_WAVES-th query; the next query's pass word and
+  // output offset are loaded before the current one is expanded
+  uint32_t q = wave;
+  uint64_t nx_words = 0;
+  uint32_t nx_off = 0;
+  if (q < a.nq && tile < a.q[q].tiles) {
+    nx_words = a.q[q].bm[(uint64_t)tile * 64 + lane];
+    nx_off = a.q[q].counts[tile];
   }
-#pragma unroll
-  for (int j = 0; j < QW; ++j) {
-    const uint32_t q = wave + j * MQ_WAVES;
-    if (q >= a.nq) break;
+#pragma unroll 1
+  for (; q < a.nq; q += MQ_WAVES) {
     const MQScanQ& Q = a.q[q];
+    const uint64_t mine = nx_words;
+    const uint32_t off = nx_off;
+    const uint32_t qn = q + MQ_WAVES;
+    if (qn < a.nq && tile < a.q[qn].tiles) {
+      nx_words = a.q[qn].bm[(uint64_t)tile * 64 + lane];
+      nx_off = a.q[qn].counts[tile];
+    }
     if (tile >= Q.tiles) continue;  // wave-uniform
-    const uint64_t mine = words_of[j];
-    const uint32_t off = off_of[j];
     const uint32_t c = (uint32_t)__popcll(mine);
     const uint32_t incl = wave_inclusive_u32(c);
     const uint32_t excl = incl - c;
@@ -1549,21 +1558,63 @@ __global__ __launch_bounds__(MQ_BLOCK) void k_mq_expand(MQScanArgs a) {
     bool ovf = false;
     uint64_t h = 0;
     const uint32_t mlo = (uint32_t)mine, mhi = (uint32_t)(mine >> 32);
-    for (int i = 0; i < 64; ++i) {
-      // strip i's word and output base from lane i (scalar reads)
-      const uint64_t w = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, i) << 32) |
-                         (uint32_t)__builtin_amdgcn_readlane((int)mlo, i);
-      if (!w) continue;  // uniform
-      const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)excl, i);
-      const uint32_t li = (uint32_t)i * 64 + lane;
-      uint64_t x = 0;
-      if constexpr (PK != 0) x = (uint64_t)s_val[li];  // before the branch: no wait inside it
-      if ((w >> lane) & 1ull) {
-        const uint32_t r = (uint32_t)(t0 + li);
-        const uint64_t o = (uint64_t)off + base + (uint32_t)__popcll(w & lt);
-        out_sel[o] = r;
-        const uint64_t vb = proj_store<PK>(pj, o, x, ovf);
-        h += fp_term(o, r, PK ? vb : r);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
+    if (total * 2 < FTILE) {
+      // fewer than half the rows pass: one output position per lane, its
+      // strip found by a binary search over the lanes' exclusive counts and
+      // its row by selecting the k-th set bit of the strip's word
+      for (uint32_t p0 = 0; p0 < total; p0 += WAVE) {
+        const uint32_t p = p0 + (uint32_t)lane;
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1) {
+          const uint32_t e = (uint32_t)__shfl((int)excl, (int)(s + step), WAVE);
+          s = e <= p ? s + step : s;
+        }
+        uint32_t k = p - (uint32_t)__shfl((int)excl, (int)s, WAVE);
+        const uint32_t wl = (uint32_t)__shfl((int)mlo, (int)s, WAVE);
+        const uint32_t wh = (uint32_t)__shfl((int)mhi, (int)s, WAVE);
+        const uint32_t cl = (uint32_t)__popc(wl);
+        uint32_t x = k < cl ? wl : wh;
+        uint32_t bit = k < cl ? 0u : 32u;
+        k = k < cl ? k : k - cl;
+#pragma unroll
+        for (uint32_t wdt = 16; wdt > 0; wdt >>= 1) {
+          const uint32_t cw = (uint32_t)__popc(x & ((1u << wdt) - 1u));
+          const bool up = k >= cw;
+          k = up ? k - cw : k;
+          x = up ? x >> wdt : x;
+          bit = up ? bit + wdt : bit;
+        }
+        const uint32_t li = s * 64 + bit;
+        uint64_t xv = 0;
+        if constexpr (PK != 0) xv = (uint64_t)s_val[li & (FTILE - 1)];
+        if (p < total) {
+          const uint32_t r = (uint32_t)(t0 + li);
+          const uint64_t o = (uint64_t)off + p;
+          out_sel[o] = r;
+          const uint64_t vb = proj_store<PK>(pj, o, xv, ovf);
+          h += fp_term(o, r, PK ? vb : r);
+        }
+      }
+    } else {
+      // half or more pass: strip by strip, every lane one row of the strip
+      for (int i = 0; i < 64; ++i) {
+        // strip i's word and output base from lane i (scalar reads)
+        const uint64_t w = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, i) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)mlo, i);
+        if (!w) continue;  // uniform
+        const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)excl, i);
+        const uint32_t li = (uint32_t)i * 64 + lane;
+        uint64_t x = 0;
+        if constexpr (PK != 0) x = (uint64_t)s_val[li];  // before the branch: no wait inside it
+        if ((w >> lane) & 1ull) {
+          const uint32_t r = (uint32_t)(t0 + li);
+          const uint64_t o = (uint64_t)off + base + (uint32_t)__popcll(w & lt);
+          out_sel[o] = r;
+          const uint64_t vb = proj_store<PK>(pj, o, x, ovf);
+          h += fp_term(o, r, PK ? vb : r);
+        }
       }
     }
     if constexpr (PK == 1 || PK == 2) {
@@ -1594,20 +1645,6 @@ __global__ __launch_bounds__(1024) void k_mq_fp_sum(const MQScanQ* qs) {
     Q.words[2] = (uint32_t)s;
     Q.words[3] = (uint32_t)(s >> 32);
   }
-}
-
-uint32_t mq_map(const uint32_t* tiles, uint32_t n, MQMap* m) {
-  *m = MQMap{};
-  m->nq = n;
-  for (uint32_t i = 0; i < n; ++i) m->order[i] = i;
-  std::stable_sort(m->order, m->order + n,
-                   [&](uint32_t x, uint32_t y) { return tiles[x] > tiles[y]; });
-  uint64_t blocks = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    m->chunks[i] = (uint32_t)ceil_div(tiles[m->order[i]], kMQChunk);
-    blocks += (uint64_t)m->chunks[i] * kMQChunk;
-  }
-  return (uint32_t)blocks;
 }
 
 bool scan_batchable(const mtl_query& q) {

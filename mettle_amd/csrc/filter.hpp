@@ -157,9 +157,6 @@ int32_t filter_project_ex(const mtl_pred* preds, uint32_t n_preds, uint64_t n_ro
 // of the resident table), operator, literal and outputs.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMQMax = dev::kMQQueries;  // queries per share group
-// The block map of a shared pass over queries of tiles[i] tiles each; the
-// grid is the return value (blocks).
-uint32_t mq_map(const uint32_t* tiles, uint32_t n, dev::MQMap* m);
 struct ScanBatchItem {
   const mtl_query* q;
   uint32_t* out_sel;  // q->n_rows row ids

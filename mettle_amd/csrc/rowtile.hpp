@@ -226,6 +226,23 @@ __device__ __forceinline__ void load_u32x16(const void* col, uint32_t width, uin
   }
 }
 
+// An empty asm statement the compiler must assume rewrites x.  Called at
+// the top of a loop over queries on a lane's 16 loaded rows, it keeps the
+// work derived from those rows inside the loop: hoisted out, the per-row
+// values (encodings, masks, addresses) would all stay live across the loop
+// and drive the kernel to the register limit.
+template <typename T>
+__device__ __forceinline__ void keep_in_loop(T& x) {
+  asm volatile("" : "+v"(x));
+}
+template <typename T>
+__device__ __forceinline__ void keep_in_loop(T (&x)[FJ][4]) {
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) keep_in_loop(x[j][v]);
+}
+
 // Raw narrow key -> offset from min, 0xFFFFFFFF when outside [0, span).
 __device__ __forceinline__ uint32_t narrow_off(uint32_t raw, int32_t type, int64_t min,
                                                uint64_t span) {
@@ -254,45 +271,7 @@ __device__ __forceinline__ void store_pass_bits(uint64_t* sel_bm, uint64_t seg, 
   }
 }
 
-// ---------------------------------------------------------------------------
-// Block -> (query, tile) of a pass shared by a batch of queries over one
-// table.  Each block does one tile of one query, like the single-query
-// kernels.  The queries are ordered by tile count, descending (order[i]
-// is the query index of rank i); tiles are taken in chunks of kMQChunk,
-// and within a chunk the blocks of every query that covers it follow one
-// another query by query.  The blocks that read one tile for different
-// queries are thus dispatched close together and, kMQChunk being a
-// multiple of the 8 XCDs that blocks are dealt to in turn, on the same
-// XCD: the tile comes from HBM once and from that XCD's L2 after.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kMQQueries = 32;  // queries per shared pass
-constexpr uint32_t kMQChunk = 64;    // tiles per chunk (a multiple of 8)
-struct MQMap {
-  uint32_t nq;
-  uint32_t chunks[kMQQueries];  // chunks of each ranked query (descending)
-  uint32_t order[kMQQueries];   // query index of each rank
-};
-
-// Blocks before chunk c: kMQChunk * sum over queries of min(c, chunks).
-__device__ __forceinline__ uint64_t mq_before(const MQMap& m, uint32_t c) {
-  uint64_t s = 0;
-  for (uint32_t i = 0; i < m.nq; ++i) s += c < m.chunks[i] ? c : m.chunks[i];
-  return s * kMQChunk;
-}
-
-// The query index and tile of block b (uniform, scalar work).
-__device__ __forceinline__ void mq_block(const MQMap& m, uint32_t b, uint32_t& q,
-                                         uint32_t& tile) {
-  uint32_t lo = 0, hi = m.chunks[0];
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (mq_before(m, mid) <= b) lo = mid;
-    else hi = mid;
-  }
-  const uint32_t r = (uint32_t)(b - mq_before(m, lo));
-  q = m.order[r / kMQChunk];
-  tile = lo * kMQChunk + r % kMQChunk;
-}
+constexpr uint32_t kMQQueries = 64;  // queries per shared scan pass
 
 }  // namespace dev
 }  // namespace mtl

@@ -229,6 +229,8 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
   std::vector<int> group(n, -1);  // share group of each shared query
   std::vector<std::vector<uint32_t>> groups;
   std::vector<int> kind;  // per group: MTL_Q_SCAN or MTL_Q_STAR
+  // per star group: one query of each distinct dimension predicate
+  std::vector<std::vector<uint32_t>> dims;
   for (uint32_t i = 0; i < n; ++i) {
     const mtl_query& q = qs[i];
     const bool scan = q.kind == MTL_Q_SCAN && scan_batchable(q);
@@ -238,15 +240,25 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
       const mtl_query& h = qs[groups[g][0]];
       const uint32_t cap = scan ? kMQMax : kStarMQMax;
       if (kind[g] != q.kind || groups[g].size() >= cap) continue;
-      if (scan ? scan_shareable(h, q) : star_shareable(h.star, q.star)) {
-        group[i] = (int)g;
-        groups[g].push_back(i);
+      if (scan) {
+        if (!scan_shareable(h, q)) continue;
+      } else {
+        if (!star_shareable(h.star, q.star)) continue;
+        bool known = false;
+        for (uint32_t r : dims[g]) known = known || star_dim_same(qs[r].star, q.star);
+        if (!known) {
+          if (dims[g].size() >= kDimSlots) continue;
+          dims[g].push_back(i);
+        }
       }
+      group[i] = (int)g;
+      groups[g].push_back(i);
     }
     if (group[i] < 0) {
       group[i] = (int)groups.size();
       groups.push_back({i});
       kind.push_back(q.kind);
+      dims.push_back(star ? std::vector<uint32_t>{i} : std::vector<uint32_t>{});
     }
   }
   if (!groups.empty()) {

@@ -1387,12 +1387,13 @@ extern "C" int32_t mtl_star_status(uint32_t error_bits) {
 
 // ---------------------------------------------------------------------------
 // Shared passes of a batch of star queries (star.hpp).  Each of the three
-// table passes is one launch for the whole batch with one block per (tile,
-// query) (rowtile.hpp MQMap): the blocks that read one tile for different
-// queries run together on one XCD, so the tile comes from HBM once.  A
-// block loads its tile's key and predicate columns, applies its query's
-// predicate and row count, and writes that query's pass bits and bitmap
-// marks.  The fact pass adds each hit straight into its query's
+// table passes is one launch for the whole batch with one block per tile of
+// the table's largest row count in the batch: the block loads the tile's
+// key and predicate columns once and then applies, query by query, each
+// covering query's predicate and row count, writing that query's pass bits
+// and bitmap marks.  Queries with the same dimension predicate share one
+// dimension slot (bitmap and pass bits), built once and probed once per
+// parent tile.  The fact pass adds each hit straight into its query's
 // accumulator (rank, expression columns, value): there are no hit lists, so
 // a batch needs no scratch proportional to the fact table.  The per-query
 // steps between the passes (ranks, slot map, top-K and its merge levels)
@@ -1403,6 +1404,7 @@ namespace mtl {
 struct MQStarQ {
   uint64_t d_rows, p_rows, f_rows;
   PredC dpred, ppred, fpred;
+  uint32_t dslot;  // dimension slot
   uint64_t pspan;  // parent key span of this query
   uint64_t* dim_bm;
   uint64_t* dbits;
@@ -1415,9 +1417,26 @@ struct MQStarQ {
   unsigned long long* tmax;  // max |value| added (overflow bound)
 };
 
+// A dimension predicate shared by the queries that use it: its rows, key
+// bitmap, pass bits and error word.
+struct MQDimSlot {
+  PredC pred;
+  uint64_t rows;
+  uint64_t* dim_bm;
+  uint64_t* dbits;
+  uint32_t* err;
+};
+
 struct MQStarArgs {
   const MQStarQ* q;
-  MQMap map;  // blocks of the pass being launched
+  uint32_t nq;
+  const MQDimSlot* slots;
+  // the pass's queries (or slots) by its table's row count, descending, in
+  // device memory: an index into an array passed by value would make the
+  // compiler hold the whole argument block in registers
+  const uint8_t* ord;
+  uint32_t n_ord;
+  uint64_t n_max;  // that table's largest row count in the batch
   const void* d_key;
   int32_t d_key_type;
   uint32_t d_kvec;
@@ -1455,6 +1474,16 @@ __device__ __forceinline__ uint32_t rows_below(uint64_t k0, uint64_t n) {
 
 __device__ __forceinline__ void pred_u32x16(const PredC& p, const uint32_t (&x)[FJ][4],
                                             uint32_t& bits) {
+  if (p.kind == PK_I32 || p.kind == PK_U8) {  // uniform
+    // an int32 value or a zero-extended byte: the compiled int32 interval
+    // (three operations per row instead of the 64-bit encoding)
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (!pred_pass_i32(x[j][v], p)) bits &= ~(1u << (4 * j + v));
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
@@ -1506,129 +1535,167 @@ __device__ __forceinline__ void mark_global(uint64_t* bm, const uint32_t (&off)[
   if (e) atomicOr(err, e);
 }
 
-// Dimension pass: one query's pass bits and key bitmap for one tile.
-__global__ __launch_bounds__(FBLOCK) void k_mq_dim(MQStarArgs a) {
-  uint32_t qi, tile;
-  mq_block(a.map, blockIdx.x, qi, tile);
-  const MQStarQ& Q = a.q[qi];
-  const uint64_t n = Q.d_rows;
-  if ((uint64_t)tile * FTILE >= n) return;  // block-uniform
+// The tile-major passes below take one tile per block and every query (or
+// dimension slot) that covers it: the covering ones are a prefix of
+// `ord`, which lists them by the pass table's row count, descending.  The
+// tile's key and predicate columns are loaded once, for the table's largest
+// row count in the batch, and each query then masks its own rows.
+
+// Dimension pass: each slot's pass bits and key bitmap for one tile.
+__global__ __launch_bounds__(FBLOCK) void k_mqs_dim(MQStarArgs a) {
+  const uint32_t tile = blockIdx.x;
   const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
+  const uint64_t n = a.n_max;
   uint32_t px[FJ][4], ko[FJ][4];
   if (a.d_pw) load_u32x16(a.d_pcol, a.d_pw, k0, n, a.d_pvec, px);
   load_u32x16(a.d_key, a.d_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.d_kvec, ko);
-  uint32_t bits = rows_below(k0, n);
-  if (a.d_pw) pred_u32x16(Q.dpred, px, bits);
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) ko[j][v] = narrow_off(ko[j][v], a.d_key_type, a.d_min, a.dspan);
-  store_pass_bits(Q.dbits, seg, bits, lane);
-  mark_global(Q.dim_bm, ko, bits, Q.err, lane);
+#pragma unroll 1
+  for (uint32_t i = 0; i < a.n_ord; ++i) {
+    const MQDimSlot& S = a.slots[a.ord[i]];
+    if ((uint64_t)tile * FTILE >= S.rows) break;  // block-uniform
+    uint64_t kk = k0;
+    int ln = lane;
+    keep_in_loop(px);
+    keep_in_loop(ko);
+    keep_in_loop(kk);
+    keep_in_loop(ln);
+    uint32_t bits = rows_below(kk, S.rows);
+    if (a.d_pw) pred_u32x16(S.pred, px, bits);
+    store_pass_bits(S.dbits, seg, bits, ln);
+    mark_global(S.dim_bm, ko, bits, S.err, ln);
+  }
 }
 
-// Parent pass: predicate, semi-join probe of the query's dimension bitmap,
-// pass bits and parent key bitmap for one tile of one query.
-__global__ __launch_bounds__(FBLOCK) void k_mq_parent(MQStarArgs a) {
-  uint32_t qi, tile;
-  mq_block(a.map, blockIdx.x, qi, tile);
-  const MQStarQ& Q = a.q[qi];
-  const uint64_t n = Q.p_rows;
-  if ((uint64_t)tile * FTILE >= n) return;  // block-uniform
+// Bits of the lane's rows in `live` whose dimension offset is set in bm.
+__device__ __forceinline__ uint32_t probe_dim(const uint64_t* bm, const uint32_t (&fo)[FJ][4],
+                                              uint32_t live) {
+  const uint32_t* bm32 = (const uint32_t*)bm;
+  uint32_t w[FJ][4];
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const bool need = ((live >> (4 * j + v)) & 1u) && fo[j][v] != 0xFFFFFFFFu;
+      w[j][v] = need ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
+    }
+  uint32_t hit = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (fo[j][v] != 0xFFFFFFFFu && ((w[j][v] >> (fo[j][v] & 31)) & 1u)) hit |= 1u << (4 * j + v);
+  return hit & live;
+}
+
+// Parent pass: for each query covering the tile, its predicate, the
+// semi-join with its dimension slot's bitmap (probed once per slot and
+// tile, on the first query of the slot), its pass bits and parent key
+// bitmap.  Block 0 also carries each slot's dimension-pass errors over to
+// the slot's queries.
+__global__ __launch_bounds__(FBLOCK) void k_mqs_parent(MQStarArgs a) {
+  const uint32_t tile = blockIdx.x;
   const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  if (tile == 0 && threadIdx.x < a.nq) {
+    const MQStarQ& Q = a.q[threadIdx.x];
+    const uint32_t e = *a.slots[Q.dslot].err;
+    if (e) atomicOr(Q.err, e);
+  }
   const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
   const uint64_t k0 = seg + (uint64_t)lane * 4;
+  const uint64_t n = a.n_max;
   uint32_t px[FJ][4], fo[FJ][4], ko[FJ][4];
   if (a.p_pw) load_u32x16(a.p_pcol, a.p_pw, k0, n, a.p_pvec, px);
   load_u32x16(a.p_fk, a.p_fk_type == MTL_DICT8 ? 1u : 4u, k0, n, a.p_fvec, fo);
   load_u32x16(a.p_key, a.p_key_type == MTL_DICT8 ? 1u : 4u, k0, n, a.p_kvec, ko);
-  uint32_t bits = rows_below(k0, n);
-  if (a.p_pw) pred_u32x16(Q.ppred, px, bits);
-  const uint32_t* bm32 = (const uint32_t*)Q.dim_bm;
-  uint32_t w[FJ][4];
 #pragma unroll
   for (int j = 0; j < FJ; ++j)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       fo[j][v] = narrow_off(fo[j][v], a.p_fk_type, a.d_min, a.dspan);
-      const bool need = ((bits >> (4 * j + v)) & 1u) && fo[j][v] != 0xFFFFFFFFu;
-      w[j][v] = need ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
+      ko[j][v] = narrow_off(ko[j][v], a.p_key_type, a.p_min, 1ull << 32);
     }
+  // the probe bits of each slot, computed on its first query in this tile
+  uint32_t pb[kDimSlots];
+  uint32_t have = 0;  // bit s: pb[s] is valid (uniform)
 #pragma unroll
-  for (int j = 0; j < FJ; ++j)
+  for (uint32_t s = 0; s < kDimSlots; ++s) pb[s] = 0;
+  uint32_t live = 0;
+#pragma unroll 1
+  for (uint32_t i = 0; i < a.n_ord; ++i) {
+    const MQStarQ& Q = a.q[a.ord[i]];
+    if ((uint64_t)tile * FTILE >= Q.p_rows) break;  // block-uniform
+    uint64_t kk = k0;
+    int ln = lane;
+    keep_in_loop(px);
+    keep_in_loop(fo);
+    keep_in_loop(ko);
+    keep_in_loop(kk);
+    keep_in_loop(ln);
+    if (i == 0) live = rows_below(kk, Q.p_rows);    // the largest covering query's rows
+    const uint32_t sl = Q.dslot;
+    if (!((have >> sl) & 1u)) {
+      const uint32_t h = probe_dim(a.slots[sl].dim_bm, fo, live);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      if (fo[j][v] == 0xFFFFFFFFu || !((w[j][v] >> (fo[j][v] & 31)) & 1u))
-        bits &= ~(1u << (4 * j + v));
-      const uint32_t o = narrow_off(ko[j][v], a.p_key_type, a.p_min, 1ull << 32);
-      ko[j][v] = o < Q.pspan ? o : 0xFFFFFFFFu;
+      for (uint32_t s = 0; s < kDimSlots; ++s) pb[s] = s == sl ? h : pb[s];
+      have |= 1u << sl;
     }
-  store_pass_bits(Q.pbits, seg, bits, lane);
-  mark_global(Q.par_bm, ko, bits, Q.err, lane);
-}
-
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    uint32_t bits = rows_below(kk, Q.p_rows);
+    if (a.p_pw) pred_u32x16(Q.ppred, px, bits);
+    uint32_t m = 0;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t o = __shfl_xor((unsigned long long)v, off, WAVE);
-    v = o > v ? o : v;
+    for (uint32_t s = 0; s < kDimSlots; ++s) m = s == sl ? pb[s] : m;
+    bits &= m;
+    uint32_t kq[FJ][4];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) kq[j][v] = ko[j][v] < Q.pspan ? ko[j][v] : 0xFFFFFFFFu;
+    store_pass_bits(Q.pbits, seg, bits, ln);
+    mark_global(Q.par_bm, kq, bits, Q.err, ln);
   }
-  return v;
 }
 
-// Fact pass, one tile of one query: predicate and parent probe; each hit's
-// slot (rank), expression columns and value, summed per lane over runs of
-// one slot and added to the query's accumulator.
+// Fact pass: the tile's foreign keys and predicate column are loaded once;
+// queries are taken two at a time (both queries' probe loads in flight
+// together): each query's predicate and row count, the probe of its parent
+// bitmap (one two-word window per lane and step: the facts are clustered by
+// parent key, so a step's four rows name one or two parents; rows outside
+// the window probe one by one), and its hits appended to the wave's LDS hit
+// list.  A full list is flushed by the whole wave, one hit per lane with
+// every load in flight: slot rank, expression columns, value, atomic add
+// to the query's accumulator.  Overflow flags and the largest |value| per
+// query gather in LDS and reach global memory once per block.
+constexpr uint32_t kHitCap = 256;  // hits per wave list
+
+struct MQHitLds {
+  uint32_t key[FWAVES][kHitCap];   // query << 16 | row within the tile
+  uint32_t slot[FWAVES][kHitCap];  // parent key offset
+  uint64_t* acc[kStarMQMax];
+  uint32_t* cnt[kStarMQMax];
+  const uint64_t* par_bm[kStarMQMax];
+  const uint32_t* rank[kStarMQMax];
+  unsigned long long tmax[kStarMQMax];
+  unsigned long long ovf;  // bit q: query q's expression overflowed
+};
+
 template <bool DBL>
-__global__ __launch_bounds__(FBLOCK) void k_mq_fact(MQStarArgs a) {
+__device__ __forceinline__ void mqs_flush(const MQStarArgs& a, MQHitLds& L, int wave, int lane,
+                                          uint32_t n, uint64_t t0) {
   using V = typename std::conditional<DBL, double, int64_t>::type;
-  uint32_t qi, tile;
-  mq_block(a.map, blockIdx.x, qi, tile);
-  const MQStarQ& Q = a.q[qi];
-  const uint64_t n = Q.f_rows;
-  if ((uint64_t)tile * FTILE >= n) return;  // block-uniform
-  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  const uint64_t seg = (uint64_t)tile * FTILE + (uint64_t)wave * (FJ * 256);
-  const uint64_t k0 = seg + (uint64_t)lane * 4;
-  uint32_t px[FJ][4], fo[FJ][4];
-  if (a.f_pw) load_u32x16(a.f_pcol, a.f_pw, k0, n, a.f_pvec, px);
-  load_u32x16(a.f_fk, a.f_fk_type == MTL_DICT8 ? 1u : 4u, k0, n, a.f_fvec, fo);
-  uint32_t cand = rows_below(k0, n);
-  if (a.f_pw) pred_u32x16(Q.fpred, px, cand);
-  const uint32_t* bm32 = (const uint32_t*)Q.par_bm;
-  uint32_t w[FJ][4];
-#pragma unroll
-  for (int j = 0; j < FJ; ++j)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      fo[j][v] = narrow_off(fo[j][v], a.f_fk_type, a.p_min, 1ull << 32);
-      if (fo[j][v] >= Q.pspan) cand &= ~(1u << (4 * j + v));
-      w[j][v] = ((cand >> (4 * j + v)) & 1u) ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
-    }
-  uint32_t hits = 0;
-#pragma unroll
-  for (int j = 0; j < FJ; ++j)
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if (((cand >> (4 * j + v)) & 1u) && ((w[j][v] >> (fo[j][v] & 31)) & 1u))
-        hits |= 1u << (4 * j + v);
-  if (!__ballot(hits != 0)) return;  // wave-uniform: most waves of a sparse join
-  uint32_t cur = 0xFFFFFFFFu, cn = 0;
-  V cs = 0;
-  uint64_t mx = 0;
-  bool ovf = false;
-  uint32_t h = hits;
-  while (h) {
-    const int i = __ffs(h) - 1;
-    h &= h - 1;
-    uint32_t sv = fo[0][0];
-#pragma unroll
-    for (int t = 1; t < FJ * 4; ++t) sv = i == t ? fo[t >> 2][t & 3] : sv;
-    const uint64_t row = k0 + (uint64_t)(i >> 2) * 256 + (uint64_t)(i & 3);
-    const uint64_t wd = Q.par_bm[sv >> 6];
-    const uint32_t rk = Q.rank[sv >> 6];
+  for (uint32_t e0 = 0; e0 < n; e0 += WAVE) {
+    const uint32_t e = e0 + (uint32_t)lane;
+    if (e >= n) break;
+    const uint32_t key = L.key[wave][e], sv = L.slot[wave][e];
+    const uint32_t q = key >> 16;
+    const uint64_t row = t0 + (key & 0xFFFFu);
+    const uint64_t wd = L.par_bm[q][sv >> 6];
+    const uint32_t rk = L.rank[q][sv >> 6];
     V cv[MTL_EXPR_MAX_COLS];
 #pragma unroll
     for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c) {
@@ -1641,37 +1708,174 @@ __global__ __launch_bounds__(FBLOCK) void k_mq_fact(MQStarArgs a) {
     }
     const uint64_t below = (sv & 63) ? (wd & ((1ull << (sv & 63)) - 1)) : 0ull;
     const uint32_t r = rk + (uint32_t)__popcll(below);
-    V val;
     if constexpr (DBL) {
-      val = expr_f64_vals(a.expr, cv);
+      unsafeAtomicAdd((double*)&L.acc[q][r], expr_f64_vals(a.expr, cv));
     } else {
-      val = expr_i64_vals(a.expr, cv, ovf);
+      bool ovf = false;
+      const int64_t val = expr_i64_vals(a.expr, cv, ovf);
+      atomicAdd((unsigned long long*)&L.acc[q][r], (unsigned long long)val);
       const uint64_t av = val < 0 ? (uint64_t)0 - (uint64_t)val : (uint64_t)val;
-      mx = av > mx ? av : mx;
+      if (av) atomicMax(&L.tmax[q], (unsigned long long)av);
+      if (ovf) atomicOr(&L.ovf, 1ull << q);
     }
-    if (r != cur) {
-      if (cur != 0xFFFFFFFFu) {
-        if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
-        else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
-        atomicAdd(&Q.cnt[cur], cn);
+    atomicAdd(&L.cnt[q][r], 1u);
+  }
+}
+
+// Candidates of query Q among the lane's rows and the two-word window of
+// each step (loads issued, not yet used).
+struct FactProbe {
+  uint32_t cand, wb[FJ], lo[FJ], hi[FJ];
+};
+
+__device__ __forceinline__ void fact_probe_issue(const MQStarArgs& a, const MQStarQ& Q,
+                                                 const uint32_t (&px)[FJ][4],
+                                                 const uint32_t (&fo)[FJ][4], uint64_t k0,
+                                                 FactProbe& P) {
+  uint32_t cand = rows_below(k0, Q.f_rows);
+  if (a.f_pw) pred_u32x16(Q.fpred, px, cand);
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (fo[j][v] >= Q.pspan) cand &= ~(1u << (4 * j + v));
+  P.cand = cand;
+  const uint32_t* bm32 = (const uint32_t*)Q.par_bm;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    uint32_t s0 = 0;
+#pragma unroll
+    for (int v = 3; v >= 0; --v) s0 = ((cand >> (4 * j + v)) & 1u) ? fo[j][v] : s0;
+    P.wb[j] = s0 >> 5;
+    // the parent bitmap has a zero word past its end: wb + 1 is in bounds
+    P.lo[j] = ld_off32(bm32, P.wb[j] << 2);
+    P.hi[j] = ld_off32(bm32, (P.wb[j] + 1) << 2);
+  }
+}
+
+__device__ __forceinline__ uint32_t fact_probe_hits(const MQStarQ& Q, const FactProbe& P,
+                                                    const uint32_t (&fo)[FJ][4]) {
+  uint32_t hits = 0, far = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t b = 1u << (4 * j + v);
+      if (!(P.cand & b)) continue;
+      const uint32_t d = fo[j][v] - (P.wb[j] << 5);
+      if (d < 64) {
+        if (((d < 32 ? P.lo[j] : P.hi[j]) >> (d & 31)) & 1u) hits |= b;
+      } else {
+        far |= b;
       }
-      cur = r;
-      cs = 0;
-      cn = 0;
     }
-    if constexpr (DBL) cs += val;
-    else cs = (int64_t)((uint64_t)cs + (uint64_t)val);  // wrapping, as the atomics
-    ++cn;
+  if (__ballot(far != 0)) {  // wave-uniform
+    const uint32_t* bm32 = (const uint32_t*)Q.par_bm;
+    uint32_t w[FJ][4];
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        w[j][v] = ((far >> (4 * j + v)) & 1u) ? ld_off32(bm32, (fo[j][v] >> 5) << 2) : 0u;
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (((far >> (4 * j + v)) & 1u) && ((w[j][v] >> (fo[j][v] & 31)) & 1u))
+          hits |= 1u << (4 * j + v);
   }
-  if (cur != 0xFFFFFFFFu) {
-    if constexpr (DBL) unsafeAtomicAdd((double*)&Q.acc[cur], cs);
-    else atomicAdd((unsigned long long*)&Q.acc[cur], (unsigned long long)cs);
-    atomicAdd(&Q.cnt[cur], cn);
+  return hits;
+}
+
+// Append query q's hits of this lane to the wave's list; *n is the list
+// length (uniform).  A query adds at most 1024 hits per wave, so the list
+// may fill more than once: it is filled to kHitCap, flushed, and refilled.
+template <bool DBL>
+__device__ __forceinline__ void mqs_append(const MQStarArgs& a, MQHitLds& L, int wave, int lane,
+                                           uint32_t& n, uint32_t q, uint32_t hits,
+                                           const uint32_t (&fo)[FJ][4], uint64_t t0,
+                                           uint32_t seg_row) {
+  const uint32_t c = (uint32_t)__popc(hits);
+  const uint32_t incl = wave_inclusive_u32(c);
+  uint32_t end = n + (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);  // uniform
+  uint32_t p = n + incl - c;  // list position of the lane's next hit
+  uint32_t h = hits;
+  while (true) {
+    while (h && p < kHitCap) {
+      const int i = __ffs(h) - 1;
+      h &= h - 1;
+      uint32_t sv = fo[0][0];
+#pragma unroll
+      for (int t = 1; t < FJ * 4; ++t) sv = i == t ? fo[t >> 2][t & 3] : sv;
+      L.key[wave][p] = (q << 16) | (seg_row + (uint32_t)(i >> 2) * 256 + (uint32_t)(i & 3));
+      L.slot[wave][p] = sv;
+      ++p;
+    }
+    if (end <= kHitCap) break;
+    // full: entries written by other lanes of this wave are read below
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    mqs_flush<DBL>(a, L, wave, lane, kHitCap, t0);
+    __builtin_amdgcn_wave_barrier();
+    p -= kHitCap;  // lanes with hits left were at or past the end
+    end -= kHitCap;
   }
-  if (ovf) atomicOr(Q.err, 16u);
-  if constexpr (!DBL) {
-    mx = wave_max_u64(mx);
-    if (lane == 0 && mx) atomicMax(Q.tmax, (unsigned long long)mx);
+  n = end;
+}
+
+template <bool DBL>
+__global__ __launch_bounds__(FBLOCK) void k_mqs_fact(MQStarArgs a) {
+  __shared__ MQHitLds L;
+  const uint32_t tile = blockIdx.x;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  if (threadIdx.x < a.nq) {
+    const MQStarQ& Q = a.q[threadIdx.x];
+    L.acc[threadIdx.x] = Q.acc;
+    L.cnt[threadIdx.x] = Q.cnt;
+    L.par_bm[threadIdx.x] = Q.par_bm;
+    L.rank[threadIdx.x] = Q.rank;
+    L.tmax[threadIdx.x] = 0;
+  }
+  if (threadIdx.x == 0) L.ovf = 0;
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)tile * FTILE;
+  const uint32_t seg_row = (uint32_t)wave * (FJ * 256) + (uint32_t)lane * 4;
+  const uint64_t k0 = t0 + seg_row;
+  const uint64_t nmax = a.n_max;
+  uint32_t px[FJ][4], fo[FJ][4];
+  if (a.f_pw) load_u32x16(a.f_pcol, a.f_pw, k0, nmax, a.f_pvec, px);
+  load_u32x16(a.f_fk, a.f_fk_type == MTL_DICT8 ? 1u : 4u, k0, nmax, a.f_fvec, fo);
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) fo[j][v] = narrow_off(fo[j][v], a.f_fk_type, a.p_min, 1ull << 32);
+  uint32_t n = 0;  // the wave's list length
+#pragma unroll 1
+  for (uint32_t i = 0; i < a.n_ord; ++i) {
+    const uint32_t q = a.ord[i];
+    const MQStarQ& Q = a.q[q];
+    if (t0 >= Q.f_rows) break;  // block-uniform
+    uint64_t kk = k0;
+    int ln = lane;
+    uint32_t sr = seg_row;
+    keep_in_loop(px);
+    keep_in_loop(fo);
+    keep_in_loop(kk);
+    keep_in_loop(ln);
+    keep_in_loop(sr);
+    FactProbe P;
+    fact_probe_issue(a, Q, px, fo, kk, P);
+    const uint32_t h = fact_probe_hits(Q, P, fo);
+    mqs_append<DBL>(a, L, wave, ln, n, q, h, fo, t0, sr);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (n) mqs_flush<DBL>(a, L, wave, lane, n, t0);
+  __syncthreads();
+  if (threadIdx.x < a.nq) {
+    const MQStarQ& Q = a.q[threadIdx.x];
+    if (L.tmax[threadIdx.x]) atomicMax(Q.tmax, L.tmax[threadIdx.x]);
+    if ((L.ovf >> threadIdx.x) & 1ull) atomicOr(Q.err, 16u);
   }
 }
 
@@ -1707,6 +1911,12 @@ bool star_shareable(const mtl_star_desc& a, const mtl_star_desc& b) {
          std::memcmp(&a.agg, &b.agg, sizeof(mtl_expr)) == 0;
 }
 
+bool star_dim_same(const mtl_star_desc& a, const mtl_star_desc& b) {
+  if (a.d_rows != b.d_rows || a.d_n_preds != b.d_n_preds) return false;
+  return a.d_n_preds == 0 ||
+         (a.d_preds[0].op == b.d_preds[0].op && a.d_preds[0].scalar == b.d_preds[0].scalar);
+}
+
 // Merge levels of one query's top-K: lists per level, from its phase-1 grid.
 static uint32_t merge_levels(uint32_t grid, uint32_t k) {
   const uint32_t fan = topk_fan(k);
@@ -1723,7 +1933,8 @@ size_t star_batch_stage_bytes(uint32_t n) {
   // per query: its pass arguments, rank, map and top-K arguments, and up to
   // 8 merge levels
   return (size_t)n * (sizeof(MQStarQ) + sizeof(RankLaunch) + sizeof(MapArgs) + sizeof(TopkArgs) +
-                      8 * sizeof(MQMergeQ)) + 1024;
+                      8 * sizeof(MQMergeQ)) +
+         kDimSlots * sizeof(MQDimSlot) + 1024;
 }
 
 int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_stage,
@@ -1741,27 +1952,59 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
     if (lv > 8) return set_error(MTL_UNSUPPORTED, "top-K of %u merge levels", lv);
     levels += lv;
   }
+  // dimension slots: one per distinct dimension predicate (and row count)
+  std::vector<uint32_t> slot_of(n), slot_rep;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t sl = 0;
+    while (sl < slot_rep.size() && !star_dim_same(*items[slot_rep[sl]].d, *items[i].d)) ++sl;
+    if (sl == slot_rep.size()) {
+      if (sl == kDimSlots)
+        return set_error(MTL_ERR_INVALID_ARGUMENT, "more than %u dimension predicates",
+                         kDimSlots);
+      slot_rep.push_back(i);
+    }
+    slot_of[i] = sl;
+  }
+  const uint32_t n_slots = (uint32_t)slot_rep.size();
   Arena& ar = s->arena;
   hipStream_t st = s->stream;
   // regions to clear, carved from one block (each a multiple of 64 words)
   auto pad = [](uint64_t w) { return (w + 63) & ~uint64_t(63); };
   std::vector<StarBufs> b(n);
   uint64_t clear_words = 0;
+  for (uint32_t sl = 0; sl < n_slots; ++sl) clear_words += pad(sp[slot_rep[sl]].dwords * 2) + 64;
   for (uint32_t i = 0; i < n; ++i) {
     b[i].rtiles = (uint32_t)ceil_div(sp[i].pwords, RTILE);
-    clear_words += pad(sp[i].dwords * 2) + pad((sp[i].pwords + 1) * 2) + 64 +
+    clear_words += pad((sp[i].pwords + 1) * 2) + 64 +
                    pad((uint64_t)std::max<uint32_t>(b[i].rtiles, 1) * 2) + 64;
   }
   void* clr;
   MTL_TRY(ar.alloc(clear_words * 4, &clr));
   uint32_t* cw = (uint32_t*)clr;
+  // the slots' bitmaps (cleared) and pass bits
+  std::vector<MQDimSlot> slots(n_slots);
+  for (uint32_t sl = 0; sl < n_slots; ++sl) {
+    const mtl_star_desc& d = *items[slot_rep[sl]].d;
+    MQDimSlot& S = slots[sl];
+    S.pred = d.d_n_preds ? compile_pred(d.d_preds[0]) : PredC{};
+    S.rows = d.d_rows;
+    S.dim_bm = (uint64_t*)cw;
+    cw += pad(sp[slot_rep[sl]].dwords * 2);
+    S.err = cw;
+    cw += 64;
+    void* db;
+    MTL_TRY(ar.alloc((uint64_t)filter_tiles(d.d_rows) * kTileRows / 8, &db));
+    S.dbits = (uint64_t*)db;
+  }
   // staged arguments: [MQStarQ n][RankLaunch n][MapArgs n][TopkArgs n, by
-  // top-K shape][MQMergeQ, by level]
+  // top-K shape][MQMergeQ, by level][MQDimSlot n_slots]
   const size_t o_r = (size_t)n * sizeof(MQStarQ);
   const size_t o_m = o_r + (size_t)n * sizeof(RankLaunch);
   const size_t o_t = o_m + (size_t)n * sizeof(MapArgs);
   const size_t o_g = o_t + (size_t)n * sizeof(TopkArgs);
-  const size_t stage_bytes = o_g + (size_t)levels * sizeof(MQMergeQ);
+  const size_t o_s = o_g + (size_t)levels * sizeof(MQMergeQ);
+  const size_t o_o = o_s + (size_t)n_slots * sizeof(MQDimSlot);  // 3 order lists
+  const size_t stage_bytes = o_o + 3 * (size_t)kStarMQMax;
   void* dq;
   MTL_TRY(ar.alloc(stage_bytes, &dq));
   char* hs = (char*)host_stage;
@@ -1770,14 +2013,13 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
   MapArgs* hm = (MapArgs*)(hs + o_m);
   TopkArgs* ht = (TopkArgs*)(hs + o_t);
   MQMergeQ* hg = (MQMergeQ*)(hs + o_g);
-  std::vector<uint32_t> dt(n), pt(n), ft(n);
   std::vector<TopkArgs> ta(n);
   uint32_t rgrid = 1, mgrid = 1;
   for (uint32_t i = 0; i < n; ++i) {
     const mtl_star_desc& d = *items[i].d;
     StarBufs& bi = b[i];
-    bi.dim_bm = cw;
-    cw += pad(sp[i].dwords * 2);
+    bi.dim_bm = slots[slot_of[i]].dim_bm;
+    bi.dbits = slots[slot_of[i]].dbits;
     bi.par_bm = cw;
     cw += pad((sp[i].pwords + 1) * 2);
     bi.misc = cw;
@@ -1788,7 +2030,6 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
     cw += 64;
     bi.n_tmax = 1;
     const uint64_t prows = d.p_rows ? d.p_rows : 1;
-    MTL_TRY(ar.alloc((uint64_t)filter_tiles(d.d_rows) * kTileRows / 8, &bi.dbits));
     MTL_TRY(ar.alloc(sp[i].pwords * 4, &bi.par_rank));
     MTL_TRY(ar.alloc((uint64_t)filter_tiles(d.p_rows) * kTileRows / 8, &bi.pbits));
     MTL_TRY(ar.alloc(prows * 4, &bi.row_of));
@@ -1804,6 +2045,7 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
     if (d.p_n_preds) Q.ppred = compile_pred(d.p_preds[0]);
     if (d.f_n_preds) Q.fpred = compile_pred(d.f_preds[0]);
     Q.pspan = sp[i].pspan;
+    Q.dslot = slot_of[i];
     Q.dim_bm = (uint64_t*)bi.dim_bm;
     Q.dbits = (uint64_t*)bi.dbits;
     Q.pbits = (uint64_t*)bi.pbits;
@@ -1818,9 +2060,6 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
     ta[i] = make_topk(&d, sp[i], bi, items[i].out, items[i].out_n);
     rgrid = std::max(rgrid, hr[i].grid);
     mgrid = std::max(mgrid, hm[i].grid);
-    dt[i] = filter_tiles(d.d_rows);
-    pt[i] = filter_tiles(d.p_rows);
-    ft[i] = d.f_rows ? filter_tiles(d.f_rows) : 0u;
   }
   // top-K arguments grouped by wave-list shape (K <= 64, <= 128, above)
   auto shape = [](uint32_t k) { return k <= 64 ? 0 : k <= 128 ? 1 : 2; };
@@ -1875,6 +2114,20 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
       lvl_grid.push_back(g);
     }
   }
+  std::memcpy(hs + o_s, slots.data(), (size_t)n_slots * sizeof(MQDimSlot));
+  // each pass's queries (slots) by its table's row count, descending
+  auto order = [&](int pass, auto rows_of, uint32_t cnt) {
+    std::vector<uint32_t> ix(cnt);
+    for (uint32_t i = 0; i < cnt; ++i) ix[i] = i;
+    std::stable_sort(ix.begin(), ix.end(),
+                     [&](uint32_t x, uint32_t y) { return rows_of(x) > rows_of(y); });
+    uint8_t* o = (uint8_t*)hs + o_o + (size_t)pass * kStarMQMax;
+    for (uint32_t i = 0; i < cnt; ++i) o[i] = (uint8_t)ix[i];
+    return ix[0];
+  };
+  const uint32_t dim_first = order(0, [&](uint32_t sl) { return slots[sl].rows; }, n_slots);
+  const uint32_t par_first = order(1, [&](uint32_t i) { return items[i].d->p_rows; }, n);
+  const uint32_t fact_first = order(2, [&](uint32_t i) { return items[i].d->f_rows; }, n);
   MTL_HIP(hipMemcpyAsync(dq, hs, stage_bytes, hipMemcpyHostToDevice, st));
   ZeroList zl;
   zl.add((uint32_t*)clr, clear_words);
@@ -1910,16 +2163,23 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
   a.f_pcol = d0.f_n_preds ? d0.f_preds[0].column : nullptr;
   a.f_pvec = a.f_pw ? vec(a.f_pcol, a.f_pw) : 0u;
   a.expr = d0.agg;
+  a.nq = n;
+  a.slots = (const MQDimSlot*)((char*)dq + o_s);
   {
-    ProfScope ps(s, "k_mq_dim");
-    const uint32_t grid = mq_map(dt.data(), n, &a.map);
-    hipLaunchKernelGGL(k_mq_dim, dim3(grid), dim3(FBLOCK), 0, st, a);
+    ProfScope ps(s, "k_mqs_dim");
+    a.ord = (const uint8_t*)dq + o_o;
+    a.n_ord = n_slots;
+    a.n_max = slots[dim_first].rows;
+    hipLaunchKernelGGL(k_mqs_dim, dim3(filter_tiles(a.n_max)), dim3(FBLOCK), 0, st, a);
     MTL_HIP(hipGetLastError());
   }
   {
-    ProfScope ps(s, "k_mq_parent");
-    const uint32_t grid = mq_map(pt.data(), n, &a.map);
-    hipLaunchKernelGGL(k_mq_parent, dim3(grid), dim3(FBLOCK), 0, st, a);
+    ProfScope ps(s, "k_mqs_parent");
+    a.ord = (const uint8_t*)dq + o_o + kStarMQMax;
+    a.n_ord = n;
+    a.n_max = items[par_first].d->p_rows;
+    hipLaunchKernelGGL(k_mqs_parent, dim3(std::max(1u, filter_tiles(a.n_max))), dim3(FBLOCK), 0,
+                       st, a);
     MTL_HIP(hipGetLastError());
   }
   {
@@ -1935,13 +2195,14 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
     else hipLaunchKernelGGL(k_mq_map<false>, dim3(mgrid, n), dim3(256), 0, st, dm);
     MTL_HIP(hipGetLastError());
   }
-  uint32_t fmax = 0;
-  for (uint32_t i = 0; i < n; ++i) fmax = std::max(fmax, ft[i]);
-  if (fmax) {
-    ProfScope ps(s, "k_mq_fact");
-    const uint32_t grid = mq_map(ft.data(), n, &a.map);
-    if (d0.agg.is_double) hipLaunchKernelGGL(k_mq_fact<true>, dim3(grid), dim3(FBLOCK), 0, st, a);
-    else hipLaunchKernelGGL(k_mq_fact<false>, dim3(grid), dim3(FBLOCK), 0, st, a);
+  a.ord = (const uint8_t*)dq + o_o + 2 * kStarMQMax;
+  a.n_ord = n;
+  a.n_max = items[fact_first].d->f_rows;
+  if (a.n_max) {
+    ProfScope ps(s, "k_mqs_fact");
+    const dim3 grid(filter_tiles(a.n_max));
+    if (d0.agg.is_double) hipLaunchKernelGGL(k_mqs_fact<true>, grid, dim3(FBLOCK), 0, st, a);
+    else hipLaunchKernelGGL(k_mqs_fact<false>, grid, dim3(FBLOCK), 0, st, a);
     MTL_HIP(hipGetLastError());
   }
   {

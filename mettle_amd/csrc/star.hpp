@@ -10,7 +10,8 @@
 
 namespace mtl {
 
-constexpr uint32_t kStarMQMax = 32;  // queries per share group
+constexpr uint32_t kStarMQMax = 64;  // queries per share group
+constexpr uint32_t kDimSlots = 8;    // distinct dimension predicates per share group
 
 struct StarBatchItem {
   const mtl_star_desc* d;
@@ -23,8 +24,11 @@ struct StarBatchItem {
 // 4 bytes, and a descriptor mtl_star_topk accepts.
 bool star_batchable(const mtl_star_desc& d);
 // Same share group: the same columns (keys, predicate columns, expression)
-// and key ranges apart from the parent key's upper bound.
+// and key ranges apart from the parent key's upper bound.  A group also
+// holds at most kDimSlots distinct dimension predicates (star_dim_same).
 bool star_shareable(const mtl_star_desc& a, const mtl_star_desc& b);
+// Same dimension slot: the same dimension predicate and row count.
+bool star_dim_same(const mtl_star_desc& a, const mtl_star_desc& b);
 // Pinned host bytes the enqueue stages its per-query arguments in.
 size_t star_batch_stage_bytes(uint32_t n);
 // Enqueue n <= kStarMQMax shareable queries on s (scratch from its arena,

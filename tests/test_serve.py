@@ -11,26 +11,21 @@ from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
-M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
-
-
-def _mix(x):
-    x = x.astype(np.uint64)
-    x ^= x >> np.uint64(33)
-    x *= np.uint64(0xff51afd7ed558ccd)
-    x ^= x >> np.uint64(33)
-    x *= np.uint64(0xc4ceb9fe1a85ec53)
-    x ^= x >> np.uint64(33)
-    return x
 
 
 def host_fingerprint(rows: np.ndarray, vals_bits: np.ndarray) -> int:
-    """The device fingerprint restated: sum_i mix((i<<32)^row_i^C) + mix(v_i+i)."""
-    i = np.arange(len(rows), dtype=np.uint64)
+    """The device fingerprint restated: sum over positions i of a 32-bit
+    multiply-xorshift of (i, row_i, value bits_i), in uint64."""
+    i = np.arange(len(rows), dtype=np.uint64).astype(np.uint32)
+    v = vals_bits.astype(np.uint64)
     with np.errstate(over="ignore"):
-        a = _mix((i << np.uint64(32)) ^ rows.astype(np.uint64) ^ np.uint64(0x9e3779b97f4a7c15))
-        b = _mix(vals_bits.astype(np.uint64) + i)
-        return int((a + b).sum(dtype=np.uint64))
+        x = i * np.uint32(0x9E3779B1) + rows.astype(np.uint32)
+        x ^= (v & np.uint64(0xFFFFFFFF)).astype(np.uint32) * np.uint32(0x85EBCA77)
+        x ^= (v >> np.uint64(32)).astype(np.uint32) * np.uint32(0xC2B2AE3D)
+        x ^= x >> np.uint32(15)
+        x *= np.uint32(0x2C1B3C6D)
+        x ^= x >> np.uint32(13)
+        return int(x.astype(np.uint64).sum(dtype=np.uint64))
 
 
 @pytest.fixture(scope="module")

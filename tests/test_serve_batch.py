@@ -183,6 +183,43 @@ def test_shared_star_batch_matches_oracle_and_alone(ops):
             assert _star_rows(r, out) == exp
 
 
+def test_shared_star_batch_dimension_slots(ops):
+    """Sixty-four star queries in one batch (one share group's limit) whose
+    dimension predicates take twelve distinct (operator, literal) forms: a
+    share group holds at most eight distinct ones, so the batch splits, and
+    queries with the same dimension predicate share one dimension bitmap.
+    Every result equals the oracle's."""
+    tables = datagen.star_tables(scale=0.01, workload=61)
+    cust, orders, lines = tables
+    dev = {name: {c: ops.to_device(x) for c, x in t.items()}
+           for name, t in zip(["customers", "orders", "lines"], tables)}
+    by = dict(zip(["customers", "orders", "lines"], tables))
+    n_c, n_l = len(cust["c_key"]), len(lines["l_order"])
+    rng = np.random.default_rng(61)
+    forms = [(op, lit) for op in (abi.CMP_EQ, abi.CMP_LT, abi.CMP_GE) for lit in range(4)]
+    work = []
+    for j in range(64):
+        nl = n_l if j % 7 == 0 else int(rng.integers(1, n_l + 1))
+        no = int(lines["l_order"][nl - 1])
+        cut = datagen.DAYS_1992_01_01 + int(rng.integers(0, datagen.SPAN_7Y))
+        op, lit = forms[j % len(forms)]
+        descs = []
+        for ptr in (lambda t, c: dev[t][c].data_ptr(), lambda t, c: by[t][c].ctypes.data):
+            d = datagen.star_desc(ptr, n_c, no, nl, k=[10, 1, 100][j % 3], cutoff=cut)
+            d.d_preds[0] = abi.make_pred(ptr("customers", "segment"), abi.DICT8, op, lit)
+            descs.append(d)
+        st, ref = orc.star_topk(descs[1])
+        assert st == abi.MTL_OK
+        q, out = ops.star_query(descs[0])
+        work.append((q, out, [(r.p_row, r.count, r.sum_i64) for r in ref]))
+    with ops.Server(1, budget=BUDGET, max_batch=64) as srv:
+        ts = _arrive_together(srv, [q for q, _, _ in work])
+        res = [srv.wait(t) for t in ts]
+    assert max(r.batch for r in res) > 8
+    for (q, out, exp), r in zip(work, res):
+        assert _star_rows(r, out) == exp
+
+
 def test_shared_star_batch_full_size(ops):
     """Eight star queries over the full 60M-line tables in one batch."""
     work, keep = [], None
