@@ -1760,3 +1760,255 @@ int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* wor
 }
 
 }  // namespace mtl
+
+// ---------------------------------------------------------------------------
+// Small scans of a batch in one persistent launch.  A scan of at most
+// kSmallRows rows (any columns, up to MTL_MAX_PREDS predicates, an optional
+// projection) is done by one block from start to end: per 4096-row tile the
+// predicates, a block prefix of the passing rows (one barrier pair), the
+// row ids and projected values at their output positions, and the result
+// fingerprint; the block then writes the query's (count, overflow flag,
+// fingerprint).  No kernel boundary or host step separates a query's
+// filter from its compaction, and every small scan of the batch shares one
+// launch: a fixed grid of blocks that take queries from a ticket until the
+// batch's count is reached.  The launch sequence (descriptor upload from
+// pinned memory, the kernel, the result download) is captured once per
+// serving slot into a HIP graph and replayed for each batch.
+// ---------------------------------------------------------------------------
+namespace mtl {
+
+constexpr int SM_BLOCK = FBLOCK;  // 4 waves x 4 steps x 64 lanes x 4 rows = one tile
+
+struct SmallScanQ {
+  PredC preds[MTL_MAX_PREDS];
+  uint32_t n_preds;
+  uint32_t pk;  // projection kind: 0 none, 1 int32, 2 int64, 3 float, 4 double
+  uint64_t n;
+  const void* pcol;
+  int32_t op;
+  int64_t scalar;
+  uint32_t* out_sel;
+  void* out_proj;
+};
+
+// The pinned (and device) block the graph copies: header, then descriptors.
+struct SmallScanBlock {
+  uint32_t nq;
+  uint32_t ticket;  // uploaded as 0 with every batch
+  uint32_t pad[2];
+  SmallScanQ q[kSmallMax];
+};
+
+__device__ __forceinline__ uint64_t proj_store_rt(uint32_t pk, const ExpandProj& pj, uint64_t o,
+                                                  uint64_t x, bool& ovf) {
+  switch (pk) {
+    case 1: return proj_store<1>(pj, o, x, ovf);
+    case 2: return proj_store<2>(pj, o, x, ovf);
+    case 3: return proj_store<3>(pj, o, x, ovf);
+    case 4: return proj_store<4>(pj, o, x, ovf);
+    default: return 0;
+  }
+}
+
+__global__ __launch_bounds__(SM_BLOCK) void k_small_scan(SmallScanBlock* blk, uint32_t* res) {
+  __shared__ uint32_t s_q;
+  __shared__ uint32_t s_wtot[FWAVES];
+  __shared__ uint64_t s_h[FWAVES];
+  __shared__ uint32_t s_ovf;
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint32_t nq = blk->nq;
+  while (true) {
+    if (tid == 0) s_q = atomicAdd(&blk->ticket, 1u);
+    if (tid == 0) s_ovf = 0;
+    __syncthreads();
+    const uint32_t q = s_q;
+    __syncthreads();
+    if (q >= nq) break;  // block-uniform
+    const SmallScanQ& Q = blk->q[q];
+    const uint64_t n = Q.n;
+    const uint32_t pk = Q.pk;
+    const ExpandProj pj{Q.pcol, Q.op, Q.scalar, Q.out_proj, nullptr};
+    const uint32_t pw = pk == 2 || pk == 4 ? 8u : 4u;
+    uint32_t base = 0;
+    bool ovf = false;
+    uint64_t h = 0;
+    for (uint64_t t0 = 0; t0 < n; t0 += FTILE) {
+      const uint64_t k0 = t0 + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
+      uint32_t bits = rows_below(k0, n);
+      for (uint32_t p = 0; p < Q.n_preds; ++p) {
+        const PredC& P = Q.preds[p];
+        const uint32_t w = pred_width(P.kind);
+        // all 16 loads issued before any compare (rows past n read row
+        // n - 1, in bounds, and are masked by rows_below)
+        uint64_t x[FJ][4];
+#pragma unroll
+        for (int j = 0; j < FJ; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const uint64_t r = k0 + (uint64_t)j * 256 + v;
+            x[j][v] = load_raw(P.col, w, r < n ? r : n - 1);
+          }
+#pragma unroll
+        for (int j = 0; j < FJ; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (!pred_pass(enc_raw(x[j][v], P.kind), P)) bits &= ~(1u << (4 * j + v));
+      }
+      // positions: wave prefix per step, then the tile's wave offsets
+      uint32_t ej[FJ];
+      uint32_t wsum = 0;
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) {
+        const uint32_t c = (uint32_t)__popc((bits >> (4 * j)) & 0xFu);
+        const uint32_t incl = wave_inclusive_u32(c);
+        ej[j] = wsum + incl - c;
+        wsum += (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
+      }
+      if (lane == 0) s_wtot[wave] = wsum;
+      __syncthreads();
+      uint32_t woff = 0, ttot = 0;
+#pragma unroll
+      for (int w = 0; w < FWAVES; ++w) {
+        if (w < wave) woff += s_wtot[w];
+        ttot += s_wtot[w];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) {
+        uint32_t k = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (!((bits >> (4 * j + v)) & 1u)) continue;
+          const uint64_t r = k0 + (uint64_t)j * 256 + v;
+          const uint64_t o = (uint64_t)base + woff + ej[j] + k++;
+          Q.out_sel[o] = (uint32_t)r;
+          const uint64_t x = pk ? load_raw(Q.pcol, pw, r) : 0ull;
+          const uint64_t vb = proj_store_rt(pk, pj, o, x, ovf);
+          h += fp_term(o, (uint32_t)r, pk ? vb : r);
+        }
+      }
+      base += ttot;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, WAVE);
+    if (lane == 0) s_h[wave] = h;
+    if (ovf) atomicOr(&s_ovf, 1u);
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t t = 0;
+#pragma unroll
+      for (int w = 0; w < FWAVES; ++w) t += s_h[w];
+      res[4 * q] = base;
+      res[4 * q + 1] = s_ovf;
+      res[4 * q + 2] = (uint32_t)t;
+      res[4 * q + 3] = (uint32_t)(t >> 32);
+    }
+    __syncthreads();
+  }
+}
+
+bool small_scan_ok(const mtl_query& q) {
+  if (q.kind != MTL_Q_SCAN || q.n_rows == 0 || q.n_rows > kSmallRows || q.n_preds == 0 ||
+      q.n_preds > MTL_MAX_PREDS)
+    return false;
+  for (uint32_t i = 0; i < q.n_preds; ++i) {
+    const mtl_pred& p = q.preds[i];
+    if (!p.column || !type_width(p.type) || p.type == MTL_HUGEINT || p.op < 0 || p.op > 4)
+      return false;
+  }
+  if (q.proj_col) {
+    int pk = 0;
+    if (proj_kind(q.proj_type, q.proj_op, &pk) != MTL_OK) return false;
+  }
+  return true;
+}
+
+struct SmallGraph {
+  SmallScanBlock* h_blk = nullptr;  // pinned
+  SmallScanBlock* d_blk = nullptr;
+  uint32_t* h_res = nullptr;  // pinned, 4 words per query
+  uint32_t* d_res = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void small_graph_free(SmallGraph* g) {
+  if (!g) return;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->h_blk) hipHostFree(g->h_blk);
+  if (g->h_res) hipHostFree(g->h_res);
+  dev_free(g->d_blk, sizeof(SmallScanBlock));
+  dev_free(g->d_res, kSmallMax * 16);
+  delete g;
+}
+
+void small_graph_destroy(SmallGraph* g) { small_graph_free(g); }
+
+int32_t small_graph_create(mtl_stream* s, SmallGraph** out) {
+  auto* g = new SmallGraph();
+  int32_t st = MTL_OK;
+  if (hipHostMalloc((void**)&g->h_blk, sizeof(SmallScanBlock), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&g->h_res, kSmallMax * 16, hipHostMallocDefault) != hipSuccess)
+    st = set_error(MTL_ERR_PROCESSING, "processing error: pinned small-scan staging");
+  if (st == MTL_OK) st = dev_alloc(sizeof(SmallScanBlock), (void**)&g->d_blk, "small scans");
+  if (st == MTL_OK) st = dev_alloc(kSmallMax * 16, (void**)&g->d_res, "small scans");
+  if (st != MTL_OK) {
+    small_graph_free(g);
+    return st;
+  }
+  // capture: upload the block, run the persistent kernel, download the
+  // results (the ticket is uploaded as 0, so no clearing node is needed)
+  hipStream_t cs = s->stream;
+  hipGraph_t graph = nullptr;
+  const uint32_t grid = (uint32_t)std::min<int>(2 * cu_count(), (int)kSmallMax);
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    hipMemcpyAsync(g->d_blk, g->h_blk, sizeof(SmallScanBlock), hipMemcpyHostToDevice, cs);
+    hipLaunchKernelGGL(k_small_scan, dim3(grid), dim3(SM_BLOCK), 0, cs, g->d_blk, g->d_res);
+    hipMemcpyAsync(g->h_res, g->d_res, kSmallMax * 16, hipMemcpyDeviceToHost, cs);
+    e = hipStreamEndCapture(cs, &graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    small_graph_free(g);
+    return set_error(MTL_ERR_PROCESSING, "processing error: small-scan graph: %s",
+                     hipGetErrorString(e));
+  }
+  *out = g;
+  return MTL_OK;
+}
+
+int32_t small_scan_enqueue(SmallGraph* g, const SmallScanItem* items, uint32_t n,
+                           mtl_stream* s) {
+  if (n == 0) return MTL_OK;
+  if (n > kSmallMax) return set_error(MTL_ERR_INVALID_ARGUMENT, "more than %u small scans",
+                                      kSmallMax);
+  SmallScanBlock* b = g->h_blk;
+  b->nq = n;
+  b->ticket = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const mtl_query& q = *items[i].q;
+    if (!small_scan_ok(q))
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "query %u is not a small scan", i);
+    SmallScanQ& Q = b->q[i];
+    Q = SmallScanQ{};
+    for (uint32_t p = 0; p < q.n_preds; ++p) Q.preds[p] = compile_pred(q.preds[p]);
+    Q.n_preds = q.n_preds;
+    Q.n = q.n_rows;
+    int pk = 0;
+    if (q.proj_col) MTL_TRY(proj_kind(q.proj_type, q.proj_op, &pk));
+    Q.pk = (uint32_t)pk;
+    Q.pcol = q.proj_col;
+    Q.op = q.proj_op;
+    Q.scalar = q.proj_scalar;
+    Q.out_sel = items[i].out_sel;
+    Q.out_proj = items[i].out_proj;
+  }
+  MTL_HIP(hipGraphLaunch(g->exec, s->stream));
+  return MTL_OK;
+}
+
+const uint32_t* small_scan_results(const SmallGraph* g) { return g->h_res; }
+
+}  // namespace mtl

@@ -177,4 +177,29 @@ size_t scan_batch_stage_bytes(uint32_t n);
 int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* words,
                            void* host_stage, mtl_stream* s);
 
+// ---------------------------------------------------------------------------
+// Small scans of a batch (at most kSmallRows rows each; any columns, up to
+// MTL_MAX_PREDS predicates, an optional projection) in one persistent
+// launch, replayed from a HIP graph captured once per serving slot.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kSmallRows = 1ull << 16;  // 16 tiles: one block per query
+constexpr uint32_t kSmallMax = 256;          // small scans per batch
+struct SmallGraph;
+struct SmallScanItem {
+  const mtl_query* q;
+  uint32_t* out_sel;  // q->n_rows row ids
+  void* out_proj;     // q->n_rows projected values (nullptr without projection)
+};
+bool small_scan_ok(const mtl_query& q);
+// Pinned staging, device buffers and the instantiated graph (captured on s).
+int32_t small_graph_create(mtl_stream* s, SmallGraph** out);
+void small_graph_destroy(SmallGraph* g);
+// Stage n <= kSmallMax small scans and launch the graph on s.  The staging
+// is reused by the next call: the caller synchronises s in between.
+int32_t small_scan_enqueue(SmallGraph* g, const SmallScanItem* items, uint32_t n,
+                           mtl_stream* s);
+// After s is synchronised: 4 uint32 per query (count, overflow flag,
+// 64-bit fingerprint), in enqueue order.
+const uint32_t* small_scan_results(const SmallGraph* g);
+
 }  // namespace mtl

@@ -226,6 +226,17 @@ __device__ __forceinline__ void load_u32x16(const void* col, uint32_t width, uin
   }
 }
 
+// Bits of the lane's 16 rows that lie below n.
+__device__ __forceinline__ uint32_t rows_below(uint64_t k0, uint64_t n) {
+  uint32_t bits = 0;
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
+  return bits;
+}
+
 // An empty asm statement the compiler must assume rewrites x.  Called at
 // the top of a loop over queries on a lane's 16 loaded rows, it keeps the
 // work derived from those rows inside the loop: hoisted out, the per-row

@@ -112,6 +112,7 @@ struct Slot {
   size_t host_cap = 0;
   void* stage = nullptr;      // pinned per-query arguments of the shared passes
   size_t stage_cap = 0;
+  SmallGraph* small = nullptr;  // persistent small-scan launch (batching servers)
 };
 
 static int32_t slot_reserve(Slot& sl, uint64_t rows) {
@@ -231,8 +232,15 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
   std::vector<int> kind;  // per group: MTL_Q_SCAN or MTL_Q_STAR
   // per star group: one query of each distinct dimension predicate
   std::vector<std::vector<uint32_t>> dims;
+  // small scans (group -2): one persistent launch from the slot's graph
+  std::vector<uint32_t> small;
   for (uint32_t i = 0; i < n; ++i) {
     const mtl_query& q = qs[i];
+    if (sl.small && small.size() < kSmallMax && small_scan_ok(q)) {
+      group[i] = -2;
+      small.push_back(i);
+      continue;
+    }
     const bool scan = q.kind == MTL_Q_SCAN && scan_batchable(q);
     const bool star = q.kind == MTL_Q_STAR && star_batchable(q.star) && q.star_out;
     if (!scan && !star) continue;
@@ -261,7 +269,7 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
       dims.push_back(star ? std::vector<uint32_t>{i} : std::vector<uint32_t>{});
     }
   }
-  if (!groups.empty()) {
+  if (!groups.empty() || !small.empty()) {
     int32_t st = MTL_OK;
     // device results: scans 16 B each, stars out_n (8 B) + K rows each
     // (assigned group by group: a scan group's words are contiguous)
@@ -288,6 +296,22 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
       st = pinned_reserve(sl.host, sl.host_cap, rbytes);
       if (st == MTL_OK) st = pinned_reserve(sl.stage, sl.stage_cap, stage_bytes);
       if (st == MTL_OK) st = ar.alloc(rbytes, &dres);
+      if (st == MTL_OK && !small.empty()) {
+        std::vector<SmallScanItem> items(small.size());
+        for (size_t j = 0; j < small.size() && st == MTL_OK; ++j) {
+          const mtl_query& q = qs[small[j]];
+          items[j].q = &q;
+          void* o = nullptr;
+          st = ar.alloc(q.n_rows * 4, &o);
+          items[j].out_sel = (uint32_t*)o;
+          items[j].out_proj = nullptr;
+          if (st == MTL_OK && q.proj_col) {
+            st = ar.alloc(q.n_rows * std::max(type_width(q.proj_type), 4), &o);
+            items[j].out_proj = o;
+          }
+        }
+        if (st == MTL_OK) st = small_scan_enqueue(sl.small, items.data(), (uint32_t)items.size(), sl.s);
+      }
       char* stage = (char*)sl.stage;
       for (size_t g = 0; g < groups.size() && st == MTL_OK; ++g) {
         const auto& mem = groups[g];
@@ -323,7 +347,7 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
           stage += stage_of(g);
         }
       }
-      if (st == MTL_OK)
+      if (st == MTL_OK && rbytes)
         st = hipMemcpyAsync(sl.host, dres, rbytes, hipMemcpyDeviceToHost, sl.s->stream) ==
                      hipSuccess
                  ? MTL_OK
@@ -333,6 +357,22 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
       if (st == MTL_OK && se != hipSuccess)
         st = set_error(MTL_ERR_PROCESSING, "processing error: %s", hipGetErrorString(se));
       ar.release_retired();
+    }
+    for (size_t j = 0; j < small.size(); ++j) {
+      const uint32_t i = small[j];
+      mtl_query_result& r = rs[i];
+      r.batch = n;
+      if (st != MTL_OK) {
+        finish(r, st);
+        continue;
+      }
+      const uint32_t* w = small_scan_results(sl.small) + 4 * j;
+      r.count = w[0];
+      r.fingerprint = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+      int32_t qs_ = MTL_OK;
+      if (w[0] > qs[i].n_rows) qs_ = set_error(MTL_ERR_PROCESSING, "device scan failed");
+      else if (w[1]) qs_ = set_error(MTL_ERR_OVERFLOW, "Overflow in projection");
+      finish(r, qs_);
     }
     for (uint32_t i = 0; i < n; ++i) {
       if (group[i] < 0) continue;
@@ -366,7 +406,7 @@ static void run_batch(Slot& sl, std::vector<mtl_query>& qs, std::vector<mtl_quer
     }
   }
   for (uint32_t i = 0; i < n; ++i) {
-    if (group[i] >= 0) continue;
+    if (group[i] != -1) continue;
     rs[i].batch = n;
     finish(rs[i], run_one(sl, qs[i], rs[i]));
   }
@@ -509,6 +549,7 @@ int32_t mtl_server_create(uint32_t n_slots, uint32_t max_batch, uint64_t scratch
     int32_t st = mtl_stream_create(&sl.s);
     if (st == MTL_OK) st = dev_alloc(64, (void**)&sl.words, "serving slot");
     if (st == MTL_OK) st = pinned_reserve(sl.host, sl.host_cap, 4096);
+    if (st == MTL_OK && max_batch > 1) st = small_graph_create(sl.s, &sl.small);
     if (st != MTL_OK) {
       mtl_server_destroy(srv);
       return st;
@@ -592,6 +633,7 @@ int32_t mtl_server_destroy(mtl_server* srv) {
     dev_free(sl.words, 64);
     if (sl.host) hipHostFree(sl.host);
     if (sl.stage) hipHostFree(sl.stage);
+    small_graph_destroy(sl.small);
     if (sl.s) mtl_stream_destroy(sl.s);
   }
   // results nobody waited for: their SQL result handles

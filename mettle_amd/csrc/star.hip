@@ -1461,17 +1461,6 @@ struct MQStarArgs {
   mtl_expr expr;
 };
 
-// Bits of the lane's 16 rows that lie below n.
-__device__ __forceinline__ uint32_t rows_below(uint64_t k0, uint64_t n) {
-  uint32_t bits = 0;
-#pragma unroll
-  for (int j = 0; j < FJ; ++j)
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
-  return bits;
-}
-
 __device__ __forceinline__ void pred_u32x16(const PredC& p, const uint32_t (&x)[FJ][4],
                                             uint32_t& bits) {
   if (p.kind == PK_I32 || p.kind == PK_U8) {  // uniform

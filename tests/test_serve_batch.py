@@ -121,6 +121,74 @@ def test_shared_scan_batch_matches_oracle_and_alone(ops):
         assert (a.count, a.fingerprint) == (b.count, b.fingerprint)
 
 
+def test_small_scans_in_the_persistent_graph_launch(ops):
+    """Small scans (at most 2^16 rows) of every column type with one to
+    three predicates and projections run in the batch's persistent launch,
+    replayed from the slot's HIP graph: three batches in a row on one
+    server, with eager device work and host synchronisation between them.
+    Every result equals the oracle's (count and fingerprint, or the
+    overflow error)."""
+    import torch
+    rng = np.random.default_rng(23)
+    n_max = 1 << 16
+    cols = {
+        "i": (rng.integers(-2**31, 2**31, n_max).astype(np.int32), abi.INTEGER),
+        "b": (rng.integers(-2**62, 2**62, n_max).astype(np.int64), abi.BIGINT),
+        "f": (rng.standard_normal(n_max).astype(np.float32), abi.FLOAT),
+        "g": (rng.standard_normal(n_max), abi.DOUBLE),
+        "d": (rng.integers(10957, 22281, n_max).astype(np.int32), abi.DATE),
+        "t": (rng.integers(946684800 * 10**6, 1924991999 * 10**6, n_max).astype(np.int64),
+              abi.TIMESTAMP),
+        "c": (rng.integers(0, 5, n_max).astype(np.uint8), abi.DICT8),
+    }
+    dev = {k: ops.to_device(x) for k, (x, _) in cols.items()}
+    projs = {"i": [(abi.PROJ_MUL, 3), (abi.PROJ_ADD, 2**30)], "b": [(abi.PROJ_SUB, 5)],
+             "f": [(abi.PROJ_MUL, 0.5)], "g": [(abi.PROJ_ADD, 1.25)],
+             "d": [(abi.PROJ_ADD, 30)], "t": [(abi.PROJ_SUB, 10**6)], "c": [None]}
+    names = list(cols)
+    rounds = []
+    for _ in range(3):
+        work = []
+        for j in range(48):
+            n = int(10 ** rng.uniform(0, np.log10(n_max))) if j % 5 else n_max
+            preds = []
+            for k in range(1 + j % 3):
+                c = names[int(rng.integers(0, len(names)))]
+                x, t = cols[c]
+                lit = x[int(rng.integers(0, n_max))]
+                lit = float(lit) if t in (abi.FLOAT, abi.DOUBLE) else int(lit)
+                preds.append((c, t, int(rng.integers(0, 5)), lit))
+            pc = names[j % len(names)]
+            pr = projs[pc][j % len(projs[pc])]
+            x, t = cols[pc]
+            q = ops.scan_query([(dev[c], t, op, lit) for c, t, op, lit in preds], n,
+                               None if pr is None else (dev[pc], t, pr[0], pr[1]))
+            sel = orc.filter([(cols[c][0], t, op, lit) for c, t, op, lit in preds], n)
+            if pr is None:
+                exp = (len(sel), host_fingerprint(sel, sel), False)
+            else:
+                vals, ovf = orc.project(x, t, sel, pr[0], pr[1])
+                exp = (len(sel), None if ovf else host_fingerprint(sel, _bits(vals)), ovf)
+            work.append((q, exp))
+        rounds.append(work)
+    with ops.Server(1, budget=BUDGET, max_batch=64) as srv:
+        for work in rounds:
+            ts = _arrive_together(srv, [q for q, _ in work])
+            for (q, exp), t in zip(work, ts):
+                try:
+                    r = srv.wait(t)
+                except abi.MettleError as e:
+                    assert exp[2] and e.code == abi.MTL_ERR_OVERFLOW
+                    continue
+                assert not exp[2]
+                assert r.batch > 1
+                _check_scan(r, exp)
+            # eager device work and a synchronisation between the replays
+            z = torch.arange(1 << 20, device="cuda").sum()
+            torch.cuda.synchronize()
+            assert int(z) == (1 << 20) * ((1 << 20) - 1) // 2
+
+
 def test_shared_scan_batch_at_full_table_size(ops):
     """32 scans over one resident 100M-row INTEGER column in one batch."""
     rng = np.random.default_rng(22)
