@@ -201,6 +201,28 @@ def serving_benchmark(n_queries: int, slots: int, max_batch: int, seed: int = 20
             "identical_results": bool(same)}
 
 
+TUNED_FILE = os.path.join(ROOT, "mettle_amd", "tuned_mi355x.txt")
+
+
+def apply_tuning(args) -> dict:
+    """Load the tuned launch parameters (the autotuner's saved choices for
+    MI355X), or run the autotuner first with --autotune; returns the table
+    and where it came from, for the JSON line."""
+    from mettle_amd import abi, ops
+    L = abi.lib()
+    source = "defaults"
+    if args.autotune:
+        log("autotune:\n" + ops.autotune(ops.TUNE_ALL))
+        source = "autotuned in this run"
+    elif args.tuned:
+        if os.path.exists(args.tuned):
+            abi.check(L.mtl_tune_load(args.tuned.encode()))
+            source = os.path.relpath(args.tuned, ROOT)
+        else:
+            log(f"no tuned parameter file at {args.tuned}: defaults")
+    return {"source": source, "params": ops.tune_describe()}
+
+
 def timed(fn, steps: int, warmup: int):
     """Wall time per call of fn, which runs synchronously (it returns after
     its device work is done), in ms: median over steps."""
@@ -361,9 +383,14 @@ def main() -> int:
                     "configs 2 and 4 (parity configs, reported separately)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--tuned", default=TUNED_FILE,
+                    help="tuned launch parameters to load ('' keeps the defaults)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner on this device first (report on stderr)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
+    tuned = apply_tuning(args)
     if args.workload != "star":
         if args.gpus != 1:
             print(json.dumps({"error": "this library runs on one GPU"}))
@@ -373,7 +400,8 @@ def main() -> int:
         else:
             r = group_workload(args, args.workload == "group-dense")
         r.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-                  "higher_is_better": True, "scaling": "weak", "vs_baseline": None})
+                  "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                  "tuned": tuned})
         print(json.dumps(r), flush=True)
         return 0
     if args.gpus != 1:
@@ -500,6 +528,7 @@ def main() -> int:
         "latency_ms": {"serial_p50": lat[len(lat) // 2], "serial_p99": lat[-1],
                        "queries_per_s_serial": 1e3 / (sum(lat) / len(lat)),
                        "queries_per_s_batched": args.steps / (dev_ms / 1e3)},
+        "tuned": tuned,
     }
     L.mtl_stream_destroy(s)
     del dev

@@ -617,6 +617,42 @@ int32_t mtl_profile_reset(void);
 int32_t mtl_profile_get(uint32_t index, char *name, size_t name_len,
                         double *total_ms, uint64_t *count);
 
+/* ====================================================================== */
+/* Autotuner.  No counterpart in the reference, which fixes these launch  */
+/* parameters in code (join_plan.rs:29-56 load factors and cut-offs,      */
+/* aggregate_ops.rs:780-786 threadgroup sizes).  Parameters:              */
+/*   small_block, small_steps  the persistent small-scan kernel's block   */
+/*                             size and 256-row steps per wave (rows per  */
+/*                             lane / 4), read when a server is created;  */
+/*   join_load_pct             hash join table fill (mtl_join_table_bytes)*/
+/*   group_load_pct            group-by hash table fill;                  */
+/*   part_bits                 group-by spill partitioning fan-out bits   */
+/*                             per scatter level.                         */
+/* Every candidate value gives identical results; only speed changes.     */
+/* Set parameters at initialisation, before queries and servers run.      */
+/* ====================================================================== */
+#define MTL_TUNE_SMALL_SCAN 1u
+#define MTL_TUNE_JOIN 2u
+#define MTL_TUNE_GROUP 4u
+#define MTL_TUNE_ALL 7u
+int32_t mtl_tune_get(const char *name, int32_t *value);
+/* value must be one of the parameter's candidates, else                   */
+/* MTL_ERR_INVALID_ARGUMENT and the table is unchanged.                    */
+int32_t mtl_tune_set(const char *name, int32_t value);
+int32_t mtl_tune_reset(void); /* the defaults */
+/* "name=value" lines ('#' starts a comment).  A file with an unknown name */
+/* or a value outside the candidates is refused as a whole.                */
+int32_t mtl_tune_save(const char *path);
+int32_t mtl_tune_load(const char *path);
+/* The table as one line: "name=value name=value ...". */
+int32_t mtl_tune_describe(char *buf, size_t len);
+/* Times every candidate of the selected groups (MTL_TUNE_*) on synthetic  */
+/* device data, checks that each gives the default's result exactly       */
+/* (MTL_ERR_PROCESSING otherwise), keeps the fastest in the table and     */
+/* writes one "group: name=value time" line per candidate to report       */
+/* (may be NULL). */
+int32_t mtl_autotune(uint32_t groups, char *report, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -266,6 +266,13 @@ def lib() -> C.CDLL:
         "mtl_star_status": (i32, [u32]),
         "mtl_profile_enable": (i32, [i32]),
         "mtl_profile_reset": (i32, []),
+        "mtl_tune_get": (i32, [C.c_char_p, C.POINTER(i32)]),
+        "mtl_tune_set": (i32, [C.c_char_p, i32]),
+        "mtl_tune_reset": (i32, []),
+        "mtl_tune_save": (i32, [C.c_char_p]),
+        "mtl_tune_load": (i32, [C.c_char_p]),
+        "mtl_tune_describe": (i32, [C.c_char_p, C.c_size_t]),
+        "mtl_autotune": (i32, [u32, C.c_char_p, C.c_size_t]),
         "mtl_profile_get": (i32, [u32, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
                                   C.POINTER(u64)]),
         "mtl_group_agg": (i32, [C.POINTER(GroupDesc), C.POINTER(vp), vp]),

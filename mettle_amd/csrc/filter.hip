@@ -1777,8 +1777,6 @@ int32_t scan_batch_enqueue(const ScanBatchItem* items, uint32_t n, uint32_t* wor
 // ---------------------------------------------------------------------------
 namespace mtl {
 
-constexpr int SM_BLOCK = FBLOCK;  // 4 waves x 4 steps x 64 lanes x 4 rows = one tile
-
 struct SmallScanQ {
   PredC preds[MTL_MAX_PREDS];
   uint32_t n_preds;
@@ -1810,10 +1808,16 @@ __device__ __forceinline__ uint64_t proj_store_rt(uint32_t pk, const ExpandProj&
   }
 }
 
-__global__ __launch_bounds__(SM_BLOCK) void k_small_scan(SmallScanBlock* blk, uint32_t* res) {
+// BLOCK threads; J 256-row steps per wave and tile (lane l of step j holds
+// rows 4l..4l+3 of it): a tile is BLOCK * 4 * J rows.  Both are tuned
+// (T_SMALL_BLOCK, T_SMALL_STEPS); every pair gives the same results.
+template <int BLOCK, int J>
+__global__ __launch_bounds__(BLOCK) void k_small_scan(SmallScanBlock* blk, uint32_t* res) {
+  constexpr int W = BLOCK / WAVE;
+  constexpr uint64_t TILE = (uint64_t)BLOCK * 4 * J;
   __shared__ uint32_t s_q;
-  __shared__ uint32_t s_wtot[FWAVES];
-  __shared__ uint64_t s_h[FWAVES];
+  __shared__ uint32_t s_wtot[W];
+  __shared__ uint64_t s_h[W];
   __shared__ uint32_t s_ovf;
   const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
   const uint32_t nq = blk->nq;
@@ -1832,33 +1836,38 @@ __global__ __launch_bounds__(SM_BLOCK) void k_small_scan(SmallScanBlock* blk, ui
     uint32_t base = 0;
     bool ovf = false;
     uint64_t h = 0;
-    for (uint64_t t0 = 0; t0 < n; t0 += FTILE) {
-      const uint64_t k0 = t0 + (uint64_t)wave * (FJ * 256) + (uint64_t)lane * 4;
-      uint32_t bits = rows_below(k0, n);
+    for (uint64_t t0 = 0; t0 < n; t0 += TILE) {
+      const uint64_t k0 = t0 + (uint64_t)wave * (J * 256) + (uint64_t)lane * 4;
+      uint32_t bits = 0;
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (k0 + (uint64_t)j * 256 + v < n) bits |= 1u << (4 * j + v);
       for (uint32_t p = 0; p < Q.n_preds; ++p) {
         const PredC& P = Q.preds[p];
         const uint32_t w = pred_width(P.kind);
-        // all 16 loads issued before any compare (rows past n read row
-        // n - 1, in bounds, and are masked by rows_below)
-        uint64_t x[FJ][4];
+        // all loads issued before any compare (rows past n read row n - 1,
+        // in bounds, and are masked above)
+        uint64_t x[J][4];
 #pragma unroll
-        for (int j = 0; j < FJ; ++j)
+        for (int j = 0; j < J; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const uint64_t r = k0 + (uint64_t)j * 256 + v;
             x[j][v] = load_raw(P.col, w, r < n ? r : n - 1);
           }
 #pragma unroll
-        for (int j = 0; j < FJ; ++j)
+        for (int j = 0; j < J; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
             if (!pred_pass(enc_raw(x[j][v], P.kind), P)) bits &= ~(1u << (4 * j + v));
       }
       // positions: wave prefix per step, then the tile's wave offsets
-      uint32_t ej[FJ];
+      uint32_t ej[J];
       uint32_t wsum = 0;
 #pragma unroll
-      for (int j = 0; j < FJ; ++j) {
+      for (int j = 0; j < J; ++j) {
         const uint32_t c = (uint32_t)__popc((bits >> (4 * j)) & 0xFu);
         const uint32_t incl = wave_inclusive_u32(c);
         ej[j] = wsum + incl - c;
@@ -1868,13 +1877,13 @@ __global__ __launch_bounds__(SM_BLOCK) void k_small_scan(SmallScanBlock* blk, ui
       __syncthreads();
       uint32_t woff = 0, ttot = 0;
 #pragma unroll
-      for (int w = 0; w < FWAVES; ++w) {
+      for (int w = 0; w < W; ++w) {
         if (w < wave) woff += s_wtot[w];
         ttot += s_wtot[w];
       }
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < FJ; ++j) {
+      for (int j = 0; j < J; ++j) {
         uint32_t k = 0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -1897,7 +1906,7 @@ __global__ __launch_bounds__(SM_BLOCK) void k_small_scan(SmallScanBlock* blk, ui
     if (tid == 0) {
       uint64_t t = 0;
 #pragma unroll
-      for (int w = 0; w < FWAVES; ++w) t += s_h[w];
+      for (int w = 0; w < W; ++w) t += s_h[w];
       res[4 * q] = base;
       res[4 * q + 1] = s_ovf;
       res[4 * q + 2] = (uint32_t)t;
@@ -1905,6 +1914,25 @@ __global__ __launch_bounds__(SM_BLOCK) void k_small_scan(SmallScanBlock* blk, ui
     }
     __syncthreads();
   }
+}
+
+size_t small_block_bytes() { return sizeof(SmallScanBlock); }
+
+int32_t small_scan_launch(int32_t block, int32_t steps, uint32_t grid, void* d_blk,
+                          uint32_t* d_res, hipStream_t st) {
+  SmallScanBlock* b = (SmallScanBlock*)d_blk;
+#define MTL_SMALL_CASE(B, S)                                                            \
+  if (block == B && steps == S) {                                                      \
+    hipLaunchKernelGGL((k_small_scan<B, S>), dim3(grid), dim3(B), 0, st, b, d_res);      \
+    MTL_HIP(hipGetLastError());                                                        \
+    return MTL_OK;                                                                     \
+  }
+  MTL_SMALL_CASE(128, 1) MTL_SMALL_CASE(128, 2) MTL_SMALL_CASE(128, 4)
+  MTL_SMALL_CASE(256, 1) MTL_SMALL_CASE(256, 2) MTL_SMALL_CASE(256, 4)
+  MTL_SMALL_CASE(512, 1) MTL_SMALL_CASE(512, 2) MTL_SMALL_CASE(512, 4)
+  MTL_SMALL_CASE(1024, 1) MTL_SMALL_CASE(1024, 2) MTL_SMALL_CASE(1024, 4)
+#undef MTL_SMALL_CASE
+  return set_error(MTL_ERR_INVALID_ARGUMENT, "small scan block %d steps %d", block, steps);
 }
 
 bool small_scan_ok(const mtl_query& q) {
@@ -1958,15 +1986,24 @@ int32_t small_graph_create(mtl_stream* s, SmallGraph** out) {
   }
   // capture: upload the block, run the persistent kernel, download the
   // results (the ticket is uploaded as 0, so no clearing node is needed)
+  // with the tuned block shape at capture time; one block per query of the
+  // largest batch
   hipStream_t cs = s->stream;
   hipGraph_t graph = nullptr;
-  const uint32_t grid = (uint32_t)std::min<int>(2 * cu_count(), (int)kSmallMax);
+  const uint32_t grid = kSmallMax;
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int32_t lst = MTL_OK;
   if (e == hipSuccess) {
     hipMemcpyAsync(g->d_blk, g->h_blk, sizeof(SmallScanBlock), hipMemcpyHostToDevice, cs);
-    hipLaunchKernelGGL(k_small_scan, dim3(grid), dim3(SM_BLOCK), 0, cs, g->d_blk, g->d_res);
+    lst = small_scan_launch(tuned(T_SMALL_BLOCK), tuned(T_SMALL_STEPS), grid, g->d_blk,
+                            g->d_res, cs);
     hipMemcpyAsync(g->h_res, g->d_res, kSmallMax * 16, hipMemcpyDeviceToHost, cs);
     e = hipStreamEndCapture(cs, &graph);
+  }
+  if (lst != MTL_OK) {
+    if (graph) hipGraphDestroy(graph);
+    small_graph_free(g);
+    return lst;
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
   if (graph) hipGraphDestroy(graph);
@@ -1979,12 +2016,10 @@ int32_t small_graph_create(mtl_stream* s, SmallGraph** out) {
   return MTL_OK;
 }
 
-int32_t small_scan_enqueue(SmallGraph* g, const SmallScanItem* items, uint32_t n,
-                           mtl_stream* s) {
-  if (n == 0) return MTL_OK;
+int32_t small_scan_stage(void* h_blk, const SmallScanItem* items, uint32_t n) {
   if (n > kSmallMax) return set_error(MTL_ERR_INVALID_ARGUMENT, "more than %u small scans",
                                       kSmallMax);
-  SmallScanBlock* b = g->h_blk;
+  SmallScanBlock* b = (SmallScanBlock*)h_blk;
   b->nq = n;
   b->ticket = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -2005,6 +2040,13 @@ int32_t small_scan_enqueue(SmallGraph* g, const SmallScanItem* items, uint32_t n
     Q.out_sel = items[i].out_sel;
     Q.out_proj = items[i].out_proj;
   }
+  return MTL_OK;
+}
+
+int32_t small_scan_enqueue(SmallGraph* g, const SmallScanItem* items, uint32_t n,
+                           mtl_stream* s) {
+  if (n == 0) return MTL_OK;
+  MTL_TRY(small_scan_stage(g->h_blk, items, n));
   MTL_HIP(hipGraphLaunch(g->exec, s->stream));
   return MTL_OK;
 }

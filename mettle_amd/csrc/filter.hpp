@@ -201,5 +201,12 @@ int32_t small_scan_enqueue(SmallGraph* g, const SmallScanItem* items, uint32_t n
 // After s is synchronised: 4 uint32 per query (count, overflow flag,
 // 64-bit fingerprint), in enqueue order.
 const uint32_t* small_scan_results(const SmallGraph* g);
+// The pieces the autotuner times directly: the staged block's size, its
+// filling from items, and one launch of the kernel instantiation for
+// (block, steps) on a device copy of the block (its ticket uploaded as 0).
+size_t small_block_bytes();
+int32_t small_scan_stage(void* h_blk, const SmallScanItem* items, uint32_t n);
+int32_t small_scan_launch(int32_t block, int32_t steps, uint32_t grid, void* d_blk,
+                          uint32_t* d_res, hipStream_t st);
 
 }  // namespace mtl

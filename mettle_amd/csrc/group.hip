@@ -733,7 +733,6 @@ constexpr uint32_t kPartHistLds = 16384;  // partitions counted in LDS
 constexpr uint64_t kPartAggLds = 64 * 1024;  // LDS state of one partition
 constexpr int kPartAggThreads = 512;
 constexpr uint32_t kPartAggRows = 65536;  // rows per aggregation block
-constexpr int kPartLevelBits = 7;
 
 // Exclusive scan of one value per thread over a block of 64-lane waves;
 // s_w holds blockDim.x / 64 words.  Every thread of the block calls it.
@@ -1687,9 +1686,11 @@ static int32_t group_plan(const mtl_group_desc* d, GroupArgs* a, uint64_t* entri
     *entries = D;
   } else {
     a->dense = 0;
+    // the tuned fill (T_GROUP_LOAD percent of the slots at most)
     uint64_t want = std::min<uint64_t>(rows, a->raw ? rows : D);
+    const uint64_t load = (uint64_t)tuned(T_GROUP_LOAD);
     a->cap = 16;
-    while (a->cap < want * 2) a->cap <<= 1;
+    while (a->cap * load < want * 100) a->cap <<= 1;
     *entries = a->cap + 1;
   }
   // LDS cache of hot groups: <= 60 KB (two 768-thread blocks per CU)
@@ -1755,7 +1756,9 @@ static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t seg_max, u
   const uint32_t np = (uint32_t)np64;
   uint32_t pb = 0;
   while ((1ull << pb) < np) ++pb;
-  const uint32_t levels = pb == 0 ? 1 : (pb + kPartLevelBits - 1) / kPartLevelBits;
+  // scatter levels of at most the tuned fan-out bits each (T_PART_BITS)
+  const uint32_t lbits = (uint32_t)tuned(T_PART_BITS);
+  const uint32_t levels = pb == 0 ? 1 : (pb + lbits - 1) / lbits;
   const uint32_t lb = (pb + levels - 1) / levels;
   uint32_t ush[8];  // bucket of level l = partition >> ush[l]
   uint64_t nb[8];

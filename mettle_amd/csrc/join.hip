@@ -41,14 +41,26 @@ struct HashTable {
   uint64_t cap;   // power of two
 };
 
-__host__ __device__ inline uint64_t hash_cap_for(uint64_t rows) {
-  uint64_t want = rows * 2 > 16 ? rows * 2 : 16, c = 1;
-  while (c < want) c <<= 1;
-  return c;
-}
 __host__ __device__ inline uint64_t hash_bytes(uint64_t cap) {
   uint64_t kb = ((cap + 1) * 8 + 255) & ~255ull;
   return kb + (cap + 1) * 4;
+}
+// Slots for `rows` build rows at the tuned fill (T_JOIN_LOAD percent): the
+// smallest power of two, at least 16, with rows <= load% of it.
+inline uint64_t hash_cap_for(uint64_t rows) {
+  const uint64_t load = (uint64_t)tuned(T_JOIN_LOAD);
+  uint64_t c = 16;
+  while (c * load < rows * 100) c <<= 1;
+  return c;
+}
+// Slots of a table of `bytes` (what hash_cap_for gave when it was sized):
+// build and probe derive it from the caller's allocation, so they agree
+// whatever the tuned fill is at each call.  0: smaller than 16 slots.
+inline uint64_t hash_cap_of_bytes(uint64_t bytes) {
+  if (bytes < hash_bytes(16)) return 0;
+  uint64_t c = 16;
+  while (hash_bytes(c * 2) <= bytes) c <<= 1;
+  return c;
 }
 __host__ __device__ inline HashTable hash_view(void* base, uint64_t cap) {
   HashTable h;
@@ -250,10 +262,11 @@ int32_t mtl_join_build(int32_t strategy, const void* keys, int32_t key_type,
   if (!type_is_int(key_type)) return set_error(MTL_UNSUPPORTED, "join key type %d", key_type);
   if (!table || !dup_flag) return set_error(MTL_ERR_INVALID_ARGUMENT, "table/dup_flag NULL");
   if (strategy == MTL_JOIN_HASH) {
-    slots = hash_cap_for(capacity);
-    if (table_bytes < hash_bytes(slots))
+    // at least one empty slot besides the build rows
+    slots = hash_cap_of_bytes(table_bytes);
+    if (slots <= capacity)
       return set_error(MTL_ERR_INVALID_ARGUMENT, "hash table needs %llu bytes",
-                       (unsigned long long)hash_bytes(slots));
+                       (unsigned long long)hash_bytes(hash_cap_for(capacity)));
   } else if (table_bytes < slots * 4) {
     return set_error(MTL_ERR_INVALID_ARGUMENT, "dense table needs %llu bytes",
                      (unsigned long long)(slots * 4));
@@ -288,10 +301,10 @@ int32_t mtl_join_probe(int32_t strategy, const void* table, uint64_t table_bytes
     return set_error(MTL_ERR_INVALID_ARGUMENT, "outputs are NULL");
   if (capacity > 0xFFFFFFFFull) return set_error(MTL_UNSUPPORTED, "more than 2^32-1 rows");
   if (strategy == MTL_JOIN_HASH) {
-    // `slots` is the build capacity the table was built for
-    uint64_t cap = hash_cap_for(slots);
-    if (hash_bytes(cap) > table_bytes)
-      return set_error(MTL_ERR_INVALID_ARGUMENT, "hash table too small");
+    // the slots the build used (from the same allocation); `slots` is the
+    // build capacity, which must have left an empty slot
+    const uint64_t cap = hash_cap_of_bytes(table_bytes);
+    if (cap <= slots) return set_error(MTL_ERR_INVALID_ARGUMENT, "hash table too small");
     slots = cap;
   } else if (table_bytes < slots * 4) {
     return set_error(MTL_ERR_INVALID_ARGUMENT, "dense table too small");

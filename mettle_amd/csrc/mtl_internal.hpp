@@ -128,6 +128,27 @@ inline hipError_t stream_wait(hipStream_t st) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Tuned launch parameters: the autotuner's choices (tune.hip), or the
+// defaults below until it runs or a saved table is loaded.  Read on the host
+// when an operator plans its launches; every value gives the same results.
+// ---------------------------------------------------------------------------
+struct TuneParam {
+  const char* name;
+  int32_t value;
+  int32_t candidates[8];  // allowed values (0 ends the list)
+};
+enum TuneId : int {
+  T_SMALL_BLOCK,   // persistent small-scan kernel: threads per block
+  T_SMALL_STEPS,   // its 256-row steps per wave and tile (rows per lane / 4)
+  T_JOIN_LOAD,     // hash join table fill, percent of slots
+  T_GROUP_LOAD,    // group-by hash table fill, percent of slots
+  T_PART_BITS,     // group-by spill partitioning: fan-out bits per scatter level
+  T_COUNT
+};
+TuneParam* tune_table();
+inline int32_t tuned(TuneId id) { return tune_table()[id].value; }
+
 // Device count of CUs (for grid sizing of persistent/top-k kernels).
 inline int cu_count() { return runtime().cu_count > 0 ? runtime().cu_count : 256; }
 

@@ -340,3 +340,27 @@ def sort(column: torch.Tensor, type_code: int, n: int, sel: torch.Tensor | None 
                                  1 if desc else 0, ptr(out), C.byref(got), None))
     torch.cuda.synchronize()
     return out[:got.value], got.value
+
+
+# ---------------------------------------------------------------------------
+# Tuned launch parameters
+# ---------------------------------------------------------------------------
+TUNE_SMALL_SCAN, TUNE_JOIN, TUNE_GROUP, TUNE_ALL = 1, 2, 4, 7
+
+
+def tune_describe() -> dict:
+    """The tuned parameter table as {name: value}."""
+    buf = C.create_string_buffer(512)
+    abi.check(abi.lib().mtl_tune_describe(buf, len(buf)))
+    return {k: int(v) for k, v in (kv.split("=") for kv in buf.value.decode().split())}
+
+
+def tune_set(name: str, value: int) -> None:
+    abi.check(abi.lib().mtl_tune_set(name.encode(), value))
+
+
+def autotune(groups: int = TUNE_ALL) -> str:
+    """Run the autotuner on the device; returns its report."""
+    buf = C.create_string_buffer(16384)
+    abi.check(abi.lib().mtl_autotune(groups, buf, len(buf)))
+    return buf.value.decode()
