@@ -1877,6 +1877,15 @@ static int32_t spill_finish(GroupArgs& a, uint32_t agg_grid, uint64_t seg_max, u
 // Device bytes a grouped aggregate may take at most: the table, the spill
 // arrays of a large dense domain, and the top-K and emission scratch (an
 // upper bound for admission; 0 entries when the plan is refused).
+// Whether the plan for d indexes its table directly, so that
+// mtl_groups_fetch returns the groups ascending by their keys (the first
+// key most significant).
+bool group_is_dense(const mtl_group_desc& d) {
+  GroupArgs a;
+  uint64_t entries = 0;
+  return group_plan(&d, &a, &entries) == MTL_OK && a.dense;
+}
+
 uint64_t group_scratch_bytes(const mtl_group_desc& d) {
   GroupArgs a;
   uint64_t entries = 0;

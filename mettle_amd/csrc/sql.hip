@@ -54,6 +54,7 @@ struct mtl_result {
 
 namespace mtl {
 uint64_t group_scratch_bytes(const mtl_group_desc& d);  // group.hip
+bool group_is_dense(const mtl_group_desc& d);           // group.hip
 uint64_t query_scratch(const mtl_query& q);             // serve.hip
 namespace sql {
 
@@ -901,6 +902,110 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
   return MTL_OK;
 }
 
+// Output column of a grouped query: a group key or an aggregate.
+struct GroupOut {
+  std::string name;
+  int key = -1, agg = -1;
+};
+// How emit_groups reads the group table back: top-K by one aggregate
+// (order_agg >= 0, k = LIMIT), or every group in the table's order
+// (key_order 1: ascending by the keys, which a dense table gives; -1: that
+// order reversed; 0: as fetched), then LIMIT.
+struct GroupEmit {
+  const std::vector<const CatCol*>& keys;
+  const std::vector<int32_t>& agg_func;
+  const std::vector<int32_t>& agg_type;
+  const std::vector<int32_t>& agg_scale;
+  int order_agg;
+  bool odesc;
+  int key_order;
+};
+
+// Runs the group-by over d and materialises the result columns (DuckDB's
+// result types: COUNT BIGINT, integer SUM HUGEINT, AVG and float
+// aggregates DOUBLE, DECIMAL SUM/MIN/MAX keeping the scale).
+static int32_t emit_groups(const Query& q, mtl_group_desc& d, const GroupEmit& ge,
+                           const std::vector<GroupOut>& outs, mtl_result& res, mtl_stream* s) {
+  mtl_groups* g = nullptr;
+  MTL_TRY(mtl_group_agg(&d, &g, s));
+  struct Release {
+    mtl_groups* g;
+    ~Release() { mtl_groups_release(g); }
+  } rel{g};
+  uint64_t ng = 0;
+  std::vector<int64_t> kbuf;
+  std::vector<std::vector<uint8_t>> abuf(d.n_aggs);
+  std::vector<void*> aptr(d.n_aggs);
+  auto agg_width = [&](size_t a) -> uint64_t {
+    const int32_t f = ge.agg_func[a], t = ge.agg_type[a];
+    if (f == MTL_AGG_SUM && is_int_type(t)) return 16;
+    return 8;
+  };
+  if (ge.order_agg >= 0) {
+    const uint32_t k = (uint32_t)q.limit;
+    kbuf.resize((uint64_t)std::max<uint32_t>(k, 1) * d.n_keys);
+    for (uint32_t a = 0; a < d.n_aggs; ++a) {
+      abuf[a].resize((uint64_t)std::max<uint32_t>(k, 1) * agg_width(a));
+      aptr[a] = abuf[a].data();
+    }
+    uint32_t n = 0;
+    if (k > 0) MTL_TRY(mtl_groups_topk(g, (uint32_t)ge.order_agg, ge.odesc ? 1 : 0, k,
+                                       kbuf.data(), aptr.data(), &n));
+    ng = n;
+  } else {
+    MTL_TRY(mtl_groups_count(g, &ng));
+    kbuf.resize(std::max<uint64_t>(ng, 1) * d.n_keys);
+    for (uint32_t a = 0; a < d.n_aggs; ++a) {
+      abuf[a].resize(std::max<uint64_t>(ng, 1) * agg_width(a));
+      aptr[a] = abuf[a].data();
+    }
+    uint64_t got = 0;
+    MTL_TRY(mtl_groups_fetch(g, kbuf.data(), aptr.data(), ng, &got));
+    ng = got;
+    if (ge.key_order < 0) {  // ORDER BY the keys DESC: the dense table's order reversed
+      for (uint64_t i = 0, j = ng ? ng - 1 : 0; i < j; ++i, --j) {
+        for (uint32_t k = 0; k < d.n_keys; ++k)
+          std::swap(kbuf[i * d.n_keys + k], kbuf[j * d.n_keys + k]);
+        for (uint32_t a = 0; a < d.n_aggs; ++a) {
+          const uint64_t w = agg_width(a);
+          std::swap_ranges(abuf[a].begin() + i * w, abuf[a].begin() + (i + 1) * w,
+                           abuf[a].begin() + j * w);
+        }
+      }
+    }
+    if (q.limit >= 0 && (uint64_t)q.limit < ng) ng = (uint64_t)q.limit;
+  }
+  res.rows = ng;
+  for (auto& o : outs) {
+    mtl_result::Column col;
+    col.name = o.name;
+    if (o.key >= 0) {
+      const CatCol* c = ge.keys[o.key];
+      col.type = c->type;
+      col.scale = c->scale;
+      const int w = type_width(c->type);
+      col.data.resize(ng * w);
+      for (uint64_t r = 0; r < ng; ++r) {
+        const int64_t v = kbuf[r * d.n_keys + o.key];
+        std::memcpy(col.data.data() + r * w, &v, w);  // little-endian narrowing
+      }
+    } else {
+      const int32_t f = ge.agg_func[o.agg], t = ge.agg_type[o.agg];
+      const uint64_t w = agg_width(o.agg);
+      if (f == MTL_AGG_COUNT) col.type = MTL_BIGINT;
+      else if (f == MTL_AGG_SUM && is_int_type(t)) col.type = MTL_HUGEINT;
+      else if (f == MTL_AGG_AVG || t == MTL_FLOAT || t == MTL_DOUBLE) col.type = MTL_DOUBLE;
+      else col.type = MTL_BIGINT;
+      if ((f == MTL_AGG_SUM || f == MTL_AGG_MIN || f == MTL_AGG_MAX) && t == MTL_DECIMAL)
+        col.scale = ge.agg_scale[o.agg];
+      if (f == MTL_AGG_AVG) col.scale = 0;  // a DOUBLE: the group engine divided by 10^scale
+      col.data.assign(abuf[o.agg].begin(), abuf[o.agg].begin() + ng * w);
+    }
+    res.cols.push_back(std::move(col));
+  }
+  return MTL_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Shape 2: single-table GROUP BY
 // ---------------------------------------------------------------------------
@@ -940,10 +1045,7 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
     d.key_max[i] = keys[i]->max;
   }
   // select items: group columns or aggregates over a column
-  struct Out {
-    std::string name;
-    int key = -1, agg = -1;
-  };
+  using Out = GroupOut;
   std::vector<Out> outs;
   std::vector<int32_t> agg_func, agg_type;
   std::vector<int32_t> agg_scale;
@@ -1028,73 +1130,8 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
     d.sel_count = (const uint32_t*)cnt.p;
     d.sel_capacity = T.rows;
   }
-  mtl_groups* g = nullptr;
-  MTL_TRY(mtl_group_agg(&d, &g, s));
-  struct Release {
-    mtl_groups* g;
-    ~Release() { mtl_groups_release(g); }
-  } rel{g};
-  uint64_t ng = 0;
-  std::vector<int64_t> kbuf;
-  std::vector<std::vector<uint8_t>> abuf(d.n_aggs);
-  std::vector<void*> aptr(d.n_aggs);
-  auto agg_width = [&](size_t a) -> uint64_t {
-    const int32_t f = agg_func[a], t = agg_type[a];
-    if (f == MTL_AGG_SUM && is_int_type(t)) return 16;
-    return 8;
-  };
-  if (order_agg >= 0) {
-    const uint32_t k = (uint32_t)q.limit;
-    kbuf.resize((uint64_t)std::max<uint32_t>(k, 1) * d.n_keys);
-    for (uint32_t a = 0; a < d.n_aggs; ++a) {
-      abuf[a].resize((uint64_t)std::max<uint32_t>(k, 1) * agg_width(a));
-      aptr[a] = abuf[a].data();
-    }
-    uint32_t n = 0;
-    if (k > 0) MTL_TRY(mtl_groups_topk(g, (uint32_t)order_agg, odesc ? 1 : 0, k, kbuf.data(),
-                                       aptr.data(), &n));
-    ng = n;
-  } else {
-    MTL_TRY(mtl_groups_count(g, &ng));
-    kbuf.resize(std::max<uint64_t>(ng, 1) * d.n_keys);
-    for (uint32_t a = 0; a < d.n_aggs; ++a) {
-      abuf[a].resize(std::max<uint64_t>(ng, 1) * agg_width(a));
-      aptr[a] = abuf[a].data();
-    }
-    uint64_t got = 0;
-    MTL_TRY(mtl_groups_fetch(g, kbuf.data(), aptr.data(), ng, &got));
-    ng = got;
-    if (q.limit >= 0 && (uint64_t)q.limit < ng) ng = (uint64_t)q.limit;
-  }
-  res.rows = ng;
-  for (auto& o : outs) {
-    mtl_result::Column col;
-    col.name = o.name;
-    if (o.key >= 0) {
-      const CatCol* c = keys[o.key];
-      col.type = c->type;
-      col.scale = c->scale;
-      const int w = type_width(c->type);
-      col.data.resize(ng * w);
-      for (uint64_t r = 0; r < ng; ++r) {
-        const int64_t v = kbuf[r * d.n_keys + o.key];
-        std::memcpy(col.data.data() + r * w, &v, w);  // little-endian narrowing
-      }
-    } else {
-      const int32_t f = agg_func[o.agg], t = agg_type[o.agg];
-      const uint64_t w = agg_width(o.agg);
-      if (f == MTL_AGG_COUNT) col.type = MTL_BIGINT;
-      else if (f == MTL_AGG_SUM && is_int_type(t)) col.type = MTL_HUGEINT;
-      else if (f == MTL_AGG_AVG || t == MTL_FLOAT || t == MTL_DOUBLE) col.type = MTL_DOUBLE;
-      else col.type = MTL_BIGINT;
-      if ((f == MTL_AGG_SUM || f == MTL_AGG_MIN || f == MTL_AGG_MAX) && t == MTL_DECIMAL)
-        col.scale = agg_scale[o.agg];
-      if (f == MTL_AGG_AVG) col.scale = 0;  // a DOUBLE: the group engine divided by 10^scale
-      col.data.assign(abuf[o.agg].begin(), abuf[o.agg].begin() + ng * w);
-    }
-    res.cols.push_back(std::move(col));
-  }
-  return MTL_OK;
+  GroupEmit ge{keys, agg_func, agg_type, agg_scale, order_agg, odesc, 0};
+  return emit_groups(q, d, ge, outs, res, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1379,6 +1416,426 @@ static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
 }
 
 // ---------------------------------------------------------------------------
+// Shape 4: tables joined along equi-join edges that form a tree (each with
+// its own filters), then a projection of columns, a GROUP BY over columns of
+// any table, or aggregates without GROUP BY (one table or several).
+//   filters   : mtl_filter per table
+//   joins     : the relation starts at the table with the most filtered
+//               rows; each edge to a new table builds a join table over the
+//               side whose key is unique (the new table's filtered rows, else
+//               the relation so far), probes it with the other side, and
+//               gathers the row ids every relation column carries
+//               (mtl_join_build / mtl_join_probe / mtl_gather)
+//   outputs   : mtl_gather of each column through its table's row ids;
+//               ORDER BY one column with mtl_sort; aggregates by
+//               mtl_group_agg over the gathered key and value columns
+// Reference: the generic GPU plan (executor/mod.rs:257-386, join order and
+// strategies join_plan.rs:58-298) and ungrouped aggregates
+// (aggregate.rs:424-455).  Joins where neither side's key is unique (the
+// reference overwrites duplicate build keys, kernels.metal:202-206) return
+// MTL_UNSUPPORTED.
+// ---------------------------------------------------------------------------
+struct JoinRel {
+  std::vector<std::unique_ptr<DevBuf>> ids;  // per source: row ids (nullptr: not joined)
+  uint64_t m = 0;                            // rows of the relation
+};
+
+// Build a join table over (bkeys, btype) through bsel (brows rows, keys in
+// [bmin, bmax]) and probe it with (pkeys, ptype) through psel (prows rows;
+// NULL: rows 0..prows-1): (probe row, build row) pairs in probe order.
+// MTL_UNSUPPORTED when a build key repeats.
+static int32_t join_pairs(const void* bkeys, int32_t btype, const uint32_t* bsel, uint64_t brows,
+                          int64_t bmin, int64_t bmax, const void* pkeys, int32_t ptype,
+                          const uint32_t* psel, uint64_t prows, DevBuf& out_p, DevBuf& out_b,
+                          uint64_t& n_pairs, mtl_stream* s) {
+  n_pairs = 0;
+  MTL_TRY(out_p.alloc(std::max<uint64_t>(prows, 1) * 4));
+  MTL_TRY(out_b.alloc(std::max<uint64_t>(prows, 1) * 4));
+  if (brows == 0 || prows == 0) return MTL_OK;
+  // the dense-vs-hash rule of join_plan.rs:29-56: build >= 1024 rows, key
+  // span <= 2^24 and <= max(16 x build, 4096)
+  const uint64_t span = (uint64_t)bmax - (uint64_t)bmin + 1;
+  const bool dense = bmax >= bmin && brows >= 1024 && span <= (1ull << 24) &&
+                     span <= std::max<uint64_t>(16 * brows, 4096);
+  const int32_t strategy = dense ? MTL_JOIN_DENSE : MTL_JOIN_HASH;
+  const uint64_t slots = dense ? span : brows;
+  const uint64_t tb = mtl_join_table_bytes(strategy, brows, dense ? span : 0);
+  DevBuf table, flag, cnt;
+  MTL_TRY(table.alloc(tb));
+  MTL_TRY(flag.alloc(16));
+  MTL_TRY(cnt.alloc(16));
+  const uint32_t zero = 0, br = (uint32_t)brows;
+  MTL_TRY(mtl_memcpy_h2d(flag.p, &zero, 4, s));
+  MTL_TRY(mtl_memcpy_h2d(cnt.p, &br, 4, s));  // the build selection's count
+  MTL_TRY(mtl_join_build(strategy, bkeys, btype, bsel, bsel ? (const uint32_t*)cnt.p : nullptr,
+                         brows, bmin, slots, table.p, tb, (uint32_t*)flag.p, s));
+  uint32_t err = 0;
+  MTL_TRY(mtl_memcpy_d2h(&err, flag.p, 4, s));
+  if (err & 2u) return set_error(MTL_UNSUPPORTED, "join with duplicate keys on the build side");
+  if (err & 8u) return set_error(MTL_ERR_PROCESSING, "join key outside the column's stated range");
+  DevBuf pcnt;
+  MTL_TRY(pcnt.alloc(16));
+  uint32_t pr = (uint32_t)prows;
+  MTL_TRY(mtl_memcpy_h2d(pcnt.p, &pr, 4, s));
+  MTL_TRY(mtl_join_probe(strategy, table.p, tb, bmin, slots, pkeys, ptype, psel,
+                         psel ? (const uint32_t*)pcnt.p : nullptr, prows, (uint32_t*)out_p.p,
+                         (uint32_t*)out_b.p, (uint32_t*)cnt.p, s));
+  uint32_t n = 0;
+  MTL_TRY(mtl_memcpy_d2h(&n, cnt.p, 4, s));
+  if (n > prows) return set_error(MTL_ERR_PROCESSING, "device join failed");
+  n_pairs = n;
+  return MTL_OK;
+}
+
+// dst[i] = src[idx[i]] for i < n (uint32 row ids).
+static int32_t gather_ids(const void* src, const void* idx, uint64_t n, DevBuf& dst,
+                          mtl_stream* s) {
+  MTL_TRY(dst.alloc(std::max<uint64_t>(n, 1) * 4));
+  if (n) MTL_TRY(mtl_gather(src, 4, (const uint32_t*)idx, nullptr, n, dst.p, s));
+  return MTL_OK;
+}
+
+static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
+  const size_t ns = b.tabs.size();
+  if (ns > 4) return set_error(MTL_UNSUPPORTED, "more than four joined tables");
+  std::vector<std::vector<mtl_pred>> preds(ns);
+  for (auto& pr : q.preds) {
+    int src;
+    const CatCol* c;
+    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    mtl_pred mp{};
+    mp.column = c->dev;
+    mp.type = c->type;
+    mp.op = pr.op;
+    std::string err;
+    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
+      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+    preds[src].push_back(mp);
+    if (preds[src].size() > MTL_MAX_PREDS) return set_error(MTL_UNSUPPORTED, "too many predicates");
+  }
+  struct Edge {
+    int a, c;
+    const CatCol *ca, *cc;
+    bool used = false;
+  };
+  std::vector<Edge> edges;
+  for (auto& j : q.joins) {
+    Edge e;
+    if (!b.resolve(j.l, e.a, e.ca) || !b.resolve(j.r, e.c, e.cc))
+      return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    if (e.a == e.c) return set_error(MTL_UNSUPPORTED, "join condition within one table");
+    if (!is_int_type(e.ca->type) || !is_int_type(e.cc->type) || e.ca->type == MTL_DICT8 ||
+        e.cc->type == MTL_DICT8 || e.ca->type == MTL_DICT32 || e.cc->type == MTL_DICT32)
+      return set_error(MTL_UNSUPPORTED, "join on a non-integer column");
+    edges.push_back(e);
+  }
+  if (edges.size() != ns - 1)
+    return set_error(MTL_UNSUPPORTED, "joins must connect the tables as a tree");
+  // outputs
+  bool has_agg = false;
+  for (auto& it : q.items) has_agg |= it.e->k == Expr::AGG;
+  struct Col {
+    int src;
+    const CatCol* c;
+  };
+  std::vector<Col> proj;  // projection outputs
+  std::vector<std::string> proj_name;
+  std::vector<Col> gkeys;  // group keys
+  std::vector<const CatCol*> gkey_cols;
+  std::vector<Col> gvals;  // aggregate value columns (src -1: COUNT(*))
+  std::vector<GroupOut> gouts;
+  std::vector<int32_t> agg_func, agg_type, agg_scale;
+  mtl_group_desc d{};
+  if (!has_agg && q.group.empty()) {
+    for (size_t i = 0; i < q.items.size(); ++i) {
+      const Item& it = q.items[i];
+      if (it.e->k != Expr::COL)
+        return set_error(MTL_UNSUPPORTED, "join select item %zu is not a column", i);
+      Col c;
+      if (!b.resolve(it.e->col, c.src, c.c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+      proj.push_back(c);
+      proj_name.push_back(item_name(it, i));
+    }
+  } else {
+    for (auto& g : q.group) {
+      Col c;
+      if (!b.resolve(g, c.src, c.c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+      if (!is_int_type(c.c->type)) return set_error(MTL_UNSUPPORTED, "GROUP BY a non-integer column");
+      gkeys.push_back(c);
+      gkey_cols.push_back(c.c);
+    }
+    if (gkeys.size() > MTL_GROUP_MAX_KEYS)
+      return set_error(MTL_UNSUPPORTED, "GROUP BY more than %d columns", MTL_GROUP_MAX_KEYS);
+    for (size_t i = 0; i < q.items.size(); ++i) {
+      const Item& it = q.items[i];
+      GroupOut o;
+      o.name = item_name(it, i);
+      if (it.e->k == Expr::COL) {
+        Col c;
+        if (!b.resolve(it.e->col, c.src, c.c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+        for (size_t k = 0; k < gkeys.size(); ++k)
+          if (gkeys[k].c == c.c && gkeys[k].src == c.src) o.key = (int)k;
+        if (o.key < 0)
+          return set_error(MTL_UNSUPPORTED, "%s is neither grouped nor aggregated", c.c->name.c_str());
+      } else if (it.e->k == Expr::AGG) {
+        if (d.n_aggs >= MTL_GROUP_MAX_AGGS) return set_error(MTL_UNSUPPORTED, "too many aggregates");
+        const std::string& f = it.e->fn;
+        const int32_t func = f == "sum" ? MTL_AGG_SUM : f == "count" ? MTL_AGG_COUNT
+                             : f == "min" ? MTL_AGG_MIN : f == "max" ? MTL_AGG_MAX : MTL_AGG_AVG;
+        Col c{-1, nullptr};
+        if (it.e->a) {
+          if (it.e->a->k != Expr::COL) return set_error(MTL_UNSUPPORTED, "aggregate over an expression");
+          if (!b.resolve(it.e->a->col, c.src, c.c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+          if (c.c->type == MTL_DICT8 || c.c->type == MTL_DICT32)
+            return set_error(MTL_UNSUPPORTED, "aggregate over a dictionary column");
+        } else if (func != MTL_AGG_COUNT) {
+          return set_error(MTL_UNSUPPORTED, "%s(*)", f.c_str());
+        }
+        d.aggs[d.n_aggs].func = func;
+        d.aggs[d.n_aggs].type = c.c ? c.c->type : 0;
+        d.aggs[d.n_aggs].scale = c.c && c.c->type == MTL_DECIMAL ? c.c->scale : 0;
+        agg_func.push_back(func);
+        agg_type.push_back(c.c ? c.c->type : 0);
+        agg_scale.push_back(c.c ? c.c->scale : 0);
+        gvals.push_back(c);
+        o.agg = (int)d.n_aggs++;
+      } else {
+        return set_error(MTL_UNSUPPORTED, "select item %zu of an aggregate query", i);
+      }
+      gouts.push_back(o);
+    }
+    if (gkeys.empty())
+      for (auto& o : gouts)
+        if (o.key >= 0) return set_error(MTL_UNSUPPORTED, "a column beside ungrouped aggregates");
+  }
+  // ORDER BY
+  const bool grouped = !gkeys.empty() || has_agg;
+  int order_agg = -1, key_order = 0;
+  bool odesc = false;
+  Col okey{-1, nullptr};
+  if (!grouped) {
+    if (q.order.size() > 1) return set_error(MTL_UNSUPPORTED, "ORDER BY more than one column");
+    if (q.order.size() == 1) {
+      const ExprP& e = q.order[0].e;
+      if (e->k != Expr::COL) return set_error(MTL_UNSUPPORTED, "ORDER BY an expression");
+      for (size_t i = 0; i < proj.size(); ++i)
+        if (e->col.rel.empty() && proj_name[i] == e->col.name) okey = proj[i];
+      if (!okey.c && !b.resolve(e->col, okey.src, okey.c))
+        return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+      odesc = q.order[0].desc;
+    }
+  } else if (!q.order.empty()) {
+    // the group keys in GROUP BY order, one direction; or one aggregate + LIMIT
+    bool keys_ok = q.order.size() <= gkeys.size();
+    for (size_t i = 0; keys_ok && i < q.order.size(); ++i) {
+      const ExprP& e = q.order[i].e;
+      keys_ok = e->k == Expr::COL && q.order[i].desc == q.order[0].desc;
+      if (!keys_ok) break;
+      int src;
+      const CatCol* c = nullptr;
+      bool hit = false;
+      for (auto& o : gouts)
+        if (o.key == (int)i && e->col.rel.empty() && o.name == e->col.name) hit = true;
+      if (!hit && b.resolve(e->col, src, c)) hit = gkeys[i].c == c && gkeys[i].src == src;
+      keys_ok = hit;
+    }
+    if (keys_ok) {
+      key_order = q.order[0].desc ? -1 : 1;
+    } else if (q.order.size() == 1) {
+      const ExprP& e = q.order[0].e;
+      for (size_t i = 0; i < q.items.size(); ++i) {
+        if (gouts[i].agg < 0) continue;
+        const Item& it = q.items[i];
+        const bool by_alias = e->k == Expr::COL && e->col.rel.empty() && e->col.name == gouts[i].name;
+        const bool same_call = e->k == Expr::AGG && e->fn == it.e->fn &&
+                               ((!e->a && !it.e->a) ||
+                                (e->a && it.e->a && e->a->k == Expr::COL &&
+                                 it.e->a->k == Expr::COL && e->a->col.name == it.e->a->col.name));
+        if (by_alias || same_call) order_agg = gouts[i].agg;
+      }
+      if (order_agg < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY of an aggregate query");
+      if (q.limit < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY an aggregate needs LIMIT");
+      if (q.limit > 0xFFFFFFFFll)
+        return set_error(MTL_UNSUPPORTED, "LIMIT %lld exceeds 2^32-1", (long long)q.limit);
+      odesc = q.order[0].desc;
+    } else {
+      return set_error(MTL_UNSUPPORTED, "ORDER BY of an aggregate query");
+    }
+  }
+  // group descriptor (columns filled after the joins)
+  d.n_keys = gkeys.empty() ? 1 : (uint32_t)gkeys.size();
+  for (size_t k = 0; k < gkeys.size(); ++k) {
+    d.key_type[k] = gkeys[k].c->type;
+    d.key_min[k] = gkeys[k].c->min;
+    d.key_max[k] = gkeys[k].c->max;
+  }
+  if (gkeys.empty()) {  // ungrouped: one constant key
+    d.key_type[0] = MTL_INTEGER;
+    d.key_min[0] = d.key_max[0] = 0;
+  }
+  if (grouped && key_order != 0) {
+    // the plan's table layout depends only on the key types and ranges:
+    // placeholders stand in for the columns gathered after the joins
+    mtl_group_desc probe = d;
+    probe.n_rows = std::max<uint64_t>(1, b.tabs[0]->rows);
+    for (uint32_t k = 0; k < probe.n_keys; ++k) probe.key_col[k] = b.tabs[0];
+    for (uint32_t a = 0; a < probe.n_aggs; ++a)
+      if (gvals[a].c) probe.aggs[a].values = b.tabs[0];
+    if (!group_is_dense(probe))
+      return set_error(MTL_UNSUPPORTED, "ORDER BY group keys of a hashed group table");
+  }
+  res.shape = "join";
+  uint64_t rows_max = 0, rows_sum = 0;
+  for (auto* t : b.tabs) {
+    rows_max = std::max(rows_max, t->rows);
+    rows_sum += t->rows;
+  }
+  const uint64_t ncols = proj.size() + gkeys.size() + gvals.size() + 1;
+  res.scratch = rows_sum * 8 + rows_max * (4 * ns * 2 + 16 + 8 * ncols) +
+                (grouped ? group_scratch_bytes(d) : 0) + (1u << 20);
+  if (dry) return MTL_OK;
+  MTL_TRY(resolve_stream(s, &s));  // NULL: the library's default stream
+
+  // filters
+  std::vector<std::unique_ptr<DevBuf>> sel(ns);
+  std::vector<uint64_t> nsel(ns);
+  DevBuf cnt;
+  MTL_TRY(cnt.alloc(16));
+  for (size_t i = 0; i < ns; ++i) {
+    const CatTable& T = *b.tabs[i];
+    sel[i].reset(new DevBuf());
+    MTL_TRY(sel[i]->alloc(std::max<uint64_t>(T.rows, 1) * 4));
+    MTL_TRY(mtl_filter(preds[i].data(), (uint32_t)preds[i].size(), T.rows, nullptr, nullptr, 0,
+                       (uint32_t*)sel[i]->p, (uint32_t*)cnt.p, s));
+    uint32_t n = 0;
+    MTL_TRY(mtl_memcpy_d2h(&n, cnt.p, 4, s));
+    if (n > T.rows) return set_error(MTL_ERR_PROCESSING, "device scan failed");
+    nsel[i] = n;
+  }
+  // joins
+  JoinRel R;
+  R.ids.resize(ns);
+  size_t start = 0;
+  for (size_t i = 1; i < ns; ++i)
+    if (nsel[i] > nsel[start]) start = i;
+  R.ids[start] = std::move(sel[start]);
+  R.m = nsel[start];
+  for (size_t step = 1; step < ns; ++step) {
+    Edge* e = nullptr;
+    for (auto& x : edges)
+      if (!x.used && ((R.ids[x.a] != nullptr) != (R.ids[x.c] != nullptr))) e = &x;
+    if (!e) return set_error(MTL_UNSUPPORTED, "joins do not connect every table");
+    e->used = true;
+    int a = e->a, c = e->c;
+    const CatCol *xa = e->ca, *yc = e->cc;
+    if (!R.ids[a]) {
+      std::swap(a, c);
+      std::swap(xa, yc);
+    }
+    // the relation's key column, gathered through its row ids
+    DevBuf kx, pp, pb;
+    const uint32_t wx = (uint32_t)type_width(xa->type);
+    MTL_TRY(kx.alloc(std::max<uint64_t>(R.m, 1) * wx));
+    if (R.m) MTL_TRY(mtl_gather(xa->dev, wx, (const uint32_t*)R.ids[a]->p, nullptr, R.m, kx.p, s));
+    uint64_t np = 0;
+    // build over the new table's rows (its key unique), probe with the relation
+    int32_t st = join_pairs(yc->dev, yc->type, (const uint32_t*)sel[c]->p, nsel[c], yc->min,
+                            yc->max, kx.p, xa->type, nullptr, R.m, pp, pb, np, s);
+    if (st == MTL_UNSUPPORTED) {
+      // build over the relation (its key unique), probe with the new table
+      DevBuf pp2, pb2;
+      st = join_pairs(kx.p, xa->type, nullptr, R.m, xa->min, xa->max, yc->dev, yc->type,
+                      (const uint32_t*)sel[c]->p, nsel[c], pp2, pb2, np, s);
+      if (st == MTL_UNSUPPORTED)
+        return set_error(MTL_UNSUPPORTED, "join of %s and %s: keys repeat on both sides",
+                         xa->name.c_str(), yc->name.c_str());
+      MTL_TRY(st);
+      std::swap(pp.p, pb2.p);
+      std::swap(pp.n, pb2.n);
+      std::swap(pb.p, pp2.p);
+      std::swap(pb.n, pp2.n);
+    }
+    MTL_TRY(st);
+    // pp: relation positions, pb: rows of table c (in the order of the
+    // probing side)
+    for (size_t i = 0; i < ns; ++i) {
+      if (!R.ids[i]) continue;
+      std::unique_ptr<DevBuf> g(new DevBuf());
+      MTL_TRY(gather_ids(R.ids[i]->p, pp.p, np, *g, s));
+      R.ids[i] = std::move(g);
+    }
+    R.ids[c].reset(new DevBuf());
+    std::swap(R.ids[c]->p, pb.p);
+    std::swap(R.ids[c]->n, pb.n);
+    R.m = np;
+  }
+  const uint64_t m = R.m;
+  if (!grouped) {
+    DevBuf perm, out;
+    uint64_t n_out = m;
+    const void* order = nullptr;  // positions in output order (NULL: relation order)
+    if (okey.c) {
+      DevBuf kcol;
+      const uint32_t w = (uint32_t)type_width(okey.c->type);
+      MTL_TRY(kcol.alloc(std::max<uint64_t>(m, 1) * w));
+      if (m) MTL_TRY(mtl_gather(okey.c->dev, w, (const uint32_t*)R.ids[okey.src]->p, nullptr, m,
+                                kcol.p, s));
+      MTL_TRY(perm.alloc(std::max<uint64_t>(m, 1) * 4));
+      uint32_t n = 0;
+      MTL_TRY(mtl_sort(kcol.p, okey.c->type, nullptr, nullptr, m, odesc ? 1 : 0,
+                       (uint32_t*)perm.p, &n, s));
+      order = perm.p;
+    }
+    if (q.limit >= 0 && (uint64_t)q.limit < n_out) n_out = (uint64_t)q.limit;
+    res.rows = n_out;
+    MTL_TRY(out.alloc(std::max<uint64_t>(n_out, 1) * 8));
+    for (size_t i = 0; i < proj.size(); ++i) {
+      mtl_result::Column col;
+      col.name = proj_name[i];
+      col.type = proj[i].c->type;
+      col.scale = proj[i].c->scale;
+      const uint32_t w = (uint32_t)type_width(col.type);
+      const void* rows = R.ids[proj[i].src]->p;
+      DevBuf idx;  // the column's row ids in output order
+      if (order && n_out) {
+        MTL_TRY(gather_ids(rows, order, n_out, idx, s));
+        rows = idx.p;
+      }
+      if (n_out) MTL_TRY(mtl_gather(proj[i].c->dev, w, (const uint32_t*)rows, nullptr, n_out, out.p, s));
+      MTL_TRY(download(col.data, out.p, n_out * w, s));
+      res.cols.push_back(std::move(col));
+    }
+    return MTL_OK;
+  }
+  // aggregates over the gathered key and value columns
+  if (gkeys.empty() && m == 0)
+    return set_error(MTL_UNSUPPORTED, "aggregates over no rows (NULL results)");
+  std::vector<std::unique_ptr<DevBuf>> mats;
+  auto gathered = [&](const Col& c, const void** out) -> int32_t {
+    const uint32_t w = (uint32_t)type_width(c.c->type);
+    mats.emplace_back(new DevBuf());
+    MTL_TRY(mats.back()->alloc(std::max<uint64_t>(m, 1) * w));
+    if (m) MTL_TRY(mtl_gather(c.c->dev, w, (const uint32_t*)R.ids[c.src]->p, nullptr, m,
+                              mats.back()->p, s));
+    *out = mats.back()->p;
+    return MTL_OK;
+  };
+  d.n_rows = m;
+  if (gkeys.empty()) {
+    mats.emplace_back(new DevBuf());
+    MTL_TRY(mats.back()->alloc(std::max<uint64_t>(m, 1) * 4));
+    MTL_HIP(hipMemsetAsync(mats.back()->p, 0, std::max<uint64_t>(m, 1) * 4, s->stream));
+    d.key_col[0] = mats.back()->p;
+  }
+  for (size_t k = 0; k < gkeys.size(); ++k) MTL_TRY(gathered(gkeys[k], &d.key_col[k]));
+  for (size_t a = 0; a < gvals.size(); ++a)
+    if (gvals[a].c) MTL_TRY(gathered(gvals[a], &d.aggs[a].values));
+  std::vector<const CatCol*> ek = gkey_cols;
+  GroupEmit ge{ek, agg_func, agg_type, agg_scale, order_agg, odesc, key_order};
+  return emit_groups(q, d, ge, gouts, res, s);
+}
+
+// ---------------------------------------------------------------------------
 // Driver
 // ---------------------------------------------------------------------------
 static int32_t plan_and_run(const char* sql, mtl_result& res, mtl_stream* s, bool dry) {
@@ -1402,12 +1859,22 @@ static int32_t plan_and_run(const char* sql, mtl_result& res, mtl_stream* s, boo
   for (auto& it : q.items) has_agg |= it.e->k == Expr::AGG;
   if (b.tabs.size() == 1 && q.joins.empty()) {
     if (q.group.empty() && !has_agg) return run_scan(q, b, res, s, dry);
-    if (!q.group.empty()) return run_group(q, b, res, s, dry);
-    return set_error(MTL_UNSUPPORTED, "aggregates without GROUP BY");
+    if (!q.group.empty()) {
+      // ORDER BY the group keys is the generic path's
+      const int32_t st = run_group(q, b, res, s, dry);
+      if (st != MTL_UNSUPPORTED) return st;
+      res = mtl_result{};
+    }
+    return run_join(q, b, res, s, dry);  // aggregates without GROUP BY
   }
-  if ((b.tabs.size() == 2 || b.tabs.size() == 3) && !q.group.empty() && has_agg)
-    return run_star(q, b, res, s, dry);
-  return set_error(MTL_UNSUPPORTED, "query shape not handled on the GPU path");
+  if ((b.tabs.size() == 2 || b.tabs.size() == 3) && !q.group.empty() && has_agg) {
+    // the star pipeline when the query has its shape (grouped by the parent
+    // key, one SUM, top-K), else the generic joins
+    const int32_t st = run_star(q, b, res, s, dry);
+    if (st != MTL_UNSUPPORTED) return st;
+    res = mtl_result{};
+  }
+  return run_join(q, b, res, s, dry);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,189 @@
+"""The generic join lowering: the reference's SQL fixtures for ungrouped
+aggregates, a two-table join with a projection, and joins grouped by
+non-key columns (tests/extension.rs:239-276, 279-317, 475-525, 973-1016,
+expected values as written there), then larger random joins checked
+against numpy/pandas (merge, groupby) on the host, and the shapes that fall
+back (keys repeating on both sides of a join)."""
+import numpy as np
+import pytest
+
+from mettle_amd import abi
+from mettle_amd import sql as S
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+pd = pytest.importorskip("pandas")
+
+
+@pytest.fixture(scope="module")
+def ops():
+    from mettle_amd import ops as _ops
+    return _ops
+
+
+@pytest.fixture(autouse=True)
+def clean_catalog():
+    S.drop()
+    yield
+    S.drop()
+
+
+_keep = []
+
+
+def _table(ops, name, cols):
+    """cols: list of (name, type, numpy values[, unique]); registers the
+    table with each column's [min, max]."""
+    n = len(cols[0][2])
+    spec = []
+    for c in cols:
+        cname, t, v = c[:3]
+        unique = c[3] if len(c) > 3 else False
+        d = ops.to_device(np.ascontiguousarray(v))
+        _keep.append(d)
+        mn, mx = (int(v.min()), int(v.max())) if n and t not in (abi.FLOAT, abi.DOUBLE) else (0, 0)
+        spec.append((cname, t, d.data_ptr(), mn, mx, unique))
+    S.register_table(name, n, spec)
+
+
+def _ints(out, name):
+    t, sc, v = out[name]
+    if t == abi.HUGEINT:
+        return [int(x) for x in v]
+    return [int(x) for x in np.asarray(v)]
+
+
+def test_reference_ungrouped_aggregate(ops):
+    # tests/extension.rs:239-276
+    _table(ops, "prices", [("value", abi.INTEGER, np.array([10, 20, 30, 40, 50], np.int32))])
+    out = S.execute("SELECT SUM(value) AS total, COUNT(*) AS cnt, MIN(value) AS lo, "
+                    "MAX(value) AS hi FROM prices WHERE value >= 20")
+    assert list(out) == ["total", "cnt", "lo", "hi"]
+    assert (_ints(out, "total"), _ints(out, "cnt"), _ints(out, "lo"), _ints(out, "hi")) == \
+        ([140], [4], [20], [50])
+
+
+def test_reference_two_table_join(ops):
+    # tests/extension.rs:279-317 (dept_name VARCHAR, dictionary-coded here)
+    _table(ops, "departments", [("dept_id", abi.INTEGER, np.array([1, 2, 3], np.int32), True),
+                                ("dept_name", abi.DICT8, np.array([0, 1, 2], np.uint8))])
+    S.set_dictionary("departments", "dept_name", ["Engineering", "Sales", "HR"])
+    _table(ops, "employees", [("emp_id", abi.INTEGER, np.array([10, 11, 12, 13], np.int32), True),
+                              ("dept_id", abi.INTEGER, np.array([1, 1, 2, 3], np.int32)),
+                              ("salary", abi.INTEGER, np.array([100, 200, 150, 300], np.int32))])
+    out = S.execute("SELECT e.emp_id, e.salary FROM departments d JOIN employees e "
+                    "ON d.dept_id = e.dept_id WHERE d.dept_name = 'Engineering'")
+    rows = sorted(zip(_ints(out, "emp_id"), _ints(out, "salary")))
+    assert rows == [(10, 100), (11, 200)]
+
+
+def test_reference_two_table_grouped_aggregate(ops):
+    # tests/extension.rs:973-1016
+    _table(ops, "departments", [("id", abi.INTEGER, np.array([1, 2, 3], np.int32), True),
+                                ("region", abi.INTEGER, np.array([10, 10, 20], np.int32))])
+    _table(ops, "employees", [("dept_id", abi.INTEGER, np.array([1, 1, 2, 3, 3], np.int32)),
+                              ("salary", abi.DOUBLE,
+                               np.array([100.0, 150.0, 200.0, 90.0, 110.0]))])
+    out = S.execute("SELECT d.region, SUM(e.salary) AS payroll FROM departments d "
+                    "JOIN employees e ON d.id = e.dept_id GROUP BY d.region ORDER BY d.region")
+    assert _ints(out, "region") == [10, 20]
+    assert list(out["payroll"][2]) == [450.0, 200.0]
+
+
+def test_reference_three_table_grouped_aggregate(ops):
+    # tests/extension.rs:475-525
+    _table(ops, "regions", [("region_id", abi.INTEGER, np.array([10, 20], np.int32), True),
+                            ("market", abi.INTEGER, np.array([1, 2], np.int32))])
+    _table(ops, "departments", [("dept_id", abi.INTEGER, np.array([1, 2, 3], np.int32), True),
+                                ("region_id", abi.INTEGER, np.array([10, 10, 20], np.int32))])
+    _table(ops, "employees", [("emp_id", abi.INTEGER, np.arange(100, 105, dtype=np.int32), True),
+                              ("dept_id", abi.INTEGER, np.array([1, 1, 2, 3, 3], np.int32)),
+                              ("salary", abi.DOUBLE,
+                               np.array([100.0, 150.0, 200.0, 90.0, 110.0]))])
+    out = S.execute("SELECT r.market, SUM(e.salary) AS payroll FROM regions r "
+                    "JOIN departments d ON r.region_id = d.region_id "
+                    "JOIN employees e ON d.dept_id = e.dept_id "
+                    "GROUP BY r.market ORDER BY r.market")
+    assert _ints(out, "market") == [1, 2]
+    assert list(out["payroll"][2]) == [450.0, 200.0]
+
+
+def _star(ops, seed, n_c=3000, n_o=40_000, n_l=200_000):
+    rng = np.random.default_rng(seed)
+    cust = {"c_key": rng.permutation(n_c).astype(np.int32) + 1,
+            "c_region": rng.integers(0, 7, n_c).astype(np.int32)}
+    orders = {"o_key": rng.permutation(n_o).astype(np.int32) * 3 + 5,
+              "o_cust": rng.integers(1, n_c + 1, n_o).astype(np.int32),
+              "o_prio": rng.integers(0, 5, n_o).astype(np.int32)}
+    lines = {"l_order": orders["o_key"][rng.integers(0, n_o, n_l)],
+             "l_qty": rng.integers(-50, 51, n_l).astype(np.int64),
+             "l_disc": rng.integers(0, 11, n_l).astype(np.int32)}
+    _table(ops, "cust", [("c_key", abi.INTEGER, cust["c_key"], True),
+                         ("c_region", abi.INTEGER, cust["c_region"])])
+    _table(ops, "orders", [("o_key", abi.INTEGER, orders["o_key"], True),
+                           ("o_cust", abi.INTEGER, orders["o_cust"]),
+                           ("o_prio", abi.INTEGER, orders["o_prio"])])
+    _table(ops, "lines", [("l_order", abi.INTEGER, lines["l_order"]),
+                          ("l_qty", abi.BIGINT, lines["l_qty"]),
+                          ("l_disc", abi.INTEGER, lines["l_disc"])])
+    return pd.DataFrame(cust), pd.DataFrame(orders), pd.DataFrame(lines)
+
+
+def test_three_table_join_grouped_by_non_key_columns(ops):
+    c, o, l = _star(ops, 1)
+    out = S.execute("SELECT c.c_region, o.o_prio, SUM(l.l_qty) AS q, COUNT(*) AS n, "
+                    "MIN(l.l_disc) AS lo, MAX(l.l_qty) AS hi "
+                    "FROM lines l JOIN orders o ON l.l_order = o.o_key "
+                    "JOIN cust c ON o.o_cust = c.c_key "
+                    "WHERE o.o_prio < 4 AND l.l_disc >= 2 "
+                    "GROUP BY c.c_region, o.o_prio ORDER BY c.c_region, o.o_prio")
+    j = l[l.l_disc >= 2].merge(o[o.o_prio < 4], left_on="l_order", right_on="o_key") \
+        .merge(c, left_on="o_cust", right_on="c_key")
+    g = j.groupby(["c_region", "o_prio"]).agg(q=("l_qty", "sum"), n=("l_qty", "size"),
+                                              lo=("l_disc", "min"), hi=("l_qty", "max"))
+    g = g.reset_index().sort_values(["c_region", "o_prio"])
+    assert _ints(out, "c_region") == list(g.c_region)
+    assert _ints(out, "o_prio") == list(g.o_prio)
+    assert _ints(out, "q") == [int(x) for x in g.q]
+    assert _ints(out, "n") == [int(x) for x in g.n]
+    assert _ints(out, "lo") == [int(x) for x in g.lo]
+    assert _ints(out, "hi") == [int(x) for x in g.hi]
+    # descending key order: the same groups reversed
+    out_d = S.execute("SELECT c.c_region, SUM(l.l_qty) AS q FROM lines l "
+                      "JOIN orders o ON l.l_order = o.o_key JOIN cust c ON o.o_cust = c.c_key "
+                      "GROUP BY c.c_region ORDER BY c.c_region DESC")
+    g2 = l.merge(o, left_on="l_order", right_on="o_key").merge(c, left_on="o_cust",
+                                                               right_on="c_key")
+    g2 = g2.groupby("c_region").l_qty.sum().sort_index(ascending=False)
+    assert _ints(out_d, "c_region") == list(g2.index)
+    assert _ints(out_d, "q") == [int(x) for x in g2]
+
+
+def test_join_projection_with_order_and_limit(ops):
+    c, o, l = _star(ops, 2)
+    out = S.execute("SELECT o.o_key, c.c_region, o.o_prio FROM orders o "
+                    "JOIN cust c ON o.o_cust = c.c_key WHERE c.c_region = 3 AND o.o_prio >= 2 "
+                    "ORDER BY o.o_key DESC LIMIT 500")
+    j = o[o.o_prio >= 2].merge(c[c.c_region == 3], left_on="o_cust", right_on="c_key")
+    j = j.sort_values("o_key", ascending=False, kind="stable").head(500)
+    assert _ints(out, "o_key") == list(j.o_key)
+    assert _ints(out, "c_region") == list(j.c_region)
+    assert _ints(out, "o_prio") == list(j.o_prio)
+
+
+def test_ungrouped_aggregates_over_a_join(ops):
+    c, o, l = _star(ops, 3)
+    out = S.execute("SELECT COUNT(*) AS n, SUM(l.l_qty) AS q, MAX(o.o_prio) AS p "
+                    "FROM lines l JOIN orders o ON l.l_order = o.o_key WHERE o.o_prio = 1")
+    j = l.merge(o[o.o_prio == 1], left_on="l_order", right_on="o_key")
+    assert (_ints(out, "n"), _ints(out, "q"), _ints(out, "p")) == \
+        ([len(j)], [int(j.l_qty.sum())], [1])
+
+
+def test_join_with_keys_repeating_on_both_sides_falls_back(ops):
+    _table(ops, "a", [("k", abi.INTEGER, np.array([1, 1, 2], np.int32)),
+                      ("v", abi.INTEGER, np.array([5, 6, 7], np.int32))])
+    _table(ops, "b", [("k", abi.INTEGER, np.array([1, 2, 2], np.int32)),
+                      ("w", abi.INTEGER, np.array([8, 9, 10], np.int32))])
+    with pytest.raises(abi.Unsupported):
+        S.execute("SELECT a.v, b.w FROM a JOIN b ON a.k = b.k")

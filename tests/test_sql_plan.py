@@ -58,6 +58,18 @@ Q3 = ("SELECT l.l_order, SUM(l.price * (100 - l.discount)) AS revenue, o.o_date,
     (Q3, "star3"),
     ("SELECT o.o_key, SUM(l.price) AS r FROM orders o JOIN lines l ON l.l_order = o.o_key "
      "GROUP BY o.o_key ORDER BY r DESC LIMIT 3", "star2"),
+    # the generic joins: grouped by a non-key column, projections, ungrouped
+    ("SELECT o.o_date, SUM(l.price) AS r FROM orders o, lines l WHERE l.l_order = o.o_key "
+     "GROUP BY o.o_date ORDER BY r DESC LIMIT 3", "join"),
+    ("SELECT g, SUM(c_i) FROM t GROUP BY g ORDER BY g DESC", "join"),
+    ("SELECT SUM(value) AS total, COUNT(*) AS cnt, MIN(value), MAX(value) FROM numbers "
+     "WHERE value >= 20", "join"),
+    ("SELECT o.o_key, l.price FROM orders o JOIN lines l ON l.l_order = o.o_key "
+     "WHERE o.o_prio = 2 ORDER BY l.price DESC LIMIT 10", "join"),
+    ("SELECT c.segment, o.o_prio, COUNT(*), SUM(l.discount) FROM customers c "
+     "JOIN orders o ON c.c_key = o.o_cust JOIN lines l ON l.l_order = o.o_key "
+     "GROUP BY c.segment, o.o_prio ORDER BY c.segment, o.o_prio", "join"),
+    ("SELECT COUNT(*) FROM orders o JOIN lines l ON l.l_order = o.o_key", "join"),
 ])
 def test_admitted_shapes(sql, shape):
     assert S.explain(sql) == shape
@@ -75,8 +87,15 @@ def test_admitted_shapes(sql, shape):
     ("SELECT g, SUM(c_b) FROM t GROUP BY g HAVING SUM(c_b) > 0", "HAVING"),
     ("SELECT segment FROM customers WHERE segment < 'B'", "only '='"),
     ("SELECT price FROM t WHERE price > 1.255", "more scale"),
+    ("SELECT o.o_key, l.price FROM orders o, lines l", "tree"),
+    ("SELECT o.o_key, l.price FROM orders o JOIN lines l ON l.l_order = o.o_key "
+     "ORDER BY o.o_key, l.price", "more than one"),
+    ("SELECT SUM(c_b), c_i FROM t", "neither grouped"),
     ("SELECT o.o_key, SUM(l.price) AS r FROM orders o, lines l WHERE l.l_order = o.o_key "
-     "GROUP BY o.o_date ORDER BY r DESC LIMIT 3", "parent key"),
+     "GROUP BY o.o_date ORDER BY r DESC LIMIT 3", "neither grouped"),
+    ("SELECT c_b, SUM(c_i) FROM t GROUP BY c_b ORDER BY c_b", "hashed"),
+    ("SELECT o.o_prio, SUM(l.price) FROM orders o JOIN lines l ON l.l_order = o.o_key "
+     "GROUP BY o.o_prio ORDER BY SUM(l.discount)", "ORDER BY of an aggregate"),
     ("SELECT c_b FROM t WHERE c_b = 1; SELECT 1", "unexpected"),
     ("SELECT g, SUM(c_b) AS s FROM t GROUP BY g ORDER BY s DESC LIMIT 4294967296",
      "exceeds 2^32-1"),
