@@ -486,6 +486,45 @@ int32_t mtl_result_column(const mtl_result *r, uint32_t i, const char **name,
                           int32_t *type, int32_t *scale, const void **data);
 int32_t mtl_result_free(mtl_result *r);
 
+/* Arrow C data interface structs, as the specification defines them
+ * (arrow.apache.org/docs/format/CDataInterface.html). */
+#ifndef ARROW_C_DATA_INTERFACE
+#define ARROW_C_DATA_INTERFACE
+struct ArrowSchema {
+  const char *format;
+  const char *name;
+  const char *metadata;
+  int64_t flags;
+  int64_t n_children;
+  struct ArrowSchema **children;
+  struct ArrowSchema *dictionary;
+  void (*release)(struct ArrowSchema *);
+  void *private_data;
+};
+struct ArrowArray {
+  int64_t length;
+  int64_t null_count;
+  int64_t offset;
+  int64_t n_buffers;
+  int64_t n_children;
+  const void **buffers;
+  struct ArrowArray **children;
+  struct ArrowArray *dictionary;
+  void (*release)(struct ArrowArray *);
+  void *private_data;
+};
+#endif
+/* The result as one Arrow record batch: a struct array ("+s") with one
+ * child per column, in select order.  Types: INTEGER int32, BIGINT int64,
+ * FLOAT float32, DOUBLE float64, DATE date32, TIMESTAMP timestamp[us],
+ * DECIMAL decimal128(18, scale), HUGEINT sums decimal128(38, scale),
+ * dictionary columns dictionary<uint8 | int32, utf8>.  No NULLs.  The
+ * caller owns both structs (release callbacks); they stay valid after
+ * mtl_result_free.  Replaces the RecordBatch hand-back of gpu_processing
+ * (processing.rs:77-96), whose computed columns were all Float64. */
+int32_t mtl_result_export_arrow(const mtl_result *r, struct ArrowSchema *schema,
+                                struct ArrowArray *array);
+
 /* gpu_rewrite_query(sql): the query printed back with every predicate
  * `col op literal` written as gpu_{eq,lt,le,gt,ge}_scalar(col, literal) and
  * every projection `col + literal` / `col * literal` as

@@ -266,6 +266,7 @@ def lib() -> C.CDLL:
         "mtl_star_status": (i32, [u32]),
         "mtl_profile_enable": (i32, [i32]),
         "mtl_profile_reset": (i32, []),
+        "mtl_result_export_arrow": (i32, [vp, vp, vp]),
         "mtl_tune_get": (i32, [C.c_char_p, C.POINTER(i32)]),
         "mtl_tune_set": (i32, [C.c_char_p, i32]),
         "mtl_tune_reset": (i32, []),

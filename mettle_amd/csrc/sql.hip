@@ -45,6 +45,7 @@ struct mtl_result {
     int32_t type;
     int32_t scale = 0;  // DECIMAL scale
     std::vector<uint8_t> data;
+    std::vector<std::string> dict;  // dictionary columns: the value of each code
   };
   std::vector<Column> cols;
   uint64_t rows = 0;
@@ -882,6 +883,7 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     col.name = o.name;
     col.type = o.c->type;
     col.scale = o.c->scale;
+    col.dict = o.c->dict;
     const uint32_t w = (uint32_t)type_width(o.c->type);
     if (o.op < 0) {
       MTL_TRY(mtl_gather(o.c->dev, w, rows, nullptr, n, out.p, s));
@@ -983,6 +985,7 @@ static int32_t emit_groups(const Query& q, mtl_group_desc& d, const GroupEmit& g
       const CatCol* c = ge.keys[o.key];
       col.type = c->type;
       col.scale = c->scale;
+      col.dict = c->dict;
       const int w = type_width(c->type);
       col.data.resize(ng * w);
       for (uint64_t r = 0; r < ng; ++r) {
@@ -1401,6 +1404,7 @@ static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     if (o.c) {
       col.type = o.c->type;
       col.scale = o.c->scale;
+      col.dict = o.c->dict;
       const uint32_t w = (uint32_t)type_width(o.c->type);
       if (n) MTL_TRY(mtl_gather(o.c->dev, w, (const uint32_t*)prow.p, nullptr, n, out.p, s));
       MTL_TRY(download(col.data, out.p, (uint64_t)n * w, s));
@@ -1794,6 +1798,7 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
       col.name = proj_name[i];
       col.type = proj[i].c->type;
       col.scale = proj[i].c->scale;
+      col.dict = proj[i].c->dict;
       const uint32_t w = (uint32_t)type_width(col.type);
       const void* rows = R.ids[proj[i].src]->p;
       DevBuf idx;  // the column's row ids in output order
@@ -2074,6 +2079,209 @@ int32_t mtl_result_column(const mtl_result* r, uint32_t i, const char** name, in
 
 int32_t mtl_result_free(mtl_result* r) {
   delete r;
+  return MTL_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Arrow hand-back: a result as one record batch in the Arrow C data
+// interface (a struct array with one child per column).  Every array and
+// schema owns its buffers through private_data and frees them in its
+// release callback, so the consumer (DuckDB's Arrow scan, pyarrow) keeps
+// the batch independent of the mtl_result.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ArrowOwn {
+  std::vector<std::vector<uint8_t>> bufs;
+  std::vector<const void*> ptrs;
+  std::vector<ArrowArray*> kids;
+  ArrowArray* dict = nullptr;
+};
+
+struct SchemaOwn {
+  std::string format, name;
+  std::vector<ArrowSchema*> kids;
+  ArrowSchema* dict = nullptr;
+};
+
+void release_array(ArrowArray* a) {
+  if (!a || !a->release) return;
+  auto* own = (ArrowOwn*)a->private_data;
+  for (ArrowArray* k : own->kids) {
+    if (k->release) k->release(k);
+    delete k;
+  }
+  if (own->dict) {
+    if (own->dict->release) own->dict->release(own->dict);
+    delete own->dict;
+  }
+  delete own;
+  a->release = nullptr;
+}
+
+void release_schema(ArrowSchema* s) {
+  if (!s || !s->release) return;
+  auto* own = (SchemaOwn*)s->private_data;
+  for (ArrowSchema* k : own->kids) {
+    if (k->release) k->release(k);
+    delete k;
+  }
+  if (own->dict) {
+    if (own->dict->release) own->dict->release(own->dict);
+    delete own->dict;
+  }
+  delete own;
+  s->release = nullptr;
+}
+
+void make_schema(ArrowSchema* s, const std::string& format, const std::string& name,
+                 std::vector<ArrowSchema*> kids, ArrowSchema* dict) {
+  auto* own = new SchemaOwn{format, name, std::move(kids), dict};
+  s->format = own->format.c_str();
+  s->name = own->name.c_str();
+  s->metadata = nullptr;
+  s->flags = 0;  // not nullable: these results hold no NULLs
+  s->n_children = (int64_t)own->kids.size();
+  s->children = own->kids.empty() ? nullptr : own->kids.data();
+  s->dictionary = dict;
+  s->release = release_schema;
+  s->private_data = own;
+}
+
+// buffers: the validity slot (always absent: no NULLs) then `bufs`
+void make_array(ArrowArray* a, int64_t length, std::vector<std::vector<uint8_t>> bufs,
+                std::vector<ArrowArray*> kids, ArrowArray* dict) {
+  auto* own = new ArrowOwn();
+  own->bufs = std::move(bufs);
+  own->ptrs.push_back(nullptr);
+  for (auto& b : own->bufs) own->ptrs.push_back(b.data());
+  own->kids = std::move(kids);
+  own->dict = dict;
+  a->length = length;
+  a->null_count = 0;
+  a->offset = 0;
+  a->n_buffers = (int64_t)own->ptrs.size();
+  a->n_children = (int64_t)own->kids.size();
+  a->buffers = own->ptrs.data();
+  a->children = own->kids.empty() ? nullptr : own->kids.data();
+  a->dictionary = dict;
+  a->release = release_array;
+  a->private_data = own;
+}
+
+// A utf8 array of the dictionary's values.
+void make_utf8(ArrowSchema* s, ArrowArray* a, const std::vector<std::string>& vals) {
+  std::vector<uint8_t> offs((vals.size() + 1) * 4), chars;
+  int32_t at = 0;
+  std::memcpy(offs.data(), &at, 4);
+  for (size_t i = 0; i < vals.size(); ++i) {
+    chars.insert(chars.end(), vals[i].begin(), vals[i].end());
+    at += (int32_t)vals[i].size();
+    std::memcpy(offs.data() + (i + 1) * 4, &at, 4);
+  }
+  make_schema(s, "u", "", {}, nullptr);
+  make_array(a, (int64_t)vals.size(), {std::move(offs), std::move(chars)}, {}, nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mtl_result_export_arrow(const mtl_result* r, ArrowSchema* schema, ArrowArray* array) {
+  using namespace mtl;
+  if (!r || !schema || !array) return set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  const int64_t n = (int64_t)r->rows;
+  std::vector<ArrowSchema*> cs;
+  std::vector<ArrowArray*> ca;
+  auto fail = [&](const char* why, int32_t t) {
+    for (auto* s : cs) {
+      s->release(s);
+      delete s;
+    }
+    for (auto* a : ca) {
+      a->release(a);
+      delete a;
+    }
+    return set_error(MTL_UNSUPPORTED, "%s (type %d)", why, t);
+  };
+  for (const auto& c : r->cols) {
+    auto* s = new ArrowSchema();
+    auto* a = new ArrowArray();
+    const int w = type_width(c.type);
+    if ((uint64_t)c.data.size() < (uint64_t)n * (uint64_t)w) {
+      delete s;
+      delete a;
+      return fail("result column shorter than its rows", c.type);
+    }
+    std::vector<uint8_t> data(c.data.begin(), c.data.begin() + (size_t)n * w);
+    std::string fmt;
+    switch (c.type) {
+      case MTL_INTEGER: fmt = "i"; break;
+      case MTL_BIGINT: fmt = "l"; break;
+      case MTL_FLOAT: fmt = "f"; break;
+      case MTL_DOUBLE: fmt = "g"; break;
+      case MTL_DATE: fmt = "tdD"; break;
+      case MTL_TIMESTAMP: fmt = "tsu:"; break;
+      case MTL_DECIMAL:
+      case MTL_HUGEINT: {
+        // decimal128 (DuckDB's Arrow type for DECIMAL and HUGEINT): int64
+        // values sign-extended to 16 bytes, HUGEINT {lower, upper} as is
+        fmt = std::string("d:") + (c.type == MTL_DECIMAL ? "18," : "38,") +
+              std::to_string(c.scale);
+        if (c.type == MTL_DECIMAL) {
+          std::vector<uint8_t> wide((size_t)n * 16);
+          for (int64_t i = 0; i < n; ++i) {
+            int64_t v;
+            std::memcpy(&v, data.data() + i * 8, 8);
+            const int64_t hi = v < 0 ? -1 : 0;
+            std::memcpy(wide.data() + i * 16, &v, 8);
+            std::memcpy(wide.data() + i * 16 + 8, &hi, 8);
+          }
+          data.swap(wide);
+        }
+        break;
+      }
+      case MTL_DICT8:
+      case MTL_DICT32: {
+        // dictionary<uint8 | int32, utf8>; codes without a dictionary as
+        // plain integers
+        if (c.dict.empty()) {
+          fmt = c.type == MTL_DICT8 ? "C" : "i";
+          break;
+        }
+        for (int64_t i = 0; i < n; ++i) {
+          uint32_t code = 0;
+          if (c.type == MTL_DICT8) code = data[i];
+          else std::memcpy(&code, data.data() + i * 4, 4);
+          if (code >= c.dict.size()) {
+            delete s;
+            delete a;
+            return fail("dictionary code outside the dictionary", c.type);
+          }
+        }
+        auto* ds = new ArrowSchema();
+        auto* da = new ArrowArray();
+        make_utf8(ds, da, c.dict);
+        make_schema(s, c.type == MTL_DICT8 ? "C" : "i", c.name, {}, ds);
+        make_array(a, n, {std::move(data)}, {}, da);
+        cs.push_back(s);
+        ca.push_back(a);
+        continue;
+      }
+      default:
+        delete s;
+        delete a;
+        return fail("no Arrow export for the result type", c.type);
+    }
+    make_schema(s, fmt, c.name, {}, nullptr);
+    make_array(a, n, {std::move(data)}, {}, nullptr);
+    cs.push_back(s);
+    ca.push_back(a);
+  }
+  make_schema(schema, "+s", "", std::move(cs), nullptr);
+  make_array(array, n, {}, std::move(ca), nullptr);
   return MTL_OK;
 }
 

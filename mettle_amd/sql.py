@@ -106,3 +106,22 @@ def scalar_hook(hook: int, values: np.ndarray, type_code: int, scalar) -> np.nda
     abi.check(abi.lib().mtl_scalar_hook(hook, type_code, values.ctypes.data, len(values),
                                         abi.scalar_bits(scalar, type_code), out.ctypes.data))
     return out
+
+
+def execute_arrow(sql: str):
+    """Run through the GPU path and hand the result back as a
+    pyarrow.RecordBatch through the Arrow C data interface
+    (mtl_result_export_arrow)."""
+    import pyarrow as pa
+    L = abi.lib()
+    h = C.c_void_p()
+    abi.check(L.mtl_sql_execute(sql.encode(), C.byref(h), None))
+    # ArrowSchema (9 words) and ArrowArray (10 words), owned by pyarrow
+    # once imported (it calls their release callbacks)
+    schema = C.create_string_buffer(72)
+    array = C.create_string_buffer(80)
+    try:
+        abi.check(L.mtl_result_export_arrow(h, C.addressof(schema), C.addressof(array)))
+    finally:
+        L.mtl_result_free(h)
+    return pa.RecordBatch._import_from_c(C.addressof(array), C.addressof(schema))
