@@ -254,6 +254,7 @@ def scan_workload(args) -> dict:
     dev = {c: ops.to_device(t[c]) for c in ("c_i", "c_b")}
     del t
     sel = torch.empty(n, dtype=torch.int32, device="cuda")
+    perm = torch.empty(n, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
     out = torch.empty(n, dtype=torch.int64, device="cuda")
     ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
@@ -280,17 +281,30 @@ def scan_workload(args) -> dict:
                                         cnt.data_ptr(), n, abi.PROJ_ADD, 1, out.data_ptr(),
                                         ovf.data_ptr(), None))
                 abi.check(L.mtl_stream_sync(None))
+            def order_by():
+                got = C.c_uint32()
+                abi.check(L.mtl_sort(dev[cname].data_ptr(), ctype, sel.data_ptr(),
+                                     cnt.data_ptr(), n, 0, perm.data_ptr(), C.byref(got), None))
+                abi.check(L.mtl_stream_sync(None))
             ms_sep = timed(separate, args.steps, args.warmup)
             ms = timed(fused, args.steps, args.warmup)
             s_rows = int(cnt.item())
+            # ORDER BY the filtered column over the selection (the config-2
+            # query with its optional ORDER BY): radix sort of S (key, row)
+            # pairs; algorithmic bytes = S x (key width + row id in + out)
+            ms_sort = timed(order_by, max(3, args.steps // 3), 1)
+            sort_alg = s_rows * (width + 8)
             alg = n * width + s_rows * (4 + width)
             rows.append({"column": cname, "selectivity": frac, "selected": s_rows, "ms": ms,
                          "ms_filter_then_project": ms_sep,
+                         "order_by": {"ms": ms_sort,
+                                      "rows_per_s": s_rows / (ms_sort * 1e-3),
+                                      "algorithmic_GBs": sort_alg / (ms_sort * 1e-3) / 1e9},
                          "rows_per_s": n / (ms * 1e-3), "algorithmic_bytes": alg,
                          "GBs": alg / (ms * 1e-3) / 1e9,
                          "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
             log(f"scan {cname} {frac:.0%}: fused {ms:.3f} ms ({rows[-1]['GBs']:.0f} GB/s), "
-                f"filter then project {ms_sep:.3f} ms")
+                f"filter then project {ms_sep:.3f} ms, ORDER BY {ms_sort:.3f} ms")
     return {"metric": "rows/s & % HBM roofline (scan-filter-project)",
             "value": min(r["rows_per_s"] for r in rows), "unit": "rows/s",
             "config": {"workload": "scan-filter-project 100M rows, "

@@ -5,12 +5,30 @@
 // gathered through the selection and mapped to order-preserving unsigned
 // integers (signed integers: sign bit flipped; floats: DuckDB's total order
 // with NaN largest and -0 == 0; DESC: complemented), then sorted as (key,
-// row id) pairs by rocPRIM's LSD radix sort, which is stable: equal keys
+// row id) pairs by the LSD radix sort below, which is stable: equal keys
 // keep the selection's (ascending row) order, so the result is exactly
 // ORDER BY key [DESC], row.
+//
+// The radix sort (8-bit digits, 4 passes for 4-byte keys, 8 for 8-byte):
+//   k_rs_digits  : one read of the keys counts every digit position's 256
+//                  values; a pass whose digit is the same for every key
+//                  changes nothing and is skipped (dates, narrow integers
+//                  and small tables skip most of their passes);
+//   per pass:
+//   k_rs_upsweep : each block counts the digit over its fixed range of
+//                  tiles (a digit-major table of per-block counts);
+//   k_rs_scan    : one exclusive scan of that table: the output offset of
+//                  each (digit, block);
+//   k_rs_scatter : each block walks its tiles in order; a wave ranks its 64
+//                  keys per step stably (the lanes with the same digit from
+//                  eight ballots, plus a per-wave LDS count per digit), the
+//                  tile is staged in LDS ordered by digit, and each digit's
+//                  run is written out with consecutive threads on
+//                  consecutive addresses.
 #include <hip/hip_runtime.h>
 
-#include <rocprim/device/device_radix_sort.hpp>
+#include <cstring>
+#include <vector>
 
 #include "device.hpp"
 #include "mtl_internal.hpp"
@@ -37,6 +55,165 @@ __global__ void k_sort_keys(const void* col, int32_t type, const uint32_t* sel,
   }
 }
 
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / WAVE;
+constexpr int RS_STEPS = 16;                          // 64-key steps per wave and tile
+constexpr int RS_TILE = RS_THREADS * RS_STEPS;       // 4096 keys
+constexpr int RS_BINS = 256;
+constexpr uint32_t RS_MAX_BLOCKS = 2048;
+
+// Counts of every digit position: hist[p * 256 + digit p of key].
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_digits(const K* __restrict__ keys, uint64_t n,
+                                                          uint32_t* hist) {
+  constexpr int P = (int)sizeof(K);
+  __shared__ uint32_t h[P * RS_BINS];
+  for (int i = threadIdx.x; i < P * RS_BINS; i += RS_THREADS) h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * RS_THREADS + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * RS_THREADS) {
+    const K k = keys[i];
+#pragma unroll
+    for (int p = 0; p < P; ++p) atomicAdd(&h[p * RS_BINS + (uint32_t)((k >> (8 * p)) & 0xFF)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < P * RS_BINS; i += RS_THREADS)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Block b's digit counts over keys [b * per, min(n, (b + 1) * per)), stored
+// digit-major: cnt[digit * nb + b].
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_upsweep(const K* __restrict__ keys, uint64_t n,
+                                                           uint32_t shift, uint64_t per,
+                                                           uint32_t* cnt) {
+  __shared__ uint32_t h[RS_WAVES][RS_BINS];
+  const int wave = threadIdx.x / WAVE;
+  for (int i = threadIdx.x; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += RS_THREADS)
+    atomicAdd(&h[wave][(uint32_t)((keys[i] >> shift) & 0xFF)], 1u);
+  __syncthreads();
+  const uint32_t d = threadIdx.x;  // RS_THREADS == RS_BINS: one digit per thread
+  uint32_t t = 0;
+#pragma unroll
+  for (int w = 0; w < RS_WAVES; ++w) t += h[w][d];
+  cnt[(uint64_t)d * gridDim.x + blockIdx.x] = t;
+}
+
+// Exclusive scan in place of m counts (one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* cnt, uint32_t m) {
+  __shared__ uint32_t s_wave[1024 / WAVE];
+  const uint32_t per = (m + 1023) / 1024;
+  const uint32_t a = threadIdx.x * per, e = a + per < m ? a + per : m;
+  uint32_t sum = 0;
+  for (uint32_t i = a; i < e; ++i) sum += cnt[i];
+  const uint32_t incl = wave_inclusive_u32(sum);
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  if (lane == WAVE - 1) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t run = incl - sum;
+  for (int w = 0; w < wave; ++w) run += s_wave[w];
+  for (uint32_t i = a; i < e; ++i) {
+    const uint32_t c = cnt[i];
+    cnt[i] = run;
+    run += c;
+  }
+}
+
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           K* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout, uint64_t n,
+                                                           uint32_t shift, uint64_t per,
+                                                           const uint32_t* __restrict__ off) {
+  __shared__ K s_key[RS_TILE];
+  __shared__ uint32_t s_val[RS_TILE];
+  __shared__ uint32_t s_wcnt[RS_WAVES][RS_BINS];  // per wave: keys of each digit so far
+  __shared__ uint32_t s_run[RS_BINS];             // the block's next output offset per digit
+  __shared__ uint32_t s_start[RS_BINS];           // digit's first staged position in the tile
+  __shared__ uint32_t s_scan[RS_WAVES];
+  const int tid = threadIdx.x, wave = tid / WAVE, lane = tid % WAVE;
+  const uint64_t lt = lanemask_lt();
+  s_run[tid] = off[(uint64_t)tid * gridDim.x + blockIdx.x];
+  const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
+  for (uint64_t t0 = b0; t0 < b1; t0 += RS_TILE) {
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][tid] = 0;
+    __syncthreads();
+    // this wave's 16 steps of 64 consecutive keys
+    K key[RS_STEPS];
+    uint32_t val[RS_STEPS], rank[RS_STEPS];
+    const uint64_t w0 = t0 + (uint64_t)wave * (RS_STEPS * WAVE);
+#pragma unroll
+    for (int s = 0; s < RS_STEPS; ++s) {
+      const uint64_t i = w0 + (uint64_t)s * WAVE + lane;
+      key[s] = i < b1 ? kin[i] : (K)0;
+      val[s] = i < b1 ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (int s = 0; s < RS_STEPS; ++s) {
+      const uint64_t i = w0 + (uint64_t)s * WAVE + lane;
+      const bool live = i < b1;
+      const uint32_t d = (uint32_t)((key[s] >> shift) & 0xFF);
+      // lanes (live) whose digit equals this lane's, from one ballot per bit
+      uint64_t same = __ballot(live);
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        const uint64_t bb = __ballot((d >> bit) & 1u);
+        same &= ((d >> bit) & 1u) ? bb : ~bb;
+      }
+      const uint32_t before = s_wcnt[wave][d];  // every lane reads before the update
+      rank[s] = before + (uint32_t)__popcll(same & lt);
+      // the highest lane of each digit group adds the group's size
+      const bool last = live && (same >> lane) == 1ull;
+      if (last) s_wcnt[wave][d] = before + (uint32_t)__popcll(same);
+    }
+    __syncthreads();
+    // per digit (one per thread): offsets of the waves, the tile's total and
+    // the digit's first position in the staged tile
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+      const uint32_t c = s_wcnt[w][tid];
+      s_wcnt[w][tid] = tot;
+      tot += c;
+    }
+    const uint32_t incl = wave_inclusive_u32(tot);
+    if (lane == WAVE - 1) s_scan[wave] = incl;
+    __syncthreads();
+    uint32_t start = incl - tot;
+    for (int w = 0; w < wave; ++w) start += s_scan[w];
+    s_start[tid] = start;
+    __syncthreads();
+    // stage the tile ordered by digit
+#pragma unroll
+    for (int s = 0; s < RS_STEPS; ++s) {
+      const uint64_t i = w0 + (uint64_t)s * WAVE + lane;
+      if (i < b1) {
+        const uint32_t d = (uint32_t)((key[s] >> shift) & 0xFF);
+        const uint32_t p = s_start[d] + s_wcnt[wave][d] + rank[s];
+        s_key[p] = key[s];
+        s_val[p] = val[s];
+      }
+    }
+    __syncthreads();
+    // write each digit's run at the block's offset for that digit
+    const uint32_t live_n = (uint32_t)(b1 - t0 < (uint64_t)RS_TILE ? b1 - t0 : RS_TILE);
+    for (uint32_t p = tid; p < live_n; p += RS_THREADS) {
+      const K k = s_key[p];
+      const uint32_t d = (uint32_t)((k >> shift) & 0xFF);
+      const uint64_t o = (uint64_t)s_run[d] + (p - s_start[d]);
+      kout[o] = k;
+      vout[o] = s_val[p];
+    }
+    __syncthreads();
+    s_run[tid] += tot;
+  }
+}
+
 static bool narrow_sort_type(int32_t t) {
   return t == MTL_INTEGER || t == MTL_DATE || t == MTL_DICT32 || t == MTL_DICT8;
 }
@@ -45,29 +222,79 @@ template <typename K>
 static int32_t sort_pairs(const void* col, int32_t type, const uint32_t* sel,
                           const uint32_t* count, uint64_t n, int32_t desc, uint32_t* out_rows,
                           mtl_stream* s) {
+  constexpr int P = (int)sizeof(K);
   hipStream_t st = s->stream;
   Arena& ar = s->arena;
-  void *kin, *kout, *rin;
-  MTL_TRY(ar.alloc(n * sizeof(K), &kin));
-  MTL_TRY(ar.alloc(n * sizeof(K), &kout));
-  MTL_TRY(ar.alloc(n * 4, &rin));
+  void *ka, *kb, *r0, *r1, *hist;
+  const uint64_t tiles = ceil_div(n, RS_TILE);
+  const uint32_t nb = (uint32_t)std::min<uint64_t>(tiles, RS_MAX_BLOCKS);
+  const uint64_t per = ceil_div(tiles, nb) * RS_TILE;
+  MTL_TRY(ar.alloc(n * sizeof(K), &ka));
+  MTL_TRY(ar.alloc(n * sizeof(K), &kb));
+  MTL_TRY(ar.alloc(n * 4, &r0));
+  MTL_TRY(ar.alloc(n * 4, &r1));
+  MTL_TRY(ar.alloc((uint64_t)RS_BINS * std::max<uint64_t>(nb, P) * 4, &hist));
   const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 8), 4096));
   {
     ProfScope ps(s, "k_sort_keys");
     hipLaunchKernelGGL(k_sort_keys<K>, dim3(grid), dim3(256), 0, st, col, type, sel, count, n,
-                       desc, (K*)kin, (uint32_t*)rin);
+                       desc, (K*)ka, (uint32_t*)r0);
   }
   MTL_HIP(hipGetLastError());
-  size_t temp_bytes = 0;
-  MTL_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, (K*)kin, (K*)kout, (uint32_t*)rin,
-                                    out_rows, (unsigned int)n, 0, (unsigned int)(8 * sizeof(K)),
-                                    st));
-  void* temp;
-  MTL_TRY(ar.alloc(temp_bytes ? temp_bytes : 1, &temp));
-  ProfScope ps(s, "radix_sort_pairs");
-  MTL_HIP(rocprim::radix_sort_pairs(temp, temp_bytes, (K*)kin, (K*)kout, (uint32_t*)rin,
-                                    out_rows, (unsigned int)n, 0, (unsigned int)(8 * sizeof(K)),
-                                    st));
+  // which digit positions vary
+  MTL_HIP(hipMemsetAsync(hist, 0, (size_t)P * RS_BINS * 4, st));
+  {
+    ProfScope ps(s, "k_rs_digits");
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256 * 16), 1024));
+    hipLaunchKernelGGL(k_rs_digits<K>, dim3(g), dim3(RS_THREADS), 0, st, (const K*)ka, n,
+                       (uint32_t*)hist);
+  }
+  MTL_HIP(hipGetLastError());
+  std::vector<uint32_t> hh((size_t)P * RS_BINS);
+  {
+    void* host;
+    MTL_TRY(stream_pinned(s, hh.size() * 4, &host));
+    MTL_HIP(hipMemcpyAsync(host, hist, hh.size() * 4, hipMemcpyDeviceToHost, st));
+    MTL_HIP(stream_wait(st));
+    std::memcpy(hh.data(), host, hh.size() * 4);
+  }
+  std::vector<int> passes;
+  for (int p = 0; p < P; ++p) {
+    int nz = 0;
+    for (int d = 0; d < RS_BINS; ++d) nz += hh[(size_t)p * RS_BINS + d] != 0;
+    if (nz > 1) passes.push_back(p);
+  }
+  const int np = (int)passes.size();
+  if (np == 0) {  // every key equal: the selection order is the result
+    MTL_HIP(hipMemcpyAsync(out_rows, r0, n * 4, hipMemcpyDeviceToDevice, st));
+    return MTL_OK;
+  }
+  // the row ids of pass j go to out_rows when (np - 1 - j) is even, so the
+  // last pass writes out_rows
+  const K* kin = (const K*)ka;
+  const uint32_t* vin = (const uint32_t*)r0;
+  for (int j = 0; j < np; ++j) {
+    K* kout = (K*)(kin == (const K*)ka ? kb : ka);
+    uint32_t* vout = ((np - 1 - j) % 2 == 0) ? out_rows : (uint32_t*)r1;
+    const uint32_t shift = (uint32_t)(8 * passes[j]);
+    {
+      ProfScope ps(s, "k_rs_upsweep");
+      hipLaunchKernelGGL(k_rs_upsweep<K>, dim3(nb), dim3(RS_THREADS), 0, st, kin, n, shift, per,
+                         (uint32_t*)hist);
+    }
+    MTL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, (uint32_t*)hist,
+                       (uint32_t)(RS_BINS * nb));
+    MTL_HIP(hipGetLastError());
+    {
+      ProfScope ps(s, "k_rs_scatter");
+      hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, st, kin, vin, kout, vout,
+                         n, shift, per, (const uint32_t*)hist);
+    }
+    MTL_HIP(hipGetLastError());
+    kin = kout;
+    vin = vout;
+  }
   return MTL_OK;
 }
 

@@ -524,6 +524,45 @@ def test_sort_parity(ops, t, dtype, desc):
         np.testing.assert_array_equal(ops.to_host(got, np.uint32), exp)
 
 
+@pytest.mark.parametrize("case", ["constant", "high_digits", "narrow", "ties", "multi_tile",
+                                  "one_row", "tail"])
+def test_sort_radix_pass_shapes(ops, case):
+    """The radix sort skips digit positions that are constant over the keys
+    and runs the others: every key equal (no pass), keys differing only in
+    their high bytes, a narrow BIGINT range (one or two passes of eight),
+    many ties; 9M rows (more than one 4096-row tile per block of the 2048
+    block ranges); one row; a partial last tile.  Each against numpy's
+    stable argsort."""
+    rng = np.random.default_rng(["constant", "high_digits", "narrow", "ties", "multi_tile",
+                                 "one_row", "tail"].index(case) + 400)
+    t, dtype, n = abi.BIGINT, np.int64, 300_001
+    if case == "constant":
+        col = np.full(n, -17, dtype=np.int64)
+    elif case == "high_digits":
+        col = (rng.integers(0, 256, n).astype(np.int64) << 56) - (1 << 62)
+    elif case == "narrow":
+        col = rng.integers(1_000_000, 1_000_300, n).astype(np.int64)
+    elif case == "ties":
+        col = rng.integers(0, 3, n).astype(np.int64) * (1 << 40)
+    elif case == "multi_tile":
+        t, dtype, n = abi.INTEGER, np.int32, 9_000_011
+        col = rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    elif case == "one_row":
+        col, n = np.array([5], dtype=np.int64), 1
+    else:
+        t, dtype, n = abi.DATE, np.int32, 4096 * 3 + 17
+        col = rng.integers(10957, 22281, n).astype(np.int32)
+    dcol = ops.to_device(col)
+    for desc in (False, True):
+        k = _order_key(col, t)
+        if desc:
+            k = ~k
+        exp = np.argsort(k, kind="stable").astype(np.uint32)
+        got, m = ops.sort(dcol, t, n, desc=desc)
+        assert m == n
+        np.testing.assert_array_equal(ops.to_host(got, np.uint32), exp)
+
+
 def test_sort_empty_selection(ops):
     col = ops.to_device(np.arange(10, dtype=np.int32))
     sel = ops.to_device(np.zeros(4, dtype=np.int32))
