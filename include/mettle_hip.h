@@ -249,6 +249,16 @@ int32_t mtl_gather(const void *column, uint32_t width, const uint32_t *sel,
                    const uint32_t *count, uint64_t capacity, void *out,
                    mtl_stream *s);
 
+/* mtl_filter over rows 0..n_rows-1 for a conjunction of disjunctions:
+ * predicates p and p+1 are OR-ed when bit p of or_next is set, and the runs
+ * of OR-ed predicates are AND-ed, so `a AND (b OR c) AND d IN (1, 2)` is
+ * preds {a, b, c, d=1, d=2} with or_next = 0b0010 | 0b1000.  No
+ * counterpart in the reference, whose planner rejects OR and IN
+ * (parser.rs:277-367); DuckDB ran those on the CPU. */
+int32_t mtl_filter_or(const mtl_pred *preds, uint32_t n_preds, uint32_t or_next,
+                      uint64_t n_rows, uint32_t *out_sel, uint32_t *out_count,
+                      mtl_stream *s);
+
 /* Scalar projection out[i] = column[sel[i]] op scalar (sel NULL -> i).
  * INTEGER/BIGINT/DATE(+/-)/TIMESTAMP(+/-)/DECIMAL: exact, sets
  * *overflow_flag (device uint32) to 1 on overflow.  FLOAT: IEEE f32.

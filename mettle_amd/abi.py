@@ -234,6 +234,7 @@ def lib() -> C.CDLL:
         "mtl_memcpy_d2h": (i32, [vp, vp, u64, vp]),
         "mtl_filter": (i32, [C.POINTER(Pred), u32, u64, vp, vp, u64, vp, vp, vp]),
         "mtl_filter_noalloc": (i32, [C.POINTER(Pred), u32, u64, vp, vp, u64, vp, vp, vp]),
+        "mtl_filter_or": (i32, [C.POINTER(Pred), u32, u32, u64, vp, vp, vp]),
         "mtl_compare": (i32, [C.POINTER(Pred), u32, u64, vp, vp]),
         "mtl_gather": (i32, [vp, u32, vp, vp, u64, vp, vp]),
         "mtl_project": (i32, [vp, i32, vp, vp, u64, i32, i64, vp, vp, vp]),

@@ -792,7 +792,8 @@ __global__ __launch_bounds__(FBLOCK) void k_expand(const uint64_t* __restrict__ 
 struct DenseArgs {
   PredC preds[MTL_MAX_PREDS];
   uint32_t n_preds;
-  uint32_t vec_ok;  // bit p: column p is aligned for vector loads
+  uint32_t vec_ok;   // bit p: column p is aligned for vector loads
+  uint32_t or_next;  // bit p: predicate p is OR-ed with predicate p + 1
   uint64_t n_rows;
   uint64_t* sel_bm;
   uint32_t* counts;
@@ -845,9 +846,26 @@ __global__ __launch_bounds__(FBLOCK) void k_filter_dense(DenseArgs a) {
   // streaming loads (block-uniform choice)
   const bool full = (uint64_t)(tile + 1) * FTILE <= n &&
                     a.vec_ok == (a.n_preds >= 32 ? ~0u : (1u << a.n_preds) - 1u);
-  for (uint32_t p = 0; p < a.n_preds; ++p) {
-    if (full) pred_dense_typed<true>(a.preds[p], k0, n, true, bits);
-    else pred_dense_typed<false>(a.preds[p], k0, n, (a.vec_ok >> p) & 1u, bits);
+  if (!a.or_next) {
+    for (uint32_t p = 0; p < a.n_preds; ++p) {
+      if (full) pred_dense_typed<true>(a.preds[p], k0, n, true, bits);
+      else pred_dense_typed<false>(a.preds[p], k0, n, (a.vec_ok >> p) & 1u, bits);
+    }
+  } else {
+    // a conjunction of disjunctions: each run of OR-ed predicates collects
+    // its rows in `any`, and-ed into bits at the run's last predicate
+    const uint32_t live = bits;
+    uint32_t any = 0;
+    for (uint32_t p = 0; p < a.n_preds; ++p) {
+      uint32_t m = live;
+      if (full) pred_dense_typed<true>(a.preds[p], k0, n, true, m);
+      else pred_dense_typed<false>(a.preds[p], k0, n, (a.vec_ok >> p) & 1u, m);
+      any |= m;
+      if (!((a.or_next >> p) & 1u)) {
+        bits &= any;
+        any = 0;
+      }
+    }
   }
   store_pass_bits(a.sel_bm, seg, bits, lane);
   const uint32_t c = wave_sum_u32((uint32_t)__popc(bits));
@@ -1124,7 +1142,10 @@ static int32_t filter_enqueue(const mtl_pred* preds, uint32_t n_preds, uint64_t 
                               const uint32_t* in_sel, const uint32_t* in_count,
                               uint32_t* out_sel, uint32_t* out_count, const CompactProj* proj,
                               ScanExtras* ex, const FilterScratch& sc, uint64_t* fpg,
-                              uint32_t tiles, uint32_t groups, bool prof, mtl_stream* s) {
+                              uint32_t tiles, uint32_t groups, bool prof, mtl_stream* s,
+                              uint32_t or_next = 0) {
+  if (or_next && in_sel)
+    return set_error(MTL_UNSUPPORTED, "OR-ed predicates over an input selection");
   ZeroList zl = ex ? ex->zero : ZeroList{};
   if (zl.n > 5) return set_error(MTL_ERR_INVALID_ARGUMENT, "too many regions to clear");
   zl.add((uint32_t*)sc.ctr, 64);
@@ -1135,6 +1156,7 @@ static int32_t filter_enqueue(const mtl_pred* preds, uint32_t n_preds, uint64_t 
     DenseArgs a{};
     for (uint32_t p = 0; p < n_preds; ++p) a.preds[p] = compile_pred(preds[p]);
     a.n_preds = n_preds;
+    a.or_next = n_preds ? or_next & ((1u << (n_preds - 1)) - 1u) : 0u;  // last ends a run
     a.n_rows = cap;
     a.sel_bm = (uint64_t*)sc.bm;
     a.counts = (uint32_t*)sc.counts;
@@ -1163,7 +1185,8 @@ static int32_t filter_enqueue(const mtl_pred* preds, uint32_t n_preds, uint64_t 
 static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
                            const uint32_t* in_sel, const uint32_t* in_count,
                            uint64_t in_capacity, uint32_t* out_sel, uint32_t* out_count,
-                           const CompactProj* proj, ScanExtras* ex, mtl_stream* s) {
+                           const CompactProj* proj, ScanExtras* ex, mtl_stream* s,
+                           uint32_t or_next = 0) {
   uint64_t cap;
   MTL_TRY(filter_check(preds, n_preds, n_rows, in_sel, in_capacity, out_sel, out_count, &cap));
   MTL_TRY(resolve_stream(s, &s));
@@ -1185,7 +1208,7 @@ static int32_t filter_impl(const mtl_pred* preds, uint32_t n_preds, uint64_t n_r
     ex->fp_n = groups;
   }
   return filter_enqueue(preds, n_preds, cap, in_sel, in_count, out_sel, out_count, proj, ex,
-                        sc, fpg, tiles, groups, true, s);
+                        sc, fpg, tiles, groups, true, s, or_next);
 }
 
 }  // extern "C"
@@ -1271,6 +1294,12 @@ int32_t mtl_filter(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,
                    mtl_stream* s) {
   return filter_impl(preds, n_preds, n_rows, in_sel, in_count, in_capacity, out_sel,
                      out_count, nullptr, nullptr, s);
+}
+
+int32_t mtl_filter_or(const mtl_pred* preds, uint32_t n_preds, uint32_t or_next,
+                      uint64_t n_rows, uint32_t* out_sel, uint32_t* out_count, mtl_stream* s) {
+  return filter_impl(preds, n_preds, n_rows, nullptr, nullptr, 0, out_sel, out_count, nullptr,
+                     nullptr, s, or_next);
 }
 
 int32_t mtl_filter_project(const mtl_pred* preds, uint32_t n_preds, uint64_t n_rows,

@@ -24,6 +24,7 @@
 // results carry DuckDB's types (exact integer sums, no float narrowing).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -35,6 +36,7 @@
 
 #include "filter.hpp"
 #include "mtl_internal.hpp"
+#include "star.hpp"
 
 // ---------------------------------------------------------------------------
 // Results (host-side materialised columns)
@@ -169,6 +171,7 @@ struct Pred {
   ColRef col;
   int op;  // MTL_CMP_*
   Lit lit;
+  bool or_next = false;  // OR-ed with the next predicate (IN lists, OR runs)
 };
 
 struct JoinEq {
@@ -228,7 +231,7 @@ struct Parser {
     static const char* kw[] = {"select", "from", "where", "group", "by", "order", "limit",
                                "and", "or", "not", "as", "join", "inner", "on", "asc", "desc",
                                "between", "having", "distinct", "union", "left", "right",
-                               "full", "outer", "cross", "with", "offset", nullptr};
+                               "full", "outer", "cross", "with", "offset", "in", nullptr};
     for (int i = 0; kw[i]; ++i)
       if (s == kw[i]) return true;
     return false;
@@ -423,6 +426,24 @@ struct Parser {
     }
     ColRef c;
     if (!colref(c)) return false;
+    if (is_kw("not")) return fail("NOT is not supported");
+    if (accept_kw("in")) {
+      // col IN (v1, ..., vk): k equalities OR-ed together
+      if (!expect_op("(")) return false;
+      size_t k = 0;
+      do {
+        Pred pr;
+        pr.col = c;
+        pr.op = MTL_CMP_EQ;
+        if (!literal(pr.lit)) return false;
+        pr.or_next = true;
+        q.preds.push_back(pr);
+        ++k;
+      } while (accept_op(","));
+      if (!expect_op(")")) return false;
+      q.preds.back().or_next = false;
+      return true;
+    }
     if (accept_kw("between")) {
       Pred lo, hi;
       lo.col = hi.col = c;
@@ -451,11 +472,51 @@ struct Parser {
     return true;
   }
 
-  bool conjunction(Query& q) {
+  // The predicates a condition added form one disjunction: a single
+  // predicate, an IN list or a parenthesised OR of such (every predicate but
+  // the last OR-ed with the next).
+  static bool one_disjunction(const Query& q, size_t first) {
+    if (q.preds.size() <= first) return false;
+    for (size_t i = first; i + 1 < q.preds.size(); ++i)
+      if (!q.preds[i].or_next) return false;
+    return true;
+  }
+
+  // A term of the WHERE conjunction: a condition, or conditions joined by
+  // OR when each is a single comparison or an IN list (OR over a conjunction
+  // or over a join condition is not admitted).
+  bool term(Query& q, bool& used_or) {
+    used_or = false;
+    const size_t first = q.preds.size(), joins = q.joins.size();
     if (!condition(q)) return false;
-    while (accept_kw("and"))
+    if (!is_kw("or")) return true;
+    if (q.joins.size() != joins || !one_disjunction(q, first))
+      return fail("OR over a conjunction or a join condition");
+    used_or = true;
+    while (accept_kw("or")) {
+      q.preds.back().or_next = true;
+      const size_t at = q.preds.size();
       if (!condition(q)) return false;
-    if (is_kw("or")) return fail("OR is not supported");
+      if (q.joins.size() != joins || !one_disjunction(q, at))
+        return fail("OR over a conjunction or a join condition");
+    }
+    return true;
+  }
+
+  // AND binds tighter than OR: an OR outside parentheses is admitted only
+  // when it is the whole conjunction (`a OR b`), not beside an AND.
+  bool conjunction(Query& q) {
+    bool used_or = false, any_or = false;
+    size_t terms = 1;
+    if (!term(q, used_or)) return false;
+    any_or |= used_or;
+    while (accept_kw("and")) {
+      if (!term(q, used_or)) return false;
+      any_or |= used_or;
+      ++terms;
+    }
+    if (is_kw("or") || (any_or && terms > 1))
+      return fail("OR beside AND needs parentheses");
     return true;
   }
 
@@ -718,6 +779,43 @@ static bool literal_bits(const Lit& l, const CatCol& c, int op, int64_t& bits, s
 // ---------------------------------------------------------------------------
 // Execution helpers
 // ---------------------------------------------------------------------------
+// The WHERE predicates bound per source: preds[src] with or_next[src] (bit i:
+// predicate i is OR-ed with predicate i + 1).  A run of OR-ed predicates must
+// stay within one table.
+static int32_t bind_preds(const Query& q, Bound& b, std::vector<std::vector<mtl_pred>>& preds,
+                          std::vector<uint32_t>& or_next) {
+  preds.assign(b.tabs.size(), {});
+  or_next.assign(b.tabs.size(), 0u);
+  int run_src = -1;
+  for (auto& pr : q.preds) {
+    int src;
+    const CatCol* c;
+    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
+    if (run_src >= 0 && src != run_src)
+      return set_error(MTL_UNSUPPORTED, "OR across tables");
+    mtl_pred mp{};
+    mp.column = c->dev;
+    mp.type = c->type;
+    mp.op = pr.op;
+    std::string err;
+    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
+      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+    if (preds[src].size() >= MTL_MAX_PREDS)
+      return set_error(MTL_UNSUPPORTED, "more than %d predicates on a table", MTL_MAX_PREDS);
+    if (pr.or_next) or_next[src] |= 1u << preds[src].size();
+    preds[src].push_back(mp);
+    run_src = pr.or_next ? src : -1;
+  }
+  return MTL_OK;
+}
+
+static bool any_or(const Query& q) {
+  for (auto& pr : q.preds)
+    if (pr.or_next) return true;
+  return false;
+}
+
+
 // Device buffer charged to the processing budget (MTL_ERR_BUDGET when it
 // would exceed it).
 struct DevBuf {
@@ -779,21 +877,10 @@ static std::string item_name(const Item& it, size_t i) {
 // ---------------------------------------------------------------------------
 static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
   const CatTable& T = *b.tabs[0];
-  std::vector<mtl_pred> preds;
-  for (auto& pr : q.preds) {
-    int src;
-    const CatCol* c;
-    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
-    mtl_pred mp{};
-    mp.column = c->dev;
-    mp.type = c->type;
-    mp.op = pr.op;
-    std::string err;
-    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
-      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
-    preds.push_back(mp);
-  }
-  if (preds.size() > MTL_MAX_PREDS) return set_error(MTL_UNSUPPORTED, "too many predicates");
+  std::vector<std::vector<mtl_pred>> tp;
+  std::vector<uint32_t> tor;
+  MTL_TRY(bind_preds(q, b, tp, tor));
+  const std::vector<mtl_pred>& preds = tp[0];
   // outputs
   struct Out {
     std::string name;
@@ -861,8 +948,8 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
   DevBuf sel, cnt, perm, out, ovf;
   MTL_TRY(sel.alloc(std::max<uint64_t>(T.rows, 1) * 4));
   MTL_TRY(cnt.alloc(16));
-  MTL_TRY(mtl_filter(preds.data(), (uint32_t)preds.size(), T.rows, nullptr, nullptr, 0,
-                     (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
+  MTL_TRY(mtl_filter_or(preds.data(), (uint32_t)preds.size(), tor[0], T.rows,
+                        (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
   uint32_t n = 0;
   const uint32_t* rows = (const uint32_t*)sel.p;
   if (okey) {
@@ -921,7 +1008,106 @@ struct GroupEmit {
   int order_agg;
   bool odesc;
   int key_order;
+  // ORDER BY the aggregate, then these group keys (index, descending)
+  std::vector<std::pair<int, bool>> then = {};
 };
+
+// Three-way comparison of aggregate a between groups i and j of host
+// buffers in mtl_groups_fetch layout.
+static int agg_cmp(const GroupEmit& ge, const std::vector<std::vector<uint8_t>>& abuf, size_t a,
+                   uint64_t w, uint64_t i, uint64_t j) {
+  const uint8_t* x = abuf[a].data() + i * w;
+  const uint8_t* y = abuf[a].data() + j * w;
+  const int32_t f = ge.agg_func[a], t = ge.agg_type[a];
+  if (w == 16) {  // int128 {uint64 lower; int64 upper}
+    int64_t xh, yh;
+    uint64_t xl, yl;
+    std::memcpy(&xl, x, 8);
+    std::memcpy(&xh, x + 8, 8);
+    std::memcpy(&yl, y, 8);
+    std::memcpy(&yh, y + 8, 8);
+    if (xh != yh) return xh < yh ? -1 : 1;
+    return xl == yl ? 0 : (xl < yl ? -1 : 1);
+  }
+  if (f != MTL_AGG_COUNT && (f == MTL_AGG_AVG || t == MTL_FLOAT || t == MTL_DOUBLE)) {
+    double u, v;
+    std::memcpy(&u, x, 8);
+    std::memcpy(&v, y, 8);
+    return u == v ? 0 : (u < v ? -1 : 1);
+  }
+  int64_t u, v;
+  std::memcpy(&u, x, 8);
+  std::memcpy(&v, y, 8);
+  return u == v ? 0 : (u < v ? -1 : 1);
+}
+
+// ORDER BY an aggregate, then group keys, LIMIT k: the device top-K by the
+// aggregate takes a wider candidate set (every group tied with the k-th
+// lies in it once the last candidate's value differs from the k-th; failing
+// that, every group is fetched), which is then ordered by the full ORDER BY
+// on the host and cut to k.
+template <typename W>
+static int32_t topk_then_keys(mtl_groups* g, const mtl_group_desc& d, const GroupEmit& ge,
+                              uint32_t k, std::vector<int64_t>& kbuf,
+                              std::vector<std::vector<uint8_t>>& abuf, std::vector<void*>& aptr,
+                              W agg_width, uint32_t& n_out) {
+  const uint32_t a0 = (uint32_t)ge.order_agg;
+  const uint64_t w0 = agg_width(a0);
+  auto resize = [&](uint64_t cap) {
+    kbuf.resize(std::max<uint64_t>(cap, 1) * d.n_keys);
+    for (uint32_t a = 0; a < d.n_aggs; ++a) {
+      abuf[a].resize(std::max<uint64_t>(cap, 1) * agg_width(a));
+      aptr[a] = abuf[a].data();
+    }
+  };
+  const uint32_t kc = std::min<uint32_t>(512, std::max<uint32_t>(2 * k, k + 32));
+  uint64_t n = 0;
+  bool whole = k > 512;
+  if (!whole) {
+    resize(kc);
+    uint32_t got = 0;
+    MTL_TRY(mtl_groups_topk(g, a0, ge.odesc ? 1 : 0, kc, kbuf.data(), aptr.data(), &got));
+    n = got;
+    whole = got == kc && got > k && agg_cmp(ge, abuf, a0, w0, k - 1, got - 1) == 0;
+  }
+  if (whole) {
+    uint64_t ng = 0;
+    MTL_TRY(mtl_groups_count(g, &ng));
+    resize(ng);
+    MTL_TRY(mtl_groups_fetch(g, kbuf.data(), aptr.data(), ng, &n));
+  }
+  std::vector<uint64_t> perm(n);
+  for (uint64_t i = 0; i < n; ++i) perm[i] = i;
+  auto less = [&](uint64_t i, uint64_t j) {
+    int c = agg_cmp(ge, abuf, a0, w0, i, j);
+    if (c) return ge.odesc ? c > 0 : c < 0;
+    for (auto& t : ge.then) {
+      const int64_t u = kbuf[i * d.n_keys + t.first], v = kbuf[j * d.n_keys + t.first];
+      if (u != v) return t.second ? u > v : u < v;
+    }
+    for (uint32_t q = 0; q < d.n_keys; ++q) {
+      const int64_t u = kbuf[i * d.n_keys + q], v = kbuf[j * d.n_keys + q];
+      if (u != v) return u < v;
+    }
+    return false;
+  };
+  const uint64_t keep = std::min<uint64_t>(k, n);
+  std::partial_sort(perm.begin(), perm.begin() + keep, perm.end(), less);
+  std::vector<int64_t> kb(std::max<uint64_t>(keep, 1) * d.n_keys);
+  for (uint64_t r = 0; r < keep; ++r)
+    for (uint32_t q = 0; q < d.n_keys; ++q) kb[r * d.n_keys + q] = kbuf[perm[r] * d.n_keys + q];
+  kbuf.swap(kb);
+  for (uint32_t a = 0; a < d.n_aggs; ++a) {
+    const uint64_t w = agg_width(a);
+    std::vector<uint8_t> ab(std::max<uint64_t>(keep, 1) * w);
+    for (uint64_t r = 0; r < keep; ++r)
+      std::memcpy(ab.data() + r * w, abuf[a].data() + perm[r] * w, w);
+    abuf[a].swap(ab);
+    aptr[a] = abuf[a].data();
+  }
+  n_out = (uint32_t)keep;
+  return MTL_OK;
+}
 
 // Runs the group-by over d and materialises the result columns (DuckDB's
 // result types: COUNT BIGINT, integer SUM HUGEINT, AVG and float
@@ -951,8 +1137,12 @@ static int32_t emit_groups(const Query& q, mtl_group_desc& d, const GroupEmit& g
       aptr[a] = abuf[a].data();
     }
     uint32_t n = 0;
-    if (k > 0) MTL_TRY(mtl_groups_topk(g, (uint32_t)ge.order_agg, ge.odesc ? 1 : 0, k,
-                                       kbuf.data(), aptr.data(), &n));
+    if (k > 0 && ge.then.empty()) {
+      MTL_TRY(mtl_groups_topk(g, (uint32_t)ge.order_agg, ge.odesc ? 1 : 0, k, kbuf.data(),
+                              aptr.data(), &n));
+    } else if (k > 0) {
+      MTL_TRY(topk_then_keys(g, d, ge, k, kbuf, abuf, aptr, agg_width, n));
+    }
     ng = n;
   } else {
     MTL_TRY(mtl_groups_count(g, &ng));
@@ -1014,20 +1204,10 @@ static int32_t emit_groups(const Query& q, mtl_group_desc& d, const GroupEmit& g
 // ---------------------------------------------------------------------------
 static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
   const CatTable& T = *b.tabs[0];
-  std::vector<mtl_pred> preds;
-  for (auto& pr : q.preds) {
-    int src;
-    const CatCol* c;
-    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
-    mtl_pred mp{};
-    mp.column = c->dev;
-    mp.type = c->type;
-    mp.op = pr.op;
-    std::string err;
-    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
-      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
-    preds.push_back(mp);
-  }
+  std::vector<std::vector<mtl_pred>> tp;
+  std::vector<uint32_t> tor;
+  MTL_TRY(bind_preds(q, b, tp, tor));
+  const std::vector<mtl_pred>& preds = tp[0];
   mtl_group_desc d{};
   d.n_rows = T.rows;
   std::vector<const CatCol*> keys;
@@ -1124,8 +1304,8 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
   if (!preds.empty()) {
     MTL_TRY(sel.alloc(std::max<uint64_t>(T.rows, 1) * 4));
     MTL_TRY(cnt.alloc(16));
-    MTL_TRY(mtl_filter(preds.data(), (uint32_t)preds.size(), T.rows, nullptr, nullptr, 0,
-                       (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
+    MTL_TRY(mtl_filter_or(preds.data(), (uint32_t)preds.size(), tor[0], T.rows,
+                          (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
     uint32_t n_sel = 0;
     MTL_TRY(mtl_memcpy_d2h(&n_sel, cnt.p, 4, s));
     if (n_sel > T.rows) return set_error(MTL_ERR_PROCESSING, "device scan failed");
@@ -1140,27 +1320,30 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
 // ---------------------------------------------------------------------------
 // Shape 3: star join -> SUM -> top-K
 // ---------------------------------------------------------------------------
-// Postfix expression over fact columns; DECIMAL operands of + and - must
-// share a scale (constants are rescaled), * adds scales.
+// Postfix expression over the columns of table `fact` (fact < 0: of any
+// table, each column's table recorded in *srcs); DECIMAL operands of + and -
+// must share a scale (constants are rescaled), * adds scales.
 static bool to_postfix(const ExprP& e, Bound& b, int fact, mtl_expr& x, int& scale,
-                       std::vector<const CatCol*>& cols, std::string& err) {
+                       std::vector<const CatCol*>& cols, std::string& err,
+                       std::vector<int>* srcs = nullptr) {
   if (x.n_ops >= MTL_EXPR_MAX_OPS) return err = "expression too long", false;
   switch (e->k) {
     case Expr::COL: {
       int src;
       const CatCol* c;
       if (!b.resolve(e->col, src, c)) return err = b.err, false;
-      if (src != fact) return err = "SUM over a column of another table", false;
+      if (fact >= 0 && src != fact) return err = "SUM over a column of another table", false;
       if (!is_int_type(c->type) || c->type == MTL_DICT8 || c->type == MTL_DICT32 ||
           c->type == MTL_DATE || c->type == MTL_TIMESTAMP)
         return err = "SUM over a non-numeric integer column", false;
       int idx = -1;
       for (size_t i = 0; i < cols.size(); ++i)
-        if (cols[i] == c) idx = (int)i;
+        if (cols[i] == c && (!srcs || (*srcs)[i] == src)) idx = (int)i;
       if (idx < 0) {
         if (cols.size() >= MTL_EXPR_MAX_COLS) return err = "too many columns in SUM", false;
         idx = (int)cols.size();
         cols.push_back(c);
+        if (srcs) srcs->push_back(src);
         x.col[idx] = c->dev;
         x.col_type[idx] = c->type;
         x.n_cols = (uint32_t)cols.size();
@@ -1181,9 +1364,9 @@ static bool to_postfix(const ExprP& e, Bound& b, int fact, mtl_expr& x, int& sca
     case Expr::MUL: {
       const uint32_t start = x.n_ops;
       int sa, sb;
-      if (!to_postfix(e->a, b, fact, x, sa, cols, err)) return false;
+      if (!to_postfix(e->a, b, fact, x, sa, cols, err, srcs)) return false;
       const uint32_t mid = x.n_ops;
-      if (!to_postfix(e->b, b, fact, x, sb, cols, err)) return false;
+      if (!to_postfix(e->b, b, fact, x, sb, cols, err, srcs)) return false;
       if (e->k == Expr::MUL) {
         scale = (sa < 0 ? 0 : sa) + (sb < 0 ? 0 : sb);
       } else {
@@ -1213,6 +1396,7 @@ static bool to_postfix(const ExprP& e, Bound& b, int fact, mtl_expr& x, int& sca
 
 static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
   const int nt = (int)b.tabs.size();
+  if (any_or(q)) return set_error(MTL_UNSUPPORTED, "OR in a star query");
   // join edges -> (child source, fk column) -> (parent source, unique key)
   struct Edge {
     int child, parent;
@@ -1439,6 +1623,39 @@ static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
 // reference overwrites duplicate build keys, kernels.metal:202-206) return
 // MTL_UNSUPPORTED.
 // ---------------------------------------------------------------------------
+// Multi-column join keys packed into one int64: sum over k of (key_k -
+// min_k) * mult_k (mixed radix over each key pair's combined range), for rows
+// sel[i] (i without a selection), i < n.
+struct PackKeys {
+  const void* col[MTL_GROUP_MAX_KEYS];
+  int32_t type[MTL_GROUP_MAX_KEYS];
+  int64_t min[MTL_GROUP_MAX_KEYS];
+  uint64_t mult[MTL_GROUP_MAX_KEYS];
+  uint32_t n_keys;
+};
+
+__global__ void k_pack_keys(PackKeys pk, const uint32_t* sel, uint64_t n, int64_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = sel ? sel[i] : (uint32_t)i;
+    uint64_t v = 0;
+    for (uint32_t k = 0; k < pk.n_keys; ++k)
+      v += (uint64_t)(dev::load_key(pk.col[k], pk.type[k], r) - pk.min[k]) * pk.mult[k];
+    out[i] = (int64_t)v;
+  }
+}
+
+static int32_t pack_keys(const PackKeys& pk, const uint32_t* sel, uint64_t n, DevBuf& out,
+                         mtl_stream* s) {
+  MTL_TRY(out.alloc(std::max<uint64_t>(n, 1) * 8));
+  if (!n) return MTL_OK;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n, 256), 4096);
+  hipLaunchKernelGGL(k_pack_keys, dim3(grid), dim3(256), 0, s->stream, pk, sel, n,
+                     (int64_t*)out.p);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
+}
+
 struct JoinRel {
   std::vector<std::unique_ptr<DevBuf>> ids;  // per source: row ids (nullptr: not joined)
   uint64_t m = 0;                            // rows of the relation
@@ -1502,24 +1719,15 @@ static int32_t gather_ids(const void* src, const void* idx, uint64_t n, DevBuf& 
 static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
   const size_t ns = b.tabs.size();
   if (ns > 4) return set_error(MTL_UNSUPPORTED, "more than four joined tables");
-  std::vector<std::vector<mtl_pred>> preds(ns);
-  for (auto& pr : q.preds) {
-    int src;
-    const CatCol* c;
-    if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
-    mtl_pred mp{};
-    mp.column = c->dev;
-    mp.type = c->type;
-    mp.op = pr.op;
-    std::string err;
-    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
-      return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
-    preds[src].push_back(mp);
-    if (preds[src].size() > MTL_MAX_PREDS) return set_error(MTL_UNSUPPORTED, "too many predicates");
-  }
+  std::vector<std::vector<mtl_pred>> preds;
+  std::vector<uint32_t> pred_or;
+  MTL_TRY(bind_preds(q, b, preds, pred_or));
+  // one edge per joined pair of tables; several equalities between the same
+  // pair are a multi-column key
   struct Edge {
     int a, c;
-    const CatCol *ca, *cc;
+    const CatCol *ca, *cc;              // the first key pair
+    std::vector<const CatCol*> xa, xc;  // every key pair (a side, c side)
     bool used = false;
   };
   std::vector<Edge> edges;
@@ -1531,6 +1739,19 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     if (!is_int_type(e.ca->type) || !is_int_type(e.cc->type) || e.ca->type == MTL_DICT8 ||
         e.cc->type == MTL_DICT8 || e.ca->type == MTL_DICT32 || e.cc->type == MTL_DICT32)
       return set_error(MTL_UNSUPPORTED, "join on a non-integer column");
+    Edge* same = nullptr;
+    for (auto& x : edges)
+      if ((x.a == e.a && x.c == e.c) || (x.a == e.c && x.c == e.a)) same = &x;
+    if (same) {
+      const bool flip = same->a != e.a;
+      same->xa.push_back(flip ? e.cc : e.ca);
+      same->xc.push_back(flip ? e.ca : e.cc);
+      if (same->xa.size() > MTL_GROUP_MAX_KEYS)
+        return set_error(MTL_UNSUPPORTED, "more than %d join keys", MTL_GROUP_MAX_KEYS);
+      continue;
+    }
+    e.xa.push_back(e.ca);
+    e.xc.push_back(e.cc);
     edges.push_back(e);
   }
   if (edges.size() != ns - 1)
@@ -1547,6 +1768,11 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
   std::vector<Col> gkeys;  // group keys
   std::vector<const CatCol*> gkey_cols;
   std::vector<Col> gvals;  // aggregate value columns (src -1: COUNT(*))
+  struct AggExpr {
+    mtl_expr x{};
+    std::vector<Col> cols;  // the expression's columns and their tables
+  };
+  std::vector<std::unique_ptr<AggExpr>> gexpr;  // per aggregate; null: a column
   std::vector<GroupOut> gouts;
   std::vector<int32_t> agg_func, agg_type, agg_scale;
   mtl_group_desc d{};
@@ -1587,21 +1813,45 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
         const int32_t func = f == "sum" ? MTL_AGG_SUM : f == "count" ? MTL_AGG_COUNT
                              : f == "min" ? MTL_AGG_MIN : f == "max" ? MTL_AGG_MAX : MTL_AGG_AVG;
         Col c{-1, nullptr};
-        if (it.e->a) {
-          if (it.e->a->k != Expr::COL) return set_error(MTL_UNSUPPORTED, "aggregate over an expression");
+        int32_t vtype = 0, vscale = 0;
+        std::unique_ptr<AggExpr> ax;
+        if (it.e->a && it.e->a->k != Expr::COL) {
+          // an integer/DECIMAL expression of columns of any joined tables,
+          // evaluated per joined row into an int64 column
+          if (func == MTL_AGG_COUNT) return set_error(MTL_UNSUPPORTED, "COUNT of an expression");
+          ax.reset(new AggExpr());
+          std::vector<const CatCol*> cols;
+          std::vector<int> srcs;
+          std::string err;
+          int sc = 0;
+          if (!to_postfix(it.e->a, b, -1, ax->x, sc, cols, err, &srcs))
+            return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
+          bool dec = false;
+          for (size_t k = 0; k < cols.size(); ++k) {
+            ax->cols.push_back(Col{srcs[k], cols[k]});
+            dec |= cols[k]->type == MTL_DECIMAL;
+          }
+          if (cols.empty()) return set_error(MTL_UNSUPPORTED, "aggregate of a constant");
+          vtype = dec ? MTL_DECIMAL : MTL_BIGINT;
+          vscale = dec ? std::max(sc, 0) : 0;
+          c = Col{ax->cols[0].src, ax->cols[0].c};  // non-null: the aggregate reads values
+        } else if (it.e->a) {
           if (!b.resolve(it.e->a->col, c.src, c.c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
           if (c.c->type == MTL_DICT8 || c.c->type == MTL_DICT32)
             return set_error(MTL_UNSUPPORTED, "aggregate over a dictionary column");
+          vtype = c.c->type;
+          vscale = c.c->type == MTL_DECIMAL ? c.c->scale : 0;
         } else if (func != MTL_AGG_COUNT) {
           return set_error(MTL_UNSUPPORTED, "%s(*)", f.c_str());
         }
         d.aggs[d.n_aggs].func = func;
-        d.aggs[d.n_aggs].type = c.c ? c.c->type : 0;
-        d.aggs[d.n_aggs].scale = c.c && c.c->type == MTL_DECIMAL ? c.c->scale : 0;
+        d.aggs[d.n_aggs].type = vtype;
+        d.aggs[d.n_aggs].scale = vscale;
         agg_func.push_back(func);
-        agg_type.push_back(c.c ? c.c->type : 0);
-        agg_scale.push_back(c.c ? c.c->scale : 0);
+        agg_type.push_back(vtype);
+        agg_scale.push_back(vscale);
         gvals.push_back(c);
+        gexpr.push_back(std::move(ax));
         o.agg = (int)d.n_aggs++;
       } else {
         return set_error(MTL_UNSUPPORTED, "select item %zu of an aggregate query", i);
@@ -1616,6 +1866,7 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
   const bool grouped = !gkeys.empty() || has_agg;
   int order_agg = -1, key_order = 0;
   bool odesc = false;
+  std::vector<std::pair<int, bool>> then_keys;
   Col okey{-1, nullptr};
   if (!grouped) {
     if (q.order.size() > 1) return set_error(MTL_UNSUPPORTED, "ORDER BY more than one column");
@@ -1645,7 +1896,8 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     }
     if (keys_ok) {
       key_order = q.order[0].desc ? -1 : 1;
-    } else if (q.order.size() == 1) {
+    } else {
+      // one aggregate, then group keys as tie-breaks
       const ExprP& e = q.order[0].e;
       for (size_t i = 0; i < q.items.size(); ++i) {
         if (gouts[i].agg < 0) continue;
@@ -1658,12 +1910,25 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
         if (by_alias || same_call) order_agg = gouts[i].agg;
       }
       if (order_agg < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY of an aggregate query");
+      for (size_t i = 1; i < q.order.size(); ++i) {
+        const ExprP& x = q.order[i].e;
+        if (x->k != Expr::COL) return set_error(MTL_UNSUPPORTED, "ORDER BY of an aggregate query");
+        int key = -1;
+        for (auto& o : gouts)
+          if (o.key >= 0 && x->col.rel.empty() && o.name == x->col.name) key = o.key;
+        int src;
+        const CatCol* c = nullptr;
+        if (key < 0 && b.resolve(x->col, src, c))
+          for (size_t k = 0; k < gkeys.size(); ++k)
+            if (gkeys[k].c == c && gkeys[k].src == src) key = (int)k;
+        if (key < 0)
+          return set_error(MTL_UNSUPPORTED, "ORDER BY a column that is not a group key");
+        then_keys.emplace_back(key, q.order[i].desc);
+      }
       if (q.limit < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY an aggregate needs LIMIT");
       if (q.limit > 0xFFFFFFFFll)
         return set_error(MTL_UNSUPPORTED, "LIMIT %lld exceeds 2^32-1", (long long)q.limit);
       odesc = q.order[0].desc;
-    } else {
-      return set_error(MTL_UNSUPPORTED, "ORDER BY of an aggregate query");
     }
   }
   // group descriptor (columns filled after the joins)
@@ -1694,7 +1959,9 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     rows_max = std::max(rows_max, t->rows);
     rows_sum += t->rows;
   }
-  const uint64_t ncols = proj.size() + gkeys.size() + gvals.size() + 1;
+  uint64_t ncols = proj.size() + gkeys.size() + gvals.size() + 1;
+  for (auto& ax : gexpr)
+    if (ax) ncols += ax->cols.size();
   res.scratch = rows_sum * 8 + rows_max * (4 * ns * 2 + 16 + 8 * ncols) +
                 (grouped ? group_scratch_bytes(d) : 0) + (1u << 20);
   if (dry) return MTL_OK;
@@ -1709,8 +1976,8 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     const CatTable& T = *b.tabs[i];
     sel[i].reset(new DevBuf());
     MTL_TRY(sel[i]->alloc(std::max<uint64_t>(T.rows, 1) * 4));
-    MTL_TRY(mtl_filter(preds[i].data(), (uint32_t)preds[i].size(), T.rows, nullptr, nullptr, 0,
-                       (uint32_t*)sel[i]->p, (uint32_t*)cnt.p, s));
+    MTL_TRY(mtl_filter_or(preds[i].data(), (uint32_t)preds[i].size(), pred_or[i], T.rows,
+                          (uint32_t*)sel[i]->p, (uint32_t*)cnt.p, s));
     uint32_t n = 0;
     MTL_TRY(mtl_memcpy_d2h(&n, cnt.p, 4, s));
     if (n > T.rows) return set_error(MTL_ERR_PROCESSING, "device scan failed");
@@ -1732,23 +1999,57 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     e->used = true;
     int a = e->a, c = e->c;
     const CatCol *xa = e->ca, *yc = e->cc;
+    std::vector<const CatCol*> ka = e->xa, kc = e->xc;
     if (!R.ids[a]) {
       std::swap(a, c);
       std::swap(xa, yc);
+      std::swap(ka, kc);
     }
-    // the relation's key column, gathered through its row ids
-    DevBuf kx, pp, pb;
-    const uint32_t wx = (uint32_t)type_width(xa->type);
-    MTL_TRY(kx.alloc(std::max<uint64_t>(R.m, 1) * wx));
-    if (R.m) MTL_TRY(mtl_gather(xa->dev, wx, (const uint32_t*)R.ids[a]->p, nullptr, R.m, kx.p, s));
+    // the relation's key column, gathered through its row ids; a
+    // multi-column key is packed on both sides (the new table's over all its
+    // rows, read through its selection)
+    DevBuf kx, ky, pp, pb;
+    const void* ycol = yc->dev;
+    int32_t xtype = xa->type, ytype = yc->type;
+    int64_t xmin = xa->min, xmax = xa->max, ymin = yc->min, ymax = yc->max;
+    if (ka.size() > 1) {
+      PackKeys pa{}, pc{};
+      pa.n_keys = pc.n_keys = (uint32_t)ka.size();
+      unsigned __int128 total = 1;
+      for (int k = (int)ka.size() - 1; k >= 0; --k) {
+        const int64_t lo = std::min(ka[k]->min, kc[k]->min);
+        const int64_t hi = std::max(ka[k]->max, kc[k]->max);
+        const unsigned __int128 span = (unsigned __int128)((uint64_t)hi - (uint64_t)lo) + 1;
+        pa.col[k] = ka[k]->dev;
+        pc.col[k] = kc[k]->dev;
+        pa.type[k] = ka[k]->type;
+        pc.type[k] = kc[k]->type;
+        pa.min[k] = pc.min[k] = lo;
+        pa.mult[k] = pc.mult[k] = (uint64_t)total;
+        total *= span;
+        if (total > ((unsigned __int128)1 << 62))
+          return set_error(MTL_UNSUPPORTED, "multi-column join key ranges too wide to pack");
+      }
+      MTL_TRY(pack_keys(pa, (const uint32_t*)R.ids[a]->p, R.m, kx, s));
+      MTL_TRY(pack_keys(pc, nullptr, b.tabs[c]->rows, ky, s));
+      ycol = ky.p;
+      xtype = ytype = MTL_BIGINT;
+      xmin = ymin = 0;
+      xmax = ymax = (int64_t)(total - 1);
+    } else {
+      const uint32_t wx = (uint32_t)type_width(xa->type);
+      MTL_TRY(kx.alloc(std::max<uint64_t>(R.m, 1) * wx));
+      if (R.m)
+        MTL_TRY(mtl_gather(xa->dev, wx, (const uint32_t*)R.ids[a]->p, nullptr, R.m, kx.p, s));
+    }
     uint64_t np = 0;
     // build over the new table's rows (its key unique), probe with the relation
-    int32_t st = join_pairs(yc->dev, yc->type, (const uint32_t*)sel[c]->p, nsel[c], yc->min,
-                            yc->max, kx.p, xa->type, nullptr, R.m, pp, pb, np, s);
+    int32_t st = join_pairs(ycol, ytype, (const uint32_t*)sel[c]->p, nsel[c], ymin, ymax, kx.p,
+                            xtype, nullptr, R.m, pp, pb, np, s);
     if (st == MTL_UNSUPPORTED) {
       // build over the relation (its key unique), probe with the new table
       DevBuf pp2, pb2;
-      st = join_pairs(kx.p, xa->type, nullptr, R.m, xa->min, xa->max, yc->dev, yc->type,
+      st = join_pairs(kx.p, xtype, nullptr, R.m, xmin, xmax, ycol, ytype,
                       (const uint32_t*)sel[c]->p, nsel[c], pp2, pb2, np, s);
       if (st == MTL_UNSUPPORTED)
         return set_error(MTL_UNSUPPORTED, "join of %s and %s: keys repeat on both sides",
@@ -1833,10 +2134,34 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     d.key_col[0] = mats.back()->p;
   }
   for (size_t k = 0; k < gkeys.size(); ++k) MTL_TRY(gathered(gkeys[k], &d.key_col[k]));
-  for (size_t a = 0; a < gvals.size(); ++a)
-    if (gvals[a].c) MTL_TRY(gathered(gvals[a], &d.aggs[a].values));
+  DevBuf eflag;
+  bool any_expr = false;
+  for (size_t a = 0; a < gvals.size(); ++a) {
+    if (!gvals[a].c) continue;
+    if (!gexpr[a]) {
+      MTL_TRY(gathered(gvals[a], &d.aggs[a].values));
+      continue;
+    }
+    // gather the expression's columns into row order, then evaluate it
+    AggExpr& ax = *gexpr[a];
+    for (size_t k = 0; k < ax.cols.size(); ++k) MTL_TRY(gathered(ax.cols[k], &ax.x.col[k]));
+    if (!any_expr) {
+      MTL_TRY(eflag.alloc(4));
+      MTL_HIP(hipMemsetAsync(eflag.p, 0, 4, s->stream));
+      any_expr = true;
+    }
+    mats.emplace_back(new DevBuf());
+    MTL_TRY(mats.back()->alloc(std::max<uint64_t>(m, 1) * 8));
+    MTL_TRY(mtl::expr_eval_i64(ax.x, m, (int64_t*)mats.back()->p, (uint32_t*)eflag.p, s->stream));
+    d.aggs[a].values = mats.back()->p;
+  }
+  if (any_expr) {
+    uint32_t ef = 0;
+    MTL_TRY(mtl_memcpy_d2h(&ef, eflag.p, 4, s));
+    if (ef) return set_error(MTL_ERR_OVERFLOW, "Overflow in aggregate expression");
+  }
   std::vector<const CatCol*> ek = gkey_cols;
-  GroupEmit ge{ek, agg_func, agg_type, agg_scale, order_agg, odesc, key_order};
+  GroupEmit ge{ek, agg_func, agg_type, agg_scale, order_agg, odesc, key_order, then_keys};
   return emit_groups(q, d, ge, gouts, res, s);
 }
 
@@ -1945,8 +2270,18 @@ static int32_t rewrite(const char* sql, std::string& out) {
   }
   std::vector<std::string> conj;
   for (auto& j : q.joins) conj.push_back(col_sql(j.l) + " = " + col_sql(j.r));
-  for (auto& p : q.preds)
-    conj.push_back(std::string(hook[p.op]) + "(" + col_sql(p.col) + ", " + lit_sql(p.lit) + ")");
+  std::string run;  // the current run of OR-ed predicates
+  size_t in_run = 0;
+  for (auto& p : q.preds) {
+    run += (in_run ? " OR " : "") + std::string(hook[p.op]) + "(" + col_sql(p.col) + ", " +
+           lit_sql(p.lit) + ")";
+    ++in_run;
+    if (!p.or_next) {
+      conj.push_back(in_run > 1 ? "(" + run + ")" : run);
+      run.clear();
+      in_run = 0;
+    }
+  }
   for (size_t i = 0; i < conj.size(); ++i) out += (i ? " AND " : " WHERE ") + conj[i];
   for (size_t i = 0; i < q.group.size(); ++i) out += (i ? ", " : " GROUP BY ") + col_sql(q.group[i]);
   for (size_t i = 0; i < q.order.size(); ++i)

@@ -1338,6 +1338,32 @@ static int32_t star_error(uint32_t err) {
 
 int32_t star_check_error(uint32_t err) { return star_error(err); }
 
+// One integer expression value per row (an aggregate over an expression of
+// joined columns, gathered into row order first).
+__global__ __launch_bounds__(256) void k_expr_i64(mtl_expr e, uint64_t n, int64_t* out,
+                                                  uint32_t* err) {
+  bool ovf = false;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    int64_t cv[MTL_EXPR_MAX_COLS];
+#pragma unroll
+    for (int c = 0; c < MTL_EXPR_MAX_COLS; ++c)
+      cv[c] = (uint32_t)c < e.n_cols ? load_key(e.col[c], e.col_type[c], i) : 0;
+    out[i] = expr_i64_vals(e, cv, ovf);
+  }
+  if (__any(ovf) && (threadIdx.x & 63) == 0) atomicOr(err, 1u);
+}
+
+int32_t expr_eval_i64(const mtl_expr& e, uint64_t n, int64_t* out, uint32_t* err,
+                      hipStream_t st) {
+  if (e.is_double) return set_error(MTL_UNSUPPORTED, "floating-point aggregate expression");
+  MTL_TRY(check_expr(e));
+  if (!n) return MTL_OK;
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_expr_i64, dim3((uint32_t)blocks), dim3(256), 0, st, e, n, out, err);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
+}
+
 }  // namespace mtl
 
 using namespace mtl;

@@ -37,5 +37,9 @@ int32_t star_batch_enqueue(const StarBatchItem* items, uint32_t n, void* host_st
                            mtl_stream* s);
 // Status of the error bits a star run leaves in out_n[1].
 int32_t star_check_error(uint32_t err);
+// out[i] = the integer expression e over row i of its columns, for i < n;
+// an overflow sets err[0] bit 0 (err is not cleared here).
+int32_t expr_eval_i64(const mtl_expr& e, uint64_t n, int64_t* out, uint32_t* err,
+                      hipStream_t st);
 
 }  // namespace mtl

@@ -75,6 +75,19 @@ def filter(preds, n_rows: int, in_sel: torch.Tensor | None = None,
     return out[:n], n
 
 
+def filter_or(preds, or_next: int, n_rows: int):
+    """mtl_filter_or: predicates p and p+1 OR-ed where bit p of or_next is
+    set, the runs AND-ed.  Returns (sel, count)."""
+    torch.cuda.synchronize()
+    out = torch.empty(max(n_rows, 4), dtype=torch.int32, device=device())
+    cnt = torch.zeros(1, dtype=torch.int32, device=device())
+    abi.check(abi.lib().mtl_filter_or(_preds(preds), len(preds), or_next, n_rows, ptr(out),
+                                      ptr(cnt), None))
+    sync()
+    n = _count(cnt, n_rows)
+    return out[:n], n
+
+
 def filter_project(preds, n_rows: int, col: torch.Tensor, t: int, op: int, value,
                    in_sel: torch.Tensor | None = None, in_count: torch.Tensor | None = None):
     """Fused filter + projection of the passing rows.

@@ -45,6 +45,42 @@ def test_filter_parity(ops, t, dtype, op, n):
     np.testing.assert_array_equal(ops.to_host(got, np.uint32), ref)
 
 
+@pytest.mark.parametrize("n", [0, 1, 4097, 300_001])
+@pytest.mark.parametrize("seed", range(4))
+def test_filter_or_runs(ops, n, seed):
+    """mtl_filter_or against set algebra over the oracle's one-predicate
+    filters: each OR run is the union of its predicates' row sets, the runs
+    are intersected."""
+    rng = np.random.default_rng(50 + seed)
+    cols = {abi.INTEGER: rng.integers(-1000, 1000, max(n, 1)).astype(np.int32)[:n],
+            abi.BIGINT: rng.integers(-50, 50, max(n, 1)).astype(np.int64)[:n],
+            abi.DOUBLE: rng.standard_normal(max(n, 1))[:n],
+            abi.DICT8: rng.integers(0, 9, max(n, 1)).astype(np.uint8)[:n]}
+    dev = {t: ops.to_device(c if n else np.zeros(1, c.dtype)) for t, c in cols.items()}
+    k = int(rng.integers(2, 9))
+    preds, dpreds = [], []
+    for i in range(k):
+        t = [abi.INTEGER, abi.BIGINT, abi.DOUBLE, abi.DICT8][int(rng.integers(0, 4))]
+        op = int(rng.integers(0, 5))
+        c = cols[t]
+        lit = c[int(rng.integers(0, n))].item() if n else 0
+        preds.append((c, t, op, lit))
+        dpreds.append((dev[t], t, op, lit))
+    or_next = int(rng.integers(0, 1 << (k - 1)))
+    keep = np.ones(n, bool)
+    run = np.zeros(n, bool)
+    for i, p in enumerate(preds):
+        m = np.zeros(n, bool)
+        m[orc.filter([p], n)] = True
+        run |= m
+        if not (or_next >> i) & 1:
+            keep &= run
+            run = np.zeros(n, bool)
+    got, cnt = ops.filter_or(dpreds, or_next, n)
+    assert cnt == int(keep.sum())
+    np.testing.assert_array_equal(ops.to_host(got, np.uint32), np.flatnonzero(keep))
+
+
 def test_filter_multi_predicate_and_refine(ops):
     rng = np.random.default_rng(2)
     n = 300_001

@@ -155,7 +155,9 @@ def test_projection_overflow_and_unsupported(ops):
     np.testing.assert_array_equal(S.execute("SELECT x * 4 FROM o WHERE x < 100")["col0"][2],
                                   [20, -28])
     with pytest.raises(abi.Unsupported):
-        S.execute("SELECT x FROM o WHERE x IN (1, 2)")
+        S.execute("SELECT x FROM o WHERE x NOT IN (1, 2)")
+    np.testing.assert_array_equal(S.execute("SELECT x FROM o WHERE x IN (5, -7)")["x"][2],
+                                  [5, -7])
 
 
 @pytest.mark.parametrize("t,dtype", [(abi.INTEGER, np.int32), (abi.BIGINT, np.int64),

@@ -39,10 +39,11 @@ def _table(ops, name, cols):
     for c in cols:
         cname, t, v = c[:3]
         unique = c[3] if len(c) > 3 else False
+        scale = c[4] if len(c) > 4 else 0
         d = ops.to_device(np.ascontiguousarray(v))
         _keep.append(d)
         mn, mx = (int(v.min()), int(v.max())) if n and t not in (abi.FLOAT, abi.DOUBLE) else (0, 0)
-        spec.append((cname, t, d.data_ptr(), mn, mx, unique))
+        spec.append((cname, t, d.data_ptr(), mn, mx, unique, scale))
     S.register_table(name, n, spec)
 
 
@@ -187,3 +188,147 @@ def test_join_with_keys_repeating_on_both_sides_falls_back(ops):
                       ("w", abi.INTEGER, np.array([8, 9, 10], np.int32))])
     with pytest.raises(abi.Unsupported):
         S.execute("SELECT a.v, b.w FROM a JOIN b ON a.k = b.k")
+
+
+def test_or_and_in_predicates(ops):
+    """IN lists and parenthesised ORs (a conjunction of disjunctions) in a
+    scan, a group-by and a join, against numpy."""
+    rng = np.random.default_rng(11)
+    n = 300_000
+    v = rng.integers(-1000, 1000, n).astype(np.int32)
+    w = rng.integers(0, 50, n).astype(np.int64)
+    d = rng.integers(10957, 22281, n).astype(np.int32)
+    _table(ops, "t", [("v", abi.INTEGER, v), ("w", abi.BIGINT, w), ("d", abi.DATE, d)])
+    out = S.execute("SELECT v, w FROM t WHERE w IN (3, 7, 11) AND (v < -900 OR v >= 950) "
+                    "ORDER BY v")
+    m = np.isin(w, [3, 7, 11]) & ((v < -900) | (v >= 950))
+    sel = np.flatnonzero(m)
+    exp = sel[np.argsort(v[sel], kind="stable")]
+    assert _ints(out, "v") == [int(x) for x in v[exp]]
+    assert _ints(out, "w") == [int(x) for x in w[exp]]
+    # bare OR as the whole WHERE, and an IN on a DATE column
+    out = S.execute("SELECT COUNT(*) AS n FROM t WHERE v = 0 OR w = 49")
+    assert _ints(out, "n") == [int(((v == 0) | (w == 49)).sum())]
+    out = S.execute("SELECT w, COUNT(*) AS n, SUM(v) AS s FROM t "
+                    "WHERE d IN (DATE '2000-01-01', DATE '2010-06-15', DATE '2030-12-31') "
+                    "GROUP BY w ORDER BY w")
+    import datetime
+    days = [(datetime.date(y, mo, dd) - datetime.date(1970, 1, 1)).days
+            for y, mo, dd in ((2000, 1, 1), (2010, 6, 15), (2030, 12, 31))]
+    m = np.isin(d, days)
+    g = pd.DataFrame({"w": w[m], "v": v[m]}).groupby("w").v.agg(["size", "sum"])
+    assert _ints(out, "w") == list(g.index)
+    assert _ints(out, "n") == [int(x) for x in g["size"]]
+    assert _ints(out, "s") == [int(x) for x in g["sum"]]
+
+
+def test_or_in_a_join_and_the_star_shape(ops):
+    c, o, l = _star(ops, 7)
+    out = S.execute("SELECT c.c_region, COUNT(*) AS n FROM lines l "
+                    "JOIN orders o ON l.l_order = o.o_key JOIN cust c ON o.o_cust = c.c_key "
+                    "WHERE c.c_region IN (1, 4, 6) AND (o.o_prio = 0 OR o.o_prio = 3) "
+                    "GROUP BY c.c_region ORDER BY c.c_region")
+    j = l.merge(o[o.o_prio.isin([0, 3])], left_on="l_order", right_on="o_key") \
+        .merge(c[c.c_region.isin([1, 4, 6])], left_on="o_cust", right_on="c_key")
+    g = j.groupby("c_region").size()
+    assert _ints(out, "c_region") == list(g.index)
+    assert _ints(out, "n") == [int(x) for x in g]
+
+
+def test_multi_column_join_key(ops):
+    """ON a.x = b.x AND a.y = b.y: the two columns are packed into one key
+    on both sides; checked against pandas."""
+    rng = np.random.default_rng(13)
+    na, nb = 20_000, 400_000
+    pairs = np.unique(rng.integers(0, 300, (na * 2, 2)), axis=0)[:na]
+    rng.shuffle(pairs)
+    ax, ay = pairs[:, 0].astype(np.int32), (pairs[:, 1] - 100).astype(np.int64)
+    av = rng.integers(-10**6, 10**6, len(ax)).astype(np.int64)
+    bx = rng.integers(0, 320, nb).astype(np.int32)
+    by = (rng.integers(0, 300, nb) - 100).astype(np.int64)
+    bw = rng.integers(0, 100, nb).astype(np.int32)
+    _table(ops, "a", [("x", abi.INTEGER, ax), ("y", abi.BIGINT, ay), ("v", abi.BIGINT, av)])
+    _table(ops, "b", [("x", abi.INTEGER, bx), ("y", abi.BIGINT, by), ("w", abi.INTEGER, bw)])
+    out = S.execute("SELECT b.w, SUM(a.v) AS s, COUNT(*) AS n FROM a JOIN b "
+                    "ON a.x = b.x AND b.y = a.y WHERE b.w < 60 GROUP BY b.w ORDER BY b.w")
+    A = pd.DataFrame({"x": ax, "y": ay, "v": av})
+    B = pd.DataFrame({"x": bx, "y": by, "w": bw})
+    j = B[B.w < 60].merge(A, on=["x", "y"])
+    g = j.groupby("w").v.agg(["sum", "size"])
+    assert _ints(out, "w") == list(g.index)
+    assert _ints(out, "s") == [int(x) for x in g["sum"]]
+    assert _ints(out, "n") == [int(x) for x in g["size"]]
+
+
+
+def test_aggregates_over_expressions_of_joined_columns(ops):
+    """SUM/MIN/MAX/AVG of integer and DECIMAL expressions whose columns
+    come from different joined tables (Q3's SUM(price * (100 - discount))
+    once an IN list moves it off the star shape), against pandas."""
+    c, o, l = _star(ops, 17)
+    price = np.random.default_rng(5).integers(100, 100_000, len(l)).astype(np.int64)
+    _table(ops, "items", [("l_order", abi.INTEGER, l.l_order.to_numpy()),
+                          ("l_disc", abi.INTEGER, l.l_disc.to_numpy()),
+                          ("l_qty", abi.BIGINT, l.l_qty.to_numpy()),
+                          ("price", abi.DECIMAL, price, False, 2)])
+    out = S.execute("SELECT c.c_region, SUM(l.price * (100 - l.l_disc)) AS rev, "
+                    "MAX(l.l_qty - o.o_prio) AS hi, MIN(o.o_prio * 7 + l.l_disc) AS lo, "
+                    "AVG(l.l_qty * 2) AS avg2, COUNT(*) AS n "
+                    "FROM items l JOIN orders o ON l.l_order = o.o_key "
+                    "JOIN cust c ON o.o_cust = c.c_key "
+                    "WHERE c.c_region IN (0, 2, 5) GROUP BY c.c_region ORDER BY c.c_region")
+    L = l.assign(price=price)
+    j = L.merge(o, left_on="l_order", right_on="o_key") \
+        .merge(c[c.c_region.isin([0, 2, 5])], left_on="o_cust", right_on="c_key")
+    j["rev"] = j.price * (100 - j.l_disc)
+    j["hi"] = j.l_qty - j.o_prio
+    j["lo"] = j.o_prio * 7 + j.l_disc
+    j["q2"] = j.l_qty * 2
+    g = j.groupby("c_region").agg(rev=("rev", "sum"), hi=("hi", "max"), lo=("lo", "min"),
+                                  q2=("q2", "mean"), n=("rev", "size"))
+    assert _ints(out, "c_region") == list(g.index)
+    t, sc, _ = out["rev"]
+    assert (t, sc) == (abi.HUGEINT, 2) or (t, sc) == (abi.DECIMAL, 2)
+    assert _ints(out, "rev") == [int(x) for x in g.rev]
+    assert _ints(out, "hi") == [int(x) for x in g.hi]
+    assert _ints(out, "lo") == [int(x) for x in g.lo]
+    assert _ints(out, "n") == [int(x) for x in g.n]
+    np.testing.assert_allclose(np.asarray(out["avg2"][2], np.float64), g.q2.to_numpy(), rtol=1e-12)
+
+
+def test_expression_aggregate_overflow_is_an_error(ops):
+    big = np.full(1000, 2**40, np.int64)
+    _table(ops, "a", [("k", abi.INTEGER, np.arange(1000, dtype=np.int32), True),
+                      ("v", abi.BIGINT, big)])
+    _table(ops, "b", [("k", abi.INTEGER, np.arange(1000, dtype=np.int32))])
+    with pytest.raises(abi.MettleError) as ei:
+        S.execute("SELECT SUM(a.v * a.v) AS s FROM a JOIN b ON a.k = b.k")
+    assert "Overflow" in str(ei.value)
+
+
+def test_order_by_aggregate_then_group_keys(ops):
+    """ORDER BY an aggregate, then group keys, LIMIT k (Q3's ORDER BY
+    revenue DESC, o_orderdate): ties at the LIMIT boundary are ordered by the
+    named keys, then by the remaining group keys ascending."""
+    c, o, l = _star(ops, 19)
+    j = l.merge(o, left_on="l_order", right_on="o_key").merge(c, left_on="o_cust", right_on="c_key")
+    # many groups, heavy ties in the count: the top-K widens to every group
+    out = S.execute("SELECT o.o_key, o.o_prio, COUNT(*) AS n, SUM(l.l_qty * l.l_disc) AS s "
+                    "FROM lines l JOIN orders o ON l.l_order = o.o_key "
+                    "GROUP BY o.o_key, o.o_prio ORDER BY n DESC, o.o_prio DESC, o.o_key LIMIT 20")
+    g = j.assign(x=j.l_qty * j.l_disc).groupby(["o_key", "o_prio"]).agg(
+        n=("x", "size"), s=("x", "sum")).reset_index()
+    g = g.sort_values(["n", "o_prio", "o_key"], ascending=[False, False, True]).head(20)
+    assert _ints(out, "o_key") == [int(x) for x in g.o_key]
+    assert _ints(out, "o_prio") == [int(x) for x in g.o_prio]
+    assert _ints(out, "n") == [int(x) for x in g.n]
+    assert _ints(out, "s") == [int(x) for x in g.s]
+    # few groups: the candidate set holds them all
+    out = S.execute("SELECT c.c_region, o.o_prio, SUM(l.l_qty) AS q FROM lines l "
+                    "JOIN orders o ON l.l_order = o.o_key JOIN cust c ON o.o_cust = c.c_key "
+                    "GROUP BY o.o_prio, c.c_region ORDER BY q, c.c_region DESC LIMIT 7")
+    g = j.groupby(["o_prio", "c_region"]).l_qty.sum().rename("q").reset_index()
+    g = g.sort_values(["q", "c_region", "o_prio"], ascending=[True, False, True]).head(7)
+    assert _ints(out, "c_region") == [int(x) for x in g.c_region]
+    assert _ints(out, "o_prio") == [int(x) for x in g.o_prio]
+    assert _ints(out, "q") == [int(x) for x in g.q]

@@ -70,13 +70,29 @@ Q3 = ("SELECT l.l_order, SUM(l.price * (100 - l.discount)) AS revenue, o.o_date,
      "JOIN orders o ON c.c_key = o.o_cust JOIN lines l ON l.l_order = o.o_key "
      "GROUP BY c.segment, o.o_prio ORDER BY c.segment, o.o_prio", "join"),
     ("SELECT COUNT(*) FROM orders o JOIN lines l ON l.l_order = o.o_key", "join"),
+    # OR runs and IN lists
+    ("SELECT value FROM numbers WHERE value > 1 OR value < 0", "scan"),
+    ("SELECT value FROM numbers WHERE value IN (1, 5, 9) AND (value < 4 OR value > 8)", "scan"),
+    ("SELECT g, COUNT(*) FROM t WHERE g IN (1, 3) OR c_i < 0 GROUP BY g", "group"),
+    ("SELECT o.o_key FROM customers c JOIN orders o ON c.c_key = o.o_cust "
+     "WHERE c.segment IN ('BUILDING', 'MACHINERY') AND (o.o_prio = 1 OR o.o_prio > 3)", "join"),
+    (Q3.replace("c.segment = 'BUILDING'", "c.segment IN ('BUILDING', 'FURNITURE')"), "join"),
+    # two equalities between one pair of tables: a packed two-column key
+    ("SELECT o.o_key, l.price FROM orders o JOIN lines l ON l.l_order = o.o_key "
+     "AND l.discount = o.o_prio", "join"),
 ])
 def test_admitted_shapes(sql, shape):
     assert S.explain(sql) == shape
 
 
 @pytest.mark.parametrize("sql,why", [
-    ("SELECT value FROM numbers WHERE value > 1 OR value < 0", "OR"),
+    ("SELECT value FROM numbers WHERE value > 1 AND value < 9 OR value = 0", "parentheses"),
+    ("SELECT value FROM numbers WHERE value = 0 OR value > 1 AND value < 9", "parentheses"),
+    ("SELECT value FROM numbers WHERE (value > 1 AND value < 9) OR value = 0", "OR over a conj"),
+    ("SELECT value FROM numbers WHERE value BETWEEN 1 AND 9 OR value = 0", "OR over a conj"),
+    ("SELECT value FROM numbers WHERE value NOT IN (1, 2)", "NOT"),
+    ("SELECT o.o_key FROM customers c JOIN orders o ON c.c_key = o.o_cust "
+     "WHERE c.c_key = 1 OR o.o_prio = 2", "OR across tables"),
     ("SELECT DISTINCT value FROM numbers", "DISTINCT"),
     ("SELECT value FROM numbers WHERE value / 2 > 1", "expected a comparison"),
     ("SELECT value / 2 FROM numbers", "division"),
@@ -124,4 +140,4 @@ def test_rewrite_uses_hooks_and_round_trips():
 
 def test_rewrite_refuses_outside_the_subset():
     with pytest.raises(abi.Unsupported):
-        S.rewrite("SELECT a FROM t WHERE a IN (1, 2)")
+        S.rewrite("SELECT a FROM t WHERE a NOT IN (1, 2)")
