@@ -17,6 +17,7 @@
 
 #include "device.hpp"
 #include "filter.hpp"
+#include "join.hpp"
 #include "mtl_internal.hpp"
 #include "prof.hpp"
 #include "rowtile.hpp"
@@ -343,3 +344,216 @@ int32_t mtl_join_probe(int32_t strategy, const void* table, uint64_t table_bytes
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Many-to-many joins (keys repeating on both sides).  The reference's build
+// overwrites a repeated key (kernels.metal:202-206), losing matches; here the
+// build side is sorted by key (stable), its runs of equal keys become a
+// unique-key table (run start, key), the probe side joins that table, and
+// each (probe, run) pair expands to one pair per build row of the run.
+// ---------------------------------------------------------------------------
+namespace mtl {
+using namespace dev;
+
+constexpr int MM_BLOCK = 256;
+constexpr int MM_STEPS = 16;
+constexpr uint64_t MM_TILE = (uint64_t)MM_BLOCK * MM_STEPS;
+
+__device__ __forceinline__ bool run_head(const void* keys, int32_t type, uint64_t i, uint64_t n) {
+  return i < n && (i == 0 || load_key(keys, type, i) != load_key(keys, type, i - 1));
+}
+
+// Exclusive rank of this thread's flag among the block's flags of this step;
+// *step_total receives the step's count.  Every thread of the block calls it.
+__device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* s_wave, uint32_t* step_total) {
+  const int tid = threadIdx.x, wave = tid / WAVE;
+  const uint64_t bal = __ballot(flag);
+  if (lane_id() == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < MM_BLOCK / WAVE; ++w) {
+    before += w < wave ? s_wave[w] : 0;
+    total += s_wave[w];
+  }
+  __syncthreads();
+  *step_total = total;
+  return before + (uint32_t)__popcll(bal & lanemask_lt());
+}
+
+__global__ __launch_bounds__(MM_BLOCK) void k_mm_head_count(const void* keys, int32_t type,
+                                                           uint64_t n, uint32_t* tile_cnt) {
+  __shared__ uint32_t s_wave[MM_BLOCK / WAVE];
+  const uint64_t base = blockIdx.x * MM_TILE;
+  uint32_t c = 0;
+  for (int it = 0; it < MM_STEPS; ++it) c += run_head(keys, type, base + it * MM_BLOCK + threadIdx.x, n);
+  c = wave_sum_u32(c);
+  if (lane_id() == 0) s_wave[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < MM_BLOCK / WAVE; ++w) t += s_wave[w];
+    tile_cnt[blockIdx.x] = t;
+  }
+}
+
+// run_start[r] = first sorted position of run r, run_key[r] = its key;
+// run_start[runs] = n (runs = *n_runs, the scanned total).
+__global__ __launch_bounds__(MM_BLOCK) void k_mm_head_write(const void* keys, int32_t type,
+                                                           uint64_t n, const uint32_t* tile_off,
+                                                           const uint32_t* n_runs,
+                                                           uint32_t* run_start, int64_t* run_key) {
+  __shared__ uint32_t s_wave[MM_BLOCK / WAVE];
+  const uint64_t base = blockIdx.x * MM_TILE;
+  uint32_t off = tile_off[blockIdx.x];
+  for (int it = 0; it < MM_STEPS; ++it) {
+    const uint64_t i = base + it * MM_BLOCK + threadIdx.x;
+    const bool h = run_head(keys, type, i, n);
+    uint32_t step = 0;
+    const uint32_t r = off + block_rank(h, s_wave, &step);
+    if (h) {
+      run_start[r] = (uint32_t)i;
+      run_key[r] = load_key(keys, type, i);
+    }
+    off += step;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) run_start[*n_runs] = (uint32_t)n;
+}
+
+__device__ __forceinline__ uint32_t run_len(const uint32_t* run_start, uint32_t r) {
+  return run_start[r + 1] - run_start[r];
+}
+
+// Per tile of pairs: the number of build rows they expand to.
+__global__ __launch_bounds__(MM_BLOCK) void k_mm_pair_count(const uint32_t* pair_run, uint64_t np,
+                                                           const uint32_t* run_start,
+                                                           uint32_t* tile_sum,
+                                                           unsigned long long* total) {
+  __shared__ unsigned long long s_wave[MM_BLOCK / WAVE];
+  const uint64_t base = blockIdx.x * MM_TILE;
+  unsigned long long c = 0;
+  for (int it = 0; it < MM_STEPS; ++it) {
+    const uint64_t j = base + it * MM_BLOCK + threadIdx.x;
+    if (j < np) c += run_len(run_start, pair_run[j]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, WAVE);
+  if (lane_id() == 0) s_wave[threadIdx.x / WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < MM_BLOCK / WAVE; ++w) t += s_wave[w];
+    tile_sum[blockIdx.x] = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+    atomicAdd(total, t);
+  }
+}
+
+// off[j] = output position of pair j's first row (exclusive prefix).
+__global__ __launch_bounds__(MM_BLOCK) void k_mm_offsets(const uint32_t* pair_run, uint64_t np,
+                                                        const uint32_t* run_start,
+                                                        const uint32_t* tile_off, uint32_t* off) {
+  __shared__ uint32_t s_wave[MM_BLOCK / WAVE];
+  const int wave = threadIdx.x / WAVE;
+  const uint64_t base = blockIdx.x * MM_TILE;
+  uint32_t acc = tile_off[blockIdx.x];
+  for (int it = 0; it < MM_STEPS; ++it) {
+    const uint64_t j = base + it * MM_BLOCK + threadIdx.x;
+    const uint32_t c = j < np ? run_len(run_start, pair_run[j]) : 0;
+    const uint32_t inc = wave_inclusive_u32(c);
+    if (lane_id() == WAVE - 1) s_wave[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, step = 0;
+#pragma unroll
+    for (int w = 0; w < MM_BLOCK / WAVE; ++w) {
+      before += w < wave ? s_wave[w] : 0;
+      step += s_wave[w];
+    }
+    __syncthreads();
+    if (j < np) off[j] = acc + before + inc - c;
+    acc += step;
+  }
+}
+
+// One thread per output row: its pair is the last j with off[j] <= t (every
+// pair expands to at least one row, so a block's MM_BLOCK rows span at most
+// MM_BLOCK pairs, found from the block's first row).
+__global__ __launch_bounds__(MM_BLOCK) void k_mm_expand(const uint32_t* pair_probe,
+                                                       const uint32_t* pair_run, uint64_t np,
+                                                       const uint32_t* off,
+                                                       const uint32_t* run_start,
+                                                       const uint32_t* sorted_rows, uint64_t total,
+                                                       uint32_t* out_probe, uint32_t* out_build) {
+  __shared__ uint64_t s_lo;
+  for (uint64_t b0 = (uint64_t)blockIdx.x * MM_BLOCK; b0 < total;
+       b0 += (uint64_t)gridDim.x * MM_BLOCK) {
+    if (threadIdx.x == 0) {
+      uint64_t lo = 0, hi = np;  // last j with off[j] <= b0
+      while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (off[mid] <= b0) lo = mid;
+        else hi = mid;
+      }
+      s_lo = lo;
+    }
+    __syncthreads();
+    const uint64_t t = b0 + threadIdx.x;
+    if (t < total) {
+      uint64_t lo = s_lo, hi = std::min<uint64_t>(np, s_lo + MM_BLOCK + 1);
+      while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t r = pair_run[lo];
+      out_probe[t] = pair_probe[lo];
+      out_build[t] = sorted_rows[run_start[r] + (uint32_t)(t - off[lo])];
+    }
+    __syncthreads();
+  }
+}
+
+int32_t join_runs(const void* sorted_keys, int32_t type, uint64_t n, uint32_t* tile_buf,
+                  uint32_t* run_start, int64_t* run_key, uint32_t* n_runs, hipStream_t st) {
+  if (n == 0 || n > 0xFFFFFFFEull) return set_error(MTL_ERR_INVALID_ARGUMENT, "join runs: %llu rows", (unsigned long long)n);
+  const uint32_t tiles = (uint32_t)((n + MM_TILE - 1) / MM_TILE);
+  hipLaunchKernelGGL(k_mm_head_count, dim3(tiles), dim3(MM_BLOCK), 0, st, sorted_keys, type, n,
+                     tile_buf);
+  MTL_HIP(hipGetLastError());
+  scan_counts(tile_buf, tiles, n_runs, st);
+  MTL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mm_head_write, dim3(tiles), dim3(MM_BLOCK), 0, st, sorted_keys, type, n,
+                     tile_buf, n_runs, run_start, run_key);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
+}
+
+int32_t join_expand_count(const uint32_t* pair_run, uint64_t np, const uint32_t* run_start,
+                          uint32_t* tile_buf, unsigned long long* total, hipStream_t st) {
+  if (np == 0) return MTL_OK;
+  const uint32_t tiles = (uint32_t)((np + MM_TILE - 1) / MM_TILE);
+  hipLaunchKernelGGL(k_mm_pair_count, dim3(tiles), dim3(MM_BLOCK), 0, st, pair_run, np,
+                     run_start, tile_buf, total);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
+}
+
+int32_t join_expand_write(const uint32_t* pair_probe, const uint32_t* pair_run, uint64_t np,
+                          const uint32_t* run_start, const uint32_t* sorted_rows,
+                          uint32_t* tile_buf, uint32_t* scan_total, uint32_t* off,
+                          uint64_t total, uint32_t* out_probe, uint32_t* out_build,
+                          hipStream_t st) {
+  if (np == 0 || total == 0) return MTL_OK;
+  const uint32_t tiles = (uint32_t)((np + MM_TILE - 1) / MM_TILE);
+  scan_counts(tile_buf, tiles, scan_total, st);
+  MTL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mm_offsets, dim3(tiles), dim3(MM_BLOCK), 0, st, pair_run, np, run_start,
+                     tile_buf, off);
+  MTL_HIP(hipGetLastError());
+  const uint64_t blocks = std::min<uint64_t>((total + MM_BLOCK - 1) / MM_BLOCK, 1u << 16);
+  hipLaunchKernelGGL(k_mm_expand, dim3((uint32_t)blocks), dim3(MM_BLOCK), 0, st, pair_probe,
+                     pair_run, np, off, run_start, sorted_rows, total, out_probe, out_build);
+  MTL_HIP(hipGetLastError());
+  return MTL_OK;
+}
+
+}  // namespace mtl

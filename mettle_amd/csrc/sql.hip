@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "filter.hpp"
+#include "join.hpp"
 #include "mtl_internal.hpp"
 #include "star.hpp"
 
@@ -823,6 +824,9 @@ struct DevBuf {
   uint64_t n = 0;
   ~DevBuf() { dev_free(p, n); }
   int32_t alloc(uint64_t bytes) {
+    if (p) dev_free(p, n);  // a buffer reused for a new result
+    p = nullptr;
+    n = 0;
     MTL_TRY(dev_alloc(bytes, &p, "SQL buffer"));
     n = bytes;
     return MTL_OK;
@@ -1708,6 +1712,76 @@ static int32_t join_pairs(const void* bkeys, int32_t btype, const uint32_t* bsel
   return MTL_OK;
 }
 
+// Many-to-many join: every (probe, build) pair of equal keys, in probe order
+// and, within a probe row, in build-selection order.  The build side is
+// sorted by key into runs; join_pairs over the runs' unique keys gives
+// (probe, run) pairs, which expand to one pair per build row of the run.
+static int32_t join_pairs_multi(const void* bkeys, int32_t btype, const uint32_t* bsel,
+                                uint64_t brows, int64_t bmin, int64_t bmax, const void* pkeys,
+                                int32_t ptype, const uint32_t* psel, uint64_t prows,
+                                DevBuf& out_p, DevBuf& out_b, uint64_t& n_pairs, mtl_stream* s) {
+  n_pairs = 0;
+  if (brows == 0 || prows == 0) {
+    MTL_TRY(out_p.alloc(4));
+    MTL_TRY(out_b.alloc(4));
+    return MTL_OK;
+  }
+  if (brows > 0xFFFFFFFEull) return set_error(MTL_UNSUPPORTED, "join build over 2^32 rows");
+  // build rows in key order (stable), and their keys
+  DevBuf srow, skeys, cnt;
+  MTL_TRY(srow.alloc(brows * 4));
+  MTL_TRY(cnt.alloc(16));
+  const uint32_t br = (uint32_t)brows;
+  MTL_TRY(mtl_memcpy_h2d(cnt.p, &br, 4, s));
+  uint32_t ns = 0;
+  MTL_TRY(mtl_sort(bkeys, btype, bsel, bsel ? (const uint32_t*)cnt.p : nullptr, brows, 0,
+                   (uint32_t*)srow.p, &ns, s));
+  if (ns != brows) return set_error(MTL_ERR_PROCESSING, "join build sort returned %u rows", ns);
+  const uint32_t w = (uint32_t)type_width(btype);
+  MTL_TRY(skeys.alloc(brows * w));
+  MTL_TRY(mtl_gather(bkeys, w, (const uint32_t*)srow.p, nullptr, brows, skeys.p, s));
+  // runs of equal keys
+  const uint64_t tiles = (brows + 4095) / 4096 + 1;
+  DevBuf tb, rstart, rkey;
+  MTL_TRY(tb.alloc(tiles * 4));
+  MTL_TRY(rstart.alloc((brows + 1) * 4));
+  MTL_TRY(rkey.alloc(brows * 8));
+  MTL_TRY(join_runs(skeys.p, btype, brows, (uint32_t*)tb.p, (uint32_t*)rstart.p,
+                    (int64_t*)rkey.p, (uint32_t*)cnt.p, s->stream));
+  uint32_t runs = 0;
+  MTL_TRY(mtl_memcpy_d2h(&runs, cnt.p, 4, s));
+  if (runs == 0 || runs > brows) return set_error(MTL_ERR_PROCESSING, "join runs: %u", runs);
+  // (probe, run) pairs over the unique run keys
+  DevBuf pp, pr;
+  uint64_t np = 0;
+  MTL_TRY(join_pairs(rkey.p, MTL_BIGINT, nullptr, runs, bmin, bmax, pkeys, ptype, psel, prows, pp,
+                     pr, np, s));
+  // expand each pair to its run's rows
+  const uint64_t ptiles = (np + 4095) / 4096 + 1;
+  DevBuf ptb, total;
+  MTL_TRY(ptb.alloc(ptiles * 4));
+  MTL_TRY(total.alloc(16));
+  const uint64_t zero[2] = {0, 0};
+  MTL_TRY(mtl_memcpy_h2d(total.p, zero, 16, s));
+  MTL_TRY(join_expand_count((const uint32_t*)pr.p, np, (const uint32_t*)rstart.p,
+                            (uint32_t*)ptb.p, (unsigned long long*)total.p, s->stream));
+  uint64_t t = 0;
+  MTL_TRY(mtl_memcpy_d2h(&t, total.p, 8, s));
+  if (t > 0xFFFFFFFFull)
+    return set_error(MTL_UNSUPPORTED, "join result of %llu rows exceeds 2^32-1",
+                     (unsigned long long)t);
+  MTL_TRY(out_p.alloc(std::max<uint64_t>(t, 1) * 4));
+  MTL_TRY(out_b.alloc(std::max<uint64_t>(t, 1) * 4));
+  DevBuf off;
+  MTL_TRY(off.alloc(std::max<uint64_t>(np, 1) * 4));
+  MTL_TRY(join_expand_write((const uint32_t*)pp.p, (const uint32_t*)pr.p, np,
+                            (const uint32_t*)rstart.p, (const uint32_t*)srow.p, (uint32_t*)ptb.p,
+                            (uint32_t*)total.p + 2, (uint32_t*)off.p, t, (uint32_t*)out_p.p,
+                            (uint32_t*)out_b.p, s->stream));
+  n_pairs = t;
+  return MTL_OK;
+}
+
 // dst[i] = src[idx[i]] for i < n (uint32 row ids).
 static int32_t gather_ids(const void* src, const void* idx, uint64_t n, DevBuf& dst,
                           mtl_stream* s) {
@@ -2051,14 +2125,17 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
       DevBuf pp2, pb2;
       st = join_pairs(kx.p, xtype, nullptr, R.m, xmin, xmax, ycol, ytype,
                       (const uint32_t*)sel[c]->p, nsel[c], pp2, pb2, np, s);
-      if (st == MTL_UNSUPPORTED)
-        return set_error(MTL_UNSUPPORTED, "join of %s and %s: keys repeat on both sides",
-                         xa->name.c_str(), yc->name.c_str());
-      MTL_TRY(st);
-      std::swap(pp.p, pb2.p);
-      std::swap(pp.n, pb2.n);
-      std::swap(pb.p, pp2.p);
-      std::swap(pb.n, pp2.n);
+      if (st == MTL_UNSUPPORTED) {
+        // keys repeat on both sides: many-to-many, the new table sorted into runs
+        st = join_pairs_multi(ycol, ytype, (const uint32_t*)sel[c]->p, nsel[c], ymin, ymax, kx.p,
+                              xtype, nullptr, R.m, pp, pb, np, s);
+      } else {
+        MTL_TRY(st);
+        std::swap(pp.p, pb2.p);
+        std::swap(pp.n, pb2.n);
+        std::swap(pb.p, pp2.p);
+        std::swap(pb.n, pp2.n);
+      }
     }
     MTL_TRY(st);
     // pp: relation positions, pb: rows of table c (in the order of the

@@ -2,8 +2,8 @@
 aggregates, a two-table join with a projection, and joins grouped by
 non-key columns (tests/extension.rs:239-276, 279-317, 475-525, 973-1016,
 expected values as written there), then larger random joins checked
-against numpy/pandas (merge, groupby) on the host, and the shapes that fall
-back (keys repeating on both sides of a join)."""
+against numpy/pandas (merge, groupby) on the host, and many-to-many joins
+(keys repeating on both sides)."""
 import numpy as np
 import pytest
 
@@ -181,13 +181,75 @@ def test_ungrouped_aggregates_over_a_join(ops):
         ([len(j)], [int(j.l_qty.sum())], [1])
 
 
-def test_join_with_keys_repeating_on_both_sides_falls_back(ops):
-    _table(ops, "a", [("k", abi.INTEGER, np.array([1, 1, 2], np.int32)),
-                      ("v", abi.INTEGER, np.array([5, 6, 7], np.int32))])
-    _table(ops, "b", [("k", abi.INTEGER, np.array([1, 2, 2], np.int32)),
-                      ("w", abi.INTEGER, np.array([8, 9, 10], np.int32))])
-    with pytest.raises(abi.Unsupported):
-        S.execute("SELECT a.v, b.w FROM a JOIN b ON a.k = b.k")
+def test_many_to_many_join_small(ops):
+    """Keys repeating on both sides: every matching pair (DuckDB's inner
+    join), not the reference's overwritten build keys (kernels.metal:202-206)."""
+    _table(ops, "a", [("k", abi.INTEGER, np.array([1, 1, 2, 3], np.int32)),
+                      ("v", abi.INTEGER, np.array([5, 6, 7, 4], np.int32))])
+    _table(ops, "b", [("k", abi.INTEGER, np.array([1, 2, 2, 1, 9], np.int32)),
+                      ("w", abi.INTEGER, np.array([8, 9, 10, 11, 12], np.int32))])
+    out = S.execute("SELECT a.v, b.w FROM a JOIN b ON a.k = b.k")
+    got = sorted(zip(_ints(out, "v"), _ints(out, "w")))
+    assert got == [(5, 8), (5, 11), (6, 8), (6, 11), (7, 9), (7, 10)]
+    out = S.execute("SELECT a.k, COUNT(*) AS n, SUM(b.w) AS s FROM a JOIN b ON a.k = b.k "
+                    "GROUP BY a.k ORDER BY a.k")
+    assert _ints(out, "k") == [1, 2]
+    assert _ints(out, "n") == [4, 2]
+    assert _ints(out, "s") == [38, 19]
+
+
+@pytest.mark.parametrize("seed,na,nb,dom", [(1, 50_000, 60_000, 20_000),
+                                            (2, 200_000, 3_000, 1_000),
+                                            (3, 5_000, 400_000, 70)])
+def test_many_to_many_join_random(ops, seed, na, nb, dom):
+    """Random many-to-many joins with heavy duplication on both sides,
+    filtered, projected and grouped, against pandas merge."""
+    rng = np.random.default_rng(seed)
+    ak = (rng.integers(0, dom, na) * 7 - 1000).astype(np.int64)
+    av = rng.integers(-10**6, 10**6, na).astype(np.int64)
+    bk = (rng.integers(0, dom, nb) * 7 - 1000).astype(np.int64)
+    bw = rng.integers(0, 100, nb).astype(np.int32)
+    _table(ops, "a", [("k", abi.BIGINT, ak), ("v", abi.BIGINT, av)])
+    _table(ops, "b", [("k", abi.BIGINT, bk), ("w", abi.INTEGER, bw)])
+    A = pd.DataFrame({"k": ak, "v": av})
+    B = pd.DataFrame({"k": bk, "w": bw})
+    j = A[A.v > 0].merge(B[B.w < 30], on="k")
+    out = S.execute("SELECT b.w, COUNT(*) AS n, SUM(a.v) AS s, MIN(a.v) AS lo FROM a "
+                    "JOIN b ON a.k = b.k WHERE a.v > 0 AND b.w < 30 GROUP BY b.w ORDER BY b.w")
+    g = j.groupby("w").v.agg(["size", "sum", "min"])
+    assert _ints(out, "w") == list(g.index)
+    assert _ints(out, "n") == [int(x) for x in g["size"]]
+    assert _ints(out, "s") == [int(x) for x in g["sum"]]
+    assert _ints(out, "lo") == [int(x) for x in g["min"]]
+    if len(j) < 3_000_000:
+        out = S.execute("SELECT a.v, b.w FROM a JOIN b ON a.k = b.k WHERE a.v > 0 AND b.w < 30")
+        got = np.stack([np.asarray(_ints(out, "v")), np.asarray(_ints(out, "w"))], 1)
+        exp = j[["v", "w"]].to_numpy()
+        assert len(got) == len(exp)
+        got = got[np.lexsort(got.T[::-1])]
+        exp = exp[np.lexsort(exp.T[::-1])]
+        np.testing.assert_array_equal(got, exp)
+
+
+def test_many_to_many_three_tables(ops):
+    """A chain a-b-c where both joins repeat keys on both sides."""
+    rng = np.random.default_rng(21)
+    a = pd.DataFrame({"x": rng.integers(0, 300, 4000).astype(np.int32),
+                      "v": rng.integers(0, 1000, 4000).astype(np.int64)})
+    b = pd.DataFrame({"x": rng.integers(0, 300, 3000).astype(np.int32),
+                      "y": rng.integers(0, 50, 3000).astype(np.int32)})
+    c = pd.DataFrame({"y": rng.integers(0, 50, 2000).astype(np.int32),
+                      "g": rng.integers(0, 9, 2000).astype(np.int32)})
+    _table(ops, "a", [("x", abi.INTEGER, a.x.to_numpy()), ("v", abi.BIGINT, a.v.to_numpy())])
+    _table(ops, "b", [("x", abi.INTEGER, b.x.to_numpy()), ("y", abi.INTEGER, b.y.to_numpy())])
+    _table(ops, "c", [("y", abi.INTEGER, c.y.to_numpy()), ("g", abi.INTEGER, c.g.to_numpy())])
+    out = S.execute("SELECT c.g, COUNT(*) AS n, SUM(a.v) AS s FROM a JOIN b ON a.x = b.x "
+                    "JOIN c ON b.y = c.y WHERE a.v < 500 GROUP BY c.g ORDER BY c.g")
+    j = a[a.v < 500].merge(b, on="x").merge(c, on="y")
+    g = j.groupby("g").v.agg(["size", "sum"])
+    assert _ints(out, "g") == list(g.index)
+    assert _ints(out, "n") == [int(x) for x in g["size"]]
+    assert _ints(out, "s") == [int(x) for x in g["sum"]]
 
 
 def test_or_and_in_predicates(ops):
