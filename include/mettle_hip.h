@@ -481,7 +481,20 @@ int32_t mtl_catalog_add_column(const char *table, const char *column,
 /* Dictionary of a MTL_DICT8/MTL_DICT32 column: code i = values[i]. */
 int32_t mtl_catalog_set_dictionary(const char *table, const char *column,
                                    const char *const *values, uint32_t n);
-/* Drop one table, or every table when table == NULL. */
+/* NULLs of a column: valid = device bytes, one per row, 1 = valid, 0 = NULL
+ * (NULL: the column has no NULLs).  The value bytes of a NULL row must be
+ * zero and min/max must bound the valid values (mtl_catalog_ingest_arrow
+ * writes columns so).  Filters, joins, projections, GROUP BY and the
+ * aggregates follow DuckDB's NULL semantics: a comparison with NULL is not
+ * true, join keys that are NULL match nothing, aggregates skip NULLs (SUM,
+ * MIN, MAX, AVG of no valid value are NULL, COUNT(col) counts valid values),
+ * NULL keys form one group, ORDER BY puts NULLs last.  The reference pins no
+ * column with NULLs (pinned.rs:295-413 rejects them), so its queries fall
+ * back to DuckDB's CPU path. */
+int32_t mtl_catalog_set_validity(const char *table, const char *column,
+                                 const uint8_t *valid);
+/* Drop one table, or every table when table == NULL (an ingested table's
+ * cached columns are released). */
 int32_t mtl_catalog_drop(const char *table);
 /* Plan without running: buf receives the shape ("scan", "group", "star2",
  * "star3") or, on MTL_UNSUPPORTED, the reason. */
@@ -494,6 +507,10 @@ int32_t mtl_result_shape(const mtl_result *r, uint32_t *n_cols,
  * MTL_DECIMAL with `scale`), and n_rows values of that type's width. */
 int32_t mtl_result_column(const mtl_result *r, uint32_t i, const char **name,
                           int32_t *type, int32_t *scale, const void **data);
+/* NULLs of result column i: one byte per row (1 = valid), or *valid = NULL
+ * when the column has none.  The value bytes of a NULL row are unspecified. */
+int32_t mtl_result_validity(const mtl_result *r, uint32_t i,
+                            const uint8_t **valid);
 int32_t mtl_result_free(mtl_result *r);
 
 /* Arrow C data interface structs, as the specification defines them
@@ -528,12 +545,38 @@ struct ArrowArray {
  * child per column, in select order.  Types: INTEGER int32, BIGINT int64,
  * FLOAT float32, DOUBLE float64, DATE date32, TIMESTAMP timestamp[us],
  * DECIMAL decimal128(18, scale), HUGEINT sums decimal128(38, scale),
- * dictionary columns dictionary<uint8 | int32, utf8>.  No NULLs.  The
+ * dictionary columns dictionary<uint8 | int32, utf8>.  NULLs are validity
+ * bitmaps (the column's schema then carries ARROW_FLAG_NULLABLE).  The
  * caller owns both structs (release callbacks); they stay valid after
  * mtl_result_free.  Replaces the RecordBatch hand-back of gpu_processing
  * (processing.rs:77-96), whose computed columns were all Float64. */
 int32_t mtl_result_export_arrow(const mtl_result *r, struct ArrowSchema *schema,
                                 struct ArrowArray *array);
+
+/* gpu_register_table (pinned.rs:187-453): register `table` from Arrow
+ * record batches (the C data interface: schema is a struct "+s", one child
+ * per column; every batch matches it).  Each column is converted on the
+ * host and uploaded through the HBM column cache (mtl_cache_put, so
+ * gpu_buffer_init comes first); the catalog keeps the cache references
+ * until mtl_catalog_drop or the next ingestion of the table.  Types:
+ * int8/16/32 and uint8/16 -> INTEGER; int64, uint32, uint64 -> BIGINT;
+ * float32/64 -> FLOAT/DOUBLE; date32/date64 -> DATE; timestamp s/ms/us ->
+ * TIMESTAMP (us); decimal128(p, s <= 18) -> DECIMAL when every value fits
+ * int64; utf8, large_utf8 and dictionary<int, utf8> -> a dictionary column
+ * whose sorted dictionary orders codes like the strings (DICT8 up to 256
+ * values, else DICT32).  NULLs become validity bytes (see
+ * mtl_catalog_set_validity).  Columns of other types are skipped, as the
+ * reference skips them, and listed comma-separated in `skipped`.
+ * unique_cols: key columns the caller declares unique (primary keys).
+ * The caller keeps ownership of schema and batches (nothing is released). */
+int32_t mtl_catalog_ingest_arrow(const char *table,
+                                 const struct ArrowSchema *schema,
+                                 struct ArrowArray *const *batches,
+                                 uint32_t n_batches,
+                                 const char *const *unique_cols,
+                                 uint32_t n_unique, uint64_t *out_rows,
+                                 uint64_t *out_bytes, char *skipped,
+                                 size_t skipped_len);
 
 /* gpu_rewrite_query(sql): the query printed back with every predicate
  * `col op literal` written as gpu_{eq,lt,le,gt,ge}_scalar(col, literal) and

@@ -249,12 +249,14 @@ def lib() -> C.CDLL:
         "mtl_catalog_add_table": (i32, [C.c_char_p, u64]),
         "mtl_catalog_add_column": (i32, [C.c_char_p, C.c_char_p, i32, i32, vp, i64, i64, i32]),
         "mtl_catalog_set_dictionary": (i32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), u32]),
+        "mtl_catalog_set_validity": (i32, [C.c_char_p, C.c_char_p, vp]),
         "mtl_catalog_drop": (i32, [C.c_char_p]),
         "mtl_sql_explain": (i32, [C.c_char_p, C.c_char_p, C.c_size_t]),
         "mtl_sql_execute": (i32, [C.c_char_p, C.POINTER(vp), vp]),
         "mtl_result_shape": (i32, [vp, C.POINTER(u32), C.POINTER(u64)]),
         "mtl_result_column": (i32, [vp, u32, C.POINTER(C.c_char_p), C.POINTER(i32),
                                     C.POINTER(i32), C.POINTER(vp)]),
+        "mtl_result_validity": (i32, [vp, u32, C.POINTER(vp)]),
         "mtl_result_free": (i32, [vp]),
         "mtl_rewrite_query": (i32, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "mtl_scalar_hook": (i32, [i32, i32, vp, u64, i64, vp]),
@@ -268,6 +270,9 @@ def lib() -> C.CDLL:
         "mtl_profile_enable": (i32, [i32]),
         "mtl_profile_reset": (i32, []),
         "mtl_result_export_arrow": (i32, [vp, vp, vp]),
+        "mtl_catalog_ingest_arrow": (i32, [C.c_char_p, vp, C.POINTER(vp), u32,
+                                           C.POINTER(C.c_char_p), u32, C.POINTER(u64),
+                                           C.POINTER(u64), C.c_char_p, C.c_size_t]),
         "mtl_tune_get": (i32, [C.c_char_p, C.POINTER(i32)]),
         "mtl_tune_set": (i32, [C.c_char_p, i32]),
         "mtl_tune_reset": (i32, []),

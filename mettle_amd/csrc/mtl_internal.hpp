@@ -116,6 +116,11 @@ Runtime& runtime();
 int32_t ensure_device();
 // Resolve NULL to the default stream.
 int32_t resolve_stream(mtl_stream* s, mtl_stream** out);
+// Catalog ingestion (ingest.hip): release the cache references an ingested
+// table holds (every table when table == NULL); the catalog entry itself is
+// dropped by catalog_drop_table (sql.hip), which releases nothing.
+void ingest_release(const char* table);
+void catalog_drop_table(const char* table);
 // Pinned staging for small synchronous copies.
 int32_t stream_pinned(mtl_stream* s, size_t bytes, void** out);
 // Wait for a stream's work by polling it: a blocking synchronise can return

@@ -49,6 +49,7 @@ struct mtl_result {
     int32_t scale = 0;  // DECIMAL scale
     std::vector<uint8_t> data;
     std::vector<std::string> dict;  // dictionary columns: the value of each code
+    std::vector<uint8_t> valid;     // one byte per row, 1 = valid; empty: no NULLs
   };
   std::vector<Column> cols;
   uint64_t rows = 0;
@@ -173,6 +174,7 @@ struct Pred {
   int op;  // MTL_CMP_*
   Lit lit;
   bool or_next = false;  // OR-ed with the next predicate (IN lists, OR runs)
+  int nulls = 0;         // 1: col IS NULL, 2: col IS NOT NULL (op and lit unused)
 };
 
 struct JoinEq {
@@ -232,7 +234,8 @@ struct Parser {
     static const char* kw[] = {"select", "from", "where", "group", "by", "order", "limit",
                                "and", "or", "not", "as", "join", "inner", "on", "asc", "desc",
                                "between", "having", "distinct", "union", "left", "right",
-                               "full", "outer", "cross", "with", "offset", "in", nullptr};
+                               "full", "outer", "cross", "with", "offset", "in", "is", "null",
+                               nullptr};
     for (int i = 0; kw[i]; ++i)
       if (s == kw[i]) return true;
     return false;
@@ -427,6 +430,15 @@ struct Parser {
     }
     ColRef c;
     if (!colref(c)) return false;
+    if (accept_kw("is")) {
+      Pred pr;
+      pr.col = c;
+      pr.op = MTL_CMP_EQ;
+      pr.nulls = accept_kw("not") ? 2 : 1;
+      if (!expect_kw("null")) return false;
+      q.preds.push_back(pr);
+      return true;
+    }
     if (is_kw("not")) return fail("NOT is not supported");
     if (accept_kw("in")) {
       // col IN (v1, ..., vk): k equalities OR-ed together
@@ -657,6 +669,9 @@ struct CatCol {
   int64_t min = 0, max = 0;
   bool unique = false;
   std::vector<std::string> dict;  // dictionary columns: code -> value
+  // NULLs: one byte per row, 1 = valid (null: no NULLs).  The value bytes of
+  // a NULL row are zero, and min/max bound the valid values.
+  const uint8_t* valid = nullptr;
 };
 struct CatTable {
   std::string name;
@@ -719,7 +734,11 @@ static bool pow10_scale(int64_t v, int sc, int64_t& out) {
 
 // Literal -> the int64 bits mtl_pred expects for the column.  Integer
 // literals compare exactly (DuckDB widens, never truncates).
-static bool literal_bits(const Lit& l, const CatCol& c, int op, int64_t& bits, std::string& err) {
+// The literal of `col op lit` in the column's representation.  A string
+// compared with a dictionary column is its code; with a sorted dictionary
+// (codes order like the strings) <, <=, >, >= become code comparisons and
+// op may change (s <= 'x' -> code < first code above 'x').
+static bool literal_bits(const Lit& l, const CatCol& c, int& op, int64_t& bits, std::string& err) {
   switch (c.type) {
     case MTL_FLOAT:
     case MTL_DOUBLE: {
@@ -759,14 +778,25 @@ static bool literal_bits(const Lit& l, const CatCol& c, int op, int64_t& bits, s
     case MTL_DICT8:
     case MTL_DICT32:
       if (l.k != Lit::STR) return err = "dictionary column compared with a non-string", false;
-      if (op != MTL_CMP_EQ) return err = "only '=' on dictionary columns", false;
+      if (op != MTL_CMP_EQ) {
+        if (!std::is_sorted(c.dict.begin(), c.dict.end()))
+          return err = "only '=' on a dictionary column whose dictionary is not sorted", false;
+        const int64_t lb = std::lower_bound(c.dict.begin(), c.dict.end(), l.s) - c.dict.begin();
+        const int64_t ub = std::upper_bound(c.dict.begin(), c.dict.end(), l.s) - c.dict.begin();
+        switch (op) {
+          case MTL_CMP_LT: bits = lb; break;                     // code < lb
+          case MTL_CMP_LE: op = MTL_CMP_LT; bits = ub; break;    // code < ub
+          case MTL_CMP_GT: op = MTL_CMP_GE; bits = ub; break;    // code >= ub
+          default: bits = lb; break;                             // GE: code >= lb
+        }
+        return true;
+      }
       for (size_t i = 0; i < c.dict.size(); ++i)
         if (c.dict[i] == l.s) {
           bits = (int64_t)i;
           return true;
         }
-      bits = (int64_t)c.dict.size();  // matches no row
-      if (c.type == MTL_DICT8 && bits > 255) bits = 255;
+      bits = -1;  // matches no code (literals compare exactly as integers)
       return true;
     default:  // INTEGER, BIGINT
       if (l.k == Lit::INT) {
@@ -783,30 +813,102 @@ static bool literal_bits(const Lit& l, const CatCol& c, int op, int64_t& bits, s
 // The WHERE predicates bound per source: preds[src] with or_next[src] (bit i:
 // predicate i is OR-ed with predicate i + 1).  A run of OR-ed predicates must
 // stay within one table.
+// A predicate no row passes: col < the smallest value of its type.
+static mtl_pred never_pred(const CatCol& c) {
+  mtl_pred mp{};
+  mp.column = c.dev;
+  mp.type = c.type;
+  mp.op = MTL_CMP_LT;
+  if (c.type == MTL_FLOAT || c.type == MTL_DOUBLE) {
+    const double d = -HUGE_VAL;
+    std::memcpy(&mp.scalar, &d, 8);
+  } else if (c.type == MTL_DICT8 || c.type == MTL_DICT32) {
+    mp.scalar = 0;
+  } else {
+    mp.scalar = INT64_MIN;
+  }
+  return mp;
+}
+
+// valid = 1 (or 0): the rows where a column with NULLs is not NULL (is NULL).
+static mtl_pred valid_pred(const CatCol& c, int64_t v = 1) {
+  mtl_pred mp{};
+  mp.column = c.valid;
+  mp.type = MTL_DICT8;
+  mp.op = MTL_CMP_EQ;
+  mp.scalar = v;
+  return mp;
+}
+
+// WHERE -> per-table predicate lists and OR masks.  NULL semantics: a
+// comparison with NULL is not true, so a run of OR-ed comparisons on one
+// column with NULLs is AND-ed with that column's valid = 1 (a run over
+// several columns, any with NULLs, would need OR of ANDs: refused);
+// IS [NOT] NULL tests the valid bytes.
 static int32_t bind_preds(const Query& q, Bound& b, std::vector<std::vector<mtl_pred>>& preds,
                           std::vector<uint32_t>& or_next) {
   preds.assign(b.tabs.size(), {});
   or_next.assign(b.tabs.size(), 0u);
+  auto push = [&](int src, const mtl_pred& mp, bool ored) -> int32_t {
+    if (preds[src].size() >= MTL_MAX_PREDS)
+      return set_error(MTL_UNSUPPORTED, "more than %d predicates on a table", MTL_MAX_PREDS);
+    if (ored) or_next[src] |= 1u << preds[src].size();
+    preds[src].push_back(mp);
+    return MTL_OK;
+  };
   int run_src = -1;
+  const CatCol* run_col = nullptr;
+  bool run_mixed = false, run_null = false;
   for (auto& pr : q.preds) {
     int src;
     const CatCol* c;
     if (!b.resolve(pr.col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
     if (run_src >= 0 && src != run_src)
       return set_error(MTL_UNSUPPORTED, "OR across tables");
+    if (pr.nulls) {
+      if (run_src >= 0 || pr.or_next) return set_error(MTL_UNSUPPORTED, "IS NULL inside an OR");
+      if (c->valid) MTL_TRY(push(src, valid_pred(*c, pr.nulls == 2 ? 1 : 0), false));
+      else if (pr.nulls == 1) MTL_TRY(push(src, never_pred(*c), false));
+      continue;  // IS NOT NULL of a column without NULLs holds for every row
+    }
     mtl_pred mp{};
     mp.column = c->dev;
     mp.type = c->type;
     mp.op = pr.op;
     std::string err;
-    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
+    if (!literal_bits(pr.lit, *c, mp.op, mp.scalar, err))
       return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
-    if (preds[src].size() >= MTL_MAX_PREDS)
-      return set_error(MTL_UNSUPPORTED, "more than %d predicates on a table", MTL_MAX_PREDS);
-    if (pr.or_next) or_next[src] |= 1u << preds[src].size();
-    preds[src].push_back(mp);
-    run_src = pr.or_next ? src : -1;
+    MTL_TRY(push(src, mp, pr.or_next));
+    if (run_src < 0) {
+      run_col = c;
+      run_mixed = false;
+      run_null = false;
+    } else if (c != run_col) {
+      run_mixed = true;
+    }
+    run_null |= c->valid != nullptr;
+    if (pr.or_next) {
+      run_src = src;
+      continue;
+    }
+    if (run_null) {
+      if (run_mixed) return set_error(MTL_UNSUPPORTED, "OR over several columns with NULLs");
+      MTL_TRY(push(src, valid_pred(*run_col), false));
+    }
+    run_src = -1;
   }
+  return MTL_OK;
+}
+
+// Key columns with NULLs: NULL keys match nothing (joins).
+static int32_t add_not_null(std::vector<std::vector<mtl_pred>>& preds, int src, const CatCol& c) {
+  if (!c.valid) return MTL_OK;
+  for (auto& p : preds[src])
+    if (p.column == c.valid && p.type == MTL_DICT8 && p.op == MTL_CMP_EQ && p.scalar == 1)
+      return MTL_OK;
+  if (preds[src].size() >= MTL_MAX_PREDS)
+    return set_error(MTL_UNSUPPORTED, "more than %d predicates on a table", MTL_MAX_PREDS);
+  preds[src].push_back(valid_pred(c));
   return MTL_OK;
 }
 
@@ -946,8 +1048,9 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
   }
   res.shape = "scan";
   // selection, count, sort permutation and one output column at a time
-  res.scratch = T.rows * (4 + 4 + 8 + 8) + (1u << 20);
+  res.scratch = T.rows * (4 + 4 + 8 + 8) + (okey && okey->valid ? T.rows * 8 : 0) + (1u << 20);
   if (dry) return MTL_OK;
+  MTL_TRY(resolve_stream(s, &s));  // NULL: the library's default stream
   // filter
   DevBuf sel, cnt, perm, out, ovf;
   MTL_TRY(sel.alloc(std::max<uint64_t>(T.rows, 1) * 4));
@@ -956,7 +1059,31 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
                         (uint32_t*)sel.p, (uint32_t*)cnt.p, s));
   uint32_t n = 0;
   const uint32_t* rows = (const uint32_t*)sel.p;
-  if (okey) {
+  if (okey && okey->valid) {
+    // NULLs last (DuckDB's default order): the valid rows sorted, then the
+    // NULL rows in selection order
+    DevBuf vsel, nsel, vcnt;
+    MTL_TRY(vsel.alloc(std::max<uint64_t>(T.rows, 1) * 4));
+    MTL_TRY(nsel.alloc(std::max<uint64_t>(T.rows, 1) * 4));
+    MTL_TRY(vcnt.alloc(16));
+    const mtl_pred pv = valid_pred(*okey, 1), pn = valid_pred(*okey, 0);
+    MTL_TRY(mtl_filter(&pv, 1, T.rows, (const uint32_t*)sel.p, (const uint32_t*)cnt.p, T.rows,
+                       (uint32_t*)vsel.p, (uint32_t*)vcnt.p, s));
+    MTL_TRY(mtl_filter(&pn, 1, T.rows, (const uint32_t*)sel.p, (const uint32_t*)cnt.p, T.rows,
+                       (uint32_t*)nsel.p, (uint32_t*)vcnt.p + 1, s));
+    uint32_t c2[2] = {0, 0};
+    MTL_TRY(mtl_memcpy_d2h(c2, vcnt.p, 8, s));
+    if (c2[0] > T.rows || c2[1] > T.rows - c2[0])
+      return set_error(MTL_ERR_PROCESSING, "device scan failed");
+    MTL_TRY(perm.alloc(std::max<uint64_t>(T.rows, 1) * 4));
+    MTL_TRY(mtl_sort(okey->dev, okey->type, (const uint32_t*)vsel.p, (const uint32_t*)vcnt.p,
+                     T.rows, odesc ? 1 : 0, (uint32_t*)perm.p, &n, s));
+    if (c2[1])
+      MTL_HIP(hipMemcpyAsync((uint32_t*)perm.p + n, nsel.p, (uint64_t)c2[1] * 4,
+                             hipMemcpyDeviceToDevice, s->stream));
+    n += c2[1];
+    rows = (const uint32_t*)perm.p;
+  } else if (okey) {
     MTL_TRY(perm.alloc(std::max<uint64_t>(T.rows, 1) * 4));
     MTL_TRY(mtl_sort(okey->dev, okey->type, (const uint32_t*)sel.p, (const uint32_t*)cnt.p,
                      T.rows, odesc ? 1 : 0, (uint32_t*)perm.p, &n, s));
@@ -990,6 +1117,10 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
                                  o.c->name.c_str());
     }
     MTL_TRY(download(col.data, out.p, (uint64_t)n * w, s));
+    if (o.c->valid) {  // a projection of NULL is NULL
+      if (n) MTL_TRY(mtl_gather(o.c->valid, 1, rows, nullptr, n, out.p, s));
+      MTL_TRY(download(col.valid, out.p, n, s));
+    }
     res.cols.push_back(std::move(col));
   }
   return MTL_OK;
@@ -999,6 +1130,10 @@ static int32_t run_scan(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
 struct GroupOut {
   std::string name;
   int key = -1, agg = -1;
+  // NULLs: nn = the hidden aggregate counting the valid values (-1: none);
+  // mode 1 = COUNT(col) (the engine sums an int32 0/1 column), 2 = AVG
+  // computed on the host from the engine's SUM and that count
+  int nn = -1, mode = 0;
 };
 // How emit_groups reads the group table back: top-K by one aggregate
 // (order_agg >= 0, k = LIMIT), or every group in the table's order
@@ -1014,7 +1149,21 @@ struct GroupEmit {
   int key_order;
   // ORDER BY the aggregate, then these group keys (index, descending)
   std::vector<std::pair<int, bool>> then = {};
+  // per group key: the value standing for NULL (INT64_MIN: the key has none)
+  std::vector<int64_t> key_null = {};
 };
+
+// DuckDB's result type of an aggregate: COUNT BIGINT, integer SUM HUGEINT,
+// AVG and float aggregates DOUBLE, DECIMAL SUM/MIN/MAX keeping the scale.
+static void agg_out_type(int32_t f, int32_t t, int32_t scale, int32_t& type, int32_t& out_scale) {
+  if (f == MTL_AGG_COUNT) type = MTL_BIGINT;
+  else if (f == MTL_AGG_SUM && is_int_type(t)) type = MTL_HUGEINT;
+  else if (f == MTL_AGG_AVG || t == MTL_FLOAT || t == MTL_DOUBLE) type = MTL_DOUBLE;
+  else type = MTL_BIGINT;
+  out_scale = 0;
+  if ((f == MTL_AGG_SUM || f == MTL_AGG_MIN || f == MTL_AGG_MAX) && t == MTL_DECIMAL)
+    out_scale = scale;  // AVG is a DOUBLE: the group engine divided by 10^scale
+}
 
 // Three-way comparison of aggregate a between groups i and j of host
 // buffers in mtl_groups_fetch layout.
@@ -1172,6 +1321,12 @@ static int32_t emit_groups(const Query& q, mtl_group_desc& d, const GroupEmit& g
     if (q.limit >= 0 && (uint64_t)q.limit < ng) ng = (uint64_t)q.limit;
   }
   res.rows = ng;
+  // valid values per group of a NULL-skipping aggregate (its hidden count)
+  auto nn_count = [&](int a, uint64_t r) -> int64_t {
+    uint64_t lo;
+    std::memcpy(&lo, abuf[a].data() + r * agg_width(a), 8);
+    return (int64_t)lo;
+  };
   for (auto& o : outs) {
     mtl_result::Column col;
     col.name = o.name;
@@ -1182,21 +1337,63 @@ static int32_t emit_groups(const Query& q, mtl_group_desc& d, const GroupEmit& g
       col.dict = c->dict;
       const int w = type_width(c->type);
       col.data.resize(ng * w);
+      const int64_t null_key =
+          (size_t)o.key < ge.key_null.size() ? ge.key_null[o.key] : INT64_MIN;
+      if (null_key != INT64_MIN) col.valid.assign(ng, 1);
       for (uint64_t r = 0; r < ng; ++r) {
-        const int64_t v = kbuf[r * d.n_keys + o.key];
+        int64_t v = kbuf[r * d.n_keys + o.key];
+        if (null_key != INT64_MIN && v == null_key) {
+          col.valid[r] = 0;
+          v = 0;
+        }
         std::memcpy(col.data.data() + r * w, &v, w);  // little-endian narrowing
+      }
+    } else if (o.mode == 1) {  // COUNT(col): the engine's SUM of 0/1
+      col.type = MTL_BIGINT;
+      col.data.resize(ng * 8);
+      for (uint64_t r = 0; r < ng; ++r) {
+        const int64_t v = nn_count(o.agg, r);
+        std::memcpy(col.data.data() + r * 8, &v, 8);
+      }
+    } else if (o.mode == 2) {  // AVG skipping NULLs: SUM / COUNT(col)
+      col.type = MTL_DOUBLE;
+      col.data.resize(ng * 8);
+      col.valid.assign(ng, 1);
+      const int32_t t = ge.agg_type[o.agg];
+      const uint64_t w = agg_width(o.agg);
+      for (uint64_t r = 0; r < ng; ++r) {
+        const int64_t cnt = nn_count(o.nn, r);
+        double v = 0;
+        if (cnt == 0) {
+          col.valid[r] = 0;
+        } else if (w == 16) {
+          // as the group engine's integer / DECIMAL AVG: the 128-bit sum over
+          // count x 10^scale in long double
+          uint64_t lo;
+          int64_t hi;
+          std::memcpy(&lo, abuf[o.agg].data() + r * 16, 8);
+          std::memcpy(&hi, abuf[o.agg].data() + r * 16 + 8, 8);
+          const __int128 sum = (__int128)(((unsigned __int128)(uint64_t)hi << 64) | lo);
+          long double div = (long double)cnt;
+          if (t == MTL_DECIMAL)
+            for (int k = 0; k < ge.agg_scale[o.agg]; ++k) div *= 10.0L;
+          v = (double)((long double)sum / div);
+        } else {
+          double sum;
+          std::memcpy(&sum, abuf[o.agg].data() + r * 8, 8);
+          v = sum / (double)cnt;
+        }
+        std::memcpy(col.data.data() + r * 8, &v, 8);
       }
     } else {
       const int32_t f = ge.agg_func[o.agg], t = ge.agg_type[o.agg];
       const uint64_t w = agg_width(o.agg);
-      if (f == MTL_AGG_COUNT) col.type = MTL_BIGINT;
-      else if (f == MTL_AGG_SUM && is_int_type(t)) col.type = MTL_HUGEINT;
-      else if (f == MTL_AGG_AVG || t == MTL_FLOAT || t == MTL_DOUBLE) col.type = MTL_DOUBLE;
-      else col.type = MTL_BIGINT;
-      if ((f == MTL_AGG_SUM || f == MTL_AGG_MIN || f == MTL_AGG_MAX) && t == MTL_DECIMAL)
-        col.scale = ge.agg_scale[o.agg];
-      if (f == MTL_AGG_AVG) col.scale = 0;  // a DOUBLE: the group engine divided by 10^scale
+      agg_out_type(f, t, ge.agg_scale[o.agg], col.type, col.scale);
       col.data.assign(abuf[o.agg].begin(), abuf[o.agg].begin() + ng * w);
+      if (o.nn >= 0) {  // NULL when the group has no valid value
+        col.valid.assign(ng, 1);
+        for (uint64_t r = 0; r < ng; ++r) col.valid[r] = nn_count(o.nn, r) > 0;
+      }
     }
     res.cols.push_back(std::move(col));
   }
@@ -1220,6 +1417,7 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
     const CatCol* c;
     if (!b.resolve(g, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
     if (!is_int_type(c->type)) return set_error(MTL_UNSUPPORTED, "GROUP BY a non-integer column");
+    if (c->valid) return set_error(MTL_UNSUPPORTED, "GROUP BY a column with NULLs (join shape)");
     keys.push_back(c);
   }
   if (keys.empty() || keys.size() > MTL_GROUP_MAX_KEYS)
@@ -1263,6 +1461,7 @@ static int32_t run_group(const Query& q, Bound& b, mtl_result& res, mtl_stream* 
         if (!b.resolve(it.e->a->col, src, c)) return set_error(MTL_UNSUPPORTED, "%s", b.err.c_str());
         if (c->type == MTL_DICT8 || c->type == MTL_DICT32)
           return set_error(MTL_UNSUPPORTED, "aggregate over a dictionary column");
+        if (c->valid) return set_error(MTL_UNSUPPORTED, "aggregate over a column with NULLs (join shape)");
         a.values = c->dev;
         a.type = c->type;
         scale = c->scale;
@@ -1401,6 +1600,11 @@ static bool to_postfix(const ExprP& e, Bound& b, int fact, mtl_expr& x, int& sca
 static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s, bool dry) {
   const int nt = (int)b.tabs.size();
   if (any_or(q)) return set_error(MTL_UNSUPPORTED, "OR in a star query");
+  for (auto& pr : q.preds)
+    if (pr.nulls) return set_error(MTL_UNSUPPORTED, "IS NULL in a star query");
+  for (auto* t : b.tabs)
+    for (auto& c : t->cols)
+      if (c.valid) return set_error(MTL_UNSUPPORTED, "star query over a table with NULLs");
   // join edges -> (child source, fk column) -> (parent source, unique key)
   struct Edge {
     int child, parent;
@@ -1533,7 +1737,7 @@ static int32_t run_star(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     mp.type = c->type;
     mp.op = pr.op;
     std::string err;
-    if (!literal_bits(pr.lit, *c, pr.op, mp.scalar, err))
+    if (!literal_bits(pr.lit, *c, mp.op, mp.scalar, err))
       return set_error(MTL_UNSUPPORTED, "%s", err.c_str());
     arr[n++] = mp;
     (void)src;
@@ -1658,6 +1862,35 @@ static int32_t pack_keys(const PackKeys& pk, const uint32_t* sel, uint64_t n, De
                      (int64_t*)out.p);
   MTL_HIP(hipGetLastError());
   return MTL_OK;
+}
+
+// NULL handling over gathered columns (one byte of validity per row).
+// acc[i] &= v[i]: the validity of an expression is that of all its columns.
+__global__ void k_valid_and(uint8_t* acc, const uint8_t* v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    acc[i] &= v[i];
+}
+// col[i] = fill where valid[i] == 0 (width 4 or 8): the identity of an
+// aggregate (0 for SUM, the type's maximum for MIN, ...) replaces NULLs.
+__global__ void k_null_fill(void* col, uint32_t width, const uint8_t* valid, uint64_t n,
+                            uint64_t fill) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (valid[i]) continue;
+    if (width == 8) ((uint64_t*)col)[i] = fill;
+    else ((uint32_t*)col)[i] = (uint32_t)fill;
+  }
+}
+// out[i] = valid[i] (int32): COUNT(col) as the SUM of this column.
+__global__ void k_valid_i32(const uint8_t* valid, uint64_t n, int32_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = valid[i];
+}
+
+static uint32_t grid_for(uint64_t n) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ceil_div(n, 256), 4096));
 }
 
 struct JoinRel {
@@ -1830,6 +2063,11 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
   }
   if (edges.size() != ns - 1)
     return set_error(MTL_UNSUPPORTED, "joins must connect the tables as a tree");
+  for (auto& e : edges)  // NULL keys match nothing
+    for (size_t k = 0; k < e.xa.size(); ++k) {
+      MTL_TRY(add_not_null(preds, e.a, *e.xa[k]));
+      MTL_TRY(add_not_null(preds, e.c, *e.xc[k]));
+    }
   // outputs
   bool has_agg = false;
   for (auto& it : q.items) has_agg |= it.e->k == Expr::AGG;
@@ -1847,6 +2085,10 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     std::vector<Col> cols;  // the expression's columns and their tables
   };
   std::vector<std::unique_ptr<AggExpr>> gexpr;  // per aggregate; null: a column
+  // per aggregate: the columns with NULLs its value depends on, and whether
+  // the aggregate is the hidden count of their valid rows
+  std::vector<std::vector<Col>> gnull;
+  std::vector<bool> gcount;
   std::vector<GroupOut> gouts;
   std::vector<int32_t> agg_func, agg_type, agg_scale;
   mtl_group_desc d{};
@@ -1889,6 +2131,7 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
         Col c{-1, nullptr};
         int32_t vtype = 0, vscale = 0;
         std::unique_ptr<AggExpr> ax;
+        std::vector<Col> nulls;
         if (it.e->a && it.e->a->k != Expr::COL) {
           // an integer/DECIMAL expression of columns of any joined tables,
           // evaluated per joined row into an int64 column
@@ -1907,6 +2150,8 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
           }
           if (cols.empty()) return set_error(MTL_UNSUPPORTED, "aggregate of a constant");
           vtype = dec ? MTL_DECIMAL : MTL_BIGINT;
+          for (auto& xc : ax->cols)
+            if (xc.c->valid) nulls.push_back(xc);
           vscale = dec ? std::max(sc, 0) : 0;
           c = Col{ax->cols[0].src, ax->cols[0].c};  // non-null: the aggregate reads values
         } else if (it.e->a) {
@@ -1915,6 +2160,7 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
             return set_error(MTL_UNSUPPORTED, "aggregate over a dictionary column");
           vtype = c.c->type;
           vscale = c.c->type == MTL_DECIMAL ? c.c->scale : 0;
+          if (c.c->valid) nulls.push_back(c);
         } else if (func != MTL_AGG_COUNT) {
           return set_error(MTL_UNSUPPORTED, "%s(*)", f.c_str());
         }
@@ -1926,6 +2172,8 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
         agg_scale.push_back(vscale);
         gvals.push_back(c);
         gexpr.push_back(std::move(ax));
+        gnull.push_back(std::move(nulls));
+        gcount.push_back(false);
         o.agg = (int)d.n_aggs++;
       } else {
         return set_error(MTL_UNSUPPORTED, "select item %zu of an aggregate query", i);
@@ -1935,6 +2183,56 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     if (gkeys.empty())
       for (auto& o : gouts)
         if (o.key >= 0) return set_error(MTL_UNSUPPORTED, "a column beside ungrouped aggregates");
+    // aggregates over values with NULLs skip them: COUNT(col) becomes the
+    // SUM of a 0/1 column; SUM/MIN/MAX/AVG get a count of the valid values
+    // (NULL where it is 0; aggregates over the same columns share one, a
+    // COUNT(col) among them included), AVG is that SUM over that count
+    auto same_nulls = [](const std::vector<Col>& x, const std::vector<Col>& y) {
+      if (x.size() != y.size()) return false;
+      for (size_t i = 0; i < x.size(); ++i)
+        if (x[i].c != y[i].c || x[i].src != y[i].src) return false;
+      return true;
+    };
+    std::vector<int> counts;  // engine aggregates counting valid rows
+    for (auto& o : gouts) {
+      if (o.agg < 0 || gnull[o.agg].empty() || agg_func[o.agg] != MTL_AGG_COUNT) continue;
+      const int a = o.agg;
+      d.aggs[a].func = agg_func[a] = MTL_AGG_SUM;
+      d.aggs[a].type = agg_type[a] = MTL_INTEGER;
+      d.aggs[a].scale = agg_scale[a] = 0;
+      gcount[a] = true;
+      o.mode = 1;
+      counts.push_back(a);
+    }
+    for (auto& o : gouts) {
+      if (o.agg < 0 || gnull[o.agg].empty() || o.mode == 1) continue;
+      const int a = o.agg;
+      int h = -1;
+      for (int c : counts)
+        if (same_nulls(gnull[c], gnull[a])) h = c;
+      if (h < 0) {
+        if (d.n_aggs >= MTL_GROUP_MAX_AGGS)
+          return set_error(MTL_UNSUPPORTED, "too many aggregates with NULLs");
+        h = (int)d.n_aggs++;
+        d.aggs[h].func = MTL_AGG_SUM;
+        d.aggs[h].type = MTL_INTEGER;
+        d.aggs[h].scale = 0;
+        agg_func.push_back(MTL_AGG_SUM);
+        agg_type.push_back(MTL_INTEGER);
+        agg_scale.push_back(0);
+        gvals.push_back(gnull[a][0]);
+        gexpr.emplace_back();
+        std::vector<Col> nl = gnull[a];
+        gnull.push_back(std::move(nl));
+        gcount.push_back(true);
+        counts.push_back(h);
+      }
+      o.nn = h;
+      if (agg_func[a] == MTL_AGG_AVG) {
+        d.aggs[a].func = agg_func[a] = MTL_AGG_SUM;
+        o.mode = 2;
+      }
+    }
   }
   // ORDER BY
   const bool grouped = !gkeys.empty() || has_agg;
@@ -1970,6 +2268,10 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     }
     if (keys_ok) {
       key_order = q.order[0].desc ? -1 : 1;
+      if (key_order < 0)
+        for (auto& g : gkeys)
+          if (g.c->valid)
+            return set_error(MTL_UNSUPPORTED, "ORDER BY DESC over group keys with NULLs");
     } else {
       // one aggregate, then group keys as tie-breaks
       const ExprP& e = q.order[0].e;
@@ -1984,6 +2286,9 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
         if (by_alias || same_call) order_agg = gouts[i].agg;
       }
       if (order_agg < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY of an aggregate query");
+      for (auto& o : gouts)
+        if (o.agg == order_agg && (o.nn >= 0 || o.mode != 0))
+          return set_error(MTL_UNSUPPORTED, "ORDER BY an aggregate with NULLs");
       for (size_t i = 1; i < q.order.size(); ++i) {
         const ExprP& x = q.order[i].e;
         if (x->k != Expr::COL) return set_error(MTL_UNSUPPORTED, "ORDER BY of an aggregate query");
@@ -1997,6 +2302,8 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
             if (gkeys[k].c == c && gkeys[k].src == src) key = (int)k;
         if (key < 0)
           return set_error(MTL_UNSUPPORTED, "ORDER BY a column that is not a group key");
+        if (q.order[i].desc && gkeys[key].c->valid)
+          return set_error(MTL_UNSUPPORTED, "ORDER BY DESC over group keys with NULLs");
         then_keys.emplace_back(key, q.order[i].desc);
       }
       if (q.limit < 0) return set_error(MTL_UNSUPPORTED, "ORDER BY an aggregate needs LIMIT");
@@ -2005,12 +2312,22 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
       odesc = q.order[0].desc;
     }
   }
-  // group descriptor (columns filled after the joins)
+  // group descriptor (columns filled after the joins); a key with NULLs
+  // gets one more value above its range standing for NULL (NULLs last)
   d.n_keys = gkeys.empty() ? 1 : (uint32_t)gkeys.size();
+  std::vector<int64_t> key_null(gkeys.size(), INT64_MIN);
   for (size_t k = 0; k < gkeys.size(); ++k) {
     d.key_type[k] = gkeys[k].c->type;
     d.key_min[k] = gkeys[k].c->min;
     d.key_max[k] = gkeys[k].c->max;
+    if (gkeys[k].c->valid) {
+      const int w = type_width(gkeys[k].c->type);
+      const int64_t top = gkeys[k].c->type == MTL_DICT8 ? 255 : w == 4 ? INT32_MAX : INT64_MAX;
+      if (gkeys[k].c->max >= top)
+        return set_error(MTL_UNSUPPORTED, "GROUP BY %s: no value left for NULL",
+                         gkeys[k].c->name.c_str());
+      key_null[k] = d.key_max[k] = gkeys[k].c->max + 1;
+    }
   }
   if (gkeys.empty()) {  // ungrouped: one constant key
     d.key_type[0] = MTL_INTEGER;
@@ -2152,6 +2469,12 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     R.m = np;
   }
   const uint64_t m = R.m;
+  // the validity bytes of a column with NULLs, gathered into relation order
+  auto gather_valid = [&](const Col& c, DevBuf& out) -> int32_t {
+    MTL_TRY(out.alloc(std::max<uint64_t>(m, 1)));
+    if (m) MTL_TRY(mtl_gather(c.c->valid, 1, (const uint32_t*)R.ids[c.src]->p, nullptr, m, out.p, s));
+    return MTL_OK;
+  };
   if (!grouped) {
     DevBuf perm, out;
     uint64_t n_out = m;
@@ -2164,8 +2487,29 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
                                 kcol.p, s));
       MTL_TRY(perm.alloc(std::max<uint64_t>(m, 1) * 4));
       uint32_t n = 0;
-      MTL_TRY(mtl_sort(kcol.p, okey.c->type, nullptr, nullptr, m, odesc ? 1 : 0,
-                       (uint32_t*)perm.p, &n, s));
+      if (okey.c->valid && m) {
+        // NULLs last: the valid positions sorted, then the NULL positions
+        DevBuf vb, vpos, npos, vcnt;
+        MTL_TRY(gather_valid(okey, vb));
+        MTL_TRY(vpos.alloc(m * 4));
+        MTL_TRY(npos.alloc(m * 4));
+        MTL_TRY(vcnt.alloc(16));
+        mtl_pred pv{vb.p, MTL_DICT8, MTL_CMP_EQ, 1}, pn{vb.p, MTL_DICT8, MTL_CMP_EQ, 0};
+        MTL_TRY(mtl_filter(&pv, 1, m, nullptr, nullptr, m, (uint32_t*)vpos.p, (uint32_t*)vcnt.p, s));
+        MTL_TRY(mtl_filter(&pn, 1, m, nullptr, nullptr, m, (uint32_t*)npos.p,
+                           (uint32_t*)vcnt.p + 1, s));
+        uint32_t c2[2] = {0, 0};
+        MTL_TRY(mtl_memcpy_d2h(c2, vcnt.p, 8, s));
+        if (c2[0] > m || c2[1] > m - c2[0]) return set_error(MTL_ERR_PROCESSING, "device scan failed");
+        MTL_TRY(mtl_sort(kcol.p, okey.c->type, (const uint32_t*)vpos.p, (const uint32_t*)vcnt.p, m,
+                         odesc ? 1 : 0, (uint32_t*)perm.p, &n, s));
+        if (c2[1])
+          MTL_HIP(hipMemcpyAsync((uint32_t*)perm.p + n, npos.p, (uint64_t)c2[1] * 4,
+                                 hipMemcpyDeviceToDevice, s->stream));
+      } else {
+        MTL_TRY(mtl_sort(kcol.p, okey.c->type, nullptr, nullptr, m, odesc ? 1 : 0,
+                         (uint32_t*)perm.p, &n, s));
+      }
       order = perm.p;
     }
     if (q.limit >= 0 && (uint64_t)q.limit < n_out) n_out = (uint64_t)q.limit;
@@ -2186,13 +2530,36 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
       }
       if (n_out) MTL_TRY(mtl_gather(proj[i].c->dev, w, (const uint32_t*)rows, nullptr, n_out, out.p, s));
       MTL_TRY(download(col.data, out.p, n_out * w, s));
+      if (proj[i].c->valid) {  // a projection of NULL is NULL
+        if (n_out) MTL_TRY(mtl_gather(proj[i].c->valid, 1, (const uint32_t*)rows, nullptr, n_out,
+                                      out.p, s));
+        MTL_TRY(download(col.valid, out.p, n_out, s));
+      }
       res.cols.push_back(std::move(col));
     }
     return MTL_OK;
   }
   // aggregates over the gathered key and value columns
-  if (gkeys.empty() && m == 0)
-    return set_error(MTL_UNSUPPORTED, "aggregates over no rows (NULL results)");
+  if (gkeys.empty() && m == 0) {
+    // no rows: one row, COUNTs 0 and every other aggregate NULL
+    res.rows = 1;
+    for (auto& o : gouts) {
+      mtl_result::Column col;
+      col.name = o.name;
+      const int a = o.agg;
+      if (agg_func[a] == MTL_AGG_COUNT || o.mode == 1) {
+        col.type = MTL_BIGINT;
+        col.data.assign(8, 0);
+      } else {
+        agg_out_type(o.mode == 2 ? MTL_AGG_AVG : agg_func[a], agg_type[a], agg_scale[a], col.type,
+                     col.scale);
+        col.data.assign((size_t)type_width(col.type), 0);
+        col.valid.assign(1, 0);
+      }
+      res.cols.push_back(std::move(col));
+    }
+    return MTL_OK;
+  }
   std::vector<std::unique_ptr<DevBuf>> mats;
   auto gathered = [&](const Col& c, const void** out) -> int32_t {
     const uint32_t w = (uint32_t)type_width(c.c->type);
@@ -2210,11 +2577,48 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     MTL_HIP(hipMemsetAsync(mats.back()->p, 0, std::max<uint64_t>(m, 1) * 4, s->stream));
     d.key_col[0] = mats.back()->p;
   }
-  for (size_t k = 0; k < gkeys.size(); ++k) MTL_TRY(gathered(gkeys[k], &d.key_col[k]));
+  for (size_t k = 0; k < gkeys.size(); ++k) {
+    MTL_TRY(gathered(gkeys[k], &d.key_col[k]));
+    if (key_null[k] != INT64_MIN && m) {  // NULL keys -> the value above the range
+      DevBuf vb;
+      MTL_TRY(gather_valid(gkeys[k], vb));
+      hipLaunchKernelGGL(k_null_fill, dim3(grid_for(m)), dim3(256), 0, s->stream,
+                         (void*)d.key_col[k], (uint32_t)type_width(gkeys[k].c->type),
+                         (const uint8_t*)vb.p, m, (uint64_t)key_null[k]);
+      MTL_HIP(hipGetLastError());
+    }
+  }
+  // validity of each NULL-dependent aggregate: the AND over its columns
+  auto agg_valid = [&](size_t a, DevBuf& vb) -> int32_t {
+    MTL_TRY(gather_valid(gnull[a][0], vb));
+    for (size_t k = 1; k < gnull[a].size(); ++k) {
+      DevBuf v2;
+      MTL_TRY(gather_valid(gnull[a][k], v2));
+      if (m) {
+        hipLaunchKernelGGL(k_valid_and, dim3(grid_for(m)), dim3(256), 0, s->stream,
+                           (uint8_t*)vb.p, (const uint8_t*)v2.p, m);
+        MTL_HIP(hipGetLastError());
+      }
+    }
+    return MTL_OK;
+  };
   DevBuf eflag;
   bool any_expr = false;
   for (size_t a = 0; a < gvals.size(); ++a) {
     if (!gvals[a].c) continue;
+    if (gcount[a]) {  // COUNT of the valid values: SUM of a 0/1 int32 column
+      DevBuf vb;
+      MTL_TRY(agg_valid(a, vb));
+      mats.emplace_back(new DevBuf());
+      MTL_TRY(mats.back()->alloc(std::max<uint64_t>(m, 1) * 4));
+      if (m) {
+        hipLaunchKernelGGL(k_valid_i32, dim3(grid_for(m)), dim3(256), 0, s->stream,
+                           (const uint8_t*)vb.p, m, (int32_t*)mats.back()->p);
+        MTL_HIP(hipGetLastError());
+      }
+      d.aggs[a].values = mats.back()->p;
+      continue;
+    }
     if (!gexpr[a]) {
       MTL_TRY(gathered(gvals[a], &d.aggs[a].values));
       continue;
@@ -2232,13 +2636,33 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     MTL_TRY(mtl::expr_eval_i64(ax.x, m, (int64_t*)mats.back()->p, (uint32_t*)eflag.p, s->stream));
     d.aggs[a].values = mats.back()->p;
   }
+  // NULL values of the other aggregates -> the aggregate's identity
+  for (size_t a = 0; a < gvals.size(); ++a) {
+    if (!gvals[a].c || gcount[a] || gnull[a].empty() || !m) continue;
+    const int32_t t = agg_type[a];
+    const uint32_t w = gexpr[a] ? 8u : (uint32_t)type_width(t);
+    uint64_t fill = 0;  // SUM (and AVG, a SUM here)
+    const bool fl = t == MTL_FLOAT || t == MTL_DOUBLE;
+    if (agg_func[a] == MTL_AGG_MIN)  // NaN sorts above every value, as in DuckDB
+      fill = fl ? (w == 8 ? 0x7FF8000000000000ull : 0x7FC00000ull)
+                : (w == 8 ? (uint64_t)INT64_MAX : (uint64_t)INT32_MAX);
+    else if (agg_func[a] == MTL_AGG_MAX)
+      fill = fl ? (w == 8 ? 0xFFF0000000000000ull : 0xFF800000ull)
+                : (w == 8 ? (uint64_t)INT64_MIN : (uint64_t)(uint32_t)INT32_MIN);
+    DevBuf vb;
+    MTL_TRY(agg_valid(a, vb));
+    hipLaunchKernelGGL(k_null_fill, dim3(grid_for(m)), dim3(256), 0, s->stream,
+                       (void*)d.aggs[a].values, w, (const uint8_t*)vb.p, m, fill);
+    MTL_HIP(hipGetLastError());
+  }
   if (any_expr) {
     uint32_t ef = 0;
     MTL_TRY(mtl_memcpy_d2h(&ef, eflag.p, 4, s));
     if (ef) return set_error(MTL_ERR_OVERFLOW, "Overflow in aggregate expression");
   }
   std::vector<const CatCol*> ek = gkey_cols;
-  GroupEmit ge{ek, agg_func, agg_type, agg_scale, order_agg, odesc, key_order, then_keys};
+  GroupEmit ge{ek, agg_func, agg_type, agg_scale, order_agg, odesc, key_order, then_keys,
+               key_null};
   return emit_groups(q, d, ge, gouts, res, s);
 }
 
@@ -2350,8 +2774,11 @@ static int32_t rewrite(const char* sql, std::string& out) {
   std::string run;  // the current run of OR-ed predicates
   size_t in_run = 0;
   for (auto& p : q.preds) {
-    run += (in_run ? " OR " : "") + std::string(hook[p.op]) + "(" + col_sql(p.col) + ", " +
-           lit_sql(p.lit) + ")";
+    if (p.nulls)
+      run += (in_run ? " OR " : "") + col_sql(p.col) + (p.nulls == 1 ? " IS NULL" : " IS NOT NULL");
+    else
+      run += (in_run ? " OR " : "") + std::string(hook[p.op]) + "(" + col_sql(p.col) + ", " +
+             lit_sql(p.lit) + ")";
     ++in_run;
     if (!p.or_next) {
       conj.push_back(in_run > 1 ? "(" + run + ")" : run);
@@ -2371,6 +2798,13 @@ static int32_t rewrite(const char* sql, std::string& out) {
 }  // namespace mtl
 
 namespace mtl {
+void catalog_drop_table(const char* table) {
+  using namespace mtl::sql;
+  std::lock_guard<std::mutex> lk(g_cat_mu);
+  if (table) catalog().erase(lower(table));
+  else catalog().clear();
+}
+
 // Admission size of a statement for the serving loop: its plan's device
 // scratch (MTL_UNSUPPORTED and the reason when it does not lower).
 int32_t sql_scratch_bytes(const char* sql, uint64_t* bytes) {
@@ -2440,11 +2874,24 @@ int32_t mtl_catalog_set_dictionary(const char* table, const char* column,
   return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "column %s is not registered", column);
 }
 
-int32_t mtl_catalog_drop(const char* table) {
+int32_t mtl_catalog_set_validity(const char* table, const char* column, const uint8_t* valid) {
   using namespace mtl::sql;
+  if (!table || !column) return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
   std::lock_guard<std::mutex> lk(g_cat_mu);
-  if (table) catalog().erase(lower(table));
-  else catalog().clear();
+  auto it = catalog().find(lower(table));
+  if (it == catalog().end())
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "table %s is not registered", table);
+  for (auto& c : it->second.cols)
+    if (c.name == lower(column)) {
+      c.valid = valid;
+      return MTL_OK;
+    }
+  return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "column %s is not registered", column);
+}
+
+int32_t mtl_catalog_drop(const char* table) {
+  mtl::catalog_drop_table(table);
+  mtl::ingest_release(table);  // after the catalog lock: ingestion takes the locks the other way
   return MTL_OK;
 }
 
@@ -2486,6 +2933,13 @@ int32_t mtl_result_column(const mtl_result* r, uint32_t i, const char** name, in
   if (type) *type = c.type;
   if (scale) *scale = c.scale;
   if (data) *data = c.data.data();
+  return MTL_OK;
+}
+
+int32_t mtl_result_validity(const mtl_result* r, uint32_t i, const uint8_t** valid) {
+  if (!r || i >= r->cols.size() || !valid)
+    return mtl::set_error(MTL_ERR_INVALID_ARGUMENT, "no result column %u", i);
+  *valid = r->cols[i].valid.empty() ? nullptr : r->cols[i].valid.data();
   return MTL_OK;
 }
 
@@ -2549,12 +3003,12 @@ void release_schema(ArrowSchema* s) {
 }
 
 void make_schema(ArrowSchema* s, const std::string& format, const std::string& name,
-                 std::vector<ArrowSchema*> kids, ArrowSchema* dict) {
+                 std::vector<ArrowSchema*> kids, ArrowSchema* dict, bool nullable = false) {
   auto* own = new SchemaOwn{format, name, std::move(kids), dict};
   s->format = own->format.c_str();
   s->name = own->name.c_str();
   s->metadata = nullptr;
-  s->flags = 0;  // not nullable: these results hold no NULLs
+  s->flags = nullable ? 2 : 0;  // ARROW_FLAG_NULLABLE
   s->n_children = (int64_t)own->kids.size();
   s->children = own->kids.empty() ? nullptr : own->kids.data();
   s->dictionary = dict;
@@ -2562,17 +3016,30 @@ void make_schema(ArrowSchema* s, const std::string& format, const std::string& n
   s->private_data = own;
 }
 
-// buffers: the validity slot (always absent: no NULLs) then `bufs`
+// buffers: the validity bitmap (absent when `valid` is empty: no NULLs)
+// then `bufs`; valid holds one byte per row (1 = valid).
 void make_array(ArrowArray* a, int64_t length, std::vector<std::vector<uint8_t>> bufs,
-                std::vector<ArrowArray*> kids, ArrowArray* dict) {
+                std::vector<ArrowArray*> kids, ArrowArray* dict,
+                const std::vector<uint8_t>& valid = {}) {
   auto* own = new ArrowOwn();
+  int64_t nulls = 0;
+  std::vector<uint8_t> bitmap;
+  if (!valid.empty()) {
+    bitmap.assign((size_t)(length + 7) / 8, 0);
+    for (int64_t i = 0; i < length; ++i) {
+      if (valid[i]) bitmap[i / 8] |= (uint8_t)(1u << (i % 8));  // LSB first
+      else ++nulls;
+    }
+  }
   own->bufs = std::move(bufs);
-  own->ptrs.push_back(nullptr);
-  for (auto& b : own->bufs) own->ptrs.push_back(b.data());
+  if (!valid.empty()) own->bufs.push_back(std::move(bitmap));
+  own->ptrs.push_back(valid.empty() ? nullptr : own->bufs.back().data());
+  for (size_t i = 0; i + (valid.empty() ? 0 : 1) < own->bufs.size(); ++i)
+    own->ptrs.push_back(own->bufs[i].data());
   own->kids = std::move(kids);
   own->dict = dict;
   a->length = length;
-  a->null_count = 0;
+  a->null_count = nulls;
   a->offset = 0;
   a->n_buffers = (int64_t)own->ptrs.size();
   a->n_children = (int64_t)own->kids.size();
@@ -2622,7 +3089,8 @@ int32_t mtl_result_export_arrow(const mtl_result* r, ArrowSchema* schema, ArrowA
     auto* s = new ArrowSchema();
     auto* a = new ArrowArray();
     const int w = type_width(c.type);
-    if ((uint64_t)c.data.size() < (uint64_t)n * (uint64_t)w) {
+    if ((uint64_t)c.data.size() < (uint64_t)n * (uint64_t)w ||
+        (!c.valid.empty() && c.valid.size() < (uint64_t)n)) {
       delete s;
       delete a;
       return fail("result column shorter than its rows", c.type);
@@ -2676,8 +3144,8 @@ int32_t mtl_result_export_arrow(const mtl_result* r, ArrowSchema* schema, ArrowA
         auto* ds = new ArrowSchema();
         auto* da = new ArrowArray();
         make_utf8(ds, da, c.dict);
-        make_schema(s, c.type == MTL_DICT8 ? "C" : "i", c.name, {}, ds);
-        make_array(a, n, {std::move(data)}, {}, da);
+        make_schema(s, c.type == MTL_DICT8 ? "C" : "i", c.name, {}, ds, !c.valid.empty());
+        make_array(a, n, {std::move(data)}, {}, da, c.valid);
         cs.push_back(s);
         ca.push_back(a);
         continue;
@@ -2687,8 +3155,8 @@ int32_t mtl_result_export_arrow(const mtl_result* r, ArrowSchema* schema, ArrowA
         delete a;
         return fail("no Arrow export for the result type", c.type);
     }
-    make_schema(s, fmt, c.name, {}, nullptr);
-    make_array(a, n, {std::move(data)}, {}, nullptr);
+    make_schema(s, fmt, c.name, {}, nullptr, !c.valid.empty());
+    make_array(a, n, {std::move(data)}, {}, nullptr, c.valid);
     cs.push_back(s);
     ca.push_back(a);
   }

@@ -34,6 +34,43 @@ def set_dictionary(table: str, column: str, values) -> None:
                                                    len(values)))
 
 
+def ingest_arrow(name: str, data, unique=()):
+    """gpu_register_table over a pyarrow Table or RecordBatch (or a list of
+    batches): mtl_catalog_ingest_arrow through the Arrow C data interface.
+    Returns (rows, bytes, skipped column names).  Needs gpu_buffer_init."""
+    import pyarrow as pa
+    batches = data.to_batches() if isinstance(data, pa.Table) else (
+        [data] if isinstance(data, pa.RecordBatch) else list(data))
+    schema = data.schema if not isinstance(data, list) else batches[0].schema
+    s_buf = C.create_string_buffer(72)
+    a_bufs = [C.create_string_buffer(80) for _ in batches]
+    schema._export_to_c(C.addressof(s_buf))
+    for b, ab in zip(batches, a_bufs):
+        b._export_to_c(C.addressof(ab))
+    try:
+        arr = (C.c_void_p * max(1, len(batches)))(*[C.addressof(ab) for ab in a_bufs])
+        uq = (C.c_char_p * max(1, len(unique)))(*[u.encode() for u in unique])
+        rows, nbytes = C.c_uint64(), C.c_uint64()
+        skipped = C.create_string_buffer(4096)
+        abi.check(abi.lib().mtl_catalog_ingest_arrow(
+            name.encode(), C.addressof(s_buf), arr, len(batches), uq, len(unique),
+            C.byref(rows), C.byref(nbytes), skipped, 4096))
+    finally:
+        # hand the exported structs back to pyarrow, which releases them
+        for ab in a_bufs:
+            pa.RecordBatch._import_from_c(C.addressof(ab), schema)
+        pa.Schema._import_from_c(C.addressof(s_buf))
+    sk = skipped.value.decode()
+    return rows.value, nbytes.value, [x for x in sk.split(",") if x]
+
+
+def set_validity(table: str, column: str, valid_dev_ptr) -> None:
+    """NULLs of a registered column: device bytes, one per row, 1 = valid
+    (0 / None: no NULLs).  Value bytes of NULL rows must be zero."""
+    abi.check(abi.lib().mtl_catalog_set_validity(table.encode(), column.encode(),
+                                                 C.c_void_p(valid_dev_ptr or None)))
+
+
 def drop(table: str | None = None) -> None:
     abi.check(abi.lib().mtl_catalog_drop(None if table is None else table.encode()))
 
@@ -61,9 +98,16 @@ def rewrite(sql: str) -> str:
     return buf.value.decode()
 
 
+class Column(tuple):
+    """(type, scale, values) of a result column; .valid is a numpy bool
+    array (True = valid) when the column holds NULLs, else None."""
+    valid = None
+
+
 def execute(sql: str):
-    """Run through the GPU path: {column name: (type, scale, numpy array)} in
-    select order.  HUGEINT sums come back as Python ints."""
+    """Run through the GPU path: {column name: Column(type, scale, values)}
+    in select order.  HUGEINT sums come back as Python ints; NULL rows are
+    marked in each Column's .valid."""
     L = abi.lib()
     h = C.c_void_p()
     abi.check(L.mtl_sql_execute(sql.encode(), C.byref(h), None))
@@ -88,12 +132,18 @@ def take_result(h):
                 raw = np.ctypeslib.as_array((C.c_uint64 * (2 * n)).from_address(data.value)) \
                     if n else np.zeros(0, np.uint64)
                 vals = [(int(np.int64(raw[2 * r + 1])) << 64) | int(raw[2 * r]) for r in range(n)]
-                out[name.value.decode()] = (t.value, sc.value, vals)
+                col = Column((t.value, sc.value, vals))
             else:
                 dt = _NP[t.value]
                 arr = np.frombuffer((C.c_char * (n * np.dtype(dt).itemsize)).from_address(
                     data.value), dtype=dt).copy() if n else np.zeros(0, dt)
-                out[name.value.decode()] = (t.value, sc.value, arr)
+                col = Column((t.value, sc.value, arr))
+            vp = C.c_void_p()
+            abi.check(L.mtl_result_validity(h, i, C.byref(vp)))
+            if vp.value and n:
+                col.valid = np.frombuffer((C.c_char * n).from_address(vp.value),
+                                          dtype=np.uint8).astype(bool)
+            out[name.value.decode()] = col
         return out
     finally:
         L.mtl_result_free(h)

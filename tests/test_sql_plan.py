@@ -26,6 +26,8 @@ def catalog():
                                        ("segment", abi.DICT8, FAKE, 0, 4, False)])
     S.set_dictionary("customers", "segment", ["AUTOMOBILE", "BUILDING", "FURNITURE",
                                               "HOUSEHOLD", "MACHINERY"])
+    S.register_table("tags", 4, [("tag", abi.DICT8, FAKE, 0, 1, False)])
+    S.set_dictionary("tags", "tag", ["zeta", "alpha"])  # not sorted
     S.register_table("orders", 100, [("o_key", abi.INTEGER, FAKE, 1, 100, True),
                                      ("o_cust", abi.INTEGER, FAKE, 1, 10, False),
                                      ("o_date", abi.DATE, FAKE, 8035, 10592, False),
@@ -70,6 +72,8 @@ Q3 = ("SELECT l.l_order, SUM(l.price * (100 - l.discount)) AS revenue, o.o_date,
      "JOIN orders o ON c.c_key = o.o_cust JOIN lines l ON l.l_order = o.o_key "
      "GROUP BY c.segment, o.o_prio ORDER BY c.segment, o.o_prio", "join"),
     ("SELECT COUNT(*) FROM orders o JOIN lines l ON l.l_order = o.o_key", "join"),
+    # a sorted dictionary admits range comparisons
+    ("SELECT segment FROM customers WHERE segment < 'B'", "scan"),
     # OR runs and IN lists
     ("SELECT value FROM numbers WHERE value > 1 OR value < 0", "scan"),
     ("SELECT value FROM numbers WHERE value IN (1, 5, 9) AND (value < 4 OR value > 8)", "scan"),
@@ -101,7 +105,7 @@ def test_admitted_shapes(sql, shape):
     ("SELECT c_i FROM t WHERE c_i > 1.5", "non-integer literal"),
     ("SELECT c_d FROM t WHERE c_d < DATE '2001-02-30'", "invalid date"),
     ("SELECT g, SUM(c_b) FROM t GROUP BY g HAVING SUM(c_b) > 0", "HAVING"),
-    ("SELECT segment FROM customers WHERE segment < 'B'", "only '='"),
+    ("SELECT tag FROM tags WHERE tag < 'b'", "not sorted"),
     ("SELECT price FROM t WHERE price > 1.255", "more scale"),
     ("SELECT o.o_key, l.price FROM orders o, lines l", "tree"),
     ("SELECT o.o_key, l.price FROM orders o JOIN lines l ON l.l_order = o.o_key "
