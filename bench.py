@@ -379,6 +379,63 @@ def cpu_baseline(scale: float, seconds: float):
             "gpu_matches_oracle_on_sample": bool(same)}
 
 
+def join_workload(args) -> dict:
+    """PK/FK join at the config-3 sizes with keys too sparse for the dense
+    table: build on 15M unique BIGINT keys, probe with 60M foreign keys
+    (every probe matches).  Times the global hash table (MTL_JOIN_HASH,
+    pairs in probe order) and the LDS-staged radix partitions
+    (MTL_JOIN_RADIX), build + probe per call; algorithmic bytes = build keys
+    8 B x 15M + probe keys 8 B x 60M + pairs 8 B x 60M."""
+    import numpy as np
+    import torch
+    from mettle_amd import abi
+    nb, npr = 15_000_000, 60_000_005
+    rng = np.random.default_rng(3)
+    log("join: generating keys")
+    bk = torch.from_numpy((rng.permutation(nb).astype(np.int64) * 2654435761 - (1 << 50))).cuda()
+    pk = bk[torch.randint(0, nb, (npr,), device="cuda")].contiguous()
+    out_p = torch.empty(npr, dtype=torch.int32, device="cuda")
+    out_b = torch.empty(npr, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(4, dtype=torch.int32, device="cuda")
+    dup = torch.zeros(4, dtype=torch.int32, device="cuda")
+    L = abi.lib()
+    alg = nb * 8 + npr * 8 + npr * 8
+    rows = []
+    for strat, name in ((abi.JOIN_HASH, "hash"), (abi.JOIN_RADIX, "radix")):
+        table = torch.empty(int(L.mtl_join_table_bytes(strat, nb, 0)), dtype=torch.uint8,
+                            device="cuda")
+
+        def call():
+            abi.check(L.mtl_join_build(strat, bk.data_ptr(), abi.BIGINT, None, None, nb, 0, 0,
+                                       table.data_ptr(), table.numel(), dup.data_ptr(), None))
+            abi.check(L.mtl_join_probe(strat, table.data_ptr(), table.numel(), 0, nb,
+                                       pk.data_ptr(), abi.BIGINT, None, None, npr,
+                                       out_p.data_ptr(), out_b.data_ptr(), cnt.data_ptr(), None))
+            abi.check(L.mtl_stream_sync(None))
+        ms = timed(call, args.steps, args.warmup)
+        L.mtl_profile_reset()
+        L.mtl_profile_enable(1)
+        call()
+        L.mtl_profile_enable(0)
+        kp = kernel_profile(L)
+        L.mtl_profile_reset()
+        assert int(cnt[0].item()) == npr and int(dup[0].item()) == 0
+        gbs = alg / (ms * 1e-3) / 1e9
+        rows.append({"strategy": name, "ms": ms, "algorithmic_GBs": gbs,
+                     "frac": gbs / HBM_PEAK_GBS,
+                     "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kp.items()}})
+        log(f"join {name}: {ms:.3f} ms ({gbs:.0f} GB/s algorithmic)")
+    best = min(rows, key=lambda r: r["ms"])
+    return {"metric": "probe rows/s & % HBM roofline (PK/FK join, sparse keys)",
+            "value": npr / (best["ms"] * 1e-3), "unit": "rows/s", "ms_per_step": best["ms"],
+            "config": {"workload": "join build 15M unique BIGINT keys, probe 60M",
+                       "build_rows": nb, "probe_rows": npr},
+            "roofline": {"bound": "hbm", "achieved": best["algorithmic_GBs"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": best["frac"],
+                         "algorithmic_bytes": alg},
+            "variants": rows, "dtype": "int64", "data": "synthetic"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,7 +449,7 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=1)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--workload", choices=["star", "scan", "group", "group-dense"],
+    ap.add_argument("--workload", choices=["star", "scan", "group", "group-dense", "join"],
                     default="star", help="star: the headline line; the others measure "
                     "configs 2 and 4 (parity configs, reported separately)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -411,6 +468,8 @@ def main() -> int:
             return 2
         if args.workload == "scan":
             r = scan_workload(args)
+        elif args.workload == "join":
+            r = join_workload(args)
         else:
             r = group_workload(args, args.workload == "group-dense")
         r.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -86,6 +86,7 @@ extern "C" {
 /* Join strategies — GpuJoinStrategy (src/plan/ir.rs:176-180). */
 #define MTL_JOIN_DENSE 0
 #define MTL_JOIN_HASH 1
+#define MTL_JOIN_RADIX 2 /* LDS-staged partitions; pairs in partition order */
 
 /* A scalar literal as raw bits: int64 for integer-like types, the IEEE bits
  * of a double for FLOAT/DOUBLE columns (rounded to float for FLOAT). */
@@ -302,9 +303,17 @@ int32_t mtl_join_build(int32_t strategy, const void *keys, int32_t key_type,
                        mtl_stream *s);
 
 /* Probe side: for probe rows (sel or 0..n-1) whose key matches, emit the
- * pair (probe row, build row) in ascending probe order.  `slots` is the
- * dense key span, or for MTL_JOIN_HASH the build capacity (rows) that was
+ * pair (probe row, build row) in ascending probe order (MTL_JOIN_RADIX: in
+ * an unspecified order).  `slots` is the dense key span, or for
+ * MTL_JOIN_HASH and MTL_JOIN_RADIX the build capacity (rows) that was
  * passed to mtl_join_build.
+ * MTL_JOIN_RADIX (build capacity <= 2^24 rows): build and probe rows are
+ * partitioned by the top bits of the key's hash, about 2048 build rows per
+ * partition; each partition's table is built in LDS (4096 slots) and probed
+ * there, so a build far larger than the caches costs streaming passes, not
+ * random HBM lookups (the LDS-staged partitions K31's global table,
+ * kernels.metal:786-837, lacks).  Build *dup_flag bits: 2 a key repeats,
+ * 16 a partition over 3072 rows (many repeats of one key): use HASH.
  * Replaces hash_join_probe_gv / dense_lookup_probe_i32_gv /
  * compact_matches_gv (join_ops.rs:72-125, 182-231; gather_ops.rs:298-393;
  * K30, K31, K32). */

@@ -44,7 +44,7 @@ EXPR_MAX_COLS = 4
 MAX_PREDS = 8
 STAR_MAX_TIES = 3
 # join strategies
-JOIN_DENSE, JOIN_HASH = 0, 1
+JOIN_DENSE, JOIN_HASH, JOIN_RADIX = 0, 1, 2
 
 
 class Pred(C.Structure):

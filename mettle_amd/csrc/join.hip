@@ -237,6 +237,372 @@ __global__ __launch_bounds__(FBLOCK) void k_probe_expand(ProbeArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Radix-partitioned join (MTL_JOIN_RADIX): LDS-staged partitions.  The
+// build rows are partitioned by the top bits of the key's hash so that each
+// partition (about kRxRows rows, at most kRxMax) fits a kRxSlots-slot hash
+// table in LDS; the probe rows are partitioned the same way, and one block
+// per (partition, slice of its probe rows) builds the partition's table in
+// LDS and probes it there.  No probe touches a table in HBM: a large build
+// costs a few streaming passes instead of two random HBM lookups per probe
+// row.  The pairs come out grouped by partition (order unspecified), for
+// callers that aggregate them.
+//
+// Table buffer: [0, 256) header (bits) | offsets (P + 1) | cursors (P) |
+// keys (n int64) | rows (n int32), each part 256-byte aligned.
+// ---------------------------------------------------------------------------
+constexpr int RX_BLOCK = 256;
+constexpr uint32_t kRxSlots = 4096;      // LDS table: 32 KB keys + 16 KB rows
+constexpr uint32_t kRxMax = 3072;        // rows per partition (75 % fill)
+constexpr uint64_t kRxRows = 2048;       // target rows per partition
+constexpr uint32_t kRxMaxBits = 13;      // P <= 8192 (LDS histograms)
+
+inline uint64_t al256(uint64_t b) { return (b + 255) & ~255ull; }
+inline uint32_t rx_bits(uint64_t n) {
+  uint32_t b = 0;
+  while (b < kRxMaxBits && (kRxRows << b) < n) ++b;
+  return b;
+}
+struct RxView {
+  uint32_t bits;
+  uint32_t* off;     // P + 1
+  uint32_t* cursor;  // P
+  int64_t* keys;
+  int32_t* rows;
+};
+inline uint64_t rx_bytes(uint64_t n) {
+  const uint64_t P = 1ull << rx_bits(n);
+  return 256 + al256((P + 1) * 4) + al256(P * 4) + al256(n * 8) + al256(n * 4);
+}
+inline RxView rx_view(void* base, uint64_t n) {
+  RxView v;
+  v.bits = rx_bits(n);
+  const uint64_t P = 1ull << v.bits;
+  char* p = (char*)base + 256;
+  v.off = (uint32_t*)p;
+  p += al256((P + 1) * 4);
+  v.cursor = (uint32_t*)p;
+  p += al256(P * 4);
+  v.keys = (int64_t*)p;
+  p += al256(n * 8);
+  v.rows = (int32_t*)p;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t rx_part(int64_t key, uint32_t bits) {
+  return bits ? (uint32_t)(fmix64((uint64_t)key) >> (64 - bits)) : 0u;
+}
+
+// Rows of a side: sel[i] (count-bounded) or i.
+struct RxSide {
+  const void* keys;
+  int32_t type;
+  const uint32_t* sel;
+  const uint32_t* count;
+  uint64_t cap;
+};
+__device__ __forceinline__ uint64_t rx_n(const RxSide& s) {
+  return s.sel ? dev_count(s.count, s.cap) : s.cap;
+}
+
+// hist[p] += rows of partition p (hist cleared by the caller).
+__global__ __launch_bounds__(RX_BLOCK) void k_rx_hist(RxSide s, uint32_t bits, uint32_t* hist) {
+  __shared__ uint32_t h[1u << kRxMaxBits];
+  const uint32_t P = 1u << bits;
+  for (uint32_t i = threadIdx.x; i < P; i += RX_BLOCK) h[i] = 0;
+  __syncthreads();
+  const uint64_t n = rx_n(s);
+  for (uint64_t i = (uint64_t)blockIdx.x * RX_BLOCK + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * RX_BLOCK) {
+    const uint32_t row = s.sel ? s.sel[i] : (uint32_t)i;
+    atomicAdd(&h[rx_part(load_key(s.keys, s.type, row), bits)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < P; i += RX_BLOCK)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// cursor[p] starts at the partition's offset; each block reserves its
+// rows of every partition with one atomic per partition, then places them.
+// Writes (key, row id) for the build side or (key, probe row) for the probe.
+__global__ __launch_bounds__(RX_BLOCK) void k_rx_scatter(RxSide s, uint32_t bits, uint64_t chunk,
+                                                        uint32_t* cursor, int64_t* okeys,
+                                                        int32_t* orows) {
+  __shared__ uint32_t h[1u << kRxMaxBits];
+  const uint32_t P = 1u << bits;
+  const uint64_t n = rx_n(s);
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk;
+  const uint64_t hi = lo + chunk < n ? lo + chunk : n;
+  if (lo >= n) return;  // uniform per block
+  for (uint32_t i = threadIdx.x; i < P; i += RX_BLOCK) h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += RX_BLOCK) {
+    const uint32_t row = s.sel ? s.sel[i] : (uint32_t)i;
+    atomicAdd(&h[rx_part(load_key(s.keys, s.type, row), bits)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < P; i += RX_BLOCK)
+    if (h[i]) h[i] = atomicAdd(&cursor[i], h[i]);
+  __syncthreads();
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += RX_BLOCK) {
+    const uint32_t row = s.sel ? s.sel[i] : (uint32_t)i;
+    const int64_t key = load_key(s.keys, s.type, row);
+    const uint32_t at = atomicAdd(&h[rx_part(key, bits)], 1u);
+    okeys[at] = key;
+    orows[at] = (int32_t)row;
+  }
+}
+
+// The LDS table of one partition: kRxSlots keys (kEmpty = free; the key
+// kEmpty itself lives in the side slot) and their rows.
+struct RxLds {
+  int64_t keys[kRxSlots];
+  int32_t rows[kRxSlots];
+  int32_t empty_row;
+};
+
+// Builds partition p's table in LDS; err bits: 2 duplicate key.
+__device__ __forceinline__ void rx_build_lds(RxLds& t, const int64_t* keys, const int32_t* rows,
+                                             uint32_t lo, uint32_t hi, uint32_t* err) {
+  for (uint32_t i = threadIdx.x; i < kRxSlots; i += RX_BLOCK) t.keys[i] = kEmpty;
+  if (threadIdx.x == 0) t.empty_row = -1;
+  __syncthreads();
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += RX_BLOCK) {
+    const int64_t key = keys[i];
+    const int32_t row = rows[i];
+    if (key == kEmpty) {
+      if (atomicCAS(&t.empty_row, -1, row) != -1) atomicOr(err, 2u);
+      continue;
+    }
+    uint32_t slot = (uint32_t)fmix64((uint64_t)key) & (kRxSlots - 1);
+    for (uint32_t k = 0; k < kRxSlots; ++k) {
+      const unsigned long long old = atomicCAS((unsigned long long*)&t.keys[slot],
+                                               (unsigned long long)kEmpty,
+                                               (unsigned long long)key);
+      if ((int64_t)old == kEmpty) {
+        t.rows[slot] = row;
+        break;
+      }
+      if ((int64_t)old == key) {
+        atomicOr(err, 2u);
+        break;
+      }
+      slot = (slot + 1) & (kRxSlots - 1);
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int32_t rx_lookup(const RxLds& t, int64_t key) {
+  if (key == kEmpty) return t.empty_row;
+  uint32_t slot = (uint32_t)fmix64((uint64_t)key) & (kRxSlots - 1);
+  for (uint32_t k = 0; k < kRxSlots; ++k) {
+    const int64_t x = t.keys[slot];
+    if (x == key) return t.rows[slot];
+    if (x == kEmpty) return -1;
+    slot = (slot + 1) & (kRxSlots - 1);
+  }
+  return -1;
+}
+
+// One block per build partition: size check (err bit 16: a partition over
+// kRxMax rows; only many repeats of one key cause it) and duplicate check.
+__global__ __launch_bounds__(RX_BLOCK) void k_rx_check(const uint32_t* off, const int64_t* keys,
+                                                      const int32_t* rows, uint32_t* err) {
+  __shared__ RxLds t;
+  const uint32_t lo = off[blockIdx.x], hi = off[blockIdx.x + 1];
+  if (hi - lo > kRxMax) {
+    if (threadIdx.x == 0) atomicOr(err, 16u);
+    return;
+  }
+  rx_build_lds(t, keys, rows, lo, hi, err);
+}
+
+struct RxProbe {
+  const uint32_t* boff;  // build partition offsets
+  const int64_t* bkeys;
+  const int32_t* brows;
+  uint32_t bits;
+  uint32_t slices;        // blocks per partition
+  const uint32_t* poff;   // probe partition offsets (NULL: one partition, raw side)
+  const int64_t* pkeys;   // partitioned probe keys
+  const int32_t* prows;   // and their probe rows
+  RxSide raw;             // the probe side when not partitioned
+  uint32_t* out_probe;
+  uint32_t* out_build;
+  uint32_t* out_count;
+  uint32_t* err;
+};
+
+// Matches collect in an LDS buffer and leave in runs of up to kRxOut
+// pairs, one global reservation per run (a reservation per wave would
+// serialise millions of atomics on out_count).
+constexpr uint32_t kRxOut = 1024;
+
+__global__ __launch_bounds__(RX_BLOCK) void k_rx_join(RxProbe a) {
+  __shared__ RxLds t;
+  __shared__ uint32_t s_wave[RX_BLOCK / WAVE];
+  __shared__ uint32_t s_buf_p[kRxOut], s_buf_b[kRxOut];
+  __shared__ uint32_t s_fill, s_base;
+  const uint32_t p = blockIdx.x / a.slices, sl = blockIdx.x % a.slices;
+  if (a.boff[p + 1] - a.boff[p] > kRxMax) {  // the build reported it (bit 16)
+    if (threadIdx.x == 0) atomicOr(a.err, 16u);
+    return;
+  }
+  rx_build_lds(t, a.bkeys, a.brows, a.boff[p], a.boff[p + 1], a.err);
+  uint64_t lo, hi;
+  if (a.poff) {
+    lo = a.poff[p];
+    hi = a.poff[p + 1];
+  } else {
+    lo = 0;
+    hi = rx_n(a.raw);
+  }
+  const uint64_t len = hi - lo;
+  const uint64_t s0 = lo + len * sl / a.slices, s1 = lo + len * (sl + 1) / a.slices;
+  const int wave = threadIdx.x / WAVE;
+  if (threadIdx.x == 0) s_fill = 0;
+  __syncthreads();
+  auto flush = [&](uint32_t n) {  // every thread; s_fill == n
+    if (threadIdx.x == 0) s_base = n ? atomicAdd(a.out_count, n) : 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += RX_BLOCK) {
+      a.out_probe[s_base + i] = s_buf_p[i];
+      a.out_build[s_base + i] = s_buf_b[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_fill = 0;
+    __syncthreads();
+  };
+  for (uint64_t b = s0; b < s1; b += RX_BLOCK) {  // the same iterations in every thread
+    const uint64_t i = b + threadIdx.x;
+    int32_t m = -1;
+    uint32_t prow = 0;
+    if (i < s1) {
+      int64_t key;
+      if (a.poff) {
+        key = a.pkeys[i];
+        prow = (uint32_t)a.prows[i];
+      } else {
+        prow = a.raw.sel ? a.raw.sel[i] : (uint32_t)i;
+        key = load_key(a.raw.keys, a.raw.type, prow);
+      }
+      m = rx_lookup(t, key);
+    }
+    const uint64_t bal = __ballot(m >= 0);
+    if (lane_id() == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = s_fill, step = 0;
+#pragma unroll
+    for (int w = 0; w < RX_BLOCK / WAVE; ++w) {
+      before += w < wave ? s_wave[w] : 0;
+      step += s_wave[w];
+    }
+    if (m >= 0) {  // s_fill + step <= kRxOut: flushed below once over kRxOut - RX_BLOCK
+      const uint32_t at = before + (uint32_t)__popcll(bal & lanemask_lt());
+      s_buf_p[at] = prow;
+      s_buf_b[at] = (uint32_t)m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_fill += step;
+    __syncthreads();
+    if (s_fill > kRxOut - RX_BLOCK) flush(s_fill);
+  }
+  flush(s_fill);
+}
+
+int32_t rx_build(const void* keys, int32_t key_type, const uint32_t* sel, const uint32_t* count,
+                 uint64_t capacity, void* table, uint32_t* err, mtl_stream* s) {
+  const RxView v = rx_view(table, capacity);
+  const uint32_t P = 1u << v.bits;
+  MTL_HIP(hipMemsetAsync(v.off, 0, (P + 1) * 4ull, s->stream));
+  RxSide side{keys, key_type, sel, sel ? count : nullptr, capacity};
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(ceil_div(capacity, RX_BLOCK * 16), (uint64_t)cu_count() * 4));
+  {
+    ProfScope ps(s, "k_rx_hist");
+    hipLaunchKernelGGL(k_rx_hist, dim3(grid), dim3(RX_BLOCK), 0, s->stream, side, v.bits, v.off);
+    MTL_HIP(hipGetLastError());
+  }
+  scan_counts(v.off, P + 1, v.off + P, s->stream);  // exclusive; off[P] = n after the scan
+  MTL_HIP(hipGetLastError());
+  MTL_HIP(hipMemcpyAsync(v.cursor, v.off, P * 4ull, hipMemcpyDeviceToDevice, s->stream));
+  const uint64_t chunk = std::max<uint64_t>(4096, ceil_div(capacity, grid));
+  {
+    ProfScope ps(s, "k_rx_scatter");
+    hipLaunchKernelGGL(k_rx_scatter, dim3((uint32_t)ceil_div(std::max<uint64_t>(capacity, 1), chunk)),
+                       dim3(RX_BLOCK), 0, s->stream, side, v.bits, chunk, v.cursor, v.keys, v.rows);
+    MTL_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(s, "k_rx_check");
+    hipLaunchKernelGGL(k_rx_check, dim3(P), dim3(RX_BLOCK), 0, s->stream, v.off, v.keys, v.rows,
+                       err);
+    MTL_HIP(hipGetLastError());
+  }
+  return MTL_OK;
+}
+
+int32_t rx_probe(const void* table, uint64_t build_cap, const void* keys, int32_t key_type,
+                 const uint32_t* sel, const uint32_t* count, uint64_t capacity,
+                 uint32_t* out_probe, uint32_t* out_build, uint32_t* out_count, mtl_stream* s) {
+  const RxView v = rx_view((void*)table, build_cap);
+  const uint32_t P = 1u << v.bits;
+  RxProbe a{};
+  a.boff = v.off;
+  a.bkeys = v.keys;
+  a.brows = v.rows;
+  a.bits = v.bits;
+  a.raw = RxSide{keys, key_type, sel, sel ? count : nullptr, capacity};
+  a.out_probe = out_probe;
+  a.out_build = out_build;
+  a.out_count = out_count;
+  void* errp;
+  MTL_TRY(s->arena.alloc(16, &errp));
+  a.err = (uint32_t*)errp;
+  MTL_HIP(hipMemsetAsync(out_count, 0, 4, s->stream));
+  MTL_HIP(hipMemsetAsync(errp, 0, 4, s->stream));
+  if (P > 1) {
+    void *poff, *pcur, *pk, *pr;
+    MTL_TRY(s->arena.alloc((P + 1) * 4ull, &poff));
+    MTL_TRY(s->arena.alloc(P * 4ull, &pcur));
+    MTL_TRY(s->arena.alloc(std::max<uint64_t>(capacity, 1) * 8, &pk));
+    MTL_TRY(s->arena.alloc(std::max<uint64_t>(capacity, 1) * 4, &pr));
+    MTL_HIP(hipMemsetAsync(poff, 0, (P + 1) * 4ull, s->stream));
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(ceil_div(capacity, RX_BLOCK * 16), (uint64_t)cu_count() * 4));
+    {
+      ProfScope ps(s, "k_rx_hist");
+      hipLaunchKernelGGL(k_rx_hist, dim3(grid), dim3(RX_BLOCK), 0, s->stream, a.raw, v.bits,
+                         (uint32_t*)poff);
+      MTL_HIP(hipGetLastError());
+    }
+    scan_counts((uint32_t*)poff, P + 1, (uint32_t*)poff + P, s->stream);
+    MTL_HIP(hipGetLastError());
+    MTL_HIP(hipMemcpyAsync(pcur, poff, P * 4ull, hipMemcpyDeviceToDevice, s->stream));
+    const uint64_t chunk = std::max<uint64_t>(4096, ceil_div(capacity, grid));
+    {
+      ProfScope ps(s, "k_rx_scatter");
+      hipLaunchKernelGGL(k_rx_scatter,
+                         dim3((uint32_t)ceil_div(std::max<uint64_t>(capacity, 1), chunk)),
+                         dim3(RX_BLOCK), 0, s->stream, a.raw, v.bits, chunk, (uint32_t*)pcur,
+                         (int64_t*)pk, (int32_t*)pr);
+      MTL_HIP(hipGetLastError());
+    }
+    a.poff = (const uint32_t*)poff;
+    a.pkeys = (const int64_t*)pk;
+    a.prows = (const int32_t*)pr;
+  }
+  // enough blocks to fill the chip: slices of each partition's probe rows
+  a.slices = (uint32_t)std::max<uint64_t>(1, (uint64_t)cu_count() * 4 / P);
+  {
+    ProfScope ps(s, "k_rx_join");
+    hipLaunchKernelGGL(k_rx_join, dim3(P * a.slices), dim3(RX_BLOCK), 0, s->stream, a);
+    MTL_HIP(hipGetLastError());
+  }
+  return MTL_OK;
+}
+
 static uint32_t grid_rows(uint64_t n) {
   uint64_t b = ceil_div(n ? n : 1, 256 * 4);
   uint64_t cap = (uint64_t)cu_count() * 16;
@@ -251,6 +617,7 @@ extern "C" {
 
 uint64_t mtl_join_table_bytes(int32_t strategy, uint64_t build_rows, uint64_t dense_slots) {
   if (strategy == MTL_JOIN_DENSE) return dense_slots * 4;
+  if (strategy == MTL_JOIN_RADIX) return rx_bytes(build_rows);
   return hash_bytes(hash_cap_for(build_rows));
 }
 
@@ -258,10 +625,20 @@ int32_t mtl_join_build(int32_t strategy, const void* keys, int32_t key_type,
                        const uint32_t* sel, const uint32_t* count, uint64_t capacity,
                        int64_t min_key, uint64_t slots, void* table, uint64_t table_bytes,
                        uint32_t* dup_flag, mtl_stream* s) {
-  if (strategy != MTL_JOIN_DENSE && strategy != MTL_JOIN_HASH)
+  if (strategy != MTL_JOIN_DENSE && strategy != MTL_JOIN_HASH && strategy != MTL_JOIN_RADIX)
     return set_error(MTL_ERR_INVALID_ARGUMENT, "join strategy %d", strategy);
   if (!type_is_int(key_type)) return set_error(MTL_UNSUPPORTED, "join key type %d", key_type);
   if (!table || !dup_flag) return set_error(MTL_ERR_INVALID_ARGUMENT, "table/dup_flag NULL");
+  if (strategy == MTL_JOIN_RADIX) {
+    if (capacity > (kRxRows << kRxMaxBits))
+      return set_error(MTL_UNSUPPORTED, "radix join build over %llu rows",
+                       (unsigned long long)(kRxRows << kRxMaxBits));
+    if (table_bytes < rx_bytes(capacity))
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "radix table needs %llu bytes",
+                       (unsigned long long)rx_bytes(capacity));
+    MTL_TRY(resolve_stream(s, &s));
+    return rx_build(keys, key_type, sel, count, capacity, table, dup_flag, s);
+  }
   if (strategy == MTL_JOIN_HASH) {
     // at least one empty slot besides the build rows
     slots = hash_cap_of_bytes(table_bytes);
@@ -295,12 +672,22 @@ int32_t mtl_join_probe(int32_t strategy, const void* table, uint64_t table_bytes
                        int32_t key_type, const uint32_t* sel, const uint32_t* count,
                        uint64_t capacity, uint32_t* out_probe, uint32_t* out_build,
                        uint32_t* out_count, mtl_stream* s) {
-  if (strategy != MTL_JOIN_DENSE && strategy != MTL_JOIN_HASH)
+  if (strategy != MTL_JOIN_DENSE && strategy != MTL_JOIN_HASH && strategy != MTL_JOIN_RADIX)
     return set_error(MTL_ERR_INVALID_ARGUMENT, "join strategy %d", strategy);
   if (!type_is_int(key_type)) return set_error(MTL_UNSUPPORTED, "join key type %d", key_type);
   if (!out_probe || !out_build || !out_count)
     return set_error(MTL_ERR_INVALID_ARGUMENT, "outputs are NULL");
   if (capacity > 0xFFFFFFFFull) return set_error(MTL_UNSUPPORTED, "more than 2^32-1 rows");
+  if (strategy == MTL_JOIN_RADIX) {
+    // `slots` is the build capacity passed to mtl_join_build
+    if (slots > (kRxRows << kRxMaxBits) || table_bytes < rx_bytes(slots))
+      return set_error(MTL_ERR_INVALID_ARGUMENT, "radix table too small");
+    MTL_TRY(resolve_stream(s, &s));
+    std::lock_guard<std::mutex> g(s->mu);
+    s->arena.reset();
+    return rx_probe(table, slots, probe_keys, key_type, sel, count, capacity, out_probe,
+                    out_build, out_count, s);
+  }
   if (strategy == MTL_JOIN_HASH) {
     // the slots the build used (from the same allocation); `slots` is the
     // build capacity, which must have left an empty slot

@@ -1900,12 +1900,13 @@ struct JoinRel {
 
 // Build a join table over (bkeys, btype) through bsel (brows rows, keys in
 // [bmin, bmax]) and probe it with (pkeys, ptype) through psel (prows rows;
-// NULL: rows 0..prows-1): (probe row, build row) pairs in probe order.
-// MTL_UNSUPPORTED when a build key repeats.
+// NULL: rows 0..prows-1): (probe row, build row) pairs, in probe order when
+// `ordered` (else a large build takes the radix-partitioned join, whose
+// pairs come in partition order).  MTL_UNSUPPORTED when a build key repeats.
 static int32_t join_pairs(const void* bkeys, int32_t btype, const uint32_t* bsel, uint64_t brows,
                           int64_t bmin, int64_t bmax, const void* pkeys, int32_t ptype,
                           const uint32_t* psel, uint64_t prows, DevBuf& out_p, DevBuf& out_b,
-                          uint64_t& n_pairs, mtl_stream* s) {
+                          uint64_t& n_pairs, mtl_stream* s, bool ordered = true) {
   n_pairs = 0;
   MTL_TRY(out_p.alloc(std::max<uint64_t>(prows, 1) * 4));
   MTL_TRY(out_b.alloc(std::max<uint64_t>(prows, 1) * 4));
@@ -1915,20 +1916,29 @@ static int32_t join_pairs(const void* bkeys, int32_t btype, const uint32_t* bsel
   const uint64_t span = (uint64_t)bmax - (uint64_t)bmin + 1;
   const bool dense = bmax >= bmin && brows >= 1024 && span <= (1ull << 24) &&
                      span <= std::max<uint64_t>(16 * brows, 4096);
-  const int32_t strategy = dense ? MTL_JOIN_DENSE : MTL_JOIN_HASH;
+  // radix partitions beat the global hash table from a few thousand build
+  // rows on (no random HBM lookups), up to its 2^24-row limit
+  int32_t strategy = dense ? MTL_JOIN_DENSE
+                     : (!ordered && brows >= 4096 && brows <= (1ull << 24)) ? MTL_JOIN_RADIX
+                                                                           : MTL_JOIN_HASH;
   const uint64_t slots = dense ? span : brows;
-  const uint64_t tb = mtl_join_table_bytes(strategy, brows, dense ? span : 0);
   DevBuf table, flag, cnt;
-  MTL_TRY(table.alloc(tb));
   MTL_TRY(flag.alloc(16));
   MTL_TRY(cnt.alloc(16));
   const uint32_t zero = 0, br = (uint32_t)brows;
-  MTL_TRY(mtl_memcpy_h2d(flag.p, &zero, 4, s));
   MTL_TRY(mtl_memcpy_h2d(cnt.p, &br, 4, s));  // the build selection's count
-  MTL_TRY(mtl_join_build(strategy, bkeys, btype, bsel, bsel ? (const uint32_t*)cnt.p : nullptr,
-                         brows, bmin, slots, table.p, tb, (uint32_t*)flag.p, s));
   uint32_t err = 0;
-  MTL_TRY(mtl_memcpy_d2h(&err, flag.p, 4, s));
+  uint64_t tb = 0;
+  for (;;) {
+    tb = mtl_join_table_bytes(strategy, brows, dense ? span : 0);
+    MTL_TRY(table.alloc(tb));
+    MTL_TRY(mtl_memcpy_h2d(flag.p, &zero, 4, s));
+    MTL_TRY(mtl_join_build(strategy, bkeys, btype, bsel, bsel ? (const uint32_t*)cnt.p : nullptr,
+                           brows, bmin, slots, table.p, tb, (uint32_t*)flag.p, s));
+    MTL_TRY(mtl_memcpy_d2h(&err, flag.p, 4, s));
+    if (!(err & 16u) || strategy != MTL_JOIN_RADIX) break;
+    strategy = MTL_JOIN_HASH;  // a partition over the LDS table: the global table
+  }
   if (err & 2u) return set_error(MTL_UNSUPPORTED, "join with duplicate keys on the build side");
   if (err & 8u) return set_error(MTL_ERR_PROCESSING, "join key outside the column's stated range");
   DevBuf pcnt;
@@ -1952,7 +1962,8 @@ static int32_t join_pairs(const void* bkeys, int32_t btype, const uint32_t* bsel
 static int32_t join_pairs_multi(const void* bkeys, int32_t btype, const uint32_t* bsel,
                                 uint64_t brows, int64_t bmin, int64_t bmax, const void* pkeys,
                                 int32_t ptype, const uint32_t* psel, uint64_t prows,
-                                DevBuf& out_p, DevBuf& out_b, uint64_t& n_pairs, mtl_stream* s) {
+                                DevBuf& out_p, DevBuf& out_b, uint64_t& n_pairs, mtl_stream* s,
+                                bool ordered = true) {
   n_pairs = 0;
   if (brows == 0 || prows == 0) {
     MTL_TRY(out_p.alloc(4));
@@ -1988,7 +1999,7 @@ static int32_t join_pairs_multi(const void* bkeys, int32_t btype, const uint32_t
   DevBuf pp, pr;
   uint64_t np = 0;
   MTL_TRY(join_pairs(rkey.p, MTL_BIGINT, nullptr, runs, bmin, bmax, pkeys, ptype, psel, prows, pp,
-                     pr, np, s));
+                     pr, np, s, ordered));
   // expand each pair to its run's rows
   const uint64_t ptiles = (np + 4095) / 4096 + 1;
   DevBuf ptb, total;
@@ -2436,16 +2447,16 @@ static int32_t run_join(const Query& q, Bound& b, mtl_result& res, mtl_stream* s
     uint64_t np = 0;
     // build over the new table's rows (its key unique), probe with the relation
     int32_t st = join_pairs(ycol, ytype, (const uint32_t*)sel[c]->p, nsel[c], ymin, ymax, kx.p,
-                            xtype, nullptr, R.m, pp, pb, np, s);
+                            xtype, nullptr, R.m, pp, pb, np, s, !grouped);
     if (st == MTL_UNSUPPORTED) {
       // build over the relation (its key unique), probe with the new table
       DevBuf pp2, pb2;
       st = join_pairs(kx.p, xtype, nullptr, R.m, xmin, xmax, ycol, ytype,
-                      (const uint32_t*)sel[c]->p, nsel[c], pp2, pb2, np, s);
+                      (const uint32_t*)sel[c]->p, nsel[c], pp2, pb2, np, s, !grouped);
       if (st == MTL_UNSUPPORTED) {
         // keys repeat on both sides: many-to-many, the new table sorted into runs
         st = join_pairs_multi(ycol, ytype, (const uint32_t*)sel[c]->p, nsel[c], ymin, ymax, kx.p,
-                              xtype, nullptr, R.m, pp, pb, np, s);
+                              xtype, nullptr, R.m, pp, pb, np, s, !grouped);
       } else {
         MTL_TRY(st);
         std::swap(pp.p, pb2.p);

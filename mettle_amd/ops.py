@@ -140,7 +140,8 @@ def project(col: torch.Tensor, t: int, sel: torch.Tensor | None, op: int, value)
 def join(strategy: int, bkeys: torch.Tensor, btype: int, pkeys: torch.Tensor, ptype: int,
          min_key: int = 0, slots: int = 0, bsel: torch.Tensor | None = None,
          psel: torch.Tensor | None = None):
-    """Build on bkeys, probe with pkeys; returns (probe rows, build rows)."""
+    """Build on bkeys, probe with pkeys; returns (probe rows, build rows)
+    (JOIN_RADIX: in partition order)."""
     torch.cuda.synchronize()
     L = abi.lib()
     bn = bsel.numel() if bsel is not None else bkeys.numel()
@@ -153,6 +154,8 @@ def join(strategy: int, bkeys: torch.Tensor, btype: int, pkeys: torch.Tensor, pt
     sync()
     if int(dup.item()) & 2:
         raise abi.Unsupported(abi.MTL_UNSUPPORTED, "duplicate build key")
+    if int(dup.item()) & 16:
+        raise abi.Unsupported(abi.MTL_UNSUPPORTED, "radix partition over 3072 rows")
     if int(dup.item()) & 8:
         raise abi.MettleError(abi.MTL_ERR_INVALID_ARGUMENT, "build key outside the dense span")
     out_p = torch.empty(max(pn, 4), dtype=torch.int32, device=device())

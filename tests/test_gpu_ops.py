@@ -323,7 +323,65 @@ def test_join_hash_extreme_keys(ops):
     assert list(ops.to_host(gb, np.uint32)) == [2, 1, 0, 3, 1]
 
 
-@pytest.mark.parametrize("strategy", [abi.JOIN_DENSE, abi.JOIN_HASH])
+def _by_probe(gp, gb):
+    p = gp.astype(np.int64)
+    o = np.argsort(p, kind="stable")
+    return p[o], gb[o]
+
+
+@pytest.mark.parametrize("t,dtype", [(abi.INTEGER, np.int32), (abi.BIGINT, np.int64)])
+@pytest.mark.parametrize("nb,npr", [(1, 1000), (2048, 100_000), (2049, 50_000),
+                                    (50_000, 400_000), (3_000_000, 6_000_000)])
+def test_join_radix_parity(ops, t, dtype, nb, npr):
+    """LDS-staged partitions: every (probe, build) pair of the oracle's join
+    (partition order, so compared by probe row); one partition at 2048 build
+    rows, 2 at 2049, 2048 partitions at 3M."""
+    rng = np.random.default_rng(nb + npr + t)
+    span = 3 * nb + 10
+    base = -(span // 2) if t == abi.INTEGER else -(1 << 40)
+    bk = (base + rng.permutation(span)[:nb]).astype(dtype)
+    pk = (base + rng.integers(-5, span + 5, npr)).astype(dtype)
+    ref_p, ref_b = orc.join(bk, t, pk, t)
+    gp, gb = ops.join(abi.JOIN_RADIX, ops.to_device(bk), t, ops.to_device(pk), t)
+    p, b = _by_probe(ops.to_host(gp, np.uint32), ops.to_host(gb, np.uint32))
+    np.testing.assert_array_equal(p, ref_p.astype(np.int64))
+    np.testing.assert_array_equal(b, ref_b)
+
+
+def test_join_radix_selections(ops):
+    rng = np.random.default_rng(9)
+    nb, npr = 40_000, 300_000
+    bk = (rng.choice(1 << 27, nb, replace=False) - (1 << 26)).astype(np.int64)
+    pk = bk[rng.integers(0, nb, npr)].copy()
+    pk[rng.random(npr) < 0.3] = 12345678  # misses (not a build key)
+    bk[bk == 12345678] = 12345679
+    bsel = np.sort(rng.choice(nb, nb // 2, replace=False)).astype(np.uint32)
+    psel = np.sort(rng.choice(npr, npr // 3, replace=False)).astype(np.uint32)
+    ref_p, ref_b = orc.join(bk, abi.BIGINT, pk, abi.BIGINT, bsel, psel)
+    gp, gb = ops.join(abi.JOIN_RADIX, ops.to_device(bk), abi.BIGINT, ops.to_device(pk),
+                      abi.BIGINT, bsel=ops.to_device(bsel.view(np.int32)),
+                      psel=ops.to_device(psel.view(np.int32)))
+    p, b = _by_probe(ops.to_host(gp, np.uint32), ops.to_host(gb, np.uint32))
+    np.testing.assert_array_equal(p, ref_p.astype(np.int64))
+    np.testing.assert_array_equal(b, ref_b)
+
+
+@pytest.mark.parametrize("nb", [4, 5000])
+def test_join_radix_extreme_keys(ops, nb):
+    # INT64_MIN (the empty sentinel) and -1 are keys, in one partition or many
+    rng = np.random.default_rng(nb)
+    extra = rng.choice(10**9, nb - 4, replace=False).astype(np.int64) + 10
+    bk = np.concatenate([np.array([np.iinfo(np.int64).min, -1, 0, np.iinfo(np.int64).max],
+                                  dtype=np.int64), extra])
+    pk = np.array([0, -1, 5, np.iinfo(np.int64).min, np.iinfo(np.int64).max, -1], dtype=np.int64)
+    gp, gb = ops.join(abi.JOIN_RADIX, ops.to_device(bk), abi.BIGINT, ops.to_device(pk),
+                      abi.BIGINT)
+    p, b = _by_probe(ops.to_host(gp, np.uint32), ops.to_host(gb, np.uint32))
+    assert list(p) == [0, 1, 3, 4, 5]
+    assert list(b) == [2, 1, 0, 3, 1]
+
+
+@pytest.mark.parametrize("strategy", [abi.JOIN_DENSE, abi.JOIN_HASH, abi.JOIN_RADIX])
 def test_join_duplicate_build_key_reported(ops, strategy):
     bk = np.array([3, 7, 3], dtype=np.int32)
     with pytest.raises(abi.Unsupported):

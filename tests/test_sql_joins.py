@@ -394,3 +394,53 @@ def test_order_by_aggregate_then_group_keys(ops):
     assert _ints(out, "c_region") == [int(x) for x in g.c_region]
     assert _ints(out, "o_prio") == [int(x) for x in g.o_prio]
     assert _ints(out, "q") == [int(x) for x in g.q]
+
+
+def _profiled_kernels(fn):
+    """Names of the library kernels fn() launched (the library's profiler)."""
+    import ctypes as C
+    L = abi.lib()
+    L.mtl_profile_reset()
+    L.mtl_profile_enable(1)
+    try:
+        out = fn()
+        abi.check(L.mtl_stream_sync(None))  # resolves the recorded launches
+    finally:
+        L.mtl_profile_enable(0)
+    names, i = [], 0
+    name, ms, cnt = C.create_string_buffer(128), C.c_double(), C.c_uint64()
+    while L.mtl_profile_get(i, name, 128, C.byref(ms), C.byref(cnt)) == 0:
+        names.append(name.value.decode())
+        i += 1
+    L.mtl_profile_reset()
+    return out, names
+
+
+@pytest.mark.parametrize("nb", [5_000, 300_000])
+def test_grouped_join_on_sparse_keys_uses_radix_partitions(ops, nb):
+    """A grouped join whose build keys are too sparse for the dense table
+    takes the LDS-staged radix partitions (pairs in partition order, which
+    aggregation does not see); a projection keeps the probe-ordered hash
+    join.  Both against pandas."""
+    rng = np.random.default_rng(nb)
+    npr = 1_000_000
+    ak = (rng.choice(10**12, nb, replace=False) - 5 * 10**11).astype(np.int64)
+    ag = rng.integers(0, 40, nb).astype(np.int32)
+    bk = ak[rng.integers(0, nb, npr)]
+    bk[rng.random(npr) < 0.2] = 10**12 + 7  # no match
+    bv = rng.integers(-10**6, 10**6, npr).astype(np.int64)
+    _table(ops, "a", [("k", abi.BIGINT, ak, True), ("g", abi.INTEGER, ag)])
+    _table(ops, "b", [("k", abi.BIGINT, bk), ("v", abi.BIGINT, bv)])
+    q = "SELECT a.g, COUNT(*) AS n, SUM(b.v) AS s FROM a JOIN b ON a.k = b.k GROUP BY a.g ORDER BY a.g"
+    out, names = _profiled_kernels(lambda: S.execute(q))
+    assert "k_rx_join" in names and "k_probe" not in names
+    j = pd.DataFrame({"k": bk, "v": bv}).merge(pd.DataFrame({"k": ak, "g": ag}), on="k")
+    g = j.groupby("g").v.agg(["size", "sum"])
+    assert _ints(out, "g") == list(g.index)
+    assert _ints(out, "n") == [int(x) for x in g["size"]]
+    assert _ints(out, "s") == [int(x) for x in g["sum"]]
+    out, names = _profiled_kernels(lambda: S.execute(
+        "SELECT b.v, a.g FROM a JOIN b ON a.k = b.k WHERE b.v > 999000"))
+    assert "k_rx_join" not in names
+    m = j[j.v > 999000]
+    assert sorted(zip(_ints(out, "v"), _ints(out, "g"))) == sorted(zip(m.v, m.g))
