@@ -269,10 +269,12 @@ struct RxView {
   uint32_t* cursor;  // P
   int64_t* keys;
   int32_t* rows;
+  int64_t* tkeys;    // first-level partitions (two-level scatter)
+  int32_t* trows;
 };
 inline uint64_t rx_bytes(uint64_t n) {
   const uint64_t P = 1ull << rx_bits(n);
-  return 256 + al256((P + 1) * 4) + al256(P * 4) + al256(n * 8) + al256(n * 4);
+  return 256 + al256((P + 1) * 4) + al256(P * 4) + 2 * (al256(n * 8) + al256(n * 4));
 }
 inline RxView rx_view(void* base, uint64_t n) {
   RxView v;
@@ -286,6 +288,10 @@ inline RxView rx_view(void* base, uint64_t n) {
   v.keys = (int64_t*)p;
   p += al256(n * 8);
   v.rows = (int32_t*)p;
+  p += al256(n * 4);
+  v.tkeys = (int64_t*)p;
+  p += al256(n * 8);
+  v.trows = (int32_t*)p;
   return v;
 }
 
@@ -322,35 +328,196 @@ __global__ __launch_bounds__(RX_BLOCK) void k_rx_hist(RxSide s, uint32_t bits, u
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
-// cursor[p] starts at the partition's offset; each block reserves its
-// rows of every partition with one atomic per partition, then places them.
-// Writes (key, row id) for the build side or (key, probe row) for the probe.
-__global__ __launch_bounds__(RX_BLOCK) void k_rx_scatter(RxSide s, uint32_t bits, uint64_t chunk,
-                                                        uint32_t* cursor, int64_t* okeys,
-                                                        int32_t* orows) {
-  __shared__ uint32_t h[1u << kRxMaxBits];
+// One level of the partition scatter, staged through LDS so that a wave
+// writes whole runs: bucket(key) = (fmix64(key) >> shift) & (nb - 1), at
+// most kRxSplitMax buckets per level (8192 partitions are written in two
+// levels, 64 then 128: one level of 8192 runs leaves every store on its own
+// cache line).  Each block takes a range of rows: it counts its buckets,
+// reserves its run in each with one atomic per bucket (cursor[base + b]),
+// then, 2048 rows at a time, sorts the rows by bucket in LDS and writes
+// every bucket's piece with consecutive lanes on consecutive addresses.
+// Level 1 reads the side (RxSide); level 2 the first level's arrays, block
+// b taking slice b % slices of first-level partition b / slices.
+constexpr uint32_t kRxSplitMax = 128;
+constexpr uint32_t kRxSub = 2048;
+
+struct RxSplit {
+  RxSide raw;              // level 1 input (in_keys == nullptr)
+  const int64_t* in_keys;  // level 2 input
+  const int32_t* in_rows;
+  const uint32_t* in_off;  // level 2: first-level partition q starts at in_off[q * nb]
+  uint32_t slices;         // level 2: blocks per first-level partition
+  uint64_t chunk;          // level 1: rows per block
+  uint32_t shift, nb;
+  uint32_t* cursor;        // next free position of each bucket (base + bucket)
+  int64_t* out_keys;
+  int32_t* out_rows;
+};
+
+__global__ __launch_bounds__(RX_BLOCK) void k_rx_split(RxSplit a) {
+  __shared__ uint32_t s_cnt[kRxSplitMax], s_cur[kRxSplitMax], s_start[kRxSplitMax];
+  __shared__ int64_t s_key[kRxSub];
+  __shared__ int32_t s_row[kRxSub];
+  __shared__ uint8_t s_b[kRxSub];
+  uint64_t lo, hi;
+  uint32_t base = 0;
+  if (a.in_keys) {
+    const uint32_t q = blockIdx.x / a.slices, sl = blockIdx.x % a.slices;
+    const uint64_t q0 = a.in_off[(uint64_t)q * a.nb], q1 = a.in_off[(uint64_t)(q + 1) * a.nb];
+    lo = q0 + (q1 - q0) * sl / a.slices;
+    hi = q0 + (q1 - q0) * (sl + 1) / a.slices;
+    base = q * a.nb;
+  } else {
+    const uint64_t n = rx_n(a.raw);
+    lo = (uint64_t)blockIdx.x * a.chunk;
+    hi = lo + a.chunk < n ? lo + a.chunk : n;
+  }
+  if (lo >= hi) return;  // block-uniform
+  auto fetch = [&](uint64_t i, int64_t& key, int32_t& row) {
+    if (a.in_keys) {
+      key = a.in_keys[i];
+      row = a.in_rows[i];
+    } else {
+      row = (int32_t)(a.raw.sel ? a.raw.sel[i] : (uint32_t)i);
+      key = load_key(a.raw.keys, a.raw.type, (uint32_t)row);
+    }
+  };
+  auto bucket = [&](int64_t key) {
+    return (uint32_t)(fmix64((uint64_t)key) >> a.shift) & (a.nb - 1);
+  };
+  for (uint32_t i = threadIdx.x; i < a.nb; i += RX_BLOCK) s_cnt[i] = 0;
+  __syncthreads();
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += RX_BLOCK) {
+    int64_t key;
+    int32_t row;
+    fetch(i, key, row);
+    atomicAdd(&s_cnt[bucket(key)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < a.nb; i += RX_BLOCK)
+    s_cur[i] = s_cnt[i] ? atomicAdd(&a.cursor[base + i], s_cnt[i]) : 0u;
+  __syncthreads();
+  for (uint64_t t0 = lo; t0 < hi; t0 += kRxSub) {
+    const uint32_t m = (uint32_t)(hi - t0 < kRxSub ? hi - t0 : kRxSub);
+    for (uint32_t i = threadIdx.x; i < a.nb; i += RX_BLOCK) s_cnt[i] = 0;
+    __syncthreads();
+    constexpr int PER = kRxSub / RX_BLOCK;
+    int64_t key[PER];
+    int32_t row[PER];
+    uint32_t b[PER], rank[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const uint32_t j = (uint32_t)u * RX_BLOCK + threadIdx.x;
+      if (j < m) {
+        fetch(t0 + j, key[u], row[u]);
+        b[u] = bucket(key[u]);
+        rank[u] = atomicAdd(&s_cnt[b[u]], 1u);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < WAVE) {  // bucket starts within the staged rows (nb <= 128)
+      const uint32_t c0 = threadIdx.x < a.nb ? s_cnt[threadIdx.x] : 0u;
+      const uint32_t c1 = threadIdx.x + WAVE < a.nb ? s_cnt[threadIdx.x + WAVE] : 0u;
+      const uint32_t i0 = wave_inclusive_u32(c0);
+      const uint32_t tot0 = __shfl(i0, WAVE - 1, WAVE);
+      const uint32_t i1 = wave_inclusive_u32(c1);
+      if (threadIdx.x < a.nb) s_start[threadIdx.x] = i0 - c0;
+      if (threadIdx.x + WAVE < a.nb) s_start[threadIdx.x + WAVE] = tot0 + i1 - c1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const uint32_t j = (uint32_t)u * RX_BLOCK + threadIdx.x;
+      if (j < m) {
+        const uint32_t at = s_start[b[u]] + rank[u];
+        s_key[at] = key[u];
+        s_row[at] = row[u];
+        s_b[at] = (uint8_t)b[u];
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += RX_BLOCK) {
+      const uint32_t bb = s_b[i];
+      const uint32_t at = s_cur[bb] + (i - s_start[bb]);
+      a.out_keys[at] = s_key[i];
+      a.out_rows[at] = s_row[i];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.nb; i += RX_BLOCK) s_cur[i] += s_cnt[i];
+    __syncthreads();
+  }
+}
+
+// cursor[q] = off[q * stride] for q < n: the first level's cursors.
+__global__ void k_rx_cursor1(const uint32_t* off, uint32_t stride, uint32_t n, uint32_t* cursor) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) cursor[q] = off[(uint64_t)q * stride];
+}
+
+// Partition side `side` (its P = 2^bits partitions' offsets `off` already
+// scanned) into (keys, rows): one level up to kRxSplitMax partitions, else
+// two (the first into (tkeys, trows)).  cursor: P words of scratch.
+int32_t rx_partition(const RxSide& side, uint64_t cap, uint32_t bits, const uint32_t* off,
+                     uint32_t* cursor, int64_t* keys, int32_t* rows, int64_t* tkeys,
+                     int32_t* trows, mtl_stream* s) {
   const uint32_t P = 1u << bits;
-  const uint64_t n = rx_n(s);
-  const uint64_t lo = (uint64_t)blockIdx.x * chunk;
-  const uint64_t hi = lo + chunk < n ? lo + chunk : n;
-  if (lo >= n) return;  // uniform per block
-  for (uint32_t i = threadIdx.x; i < P; i += RX_BLOCK) h[i] = 0;
-  __syncthreads();
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += RX_BLOCK) {
-    const uint32_t row = s.sel ? s.sel[i] : (uint32_t)i;
-    atomicAdd(&h[rx_part(load_key(s.keys, s.type, row), bits)], 1u);
+  const uint32_t b1 = bits <= 7 ? bits : 6;  // first level: 2^b1 buckets
+  const uint32_t b2 = bits - b1;
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(ceil_div(cap, RX_BLOCK * 16), (uint64_t)cu_count() * 4));
+  const uint64_t chunk = std::max<uint64_t>(kRxSub, ceil_div(cap, grid));
+  RxSplit a{};
+  a.raw = side;
+  a.chunk = chunk;
+  if (b2 == 0) {  // one level straight into the partitions
+    MTL_HIP(hipMemcpyAsync(cursor, off, P * 4ull, hipMemcpyDeviceToDevice, s->stream));
+    a.shift = bits ? 64 - bits : 0;  // one partition: bucket = h & 0
+    a.nb = P;
+    a.cursor = cursor;
+    a.out_keys = keys;
+    a.out_rows = rows;
+    ProfScope ps(s, "k_rx_split");
+    hipLaunchKernelGGL(k_rx_split, dim3((uint32_t)ceil_div(std::max<uint64_t>(cap, 1), chunk)),
+                       dim3(RX_BLOCK), 0, s->stream, a);
+    MTL_HIP(hipGetLastError());
+    return MTL_OK;
   }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < P; i += RX_BLOCK)
-    if (h[i]) h[i] = atomicAdd(&cursor[i], h[i]);
-  __syncthreads();
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += RX_BLOCK) {
-    const uint32_t row = s.sel ? s.sel[i] : (uint32_t)i;
-    const int64_t key = load_key(s.keys, s.type, row);
-    const uint32_t at = atomicAdd(&h[rx_part(key, bits)], 1u);
-    okeys[at] = key;
-    orows[at] = (int32_t)row;
+  // level 1: the first-level partition q holds final partitions
+  // [q 2^b2, (q + 1) 2^b2); its cursor is the first of them, read through
+  // a stride of 2^b2 (the final cursors double as the first level's)
+  const uint32_t nb2 = 1u << b2;
+  a.shift = 64 - b1;
+  a.nb = 1u << b1;
+  a.cursor = cursor;  // bucket q uses cursor[q]: filled from off[q * nb2] below
+  a.out_keys = tkeys;
+  a.out_rows = trows;
+  {
+    hipLaunchKernelGGL(k_rx_cursor1, dim3(1), dim3(1u << b1), 0, s->stream, off, nb2, 1u << b1,
+                       cursor);
+    MTL_HIP(hipGetLastError());
+    ProfScope ps(s, "k_rx_split");
+    hipLaunchKernelGGL(k_rx_split, dim3((uint32_t)ceil_div(std::max<uint64_t>(cap, 1), chunk)),
+                       dim3(RX_BLOCK), 0, s->stream, a);
+    MTL_HIP(hipGetLastError());
   }
+  // level 2: each first-level partition into its 2^b2 final partitions
+  MTL_HIP(hipMemcpyAsync(cursor, off, P * 4ull, hipMemcpyDeviceToDevice, s->stream));
+  RxSplit c{};
+  c.in_keys = tkeys;
+  c.in_rows = trows;
+  c.in_off = off;
+  c.slices = (uint32_t)std::max<uint64_t>(1, (uint64_t)cu_count() * 4 >> b1);
+  c.shift = 64 - bits;
+  c.nb = nb2;
+  c.cursor = cursor;
+  c.out_keys = keys;
+  c.out_rows = rows;
+  {
+    ProfScope ps(s, "k_rx_split");
+    hipLaunchKernelGGL(k_rx_split, dim3((1u << b1) * c.slices), dim3(RX_BLOCK), 0, s->stream, c);
+    MTL_HIP(hipGetLastError());
+  }
+  return MTL_OK;
 }
 
 // The LDS table of one partition: kRxSlots keys (kEmpty = free; the key
@@ -434,10 +601,13 @@ struct RxProbe {
   uint32_t* err;
 };
 
-// Matches collect in an LDS buffer and leave in runs of up to kRxOut
-// pairs, one global reservation per run (a reservation per wave would
-// serialise millions of atomics on out_count).
-constexpr uint32_t kRxOut = 1024;
+// Matches collect in an LDS buffer and leave in runs of at least
+// kRxOut - kRxStep pairs, one global reservation per run (a reservation per
+// wave would serialise millions of atomics on out_count).  Each step takes
+// kRxStep probe rows, eight per thread with their loads in flight together.
+constexpr uint32_t kRxOut = 3072;
+constexpr int kRxPer = 8;
+constexpr uint32_t kRxStep = RX_BLOCK * kRxPer;
 
 __global__ __launch_bounds__(RX_BLOCK) void k_rx_join(RxProbe a) {
   __shared__ RxLds t;
@@ -460,7 +630,7 @@ __global__ __launch_bounds__(RX_BLOCK) void k_rx_join(RxProbe a) {
   }
   const uint64_t len = hi - lo;
   const uint64_t s0 = lo + len * sl / a.slices, s1 = lo + len * (sl + 1) / a.slices;
-  const int wave = threadIdx.x / WAVE;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
   if (threadIdx.x == 0) s_fill = 0;
   __syncthreads();
   auto flush = [&](uint32_t n) {  // every thread; s_fill == n
@@ -474,39 +644,52 @@ __global__ __launch_bounds__(RX_BLOCK) void k_rx_join(RxProbe a) {
     if (threadIdx.x == 0) s_fill = 0;
     __syncthreads();
   };
-  for (uint64_t b = s0; b < s1; b += RX_BLOCK) {  // the same iterations in every thread
-    const uint64_t i = b + threadIdx.x;
-    int32_t m = -1;
-    uint32_t prow = 0;
-    if (i < s1) {
-      int64_t key;
-      if (a.poff) {
-        key = a.pkeys[i];
-        prow = (uint32_t)a.prows[i];
-      } else {
-        prow = a.raw.sel ? a.raw.sel[i] : (uint32_t)i;
-        key = load_key(a.raw.keys, a.raw.type, prow);
+  for (uint64_t b = s0; b < s1; b += kRxStep) {  // the same iterations in every thread
+    int64_t key[kRxPer];
+    uint32_t prow[kRxPer];
+#pragma unroll
+    for (int u = 0; u < kRxPer; ++u) {
+      const uint64_t i = b + (uint64_t)u * RX_BLOCK + threadIdx.x;
+      key[u] = 0;
+      prow[u] = 0;
+      if (i < s1) {
+        if (a.poff) {
+          key[u] = a.pkeys[i];
+          prow[u] = (uint32_t)a.prows[i];
+        } else {
+          prow[u] = a.raw.sel ? a.raw.sel[i] : (uint32_t)i;
+          key[u] = load_key(a.raw.keys, a.raw.type, prow[u]);
+        }
       }
-      m = rx_lookup(t, key);
     }
-    const uint64_t bal = __ballot(m >= 0);
-    if (lane_id() == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+    int32_t m[kRxPer];
+    uint32_t c = 0;
+#pragma unroll
+    for (int u = 0; u < kRxPer; ++u) {
+      const uint64_t i = b + (uint64_t)u * RX_BLOCK + threadIdx.x;
+      m[u] = i < s1 ? rx_lookup(t, key[u]) : -1;
+      c += m[u] >= 0;
+    }
+    const uint32_t incl = wave_inclusive_u32(c);
+    if (lane == WAVE - 1) s_wave[wave] = incl;
     __syncthreads();
-    uint32_t before = s_fill, step = 0;
+    uint32_t at = s_fill + incl - c, step = 0;
 #pragma unroll
     for (int w = 0; w < RX_BLOCK / WAVE; ++w) {
-      before += w < wave ? s_wave[w] : 0;
+      at += w < wave ? s_wave[w] : 0;
       step += s_wave[w];
     }
-    if (m >= 0) {  // s_fill + step <= kRxOut: flushed below once over kRxOut - RX_BLOCK
-      const uint32_t at = before + (uint32_t)__popcll(bal & lanemask_lt());
-      s_buf_p[at] = prow;
-      s_buf_b[at] = (uint32_t)m;
-    }
+#pragma unroll
+    for (int u = 0; u < kRxPer; ++u)
+      if (m[u] >= 0) {  // s_fill + step <= kRxOut: flushed below once over kRxOut - kRxStep
+        s_buf_p[at] = prow[u];
+        s_buf_b[at] = (uint32_t)m[u];
+        ++at;
+      }
     __syncthreads();
     if (threadIdx.x == 0) s_fill += step;
     __syncthreads();
-    if (s_fill > kRxOut - RX_BLOCK) flush(s_fill);
+    if (s_fill > kRxOut - kRxStep) flush(s_fill);
   }
   flush(s_fill);
 }
@@ -526,14 +709,8 @@ int32_t rx_build(const void* keys, int32_t key_type, const uint32_t* sel, const 
   }
   scan_counts(v.off, P + 1, v.off + P, s->stream);  // exclusive; off[P] = n after the scan
   MTL_HIP(hipGetLastError());
-  MTL_HIP(hipMemcpyAsync(v.cursor, v.off, P * 4ull, hipMemcpyDeviceToDevice, s->stream));
-  const uint64_t chunk = std::max<uint64_t>(4096, ceil_div(capacity, grid));
-  {
-    ProfScope ps(s, "k_rx_scatter");
-    hipLaunchKernelGGL(k_rx_scatter, dim3((uint32_t)ceil_div(std::max<uint64_t>(capacity, 1), chunk)),
-                       dim3(RX_BLOCK), 0, s->stream, side, v.bits, chunk, v.cursor, v.keys, v.rows);
-    MTL_HIP(hipGetLastError());
-  }
+  MTL_TRY(rx_partition(side, capacity, v.bits, v.off, v.cursor, v.keys, v.rows, v.tkeys, v.trows,
+                       s));
   {
     ProfScope ps(s, "k_rx_check");
     hipLaunchKernelGGL(k_rx_check, dim3(P), dim3(RX_BLOCK), 0, s->stream, v.off, v.keys, v.rows,
@@ -563,11 +740,15 @@ int32_t rx_probe(const void* table, uint64_t build_cap, const void* keys, int32_
   MTL_HIP(hipMemsetAsync(out_count, 0, 4, s->stream));
   MTL_HIP(hipMemsetAsync(errp, 0, 4, s->stream));
   if (P > 1) {
-    void *poff, *pcur, *pk, *pr;
+    void *poff, *pcur, *pk, *pr, *tk = nullptr, *tr = nullptr;
     MTL_TRY(s->arena.alloc((P + 1) * 4ull, &poff));
     MTL_TRY(s->arena.alloc(P * 4ull, &pcur));
     MTL_TRY(s->arena.alloc(std::max<uint64_t>(capacity, 1) * 8, &pk));
     MTL_TRY(s->arena.alloc(std::max<uint64_t>(capacity, 1) * 4, &pr));
+    if (v.bits > 7) {  // two levels: the first level's arrays
+      MTL_TRY(s->arena.alloc(std::max<uint64_t>(capacity, 1) * 8, &tk));
+      MTL_TRY(s->arena.alloc(std::max<uint64_t>(capacity, 1) * 4, &tr));
+    }
     MTL_HIP(hipMemsetAsync(poff, 0, (P + 1) * 4ull, s->stream));
     const uint32_t grid = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>(ceil_div(capacity, RX_BLOCK * 16), (uint64_t)cu_count() * 4));
@@ -579,16 +760,8 @@ int32_t rx_probe(const void* table, uint64_t build_cap, const void* keys, int32_
     }
     scan_counts((uint32_t*)poff, P + 1, (uint32_t*)poff + P, s->stream);
     MTL_HIP(hipGetLastError());
-    MTL_HIP(hipMemcpyAsync(pcur, poff, P * 4ull, hipMemcpyDeviceToDevice, s->stream));
-    const uint64_t chunk = std::max<uint64_t>(4096, ceil_div(capacity, grid));
-    {
-      ProfScope ps(s, "k_rx_scatter");
-      hipLaunchKernelGGL(k_rx_scatter,
-                         dim3((uint32_t)ceil_div(std::max<uint64_t>(capacity, 1), chunk)),
-                         dim3(RX_BLOCK), 0, s->stream, a.raw, v.bits, chunk, (uint32_t*)pcur,
-                         (int64_t*)pk, (int32_t*)pr);
-      MTL_HIP(hipGetLastError());
-    }
+    MTL_TRY(rx_partition(a.raw, capacity, v.bits, (const uint32_t*)poff, (uint32_t*)pcur,
+                         (int64_t*)pk, (int32_t*)pr, (int64_t*)tk, (int32_t*)tr, s));
     a.poff = (const uint32_t*)poff;
     a.pkeys = (const int64_t*)pk;
     a.prows = (const int32_t*)pr;
