@@ -23,6 +23,7 @@
 #include <mutex>
 #include <string>
 #include <string_view>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -60,6 +61,8 @@ struct ColOut {
   std::string name;
   int32_t type = 0, scale = 0;
   std::vector<uint8_t> data;
+  const void* direct = nullptr;  // one batch needing no conversion: its own value buffer
+  uint64_t bytes = 0;            // of the values
   std::vector<uint8_t> valid;  // one byte per row; empty until a NULL is seen
   bool any_null = false;
   int64_t min = INT64_MAX, max = INT64_MIN;
@@ -196,6 +199,7 @@ void convert(const ArrowSchema* cs, ArrowArray* const* batches, uint32_t nb, int
     const bool narrow = vals.size() <= 256;
     out.type = narrow ? MTL_DICT8 : MTL_DICT32;
     out.data.assign(rows * (narrow ? 1 : 4), 0);
+    out.bytes = out.data.size();
     for_rows([&](uint64_t r, const ArrowArray* a, int64_t i, bool ok) {
       if (!ok) return mark_null(r);
       const uint32_t c = code[value(a, i)];
@@ -206,10 +210,60 @@ void convert(const ArrowSchema* cs, ArrowArray* const* batches, uint32_t nb, int
     out.max = vals.empty() ? 0 : (int64_t)vals.size() - 1;
     return;
   }
+  // a batch without NULLs whose values keep their width: one copy of its
+  // value range (and, for integers, a min/max loop the compiler vectorises)
+  auto bulk = [&](int w, bool minmax) {
+    uint64_t r0 = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+      const ArrowArray* a = batches[b]->children[ci];
+      if (a->null_count != 0 && a->buffers[0]) return false;
+    }
+    out.bytes = rows * (uint64_t)w;
+    if (nb == 1) {  // upload straight from the batch's buffer
+      const ArrowArray* a = batches[0]->children[ci];
+      out.direct = (const uint8_t*)a->buffers[1] + (batches[0]->offset + a->offset) * w;
+      out.data.clear();
+      out.data.shrink_to_fit();
+    } else {
+      out.data.resize(out.bytes);
+    }
+    for (uint32_t b = 0; b < nb; ++b) {
+      const ArrowArray* a = batches[b]->children[ci];
+      const int64_t base = batches[b]->offset + a->offset, len = batches[b]->length;
+      if (!out.direct)
+        std::memcpy(out.data.data() + r0 * w, (const uint8_t*)a->buffers[1] + base * w,
+                    (size_t)len * w);
+      if (minmax) {
+        int64_t lo = out.min, hi = out.max;
+        if (w == 4) {
+          const int32_t* v = (const int32_t*)((const uint8_t*)a->buffers[1]) + base;
+          for (int64_t i = 0; i < len; ++i) {
+            lo = v[i] < lo ? v[i] : lo;
+            hi = v[i] > hi ? v[i] : hi;
+          }
+        } else {
+          const int64_t* v = (const int64_t*)((const uint8_t*)a->buffers[1]) + base;
+          for (int64_t i = 0; i < len; ++i) {
+            lo = v[i] < lo ? v[i] : lo;
+            hi = v[i] > hi ? v[i] : hi;
+          }
+        }
+        out.min = lo;
+        out.max = hi;
+      }
+      r0 += (uint64_t)len;
+    }
+    return true;
+  };
   if (st.src == S_F32 || st.src == S_F64) {
     const int w = st.src == S_F32 ? 4 : 8;
     out.type = st.src == S_F32 ? MTL_FLOAT : MTL_DOUBLE;
+    if (bulk(w, false)) {
+      out.min = out.max = 0;
+      return;
+    }
     out.data.assign(rows * w, 0);
+    out.bytes = rows * (uint64_t)w;
     for_rows([&](uint64_t r, const ArrowArray* a, int64_t i, bool ok) {
       if (!ok) return mark_null(r);
       std::memcpy(out.data.data() + r * w, (const uint8_t*)a->buffers[1] + i * w, w);
@@ -221,6 +275,7 @@ void convert(const ArrowSchema* cs, ArrowArray* const* batches, uint32_t nb, int
     out.type = MTL_DECIMAL;
     out.scale = st.scale;
     out.data.assign(rows * 8, 0);
+    out.bytes = out.data.size();
     for_rows([&](uint64_t r, const ArrowArray* a, int64_t i, bool ok) {
       if (!out.skip.empty()) return;
       if (!ok) return mark_null(r);
@@ -246,7 +301,13 @@ void convert(const ArrowSchema* cs, ArrowArray* const* batches, uint32_t nb, int
                                                       : MTL_INTEGER;
   if (st.src == S_DATE64) wide = false;
   const int w = wide ? 8 : 4;
+  const bool same = st.src == S_I32 || st.src == S_DATE32 || st.src == S_I64 ||
+                    (st.src == S_TS && st.ts_mul == 1);
+  if (same && bulk(w, true)) return;
   out.data.assign(rows * w, 0);
+  out.bytes = rows * (uint64_t)w;
+  out.min = INT64_MAX;
+  out.max = INT64_MIN;
   for_rows([&](uint64_t r, const ArrowArray* a, int64_t i, bool ok) {
     if (!out.skip.empty()) return;
     if (!ok) return mark_null(r);
@@ -323,19 +384,29 @@ extern "C" int32_t mtl_catalog_ingest_arrow(const char* table, const ArrowSchema
     catalog_drop_table(name.c_str());
     return st;
   };
-  auto upload = [&](uint32_t col_id, const std::vector<uint8_t>& bytes,
+  auto upload = [&](uint32_t col_id, const void* host, uint64_t bytes,
                     const void** dev) -> int32_t {
     static const uint8_t zeros[16] = {0};
-    const void* host = bytes.empty() ? (const void*)zeros : (const void*)bytes.data();
-    MTL_TRY(mtl_cache_put(tid, col_id, host, bytes.empty() ? 16 : bytes.size(), version, dev));
+    if (!bytes) host = zeros;
+    MTL_TRY(mtl_cache_put(tid, col_id, host, bytes ? bytes : 16, version, dev));
     held.push_back(col_id);
     return MTL_OK;
   };
   uint64_t total = 0;
   std::string skip;
-  for (int64_t ci = 0; ci < schema->n_children; ++ci) {
-    ColOut c;
-    convert(schema->children[ci], batches, n_batches, (int)ci, rows, c);
+  // convert the columns on host threads (at most 8 at a time), then upload
+  const int64_t nc = schema->n_children;
+  std::vector<ColOut> conv((size_t)nc);
+  for (int64_t c0 = 0; c0 < nc; c0 += 8) {
+    std::vector<std::thread> th;
+    for (int64_t ci = c0; ci < std::min<int64_t>(nc, c0 + 8); ++ci)
+      th.emplace_back([&, ci] {
+        convert(schema->children[ci], batches, n_batches, (int)ci, rows, conv[(size_t)ci]);
+      });
+    for (auto& t : th) t.join();
+  }
+  for (int64_t ci = 0; ci < nc; ++ci) {
+    ColOut c = std::move(conv[(size_t)ci]);
     if (!c.skip.empty()) {
       skip += (skip.empty() ? "" : ",") + c.name;
       continue;
@@ -343,8 +414,10 @@ extern "C" int32_t mtl_catalog_ingest_arrow(const char* table, const ArrowSchema
     if (c.min > c.max) c.min = c.max = 0;  // no valid value
     const void* dev = nullptr;
     const void* vdev = nullptr;
-    int32_t st = upload((uint32_t)ci, c.data, &dev);
-    if (st == MTL_OK && c.any_null) st = upload((uint32_t)ci | 0x80000000u, c.valid, &vdev);
+    int32_t st = upload((uint32_t)ci, c.direct ? c.direct : (const void*)c.data.data(), c.bytes,
+                        &dev);
+    if (st == MTL_OK && c.any_null)
+      st = upload((uint32_t)ci | 0x80000000u, c.valid.data(), c.valid.size(), &vdev);
     bool unique = false;
     for (uint32_t u = 0; u < n_unique; ++u)
       if (unique_cols && unique_cols[u] && lower_name(unique_cols[u]) == c.name) unique = true;
@@ -360,7 +433,7 @@ extern "C" int32_t mtl_catalog_ingest_arrow(const char* table, const ArrowSchema
                                       (uint32_t)vals.size());
     }
     if (st != MTL_OK) return fail(st);
-    total += c.data.size() + c.valid.size();
+    total += c.bytes + c.valid.size();
   }
   ingested()[name] = {tid, held};
   if (out_rows) *out_rows = rows;
