@@ -13,6 +13,10 @@ BIGINT column and the double sum of the FLOAT column; exact integer sums also
 from numpy bincounts of the 16-bit halves), checked against every group the
 device emits and against its top-100; the same generator at 20M rows is
 checked group by group against the C oracle.
+
+Joins: lines 60M JOIN orders 15M through SQL on sparse BIGINT keys (the
+radix-partitioned join), grouped and ungrouped, against numpy over the
+generator's dense keys.
 """
 import numpy as np
 import pytest
@@ -211,3 +215,60 @@ def test_group_agg_20m_matches_oracle(ops):
     cnt = ro[1]
     assert np.all(np.abs(go[4] - ro[4]) <= FLOAT_REL * fa + 1e-300)
     assert np.all(np.abs(go[5] - ro[5]) <= FLOAT_REL * fa / cnt + 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# joins at the star workload's sizes (lines 60M, orders 15M) through SQL
+# ---------------------------------------------------------------------------
+def test_sql_join_grouped_at_full_size_sparse_keys(ops):
+    """lines 60M JOIN orders 15M on sparse BIGINT keys (the radix-partitioned
+    join: a grouped consumer, keys too sparse for the dense table), filtered
+    by o_date, grouped by o_prio; then the same query as a projection-free
+    ungrouped aggregate.  Expected from numpy over the generator's dense
+    keys (l_order indexes orders)."""
+    from mettle_amd import sql as S
+    print("[full-size] generating the star tables", flush=True)
+    _, orders, lines = datagen.star_tables()
+    mult = np.int64(2654435761)
+    o_sk = orders["o_key"].astype(np.int64) * mult - (1 << 50)
+    l_sk = lines["l_order"].astype(np.int64) * mult - (1 << 50)
+    keep = []
+
+    def reg(name, n, cols):
+        spec = []
+        for cname, t, v, uq in cols:
+            d = ops.to_device(np.ascontiguousarray(v))
+            keep.append(d)
+            spec.append((cname, t, d.data_ptr(), int(v.min()), int(v.max()), uq))
+        S.register_table(name, n, spec)
+
+    S.drop()
+    try:
+        reg("orders", len(o_sk), [("o_skey", abi.BIGINT, o_sk, True),
+                                  ("o_date", abi.DATE, orders["o_date"], False),
+                                  ("o_prio", abi.INTEGER, orders["o_prio"], False)])
+        reg("lines", len(l_sk), [("l_skey", abi.BIGINT, l_sk, False),
+                                 ("price", abi.BIGINT, lines["price"], False)])
+        cutoff = int(datagen.DAYS_1992_01_01 + 1170)
+        import datetime
+        day = (datetime.date(1970, 1, 1) + datetime.timedelta(days=cutoff)).isoformat()
+        out = S.execute(f"SELECT o.o_prio, COUNT(*) AS n, SUM(l.price) AS s FROM lines l "
+                        f"JOIN orders o ON l.l_skey = o.o_skey WHERE o.o_date < DATE '{day}' "
+                        f"GROUP BY o.o_prio ORDER BY o.o_prio")
+        oi = lines["l_order"] - 1
+        m = orders["o_date"][oi] < cutoff
+        prio = orders["o_prio"][oi][m]
+        n_exp = np.bincount(prio, minlength=5)
+        s_exp = np.bincount(prio, weights=lines["price"][m].astype(np.float64), minlength=5)
+        # exact integer sums (float64 weights are exact below 2^53: 6e14 max)
+        assert [int(x) for x in out["o_prio"][2]] == [p for p in range(5) if n_exp[p]]
+        assert [int(x) for x in out["n"][2]] == [int(n_exp[p]) for p in range(5) if n_exp[p]]
+        assert [int(x) for x in out["s"][2]] == [int(s_exp[p]) for p in range(5) if n_exp[p]]
+        out = S.execute(f"SELECT COUNT(*) AS n, MAX(l.price) AS hi FROM lines l JOIN orders o "
+                        f"ON l.l_skey = o.o_skey WHERE o.o_date < DATE '{day}'")
+        assert int(out["n"][2][0]) == int(m.sum())
+        assert int(out["hi"][2][0]) == int(lines["price"][m].max())
+    finally:
+        S.drop()
+        keep.clear()
+        torch.cuda.empty_cache()
