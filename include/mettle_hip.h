@@ -160,6 +160,33 @@ int32_t mtl_alloc(uint64_t bytes, void **out_dev);
 int32_t mtl_free(void *dev);
 /* Device bytes charged to the processing budget now. */
 int32_t mtl_processing_in_use(uint64_t *bytes);
+
+/* Refcounted handles (GpuBuffer / GpuVector, gpu_buffer.rs:12-52): memory
+ * from the processing budget, freed at the last release.  A mtl_vec is a
+ * selection vector: capacity uint32 row ids and a device count word, which
+ * mtl_vec_count reads (the one synchronising read, as GpuVector's count is
+ * read only after end_batch, gpu_buffer.rs:94-103).  The operators take
+ * their device pointers (mtl_buf_device, mtl_vec_rows, mtl_vec_count_ptr). */
+typedef struct mtl_buf mtl_buf;
+typedef struct mtl_vec mtl_vec;
+int32_t mtl_buf_alloc(uint64_t bytes, mtl_buf **out);
+int32_t mtl_buf_retain(mtl_buf *b);
+int32_t mtl_buf_release(mtl_buf *b);
+void *mtl_buf_device(const mtl_buf *b);
+uint64_t mtl_buf_bytes(const mtl_buf *b);
+int32_t mtl_vec_alloc(uint64_t capacity, mtl_vec **out);
+int32_t mtl_vec_retain(mtl_vec *v);
+int32_t mtl_vec_release(mtl_vec *v);
+uint32_t *mtl_vec_rows(const mtl_vec *v);
+uint32_t *mtl_vec_count_ptr(const mtl_vec *v);
+uint64_t mtl_vec_capacity(const mtl_vec *v);
+int32_t mtl_vec_count(const mtl_vec *v, mtl_stream *s, uint64_t *n);
+/* mtl_filter over rows 0..n_rows-1 (in == NULL) or over the rows of `in`,
+ * into `out` (capacity at least the input's): filter_to_indices_gv
+ * (metal/mod.rs filter path, K24) with handles. */
+int32_t mtl_filter_vec(const mtl_pred *preds, uint32_t n_preds,
+                       uint64_t n_rows, const mtl_vec *in, mtl_vec *out,
+                       mtl_stream *s);
 int32_t mtl_memcpy_h2d(void *dst_dev, const void *src_host, uint64_t bytes,
                        mtl_stream *s);
 int32_t mtl_memcpy_d2h(void *dst_host, const void *src_dev, uint64_t bytes,
@@ -699,6 +726,10 @@ int32_t mtl_server_submit_sql(mtl_server *srv, const char *sql,
 /* Block until the query completes; releases the ticket. */
 int32_t mtl_server_wait(mtl_server *srv, uint64_t ticket,
                         mtl_query_result *out);
+/* Without blocking: *done = 1 and the result (the ticket released) when the
+ * query has completed, else *done = 0. */
+int32_t mtl_server_poll(mtl_server *srv, uint64_t ticket,
+                        mtl_query_result *out, int32_t *done);
 /* Steady-clock now, in the clock of the result stamps. */
 int64_t mtl_now_ns(void);
 /* Device scratch bytes a query needs at admission (its budget charge). */

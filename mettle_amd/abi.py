@@ -270,6 +270,20 @@ def lib() -> C.CDLL:
         "mtl_profile_enable": (i32, [i32]),
         "mtl_profile_reset": (i32, []),
         "mtl_result_export_arrow": (i32, [vp, vp, vp]),
+        "mtl_server_poll": (i32, [vp, u64, vp, C.POINTER(i32)]),
+        "mtl_buf_alloc": (i32, [u64, C.POINTER(vp)]),
+        "mtl_buf_retain": (i32, [vp]),
+        "mtl_buf_release": (i32, [vp]),
+        "mtl_buf_device": (vp, [vp]),
+        "mtl_buf_bytes": (u64, [vp]),
+        "mtl_vec_alloc": (i32, [u64, C.POINTER(vp)]),
+        "mtl_vec_retain": (i32, [vp]),
+        "mtl_vec_release": (i32, [vp]),
+        "mtl_vec_rows": (vp, [vp]),
+        "mtl_vec_count_ptr": (vp, [vp]),
+        "mtl_vec_capacity": (u64, [vp]),
+        "mtl_vec_count": (i32, [vp, vp, C.POINTER(u64)]),
+        "mtl_filter_vec": (i32, [C.POINTER(Pred), u32, u64, vp, vp, vp]),
         "mtl_catalog_ingest_arrow": (i32, [C.c_char_p, vp, C.POINTER(vp), u32,
                                            C.POINTER(C.c_char_p), u32, C.POINTER(u64),
                                            C.POINTER(u64), C.c_char_p, C.c_size_t]),

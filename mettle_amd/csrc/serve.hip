@@ -617,6 +617,21 @@ int32_t mtl_server_wait(mtl_server* srv, uint64_t ticket, mtl_query_result* out)
   return MTL_OK;
 }
 
+int32_t mtl_server_poll(mtl_server* srv, uint64_t ticket, mtl_query_result* out,
+                        int32_t* done) {
+  using namespace mtl;
+  if (!srv || !out || !done) return set_error(MTL_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->reqs.find(ticket);
+  if (it == srv->reqs.end()) return set_error(MTL_ERR_INVALID_ARGUMENT, "unknown ticket");
+  *done = it->second.done ? 1 : 0;
+  if (!*done) return MTL_OK;
+  *out = it->second.res;
+  srv->reqs.erase(it);
+  if (out->status == MTL_ERR_BUDGET) return set_error(MTL_ERR_BUDGET, "%s", out->error);
+  return MTL_OK;
+}
+
 int32_t mtl_server_destroy(mtl_server* srv) {
   using namespace mtl;
   if (!srv) return MTL_OK;

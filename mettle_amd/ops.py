@@ -326,6 +326,20 @@ class Server:
         abi.check(st)
         return r
 
+    def poll(self, ticket: int):
+        """The result record if the query has completed (the ticket is then
+        released), else None."""
+        r = abi.QueryResult()
+        done = C.c_int32()
+        st = abi.lib().mtl_server_poll(self.h, ticket, C.byref(r), C.byref(done))
+        if not done.value:
+            abi.check(st)
+            return None
+        if r.status != abi.MTL_OK:
+            raise abi.MettleError(r.status, r.error.decode(errors="replace"))
+        abi.check(st)
+        return r
+
     def wait_sql(self, ticket: int):
         """A SQL query's rows, as mettle_amd.sql.execute returns them."""
         from mettle_amd import sql as S
