@@ -329,12 +329,27 @@ def group_workload(args, dense: bool) -> dict:
     aggs = [(abi.AGG_SUM, dvi, abi.BIGINT), (abi.AGG_COUNT, None, 0),
             (abi.AGG_MIN, dvi, abi.BIGINT), (abi.AGG_MAX, dvi, abi.BIGINT),
             (abi.AGG_SUM, dvf, abi.FLOAT), (abi.AGG_AVG, dvf, abi.FLOAT)]
-    keys = [(dg, abi.BIGINT, 0, gmax)]
+    # the C ABI directly (as the join and scan workloads): one descriptor and
+    # the output buffers made once, then group-by + top-100 per call
+    import numpy as np
+    desc = abi.make_group_desc(n, [(dg.data_ptr(), abi.BIGINT, 0, gmax)],
+                               [(f, c.data_ptr() if c is not None else None, t)
+                                for f, c, t in aggs])
+    keys_out = np.empty((100, 1), dtype=np.int64)
+    outs = [np.empty(100, dtype=abi.agg_result_dtype(f, t)) for f, _, t in aggs]
+    ptrs = (C.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+    got = C.c_uint32()
     res = {}
 
     def call():
-        g = ops.group_agg(keys, aggs, n)
-        res["topk"] = g.topk(0, 100, desc=True)
+        h = C.c_void_p()
+        abi.check(L.mtl_group_agg(C.byref(desc), C.byref(h), None))
+        try:
+            abi.check(L.mtl_groups_topk(h, 0, 1, 100, keys_out.ctypes.data, ptrs,
+                                        C.byref(got)))
+        finally:
+            L.mtl_groups_release(h)
+        res["topk"] = got.value
     ms = timed(call, args.steps, args.warmup)
     alg = n * 20
     gbs = alg / (ms * 1e-3) / 1e9
