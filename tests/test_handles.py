@@ -20,7 +20,11 @@ torch = pytest.importorskip("torch")
 @pytest.fixture
 def lib():
     L = abi.lib()
-    abi.check(L.mtl_init(1 << 30, 4 << 30, 0, None, 0))
+    # the budget is counted from what earlier work in this process still
+    # holds (stream arenas are kept for reuse), plus 4 GiB for these tests
+    held = C.c_uint64()
+    abi.check(L.mtl_processing_in_use(C.byref(held)))
+    abi.check(L.mtl_init(1 << 30, held.value + (4 << 30), 0, None, 0))
     yield L
     L.mtl_cache_reset()
     L.mtl_shutdown()
