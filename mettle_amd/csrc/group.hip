@@ -1261,6 +1261,7 @@ struct EmitArgs {
   int64_t* out_keys;   // n x n_keys
   uint64_t* out_cnt;   // n
   uint64_t* out_aggs;  // n x (2 words per agg)
+  uint32_t* out_n;     // non-NULL: *count is copied here (read back with the rows)
 };
 
 __device__ __forceinline__ uint64_t group_key_of(const EmitArgs& a, uint32_t g) {
@@ -1268,11 +1269,11 @@ __device__ __forceinline__ uint64_t group_key_of(const EmitArgs& a, uint32_t g) 
   return g == a.cap ? GEMPTY : a.tkeys[g];
 }
 
-__global__ void k_group_emit(EmitArgs a) {
-  const uint32_t n = *a.count;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    uint32_t g = a.idx[i];
-    uint64_t key = group_key_of(a, g);
+// Output row i of the emission: group g's key columns, count and
+// aggregate words.
+__device__ __forceinline__ void emit_row(const EmitArgs& a, uint32_t i, uint32_t g) {
+  const uint64_t key = group_key_of(a, g);
+  {
     for (uint32_t j = 0; j < a.n_keys; ++j) {
       int64_t v;
       if (a.raw) v = (int64_t)key;
@@ -1296,6 +1297,13 @@ __global__ void k_group_emit(EmitArgs a) {
       }
     }
   }
+}
+
+__global__ void k_group_emit(EmitArgs a) {
+  const uint32_t n = *a.count;
+  if (a.out_n && blockIdx.x == 0 && threadIdx.x == 0) *a.out_n = n;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    emit_row(a, i, a.idx[i]);
 }
 
 // Top-K over all table entries: primary = one aggregate (two key words),
@@ -1499,6 +1507,122 @@ __global__ __launch_bounds__(TopkShape<R>::NT) void k_group_topk(GTopkArgs a) {
   topk_block_finish<S::NT>(wl, sl, a.stage, a.lists0);
 }
 
+// Top-K of a table of at most kSmallTopk entries in one launch, emission
+// included (in place of the per-block lists, their merge levels and the
+// emission kernel): every block builds the keys of all non-empty entries in
+// LDS and ranks its own 64 entries against them, by the key words in order
+// and then the entry index, so ranks are distinct; each of the block's four
+// waves counts over a quarter of the entries.  The entry of rank r < K is
+// emitted as output row r, and block 0 writes min(K, non-empty entries) to
+// *em.out_n.
+constexpr uint32_t kSmallTopk = 2048;
+constexpr int kSmallTopkThreads = 256;
+constexpr uint32_t kSmallTopkRank = 64;  // entries ranked per block
+
+__global__ __launch_bounds__(kSmallTopkThreads) void k_group_topk_small(GTopkArgs a,
+                                                                         EmitArgs em) {
+  constexpr int PER = kSmallTopk / kSmallTopkThreads;  // entries loaded per thread
+  constexpr int NW = kSmallTopkThreads / WAVE;
+  __shared__ uint64_t lk[4][kSmallTopk];
+  __shared__ uint32_t lidx[kSmallTopk];  // entry of each compacted position
+  __shared__ uint32_t lpos[kSmallTopk];  // compacted position of each entry
+  __shared__ uint32_t lpart[2][NW][WAVE];
+  __shared__ uint32_t ln;
+  if (threadIdx.x == 0) ln = 0;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const uint32_t n_ent = (uint32_t)a.n;
+  // every entry's words in flight at once
+  GRaw r[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t e = threadIdx.x + (uint32_t)i * kSmallTopkThreads;
+    r[i] = e < n_ent ? gtopk_load(a, e) : GRaw{0, 0, 0, 0};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t e = threadIdx.x + (uint32_t)i * kSmallTopkThreads;
+    const bool v = e < n_ent && r[i].cnt != 0;
+    uint64_t kk[4] = {0, 0, 0, 0};
+    if (v) gtopk_keys(a, r[i], kk);
+    // one LDS reservation per wave: the lanes' positions from the ballot
+    const uint64_t bal = __ballot(v);
+    uint32_t base = 0;
+    if (lane == 0 && bal) base = atomicAdd(&ln, (uint32_t)__popcll(bal));
+    base = (uint32_t)__shfl((int)base, 0, WAVE);
+    if (v) {
+      const uint32_t p = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+#pragma unroll
+      for (int f = 0; f < 4; ++f) lk[f][p] = kk[f];
+      lidx[p] = e;
+      lpos[e] = p;
+    } else if (e < n_ent) {
+      lpos[e] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = ln;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *em.out_n = n < a.k ? n : a.k;
+  const uint32_t mine_e = blockIdx.x * kSmallTopkRank + (uint32_t)lane;
+  const uint32_t mp = mine_e < n_ent ? lpos[mine_e] : 0xFFFFFFFFu;
+  uint32_t below = 0, tie = 0;
+  if (mp != 0xFFFFFFFFu) {
+    const uint64_t m0 = lk[0][mp];
+    // this wave's quarter of the compacted entries, sixteen LDS reads in
+    // flight per step
+    const uint32_t q = (n + NW - 1) / NW;
+    const uint32_t j0 = (uint32_t)wave * q, j1 = j0 + q < n ? j0 + q : n;
+    uint32_t j = j0;
+    for (; j + 16 <= j1; j += 16) {
+      uint64_t x[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = lk[0][j + u];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        below += x[u] < m0 ? 1u : 0u;
+        tie += x[u] == m0 ? 1u : 0u;
+      }
+    }
+    for (; j < j1; ++j) {
+      const uint64_t x = lk[0][j];
+      below += x < m0 ? 1u : 0u;
+      tie += x == m0 ? 1u : 0u;
+    }
+  }
+  lpart[0][wave][lane] = below;
+  lpart[1][wave][lane] = tie;
+  __syncthreads();
+  if (wave != 0 || mp == 0xFFFFFFFFu) return;
+  below = 0;
+  tie = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    below += lpart[0][w][lane];
+    tie += lpart[1][w][lane];
+  }
+  if (tie > 1) {  // equal first words (self included): the other words decide
+    uint64_t mk[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) mk[f] = lk[f][mp];
+    for (uint32_t j = 0; j < n; ++j) {
+      if (lk[0][j] != mk[0]) continue;
+      bool lt = false;
+      bool eq = true;
+#pragma unroll
+      for (int f = 1; f < 4; ++f) {
+        const uint64_t x = lk[f][j];
+        if (eq && x != mk[f]) {
+          lt = x < mk[f];
+          eq = false;
+        }
+      }
+      if (eq) lt = lidx[j] < mine_e;
+      below += lt ? 1u : 0u;
+    }
+  }
+  if (below < a.k) emit_row(em, below, mine_e);
+}
+
 // Merge level: block b merges lists [b*F, b*F+F) of `in`; the single block
 // of the final level writes the table indices of the top K.
 __global__ __launch_bounds__(TOPK_MERGE_THREADS) void k_group_topk_merge(GTopkArgs a,
@@ -1529,11 +1653,35 @@ struct mtl_groups {
   uint64_t entries = 0;  // table entries (dense domain or cap + 1)
   void* keys_mem = nullptr;
   void* st_mem = nullptr;
+  uint64_t mem_bytes[2] = {0, 0};  // allocated sizes of st_mem, keys_mem
   uint64_t bytes = 0;  // charged to the processing budget
   int32_t agg_func[MTL_GROUP_MAX_AGGS];
   int32_t agg_type[MTL_GROUP_MAX_AGGS];
   int32_t agg_scale[MTL_GROUP_MAX_AGGS];  // DECIMAL digits (AVG divides by 10^scale)
 };
+
+// Device memory of `bytes` for slot i (0 state words, 1 hashed keys) of a
+// group table on stream s: the stream's parked table memory when it is
+// large enough and not over four times too large, else a stream-ordered
+// allocation (a parked block that does not fit is returned to the pool).
+// Called with s->mu held.
+static hipError_t table_mem(mtl_stream* s, int i, uint64_t bytes, void** out, uint64_t* got) {
+  void*& p = s->parked_table[i];
+  if (p && s->parked_bytes[i] >= bytes && s->parked_bytes[i] / 4 <= bytes) {
+    *out = p;
+    *got = s->parked_bytes[i];
+    p = nullptr;
+    s->parked_bytes[i] = 0;
+    return hipSuccess;
+  }
+  if (p) {
+    (void)hipFreeAsync(p, s->stream);
+    p = nullptr;
+    s->parked_bytes[i] = 0;
+  }
+  *got = bytes;
+  return hipMallocAsync(out, bytes, s->stream);
+}
 
 namespace mtl {
 
@@ -1925,8 +2073,8 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
     delete g;
     return cs;
   }
-  hipError_t e = hipMallocAsync(&g->st_mem, entries * a.W * 8 + 8, s->stream);
-  if (e == hipSuccess && !a.dense) e = hipMallocAsync(&g->keys_mem, entries * 8, s->stream);
+  hipError_t e = table_mem(s, 0, entries * a.W * 8 + 8, &g->st_mem, &g->mem_bytes[0]);
+  if (e == hipSuccess && !a.dense) e = table_mem(s, 1, entries * 8, &g->keys_mem, &g->mem_bytes[1]);
   if (e != hipSuccess) {
     if (g->st_mem) (void)hipFreeAsync(g->st_mem, s->stream);
     release_processing(g->bytes);
@@ -2050,8 +2198,18 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
 
 int32_t mtl_groups_release(mtl_groups* g) {
   if (!g) return MTL_OK;
-  (void)hipFreeAsync(g->st_mem, g->s->stream);
-  if (g->keys_mem) (void)hipFreeAsync(g->keys_mem, g->s->stream);
+  {
+    // park the table's memory on its stream for the next group-by there
+    mtl_stream* s = g->s;
+    std::lock_guard<std::mutex> lk(s->mu);
+    void* mem[2] = {g->st_mem, g->keys_mem};
+    for (int i = 0; i < 2; ++i) {
+      if (!mem[i]) continue;
+      if (s->parked_table[i]) (void)hipFreeAsync(s->parked_table[i], s->stream);
+      s->parked_table[i] = mem[i];
+      s->parked_bytes[i] = g->mem_bytes[i];
+    }
+  }
   release_processing(g->bytes);
   delete g;
   return MTL_OK;
@@ -2062,18 +2220,20 @@ int32_t mtl_groups_release(mtl_groups* g) {
 // in the same round trip (*n_real).
 static int32_t emit_groups(mtl_groups* g, const uint32_t* idx, const uint32_t* count,
                            uint64_t n, int64_t* keys, void* const* agg_out,
-                           uint32_t* n_real = nullptr) {
+                           uint32_t* n_real = nullptr, const GTopkArgs* small = nullptr) {
   mtl_stream* s = g->s;
   const GroupArgs& a = g->args;
   if (n == 0) return MTL_OK;
   // keys, counts and aggregate words in one device block, so a small result
-  // comes back in one copy through the stream's pinned staging
+  // comes back in one copy through the stream's pinned staging; with n_real
+  // the block starts with a 256-byte header holding the device count
   const uint64_t kb = n * a.n_keys * 8, cb = n * 8, ab = n * std::max<uint32_t>(a.n_aggs, 1) * 16;
+  const uint64_t hb = n_real ? 256 : 0;
   void* blob;
-  MTL_TRY(s->arena.alloc(kb + cb + ab, &blob));
-  void* dk = blob;
-  void* dc = (char*)blob + kb;
-  void* da = (char*)blob + kb + cb;
+  MTL_TRY(s->arena.alloc(hb + kb + cb + ab, &blob));
+  void* dk = (char*)blob + hb;
+  void* dc = (char*)blob + hb + kb;
+  void* da = (char*)blob + hb + kb + cb;
   EmitArgs e{};
   e.idx = idx;
   e.count = count;
@@ -2094,7 +2254,14 @@ static int32_t emit_groups(mtl_groups* g, const uint32_t* idx, const uint32_t* c
   e.out_keys = (int64_t*)dk;
   e.out_cnt = (uint64_t*)dc;
   e.out_aggs = (uint64_t*)da;
-  {
+  e.out_n = n_real ? (uint32_t*)blob : nullptr;
+  if (small) {  // the small-table top-K ranks and emits in one launch
+    ProfScope ps(s, "k_group_topk_small");
+    hipLaunchKernelGGL(k_group_topk_small,
+                       dim3((uint32_t)ceil_div(std::max<uint64_t>(small->n, 1),
+                                               (uint64_t)kSmallTopkRank)),
+                       dim3(kSmallTopkThreads), 0, s->stream, *small, e);
+  } else {
     ProfScope ps(s, "k_group_emit");
     hipLaunchKernelGGL(k_group_emit, dim3((uint32_t)std::min<uint64_t>(ceil_div(n, 256), 4096)),
                        dim3(256), 0, s->stream, e);
@@ -2106,8 +2273,7 @@ static int32_t emit_groups(mtl_groups* g, const uint32_t* idx, const uint32_t* c
     void* h0;
     MTL_TRY(stream_pinned(s, 256 + kb + cb + ab, &h0));
     char* h = (char*)h0 + 256;
-    if (n_real) MTL_HIP(hipMemcpyAsync(h0, count, 4, hipMemcpyDeviceToHost, s->stream));
-    MTL_HIP(hipMemcpyAsync(h, blob, kb + cb + ab, hipMemcpyDeviceToHost, s->stream));
+    MTL_HIP(hipMemcpyAsync(h - hb, blob, hb + kb + cb + ab, hipMemcpyDeviceToHost, s->stream));
     MTL_HIP(stream_wait(s->stream));
     if (n_real) {
       *n_real = *(const uint32_t*)h0;
@@ -2241,25 +2407,9 @@ int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, i
   mtl_stream* s = g->s;
   std::lock_guard<std::mutex> lk(s->mu);
   s->arena.reset();
-  const uint64_t nt = k <= 128 ? 1024 : 256;
-  uint32_t grid = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>(ceil_div(g->entries, nt * 16), (uint64_t)cu_count()));
-  void *ck[4], *cid, *lk0[2], *lg[2], *ln[2], *misc, *oidx;
-  const uint64_t stage_n = (uint64_t)grid * (nt / 64) * k;
-  for (int f = 0; f < 4; ++f) MTL_TRY(s->arena.alloc(stage_n * 8, &ck[f]));
-  MTL_TRY(s->arena.alloc(stage_n * 4, &cid));
-  const uint64_t la = grid, lb = ceil_div(grid, topk_fan(k));
-  MTL_TRY(s->arena.alloc(la * k * 8, &lk0[0]));
-  MTL_TRY(s->arena.alloc(la * k * 4, &lg[0]));
-  MTL_TRY(s->arena.alloc(la * 4, &ln[0]));
-  MTL_TRY(s->arena.alloc(lb * k * 8, &lk0[1]));
-  MTL_TRY(s->arena.alloc(lb * k * 4, &lg[1]));
-  MTL_TRY(s->arena.alloc(lb * 4, &ln[1]));
+  void *misc, *oidx;
   MTL_TRY(s->arena.alloc(256, &misc));
   MTL_TRY(s->arena.alloc((uint64_t)k * 4, &oidx));
-  ZeroList zl;
-  zl.add((uint32_t*)misc, 64);
-  MTL_TRY(launch_zero(zl, s->stream));
   GTopkArgs t{};
   t.st = a.st;
   t.tkeys = a.tkeys;
@@ -2282,14 +2432,33 @@ int32_t mtl_groups_topk(mtl_groups* g, uint32_t agg, int32_t desc, uint32_t k, i
     for (int i = 0; i < g->agg_scale[agg]; ++i) t.avg_scale *= 10;
   }
   t.k = k;
+  t.out_idx = (uint32_t*)oidx;
+  t.out_n = (uint32_t*)misc + 1;
+  if (g->entries <= kSmallTopk)
+    return emit_groups(g, nullptr, nullptr, k, keys, agg_out, n_out, &t);
+  const uint64_t nt = k <= 128 ? 1024 : 256;
+  uint32_t grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(ceil_div(g->entries, nt * 16), (uint64_t)cu_count()));
+  void *ck[4], *cid, *lk0[2], *lg[2], *ln[2];
+  const uint64_t stage_n = (uint64_t)grid * (nt / 64) * k;
+  for (int f = 0; f < 4; ++f) MTL_TRY(s->arena.alloc(stage_n * 8, &ck[f]));
+  MTL_TRY(s->arena.alloc(stage_n * 4, &cid));
+  const uint64_t la = grid, lb = ceil_div(grid, topk_fan(k));
+  MTL_TRY(s->arena.alloc(la * k * 8, &lk0[0]));
+  MTL_TRY(s->arena.alloc(la * k * 4, &lg[0]));
+  MTL_TRY(s->arena.alloc(la * 4, &ln[0]));
+  MTL_TRY(s->arena.alloc(lb * k * 8, &lk0[1]));
+  MTL_TRY(s->arena.alloc(lb * k * 4, &lg[1]));
+  MTL_TRY(s->arena.alloc(lb * 4, &ln[1]));
+  ZeroList zl;
+  zl.add((uint32_t*)misc, 64);
+  MTL_TRY(launch_zero(zl, s->stream));
   for (int f = 0; f < 4; ++f) t.stage.ck[f] = (uint64_t*)ck[f];
   t.stage.cid = (uint32_t*)cid;
   TopkLists lists[2];
   for (int b = 0; b < 2; ++b)
     lists[b] = TopkLists{(uint64_t*)lk0[b], (uint32_t*)lg[b], (uint32_t*)ln[b]};
   t.lists0 = lists[0];
-  t.out_idx = (uint32_t*)oidx;
-  t.out_n = (uint32_t*)misc + 1;
   {
     ProfScope ps(s, "k_group_topk");
     if (k <= 64)

@@ -74,6 +74,12 @@ struct mtl_stream {
   void* pinned_small = nullptr;
   size_t pinned_small_bytes = 0;
   std::mutex mu;  // one operator at a time per handle
+  // the last released group table's memory (state words, hashed keys),
+  // kept for the next group-by on this stream: reuse is stream-ordered, and
+  // parked memory is not charged to the processing budget, like memory
+  // returned to the device's pool
+  void* parked_table[2] = {nullptr, nullptr};
+  uint64_t parked_bytes[2] = {0, 0};
   // kernel timing (prof.hpp): pending (name, start, end) event pairs and a
   // pool of reusable events
   struct ProfPair {

@@ -311,8 +311,14 @@ int32_t mtl_shutdown(void) {
   Runtime& rt = runtime();
   std::lock_guard<std::mutex> g(rt.mu);
   if (rt.default_stream) {
-    hipStreamSynchronize(rt.default_stream->stream);
-    rt.default_stream->arena.release_all();
+    mtl_stream* d = rt.default_stream;
+    hipStreamSynchronize(d->stream);
+    d->arena.release_all();
+    for (int i = 0; i < 2; ++i) {
+      if (d->parked_table[i]) hipFree(d->parked_table[i]);
+      d->parked_table[i] = nullptr;
+      d->parked_bytes[i] = 0;
+    }
   }
   rt.initialized = false;
   rt.cache_bytes = rt.processing_bytes = rt.pinned_bytes = 0;
@@ -352,6 +358,8 @@ int32_t mtl_stream_destroy(mtl_stream* s) {
     return set_error(MTL_ERR_INVALID_ARGUMENT, "cannot destroy the default stream");
   hipStreamSynchronize(s->stream);
   s->arena.release_all();
+  for (void* t : s->parked_table)
+    if (t) hipFree(t);
   if (s->pinned_small) hipHostFree(s->pinned_small);
   if (s->owns_stream) hipStreamDestroy(s->stream);
   delete s;

@@ -443,7 +443,6 @@ struct Parser {
     if (accept_kw("in")) {
       // col IN (v1, ..., vk): k equalities OR-ed together
       if (!expect_op("(")) return false;
-      size_t k = 0;
       do {
         Pred pr;
         pr.col = c;
@@ -451,7 +450,6 @@ struct Parser {
         if (!literal(pr.lit)) return false;
         pr.or_next = true;
         q.preds.push_back(pr);
-        ++k;
       } while (accept_op(","));
       if (!expect_op(")")) return false;
       q.preds.back().or_next = false;

@@ -551,3 +551,90 @@ def test_gpu_group_topk_many_rounds_per_block():
         tk, to = groups.topk(0, 100, desc=desc)
         assert tk[:, 0].tolist() == rk[order, 0].tolist()
         assert [abi.hugeint_to_int(r) for r in to[0]] == sums[order].tolist()
+
+
+def _small_topk_kernels(fn):
+    """fn()'s result and the names of the library kernels it launched."""
+    import ctypes as C
+    L = abi.lib()
+    L.mtl_profile_reset()
+    L.mtl_profile_enable(1)
+    try:
+        out = fn()
+        abi.check(L.mtl_stream_sync(None))
+    finally:
+        L.mtl_profile_enable(0)
+    names, i = [], 0
+    name, ms, cnt = C.create_string_buffer(128), C.c_double(), C.c_uint64()
+    while L.mtl_profile_get(i, name, 128, C.byref(ms), C.byref(cnt)) == 0:
+        names.append(name.value.decode())
+        i += 1
+    L.mtl_profile_reset()
+    return out, names
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("func,vtype", [(F.AGG_SUM, F.BIGINT), (F.AGG_COUNT, None),
+                                        (F.AGG_AVG, F.BIGINT), (F.AGG_MIN, F.INTEGER),
+                                        (F.AGG_SUM, F.DOUBLE)])
+@pytest.mark.parametrize("domain,holes", [(1000, False), (2048, True), (37, False)])
+def test_gpu_group_topk_small_table(func, vtype, domain, holes):
+    """Tables of at most 2048 entries take the one-launch top-K: every K
+    (below, at and above the number of non-empty groups), both directions,
+    equal values decided by the key, empty entries of a sparse key domain
+    skipped, against the oracle."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(domain + 10 * func)
+    n = 60_000
+    keys = np.arange(domain, dtype=np.int64)
+    if holes:
+        keys = keys[rng.random(domain) < 0.6]  # about 40% of the domain never occurs
+    g = rng.choice(keys, n).astype(np.int64)
+    if vtype == F.DOUBLE:
+        col = rng.integers(-50, 50, n).astype(np.float64)  # exact sums in any order
+    elif vtype == F.INTEGER:
+        col = rng.integers(-5, 5, n).astype(np.int32)  # many equal minima
+    else:
+        col = rng.integers(-10**12, 10**12, n).astype(np.int64)
+        col[g % 7 == 0] = 3  # equal sums and averages for some groups
+    aggs = [(func, None if func == F.AGG_COUNT else col, vtype or 0), (F.AGG_COUNT, None, 0)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs)
+    if func == F.AGG_SUM and vtype != F.DOUBLE:
+        vals = [abi.hugeint_to_int(r) for r in ro[0]]
+    else:
+        vals = [x.item() for x in ro[0]]
+    dcol = None if func == F.AGG_COUNT else ops.to_device(col)
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, 0, domain - 1)],
+                           [(func, dcol, vtype or 0), (F.AGG_COUNT, None, 0)], n)
+    for k in sorted({1, 10, min(len(rk), 512), min(len(rk) + 5, 512), 512}):
+        for desc in (True, False):
+            order = _topk_expected(rk, vals, k, desc)
+            (tk, to), names = _small_topk_kernels(lambda: groups.topk(0, k, desc=desc))
+            assert "k_group_topk_small" in names and "k_group_topk" not in names
+            assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order], (k, desc)
+            got = ([abi.hugeint_to_int(r) for r in to[0]]
+                   if func == F.AGG_SUM and vtype != F.DOUBLE else [x.item() for x in to[0]])
+            assert got == [vals[i] for i in order]
+            assert to[1].tolist() == [int(ro[1][i]) for i in order]
+
+
+@pytest.mark.gpu
+def test_gpu_group_topk_small_hash_table():
+    """A small hashed table (BIGINT keys over most of the int64 range, 250
+    rows: at most 1024 slots and the overflow entry at any tuned load
+    factor) also takes the one-launch top-K."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(9)
+    uniq = rng.integers(-2**62, 2**62, 100, dtype=np.int64)
+    g = rng.choice(uniq, 250)
+    v = rng.integers(-1000, 1000, 250).astype(np.int64)
+    rk, ro = orc.group_agg([(g, F.BIGINT)], [(F.AGG_SUM, v, F.BIGINT)])
+    sums = [abi.hugeint_to_int(r) for r in ro[0]]
+    groups = ops.group_agg([(ops.to_device(g), F.BIGINT, int(g.min()), int(g.max()))],
+                           [(F.AGG_SUM, ops.to_device(v), F.BIGINT)], len(g))
+    for desc in (True, False):
+        order = _topk_expected(rk, sums, 100, desc)
+        (tk, to), names = _small_topk_kernels(lambda: groups.topk(0, 100, desc=desc))
+        assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order]
+        assert [abi.hugeint_to_int(r) for r in to[0]] == [sums[i] for i in order]
+        assert "k_group_topk_small" in names
