@@ -127,6 +127,12 @@ struct GroupArgs {
   // aggregated, since that aggregation writes whole table entries
   uint64_t* hot_key;
   uint64_t* hot_st;
+  // directly indexed cache of a large input: each block's cache states are
+  // staged here at the end (word k of slot i of block b at
+  // (b * W + k) * (cache_slots + 1) + i) and reduced by k_dense_reduce,
+  // instead of every block merging its whole cache into the table with
+  // device-scope atomics
+  uint64_t* dstage;
 };
 
 __device__ __forceinline__ uint64_t gfmix(uint64_t k) {
@@ -707,6 +713,12 @@ __global__ __launch_bounds__(kAggBlock, NS <= 4 ? 6 : 4) void k_group_agg(GroupA
       a.hot_st[h0 * a.W + t] = cst[(uint64_t)(t % a.W) * ws + t / a.W];
     return;
   }
+  if (direct && a.dstage) {
+    // the cache's state words as they lie in LDS (word-major, stride ws)
+    uint64_t* o = a.dstage + (uint64_t)blockIdx.x * ws * a.W;
+    for (uint32_t t = threadIdx.x; t < ws * a.W; t += blockDim.x) o[t] = cst[t];
+    return;
+  }
   for (uint32_t i = threadIdx.x; i < cs; i += blockDim.x) {
     uint64_t key = ckeys[i];
     if (key == GEMPTY || cst[i] == 0) continue;  // word 0: the row count
@@ -1216,6 +1228,88 @@ __global__ __launch_bounds__(256) void k_stage_reduce(GroupArgs a, const uint64_
     }
     if (cur != GEMPTY) merge_regs<WM>(a, a.st + cur * W, r);
   }
+}
+
+// The staged caches of a directly indexed domain (k_group_agg with dstage:
+// word k of slot i of block b at (b * W + k) * (cs + 1) + i) into the
+// table.  Block (x, y) takes stage blocks [y * kDenseReduceG, ...) and one
+// thread a (word, slot) pair: its kDenseReduceG partial values are loaded
+// at once (consecutive threads, consecutive words), combined by the word's
+// operation (dense_word_ops) and merged with one device atomic, so a word
+// costs kDenseReduceG times fewer atomics than a merge per block.
+constexpr uint32_t kDenseReduceG = 32;
+enum : uint8_t { DW_NONE, DW_ADD, DW_ADDF, DW_MINI, DW_MAXI, DW_MINU, DW_MAXU };
+struct DenseWordOps {
+  uint8_t op[kMaxW];
+};
+
+__device__ __forceinline__ uint64_t dense_combine(uint8_t op, uint64_t r, uint64_t q) {
+  switch (op) {
+    case DW_ADD: return r + q;
+    case DW_ADDF:
+      return (uint64_t)__double_as_longlong(__longlong_as_double((long long)r) +
+                                            __longlong_as_double((long long)q));
+    case DW_MINI: return (int64_t)q < (int64_t)r ? q : r;
+    case DW_MAXI: return (int64_t)q > (int64_t)r ? q : r;
+    case DW_MINU: return q < r ? q : r;
+    case DW_MAXU: return q > r ? q : r;
+    default: return r;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dense_reduce(GroupArgs a, DenseWordOps ops,
+                                                      const uint64_t* stg, uint32_t nb,
+                                                      uint32_t cs) {
+  const uint32_t ws = cs + 1, W = a.W;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= W * ws) return;
+  const uint32_t k = t / ws, i = t % ws;
+  if (i >= cs) return;  // the spare slot
+  const uint8_t op = ops.op[k];
+  if (op == DW_NONE) return;
+  const uint32_t b0 = blockIdx.y * kDenseReduceG;
+  const uint32_t nbk = nb - b0 < kDenseReduceG ? nb - b0 : kDenseReduceG;
+  uint64_t q[kDenseReduceG];
+#pragma unroll
+  for (uint32_t u = 0; u < kDenseReduceG; ++u)
+    q[u] = u < nbk ? stg[((uint64_t)(b0 + u) * W) * ws + t] : 0ull;
+  const uint64_t id = a.init[k];  // the identity of the word's operation
+  uint64_t r = id;
+#pragma unroll
+  for (uint32_t u = 0; u < kDenseReduceG; ++u)
+    if (u < nbk) r = dense_combine(op, r, q[u]);
+  if (r == id) return;  // merging the identity changes nothing
+  unsigned long long* w = (unsigned long long*)&a.st[global_slot(a, i) * W + k];
+  switch (op) {
+    case DW_ADD: atomicAdd(w, (unsigned long long)r); break;
+    case DW_ADDF: unsafeAtomicAdd((double*)w, __longlong_as_double((long long)r)); break;
+    case DW_MINI: atomicMin((long long*)w, (long long)r); break;
+    case DW_MAXI: atomicMax((long long*)w, (long long)r); break;
+    case DW_MINU: atomicMin(w, (unsigned long long)r); break;
+    default: atomicMax(w, (unsigned long long)r); break;
+  }
+}
+
+// The operation that combines each state word of a (the count, then each
+// non-duplicate aggregate's one or two words), as merge_state applies them.
+static DenseWordOps dense_word_ops(const GroupArgs& a) {
+  DenseWordOps o{};
+  o.op[0] = DW_ADD;
+  for (uint32_t i = 0; i < a.n_aggs; ++i) {
+    const AggSlot& sl = a.aggs[i];
+    if (sl.dup || sl.kind == K_COUNT) continue;
+    const uint32_t w = sl.word;
+    switch (sl.kind) {
+      case K_SUMI: o.op[w] = DW_ADD; o.op[w + 1] = DW_ADD; break;
+      case K_SUMF: o.op[w] = DW_ADDF; break;
+      case K_MINI: o.op[w] = DW_MINI; break;
+      case K_MAXI: o.op[w] = DW_MAXI; break;
+      case K_MINF: o.op[w] = DW_MINU; break;
+      case K_MAXF: o.op[w] = DW_MAXU; break;
+      default: break;
+    }
+  }
+  return o;
 }
 
 // Non-empty groups (count word != 0) as a bitmask + per-4096-entry counts;
@@ -2128,6 +2222,12 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
     uint64_t grid = std::min<uint64_t>(ceil_div(rows, (uint64_t)kAggBlock * GE * 4),
                                        (uint64_t)cu_count() * per_cu * (a.cache_direct ? 2 : 1));
     const dim3 gd((uint32_t)std::max<uint64_t>(grid, 1));
+    if (a.cache_direct && !spill && gd.x >= 64) {
+      // many blocks over a cached domain: stage the caches and reduce them
+      void* dsp;
+      MTL_TRY(s->arena.alloc((uint64_t)gd.x * (a.cache_slots + 1) * a.W * 8, &dsp));
+      a.dstage = (uint64_t*)dsp;
+    }
     if (spill) {
       void *sn, *hk, *hv;
       const uint64_t hot = (uint64_t)gd.x * a.cache_slots;
@@ -2151,6 +2251,14 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
       } else if (a.n_src <= 2) hipLaunchKernelGGL(k_group_agg<2>, gd, dim3(kAggBlock), lds, s->stream, a);
       else if (a.n_src <= 4) hipLaunchKernelGGL(k_group_agg<4>, gd, dim3(kAggBlock), lds, s->stream, a);
       else hipLaunchKernelGGL(k_group_agg<8>, gd, dim3(kAggBlock), lds, s->stream, a);
+      MTL_HIP(hipGetLastError());
+    }
+    if (a.dstage) {
+      ProfScope ps(s, "k_dense_reduce");
+      const uint64_t words = (uint64_t)a.W * (a.cache_slots + 1);
+      const dim3 rg((uint32_t)ceil_div(words, 256), (uint32_t)ceil_div(gd.x, kDenseReduceG));
+      hipLaunchKernelGGL(k_dense_reduce, rg, dim3(256), 0, s->stream, a, dense_word_ops(a),
+                         a.dstage, gd.x, a.cache_slots);
       MTL_HIP(hipGetLastError());
     }
     if (a.spill) {
@@ -2181,6 +2289,7 @@ int32_t mtl_group_agg(const mtl_group_desc* desc, mtl_groups** out, mtl_stream* 
   g->args.spill_n = nullptr;
   g->args.hot_key = nullptr;
   g->args.hot_st = nullptr;
+  g->args.dstage = nullptr;
   // check errors now (synchronous boundary of the aggregate)
   void* host;
   MTL_TRY(stream_pinned(s, 8, &host));

@@ -209,6 +209,37 @@ def test_gpu_group_agg_int64_extreme_sums(domain, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("domain,n,wide", [(800, 3_000_000, False), (37, 1_200_000, False),
+                                           (300, 2_000_000, True)])
+def test_gpu_group_agg_staged_caches(domain, n, wide):
+    """Inputs large enough for 64 or more blocks over a directly indexed
+    domain: the blocks' caches are staged and reduced per slot (state words
+    of up to 8 and of more than 8 words), against the oracle, with values
+    over the whole int64 range, NaN-free doubles of both signs and MIN/MAX of
+    every kind."""
+    ops = _gpu_ops()
+    rng = np.random.default_rng(domain + n + wide)
+    g = rng.integers(0, domain, n).astype(np.int64) - 5
+    vi = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, n, dtype=np.int64,
+                      endpoint=True)
+    vd = rng.integers(-1000, 1000, n).astype(np.float64) * 0.5  # exact double sums
+    vs = rng.integers(-100, 100, n).astype(np.int32)
+    aggs = [(F.AGG_SUM, vi, F.BIGINT), (F.AGG_COUNT, None, 0), (F.AGG_MIN, vi, F.BIGINT),
+            (F.AGG_MAX, vd, F.DOUBLE), (F.AGG_SUM, vd, F.DOUBLE), (F.AGG_MIN, vd, F.DOUBLE)]
+    if wide:  # 1 + 2 + 1 + 1 + 1 + 1 + 2 + 2 state words
+        aggs += [(F.AGG_SUM, vs, F.INTEGER), (F.AGG_AVG, vs, F.INTEGER)]
+    rk, ro = orc.group_agg([(g, F.BIGINT)], aggs)
+    dev = {id(vi): ops.to_device(vi), id(vd): ops.to_device(vd), id(vs): ops.to_device(vs)}
+    dg = ops.to_device(g)
+    groups, names = _kernels_of(lambda: ops.group_agg(
+        [(dg, F.BIGINT, int(g.min()), int(g.max()))],
+        [(f, None if c is None else dev[id(c)], t) for f, c, t in aggs], n))
+    assert "k_dense_reduce" in names
+    gk, go = groups.fetch()
+    compare(gk, go, rk, ro, aggs, keys=[(g, F.BIGINT)])
+
+
+@pytest.mark.gpu
 def test_gpu_group_agg_zipf_spill_with_selection():
     """A skewed key over a large dense domain (hot keys in the LDS caches,
     the tail spilled), through an input selection, with FLOAT MIN/MAX."""
@@ -553,7 +584,7 @@ def test_gpu_group_topk_many_rounds_per_block():
         assert [abi.hugeint_to_int(r) for r in to[0]] == sums[order].tolist()
 
 
-def _small_topk_kernels(fn):
+def _kernels_of(fn):
     """fn()'s result and the names of the library kernels it launched."""
     import ctypes as C
     L = abi.lib()
@@ -609,7 +640,7 @@ def test_gpu_group_topk_small_table(func, vtype, domain, holes):
     for k in sorted({1, 10, min(len(rk), 512), min(len(rk) + 5, 512), 512}):
         for desc in (True, False):
             order = _topk_expected(rk, vals, k, desc)
-            (tk, to), names = _small_topk_kernels(lambda: groups.topk(0, k, desc=desc))
+            (tk, to), names = _kernels_of(lambda: groups.topk(0, k, desc=desc))
             assert "k_group_topk_small" in names and "k_group_topk" not in names
             assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order], (k, desc)
             got = ([abi.hugeint_to_int(r) for r in to[0]]
@@ -634,7 +665,7 @@ def test_gpu_group_topk_small_hash_table():
                            [(F.AGG_SUM, ops.to_device(v), F.BIGINT)], len(g))
     for desc in (True, False):
         order = _topk_expected(rk, sums, 100, desc)
-        (tk, to), names = _small_topk_kernels(lambda: groups.topk(0, 100, desc=desc))
+        (tk, to), names = _kernels_of(lambda: groups.topk(0, 100, desc=desc))
         assert tk[:, 0].tolist() == [int(rk[i, 0]) for i in order]
         assert [abi.hugeint_to_int(r) for r in to[0]] == [sums[i] for i in order]
         assert "k_group_topk_small" in names
