@@ -337,8 +337,20 @@ def scan_workload(args) -> dict:
             log(f"scan {cname} {frac:.0%}: fused {ms:.3f} ms ({rows[-1]['GBs']:.0f} GB/s; "
                 f"{ms_lat:.3f} ms synchronised per call), "
                 f"filter then project {ms_sep:.3f} ms, ORDER BY {ms_sort:.3f} ms")
+    # the scalar hooks (gpu_add_scalar, gpu_lt_scalar) on one DuckDB vector
+    # of 2048 INTEGER values: host vector in, result out, per call
+    vec = np.arange(2048, dtype=np.int32)
+    hout = np.empty(2048, dtype=np.int32)
+    hooks = {}
+    for name, hook in (("add", abi.HOOK_ADD), ("lt", abi.HOOK_LT)):
+        def hook_call():
+            abi.check(L.mtl_scalar_hook(hook, abi.INTEGER, vec.ctypes.data, len(vec),
+                                        abi.scalar_bits(7, abi.INTEGER), hout.ctypes.data))
+        hooks[name] = {"us_per_vector": timed(hook_call, 200, 20) * 1e3, "values": len(vec)}
+    log(f"scalar hooks on 2048 values: add {hooks['add']['us_per_vector']:.1f} us, "
+        f"lt {hooks['lt']['us_per_vector']:.1f} us")
     return {"metric": "rows/s & % HBM roofline (scan-filter-project)",
-            "value": min(r["rows_per_s"] for r in rows), "unit": "rows/s",
+            "value": min(r["rows_per_s"] for r in rows), "unit": "rows/s", "hooks": hooks,
             "config": {"workload": "scan-filter-project 100M rows, "
                                    "col < literal + projection col + 1", "rows": n},
             "variants": rows, "dtype": "int32/int64", "data": "synthetic"}
