@@ -411,6 +411,11 @@ int32_t mtl_groups_fetch(mtl_groups *g, int64_t *keys, void *const *agg_out,
  * aggregate_ops.rs:572-686; K <= 64 there, K <= 512 here). */
 int32_t mtl_groups_topk(mtl_groups *g, uint32_t agg, int32_t desc, uint32_t k,
                         int64_t *keys, void *const *agg_out, uint32_t *n_out);
+/* Frees the group table.  Its device memory is kept on the table's stream
+ * for the next mtl_group_agg there (reused when large enough and not over
+ * four times too large; uncharged while kept) and returned by
+ * mtl_stream_destroy or mtl_shutdown.  Release a table before destroying
+ * its stream. */
 int32_t mtl_groups_release(mtl_groups *g);
 
 /* ---------------------------------------------------------------------- */
