@@ -314,6 +314,45 @@ def test_shared_star_batch_full_size(ops):
         assert _star_rows(r, out) == exp
 
 
+def test_shared_star_batch_mixed_sizes(ops):
+    """One batch of sixteen star queries over 7.2M-line tables, half of them
+    on more than 4M lines and half on 1K to 4M (log-uniform), with K of 1, 10
+    and 100: every result equals the oracle's."""
+    tables = datagen.star_tables(scale=0.12, workload=62)
+    cust, orders, lines = tables
+    dev = {name: {c: ops.to_device(x) for c, x in t.items()}
+           for name, t in zip(["customers", "orders", "lines"], tables)}
+    by = dict(zip(["customers", "orders", "lines"], tables))
+    n_c, n_l = len(cust["c_key"]), len(lines["l_order"])
+    assert n_l > (4 << 20) + 100_000
+    rng = np.random.default_rng(62)
+    work, big = [], 0
+    for j in range(16):
+        if j % 2:
+            nl = int(rng.integers((4 << 20) + 1, n_l + 1))
+        else:
+            nl = int(10 ** rng.uniform(3, np.log10(4 << 20)))
+        big += nl >= (4 << 20)
+        no = int(lines["l_order"][nl - 1])
+        seg = int(rng.integers(0, 5))
+        cut = datagen.DAYS_1992_01_01 + int(rng.integers(0, datagen.SPAN_7Y))
+        k = [10, 100, 1][j % 3]
+        dd = datagen.star_desc(lambda t, c: dev[t][c].data_ptr(), n_c, no, nl, k=k,
+                               segment=seg, cutoff=cut)
+        st, ref = orc.star_topk(datagen.star_desc(lambda t, c: by[t][c].ctypes.data, n_c, no,
+                                                  nl, k=k, segment=seg, cutoff=cut))
+        assert st == abi.MTL_OK
+        q, out = ops.star_query(dd)
+        work.append((q, out, [(r.p_row, r.count, r.sum_i64) for r in ref]))
+    assert big == 8
+    with ops.Server(1, budget=BUDGET, max_batch=16) as srv:
+        ts = _arrive_together(srv, [q for q, _, _ in work])
+        res = [srv.wait(t) for t in ts]
+    assert max(r.batch for r in res) > 8
+    for (q, out, exp), r in zip(work, res):
+        assert _star_rows(r, out) == exp
+
+
 def _register_numbers(ops, v):
     dv = ops.to_device(v)
     S.register_table("numbers", len(v), [("value", abi.INTEGER, dv.data_ptr(), 0, 2**20 - 1,
